@@ -1,0 +1,225 @@
+/*
+ * us3d.h -- C ABI of the MI355X (gfx950) differentiable Gaussian slice renderer.
+ *
+ * Two renderers share this library:
+ *   - UltraGauss slice splat ("ug"): plane/Gaussian intersection preprocess, (tile, depth) keys with a
+ *     per-tile LDS sort and tile ranges, per-tile splatting of echo intensity, and its adjoint.
+ *   - UltraG-Ray B-mode ray renderer ("ur"): per-Gaussian scanline range cull, (ray, depth) keys with a
+ *     per-ray LDS sort, transmittance x echo compositing along each scanline, and its adjoint.
+ *
+ * Conventions
+ *   - Every pointer inside a plan, and every array argument, is a DEVICE pointer unless stated.
+ *   - Nothing is allocated inside the library. All scratch lives in a caller-provided workspace whose
+ *     size comes from the matching *_workspace_bytes() query. Every entry point is stream-ordered on
+ *     `stream` (a hipStream_t passed as void*; NULL = legacy default stream), performs no host
+ *     synchronisation and issues no hipMemset, so a call sequence can be captured into a hipGraph.
+ *   - Results that depend on data (instance counts, overflow) are written to the us3d_stats block at
+ *     offset 0 of the workspace. If the instance count exceeds the plan's k_capacity, the kernels that
+ *     would write past it return early, stats.overflow is set and the caller must retry with a larger
+ *     capacity (the reference instead blocks on a device->host copy to learn the count).
+ *   - Return value: US3D_OK or an error code; us3d_last_error() describes the last failure on the
+ *     calling thread.
+ *   - Floating point: FP32 storage and arithmetic. Tile rectangles, keys, and the q <= chi2 test use
+ *     IEEE division/sqrt and explicit FMAs only (library built with -ffp-contract=off), so they are
+ *     reproducible bit-for-bit by a CPU restatement that performs the same operations.
+ */
+#ifndef US3D_H
+#define US3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define US3D_OK 0
+#define US3D_ERR_ARGUMENT 1
+#define US3D_ERR_HIP 2
+#define US3D_ERR_CAPACITY 3
+
+/* One probe frame. 96 bytes, all fields 4 bytes, no padding. Axis vectors are used as given
+ * (the training loop normalises them). */
+typedef struct us3d_frame {
+    float origin[3];        /* probe origin, world mm */
+    float axis_lat[3];      /* lateral unit axis */
+    float axis_ax[3];       /* axial (depth) unit axis */
+    float axis_ele[3];      /* elevational unit axis */
+    float delta_lat;        /* mm per pixel, lateral (UltraGauss); width_mm / W (UltraG-Ray) */
+    float delta_ax;         /* mm per pixel, axial */
+    float near_mm;          /* depth of row 0 */
+    float col_offset;       /* lateral_col_offset: column of lateral coordinate 0 */
+    int32_t height;         /* H: rows (UltraGauss) or depth samples per scanline (UltraG-Ray) */
+    int32_t width;          /* W: columns / scanlines */
+    int32_t tiles_x;        /* tiles per row (UltraGauss); 1 (UltraG-Ray) */
+    int32_t tiles_y;        /* tile rows (UltraGauss); W scanlines (UltraG-Ray) */
+    int32_t tile_base;      /* first global tile (UltraG-Ray: scanline) index of this frame */
+    int32_t pixel_base;     /* first pixel of this frame in the concatenated pixel arrays */
+    int32_t sweep;          /* index into the sweep table */
+    int32_t ray_base;       /* first ray of this frame in concatenated per-ray arrays (UltraG-Ray) */
+} us3d_frame;
+
+/* A sweep: a set of frames rendered from one Gaussian set. Gaussians [g_begin, g_begin+g_count) of the
+ * concatenated parameter arrays; frames [f_begin, f_begin+f_count). (frame, Gaussian) pairs of a sweep are
+ * numbered Gaussian-major from pair_base: pair = pair_base + (g - g_begin) * f_count + (f - f_begin). */
+typedef struct us3d_sweep {
+    int32_t g_begin;
+    int32_t g_count;
+    int32_t f_begin;
+    int32_t f_count;
+    int64_t pair_base;
+} us3d_sweep;
+
+/* Data-dependent results, at offset 0 of every workspace. */
+typedef struct us3d_stats {
+    int64_t n_instances;    /* K: (tile, Gaussian) or (ray, Gaussian) instances of the batch */
+    int64_t max_segment;    /* longest per-tile / per-ray list */
+    int32_t overflow;       /* 1 if n_instances > k_capacity (outputs are then invalid) */
+    int32_t n_sorted_big;   /* segments that took the multi-chunk merge path */
+    int64_t reserved[5];
+} us3d_stats;
+
+/* A batch: frames (possibly from several sweeps) that are mapped, rendered and differentiated together.
+ * Host struct holding device pointers. n_tiles = sum of frames' tiles_x*tiles_y (UltraGauss) or of their
+ * widths (UltraG-Ray); n_pairs = sum over sweeps of g_count*f_count. */
+typedef struct us3d_plan {
+    int32_t n_gauss;                /* rows of the concatenated Gaussian arrays */
+    int32_t n_frames;
+    int32_t n_sweeps;
+    int32_t n_tiles;
+    int64_t n_pairs;
+    int64_t n_pixels;               /* sum of H*W */
+    int64_t k_capacity;             /* capacity of instance arrays in the workspace (< 2^31) */
+    const us3d_frame* frames;       /* [n_frames] */
+    const int32_t* frame_tile_base; /* [n_frames+1] prefix of per-frame tile counts */
+    const int32_t* frame_pixel_base;/* [n_frames+1] prefix of per-frame pixel counts */
+    const us3d_sweep* sweeps;       /* [n_sweeps] */
+    void* workspace;
+    int64_t workspace_bytes;
+} us3d_plan;
+
+/* ---------------------------------------------------------------- UltraGauss slice splat */
+
+typedef struct us3d_ug_settings {
+    float chi2_threshold;   /* q = d^T Lambda d cut (7.815); 120 reproduces the fallback's exp clamp at -60 */
+    float diag_beta;        /* l_ii = raw_ii^2 + beta */
+    float bg_alpha;         /* fixed background alpha */
+    float bg_intensity;     /* fixed background intensity */
+    float alpha_min;        /* pairs with alpha_hat <= alpha_min are skipped (1e-8); negative disables */
+    int32_t sort_by_depth;  /* 1: key = (tile, depth key); 0: key = (tile) */
+    int32_t tile_size;      /* 8, 16 or 32 pixels */
+    int32_t alpha_excludes_bg; /* alpha output = sum alpha_hat (1) or sum alpha_hat + bg_alpha (0) */
+} us3d_ug_settings;
+
+/* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by both renderers.
+ * Arrays: pair_count i32[n_pairs], pair_off i32[n_pairs+1], pair_rect i32x4[n_pairs] (tile rectangle
+ * [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)), pair_dk u32[n_pairs] (depth key), tile_count
+ * i32[n_tiles], tile_off i32[n_tiles+1], inst_rank i32[K], seg_key u64[K] (depth key << 32 | Gaussian),
+ * seg_val i32[K] (emit index), records (per Gaussian), partial (per instance). */
+typedef struct us3d_layout {
+    int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
+    int64_t inst_rank, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp, total;
+} us3d_layout;
+
+int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
+int us3d_ug_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                             us3d_layout* out);
+
+/* Mapper: replaces MapperForwardCUDA (ops/ultragauss_renderer/src/mapper/wrapper_mapper.cu:35-206,
+ * declared at ops/ultragauss_renderer/include/mapper/wrapper_mapper.cuh:4-22). Packs the Gaussians,
+ * computes per-(frame, Gaussian) tile rectangles and depth keys (core.cu:9-116), scans, emits
+ * (tile, depth, Gaussian) instances (core.cu:118-143), sorts each tile's list in LDS by
+ * (depth key, Gaussian) -- the order CUB's stable radix sort of (tile<<32|depth) produces --
+ * and leaves per-tile ranges as tile_off (core.cu:145-164). means [G,3], tril [G,6], opacity [G],
+ * intensity [G]. */
+int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, const float* opacity,
+                const float* intensity, const us3d_ug_settings* settings, void* stream);
+
+/* Forward splat: replaces ShaderForwardCUDA (ops/ultragauss_renderer/src/shader/wrapper_shader.cu:17-66,
+ * forward.cu:17-76). pixels [n_pixels,3] world mm, concatenated per frame (frame.pixel_base); NULL means the
+ * linear-probe grid origin + (x - col_offset)*delta_lat*a_lat + (near + y*delta_ax)*a_ax computed in-kernel
+ * with the FP32 operations of pipeline_god_file.py:299-306 (bit-identical to that tensor).
+ * Outputs [n_pixels]: image, alpha, depth; aux [n_pixels,4] = (sum alpha_hat, sum alpha_hat*c,
+ * sum alpha_hat*elev, 0) kept for the adjoint. Requires a prior us3d_ug_map on the same workspace. */
+int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
+                    float* image, float* alpha, float* depth, float* aux, void* stream);
+
+/* Adjoint: replaces ShaderBackwardCUDA (ops/ultragauss_renderer/src/shader/wrapper_shader.cu:68-122,
+ * backward.cu:17-131). Recomputes per-pair weights from the stored per-pixel sums, accumulates per
+ * (tile, Gaussian) in registers (no atomics) and reduces per Gaussian in a fixed order, so the result
+ * is bitwise reproducible. d_alpha / d_depth may be NULL (zero). Outputs (overwritten): d_means [G,3],
+ * d_tril [G,6] (w.r.t. the raw tril parameters), d_opacity [G], d_intensity [G]. */
+int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tril,
+                     const us3d_ug_settings* settings, const float* aux, const float* d_image,
+                     const float* d_alpha, const float* d_depth, float* d_means, float* d_tril,
+                     float* d_opacity, float* d_intensity, void* stream);
+
+/* Per-pixel candidate lists (the reference's pixel_gauss_counts/offsets/indices extras,
+ * core.cu:166-287): candidates of each pixel in sorted tile order filtered by the lateral/axial window.
+ * counts [n_pixels] is written by pass 0; offsets [n_pixels] must be the exclusive scan of counts
+ * (us3d_scan_i32) before pass 1 fills indices (Gaussian indices relative to the sweep's g_begin). */
+int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
+                        int pass, int32_t* counts, const int32_t* offsets, int32_t* indices,
+                        void* stream);
+
+/* ---------------------------------------------------------------- UltraG-Ray B-mode ray renderer */
+
+typedef struct us3d_ur_settings {
+    float k_sigma;                      /* safe radius multiple of max scale */
+    float quadrature_window_radius_mm;  /* R: GL3 window around closest approach (0 = whole ray) */
+    float elev_cutoff_margin_mm;
+    float local_transmission_max_window_mm;
+    float grad_threshold;               /* pixels with |dL/dB| < grad_threshold are skipped (1e-6) */
+    int32_t local_transmission_window_only;
+    int32_t reserved0;
+    int32_t reserved1;
+} us3d_ur_settings;
+
+int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity);
+int us3d_ur_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity,
+                             us3d_layout* out);
+
+/* Ray mapper: replaces MapperForwardCUDA of ops/ultragray_renderer (src/mapper/wrapper_mapper.cu:7-128,
+ * core.cu:3-129). means [G,3], scales [G,3], quats [G,4] (w,x,y,z), attenuations [G] (raw logits),
+ * sh [G,4]. Emits (ray, quantised axial depth, Gaussian) instances and sorts each ray's list in LDS.
+ * Frames: delta_lat = width_mm / W (FP32), tiles_x = 1, tiles_y = W; ray_base indexes the per-ray arrays. */
+int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, const float* quats,
+                const float* attenuations, const float* sh, const us3d_ur_settings* settings,
+                void* stream);
+
+/* B-mode forward: replaces ShaderForwardCUDA of ops/ultragray_renderer (src/shader/wrapper_shader.cu:7-58,
+ * forward.cu:22-119). pixels: per frame (W,H,3) world mm (pixel (x,y) at ((x*H)+y)*3), concatenated;
+ * ray_origins/ray_dirs [n_rays,3]; ray_offsets [n_rays] elevational OPS offsets (mm).
+ * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, echo num, echo den, 0). */
+int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray_origins,
+                    const float* ray_dirs, const float* ray_offsets, const us3d_ur_settings* settings,
+                    float* image, float* aux, void* stream);
+
+/* B-mode adjoint: replaces ShaderBackwardCUDA of ops/ultragray_renderer (wrapper_shader.cu:60-123,
+ * backward.cu:26-325). Outputs (overwritten): d_means [G,3], d_scales [G,3], d_quats [G,4],
+ * d_attenuations [G], d_sh [G,4]. */
+int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ray_origins,
+                     const float* ray_dirs, const float* ray_offsets, const us3d_ur_settings* settings,
+                     const float* aux, const float* d_image, float* d_means, float* d_scales,
+                     float* d_quats, float* d_attenuations, float* d_sh, void* stream);
+
+/* ---------------------------------------------------------------- training-step helpers */
+
+/* Exclusive scan of n int32 values into out[0..n) and the int64 total into *total_dev (device).
+ * tmp must hold us3d_scan_tmp_bytes(n) bytes. Replaces the CUB DeviceScan::ExclusiveSum call sites
+ * (ops/ultragauss_renderer/src/mapper/wrapper_mapper.cu:97-102,174-182). */
+int64_t us3d_scan_tmp_bytes(int64_t n);
+int us3d_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev, void* tmp,
+                  void* stream);
+
+/* Fill n int32 values with `value` (a kernel, usable inside captured graphs instead of hipMemset). */
+int us3d_fill_i32(int32_t* dst, int64_t n, int32_t value, void* stream);
+
+const char* us3d_last_error(void);
+int us3d_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* US3D_H */

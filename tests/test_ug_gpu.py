@@ -1,0 +1,193 @@
+"""Parity of the HIP slice splat (libus3d via the C ABI) with the oracle.
+
+Bars: tile keys / sort order / tile ranges / pixel lists bit-exact; FP32 image, alpha and depth within 1e-5
+relative L2; gradients within 1e-4 relative L2 of the FP64 oracle (per parameter group).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fallback as ofb
+from oracle import ug as oug
+from tests.ug_helpers import cfg_for, gpu_render, oracle_frame, rel_l2, small_case
+from us3d.batch import CapacityHint
+from us3d.scenes import linear_pixels, make_scene, straight_sweep
+
+pytestmark = pytest.mark.gpu
+
+IMG_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+@pytest.mark.parametrize("ts", [16, 8, 32])
+@pytest.mark.parametrize("sort_by_depth", [True, False])
+def test_mapper_keys_sort_ranges_bitexact(dev, ts, sort_by_depth):
+    scene, frames = small_case(n_frames=3)
+    for fr in frames:
+        out = gpu_render(scene, [fr], cfg_for(fr, tile_size=ts, mapper_sort_by_depth=sort_by_depth), dev)
+        view = out["batch"].sorted_view(out["batch"].map(*[t.to(dev) for t in (
+            scene.means, scene.tril, scene.opacity, scene.intensity)]))[0]
+        _g, _p, mr, _l = oracle_frame(scene, fr, oug.Semantics(), ts=ts, sort_by_depth=sort_by_depth)
+        assert mr.keys.size > 100
+        np.testing.assert_array_equal(view["keys"].cpu().numpy().astype(np.uint64), mr.keys)
+        np.testing.assert_array_equal(view["values"].cpu().numpy(), mr.vals)
+        np.testing.assert_array_equal(view["tile_start"].cpu().numpy(), mr.tile_start)
+        np.testing.assert_array_equal(view["tile_end"].cpu().numpy(), mr.tile_end)
+
+
+def test_pixel_lists_bitexact(dev):
+    scene, frames = small_case(n_frames=2)
+    for fr in frames:
+        out = gpu_render(scene, [fr], cfg_for(fr), dev)
+        b = out["batch"]
+        m = b.map(*[t.to(dev) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)])
+        counts, offsets, indices = b.pixel_lists(m, linear_pixels(fr).to(dev))
+        _g, _p, _mr, (c_ref, o_ref, i_ref) = oracle_frame(scene, fr, oug.Semantics())
+        np.testing.assert_array_equal(counts.cpu().numpy(), c_ref)
+        np.testing.assert_array_equal(offsets.cpu().numpy(), o_ref)
+        np.testing.assert_array_equal(indices.cpu().numpy(), i_ref)
+
+
+@pytest.mark.parametrize("ts", [16, 8, 32])
+def test_forward_backward_parity_cuda_semantics(dev, ts):
+    scene, frames = small_case(n_frames=2)
+    rng = np.random.default_rng(5)
+    sem = oug.Semantics()
+    for fr in frames:
+        P = fr.height * fr.width
+        dI = rng.normal(size=P).astype(np.float32)
+        dA = rng.normal(size=P).astype(np.float32) * 0.1
+        dD = rng.normal(size=P).astype(np.float32) * 0.01
+        out = gpu_render(scene, [fr], cfg_for(fr, tile_size=ts), dev, grads=(dI, dA, dD))
+        geom, pix, _mr, lists = oracle_frame(scene, fr, sem)
+        args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom, lists,
+                sem)
+        img32, a32, d32, _ = oug.forward(*args)
+        img64, a64, d64, _ = oug.forward(*args, fp64=True)
+        got = {k: out[k].detach().cpu().numpy() for k in ("image", "alpha", "depth")}
+        assert rel_l2(got["image"], img32) < IMG_TOL
+        assert rel_l2(got["alpha"], a32) < IMG_TOL
+        assert rel_l2(got["depth"], d32) < IMG_TOL
+        assert rel_l2(got["image"], img64) < IMG_TOL
+        ref = oug.backward_f64(*args, dI, dA, dD, gate=True)
+        for name, g, r in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref):
+            err = rel_l2(g, r)
+            assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
+        # the reference's backward omits the alpha_hat > 1e-8 gate; the difference is below the bar too
+        ref_q = oug.backward_f64(*args, dI, dA, dD, gate=False)
+        for g, r in zip(out["grads"], ref_q):
+            assert rel_l2(g, r) < GRAD_TOL
+
+
+def test_fallback_semantics_vs_torch_fallback(dev):
+    """The named oracle: the reference's PyTorch fallback renderer, restated, in float64 with autograd."""
+    scene = make_scene(512, 64.0, seed=3)
+    fr = straight_sweep(3, 48, 40, 0.3, 64.0, 10.0, 5.0, seed=4)[1]
+    P = fr.height * fr.width
+    rng = np.random.default_rng(1)
+    dI = rng.normal(size=P).astype(np.float32)
+    out = gpu_render(scene, [fr], cfg_for(fr, semantics="fallback"), dev, grads=(dI, None, None))
+    ts = [t.double().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    img, alpha = ofb.render(*ts, linear_pixels(fr).double(), fr.height, fr.width)
+    (img.reshape(-1) * torch.from_numpy(dI).double()).sum().backward()
+    assert rel_l2(out["image"].detach().cpu().numpy(), img.detach().numpy()) < IMG_TOL
+    assert rel_l2(out["alpha"].detach().cpu().numpy(), alpha.detach().numpy()) < IMG_TOL
+    for name, g, t in zip(("means", "tril", "opacity", "intensity"), out["grads"], ts):
+        err = rel_l2(g, t.grad.numpy())
+        assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
+
+
+def test_batch_equals_per_frame_and_procedural_pixels(dev):
+    scene, frames = small_case(n_frames=4)
+    cfg = cfg_for(frames[0])
+    rng = np.random.default_rng(2)
+    P = sum(f.height * f.width for f in frames)
+    dI = rng.normal(size=P).astype(np.float32)
+    batched = gpu_render(scene, frames, cfg, dev, grads=(dI, None, None))
+    proc = gpu_render(scene, frames, cfg, dev, pixels="procedural", grads=(dI, None, None))
+    np.testing.assert_array_equal(batched["image"].detach().cpu().numpy(), proc["image"].detach().cpu().numpy())
+    for a, b in zip(batched["grads"], proc["grads"]):
+        np.testing.assert_array_equal(a, b)
+    off = 0
+    acc = [np.zeros_like(g) for g in batched["grads"]]
+    for fr in frames:
+        n = fr.height * fr.width
+        single = gpu_render(scene, [fr], cfg, dev, grads=(dI[off:off + n], None, None))
+        np.testing.assert_array_equal(single["image"].detach().cpu().numpy(),
+                                      batched["image"].detach().cpu().numpy()[off:off + n])
+        acc = [a + g for a, g in zip(acc, single["grads"])]
+        off += n
+    for a, b in zip(acc, batched["grads"]):
+        assert rel_l2(b, a) < 1e-6
+
+
+def test_bitwise_deterministic(dev):
+    scene, frames = small_case(n_frames=3)
+    cfg = cfg_for(frames[0])
+    P = sum(f.height * f.width for f in frames)
+    dI = np.random.default_rng(3).normal(size=P).astype(np.float32)
+    a = gpu_render(scene, frames, cfg, dev, grads=(dI, None, None))
+    b = gpu_render(scene, frames, cfg, dev, grads=(dI, None, None))
+    for x, y in zip(a["grads"], b["grads"]):
+        np.testing.assert_array_equal(x, y)
+
+
+def test_capacity_overflow_retry(dev):
+    scene, frames = small_case(n_frames=2)
+    cfg = cfg_for(frames[0])
+    ref = gpu_render(scene, frames, cfg, dev)
+    hint = CapacityHint(floor=16, growth=1.25)
+    hint.k_last = 0
+    tiny = gpu_render(scene, frames, cfg, dev, hint=hint)
+    np.testing.assert_array_equal(ref["image"].detach().cpu().numpy(), tiny["image"].detach().cpu().numpy())
+
+
+def test_empty_and_invisible_scenes(dev):
+    scene, frames = small_case(n=64, n_frames=1)
+    scene.means[:] = 1.0e4  # far outside every slice
+    fr = frames[0]
+    P = fr.height * fr.width
+    out = gpu_render(scene, [fr], cfg_for(fr), dev, grads=(np.ones(P, np.float32), None, None))
+    assert float(out["image"].detach().abs().max()) == 0.0
+    np.testing.assert_array_equal(out["alpha"].detach().cpu().numpy(), np.float32(1e-6))
+    for g in out["grads"]:
+        assert np.all(g == 0)
+
+
+def test_nonfinite_gaussians_are_culled(dev):
+    scene, frames = small_case(n=256, n_frames=1)
+    fr = frames[0]
+    clean = gpu_render(scene, [fr], cfg_for(fr), dev)
+    scene.means[7] = float("nan")
+    scene.tril[11, 0] = float("inf")
+    scene.opacity[7] = 0.0
+    out = gpu_render(scene, [fr], cfg_for(fr), dev)
+    assert torch.isfinite(out["image"]).all()
+    geom, pix, _mr, lists = oracle_frame(scene, fr, oug.Semantics())
+    img, _a, _d, _s = oug.forward(scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(),
+                                  scene.intensity.numpy(), pix, geom, lists, oug.Semantics())
+    assert rel_l2(out["image"].detach().cpu().numpy(), img) < IMG_TOL
+    assert clean["image"].shape == out["image"].shape
+
+
+def test_config2_frame_parity(dev):
+    """One frame of the 200k-Gaussian, 512x512 freehand workload against the oracle (FP32 image, FP64 grads)."""
+    from us3d.scenes import config2
+
+    scene, _target, frames = config2(n_frames=256)
+    fr = frames[128]
+    P = fr.height * fr.width
+    dI = np.random.default_rng(9).normal(size=P).astype(np.float32) / P
+    out = gpu_render(scene, [fr], cfg_for(fr), dev, pixels="procedural", grads=(dI, None, None))
+    sem = oug.Semantics()
+    geom, pix, mr, lists = oracle_frame(scene, fr, sem)
+    args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom, lists,
+            sem)
+    img, _a, _d, _s = oug.forward(*args)
+    assert mr.keys.size > 100_000
+    assert rel_l2(out["image"].detach().cpu().numpy(), img) < IMG_TOL
+    ref = oug.backward_f64(*args, dI, None, None, gate=True)
+    for name, g, r in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref):
+        err = rel_l2(g, r)
+        assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"

@@ -1,0 +1,148 @@
+// Internal helpers shared by the gfx950 kernels of libus3d.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "us3d.h"
+
+namespace us3d {
+
+constexpr int kWave = 64;
+
+// ---------------------------------------------------------------- host-side error plumbing
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int check_launch(const char* what);
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------- device math
+// All library code is compiled with -ffp-contract=off: every FMA below is explicit, so the CPU
+// restatement that performs the same operations reproduces the results bit for bit.
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return __builtin_fmaf(az, bz, __builtin_fmaf(ay, by, ax * bx));
+}
+
+// Marginal chi2 radius of the ellipsoid {d : d^T L L^T d <= chi2} along unit axis a, by forward
+// substitution y = L^-1 a (L lower triangular, rows (l11), (l12 l22), (l13 l23 l33)).
+// Follows compute_probe_bbox_radii, ops/ultragauss_renderer/src/mapper/core.cu:9-44.
+__device__ __forceinline__ float marginal_radius(float l11, float l12, float l13, float l22, float l23,
+                                                 float l33, float ax, float ay, float az, float sqrt_chi2) {
+    const float y0 = ax / fmaxf(l11, 1e-10f);
+    const float y1 = __builtin_fmaf(-l12, y0, ay) / fmaxf(l22, 1e-10f);
+    const float y2 = __builtin_fmaf(-l23, y1, __builtin_fmaf(-l13, y0, az)) / fmaxf(l33, 1e-10f);
+    const float s = fmaxf(__builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0)), 1e-12f);
+    return fmaxf(1e-3f, sqrt_chi2 * sqrtf(s));
+}
+
+// q = || L^T d ||^2 (= d^T Lambda d with Lambda = L L^T); y = L^T d is returned for the adjoint.
+__device__ __forceinline__ float quad_form(float l11, float l12, float l13, float l22, float l23, float l33,
+                                          float dx, float dy, float dz, float& y0, float& y1, float& y2) {
+    y0 = __builtin_fmaf(l13, dz, __builtin_fmaf(l12, dy, l11 * dx));
+    y1 = __builtin_fmaf(l23, dz, l22 * dy);
+    y2 = l33 * dz;
+    return __builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0));
+}
+
+// World position of pixel (ix, iy) of a frame. With pixels == NULL the position is the linear-probe grid of
+// NpzUltrasoundDataset._pixel_positions (pipeline_god_file.py:299-306), evaluated with the same FP32
+// operations in the same order (lateral = (u - c) * d_lat; axial = near + v * d_ax;
+// p = (o + lateral * a_lat) + axial * a_ax), so it equals that tensor bit for bit.
+__device__ __forceinline__ void pixel_position(const us3d_frame& fr, const float* __restrict__ pixels, int ix, int iy,
+                                               float& x, float& y, float& z) {
+    if (pixels) {
+        const int64_t p = (int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix;
+        x = pixels[3 * p + 0];
+        y = pixels[3 * p + 1];
+        z = pixels[3 * p + 2];
+    } else {
+        const float lat = ((float)ix - fr.col_offset) * fr.delta_lat;
+        const float ax = fr.near_mm + (float)iy * fr.delta_ax;
+        x = (fr.origin[0] + lat * fr.axis_lat[0]) + ax * fr.axis_ax[0];
+        y = (fr.origin[1] + lat * fr.axis_lat[1]) + ax * fr.axis_ax[1];
+        z = (fr.origin[2] + lat * fr.axis_lat[2]) + ax * fr.axis_ax[2];
+    }
+}
+
+// Binary search: largest i in [0, n) with base[i] <= v (base ascending, base[0] <= v).
+__device__ __forceinline__ int upper_index(const int32_t* __restrict__ base, int n, int v) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (base[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Sweep of a (frame, Gaussian) pair index (sweeps ordered by pair_base; a handful per batch).
+__device__ __forceinline__ int sweep_of_pair(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int64_t pair) {
+    int s = 0;
+    while (s + 1 < n_sweeps && sweeps[s + 1].pair_base <= pair) ++s;
+    return s;
+}
+
+// Sweep owning concatenated Gaussian row g.
+__device__ __forceinline__ int sweep_of_gauss(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int g) {
+    int s = 0;
+    while (s + 1 < n_sweeps && sweeps[s + 1].g_begin <= g) ++s;
+    return s;
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fminf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// ---------------------------------------------------------------- shared pipeline pieces
+// Exclusive scan (int64 accumulation, int32 output) with the total stored to *total.
+int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* tmp, hipStream_t stream);
+int64_t scan_tmp_bytes(int64_t n);
+
+// Per-segment sort of 64-bit keys carrying a 32-bit value. Segment t is [seg_off[t], seg_off[t+1]).
+// Keys must be unique within a segment (they are: the low word is the Gaussian index).
+int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
+                  const int32_t* seg_off, int n_segments, us3d_stats* stats, int64_t k_capacity,
+                  hipStream_t stream);
+
+// Workspace layout shared by both renderers (record and partial sizes differ).
+us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
+                      int64_t partial_bytes);
+
+template <typename T>
+inline T* ws_at(const us3d_plan* plan, int64_t off) {
+    return reinterpret_cast<T*>(reinterpret_cast<char*>(plan->workspace) + off);
+}
+
+inline unsigned blocks_for(int64_t n, int threads) {
+    return (unsigned)((n + threads - 1) / threads > 0 ? (n + threads - 1) / threads : 1);
+}
+
+int check_plan(const us3d_plan* plan, int64_t ws_needed);
+
+// After the per-pair counts/rects/depth keys are in the workspace: scan pairs (instance count into stats,
+// overflow flag), per-tile counts and offsets, emit instances into tile buckets, sort every bucket by
+// (depth key, Gaussian). Pairs carry tile rectangles in frame tile coordinates.
+int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+
+__global__ void init_stats_kernel(us3d_stats* stats);
+
+__global__ void fill_i32_kernel(int32_t* dst, int64_t n, int32_t v);
+__global__ void fill_f32_kernel(float* dst, int64_t n, float v);
+
+}  // namespace us3d

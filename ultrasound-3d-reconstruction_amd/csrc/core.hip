@@ -1,0 +1,280 @@
+// Shared pipeline pieces of libus3d: error state, device-wide exclusive scan, per-segment LDS sort,
+// fill kernels.
+#include <algorithm>
+#include <cstdio>
+#include <string>
+
+#include "common.h"
+
+namespace us3d {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+    set_error(msg);
+    return code;
+}
+
+int check_launch(const char* what) {
+    const hipError_t err = hipGetLastError();
+    if (err != hipSuccess) {
+        return fail(US3D_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
+    }
+    return US3D_OK;
+}
+
+// ---------------------------------------------------------------- fill
+
+__global__ void fill_i32_kernel(int32_t* dst, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+__global__ void fill_f32_kernel(float* dst, int64_t n, float v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = v;
+}
+
+// ---------------------------------------------------------------- exclusive scan
+// Reduce-then-scan over blocks of 2048 elements (256 threads x 8 contiguous items), int64 accumulation.
+// Level 0 reads int32 counts and writes int32 offsets; block totals recurse as int64.
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+template <typename TIn, typename TOut>
+__global__ __launch_bounds__(kScanThreads) void scan_block_kernel(const TIn* __restrict__ in, TOut* __restrict__ out,
+                                                                  int64_t n, int64_t* __restrict__ block_sums,
+                                                                  int64_t* __restrict__ total) {
+    __shared__ int64_t wave_tot[kScanThreads / kWave];
+    const int tid = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanItems;
+    int64_t v[kScanItems];
+    int64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        v[k] = (i < n) ? (int64_t)in[i] : 0;
+        sum += v[k];
+    }
+    // inclusive wave scan of per-thread sums
+    const int lane = tid & (kWave - 1);
+    const int wave = tid / kWave;
+    int64_t incl = sum;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t o = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += o;
+    }
+    if (lane == kWave - 1) wave_tot[wave] = incl;
+    __syncthreads();
+    int64_t wave_base = 0;
+    for (int w = 0; w < wave; ++w) wave_base += wave_tot[w];
+    int64_t run = wave_base + incl - sum;  // exclusive prefix of this thread
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = (TOut)run;
+        run += v[k];
+    }
+    if (tid == kScanThreads - 1) {
+        int64_t block_total = 0;
+        for (int w = 0; w < kScanThreads / kWave; ++w) block_total += wave_tot[w];
+        if (block_sums) block_sums[blockIdx.x] = block_total;
+        if (total) *total = block_total;
+    }
+}
+
+template <typename TOut>
+__global__ __launch_bounds__(kScanThreads) void scan_add_kernel(TOut* __restrict__ out, int64_t n,
+                                                                const int64_t* __restrict__ block_off) {
+    const int64_t b = blockIdx.x;
+    const int64_t add = block_off[b];
+    const int64_t base = b * kScanBlock;
+    for (int k = threadIdx.x; k < kScanBlock; k += kScanThreads) {
+        const int64_t i = base + k;
+        if (i < n) out[i] = (TOut)((int64_t)out[i] + add);
+    }
+}
+
+int64_t scan_tmp_bytes(int64_t n) {
+    int64_t bytes = 0;
+    int64_t m = n;
+    while (m > kScanBlock) {
+        const int64_t nb = (m + kScanBlock - 1) / kScanBlock;
+        bytes += 2 * align_up(nb * (int64_t)sizeof(int64_t), 256);
+        m = nb;
+    }
+    return bytes + 256;
+}
+
+template <typename TIn, typename TOut>
+static int scan_rec(const TIn* in, TOut* out, int64_t n, int64_t* total, char* tmp, hipStream_t stream) {
+    if (n <= kScanBlock) {
+        hipLaunchKernelGGL((scan_block_kernel<TIn, TOut>), dim3(1), dim3(kScanThreads), 0, stream, in, out, n,
+                           (int64_t*)nullptr, total);
+        return check_launch("scan_block_kernel");
+    }
+    const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    int64_t* sums = reinterpret_cast<int64_t*>(tmp);
+    int64_t* offs = reinterpret_cast<int64_t*>(tmp + align_up(nb * (int64_t)sizeof(int64_t), 256));
+    char* next = tmp + 2 * align_up(nb * (int64_t)sizeof(int64_t), 256);
+    hipLaunchKernelGGL((scan_block_kernel<TIn, TOut>), dim3((unsigned)nb), dim3(kScanThreads), 0, stream, in, out, n,
+                       sums, (int64_t*)nullptr);
+    int rc = check_launch("scan_block_kernel");
+    if (rc) return rc;
+    rc = scan_rec<int64_t, int64_t>(sums, offs, nb, total, next, stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL((scan_add_kernel<TOut>), dim3((unsigned)nb), dim3(kScanThreads), 0, stream, out, n, offs);
+    return check_launch("scan_add_kernel");
+}
+
+int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* tmp, hipStream_t stream) {
+    if (n <= 0) return fail(US3D_ERR_ARGUMENT, "scan_i32: n must be positive");
+    return scan_rec<int32_t, int32_t>(in, out, n, total, reinterpret_cast<char*>(tmp), stream);
+}
+
+// ---------------------------------------------------------------- per-segment sort
+// One workgroup per segment. Segments of up to kSortCap keys are bitonic-sorted in LDS. Longer segments
+// are sorted in kSortCap chunks in LDS and then merged in global memory (ping-pong with the tmp arrays):
+// with unique keys, an element's merged position is its index in its run plus the number of keys of
+// the partner run that are smaller (binary search), so a merge pass is fully parallel.
+
+constexpr int kSortThreads = 256;
+constexpr int kSortCap = 2048;
+
+__device__ __forceinline__ void lds_bitonic(uint64_t* sk, int32_t* sv, int n_pad) {
+    for (int k = 2; k <= n_pad; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < (n_pad >> 1); i += kSortThreads) {
+                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
+                const int hi = lo + j;
+                const bool asc = (lo & k) == 0;
+                const uint64_t a = sk[lo];
+                const uint64_t b = sk[hi];
+                if ((a > b) == asc) {
+                    sk[lo] = b;
+                    sk[hi] = a;
+                    const int32_t t = sv[lo];
+                    sv[lo] = sv[hi];
+                    sv[hi] = t;
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_sort_chunk(uint64_t* __restrict__ keys, int32_t* __restrict__ vals, int n,
+                                               uint64_t* sk, int32_t* sv) {
+    int n_pad = 1;
+    while (n_pad < n) n_pad <<= 1;
+    for (int i = threadIdx.x; i < n_pad; i += kSortThreads) {
+        if (i < n) {
+            sk[i] = keys[i];
+            sv[i] = vals[i];
+        } else {
+            sk[i] = ~0ull;
+            sv[i] = -1;
+        }
+    }
+    __syncthreads();
+    lds_bitonic(sk, sv, n_pad);
+    for (int i = threadIdx.x; i < n; i += kSortThreads) {
+        keys[i] = sk[i];
+        vals[i] = sv[i];
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* __restrict__ keys,
+                                                                     int32_t* __restrict__ vals,
+                                                                     uint64_t* __restrict__ keys_tmp,
+                                                                     int32_t* __restrict__ vals_tmp,
+                                                                     const int32_t* __restrict__ seg_off,
+                                                                     us3d_stats* stats) {
+    __shared__ uint64_t sk[kSortCap];
+    __shared__ int32_t sv[kSortCap];
+    if (stats->overflow) return;
+    const int seg = blockIdx.x;
+    const int s0 = seg_off[seg];
+    const int n = seg_off[seg + 1] - s0;
+    if (threadIdx.x == 0 && n > 0) atomicMax((unsigned long long*)&stats->max_segment, (unsigned long long)n);
+    if (n <= 1) return;
+    uint64_t* k = keys + s0;
+    int32_t* v = vals + s0;
+    if (n <= kSortCap) {
+        lds_sort_chunk(k, v, n, sk, sv);
+        return;
+    }
+    if (threadIdx.x == 0) atomicAdd(&stats->n_sorted_big, 1);
+    for (int c = 0; c < n; c += kSortCap) lds_sort_chunk(k + c, v + c, min(kSortCap, n - c), sk, sv);
+    uint64_t* src_k = k;
+    int32_t* src_v = v;
+    uint64_t* dst_k = keys_tmp + s0;
+    int32_t* dst_v = vals_tmp + s0;
+    for (int width = kSortCap; width < n; width <<= 1) {
+        for (int i = threadIdx.x; i < n; i += kSortThreads) {
+            const int run = i / width;
+            const int pair_start = (run & ~1) * width;
+            const int partner_start = (run & 1) ? pair_start : pair_start + width;
+            const int partner_len = max(0, min(width, n - partner_start));
+            const uint64_t key = src_k[i];
+            int lo = 0, hi = partner_len;  // count of partner keys < key
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (src_k[partner_start + mid] < key) lo = mid + 1; else hi = mid;
+            }
+            const int pos = pair_start + (i - run * width) + lo;
+            dst_k[pos] = key;
+            dst_v[pos] = src_v[i];
+        }
+        __syncthreads();
+        uint64_t* tk = src_k; src_k = dst_k; dst_k = tk;
+        int32_t* tv = src_v; src_v = dst_v; dst_v = tv;
+    }
+    if (src_k != k) {
+        for (int i = threadIdx.x; i < n; i += kSortThreads) {
+            k[i] = src_k[i];
+            v[i] = src_v[i];
+        }
+    }
+}
+
+int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp, const int32_t* seg_off,
+                  int n_segments, us3d_stats* stats, int64_t k_capacity, hipStream_t stream) {
+    (void)k_capacity;
+    if (n_segments <= 0) return US3D_OK;
+    hipLaunchKernelGGL(sort_segments_kernel, dim3(n_segments), dim3(kSortThreads), 0, stream, keys, vals, keys_tmp,
+                       vals_tmp, seg_off, stats);
+    return check_launch("sort_segments_kernel");
+}
+
+}  // namespace us3d
+
+// ---------------------------------------------------------------- exported helpers
+
+extern "C" {
+
+const char* us3d_last_error(void) { return us3d::g_last_error.c_str(); }
+
+int us3d_version(void) { return 1; }
+
+int64_t us3d_scan_tmp_bytes(int64_t n) { return us3d::scan_tmp_bytes(n); }
+
+int us3d_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev, void* tmp, void* stream) {
+    return us3d::scan_i32(in, out, n, total_dev, tmp, us3d::as_stream(stream));
+}
+
+int us3d_fill_i32(int32_t* dst, int64_t n, int32_t value, void* stream) {
+    if (n <= 0) return US3D_OK;
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(us3d::fill_i32_kernel, dim3((unsigned)blocks), dim3(256), 0, us3d::as_stream(stream), dst, n,
+                       value);
+    return us3d::check_launch("fill_i32_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,796 @@
+// UltraGauss slice splat on gfx950: mapper, per-tile forward splat, per-tile adjoint, per-Gaussian
+// gradient reduction and the per-pixel candidate lists.
+//
+// Data layout in HBM (one batch = frames of one or more sweeps):
+//   records   f32x12 per Gaussian: (mu.xyz, opacity | intensity, l11, l12, l13 | l22, l23, l33, 0),
+//             l_ii = raw_ii^2 + beta, packed once per call so every gather is three aligned 16 B loads.
+//   pairs     (frame, Gaussian), numbered Gaussian-major per sweep: count, tile rect, depth key.
+//   instances (tile, Gaussian) in emit order e (pair order, then tile rows, then columns) -- the
+//             reference's emit order for one frame -- bucketed by global tile, each bucket sorted by
+//             the 64-bit key (depth key << 32 | Gaussian), value = e.
+//   partial   f32x12 per instance, written at e by the adjoint, so each Gaussian's partials across all
+//             frames are contiguous and reduce in a fixed order.
+#include <algorithm>
+
+#include "common.h"
+
+namespace us3d {
+
+// ---------------------------------------------------------------- workspace layout
+
+us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
+                      int64_t partial_bytes) {
+    us3d_layout L{};
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        const int64_t o = off;
+        off += align_up(std::max<int64_t>(bytes, 0), 256);
+        return o;
+    };
+    L.stats = take(sizeof(us3d_stats));
+    L.pair_count = take(n_pairs * 4);
+    L.pair_off = take((n_pairs + 1) * 4);
+    L.pair_rect = take(n_pairs * 16);
+    L.pair_dk = take(n_pairs * 4);
+    L.tile_count = take(n_tiles * 4);
+    L.tile_off = take((n_tiles + 1) * 4);
+    L.inst_rank = take(k_cap * 4);
+    L.seg_key = take(k_cap * 8);
+    L.seg_val = take(k_cap * 4);
+    L.seg_key_tmp = take(k_cap * 8);
+    L.seg_val_tmp = take(k_cap * 4);
+    L.records = take(n_gauss * record_bytes);
+    L.partial = take(k_cap * partial_bytes);
+    L.scan_tmp = take(std::max(scan_tmp_bytes(n_pairs + 1), scan_tmp_bytes(n_tiles + 1)));
+    L.total = off;
+    return L;
+}
+
+static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48);
+}
+
+// ---------------------------------------------------------------- mapper kernels
+
+__global__ void ug_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ tril,
+                               const float* __restrict__ opacity, const float* __restrict__ intensity, float beta,
+                               float4* __restrict__ rec) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const float* t = tril + (int64_t)g * 6;
+    const float* m = means + (int64_t)g * 3;
+    rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], opacity[g]);
+    rec[3 * (int64_t)g + 1] = make_float4(intensity[g], __builtin_fmaf(t[0], t[0], beta), t[1], t[2]);
+    rec[3 * (int64_t)g + 2] = make_float4(__builtin_fmaf(t[3], t[3], beta), t[4], __builtin_fmaf(t[5], t[5], beta), 0.f);
+}
+
+__device__ __forceinline__ int clamp_to_int(float x) {
+    return (int)fminf(fmaxf(x, -1073741824.0f), 1073741824.0f);
+}
+
+// Depth key (uint32)((v_pix + 500) * 1000) of core.cu:134, with the out-of-range cases defined:
+// negative or NaN -> 0, >= 2^32 -> 0xffffffff.
+__device__ __forceinline__ uint32_t depth_key(float v_pix) {
+    const float x = (v_pix + 500.0f) * 1000.0f;
+    if (!(x > 0.0f)) return 0u;
+    if (x >= 4294967296.0f) return 0xffffffffu;
+    return (uint32_t)x;
+}
+
+// Per (frame, Gaussian) pair: probe-frame projection, chi2 marginal radii, elevation cull, tile rectangle
+// and depth key (compute_probe_bbox_radii + kernel_project_gauss_and_compute_tile_rects,
+// ops/ultragauss_renderer/src/mapper/core.cu:9-116). Non-finite projections are culled.
+__global__ void ug_preprocess_kernel(int64_t n_pairs, const float4* __restrict__ rec,
+                                     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+                                     int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
+                                     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect,
+                                     uint32_t* __restrict__ pair_dk) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pair >= n_pairs) return;
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int64_t local = pair - sw.pair_base;
+    const int gl = (int)(local / sw.f_count);
+    const int g = sw.g_begin + gl;
+    const int f = sw.f_begin + (int)(local - (int64_t)gl * sw.f_count);
+    const us3d_frame& fr = frames[f];
+    const float4 r0 = rec[3 * (int64_t)g + 0];
+    const float4 r1 = rec[3 * (int64_t)g + 1];
+    const float4 r2 = rec[3 * (int64_t)g + 2];
+    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+    const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
+    const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+    const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+    const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+    const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+    const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+    const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+    int count = 0;
+    int4 rect = make_int4(0, 0, 0, 0);
+    uint32_t dk = 0;
+    if (fabsf(ele) <= r_ele) {
+        const float u = lat / fr.delta_lat + fr.col_offset;
+        const float v = (ax - fr.near_mm) / fr.delta_ax;
+        const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+        const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+        const float tsf = (float)ts;
+        const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
+        const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
+        if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
+            const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
+            const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
+            if (x0 < x1 && y0 < y1) {
+                count = (x1 - x0) * (y1 - y0);
+                rect = make_int4(x0, y0, x1, y1);
+                dk = sort_by_depth ? depth_key(v) : 0u;
+            }
+        }
+    }
+    pair_count[pair] = count;
+    pair_rect[pair] = rect;
+    pair_dk[pair] = dk;
+}
+
+__global__ void init_stats_kernel(us3d_stats* stats) {
+    stats->n_instances = 0;
+    stats->max_segment = 0;
+    stats->overflow = 0;
+    stats->n_sorted_big = 0;
+    for (int i = 0; i < 5; ++i) stats->reserved[i] = 0;
+}
+
+__global__ void capacity_kernel(us3d_stats* stats, int64_t k_cap) {
+    stats->overflow = stats->n_instances > k_cap ? 1 : 0;
+}
+
+// off[n] = total (or 0 when the batch overflowed), closing an (n+1)-entry offset array.
+__global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* total, const int32_t* overflow) {
+    off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
+}
+
+__device__ __forceinline__ int frame_of_pair(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int64_t pair) {
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int64_t local = pair - sw.pair_base;
+    return sw.f_begin + (int)(local % sw.f_count);
+}
+
+// Per pair: rank of each of its instances inside its tile's bucket (atomic counter per tile).
+__global__ void ug_count_kernel(int64_t n_pairs, const us3d_frame* __restrict__ frames,
+                                const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                                const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off,
+                                const int4* __restrict__ pair_rect, int32_t* __restrict__ tile_count,
+                                int32_t* __restrict__ inst_rank, const us3d_stats* stats) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pair >= n_pairs || stats->overflow) return;
+    if (pair_count[pair] == 0) return;
+    const us3d_frame& fr = frames[frame_of_pair(sweeps, n_sweeps, pair)];
+    const int4 r = pair_rect[pair];
+    int e = pair_off[pair];
+    for (int y = r.y; y < r.w; ++y)
+        for (int x = r.x; x < r.z; ++x) inst_rank[e++] = atomicAdd(&tile_count[fr.tile_base + y * fr.tiles_x + x], 1);
+}
+
+// Per pair: write (depth key << 32 | Gaussian, emit index) into each covered tile's bucket.
+__global__ void ug_emit_kernel(int64_t n_pairs, const us3d_frame* __restrict__ frames,
+                               const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                               const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off,
+                               const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
+                               const int32_t* __restrict__ tile_off, const int32_t* __restrict__ inst_rank,
+                               uint64_t* __restrict__ seg_key, int32_t* __restrict__ seg_val, const us3d_stats* stats) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pair >= n_pairs || stats->overflow) return;
+    if (pair_count[pair] == 0) return;
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int64_t local = pair - sw.pair_base;
+    const int gl = (int)(local / sw.f_count);
+    const int g = sw.g_begin + gl;
+    const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+    const int4 r = pair_rect[pair];
+    const uint64_t key = ((uint64_t)pair_dk[pair] << 32) | (uint32_t)g;
+    int e = pair_off[pair];
+    for (int y = r.y; y < r.w; ++y)
+        for (int x = r.x; x < r.z; ++x) {
+            const int slot = tile_off[fr.tile_base + y * fr.tiles_x + x] + inst_rank[e];
+            seg_key[slot] = key;
+            seg_val[slot] = e;
+            ++e;
+        }
+}
+
+// ---------------------------------------------------------------- splat geometry
+// A tile of TS x TS pixels is split into 8x8 sub-blocks; one wavefront owns a sub-block at a time, so a
+// test of a Gaussian against a sub-block is wavefront-uniform. Each Gaussian loaded into LDS carries a
+// bit mask of the sub-blocks its lateral/axial chi2 window can reach: a pixel with q <= chi2 satisfies
+// |(p - mu).a| <= r_a along both in-plane axes (r_a is the ellipsoid's marginal extent), so skipping a
+// sub-block outside the (margin-widened) window never drops a contributing pair.
+
+template <int TS>
+struct TileShape {
+    static constexpr int kSubPerRow = TS / 8;
+    static constexpr int kSub = kSubPerRow * kSubPerRow;
+    static constexpr int kThreads = (TS >= 16) ? 256 : 64;
+    static constexpr int kWaves = kThreads / kWave;
+    static constexpr int kSubPerWave = kSub / kWaves;  // pixels per thread
+};
+
+struct SubBounds {
+    float lat_lo, lat_hi, ax_lo, ax_hi;
+};
+
+// Pixel of sub-block sb handled by `lane`.
+template <int TS>
+__device__ __forceinline__ void sub_pixel(int tx, int ty, int sb, int lane, int& px, int& py) {
+    px = tx * TS + (sb % TileShape<TS>::kSubPerRow) * 8 + (lane & 7);
+    py = ty * TS + (sb / TileShape<TS>::kSubPerRow) * 8 + (lane >> 3);
+}
+
+// Lateral/axial extent of the valid pixels of a sub-block (wave reduction), widened for rounding.
+__device__ __forceinline__ SubBounds sub_bounds(bool valid, float lat, float ax) {
+    SubBounds b;
+    b.lat_lo = wave_min(valid ? lat : 3.0e38f);
+    b.lat_hi = wave_max(valid ? lat : -3.0e38f);
+    b.ax_lo = wave_min(valid ? ax : 3.0e38f);
+    b.ax_hi = wave_max(valid ? ax : -3.0e38f);
+    const float pad = 1e-3f + 4e-6f * fmaxf(fmaxf(fabsf(b.lat_lo), fabsf(b.lat_hi)), fmaxf(fabsf(b.ax_lo), fabsf(b.ax_hi)));
+    if (b.lat_lo <= b.lat_hi) {
+        b.lat_lo -= pad; b.lat_hi += pad; b.ax_lo -= pad; b.ax_hi += pad;
+    }
+    return b;
+}
+
+// Sub-block mask of one Gaussian (see above). Degenerate factors (l_ii <= 1e-10) get a full mask.
+template <int NSUB>
+__device__ __forceinline__ uint32_t gauss_sub_mask(const SubBounds* __restrict__ sb, float l11, float l12, float l13,
+                                                   float l22, float l23, float l33, float lat_g, float ax_g,
+                                                   const us3d_frame& fr, float sqrt_chi2) {
+    if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) return (NSUB >= 32) ? 0xffffffffu : ((1u << NSUB) - 1u);
+    const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+    const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+    const float wl = r_lat * 1.001f + 1e-3f + 4e-6f * fabsf(lat_g);
+    const float wa = r_ax * 1.001f + 1e-3f + 4e-6f * fabsf(ax_g);
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < NSUB; ++s) {
+        const bool hit = (lat_g + wl >= sb[s].lat_lo) && (lat_g - wl <= sb[s].lat_hi) && (ax_g + wa >= sb[s].ax_lo) &&
+                         (ax_g - wa <= sb[s].ax_hi);
+        m |= (hit ? 1u : 0u) << s;
+    }
+    return m;
+}
+
+__device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_tile_base, int n_frames, int t) {
+    return upper_index(frame_tile_base, n_frames, t);
+}
+
+// ---------------------------------------------------------------- forward splat
+// One workgroup per (frame, tile) work item. Pixel-parallel: each thread accumulates its pixel's sums over
+// the tile's sorted list, which is staged through LDS in batches of kThreads Gaussians (records gathered
+// with 16 B loads, read back as wavefront broadcasts). Follows kernel_pixel_ultragauss_forward,
+// ops/ultragauss_renderer/src/shader/forward.cu:17-76: alpha_hat = o * exp(-q/2) for q <= chi2 and
+// alpha_hat > alpha_min; A = sum alpha_hat + bg; image = clamp((sum alpha_hat c + bg I_bg) / max(A,1e-8));
+// depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8).
+
+template <int TS>
+__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const float* __restrict__ pixels, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
+    float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
+    float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux) {
+    using S = TileShape<TS>;
+    constexpr int NT = S::kThreads;
+    constexpr int PPT = S::kSubPerWave;
+    __shared__ float4 s_r0[NT], s_r1[NT], s_r2[NT];
+    __shared__ uint32_t s_mask[NT];
+    __shared__ SubBounds s_sb[S::kSub];
+    if (stats->overflow) return;
+
+    const int t = blockIdx.x;
+    const int f = frame_of_tile(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int lt = t - fr.tile_base;
+    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+
+    float px[PPT], py[PPT], pz[PPT];
+    bool valid[PPT];
+    int pidx[PPT];
+    float S_[PPT], N_[PPT], D_[PPT];
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int sb = wave + k * S::kWaves;
+        int ix, iy;
+        sub_pixel<TS>(tx, ty, sb, lane, ix, iy);
+        valid[k] = ix < fr.width && iy < fr.height;
+        pidx[k] = fr.pixel_base + iy * fr.width + ix;
+        px[k] = py[k] = pz[k] = 0.f;
+        if (valid[k]) pixel_position(fr, pixels, ix, iy, px[k], py[k], pz[k]);
+        const float dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
+        const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+        const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+        const SubBounds b = sub_bounds(valid[k], lat, ax);
+        if (lane == 0) s_sb[sb] = b;
+        S_[k] = N_[k] = D_[k] = 0.f;
+    }
+
+    for (int base = s0; base < s1; base += NT) {
+        __syncthreads();
+        const int n = min(NT, s1 - base);
+        if (tid < n) {
+            const int g = (int)(uint32_t)seg_key[base + tid];
+            const float4 r0 = rec[3 * (int64_t)g + 0];
+            const float4 r1 = rec[3 * (int64_t)g + 1];
+            const float4 r2 = rec[3 * (int64_t)g + 2];
+            const float vx = r0.x - ox, vy = r0.y - oy, vz = r0.z - oz;
+            const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+            const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+            const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+            s_mask[tid] = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, fr, sqrt_chi2);
+            s_r0[tid] = r0;                                  // mu, opacity
+            s_r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
+            s_r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
+        }
+        __syncthreads();
+        for (int j = 0; j < n; ++j) {
+            const uint32_t m = s_mask[j];
+            const float4 a = s_r0[j];
+            const float4 b = s_r1[j];
+            const float4 c = s_r2[j];
+#pragma unroll
+            for (int k = 0; k < PPT; ++k) {
+                if (!((m >> (wave + k * S::kWaves)) & 1u)) continue;
+                float y0, y1, y2;
+                const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z, y0, y1, y2);
+                if (q <= chi2) {
+                    const float ah = a.w * __expf(-0.5f * q);
+                    if (ah > alpha_min) {
+                        S_[k] += ah;
+                        N_[k] = __builtin_fmaf(ah, b.x, N_[k]);
+                        D_[k] = __builtin_fmaf(ah, b.y, D_[k]);
+                    }
+                }
+            }
+        }
+    }
+
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        if (!valid[k]) continue;
+        const float A = S_[k] + bg_alpha;
+        const float num = __builtin_fmaf(bg_alpha, bg_intensity, N_[k]);
+        const float img = num / fmaxf(A, 1e-8f);
+        out_image[pidx[k]] = fmaxf(0.0f, fminf(1.0f, img));
+        out_alpha[pidx[k]] = alpha_excludes_bg ? S_[k] : A;
+        out_depth[pidx[k]] = D_[k] / fmaxf(S_[k], 1e-8f);
+        out_aux[pidx[k]] = make_float4(S_[k], N_[k], D_[k], 0.f);
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+// One workgroup per (frame, tile). Per-pixel adjoint coefficients are computed once from the stored sums
+// and staged in LDS (sub-block-major); then each thread owns ONE Gaussian of the tile's list and loops over
+// the tile's pixels (LDS broadcasts), accumulating its 11 gradient terms in registers: no atomics, no
+// cross-lane reductions. Formulas follow kernel_pixel_ultragauss_backward,
+// ops/ultragauss_renderer/src/shader/backward.cu:17-131, with q = ||L^T d||^2:
+//   dN = gI/A', dX = gA - gI*num/A'^2 - gD*Dn/S'^2, dD = gD/S'  (A' = max(A,1e-8), S' = max(S,1e-8))
+//   dah = dN*c + dX + dD*elev; dc += dN*ah; do += dah*w; m = 2 dq = -dah*ah
+//   dmu = -L * sum(m y) + sum(dD*ah) * a_ele;  dl_ij = sum(m * y_j * d_i)
+// Pairs are gated exactly as in the forward (q <= chi2 and ah > alpha_min), so the adjoint is the exact
+// derivative of the forward. Partials go to the instance's emit index.
+
+template <int TS>
+__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels, float chi2,
+    float sqrt_chi2, float alpha_min, float bg_alpha, float bg_intensity, const float4* __restrict__ aux,
+    const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial) {
+    using S = TileShape<TS>;
+    constexpr int NT = S::kThreads;
+    constexpr int PPT = S::kSubPerWave;
+    constexpr int NPIX = TS * TS;
+    __shared__ float4 s_p[NPIX];   // px, py, pz, dN
+    __shared__ float2 s_c[NPIX];   // dX, dD
+    __shared__ SubBounds s_sb[S::kSub];
+    if (stats->overflow) return;
+
+    const int t = blockIdx.x;
+    const int f = frame_of_tile(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int lt = t - fr.tile_base;
+    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    if (s0 == s1) return;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) {
+        const int sb = wave + k * S::kWaves;
+        int ix, iy;
+        sub_pixel<TS>(tx, ty, sb, lane, ix, iy);
+        const bool valid = ix < fr.width && iy < fr.height;
+        float x = 0.f, y = 0.f, z = 0.f, dN = 0.f, dX = 0.f, dD = 0.f;
+        if (valid) {
+            const int64_t p = (int64_t)fr.pixel_base + iy * fr.width + ix;
+            pixel_position(fr, pixels, ix, iy, x, y, z);
+            const float4 sm = aux[p];
+            const float A = sm.x + bg_alpha;
+            const float inv = 1.0f / fmaxf(A, 1e-8f);
+            const float num = __builtin_fmaf(bg_alpha, bg_intensity, sm.y);
+            const float gi = d_image[p];
+            const float ga = d_alpha ? d_alpha[p] : 0.f;
+            const float gd = d_depth ? d_depth[p] : 0.f;
+            const float invS = 1.0f / fmaxf(sm.x, 1e-8f);
+            dN = gi * inv;
+            const float dA = ga - gi * num * inv * inv;
+            dD = gd * invS;
+            dX = dA - gd * sm.z * invS * invS;
+        }
+        const float dx = x - ox, dy = y - oy, dz = z - oz;
+        const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+        const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+        const SubBounds b = sub_bounds(valid, lat, ax);
+        if (lane == 0) s_sb[sb] = b;
+        s_p[sb * 64 + lane] = make_float4(x, y, z, dN);
+        s_c[sb * 64 + lane] = make_float2(dX, dD);
+    }
+    __syncthreads();
+
+    for (int base = s0; base < s1; base += NT) {
+        const int i = base + tid;
+        const bool active = i < s1;
+        float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+        uint32_t mask = 0;
+        float e = 0.f;
+        if (active) {
+            const int g = (int)(uint32_t)seg_key[i];
+            r0 = rec[3 * (int64_t)g + 0];
+            r1 = rec[3 * (int64_t)g + 1];
+            r2 = rec[3 * (int64_t)g + 2];
+            const float vx = r0.x - ox, vy = r0.y - oy, vz = r0.z - oz;
+            e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+            const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+            const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+            mask = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, fr, sqrt_chi2);
+        }
+        const float mx = r0.x, my = r0.y, mz = r0.z, o = r0.w, c = r1.x;
+        const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+        float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
+        float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
+        for (int sb = 0; sb < S::kSub; ++sb) {
+            const bool mine = (mask >> sb) & 1u;
+            if (!__any(mine)) continue;
+            if (!mine) continue;
+            for (int j = sb * 64; j < sb * 64 + 64; ++j) {
+                const float4 pp = s_p[j];
+                const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
+                float y0, y1, y2;
+                const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
+                if (q <= chi2) {
+                    const float w = __expf(-0.5f * q);
+                    const float ah = o * w;
+                    if (ah > alpha_min) {
+                        const float2 cc = s_c[j];
+                        const float da = __builtin_fmaf(cc.y, e, __builtin_fmaf(pp.w, c, cc.x));
+                        ac = __builtin_fmaf(pp.w, ah, ac);
+                        ao = __builtin_fmaf(da, w, ao);
+                        aed = __builtin_fmaf(cc.y, ah, aed);
+                        const float m = -da * ah;
+                        const float m0 = m * y0, m1 = m * y1, m2 = m * y2;
+                        Y0 += m0; Y1 += m1; Y2 += m2;
+                        g11 = __builtin_fmaf(m0, dx, g11);
+                        g12 = __builtin_fmaf(m0, dy, g12);
+                        g13 = __builtin_fmaf(m0, dz, g13);
+                        g22 = __builtin_fmaf(m1, dy, g22);
+                        g23 = __builtin_fmaf(m1, dz, g23);
+                        g33 = __builtin_fmaf(m2, dz, g33);
+                    }
+                }
+            }
+        }
+        if (active) {
+            const float dmx = __builtin_fmaf(aed, fr.axis_ele[0], -(l11 * Y0));
+            const float dmy = __builtin_fmaf(aed, fr.axis_ele[1], -__builtin_fmaf(l22, Y1, l12 * Y0));
+            const float dmz = __builtin_fmaf(aed, fr.axis_ele[2], -__builtin_fmaf(l33, Y2, __builtin_fmaf(l23, Y1, l13 * Y0)));
+            const int64_t ei = seg_val[i];
+            partial[3 * ei + 0] = make_float4(ao, ac, dmx, dmy);
+            partial[3 * ei + 1] = make_float4(dmz, g11, g12, g13);
+            partial[3 * ei + 2] = make_float4(g22, g23, g33, 0.f);
+        }
+    }
+}
+
+// One wavefront per Gaussian: lanes stride over the Gaussian's instances (contiguous in emit order across
+// all frames of its sweep), then a fixed butterfly reduction -- bitwise reproducible. Applies the raw
+// parameter chain d raw_ii = dl_ii * 2 raw_ii (backward.cu:124-129).
+__global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
+                                                        int n_sweeps, const int32_t* __restrict__ pair_off,
+                                                        const float4* __restrict__ partial,
+                                                        const float* __restrict__ tril,
+                                                        const us3d_stats* __restrict__ stats,
+                                                        float* __restrict__ d_means, float* __restrict__ d_tril,
+                                                        float* __restrict__ d_opacity,
+                                                        float* __restrict__ d_intensity) {
+    const int g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (g >= n_gauss || stats->overflow) return;
+    const us3d_sweep sw = sweeps[sweep_of_gauss(sweeps, n_sweeps, g)];
+    const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
+    const int e0 = pair_off[p0], e1 = pair_off[p0 + sw.f_count];
+    float acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0.f;
+    for (int e = e0 + lane; e < e1; e += kWave) {
+        const float4 a = partial[3 * (int64_t)e + 0];
+        const float4 b = partial[3 * (int64_t)e + 1];
+        const float4 c = partial[3 * (int64_t)e + 2];
+        acc[0] += a.x; acc[1] += a.y; acc[2] += a.z; acc[3] += a.w;
+        acc[4] += b.x; acc[5] += b.y; acc[6] += b.z; acc[7] += b.w;
+        acc[8] += c.x; acc[9] += c.y; acc[10] += c.z;
+    }
+#pragma unroll
+    for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+        const float* t = tril + (int64_t)g * 6;
+        d_opacity[g] = acc[0];
+        d_intensity[g] = acc[1];
+        d_means[3 * (int64_t)g + 0] = acc[2];
+        d_means[3 * (int64_t)g + 1] = acc[3];
+        d_means[3 * (int64_t)g + 2] = acc[4];
+        d_tril[6 * (int64_t)g + 0] = acc[5] * (2.0f * t[0]);
+        d_tril[6 * (int64_t)g + 1] = acc[6];
+        d_tril[6 * (int64_t)g + 2] = acc[7];
+        d_tril[6 * (int64_t)g + 3] = acc[8] * (2.0f * t[3]);
+        d_tril[6 * (int64_t)g + 4] = acc[9];
+        d_tril[6 * (int64_t)g + 5] = acc[10] * (2.0f * t[5]);
+    }
+}
+
+// ---------------------------------------------------------------- per-pixel candidate lists
+// Thread per pixel over its tile's sorted list, keeping g iff |(p-mu).a_lat| <= r_lat and
+// |(p-mu).a_ax| <= r_ax (pixel_hits_gaussian_probe_window, core.cu:166-182; count/fill core.cu:184-287).
+__global__ void ug_pixel_lists_kernel(int64_t n_pixels, const us3d_frame* __restrict__ frames, int n_frames,
+                                      const int32_t* __restrict__ frame_pixel_base, const us3d_sweep* __restrict__ sweeps,
+                                      const float4* __restrict__ rec, const int32_t* __restrict__ tile_off,
+                                      const uint64_t* __restrict__ seg_key, const float* __restrict__ pixels,
+                                      float sqrt_chi2, int ts, int pass, int32_t* __restrict__ counts,
+                                      const int32_t* __restrict__ offsets, int32_t* __restrict__ indices,
+                                      const us3d_stats* __restrict__ stats) {
+    const int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (p >= n_pixels || stats->overflow) return;
+    const int f = upper_index(frame_pixel_base, n_frames, (int)p);
+    const us3d_frame& fr = frames[f];
+    const int lp = (int)(p - fr.pixel_base);
+    const int ix = lp % fr.width, iy = lp / fr.width;
+    const int t = fr.tile_base + (iy / ts) * fr.tiles_x + ix / ts;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    float x, y, z;
+    pixel_position(fr, pixels, ix, iy, x, y, z);
+    const int g_begin = sweeps[fr.sweep].g_begin;
+    int c = 0;
+    int w = pass ? offsets[p] : 0;
+    for (int i = s0; i < s1; ++i) {
+        const int g = (int)(uint32_t)seg_key[i];
+        const float4 r0 = rec[3 * (int64_t)g + 0];
+        const float4 r1 = rec[3 * (int64_t)g + 1];
+        const float4 r2 = rec[3 * (int64_t)g + 2];
+        const float r_lat = marginal_radius(r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+        const float r_ax = marginal_radius(r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+        const float dx = x - r0.x, dy = y - r0.y, dz = z - r0.z;
+        const float dl = fabsf(dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]));
+        const float da = fabsf(dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]));
+        if (dl <= r_lat && da <= r_ax) {
+            if (pass) indices[w++] = g - g_begin;
+            ++c;
+        }
+    }
+    if (!pass) counts[p] = c;
+}
+
+int check_plan(const us3d_plan* plan, int64_t ws_needed) {
+    if (!plan) return fail(US3D_ERR_ARGUMENT, "plan is NULL");
+    if (plan->n_gauss < 0 || plan->n_frames <= 0 || plan->n_sweeps <= 0 || plan->n_tiles <= 0 || plan->n_pairs < 0)
+        return fail(US3D_ERR_ARGUMENT, "plan sizes out of range");
+    if (plan->k_capacity < 1 || plan->k_capacity >= (int64_t(1) << 31))
+        return fail(US3D_ERR_ARGUMENT, "k_capacity must be in [1, 2^31)");
+    if (!plan->workspace || plan->workspace_bytes < ws_needed)
+        return fail(US3D_ERR_ARGUMENT, "workspace missing or smaller than us3d_ug_workspace_bytes()");
+    if (!plan->frames || !plan->frame_tile_base || !plan->sweeps)
+        return fail(US3D_ERR_ARGUMENT, "plan tables are NULL");
+    return US3D_OK;
+}
+
+static int check_settings(const us3d_ug_settings* s) {
+    if (!s) return fail(US3D_ERR_ARGUMENT, "settings is NULL");
+    if (!(s->tile_size == 8 || s->tile_size == 16 || s->tile_size == 32))
+        return fail(US3D_ERR_ARGUMENT, "tile_size must be 8, 16 or 32");
+    if (!(s->chi2_threshold > 0.0f)) return fail(US3D_ERR_ARGUMENT, "chi2_threshold must be > 0");
+    if (!(s->diag_beta > 0.0f)) return fail(US3D_ERR_ARGUMENT, "diag_beta must be > 0");
+    return US3D_OK;
+}
+
+template <typename T>
+static T* at(const us3d_plan* plan, int64_t off) {
+    return ws_at<T>(plan, off);
+}
+
+int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
+    int rc;
+    us3d_stats* stats = at<us3d_stats>(plan, L.stats);
+    int32_t* pair_count = at<int32_t>(plan, L.pair_count);
+    int32_t* pair_off = at<int32_t>(plan, L.pair_off);
+    int4* pair_rect = at<int4>(plan, L.pair_rect);
+    uint32_t* pair_dk = at<uint32_t>(plan, L.pair_dk);
+    int32_t* tile_count = at<int32_t>(plan, L.tile_count);
+    int32_t* tile_off = at<int32_t>(plan, L.tile_off);
+    const int64_t np = plan->n_pairs;
+    if (np > 0) {
+        if ((rc = scan_i32(pair_count, pair_off, np, &stats->n_instances, at<void>(plan, L.scan_tmp), stream))) return rc;
+    } else {
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, pair_off, (int64_t)1, 0);
+    }
+    hipLaunchKernelGGL(capacity_kernel, dim3(1), dim3(1), 0, stream, stats, plan->k_capacity);
+    if ((rc = check_launch("capacity_kernel"))) return rc;
+    // Close pair_off with the total (offsets have n_pairs+1 entries).
+    hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, pair_off, np, &stats->n_instances,
+                       (const int32_t*)nullptr);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream, tile_count,
+                       (int64_t)plan->n_tiles, 0);
+    if ((rc = check_launch("fill_i32_kernel"))) return rc;
+    if (np > 0) {
+        hipLaunchKernelGGL(ug_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
+                           plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, tile_count,
+                           at<int32_t>(plan, L.inst_rank), (const us3d_stats*)stats);
+        if ((rc = check_launch("ug_count_kernel"))) return rc;
+    }
+    int64_t* tile_total = &stats->reserved[0];
+    if ((rc = scan_i32(tile_count, tile_off, plan->n_tiles, tile_total, at<void>(plan, L.scan_tmp), stream))) return rc;
+    hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, tile_off, (int64_t)plan->n_tiles,
+                       tile_total, (const int32_t*)&stats->overflow);
+    if (np > 0) {
+        hipLaunchKernelGGL(ug_emit_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
+                           plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, pair_dk, tile_off,
+                           at<int32_t>(plan, L.inst_rank), at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
+                           (const us3d_stats*)stats);
+        if ((rc = check_launch("ug_emit_kernel"))) return rc;
+    }
+    return sort_segments(at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<uint64_t>(plan, L.seg_key_tmp),
+                         at<int32_t>(plan, L.seg_val_tmp), tile_off, plan->n_tiles, stats, plan->k_capacity, stream);
+}
+
+}  // namespace us3d
+
+using namespace us3d;
+
+extern "C" {
+
+int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity) {
+    return ug_layout(n_gauss, n_pairs, n_tiles, k_capacity).total;
+}
+
+int us3d_ug_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                             us3d_layout* out) {
+    if (!out) return fail(US3D_ERR_ARGUMENT, "out is NULL");
+    *out = ug_layout(n_gauss, n_pairs, n_tiles, k_capacity);
+    return US3D_OK;
+}
+
+int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, const float* opacity,
+                const float* intensity, const us3d_ug_settings* settings, void* stream_) {
+    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                       plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    rc = check_settings(settings);
+    if (rc) return rc;
+    if (plan->n_gauss > 0 && (!means || !tril || !opacity || !intensity))
+        return fail(US3D_ERR_ARGUMENT, "Gaussian parameter pointer is NULL");
+    hipStream_t stream = as_stream(stream_);
+    us3d_stats* stats = at<us3d_stats>(plan, L.stats);
+    float4* rec = at<float4>(plan, L.records);
+    int32_t* pair_count = at<int32_t>(plan, L.pair_count);
+    int4* pair_rect = at<int4>(plan, L.pair_rect);
+    uint32_t* pair_dk = at<uint32_t>(plan, L.pair_dk);
+    const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
+
+    hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
+    if (plan->n_gauss > 0) {
+        hipLaunchKernelGGL(ug_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
+                           means, tril, opacity, intensity, settings->diag_beta, rec);
+        if ((rc = check_launch("ug_pack_kernel"))) return rc;
+    }
+    const int64_t np = plan->n_pairs;
+    if (np > 0) {
+        hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
+                           plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
+                           settings->sort_by_depth, pair_count, pair_rect, pair_dk);
+        if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
+    }
+    return build_instances(plan, L, stream);
+}
+
+int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
+                    float* alpha, float* depth, float* aux, void* stream_) {
+    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                       plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_settings(settings))) return rc;
+    if (!image || !alpha || !depth || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
+    hipStream_t stream = as_stream(stream_);
+    const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
+#define UG_FWD(TS)                                                                                                    \
+    hipLaunchKernelGGL(ug_forward_kernel<TS>, dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream,          \
+                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
+                       plan->frames, plan->frame_tile_base, plan->n_frames, pixels, settings->chi2_threshold,         \
+                       sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
+                       settings->alpha_excludes_bg, at<us3d_stats>(plan, L.stats), image, alpha, depth,              \
+                       reinterpret_cast<float4*>(aux))
+    switch (settings->tile_size) {
+        case 8: UG_FWD(8); break;
+        case 16: UG_FWD(16); break;
+        default: UG_FWD(32); break;
+    }
+#undef UG_FWD
+    return check_launch("ug_forward_kernel");
+}
+
+int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tril, const us3d_ug_settings* settings,
+                     const float* aux, const float* d_image, const float* d_alpha, const float* d_depth,
+                     float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream_) {
+    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                       plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_settings(settings))) return rc;
+    if (!tril || !aux || !d_image || !d_means || !d_tril || !d_opacity || !d_intensity)
+        return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
+    hipStream_t stream = as_stream(stream_);
+    const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
+#define UG_BWD(TS)                                                                                                    \
+    hipLaunchKernelGGL(ug_backward_kernel<TS>, dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream,         \
+                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
+                       at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
+                       settings->chi2_threshold, sqrt_chi2, settings->alpha_min, settings->bg_alpha,                  \
+                       settings->bg_intensity, reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,      \
+                       at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial))
+    switch (settings->tile_size) {
+        case 8: UG_BWD(8); break;
+        case 16: UG_BWD(16); break;
+        default: UG_BWD(32); break;
+    }
+#undef UG_BWD
+    if ((rc = check_launch("ug_backward_kernel"))) return rc;
+    if (plan->n_gauss > 0) {
+        hipLaunchKernelGGL(ug_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
+                           plan->sweeps, plan->n_sweeps, at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
+                           tril, at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
+        if ((rc = check_launch("ug_reduce_kernel"))) return rc;
+    }
+    return US3D_OK;
+}
+
+int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, int pass,
+                        int32_t* counts, const int32_t* offsets, int32_t* indices, void* stream_) {
+    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                       plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_settings(settings))) return rc;
+    if ((pass == 0 && !counts) || (pass != 0 && (!offsets || !indices)))
+        return fail(US3D_ERR_ARGUMENT, "pixel_lists: NULL array");
+    if (!plan->frame_pixel_base) return fail(US3D_ERR_ARGUMENT, "pixel_lists: frame_pixel_base is NULL");
+    const int32_t* frame_pixel_base = plan->frame_pixel_base;
+    hipStream_t stream = as_stream(stream_);
+    hipLaunchKernelGGL(ug_pixel_lists_kernel, dim3(blocks_for(plan->n_pixels, 256)), dim3(256), 0, stream,
+                       plan->n_pixels, plan->frames, plan->n_frames, frame_pixel_base, plan->sweeps,
+                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key), pixels,
+                       sqrtf(settings->chi2_threshold), settings->tile_size, pass != 0, counts, offsets, indices,
+                       at<us3d_stats>(plan, L.stats));
+    return check_launch("ug_pixel_lists_kernel");
+}
+
+}  // extern "C"

@@ -1,0 +1,684 @@
+// UltraG-Ray B-mode ray renderer on gfx950: mapper preprocess, scanline forward, thread-per-Gaussian adjoint
+// and per-Gaussian reduction. The instance stage (count, emit, per-ray LDS sort) is shared with the slice
+// splat: a (frame, Gaussian) pair carries the ray range as the tile rectangle (0, min_ray, 1, max_ray + 1) of a
+// frame whose "tiles" are its scanlines.
+//
+// Conventions for the quaternion/covariance helpers the reference includes but does not ship
+// (ops/ultragray_renderer/src/shader/forward.cu:1 includes covariance.cuh, absent from its tree):
+// q = (w, x, y, z), normalised inside; R row-major with the principal axes as columns (the reference
+// shader computes o_local = R^T (o - mu), shared.cuh:39-47); Sigma^-1 = R diag(1/s^2) R^T. The VJPs below
+// are the exact derivatives of these definitions.
+#include <algorithm>
+
+#include "common.h"
+
+namespace us3d {
+
+namespace {
+
+constexpr float kXi = 0.7745966692f;
+constexpr float kW0 = 0.5555555556f;
+constexpr float kW1 = 0.8888888889f;
+
+us3d_layout ur_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 64, 64);
+}
+
+__global__ void ur_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ scales,
+                               const float* __restrict__ quats, const float* __restrict__ atten,
+                               const float* __restrict__ sh, float4* __restrict__ rec) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const int64_t G = g;
+    rec[4 * G + 0] = make_float4(means[3 * G], means[3 * G + 1], means[3 * G + 2], atten[G]);
+    rec[4 * G + 1] = make_float4(scales[3 * G], scales[3 * G + 1], scales[3 * G + 2], 0.f);
+    rec[4 * G + 2] = make_float4(quats[4 * G], quats[4 * G + 1], quats[4 * G + 2], quats[4 * G + 3]);
+    rec[4 * G + 3] = make_float4(sh[4 * G], sh[4 * G + 1], sh[4 * G + 2], sh[4 * G + 3]);
+}
+
+// kernel_project_and_cull, ops/ultragray_renderer/src/mapper/core.cu:3-68: per (frame, Gaussian) the inclusive
+// scanline range [floor(c - r/ps), ceil(c + r/ps)] clipped to [0, W-1] with r = k * max(s); culled when
+// z + r < near or |elev| > r + margin (and, defined here, when the projection is not finite). Depth key =
+// (uint32)(max(0, z) * 1000) (core.cu:86), saturating.
+__global__ void ur_preprocess_kernel(int64_t n_pairs, const float4* __restrict__ rec,
+                                     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+                                     int n_sweeps, float k_sigma, float elev_margin, int32_t* __restrict__ pair_count,
+                                     int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (pair >= n_pairs) return;
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int64_t local = pair - sw.pair_base;
+    const int gl = (int)(local / sw.f_count);
+    const int g = sw.g_begin + gl;
+    const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+    const float4 r0 = rec[4 * (int64_t)g + 0];
+    const float4 r1 = rec[4 * (int64_t)g + 1];
+    const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
+    const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+    const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+    const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+    const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
+    const float r = ms * k_sigma;
+    int count = 0;
+    int4 rect = make_int4(0, 0, 0, 0);
+    uint32_t dk = 0;
+    if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
+        const float c = x / fr.delta_lat + fr.col_offset;
+        const float rp = r / fr.delta_lat;
+        const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
+        if (isfinite(f0) && isfinite(f1)) {
+            const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
+            const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
+            if (lo <= hi) {
+                count = hi - lo + 1;
+                rect = make_int4(0, lo, 1, hi + 1);
+                const float q = fmaxf(0.0f, z) * 1000.0f;
+                dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
+            }
+        }
+    }
+    pair_count[pair] = count;
+    pair_rect[pair] = rect;
+    pair_dk[pair] = dk;
+}
+
+// ---------------------------------------------------------------- per (ray, Gaussian) setup
+
+__device__ __forceinline__ void quat_rot(float4 q, float R[9]) {
+    const float n = sqrtf(fmaxf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w, 1e-12f));
+    const float w = q.x / n, x = q.y / n, y = q.z / n, z = q.w / n;
+    R[0] = 1.f - 2.f * (y * y + z * z); R[1] = 2.f * (x * y - w * z);       R[2] = 2.f * (x * z + w * y);
+    R[3] = 2.f * (x * y + w * z);       R[4] = 1.f - 2.f * (x * x + z * z); R[5] = 2.f * (y * z - w * x);
+    R[6] = 2.f * (x * z - w * y);       R[7] = 2.f * (y * z + w * x);       R[8] = 1.f - 2.f * (x * x + y * y);
+}
+
+// dL/dq of the unnormalised quaternion from dL/dR (through the normalisation).
+__device__ __forceinline__ float4 quat_vjp(float4 q, const float dR[9]) {
+    const float n = sqrtf(fmaxf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w, 1e-12f));
+    const float w = q.x / n, x = q.y / n, y = q.z / n, z = q.w / n;
+    const float gw = 2.f * (-z * dR[1] + y * dR[2] + z * dR[3] - x * dR[5] - y * dR[6] + x * dR[7]);
+    const float gx = 2.f * (y * dR[1] + z * dR[2] + y * dR[3] - 2.f * x * dR[4] - w * dR[5] + z * dR[6] + w * dR[7] -
+                            2.f * x * dR[8]);
+    const float gy = 2.f * (-2.f * y * dR[0] + x * dR[1] + w * dR[2] + x * dR[3] + z * dR[5] - w * dR[6] + z * dR[7] -
+                            2.f * y * dR[8]);
+    const float gz = 2.f * (-2.f * z * dR[0] - w * dR[1] + x * dR[2] + w * dR[3] - 2.f * z * dR[4] + y * dR[5] +
+                            x * dR[6] + y * dR[7]);
+    const float dot = gw * w + gx * x + gy * y + gz * z;
+    return make_float4((gw - dot * w) / n, (gx - dot * x) / n, (gy - dot * y) / n, (gz - dot * z) / n);
+}
+
+// 3-point Gauss-Legendre approximation of the ray integral over [ts, te] in whitened coordinates
+// (compute_ray_gaussian_integral, shared.cuh:27-85).
+__device__ __forceinline__ float gl3(const float og[3], const float dg[3], float ts, float te) {
+    const float half = 0.5f * (te - ts);
+    const float c = 0.5f * (te + ts);
+    const float xi[3] = {-kXi, 0.0f, kXi};
+    const float wq[3] = {kW0, kW1, kW0};
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float t = __builtin_fmaf(half, xi[i], c);
+        const float x0 = __builtin_fmaf(dg[0], t, og[0]);
+        const float x1 = __builtin_fmaf(dg[1], t, og[1]);
+        const float x2 = __builtin_fmaf(dg[2], t, og[2]);
+        acc = __builtin_fmaf(wq[i], __expf(-0.5f * dot3(x0, x1, x2, x0, x1, x2)), acc);
+    }
+    return half * acc;
+}
+
+// Everything about one Gaussian that is fixed along one ray.
+struct RayGauss {
+    float tg, rd, tc, tau;      // centre depth along the ray, echo radius, GL3 window centre, sigmoid(raw)
+    float og[3], dg[3];         // whitened ray origin / direction
+    float ts, b, f_final, e_final;  // window start/end, transmittance and exp(-psi) of the complete window
+    float mu[3], I;             // mean, SH intensity along the ray direction
+    float S00, S01, S02, S11, S12, S22;  // Sigma^-1
+    float dkf;                  // sorted depth (key / 1000)
+};
+
+__device__ __forceinline__ void ray_gauss(const float4* __restrict__ rec, int g, uint32_t dkey, const float ro[3],
+                                          const float rdir[3], float k_sigma, float Rw, RayGauss& G, float R[9]) {
+    const float4 r0 = rec[4 * (int64_t)g + 0];
+    const float4 r1 = rec[4 * (int64_t)g + 1];
+    const float4 r2 = rec[4 * (int64_t)g + 2];
+    const float4 r3 = rec[4 * (int64_t)g + 3];
+    G.mu[0] = r0.x; G.mu[1] = r0.y; G.mu[2] = r0.z;
+    G.tg = dot3(r0.x - ro[0], r0.y - ro[1], r0.z - ro[2], rdir[0], rdir[1], rdir[2]);
+    G.rd = fmaxf(r1.x, fmaxf(r1.y, r1.z)) * k_sigma;
+    quat_rot(r2, R);
+    G.tau = 1.0f / (1.0f + __expf(-r0.w));
+    const float diff[3] = {ro[0] - r0.x, ro[1] - r0.y, ro[2] - r0.z};
+    const float s[3] = {r1.x, r1.y, r1.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const float ol = __builtin_fmaf(R[6 + c], diff[2], __builtin_fmaf(R[3 + c], diff[1], R[c] * diff[0]));
+        const float dl = __builtin_fmaf(R[6 + c], rdir[2], __builtin_fmaf(R[3 + c], rdir[1], R[c] * rdir[0]));
+        G.og[c] = ol / s[c];
+        G.dg[c] = dl / s[c];
+    }
+    const float dn2 = dot3(G.dg[0], G.dg[1], G.dg[2], G.dg[0], G.dg[1], G.dg[2]);
+    G.tc = -dot3(G.og[0], G.og[1], G.og[2], G.dg[0], G.dg[1], G.dg[2]) / (dn2 + 1e-8f);
+    G.ts = fmaxf(0.0f, G.tc - Rw);
+    G.b = G.tc + Rw;
+    float psi = 0.f;
+    if (Rw > 0.0f && G.b > 0.0f && G.b > G.ts) psi = gl3(G.og, G.dg, G.ts, G.b);
+    G.e_final = __expf(-psi);
+    G.f_final = __builtin_fmaf(1.0f - G.tau, G.e_final, G.tau);
+    G.I = r3.x + r3.y * rdir[0] + r3.z * rdir[1] + r3.w * rdir[2];
+    const float d0 = 1.0f / (s[0] * s[0]), d1 = 1.0f / (s[1] * s[1]), d2 = 1.0f / (s[2] * s[2]);
+    auto sij = [&](int i, int j) { return R[3 * i] * d0 * R[3 * j] + R[3 * i + 1] * d1 * R[3 * j + 1] + R[3 * i + 2] * d2 * R[3 * j + 2]; };
+    G.S00 = sij(0, 0); G.S01 = sij(0, 1); G.S02 = sij(0, 2);
+    G.S11 = sij(1, 1); G.S12 = sij(1, 2); G.S22 = sij(2, 2);
+    G.dkf = (float)dkey * 0.001f;
+}
+
+struct RayFlags {
+    float k_sigma, Rw, local_max, grad_threshold;
+    int local_only;
+};
+
+// Which terms of pixel (depth tpx) this Gaussian enters (forward.cu:76-96).
+__device__ __forceinline__ void ray_terms(const RayGauss& G, float tpx, const RayFlags& F, bool& trans, bool& echo) {
+    bool inside = true;
+    if (F.local_only) inside = (G.dkf >= tpx - F.local_max) && !(G.dkf > tpx + F.local_max);
+    trans = inside && (G.tg < tpx);
+    if (F.local_only) trans = trans && ((tpx - G.tg) <= G.rd + F.Rw);
+    echo = inside && (fabsf(G.tg - tpx) <= G.rd);
+}
+
+// GL3 window of pixel depth l: returns false when psi = 0 (empty window or l <= 0).
+__device__ __forceinline__ bool ray_window(const RayGauss& G, float l, float Rw, float& ts, float& te) {
+    if (!(l > 0.0f)) return false;
+    ts = Rw > 0.0f ? G.ts : 0.0f;
+    te = Rw > 0.0f ? fminf(l, G.b) : l;
+    return te > ts;
+}
+
+__device__ __forceinline__ float echo_weight(const RayGauss& G, float px, float py, float pz, float& dx, float& dy,
+                                             float& dz, float& v0, float& v1, float& v2) {
+    dx = px - G.mu[0];
+    dy = py - G.mu[1];
+    dz = pz - G.mu[2];
+    v0 = __builtin_fmaf(G.S02, dz, __builtin_fmaf(G.S01, dy, G.S00 * dx));
+    v1 = __builtin_fmaf(G.S12, dz, __builtin_fmaf(G.S11, dy, G.S01 * dx));
+    v2 = __builtin_fmaf(G.S22, dz, __builtin_fmaf(G.S12, dy, G.S02 * dx));
+    return __expf(-0.5f * dot3(v0, v1, v2, dx, dy, dz));
+}
+
+__device__ __forceinline__ void ray_origin(const us3d_frame& fr, int ray, const float* __restrict__ ray_o,
+                                           const float* __restrict__ ray_d, const float* __restrict__ ray_off,
+                                           float ro[3], float rd[3]) {
+    const float delta = ray_off ? ray_off[ray] : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ro[k] = __builtin_fmaf(fr.axis_ele[k], delta, ray_o[3 * (int64_t)ray + k]);
+        rd[k] = ray_d[3 * (int64_t)ray + k];
+    }
+}
+
+// LDS image of a RayGauss (24 floats).
+struct RGPack {
+    float4 a, b, c, d, e, f;
+};
+
+__device__ __forceinline__ RGPack pack_rg(const RayGauss& G) {
+    RGPack p;
+    p.a = make_float4(G.tg, G.rd, G.tau, G.f_final);
+    p.b = make_float4(G.og[0], G.og[1], G.og[2], G.ts);
+    p.c = make_float4(G.dg[0], G.dg[1], G.dg[2], G.b);
+    p.d = make_float4(G.mu[0], G.mu[1], G.mu[2], G.I);
+    p.e = make_float4(G.S00, G.S01, G.S02, G.S11);
+    p.f = make_float4(G.S12, G.S22, G.dkf, G.e_final);
+    return p;
+}
+
+__device__ __forceinline__ void unpack_rg(const RGPack& p, RayGauss& G) {
+    G.tg = p.a.x; G.rd = p.a.y; G.tau = p.a.z; G.f_final = p.a.w;
+    G.og[0] = p.b.x; G.og[1] = p.b.y; G.og[2] = p.b.z; G.ts = p.b.w;
+    G.dg[0] = p.c.x; G.dg[1] = p.c.y; G.dg[2] = p.c.z; G.b = p.c.w;
+    G.mu[0] = p.d.x; G.mu[1] = p.d.y; G.mu[2] = p.d.z; G.I = p.d.w;
+    G.S00 = p.e.x; G.S01 = p.e.y; G.S02 = p.e.z; G.S11 = p.e.w;
+    G.S12 = p.f.x; G.S22 = p.f.y; G.dkf = p.f.z; G.e_final = p.f.w;
+    G.tc = 0.f;
+}
+
+// ---------------------------------------------------------------- forward
+// One workgroup per (frame, scanline). Threads own depth samples (kPPT per thread per chunk); the ray's sorted
+// hit list is staged through LDS in batches with everything that is fixed along the ray precomputed once
+// per (ray, Gaussian) -- rotation, whitened origin/direction, window, Sigma^-1, SH intensity and the
+// transmittance of the complete window, which every sample beyond the window reuses (the reference
+// recomputes all of it for every (sample, hit)). Per sample: T = prod over hits in front of
+// tau + (1 - tau) e^-psi, echo = (1 - e^-S) sum(I w) / (sum w + 1e-6), B = T * echo (forward.cu:22-119).
+
+constexpr int kRayThreads = 256;
+constexpr int kPPT = 4;
+
+__global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const float* __restrict__ pixels, const float* __restrict__ ray_o, const float* __restrict__ ray_d,
+    const float* __restrict__ ray_off, RayFlags F, const us3d_stats* __restrict__ stats, float* __restrict__ image,
+    float4* __restrict__ aux) {
+    __shared__ RGPack s_g[kRayThreads];
+    if (stats->overflow) return;
+    const int t = blockIdx.x;
+    const int f = upper_index(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int x = t - fr.tile_base;
+    const int W = fr.width, H = fr.height;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    float ro[3], rd[3];
+    ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
+    const int tid = threadIdx.x;
+
+    for (int c0 = 0; c0 < H; c0 += kRayThreads * kPPT) {
+        float px[kPPT], py[kPPT], pz[kPPT], tpx[kPPT], T[kPPT], num[kPPT], den[kPPT];
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) {
+            const int y = c0 + tid + k * kRayThreads;
+            px[k] = py[k] = pz[k] = tpx[k] = 0.f;
+            if (y < H) {
+                const float* p = pixels + 3 * ((int64_t)fr.pixel_base + (int64_t)x * H + y);
+                px[k] = p[0]; py[k] = p[1]; pz[k] = p[2];
+                tpx[k] = fmaxf(0.0f, dot3(px[k] - ro[0], py[k] - ro[1], pz[k] - ro[2], rd[0], rd[1], rd[2]));
+            }
+            T[k] = 1.f; num[k] = 0.f; den[k] = 0.f;
+        }
+        for (int base = s0; base < s1; base += kRayThreads) {
+            __syncthreads();
+            const int n = min(kRayThreads, s1 - base);
+            if (tid < n) {
+                const uint64_t key = seg_key[base + tid];
+                RayGauss G;
+                float R[9];
+                ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
+                s_g[tid] = pack_rg(G);
+            }
+            __syncthreads();
+            for (int j = 0; j < n; ++j) {
+                RayGauss G;
+                unpack_rg(s_g[j], G);
+#pragma unroll
+                for (int k = 0; k < kPPT; ++k) {
+                    bool trans, echo;
+                    ray_terms(G, tpx[k], F, trans, echo);
+                    if (trans) {
+                        if (F.Rw > 0.0f && tpx[k] >= G.b) {
+                            T[k] *= G.f_final;
+                        } else {
+                            float ts, te, psi = 0.f;
+                            if (ray_window(G, tpx[k], F.Rw, ts, te)) psi = gl3(G.og, G.dg, ts, te);
+                            T[k] *= __builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
+                        }
+                    }
+                    if (echo) {
+                        float dx, dy, dz, v0, v1, v2;
+                        const float w = echo_weight(G, px[k], py[k], pz[k], dx, dy, dz, v0, v1, v2);
+                        num[k] = __builtin_fmaf(G.I, w, num[k]);
+                        den[k] += w;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPPT; ++k) {
+            const int y = c0 + tid + k * kRayThreads;
+            if (y >= H) continue;
+            const int64_t o = (int64_t)fr.pixel_base + (int64_t)y * W + x;
+            float B = 0.f;
+            if (s1 > s0) {
+                const float gamma = 1.0f - __expf(-den[k]);
+                const float E = num[k] / (den[k] + 1e-6f);
+                B = T[k] * (gamma * E);
+            }
+            image[o] = B;
+            aux[o] = make_float4(T[k], num[k], den[k], 0.f);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- adjoint
+// One workgroup per (frame, scanline); each thread owns one hit (Gaussian) of the ray and walks the ray's
+// samples (per-sample adjoint coefficients staged in LDS, read as broadcasts), accumulating in registers:
+//   transmission: dL/dT_i = (dL/dT T) / T_i when T_i > 1e-6 (backward.cu:273-274); d raw tau and d psi;
+//     the GL3 VJP holds the window endpoints fixed (backward.cu:57-72); samples past the window share one
+//     psi, so their d psi is summed and pushed through the quadrature nodes once;
+//   echo: dL/dw = dNum I + dDen + dS; d mu, d Sigma^-1 accumulated as sums over samples;
+// then mapped once per (ray, Gaussian) to (mu, s, q, raw tau, sh) and written at the instance's emit index.
+// Samples with |dL/dB| < grad_threshold contribute nothing (backward.cu:170).
+
+constexpr int kRayChunk = 1024;
+
+__global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
+    const float* __restrict__ ray_o, const float* __restrict__ ray_d, const float* __restrict__ ray_off, RayFlags F,
+    const float4* __restrict__ aux, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
+    float4* __restrict__ partial) {
+    __shared__ float4 s_p[kRayChunk];  // px, py, pz, tpx
+    __shared__ float4 s_c[kRayChunk];  // cT, dNum, dW0, 0
+    if (stats->overflow) return;
+    const int t = blockIdx.x;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    if (s0 == s1) return;
+    const int f = upper_index(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int x = t - fr.tile_base;
+    const int W = fr.width, H = fr.height;
+    float ro[3], rd[3];
+    ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
+    const int tid = threadIdx.x;
+
+    for (int base = s0; base < s1; base += kRayThreads) {
+        const int i = base + tid;
+        const bool active = i < s1;
+        RayGauss G;
+        float R[9];
+        uint64_t key = active ? seg_key[i] : seg_key[s0];
+        ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
+        float dog[3] = {0.f, 0.f, 0.f}, ddg[3] = {0.f, 0.f, 0.f};
+        float fin = 0.f, dtau = 0.f, dsh = 0.f;
+        float u[3] = {0.f, 0.f, 0.f};
+        float g00 = 0.f, g01 = 0.f, g02 = 0.f, g11 = 0.f, g12 = 0.f, g22 = 0.f;
+
+        for (int c0 = 0; c0 < H; c0 += kRayChunk) {
+            const int n = min(kRayChunk, H - c0);
+            __syncthreads();
+            for (int j = tid; j < n; j += kRayThreads) {
+                const int y = c0 + j;
+                const float* p = pixels + 3 * ((int64_t)fr.pixel_base + (int64_t)x * H + y);
+                const float tp = fmaxf(0.0f, dot3(p[0] - ro[0], p[1] - ro[1], p[2] - ro[2], rd[0], rd[1], rd[2]));
+                const int64_t o = (int64_t)fr.pixel_base + (int64_t)y * W + x;
+                const float gB = d_image[o];
+                float cT = 0.f, dNum = 0.f, dW0 = 0.f;
+                if (!(fabsf(gB) < F.grad_threshold)) {
+                    const float4 a = aux[o];  // T, num, den
+                    const float gamma = 1.0f - __expf(-a.z);
+                    const float inv = 1.0f / (a.z + 1e-6f);
+                    const float Eraw = a.y * inv;
+                    const float dT = gB * (gamma * Eraw);
+                    const float dE = gB * a.x;
+                    const float dEraw = dE * gamma;
+                    const float dS = dE * Eraw * __expf(-a.z);
+                    dNum = dEraw * inv;
+                    dW0 = -dEraw * a.y * inv * inv + dS;
+                    cT = dT * a.x;
+                }
+                s_p[j] = make_float4(p[0], p[1], p[2], tp);
+                s_c[j] = make_float4(cT, dNum, dW0, 0.f);
+            }
+            __syncthreads();
+            if (!active) continue;
+            for (int j = 0; j < n; ++j) {
+                const float4 pp = s_p[j];
+                bool trans, echo;
+                ray_terms(G, pp.w, F, trans, echo);
+                if (!trans && !echo) continue;
+                const float4 cc = s_c[j];
+                if (trans) {
+                    float ts = 0.f, te = 0.f, ep, Ti;
+                    const bool finished = F.Rw > 0.0f && pp.w >= G.b;
+                    bool win = false;
+                    if (finished) {
+                        ep = G.e_final;
+                        Ti = G.f_final;
+                    } else {
+                        win = ray_window(G, pp.w, F.Rw, ts, te);
+                        ep = win ? __expf(-gl3(G.og, G.dg, ts, te)) : 1.0f;
+                        Ti = __builtin_fmaf(1.0f - G.tau, ep, G.tau);
+                    }
+                    const float dTi = Ti > 1e-6f ? cc.x / Ti : 0.f;
+                    dtau = __builtin_fmaf(dTi, 1.0f - ep, dtau);
+                    const float dpsi = -dTi * (1.0f - G.tau) * ep;
+                    if (finished) {
+                        fin += dpsi;
+                    } else if (win) {
+                        const float half = 0.5f * (te - ts), c = 0.5f * (te + ts);
+                        const float coeff = dpsi * half;
+                        const float xi[3] = {-kXi, 0.0f, kXi};
+                        const float wq[3] = {kW0, kW1, kW0};
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) {
+                            const float tq = __builtin_fmaf(half, xi[q], c);
+                            const float x0 = __builtin_fmaf(G.dg[0], tq, G.og[0]);
+                            const float x1 = __builtin_fmaf(G.dg[1], tq, G.og[1]);
+                            const float x2 = __builtin_fmaf(G.dg[2], tq, G.og[2]);
+                            const float a = -coeff * wq[q] * __expf(-0.5f * dot3(x0, x1, x2, x0, x1, x2));
+                            dog[0] = __builtin_fmaf(a, x0, dog[0]);
+                            dog[1] = __builtin_fmaf(a, x1, dog[1]);
+                            dog[2] = __builtin_fmaf(a, x2, dog[2]);
+                            const float at = a * tq;
+                            ddg[0] = __builtin_fmaf(at, x0, ddg[0]);
+                            ddg[1] = __builtin_fmaf(at, x1, ddg[1]);
+                            ddg[2] = __builtin_fmaf(at, x2, ddg[2]);
+                        }
+                    }
+                }
+                if (echo) {
+                    float dx, dy, dz, v0, v1, v2;
+                    const float w = echo_weight(G, pp.x, pp.y, pp.z, dx, dy, dz, v0, v1, v2);
+                    dsh = __builtin_fmaf(cc.y, w, dsh);
+                    const float dw = __builtin_fmaf(cc.y, G.I, cc.z);
+                    const float dQ = -0.5f * dw * w;
+                    u[0] = __builtin_fmaf(dQ, dx, u[0]);
+                    u[1] = __builtin_fmaf(dQ, dy, u[1]);
+                    u[2] = __builtin_fmaf(dQ, dz, u[2]);
+                    const float qx = dQ * dx, qy = dQ * dy;
+                    g00 = __builtin_fmaf(qx, dx, g00);
+                    g01 = __builtin_fmaf(qx, dy, g01);
+                    g02 = __builtin_fmaf(qx, dz, g02);
+                    g11 = __builtin_fmaf(qy, dy, g11);
+                    g12 = __builtin_fmaf(qy, dz, g12);
+                    g22 = __builtin_fmaf(dQ * dz, dz, g22);
+                }
+            }
+        }
+        if (!active) continue;
+        // samples past the window: one VJP with the summed d psi
+        if (fin != 0.0f && G.b > G.ts && G.b > 0.0f) {
+            const float ts = G.ts, te = G.b;
+            const float half = 0.5f * (te - ts), c = 0.5f * (te + ts);
+            const float coeff = fin * half;
+            const float xi[3] = {-kXi, 0.0f, kXi};
+            const float wq[3] = {kW0, kW1, kW0};
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                const float tq = __builtin_fmaf(half, xi[q], c);
+                const float x0 = __builtin_fmaf(G.dg[0], tq, G.og[0]);
+                const float x1 = __builtin_fmaf(G.dg[1], tq, G.og[1]);
+                const float x2 = __builtin_fmaf(G.dg[2], tq, G.og[2]);
+                const float a = -coeff * wq[q] * __expf(-0.5f * dot3(x0, x1, x2, x0, x1, x2));
+                dog[0] += a * x0; dog[1] += a * x1; dog[2] += a * x2;
+                ddg[0] += a * tq * x0; ddg[1] += a * tq * x1; ddg[2] += a * tq * x2;
+            }
+        }
+        const float4 r1 = rec[4 * (int64_t)(uint32_t)key + 1];
+        const float4 r2 = rec[4 * (int64_t)(uint32_t)key + 2];
+        const float s[3] = {r1.x, r1.y, r1.z};
+        const float diff[3] = {ro[0] - G.mu[0], ro[1] - G.mu[1], ro[2] - G.mu[2]};
+        float dmu[3], ds[3], dR[9];
+        float dol[3], ddl[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            dol[c] = dog[c] / s[c];
+            ddl[c] = ddg[c] / s[c];
+            ds[c] = dog[c] * (-G.og[c] / s[c]) + ddg[c] * (-G.dg[c] / s[c]);
+        }
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            dmu[r] = -(R[3 * r] * dol[0] + R[3 * r + 1] * dol[1] + R[3 * r + 2] * dol[2]);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) dR[3 * r + c] = dol[c] * diff[r] + ddl[c] * rd[r];
+        }
+        // echo: d mu = -2 Sigma^-1 u; Sigma^-1 = R D R^T with D = diag(1/s^2): dR += 2 Gm R D (Gm symmetric),
+        // ds_k += (R^T Gm R)_kk * (-2 / s_k^3)
+        dmu[0] += -2.0f * (G.S00 * u[0] + G.S01 * u[1] + G.S02 * u[2]);
+        dmu[1] += -2.0f * (G.S01 * u[0] + G.S11 * u[1] + G.S12 * u[2]);
+        dmu[2] += -2.0f * (G.S02 * u[0] + G.S12 * u[1] + G.S22 * u[2]);
+        const float Gm[9] = {g00, g01, g02, g01, g11, g12, g02, g12, g22};
+        const float D[3] = {1.0f / (s[0] * s[0]), 1.0f / (s[1] * s[1]), 1.0f / (s[2] * s[2])};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            float gr[3];  // Gm * column k of R
+#pragma unroll
+            for (int a = 0; a < 3; ++a) gr[a] = Gm[3 * a] * R[k] + Gm[3 * a + 1] * R[3 + k] + Gm[3 * a + 2] * R[6 + k];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) dR[3 * a + k] += 2.0f * gr[a] * D[k];
+            const float rgr = R[k] * gr[0] + R[3 + k] * gr[1] + R[6 + k] * gr[2];
+            ds[k] += rgr * (-2.0f / (s[k] * s[k] * s[k]));
+        }
+        const float4 dq = quat_vjp(r2, dR);
+        const float dt = dtau * G.tau * (1.0f - G.tau);
+        const int64_t e = seg_val[i];
+        partial[4 * e + 0] = make_float4(dmu[0], dmu[1], dmu[2], ds[0]);
+        partial[4 * e + 1] = make_float4(ds[1], ds[2], dt, 0.f);
+        partial[4 * e + 2] = dq;
+        partial[4 * e + 3] = make_float4(dsh, dsh * rd[0], dsh * rd[1], dsh * rd[2]);
+    }
+}
+
+__global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
+                                                        int n_sweeps, const int32_t* __restrict__ pair_off,
+                                                        const float4* __restrict__ partial,
+                                                        const us3d_stats* __restrict__ stats, float* __restrict__ d_means,
+                                                        float* __restrict__ d_scales, float* __restrict__ d_quats,
+                                                        float* __restrict__ d_atten, float* __restrict__ d_sh) {
+    const int g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (g >= n_gauss || stats->overflow) return;
+    const us3d_sweep sw = sweeps[sweep_of_gauss(sweeps, n_sweeps, g)];
+    const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
+    const int e0 = pair_off[p0], e1 = pair_off[p0 + sw.f_count];
+    float acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0.f;
+    for (int e = e0 + lane; e < e1; e += kWave) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const float4 a = partial[4 * (int64_t)e + v];
+            acc[4 * v] += a.x; acc[4 * v + 1] += a.y; acc[4 * v + 2] += a.z; acc[4 * v + 3] += a.w;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = wave_sum(acc[k]);
+    if (lane == 0) {
+        const int64_t G = g;
+        d_means[3 * G] = acc[0]; d_means[3 * G + 1] = acc[1]; d_means[3 * G + 2] = acc[2];
+        d_scales[3 * G] = acc[3]; d_scales[3 * G + 1] = acc[4]; d_scales[3 * G + 2] = acc[5];
+        d_atten[G] = acc[6];
+        d_quats[4 * G] = acc[8]; d_quats[4 * G + 1] = acc[9]; d_quats[4 * G + 2] = acc[10]; d_quats[4 * G + 3] = acc[11];
+        d_sh[4 * G] = acc[12]; d_sh[4 * G + 1] = acc[13]; d_sh[4 * G + 2] = acc[14]; d_sh[4 * G + 3] = acc[15];
+    }
+}
+
+int check_ur_settings(const us3d_ur_settings* s) {
+    if (!s) return fail(US3D_ERR_ARGUMENT, "settings is NULL");
+    if (!(s->k_sigma > 0.0f)) return fail(US3D_ERR_ARGUMENT, "k_sigma must be > 0");
+    if (!(s->quadrature_window_radius_mm >= 0.0f)) return fail(US3D_ERR_ARGUMENT, "quadrature_window_radius_mm must be >= 0");
+    if (!(s->local_transmission_max_window_mm > 0.0f)) return fail(US3D_ERR_ARGUMENT, "local_transmission_max_window_mm must be > 0");
+    return US3D_OK;
+}
+
+RayFlags flags_of(const us3d_ur_settings* s) {
+    RayFlags F;
+    F.k_sigma = s->k_sigma;
+    F.Rw = s->quadrature_window_radius_mm;
+    F.local_max = s->local_transmission_max_window_mm;
+    F.grad_threshold = s->grad_threshold;
+    F.local_only = s->local_transmission_window_only;
+    return F;
+}
+
+}  // namespace
+
+}  // namespace us3d
+
+using namespace us3d;
+
+extern "C" {
+
+int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity) {
+    return ur_layout(n_gauss, n_pairs, n_rays, k_capacity).total;
+}
+
+int us3d_ur_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity, us3d_layout* out) {
+    if (!out) return fail(US3D_ERR_ARGUMENT, "out is NULL");
+    *out = ur_layout(n_gauss, n_pairs, n_rays, k_capacity);
+    return US3D_OK;
+}
+
+int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, const float* quats,
+                const float* attenuations, const float* sh, const us3d_ur_settings* settings, void* stream_) {
+    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                    plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_ur_settings(settings))) return rc;
+    if (plan->n_gauss > 0 && (!means || !scales || !quats || !attenuations || !sh))
+        return fail(US3D_ERR_ARGUMENT, "Gaussian parameter pointer is NULL");
+    hipStream_t stream = as_stream(stream_);
+    us3d_stats* stats = ws_at<us3d_stats>(plan, L.stats);
+    float4* rec = ws_at<float4>(plan, L.records);
+    hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
+    if (plan->n_gauss > 0) {
+        hipLaunchKernelGGL(ur_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
+                           means, scales, quats, attenuations, sh, rec);
+        if ((rc = check_launch("ur_pack_kernel"))) return rc;
+    }
+    if (plan->n_pairs > 0) {
+        hipLaunchKernelGGL(ur_preprocess_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
+                           plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
+                           settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
+                           ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk));
+        if ((rc = check_launch("ur_preprocess_kernel"))) return rc;
+    }
+    return build_instances(plan, L, stream);
+}
+
+int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray_origins, const float* ray_dirs,
+                    const float* ray_offsets, const us3d_ur_settings* settings, float* image, float* aux,
+                    void* stream_) {
+    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                    plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_ur_settings(settings))) return rc;
+    if (!pixels || !ray_origins || !ray_dirs || !image || !aux) return fail(US3D_ERR_ARGUMENT, "ur_forward: NULL array");
+    hipLaunchKernelGGL(ur_forward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
+                       ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
+                       plan->frames, plan->frame_tile_base, plan->n_frames, pixels, ray_origins, ray_dirs, ray_offsets,
+                       flags_of(settings), ws_at<us3d_stats>(plan, L.stats), image, reinterpret_cast<float4*>(aux));
+    return check_launch("ur_forward_kernel");
+}
+
+int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ray_origins, const float* ray_dirs,
+                     const float* ray_offsets, const us3d_ur_settings* settings, const float* aux, const float* d_image,
+                     float* d_means, float* d_scales, float* d_quats, float* d_attenuations, float* d_sh,
+                     void* stream_) {
+    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
+                                    plan ? plan->k_capacity : 0);
+    int rc = check_plan(plan, L.total);
+    if (rc) return rc;
+    if ((rc = check_ur_settings(settings))) return rc;
+    if (!pixels || !ray_origins || !ray_dirs || !aux || !d_image || !d_means || !d_scales || !d_quats ||
+        !d_attenuations || !d_sh)
+        return fail(US3D_ERR_ARGUMENT, "ur_backward: NULL array");
+    hipStream_t stream = as_stream(stream_);
+    hipLaunchKernelGGL(ur_backward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
+                       ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
+                       ws_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
+                       ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
+                       d_image, ws_at<us3d_stats>(plan, L.stats), ws_at<float4>(plan, L.partial));
+    if ((rc = check_launch("ur_backward_kernel"))) return rc;
+    if (plan->n_gauss > 0) {
+        hipLaunchKernelGGL(ur_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
+                           plan->sweeps, plan->n_sweeps, ws_at<int32_t>(plan, L.pair_off), ws_at<float4>(plan, L.partial),
+                           ws_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_attenuations, d_sh);
+        if ((rc = check_launch("ur_reduce_kernel"))) return rc;
+    }
+    return US3D_OK;
+}
+
+}  // extern "C"
