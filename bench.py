@@ -1,0 +1,260 @@
+#!/usr/bin/env python
+"""Forward+backward frames/s of the differentiable slice splat on one MI355X.
+
+Workload: 200k anisotropic Gaussians in a 120 mm cube; 256 frames of 512x512 at 0.2 mm along a 150 mm freehand
+sweep; loss = L1 + 0.2 (1 - global SSIM) against the same sweep rendered from a second seeded scene; Adam step
+(lr 1e-3) over means, raw precision factors and opacity/intensity logits. One step = all 256 frames forward +
+backward + Adam. "batched" renders the whole sweep as one batch (one map / forward / adjoint launch sequence);
+"plain" renders it frame by frame through the same API (one launch sequence per frame). `value` is the batched
+frames/s; the plain rate, the roofline of the dominant kernel and the CPU baseline are reported beside it.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames 256] [--gaussians 200000] [--tile-size 16]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path[:0] = [str(ROOT / "ultrasound-3d-reconstruction_amd"), str(ROOT)]
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+METRIC = "fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--frames", type=int, default=256)
+    ap.add_argument("--gaussians", type=int, default=200_000)
+    ap.add_argument("--tile-size", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-rows", type=int, default=16)
+    ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
+    return ap.parse_args()
+
+
+def ssim_global(x, y):
+    """simple_ssim of pipeline_god_file.py:628-637, per frame over (F, H, W)."""
+    c1, c2 = 0.01 ** 2, 0.03 ** 2
+    mx, my = x.mean(dim=(1, 2)), y.mean(dim=(1, 2))
+    xc, yc = x - mx[:, None, None], y - my[:, None, None]
+    vx, vy, vxy = (xc * xc).mean(dim=(1, 2)), (yc * yc).mean(dim=(1, 2)), (xc * yc).mean(dim=(1, 2))
+    return ((2 * mx * my + c1) * (2 * vxy + c2)) / ((mx ** 2 + my ** 2 + c1) * (vx + vy + c2))
+
+
+def frame_losses(pred, gt):
+    """Per-frame loss of image_loss (pipeline_god_file.py:606-617) with w_L1 = 1, w_SSIM = 0.2."""
+    l1 = (pred - gt).abs().mean(dim=(1, 2))
+    return l1 + 0.2 * (1.0 - ssim_global(pred, gt))
+
+
+def cpu_baseline(n_rows: int):
+    """The oracle's restatement of the reference PyTorch fallback renderer (pipeline_god_file.py:556-603: every
+    pixel against every Gaussian, chunked, autograd backward) on a bounded sample of the benchmark workload:
+    n_rows full rows (512 pixels each) of the middle frame, all 200k Gaussians, L1 against the target scene's
+    render of the same pixels, all host threads. frames/s = sampled pixels / 512^2 / time."""
+    import torch
+
+    from oracle import fallback as ofb
+    from us3d.scenes import config2, linear_pixels
+
+    torch.set_num_threads(os.cpu_count() or 1)
+    scene, target, frames = config2()
+    fr = frames[len(frames) // 2]
+    pix = linear_pixels(fr).reshape(fr.height, fr.width, 3)
+    r0 = fr.height // 2 - n_rows // 2
+    pix = pix[r0:r0 + n_rows].reshape(-1, 3).contiguous()
+    ts = [t.clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    tgt = [t.clone() for t in (target.means, target.tril, target.opacity, target.intensity)]
+    with torch.no_grad():
+        gt, _ = ofb.render(*tgt, pix, n_rows, fr.width)
+    t0 = time.perf_counter()
+    img, _ = ofb.render(*ts, pix, n_rows, fr.width)
+    (img - gt).abs().mean().backward()
+    dt = time.perf_counter() - t0
+    n_pix = pix.shape[0]
+    pairs = n_pix * scene.means.shape[0]
+    cpu = "unknown"
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        cpu = m.group(1).strip() if m else platform.processor()
+    except OSError:
+        pass
+    return {"value": n_pix / (fr.height * fr.width) / dt, "unit": "frames/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"{n_rows} rows x {fr.width} pixels of frame {len(frames) // 2} of the benchmark sweep, all "
+                      f"{scene.means.shape[0]} Gaussians, forward + autograd backward of the restated PyTorch "
+                      f"fallback renderer, {dt:.1f} s ({pairs / dt / 1e6:.0f} M pair evaluations/s); frames/s = "
+                      f"sampled fraction of a 512x512 frame / time; host CPU: {cpu}"}
+
+
+def read_traffic(kernel: str, workload_tag: str):
+    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one exists for this kernel/workload."""
+    p = ROOT / "profiles" / f"{kernel}_{workload_tag}_pmc.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
+                          "n_gpus_requested": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from us3d import _lib
+    from us3d.batch import CapacityHint, read_stats
+    from us3d.scenes import config2
+    from us3d.train_step import SliceSplatTrainer
+    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
+
+    dev = torch.device("cuda:0")
+    F, G, ts = args.frames, args.gaussians, args.tile_size
+    scene, target, frames = config2(n_frames=F, n_gauss=G)
+    H, W = frames[0].height, frames[0].width
+    cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=0.2, axial_spacing_mm_per_pix=0.2,
+                                 axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=ts)
+    hint = CapacityHint()
+    with torch.no_grad():
+        tgt = [t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)]
+        gt = render_frames(*tgt, None, frames, cfg, hint=CapacityHint())[0].view(F, H, W).clone()
+
+    def logit(p):
+        return torch.log(p / (1 - p))
+
+    means = torch.nn.Parameter(scene.means.to(dev))
+    tril = torch.nn.Parameter(scene.tril.to(dev))
+    raw_o = torch.nn.Parameter(logit(scene.opacity.clamp(1e-6, 1 - 1e-6)).to(dev))
+    raw_c = torch.nn.Parameter(logit(scene.intensity.clamp(1e-6, 1 - 1e-6)).to(dev))
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    # batched: the whole sweep per step, one captured hipGraph (map, forward, loss, adjoint, Adam)
+    trainer = SliceSplatTrainer(means.detach(), tril.detach(), raw_o.detach(), raw_c.detach(), frames, gt, cfg,
+                                use_graph=not args.eager, check_every=10 ** 9)
+    dt_b = timed(trainer.step, args.steps, args.warmup)
+    if trainer.overflowed():
+        raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
+    fps_b = F * args.steps / dt_b
+    print(f"[bench] batched ({'eager' if args.eager else 'graph'}): {dt_b / args.steps * 1e3:.1f} ms/step, "
+          f"{fps_b:.1f} frames/s, loss {float(trainer.losses().mean()):.5f}", file=sys.stderr, flush=True)
+
+    # per-kernel durations of the same launch sequence (eager, HIP event pair around every launch)
+    timing_trainer = SliceSplatTrainer(means.detach(), tril.detach(), raw_o.detach(), raw_c.detach(), frames, gt, cfg,
+                                       use_graph=False, check_every=10 ** 9)
+    timing_trainer.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    for _ in range(args.timing_steps):
+        timing_trainer.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    stages = _lib.stage_timing_read()
+    st = read_stats(timing_trainer.workspace)
+    off = timing_trainer.batch.layout_offsets(timing_trainer.k_capacity)
+    lay = timing_trainer.batch.layout
+    pair_count = timing_trainer.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32)
+    g_vis = int((pair_count > 0).sum().item())
+    K = int(st.n_instances)
+    P = F * H * W
+    del timing_trainer
+
+    fps_p = None
+    if not args.no_plain:
+        opt = torch.optim.Adam([means, tril, raw_o, raw_c], lr=1e-3)
+
+        def step_plain():
+            opt.zero_grad(set_to_none=True)
+            for i, fr in enumerate(frames):
+                img = render_frames(means, tril, torch.sigmoid(raw_o), torch.sigmoid(raw_c), None, [fr], cfg,
+                                    hint=hint)[0]
+                (frame_losses(img.view(1, H, W), gt[i:i + 1]).sum() / F).backward()
+            opt.step()
+
+        dt_p = timed(step_plain, args.plain_steps, 1)
+        fps_p = F * args.plain_steps / dt_p
+        print(f"[bench] plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s", file=sys.stderr,
+              flush=True)
+
+    # roofline of the dominant kernel (largest total time in the timed batched region)
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    launch_bytes = {
+        # algorithmic HBM bytes per launch over the batch (formulas: DESIGN.md "Roofline accounting")
+        "ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis,
+        "ug_forward_kernel": 36 * K + 16 * P,
+        "sort_segments_kernel": 24 * K,
+        "ug_emit_kernel": 12 * K + 28 * g_vis,
+        "ug_count_kernel": 8 * K,
+        "ug_preprocess_kernel": 48 * G + 28 * F * G,
+        "ug_reduce_kernel": 48 * K + 48 * G,
+    }
+    alg_bytes = launch_bytes.get(dom)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if alg_bytes else None
+    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
+                "algorithmic_bytes_per_launch": alg_bytes, "stage_ms_per_launch": stage_ms}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_sample_rows)
+
+    line = {
+        "metric": METRIC,
+        "value": fps_b,
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt_b / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 120 mm cube, {F} freehand frames "
+                               f"{H}x{W} at 0.2 mm, L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} "
+                               f"step, tile {ts}",
+                   "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis},
+        "plain_frames_per_s": fps_p,
+        "batched_over_plain": (fps_b / fps_p) if fps_p else None,
+        "iters_per_s": args.steps / dt_b,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -139,8 +139,8 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
  * forward.cu:17-76). pixels [n_pixels,3] world mm, concatenated per frame (frame.pixel_base); NULL means the
  * linear-probe grid origin + (x - col_offset)*delta_lat*a_lat + (near + y*delta_ax)*a_ax computed in-kernel
  * with the FP32 operations of pipeline_god_file.py:299-306 (bit-identical to that tensor).
- * Outputs [n_pixels]: image, alpha, depth; aux [n_pixels,4] = (sum alpha_hat, sum alpha_hat*c,
- * sum alpha_hat*elev, 0) kept for the adjoint. Requires a prior us3d_ug_map on the same workspace. */
+ * Outputs [n_pixels]: image, alpha, depth; aux [n_pixels,4] = (sum alpha_hat, unclamped image,
+ * sum alpha_hat*elev, 0) kept for the adjoint (sums accumulated in FP64, stored as FP32). Requires a prior us3d_ug_map on the same workspace. */
 int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
                     float* image, float* alpha, float* depth, float* aux, void* stream);
 
@@ -190,7 +190,8 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
 /* B-mode forward: replaces ShaderForwardCUDA of ops/ultragray_renderer (src/shader/wrapper_shader.cu:7-58,
  * forward.cu:22-119). pixels: per frame (W,H,3) world mm (pixel (x,y) at ((x*H)+y)*3), concatenated;
  * ray_origins/ray_dirs [n_rays,3]; ray_offsets [n_rays] elevational OPS offsets (mm).
- * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, echo num, echo den, 0). */
+ * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, E = num/(den+1e-6), den, 0),
+ * accumulated in FP64 and stored as FP32. */
 int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray_origins,
                     const float* ray_dirs, const float* ray_offsets, const us3d_ur_settings* settings,
                     float* image, float* aux, void* stream);
@@ -212,8 +213,35 @@ int64_t us3d_scan_tmp_bytes(int64_t n);
 int us3d_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev, void* tmp,
                   void* stream);
 
+/* Fused loss of pipeline_god_file.py:606-637 over a batch of frames: per frame
+ * loss_f = w_l1 * mean|pred - gt| + w_ssim * (1 - global SSIM(pred, gt)) (single-window SSIM, :628-637); writes
+ * loss[n_frames] and d_pred = d(scale * sum_f loss_f)/d pred. Frame statistics are reduced in FP64 with a fixed
+ * order (bitwise reproducible). frame_pixel_base [n_frames+1]; tmp holds us3d_image_loss_tmp_bytes bytes. */
+int64_t us3d_image_loss_tmp_bytes(int32_t n_frames);
+int us3d_image_loss(const float* pred, const float* gt, const int32_t* frame_pixel_base, int32_t n_frames,
+                    float w_l1, float w_ssim, float scale, float* loss, float* d_pred, double* tmp, void* stream);
+
+/* Adam update with torch.optim.Adam semantics (pipeline_god_file.py:740,764; no weight decay) over n
+ * parameters, bias corrections from the device step counter (t = *step_dev + 1). skip_dev (may be NULL):
+ * when *skip_dev != 0 the update is skipped -- pass &stats.overflow so a step whose mapping overflowed never
+ * touches the parameters. us3d_adam_advance increments *step_dev (unless skipped) once all parameter groups
+ * are updated; both are kernels, so a captured graph advances the counter on every replay. */
+int us3d_adam_step(float* param, const float* grad, float* exp_avg, float* exp_avg_sq, int64_t n,
+                   int32_t* step_dev, const int32_t* skip_dev, float lr, float beta1, float beta2, float eps,
+                   void* stream);
+int us3d_adam_advance(int32_t* step_dev, const int32_t* skip_dev, void* stream);
+
 /* Fill n int32 values with `value` (a kernel, usable inside captured graphs instead of hipMemset). */
 int us3d_fill_i32(int32_t* dst, int64_t n, int32_t value, void* stream);
+
+/* Opt-in per-kernel timing (no counterpart in the reference's CUDA path; the analogue of its
+ * ULTRAGRAY_PROFILE stage timers, ops/ultragray_renderer/ultragray_renderer/autograd.py:13-70): while
+ * enabled, every kernel launch of the library is bracketed by a hipEvent pair on its stream.
+ * us3d_stage_timing_read synchronises on the recorded events, writes one "kernel launches total_ms" line per
+ * kernel into buf (NUL-terminated) and clears the records; with buf == NULL it only returns a buffer size
+ * that is large enough. */
+int us3d_stage_timing(int enable);
+int us3d_stage_timing_read(char* buf, int64_t bytes);
 
 const char* us3d_last_error(void);
 int us3d_version(void);

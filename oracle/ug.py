@@ -43,6 +43,7 @@ def lib() -> ctypes.CDLL:
         L.ug_oracle_forward_f64.argtypes = fwd
         L.ug_oracle_backward_f64.argtypes = [_I, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _F, _I,
                                              _P, _P, _P, _P, _P, _P, _P]
+        L.ug_oracle_backward_f32.argtypes = L.ug_oracle_backward_f64.argtypes
         _lib = L
     return _lib
 
@@ -160,16 +161,19 @@ def forward(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semanti
 
 
 def backward_f64(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semantics, d_image, d_alpha=None,
-                 d_depth=None, gate=True):
+                 d_depth=None, gate=True, fp32=False):
+    """FP64 restatement of the reference backward (FP32 with fp32=True: the reference's precision, naive
+    per-pair accumulation in pixel order)."""
     L = lib()
     counts, offsets, indices = lists
     P = counts.shape[0]
     G = int(np.asarray(means).shape[0])
-    dm, dt, do, di = np.zeros((G, 3)), np.zeros((G, 6)), np.zeros(G), np.zeros(G)
+    f8 = np.float32 if fp32 else np.float64
+    dm, dt, do, di = np.zeros((G, 3), f8), np.zeros((G, 6), f8), np.zeros(G, f8), np.zeros(G, f8)
     opt = lambda a: None if a is None else _c(a, np.float32)  # noqa: E731
     gA, gD = opt(d_alpha), opt(d_depth)
-    f8 = np.float64
-    L.ug_oracle_backward_f64(G, _p(_c(means, f8)), _p(_c(tril, f8)), _p(_c(opac, f8)),
+    fn = L.ug_oracle_backward_f32 if fp32 else L.ug_oracle_backward_f64
+    fn(G, _p(_c(means, f8)), _p(_c(tril, f8)), _p(_c(opac, f8)),
                              _p(_c(inten, f8)), _p(_c(pixels, f8)), P, _p(_c(fr.origin, np.float32)),
                              _p(_c(fr.axis_ele, np.float32)), _p(_c(counts, np.int32)), _p(_c(offsets, np.int32)),
                              _p(_c(indices, np.int32)), sem.chi2, sem.beta, sem.alpha_min, sem.bg, sem.bg_intensity,

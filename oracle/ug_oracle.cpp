@@ -85,6 +85,20 @@ Frame make_frame(const float* origin, const float* a_lat, const float* a_ax, con
 template <typename T>
 T exp_(T x) { return std::exp(x); }
 
+// The q <= chi2 cut evaluated in the canonical FP32 arithmetic on the FP32 values of the inputs. The
+// pair set is part of the renderer's semantics: the FP64 evaluation uses the same pairs and only measures the
+// arithmetic error of the values (deciding the cut in double would flip pairs that lie within rounding of
+// the chi2 boundary).
+template <typename In>
+bool keep_pair(const In* raw, float beta, In px, In py, In pz, In mx, In my, In mz, float chi2) {
+    float l[6];
+    const float r32[6] = {(float)raw[0], (float)raw[1], (float)raw[2], (float)raw[3], (float)raw[4], (float)raw[5]};
+    tril_factor<float>(r32, beta, l);
+    float y[3];
+    const float q = quad_form<float>(l, (float)px - (float)mx, (float)py - (float)my, (float)pz - (float)mz, y);
+    return !(q > chi2);
+}
+
 }  // namespace
 
 extern "C" {
@@ -247,7 +261,7 @@ void forward_impl(int G, const T* means, const T* tril, const T* opac, const T* 
             const T mx = means[3 * g], my = means[3 * g + 1], mz = means[3 * g + 2];
             T y[3];
             const T q = quad_form<T>(l, px - mx, py - my, pz - mz, y);
-            if (q > T(chi2)) continue;
+            if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, mx, my, mz, chi2)) continue;
             const T ah = T(opac[g]) * exp_(T(-0.5) * q);
             if (!(ah > T(alpha_min))) continue;
             const T e = dot3<T>(mx - T(origin[0]), my - T(origin[1]), mz - T(origin[2]), T(a_ele[0]), T(a_ele[1]), T(a_ele[2]));
@@ -303,7 +317,9 @@ void backward_impl(int G, const T* means, const T* tril, const T* opac, const T*
             dz = pz - T(means[3 * g + 2]);
             T y[3];
             const T q = quad_form<T>(l, dx, dy, dz, y);
-            if (q > T(chi2)) return false;
+            if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, T(means[3 * g]), T(means[3 * g + 1]),
+                              T(means[3 * g + 2]), chi2))
+                return false;
             w = exp_(T(-0.5) * q);
             return true;
         };
@@ -385,6 +401,16 @@ int ug_oracle_forward_f64(int G, const double* means, const double* tril, const 
                           float bg, float bgI, int excl, double* image, double* alpha, double* depth, double* sums) {
     forward_impl<double>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
                          alpha_min, bg, bgI, excl, image, alpha, depth, sums);
+    return 0;
+}
+
+int ug_oracle_backward_f32(int G, const float* means, const float* tril, const float* opac, const float* inten,
+                           const float* pixels, int64_t P, const float* origin, const float* a_ele,
+                           const int32_t* counts, const int32_t* offsets, const int32_t* indices, float chi2,
+                           float beta, float alpha_min, float bg, float bgI, int gate, const float* gI, const float* gA,
+                           const float* gD, float* d_means, float* d_tril, float* d_opac, float* d_inten) {
+    backward_impl<float>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
+                         alpha_min, bg, bgI, gate, gI, gA, gD, d_means, d_tril, d_opac, d_inten);
     return 0;
 }
 

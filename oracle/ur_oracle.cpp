@@ -176,6 +176,28 @@ struct Flags {
     float local_max;
 };
 
+// Membership of a hit in the transmission / echo sums of a sample, decided in the canonical FP32 arithmetic on
+// the FP32 values of the inputs (forward.cu:76-96). The set is part of the semantics: the FP64 evaluation uses
+// the same decisions and only measures the arithmetic error of the values.
+template <typename In>
+void pair_terms32(const In* mu, const In* s, const float ro[3], const float rd[3], float tpx, const Flags& F,
+                  bool& need_t, bool& need_e) {
+    const float md[3] = {(float)mu[0] - ro[0], (float)mu[1] - ro[1], (float)mu[2] - ro[2]};
+    const float tg = dot3(md, rd);
+    const float ms = std::max((float)s[0], std::max((float)s[1], (float)s[2]));
+    const float rdep = ms * F.k_sigma;
+    need_t = tg < tpx;
+    if (F.local_only) need_t = need_t && ((tpx - tg) <= rdep + F.Rw);
+    need_e = std::fabs(tg - tpx) <= rdep;
+}
+
+float sample_depth32(const Geometry& geo, int x, int y, float ro[3], float rd[3]) {
+    ray_setup<float>(geo, x, ro, rd);
+    const float* pp = geo.pixels + 3 * ((int64_t)x * geo.H + y);
+    const float pd[3] = {pp[0] - ro[0], pp[1] - ro[1], pp[2] - ro[2]};
+    return std::max(0.0f, dot3(pd, rd));
+}
+
 // Per-pixel forward (forward.cu:45-118). Returns B; T, num, den through pointers.
 template <typename T>
 T pixel_forward(const Params<T>& P, const Geometry& geo, const Lists& Ls, const Flags& F, int x, int y, T* pT, T* pnum,
@@ -193,24 +215,21 @@ T pixel_forward(const Params<T>& P, const Geometry& geo, const Lists& Ls, const 
     const T pd[3] = {p[0] - ro[0], p[1] - ro[1], p[2] - ro[2]};
     const T tpx = std::max(T(0), dot3(pd, rd));
     T trans = 1, num = 0, den = 0, S = 0;
+    float ro32[3], rd32[3];
+    const float tpx32 = sample_depth32(geo, x, y, ro32, rd32);
     int begin = 0;
-    T upper = T(1e30f);
+    float upper = 1e30f;
     if (F.local_only) {
-        begin = lower_bound_depth(Ls.sorted_depths, off, count, (float)(tpx - T(F.local_max)));
-        upper = tpx + T(F.local_max);
+        begin = lower_bound_depth(Ls.sorted_depths, off, count, tpx32 - F.local_max);
+        upper = tpx32 + F.local_max;
     }
     for (int k = begin; k < count; ++k) {
-        if (F.local_only && T(Ls.sorted_depths[off + k]) > upper) break;
+        if (F.local_only && Ls.sorted_depths[off + k] > upper) break;
         const int g = Ls.indices[off + k];
         const T* mu = P.means + 3 * g;
         const T* s = P.scales + 3 * g;
-        const T md[3] = {T(mu[0]) - ro[0], T(mu[1]) - ro[1], T(mu[2]) - ro[2]};
-        const T tg = dot3(md, rd);
-        const T ms = std::max(T(s[0]), std::max(T(s[1]), T(s[2])));
-        const T rdep = ms * T(F.k_sigma);
-        bool need_t = tg < tpx;
-        if (F.local_only) need_t = need_t && ((tpx - tg) <= rdep + T(F.Rw));
-        const bool need_e = std::fabs(tg - tpx) <= rdep;
+        bool need_t, need_e;
+        pair_terms32(mu, s, ro32, rd32, tpx32, F, need_t, need_e);
         if (!need_t && !need_e) continue;
         T R[9];
         quat_to_rot<T>(P.quats + 4 * g, R);
@@ -414,24 +433,21 @@ int ur_oracle_backward_f64(UR_ARGS_T(double), const float* dL_dB, float grad_thr
             const T p[3] = {pp[0], pp[1], pp[2]};
             const T pd[3] = {p[0] - ro[0], p[1] - ro[1], p[2] - ro[2]};
             const T tpx = std::max(T(0), dot3(pd, rd));
+            float ro32[3], rd32[3];
+            const float tpx32 = sample_depth32(geo, x, y, ro32, rd32);
             int begin = 0;
-            T upper = 1e30;
+            float upper = 1e30f;
             if (local_only) {
-                begin = lower_bound_depth(sorted_depths, off, count, (float)(tpx - T(local_max)));
-                upper = tpx + T(local_max);
+                begin = lower_bound_depth(sorted_depths, off, count, tpx32 - local_max);
+                upper = tpx32 + local_max;
             }
             for (int k = begin; k < count; ++k) {
-                if (local_only && T(sorted_depths[off + k]) > upper) break;
+                if (local_only && sorted_depths[off + k] > upper) break;
                 const int g = indices[off + k];
                 const double* mu = means + 3 * g;
                 const double* s = scales + 3 * g;
-                const T md[3] = {T(mu[0]) - ro[0], T(mu[1]) - ro[1], T(mu[2]) - ro[2]};
-                const T tg = dot3(md, rd);
-                const T ms = std::max(T(s[0]), std::max(T(s[1]), T(s[2])));
-                const T rdep = ms * T(k_sigma);
-                bool need_t = tg < tpx;
-                if (local_only) need_t = need_t && ((tpx - tg) <= rdep + T(Rw));
-                const bool need_e = std::fabs(tg - tpx) <= rdep;
+                bool need_t, need_e;
+                pair_terms32(mu, s, ro32, rd32, tpx32, F, need_t, need_e);
                 if (!need_t && !need_e) continue;
                 T R[9];
                 quat_to_rot<T>(quats + 4 * g, R);

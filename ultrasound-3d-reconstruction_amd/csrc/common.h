@@ -17,6 +17,22 @@ void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
 int check_launch(const char* what);
 
+// Opt-in per-kernel timing (us3d_stage_timing): when enabled, a hipEvent pair brackets every kernel launch of
+// the library on its stream; us3d_stage_timing_read sums the elapsed times per kernel name.
+bool timing_enabled();
+void timing_mark(const char* name, hipStream_t stream, bool begin);
+
+struct StageTimer {
+    const char* name;
+    hipStream_t stream;
+    StageTimer(const char* n, hipStream_t s) : name(n), stream(s) {
+        if (timing_enabled()) timing_mark(name, stream, true);
+    }
+    ~StageTimer() {
+        if (timing_enabled()) timing_mark(name, stream, false);
+    }
+};
+
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
 inline int64_t align_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }

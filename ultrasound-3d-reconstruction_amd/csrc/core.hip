@@ -2,7 +2,11 @@
 // fill kernels.
 #include <algorithm>
 #include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "common.h"
 
@@ -23,6 +27,42 @@ int check_launch(const char* what) {
         return fail(US3D_ERR_HIP, std::string(what) + ": " + hipGetErrorString(err));
     }
     return US3D_OK;
+}
+
+// ---------------------------------------------------------------- per-kernel timing
+
+namespace {
+struct Timing {
+    std::mutex mu;
+    bool enabled = false;
+    std::map<std::string, std::vector<std::pair<hipEvent_t, hipEvent_t>>> events;
+    std::map<std::string, hipEvent_t> open;
+};
+Timing& timing() {
+    static Timing t;
+    return t;
+}
+}  // namespace
+
+bool timing_enabled() { return timing().enabled; }
+
+void timing_mark(const char* name, hipStream_t stream, bool begin) {
+    Timing& T = timing();
+    std::lock_guard<std::mutex> lock(T.mu);
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return;
+    hipEventRecord(ev, stream);
+    if (begin) {
+        T.open[name] = ev;
+    } else {
+        auto it = T.open.find(name);
+        if (it == T.open.end()) {
+            hipEventDestroy(ev);
+            return;
+        }
+        T.events[name].push_back({it->second, ev});
+        T.open.erase(it);
+    }
 }
 
 // ---------------------------------------------------------------- fill
@@ -248,6 +288,7 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
                   int n_segments, us3d_stats* stats, int64_t k_capacity, hipStream_t stream) {
     (void)k_capacity;
     if (n_segments <= 0) return US3D_OK;
+    StageTimer timer("sort_segments_kernel", stream);
     hipLaunchKernelGGL(sort_segments_kernel, dim3(n_segments), dim3(kSortThreads), 0, stream, keys, vals, keys_tmp,
                        vals_tmp, seg_off, stats);
     return check_launch("sort_segments_kernel");
@@ -260,6 +301,36 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
 extern "C" {
 
 const char* us3d_last_error(void) { return us3d::g_last_error.c_str(); }
+
+int us3d_stage_timing(int enable) {
+    us3d::timing().enabled = enable != 0;
+    return US3D_OK;
+}
+
+int us3d_stage_timing_read(char* buf, int64_t bytes) {
+    us3d::Timing& T = us3d::timing();
+    std::lock_guard<std::mutex> lock(T.mu);
+    if (!buf || bytes <= 0) return (int)(T.events.size() * 160 + 1);  // size query: consumes nothing
+    std::string out;
+    for (auto& kv : T.events) {
+        double total = 0.0;
+        for (auto& pr : kv.second) {
+            float ms = 0.f;
+            hipEventSynchronize(pr.second);
+            if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) total += ms;
+            hipEventDestroy(pr.first);
+            hipEventDestroy(pr.second);
+        }
+        char line[256];
+        snprintf(line, sizeof(line), "%s %zu %.6f\n", kv.first.c_str(), kv.second.size(), total);
+        out += line;
+    }
+    T.events.clear();
+    const size_t n = std::min<size_t>(out.size(), (size_t)bytes - 1);
+    memcpy(buf, out.data(), n);
+    buf[n] = 0;
+    return US3D_OK;
+}
 
 int us3d_version(void) { return 1; }
 

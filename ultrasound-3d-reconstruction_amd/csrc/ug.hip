@@ -267,7 +267,9 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // with 16 B loads, read back as wavefront broadcasts). Follows kernel_pixel_ultragauss_forward,
 // ops/ultragauss_renderer/src/shader/forward.cu:17-76: alpha_hat = o * exp(-q/2) for q <= chi2 and
 // alpha_hat > alpha_min; A = sum alpha_hat + bg; image = clamp((sum alpha_hat c + bg I_bg) / max(A,1e-8));
-// depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8).
+// depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8). The per-pixel sums are accumulated in FP64: the
+// adjoint's coefficient dL/dalpha_hat = dN (c - image) depends on the image to far better than FP32-sum
+// accuracy wherever c is close to it.
 
 template <int TS>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
@@ -296,7 +298,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
     float px[PPT], py[PPT], pz[PPT];
     bool valid[PPT];
     int pidx[PPT];
-    float S_[PPT], N_[PPT], D_[PPT];
+    double S_[PPT], N_[PPT], D_[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const int sb = wave + k * S::kWaves;
@@ -311,7 +313,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
         const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
         const SubBounds b = sub_bounds(valid[k], lat, ax);
         if (lane == 0) s_sb[sb] = b;
-        S_[k] = N_[k] = D_[k] = 0.f;
+        S_[k] = N_[k] = D_[k] = 0.0;
     }
 
     for (int base = s0; base < s1; base += NT) {
@@ -345,9 +347,10 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
                 if (q <= chi2) {
                     const float ah = a.w * __expf(-0.5f * q);
                     if (ah > alpha_min) {
-                        S_[k] += ah;
-                        N_[k] = __builtin_fmaf(ah, b.x, N_[k]);
-                        D_[k] = __builtin_fmaf(ah, b.y, D_[k]);
+                        const double ahd = (double)ah;
+                        S_[k] += ahd;
+                        N_[k] = __builtin_fma(ahd, (double)b.x, N_[k]);
+                        D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
                     }
                 }
             }
@@ -357,13 +360,13 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
-        const float A = S_[k] + bg_alpha;
-        const float num = __builtin_fmaf(bg_alpha, bg_intensity, N_[k]);
-        const float img = num / fmaxf(A, 1e-8f);
-        out_image[pidx[k]] = fmaxf(0.0f, fminf(1.0f, img));
-        out_alpha[pidx[k]] = alpha_excludes_bg ? S_[k] : A;
-        out_depth[pidx[k]] = D_[k] / fmaxf(S_[k], 1e-8f);
-        out_aux[pidx[k]] = make_float4(S_[k], N_[k], D_[k], 0.f);
+        const double A = S_[k] + (double)bg_alpha;
+        const double num = __builtin_fma((double)bg_alpha, (double)bg_intensity, N_[k]);
+        const double img = num / fmax(A, 1e-8);
+        out_image[pidx[k]] = (float)fmax(0.0, fmin(1.0, img));
+        out_alpha[pidx[k]] = (float)(alpha_excludes_bg ? S_[k] : A);
+        out_depth[pidx[k]] = (float)(D_[k] / fmax(S_[k], 1e-8));
+        out_aux[pidx[k]] = make_float4((float)S_[k], (float)img, (float)D_[k], 0.f);
     }
 }
 
@@ -373,8 +376,10 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
 // the tile's pixels (LDS broadcasts), accumulating its 11 gradient terms in registers: no atomics, no
 // cross-lane reductions. Formulas follow kernel_pixel_ultragauss_backward,
 // ops/ultragauss_renderer/src/shader/backward.cu:17-131, with q = ||L^T d||^2:
-//   dN = gI/A', dX = gA - gI*num/A'^2 - gD*Dn/S'^2, dD = gD/S'  (A' = max(A,1e-8), S' = max(S,1e-8))
-//   dah = dN*c + dX + dD*elev; dc += dN*ah; do += dah*w; m = 2 dq = -dah*ah
+//   dN = gI/A', v = num/A' (unclamped image), dX = gA - gD*Dn/S'^2, dD = gD/S'
+//   (A' = max(A,1e-8), S' = max(S,1e-8)); dah = dN*(c - v) + dX + dD*elev (= dN*c + dA + ..., written so
+//   that c - v is formed from two FP32 numbers instead of cancelling dN*c against dN*v);
+//   dc += dN*ah; do += dah*w; m = 2 dq = -dah*ah
 //   dmu = -L * sum(m y) + sum(dD*ah) * a_ele;  dl_ij = sum(m * y_j * d_i)
 // Pairs are gated exactly as in the forward (q <= chi2 and ah > alpha_min), so the adjoint is the exact
 // derivative of the forward. Partials go to the instance's emit index.
@@ -392,7 +397,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     constexpr int PPT = S::kSubPerWave;
     constexpr int NPIX = TS * TS;
     __shared__ float4 s_p[NPIX];   // px, py, pz, dN
-    __shared__ float2 s_c[NPIX];   // dX, dD
+    __shared__ float4 s_c[NPIX];   // unclamped image v, dX, dD
     __shared__ SubBounds s_sb[S::kSub];
     if (stats->overflow) return;
 
@@ -412,22 +417,21 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
         int ix, iy;
         sub_pixel<TS>(tx, ty, sb, lane, ix, iy);
         const bool valid = ix < fr.width && iy < fr.height;
-        float x = 0.f, y = 0.f, z = 0.f, dN = 0.f, dX = 0.f, dD = 0.f;
+        float x = 0.f, y = 0.f, z = 0.f, dN = 0.f, dX = 0.f, dD = 0.f, vimg = 0.f;
         if (valid) {
             const int64_t p = (int64_t)fr.pixel_base + iy * fr.width + ix;
             pixel_position(fr, pixels, ix, iy, x, y, z);
-            const float4 sm = aux[p];
+            const float4 sm = aux[p];  // sum alpha_hat, unclamped image, sum alpha_hat * elev
             const float A = sm.x + bg_alpha;
             const float inv = 1.0f / fmaxf(A, 1e-8f);
-            const float num = __builtin_fmaf(bg_alpha, bg_intensity, sm.y);
             const float gi = d_image[p];
             const float ga = d_alpha ? d_alpha[p] : 0.f;
             const float gd = d_depth ? d_depth[p] : 0.f;
             const float invS = 1.0f / fmaxf(sm.x, 1e-8f);
             dN = gi * inv;
-            const float dA = ga - gi * num * inv * inv;
+            vimg = sm.y;
             dD = gd * invS;
-            dX = dA - gd * sm.z * invS * invS;
+            dX = ga - gd * sm.z * invS * invS;
         }
         const float dx = x - ox, dy = y - oy, dz = z - oz;
         const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
         const SubBounds b = sub_bounds(valid, lat, ax);
         if (lane == 0) s_sb[sb] = b;
         s_p[sb * 64 + lane] = make_float4(x, y, z, dN);
-        s_c[sb * 64 + lane] = make_float2(dX, dD);
+        s_c[sb * 64 + lane] = make_float4(vimg, dX, dD, 0.f);
     }
     __syncthreads();
 
@@ -473,11 +477,13 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                     const float w = __expf(-0.5f * q);
                     const float ah = o * w;
                     if (ah > alpha_min) {
-                        const float2 cc = s_c[j];
-                        const float da = __builtin_fmaf(cc.y, e, __builtin_fmaf(pp.w, c, cc.x));
+                        const float4 cc = s_c[j];
+                        // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
+                        // cancellation of dN c against dN v
+                        const float da = __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y));
                         ac = __builtin_fmaf(pp.w, ah, ac);
                         ao = __builtin_fmaf(da, w, ao);
-                        aed = __builtin_fmaf(cc.y, ah, aed);
+                        aed = __builtin_fmaf(cc.z, ah, aed);
                         const float m = -da * ah;
                         const float m0 = m * y0, m1 = m * y1, m2 = m * y2;
                         Y0 += m0; Y1 += m1; Y2 += m2;
@@ -641,9 +647,12 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
                        (int64_t)plan->n_tiles, 0);
     if ((rc = check_launch("fill_i32_kernel"))) return rc;
     if (np > 0) {
-        hipLaunchKernelGGL(ug_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
-                           plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, tile_count,
-                           at<int32_t>(plan, L.inst_rank), (const us3d_stats*)stats);
+        {
+            StageTimer timer_("ug_count_kernel", stream);
+            hipLaunchKernelGGL(ug_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
+                               plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, tile_count,
+                               at<int32_t>(plan, L.inst_rank), (const us3d_stats*)stats);
+        }
         if ((rc = check_launch("ug_count_kernel"))) return rc;
     }
     int64_t* tile_total = &stats->reserved[0];
@@ -651,10 +660,13 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, tile_off, (int64_t)plan->n_tiles,
                        tile_total, (const int32_t*)&stats->overflow);
     if (np > 0) {
-        hipLaunchKernelGGL(ug_emit_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
-                           plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, pair_dk, tile_off,
-                           at<int32_t>(plan, L.inst_rank), at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
-                           (const us3d_stats*)stats);
+        {
+            StageTimer timer_("ug_emit_kernel", stream);
+            hipLaunchKernelGGL(ug_emit_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
+                               plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, pair_dk, tile_off,
+                               at<int32_t>(plan, L.inst_rank), at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
+                               (const us3d_stats*)stats);
+        }
         if ((rc = check_launch("ug_emit_kernel"))) return rc;
     }
     return sort_segments(at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<uint64_t>(plan, L.seg_key_tmp),
@@ -704,9 +716,12 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     }
     const int64_t np = plan->n_pairs;
     if (np > 0) {
-        hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
-                           plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
-                           settings->sort_by_depth, pair_count, pair_rect, pair_dk);
+        {
+            StageTimer timer_("ug_preprocess_kernel", stream);
+            hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
+                               plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
+                               settings->sort_by_depth, pair_count, pair_rect, pair_dk);
+        }
         if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
     }
     return build_instances(plan, L, stream);
@@ -729,10 +744,13 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                        sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
                        settings->alpha_excludes_bg, at<us3d_stats>(plan, L.stats), image, alpha, depth,              \
                        reinterpret_cast<float4*>(aux))
-    switch (settings->tile_size) {
-        case 8: UG_FWD(8); break;
-        case 16: UG_FWD(16); break;
-        default: UG_FWD(32); break;
+    {
+        StageTimer timer_("ug_forward_kernel", stream);
+        switch (settings->tile_size) {
+            case 8: UG_FWD(8); break;
+            case 16: UG_FWD(16); break;
+            default: UG_FWD(32); break;
+        }
     }
 #undef UG_FWD
     return check_launch("ug_forward_kernel");
@@ -757,17 +775,23 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                        settings->chi2_threshold, sqrt_chi2, settings->alpha_min, settings->bg_alpha,                  \
                        settings->bg_intensity, reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,      \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial))
-    switch (settings->tile_size) {
-        case 8: UG_BWD(8); break;
-        case 16: UG_BWD(16); break;
-        default: UG_BWD(32); break;
+    {
+        StageTimer timer_("ug_backward_kernel", stream);
+        switch (settings->tile_size) {
+            case 8: UG_BWD(8); break;
+            case 16: UG_BWD(16); break;
+            default: UG_BWD(32); break;
+        }
     }
 #undef UG_BWD
     if ((rc = check_launch("ug_backward_kernel"))) return rc;
     if (plan->n_gauss > 0) {
-        hipLaunchKernelGGL(ug_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
-                           plan->sweeps, plan->n_sweeps, at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
-                           tril, at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
+        {
+            StageTimer timer_("ug_reduce_kernel", stream);
+            hipLaunchKernelGGL(ug_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
+                               plan->sweeps, plan->n_sweeps, at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
+                               tril, at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
+        }
         if ((rc = check_launch("ug_reduce_kernel"))) return rc;
     }
     return US3D_OK;
@@ -785,11 +809,14 @@ int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_u
     if (!plan->frame_pixel_base) return fail(US3D_ERR_ARGUMENT, "pixel_lists: frame_pixel_base is NULL");
     const int32_t* frame_pixel_base = plan->frame_pixel_base;
     hipStream_t stream = as_stream(stream_);
-    hipLaunchKernelGGL(ug_pixel_lists_kernel, dim3(blocks_for(plan->n_pixels, 256)), dim3(256), 0, stream,
-                       plan->n_pixels, plan->frames, plan->n_frames, frame_pixel_base, plan->sweeps,
-                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key), pixels,
-                       sqrtf(settings->chi2_threshold), settings->tile_size, pass != 0, counts, offsets, indices,
-                       at<us3d_stats>(plan, L.stats));
+    {
+        StageTimer timer_("ug_pixel_lists_kernel", stream);
+        hipLaunchKernelGGL(ug_pixel_lists_kernel, dim3(blocks_for(plan->n_pixels, 256)), dim3(256), 0, stream,
+                           plan->n_pixels, plan->frames, plan->n_frames, frame_pixel_base, plan->sweeps,
+                           at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key), pixels,
+                           sqrtf(settings->chi2_threshold), settings->tile_size, pass != 0, counts, offsets, indices,
+                           at<us3d_stats>(plan, L.stats));
+    }
     return check_launch("ug_pixel_lists_kernel");
 }
 

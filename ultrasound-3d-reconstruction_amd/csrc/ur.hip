@@ -249,6 +249,8 @@ __device__ __forceinline__ void unpack_rg(const RGPack& p, RayGauss& G) {
 // transmittance of the complete window, which every sample beyond the window reuses (the reference
 // recomputes all of it for every (sample, hit)). Per sample: T = prod over hits in front of
 // tau + (1 - tau) e^-psi, echo = (1 - e^-S) sum(I w) / (sum w + 1e-6), B = T * echo (forward.cu:22-119).
+// T and the echo sums are accumulated in FP64; aux keeps (T, E = num / (den + 1e-6), den) so the adjoint can
+// form dL/dw = dNum (I - E) + dS without cancelling dNum I against dNum E.
 
 constexpr int kRayThreads = 256;
 constexpr int kPPT = 4;
@@ -272,7 +274,8 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
     const int tid = threadIdx.x;
 
     for (int c0 = 0; c0 < H; c0 += kRayThreads * kPPT) {
-        float px[kPPT], py[kPPT], pz[kPPT], tpx[kPPT], T[kPPT], num[kPPT], den[kPPT];
+        float px[kPPT], py[kPPT], pz[kPPT], tpx[kPPT];
+        double T[kPPT], num[kPPT], den[kPPT];
 #pragma unroll
         for (int k = 0; k < kPPT; ++k) {
             const int y = c0 + tid + k * kRayThreads;
@@ -282,7 +285,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
                 px[k] = p[0]; py[k] = p[1]; pz[k] = p[2];
                 tpx[k] = fmaxf(0.0f, dot3(px[k] - ro[0], py[k] - ro[1], pz[k] - ro[2], rd[0], rd[1], rd[2]));
             }
-            T[k] = 1.f; num[k] = 0.f; den[k] = 0.f;
+            T[k] = 1.0; num[k] = 0.0; den[k] = 0.0;
         }
         for (int base = s0; base < s1; base += kRayThreads) {
             __syncthreads();
@@ -304,18 +307,18 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
                     ray_terms(G, tpx[k], F, trans, echo);
                     if (trans) {
                         if (F.Rw > 0.0f && tpx[k] >= G.b) {
-                            T[k] *= G.f_final;
+                            T[k] *= (double)G.f_final;
                         } else {
                             float ts, te, psi = 0.f;
                             if (ray_window(G, tpx[k], F.Rw, ts, te)) psi = gl3(G.og, G.dg, ts, te);
-                            T[k] *= __builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
+                            T[k] *= (double)__builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
                         }
                     }
                     if (echo) {
                         float dx, dy, dz, v0, v1, v2;
                         const float w = echo_weight(G, px[k], py[k], pz[k], dx, dy, dz, v0, v1, v2);
-                        num[k] = __builtin_fmaf(G.I, w, num[k]);
-                        den[k] += w;
+                        num[k] = __builtin_fma((double)G.I, (double)w, num[k]);
+                        den[k] += (double)w;
                     }
                 }
             }
@@ -325,14 +328,14 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
             const int y = c0 + tid + k * kRayThreads;
             if (y >= H) continue;
             const int64_t o = (int64_t)fr.pixel_base + (int64_t)y * W + x;
-            float B = 0.f;
+            double B = 0.0, E = 0.0;
             if (s1 > s0) {
-                const float gamma = 1.0f - __expf(-den[k]);
-                const float E = num[k] / (den[k] + 1e-6f);
+                const double gamma = 1.0 - exp(-den[k]);
+                E = num[k] / (den[k] + 1e-6);
                 B = T[k] * (gamma * E);
             }
-            image[o] = B;
-            aux[o] = make_float4(T[k], num[k], den[k], 0.f);
+            image[o] = (float)B;
+            aux[o] = make_float4((float)T[k], (float)E, (float)den[k], 0.f);
         }
         __syncthreads();
     }
@@ -358,7 +361,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     const float4* __restrict__ aux, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
     float4* __restrict__ partial) {
     __shared__ float4 s_p[kRayChunk];  // px, py, pz, tpx
-    __shared__ float4 s_c[kRayChunk];  // cT, dNum, dW0, 0
+    __shared__ float4 s_c[kRayChunk];  // cT, dNum, dS, E
     if (stats->overflow) return;
     const int t = blockIdx.x;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
@@ -392,22 +395,20 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                 const float tp = fmaxf(0.0f, dot3(p[0] - ro[0], p[1] - ro[1], p[2] - ro[2], rd[0], rd[1], rd[2]));
                 const int64_t o = (int64_t)fr.pixel_base + (int64_t)y * W + x;
                 const float gB = d_image[o];
-                float cT = 0.f, dNum = 0.f, dW0 = 0.f;
+                float cT = 0.f, dNum = 0.f, dS = 0.f, E = 0.f;
                 if (!(fabsf(gB) < F.grad_threshold)) {
-                    const float4 a = aux[o];  // T, num, den
-                    const float gamma = 1.0f - __expf(-a.z);
-                    const float inv = 1.0f / (a.z + 1e-6f);
-                    const float Eraw = a.y * inv;
-                    const float dT = gB * (gamma * Eraw);
-                    const float dE = gB * a.x;
+                    const float4 a = aux[o];  // T, E = num / (den + 1e-6), den
+                    const float ex = __expf(-a.z);
+                    const float gamma = 1.0f - ex;
+                    E = a.y;
+                    const float dE = gB * a.x;           // dL/dE_val
                     const float dEraw = dE * gamma;
-                    const float dS = dE * Eraw * __expf(-a.z);
-                    dNum = dEraw * inv;
-                    dW0 = -dEraw * a.y * inv * inv + dS;
-                    cT = dT * a.x;
+                    dS = dE * E * ex;                      // through gamma = 1 - e^-S
+                    dNum = dEraw / (a.z + 1e-6f);          // dL/dw = dNum (I - E) + dS (= dNum I + dDen + dS)
+                    cT = gB * (gamma * E) * a.x;           // dL/dT * T
                 }
                 s_p[j] = make_float4(p[0], p[1], p[2], tp);
-                s_c[j] = make_float4(cT, dNum, dW0, 0.f);
+                s_c[j] = make_float4(cT, dNum, dS, E);
             }
             __syncthreads();
             if (!active) continue;
@@ -460,7 +461,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                     float dx, dy, dz, v0, v1, v2;
                     const float w = echo_weight(G, pp.x, pp.y, pp.z, dx, dy, dz, v0, v1, v2);
                     dsh = __builtin_fmaf(cc.y, w, dsh);
-                    const float dw = __builtin_fmaf(cc.y, G.I, cc.z);
+                    const float dw = __builtin_fmaf(cc.y, G.I - cc.w, cc.z);
                     const float dQ = -0.5f * dw * w;
                     u[0] = __builtin_fmaf(dQ, dx, u[0]);
                     u[1] = __builtin_fmaf(dQ, dy, u[1]);
@@ -628,10 +629,13 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
         if ((rc = check_launch("ur_pack_kernel"))) return rc;
     }
     if (plan->n_pairs > 0) {
-        hipLaunchKernelGGL(ur_preprocess_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
-                           plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
-                           settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
-                           ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk));
+        {
+            StageTimer timer_("ur_preprocess_kernel", stream);
+            hipLaunchKernelGGL(ur_preprocess_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
+                               plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
+                               settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
+                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk));
+        }
         if ((rc = check_launch("ur_preprocess_kernel"))) return rc;
     }
     return build_instances(plan, L, stream);
@@ -646,10 +650,13 @@ int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
     if (!pixels || !ray_origins || !ray_dirs || !image || !aux) return fail(US3D_ERR_ARGUMENT, "ur_forward: NULL array");
-    hipLaunchKernelGGL(ur_forward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
-                       ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
-                       plan->frames, plan->frame_tile_base, plan->n_frames, pixels, ray_origins, ray_dirs, ray_offsets,
-                       flags_of(settings), ws_at<us3d_stats>(plan, L.stats), image, reinterpret_cast<float4*>(aux));
+    {
+        StageTimer timer_("ur_forward_kernel", as_stream(stream_));
+        hipLaunchKernelGGL(ur_forward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
+                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
+                           plan->frames, plan->frame_tile_base, plan->n_frames, pixels, ray_origins, ray_dirs, ray_offsets,
+                           flags_of(settings), ws_at<us3d_stats>(plan, L.stats), image, reinterpret_cast<float4*>(aux));
+    }
     return check_launch("ur_forward_kernel");
 }
 
@@ -666,16 +673,22 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
         !d_attenuations || !d_sh)
         return fail(US3D_ERR_ARGUMENT, "ur_backward: NULL array");
     hipStream_t stream = as_stream(stream_);
-    hipLaunchKernelGGL(ur_backward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
-                       ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
-                       ws_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
-                       ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
-                       d_image, ws_at<us3d_stats>(plan, L.stats), ws_at<float4>(plan, L.partial));
+    {
+        StageTimer timer_("ur_backward_kernel", stream);
+        hipLaunchKernelGGL(ur_backward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
+                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
+                           ws_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
+                           ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
+                           d_image, ws_at<us3d_stats>(plan, L.stats), ws_at<float4>(plan, L.partial));
+    }
     if ((rc = check_launch("ur_backward_kernel"))) return rc;
     if (plan->n_gauss > 0) {
-        hipLaunchKernelGGL(ur_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
-                           plan->sweeps, plan->n_sweeps, ws_at<int32_t>(plan, L.pair_off), ws_at<float4>(plan, L.partial),
-                           ws_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_attenuations, d_sh);
+        {
+            StageTimer timer_("ur_reduce_kernel", stream);
+            hipLaunchKernelGGL(ur_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
+                               plan->sweeps, plan->n_sweeps, ws_at<int32_t>(plan, L.pair_off), ws_at<float4>(plan, L.partial),
+                               ws_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_attenuations, d_sh);
+        }
         if ((rc = check_launch("ur_reduce_kernel"))) return rc;
     }
     return US3D_OK;

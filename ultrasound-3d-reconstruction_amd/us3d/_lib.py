@@ -143,7 +143,13 @@ SIGNATURES = {
     ),
     "us3d_scan_tmp_bytes": (_I64, [_I64]),
     "us3d_scan_i32": (ctypes.c_int, [_VP, _VP, _I64, _VP, _VP, _VP]),
+    "us3d_image_loss_tmp_bytes": (_I64, [_I32]),
+    "us3d_image_loss": (ctypes.c_int, [_VP, _VP, _VP, _I32, _F, _F, _F, _VP, _VP, _VP, _VP]),
+    "us3d_adam_step": (ctypes.c_int, [_VP, _VP, _VP, _VP, _I64, _VP, _VP, _F, _F, _F, _F, _VP]),
+    "us3d_adam_advance": (ctypes.c_int, [_VP, _VP, _VP]),
     "us3d_fill_i32": (ctypes.c_int, [_VP, _I64, _I32, _VP]),
+    "us3d_stage_timing": (ctypes.c_int, [ctypes.c_int]),
+    "us3d_stage_timing_read": (ctypes.c_int, [ctypes.c_char_p, _I64]),
     "us3d_last_error": (ctypes.c_char_p, []),
     "us3d_version": (ctypes.c_int, []),
 }
@@ -184,6 +190,23 @@ def ptr(t) -> int:
     if t is None:
         return 0
     return int(t.data_ptr())
+
+
+def stage_timing(enable: bool) -> None:
+    check(lib().us3d_stage_timing(1 if enable else 0), "us3d_stage_timing")
+
+
+def stage_timing_read() -> dict:
+    """{kernel: (launches, total_ms)} since the last read (synchronises on the recorded events)."""
+    L = lib()
+    need = L.us3d_stage_timing_read(None, 0)
+    buf = ctypes.create_string_buffer(max(int(need), 1) + 64)
+    check(L.us3d_stage_timing_read(buf, len(buf)), "us3d_stage_timing_read")
+    out = {}
+    for line in buf.value.decode().splitlines():
+        name, n, ms = line.split()
+        out[name] = (int(n), float(ms))
+    return out
 
 
 def stream_ptr(device: torch.device | None = None) -> int:

@@ -1,0 +1,146 @@
+"""A whole slice-splat training step over a batch of frames as one captured hipGraph.
+
+Per step (all on one stream, no host synchronisation, no allocation):
+  sigmoid activations -> us3d_ug_map -> us3d_ug_forward -> us3d_image_loss (L1 + w (1 - global SSIM) and its
+  gradient, pipeline_god_file.py:606-637) -> us3d_ug_backward -> chain through the sigmoids -> us3d_adam_step
+  for each parameter group (torch.optim.Adam semantics, :740,764) -> us3d_adam_advance.
+The instance arrays have a fixed capacity (from an eager mapping of the same batch, with headroom). If a step's
+mapping overflows it, the Adam kernels skip the update (they read the overflow flag), `step()` notices it when it
+checks the stats, grows the capacity, captures again and repeats the step -- so an overflow never corrupts the
+parameters. The overflow check synchronises; `check_every` sets how often it runs.
+"""
+
+from __future__ import annotations
+
+from typing import List, Optional, Sequence
+
+import torch
+
+from . import _lib
+from .batch import CapacityHint, FrameSpec, SweepSpec, read_stats
+from .ultragauss import UGBatch, UltraGaussRenderConfig
+
+
+class SliceSplatTrainer:
+    def __init__(self, means: torch.Tensor, tril: torch.Tensor, raw_opacity: torch.Tensor,
+                 raw_intensity: torch.Tensor, frames: Sequence[FrameSpec], target: torch.Tensor,
+                 cfg: UltraGaussRenderConfig, sweeps: Optional[Sequence[SweepSpec]] = None, lr: float = 1e-3,
+                 betas=(0.9, 0.999), eps: float = 1e-8, w_l1: float = 1.0, w_ssim: float = 0.2,
+                 headroom: float = 1.3, use_graph: bool = True, check_every: int = 1):
+        dev = means.device
+        self.dev = dev
+        G = means.shape[0]
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.means = means.detach().to(**f32).contiguous().clone()
+        self.tril = tril.detach().to(**f32).contiguous().clone()
+        self.raw_o = raw_opacity.detach().to(**f32).reshape(G).contiguous().clone()
+        self.raw_c = raw_intensity.detach().to(**f32).reshape(G).contiguous().clone()
+        self.params = [self.means, self.tril, self.raw_o, self.raw_c]
+        self.exp_avg = [torch.zeros_like(p) for p in self.params]
+        self.exp_avg_sq = [torch.zeros_like(p) for p in self.params]
+        self.step_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.lr, self.betas, self.eps = float(lr), betas, float(eps)
+        self.w_l1, self.w_ssim = float(w_l1), float(w_ssim)
+        self.cfg = cfg
+        self.batch = UGBatch(frames, sweeps or [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
+        lay = self.batch.layout
+        self.F = lay.n_frames
+        P = lay.n_pixels
+        self.target = target.detach().to(**f32).reshape(P).contiguous()
+        self.opac = torch.empty(G, **f32)
+        self.inten = torch.empty(G, **f32)
+        self.image = torch.empty(P, **f32)
+        self.alpha = torch.empty(P, **f32)
+        self.depth = torch.empty(P, **f32)
+        self.aux = torch.empty((P, 4), **f32)
+        self.d_image = torch.empty(P, **f32)
+        self.loss = torch.empty(self.F, **f32)
+        self.loss_tmp = torch.empty(int(_lib.lib().us3d_image_loss_tmp_bytes(self.F)) // 8, dtype=torch.float64,
+                                    device=dev)
+        self.grads = [torch.empty_like(p) for p in self.params]
+        self.d_opac = torch.empty(G, **f32)
+        self.d_inten = torch.empty(G, **f32)
+        self.headroom = headroom
+        self.use_graph = use_graph
+        self.check_every = max(1, int(check_every))
+        self.n_steps = 0
+        self.graph = None
+        # size the instance arrays from an eager mapping of the current parameters
+        with torch.no_grad():
+            m = self.batch.map(self.means, self.tril, torch.sigmoid(self.raw_o), torch.sigmoid(self.raw_c))
+        self._allocate(int(m.stats.n_instances * headroom) + 4096)
+
+    # ------------------------------------------------------------------ plumbing
+    def _allocate(self, k_cap: int) -> None:
+        self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+        self.workspace = torch.empty(self.batch.workspace_bytes(self.k_capacity), dtype=torch.uint8, device=self.dev)
+        self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
+        off = self.batch.layout_offsets(self.k_capacity)
+        self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
+        self.graph = None
+
+    def _body(self) -> None:
+        L = _lib.lib()
+        s = _lib.stream_ptr(self.dev)
+        st = self.batch.settings
+        lay = self.batch.layout
+        torch.sigmoid(self.raw_o, out=self.opac)
+        torch.sigmoid(self.raw_c, out=self.inten)
+        _lib.check(L.us3d_ug_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.opac),
+                                 _lib.ptr(self.inten), st, s), "us3d_ug_map")
+        _lib.check(L.us3d_ug_forward(self.plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
+                                     _lib.ptr(self.depth), _lib.ptr(self.aux), s), "us3d_ug_forward")
+        _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
+                                     lay.n_frames, self.w_l1, self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
+                                     _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
+        _lib.check(L.us3d_ug_backward(self.plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
+                                      _lib.ptr(self.d_image), None, None, _lib.ptr(self.grads[0]),
+                                      _lib.ptr(self.grads[1]), _lib.ptr(self.d_opac), _lib.ptr(self.d_inten), s),
+                   "us3d_ug_backward")
+        # d raw = d sigmoid * s (1 - s)
+        torch.mul(self.d_opac, self.opac * (1.0 - self.opac), out=self.grads[2])
+        torch.mul(self.d_inten, self.inten * (1.0 - self.inten), out=self.grads[3])
+        b1, b2 = self.betas
+        for p, g, m, v in zip(self.params, self.grads, self.exp_avg, self.exp_avg_sq):
+            _lib.check(L.us3d_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v), p.numel(),
+                                        _lib.ptr(self.step_count), self.overflow_ptr, self.lr, b1, b2, self.eps, s),
+                       "us3d_adam_step")
+        _lib.check(L.us3d_adam_advance(_lib.ptr(self.step_count), self.overflow_ptr, s), "us3d_adam_advance")
+
+    def _capture(self) -> None:
+        side = torch.cuda.Stream(device=self.dev)
+        side.wait_stream(torch.cuda.current_stream(self.dev))
+        with torch.cuda.stream(side):
+            pass  # no eager warm-up inside the graph step: it would apply an Adam update
+        torch.cuda.current_stream(self.dev).wait_stream(side)
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self._body()
+        self.graph = g
+
+    def overflowed(self) -> bool:
+        return bool(read_stats(self.workspace).overflow)
+
+    # ------------------------------------------------------------------ public
+    def step(self) -> None:
+        """One training step; repeats the step with a larger capacity if the mapping overflowed."""
+        while True:
+            if self.use_graph:
+                if self.graph is None:
+                    self._capture()
+                self.graph.replay()
+            else:
+                self._body()
+            self.n_steps += 1
+            if self.n_steps % self.check_every != 0:
+                return
+            st = read_stats(self.workspace)
+            if not st.overflow:
+                return
+            self._allocate(int(st.n_instances * self.headroom) + 4096)
+
+    def state(self) -> List[torch.Tensor]:
+        return [p.detach() for p in self.params]
+
+    def losses(self) -> torch.Tensor:
+        return self.loss.detach().clone()
