@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy 5 times (reference dispatch for HBM counters)")
     return ap.parse_args()
 
 
@@ -114,8 +116,24 @@ def read_traffic(kernel: str, workload_tag: str):
     return float(m.group(1)) if m else None
 
 
+def copy_calibration():
+    """A plain copy kernel of known size: reads and writes exactly 2^30 bytes per dispatch."""
+    import torch
+
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_dispatch": 1 << 30, "copy_bytes_written_per_dispatch": 1 << 30}))
+
+
 def main():
     args = parse()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus_requested": args.gpus}))

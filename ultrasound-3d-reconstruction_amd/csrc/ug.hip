@@ -199,9 +199,16 @@ __global__ void ug_emit_kernel(int64_t n_pairs, const us3d_frame* __restrict__ f
 // ---------------------------------------------------------------- splat geometry
 // A tile of TS x TS pixels is split into 8x8 sub-blocks; one wavefront owns a sub-block at a time, so a
 // test of a Gaussian against a sub-block is wavefront-uniform. Each Gaussian loaded into LDS carries a
-// bit mask of the sub-blocks its lateral/axial chi2 window can reach: a pixel with q <= chi2 satisfies
-// |(p - mu).a| <= r_a along both in-plane axes (r_a is the ellipsoid's marginal extent), so skipping a
-// sub-block outside the (margin-widened) window never drops a contributing pair.
+// bit mask of the sub-blocks that can hold a pixel with q <= chi2. Two conservative tests, both widened for
+// rounding, are ANDed:
+//   marginal -- such a pixel satisfies |(p - mu).a| <= r_a along both in-plane axes (r_a is the ellipsoid's
+//               marginal extent), valid for any pixel positions;
+//   section  -- with orthonormal probe axes, write q in probe coordinates (lat, ax, ele) with precision
+//               P = (R^T L)(R^T L)^T, R = [a_lat a_ax a_ele]. At elevation offset d (pixel minus mean) the set
+//               q <= chi2 is the ellipse (x - u d)^T M (x - u d) <= chi2 - k d^2, M = P[:2,:2],
+//               u = -M^-1 P[:2,2], k = P[2,2] - P[2,:2] M^-1 P[:2,2]; over the sub-block's elevation range
+//               its bounding box is [u d_lo, u d_hi] widened by sqrt(rem M^-1_ii), rem = chi2 - k d*^2 at the
+//               d* of the range closest to 0. Skipping a sub-block never drops a contributing pair.
 
 template <int TS>
 struct TileShape {
@@ -213,7 +220,28 @@ struct TileShape {
 };
 
 struct SubBounds {
-    float lat_lo, lat_hi, ax_lo, ax_hi;
+    float lat_lo, lat_hi, ax_lo, ax_hi, ele_lo, ele_hi;
+};
+
+// Grouping key of a sub-block mask: the mask itself for up to 4 sub-blocks; for a 4x4 grid of sub-blocks,
+// the 4 bits "touches quadrant q" (16 classes).
+template <int NSUB>
+struct MaskClass {
+    static constexpr int kClasses = (NSUB <= 4) ? (1 << NSUB) : 16;
+    __device__ static __forceinline__ uint32_t of(uint32_t m) {
+        if constexpr (NSUB <= 4) {
+            return m;
+        } else {
+            static_assert(NSUB == 16, "4x4 sub-block grid expected");
+            uint32_t c = 0;
+#pragma unroll
+            for (int sb = 0; sb < 16; ++sb) {
+                const int q = ((sb >> 3) << 1) | ((sb & 3) >> 1);  // row/2 * 2 + col/2 of the 4x4 grid
+                c |= ((m >> sb) & 1u) << q;
+            }
+            return c;
+        }
+    }
 };
 
 // Pixel of sub-block sb handled by `lane`.
@@ -223,35 +251,101 @@ __device__ __forceinline__ void sub_pixel(int tx, int ty, int sb, int lane, int&
     py = ty * TS + (sb / TileShape<TS>::kSubPerRow) * 8 + (lane >> 3);
 }
 
-// Lateral/axial extent of the valid pixels of a sub-block (wave reduction), widened for rounding.
-__device__ __forceinline__ SubBounds sub_bounds(bool valid, float lat, float ax) {
+// Lateral/axial/elevational extent of the valid pixels of a sub-block (wave reduction), widened for rounding.
+__device__ __forceinline__ SubBounds sub_bounds(bool valid, float lat, float ax, float ele) {
     SubBounds b;
     b.lat_lo = wave_min(valid ? lat : 3.0e38f);
     b.lat_hi = wave_max(valid ? lat : -3.0e38f);
     b.ax_lo = wave_min(valid ? ax : 3.0e38f);
     b.ax_hi = wave_max(valid ? ax : -3.0e38f);
+    b.ele_lo = wave_min(valid ? ele : 3.0e38f);
+    b.ele_hi = wave_max(valid ? ele : -3.0e38f);
     const float pad = 1e-3f + 4e-6f * fmaxf(fmaxf(fabsf(b.lat_lo), fabsf(b.lat_hi)), fmaxf(fabsf(b.ax_lo), fabsf(b.ax_hi)));
     if (b.lat_lo <= b.lat_hi) {
         b.lat_lo -= pad; b.lat_hi += pad; b.ax_lo -= pad; b.ax_hi += pad;
+        b.ele_lo -= pad; b.ele_hi += pad;
     }
     return b;
+}
+
+// Probe axes orthonormal to within 1e-5 (the section test needs it; the marginal test does not).
+__device__ __forceinline__ bool axes_orthonormal(const us3d_frame& fr) {
+    const float* a = fr.axis_lat;
+    const float* b = fr.axis_ax;
+    const float* c = fr.axis_ele;
+    const float ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const float ac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    const float bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    const float aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    const float bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    const float cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    return fabsf(ab) < 1e-5f && fabsf(ac) < 1e-5f && fabsf(bc) < 1e-5f && fabsf(aa - 1.f) < 2e-5f &&
+           fabsf(bb - 1.f) < 2e-5f && fabsf(cc - 1.f) < 2e-5f;
 }
 
 // Sub-block mask of one Gaussian (see above). Degenerate factors (l_ii <= 1e-10) get a full mask.
 template <int NSUB>
 __device__ __forceinline__ uint32_t gauss_sub_mask(const SubBounds* __restrict__ sb, float l11, float l12, float l13,
                                                    float l22, float l23, float l33, float lat_g, float ax_g,
-                                                   const us3d_frame& fr, float sqrt_chi2) {
-    if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) return (NSUB >= 32) ? 0xffffffffu : ((1u << NSUB) - 1u);
+                                                   float ele_g, const us3d_frame& fr, float sqrt_chi2, bool ortho) {
+    constexpr uint32_t kAll = (NSUB >= 32) ? 0xffffffffu : ((1u << NSUB) - 1u);
+    if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) return kAll;
     const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
     const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-    const float wl = r_lat * 1.001f + 1e-3f + 4e-6f * fabsf(lat_g);
-    const float wa = r_ax * 1.001f + 1e-3f + 4e-6f * fabsf(ax_g);
+    const float pad = 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
+    const float wl = r_lat * 1.001f + pad;
+    const float wa = r_ax * 1.001f + pad;
+    // section test setup in FP64 (no cancellation in det or the Schur complement): rows of A = R^T L,
+    // L = [[l11 0 0] [l12 l22 0] [l13 l23 l33]] so that q = |L^T d|^2
+    bool sect = ortho;
+    float u_lat = 0.f, u_ax = 0.f, i00 = 0.f, i11 = 0.f, kk = 0.f;
+    if (sect) {
+        const float* a = fr.axis_lat;
+        const float* c = fr.axis_ax;
+        const float* e = fr.axis_ele;
+        const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
+        const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
+        const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
+        const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
+        const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
+        const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
+        const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
+        const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
+        const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
+        const double cz = C0 * C0 + C1 * C1 + C2 * C2;
+        const double det = M00 * M11 - M01 * M01;
+        sect = det > 1e-8 * M00 * M11;
+        if (sect) {
+            const double j00 = M11 / det, j11 = M00 / det, j01 = -M01 / det;
+            const double w0 = j00 * b0 + j01 * b1, w1 = j01 * b0 + j11 * b1;
+            u_lat = (float)(-w0);
+            u_ax = (float)(-w1);
+            i00 = (float)j00 * 1.0001f;
+            i11 = (float)j11 * 1.0001f;
+            kk = (float)fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
+        }
+    }
+    const float chi2 = sqrt_chi2 * sqrt_chi2 * 1.002f;
     uint32_t m = 0;
 #pragma unroll
     for (int s = 0; s < NSUB; ++s) {
-        const bool hit = (lat_g + wl >= sb[s].lat_lo) && (lat_g - wl <= sb[s].lat_hi) && (ax_g + wa >= sb[s].ax_lo) &&
-                         (ax_g - wa <= sb[s].ax_hi);
+        bool hit = (lat_g + wl >= sb[s].lat_lo) && (lat_g - wl <= sb[s].lat_hi) && (ax_g + wa >= sb[s].ax_lo) &&
+                   (ax_g - wa <= sb[s].ax_hi);
+        if (hit && sect) {
+            const float d_lo = sb[s].ele_lo - ele_g, d_hi = sb[s].ele_hi - ele_g;
+            const float d0 = fminf(fmaxf(0.0f, d_lo), d_hi);
+            const float rem = chi2 - kk * d0 * d0;
+            if (rem <= 0.0f) {
+                hit = false;
+            } else {
+                const float hl = sqrtf(rem * i00) * 1.002f + pad;
+                const float ha = sqrtf(rem * i11) * 1.002f + pad;
+                const float cl0 = lat_g + u_lat * d_lo, cl1 = lat_g + u_lat * d_hi;
+                const float ca0 = ax_g + u_ax * d_lo, ca1 = ax_g + u_ax * d_hi;
+                hit = (fmaxf(cl0, cl1) + hl >= sb[s].lat_lo) && (fminf(cl0, cl1) - hl <= sb[s].lat_hi) &&
+                      (fmaxf(ca0, ca1) + ha >= sb[s].ax_lo) && (fminf(ca0, ca1) - ha <= sb[s].ax_hi);
+            }
+        }
         m |= (hit ? 1u : 0u) << s;
     }
     return m;
@@ -294,6 +388,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+    const bool ortho = axes_orthonormal(fr);
 
     float px[PPT], py[PPT], pz[PPT];
     bool valid[PPT];
@@ -311,7 +406,8 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
         const float dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
         const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
         const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        const SubBounds b = sub_bounds(valid[k], lat, ax);
+        const float ele = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+        const SubBounds b = sub_bounds(valid[k], lat, ax, ele);
         if (lane == 0) s_sb[sb] = b;
         S_[k] = N_[k] = D_[k] = 0.0;
     }
@@ -328,29 +424,38 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
             const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            s_mask[tid] = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, fr, sqrt_chi2);
+            s_mask[tid] = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, e, fr,
+                                                  sqrt_chi2, ortho);
             s_r0[tid] = r0;                                  // mu, opacity
             s_r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
             s_r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
         }
         __syncthreads();
-        for (int j = 0; j < n; ++j) {
-            const uint32_t m = s_mask[j];
-            const float4 a = s_r0[j];
-            const float4 b = s_r1[j];
-            const float4 c = s_r2[j];
+        // The Gaussians that touch this wave's sub-block, 64 list entries at a time: a ballot of their mask
+        // bits gives a wave-uniform 64-bit set that is walked lowest-first with scalar bit scans, so skipped
+        // entries cost nothing and each pixel still sums its Gaussians in list order.
+        for (int jb = 0; jb < n; jb += kWave) {
+            const uint32_t mm = (jb + lane < n) ? s_mask[jb + lane] : 0u;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                if (!((m >> (wave + k * S::kWaves)) & 1u)) continue;
-                float y0, y1, y2;
-                const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z, y0, y1, y2);
-                if (q <= chi2) {
-                    const float ah = a.w * __expf(-0.5f * q);
-                    if (ah > alpha_min) {
-                        const double ahd = (double)ah;
-                        S_[k] += ahd;
-                        N_[k] = __builtin_fma(ahd, (double)b.x, N_[k]);
-                        D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
+                uint64_t bits = __ballot((mm >> (wave + k * S::kWaves)) & 1u);
+                while (bits) {
+                    const int j = jb + (int)__builtin_ctzll(bits);
+                    bits &= bits - 1;
+                    const float4 a = s_r0[j];
+                    const float4 b = s_r1[j];
+                    const float4 c = s_r2[j];
+                    float y0, y1, y2;
+                    const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z,
+                                              y0, y1, y2);
+                    if (q <= chi2) {
+                        const float ah = a.w * __expf(-0.5f * q);
+                        if (ah > alpha_min) {
+                            const double ahd = (double)ah;
+                            S_[k] += ahd;
+                            N_[k] = __builtin_fma(ahd, (double)b.x, N_[k]);
+                            D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
+                        }
                     }
                 }
             }
@@ -372,10 +477,15 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
 
 // ---------------------------------------------------------------- adjoint
 // One workgroup per (frame, tile). Per-pixel adjoint coefficients are computed once from the stored sums
-// and staged in LDS (sub-block-major); then each thread owns ONE Gaussian of the tile's list and loops over
-// the tile's pixels (LDS broadcasts), accumulating its 11 gradient terms in registers: no atomics, no
-// cross-lane reductions. Formulas follow kernel_pixel_ultragauss_backward,
-// ops/ultragauss_renderer/src/shader/backward.cu:17-131, with q = ||L^T d||^2:
+// and staged in LDS (sub-block-major). The work unit is an ITEM: one Gaussian of the tile's list paired with
+// one 8x8 sub-block its mask keeps. A batch of up to kThreads Gaussians is loaded, its items are listed
+// Gaussian-major (a Gaussian's items are consecutive, in sub-block order), and the wavefronts take chunks of
+// 64 items: each lane loops over its item's 64 pixels (LDS reads with at most one address per sub-block),
+// accumulating the 12 gradient sums in registers with no atomics. The lanes of one Gaussian are adjacent, so
+// a segmented lane reduction in a fixed tree order combines them; a Gaussian cut by a chunk boundary keeps
+// its two halves in LDS and they are added after the pass. Every step is order-fixed: bitwise reproducible.
+// Formulas follow kernel_pixel_ultragauss_backward, ops/ultragauss_renderer/src/shader/backward.cu:17-131,
+// with q = ||L^T d||^2:
 //   dN = gI/A', v = num/A' (unclamped image), dX = gA - gD*Dn/S'^2, dD = gD/S'
 //   (A' = max(A,1e-8), S' = max(S,1e-8)); dah = dN*(c - v) + dX + dD*elev (= dN*c + dA + ..., written so
 //   that c - v is formed from two FP32 numbers instead of cancelling dN*c against dN*v);
@@ -383,6 +493,27 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
 //   dmu = -L * sum(m y) + sum(dD*ah) * a_ele;  dl_ij = sum(m * y_j * d_i)
 // Pairs are gated exactly as in the forward (q <= chi2 and ah > alpha_min), so the adjoint is the exact
 // derivative of the forward. Partials go to the instance's emit index.
+
+// float4 stride of a staged sub-block in LDS: 65 puts the sub-blocks that one instruction reads in
+// different banks.
+constexpr int kSubStride = 65;
+
+// The 12 sums of one Gaussian: do, dc, sum(dD ah), sum(m y_0..2), dl11, dl12, dl13, dl22, dl23, dl33.
+struct GradSums {
+    float v[12];
+};
+
+__device__ __forceinline__ void write_partial(float4* __restrict__ partial, int64_t ei, const GradSums& a,
+                                              const float4& r1, const float4& r2, const us3d_frame& fr) {
+    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+    const float aed = a.v[2], Y0 = a.v[3], Y1 = a.v[4], Y2 = a.v[5];
+    const float dmx = __builtin_fmaf(aed, fr.axis_ele[0], -(l11 * Y0));
+    const float dmy = __builtin_fmaf(aed, fr.axis_ele[1], -__builtin_fmaf(l22, Y1, l12 * Y0));
+    const float dmz = __builtin_fmaf(aed, fr.axis_ele[2], -__builtin_fmaf(l33, Y2, __builtin_fmaf(l23, Y1, l13 * Y0)));
+    partial[3 * ei + 0] = make_float4(a.v[0], a.v[1], dmx, dmy);
+    partial[3 * ei + 1] = make_float4(dmz, a.v[6], a.v[7], a.v[8]);
+    partial[3 * ei + 2] = make_float4(a.v[9], a.v[10], a.v[11], 0.f);
+}
 
 template <int TS>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
@@ -395,10 +526,20 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     using S = TileShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
-    constexpr int NPIX = TS * TS;
-    __shared__ float4 s_p[NPIX];   // px, py, pz, dN
-    __shared__ float4 s_c[NPIX];   // unclamped image v, dX, dD
-    __shared__ SubBounds s_sb[S::kSub];
+    constexpr int NSUB = S::kSub;
+    constexpr int NW = S::kWaves;
+    constexpr int MAXI = NT * NSUB;     // items of one batch, at most
+    constexpr int MAXC = MAXI / kWave;  // chunks of one batch, at most
+    static_assert(NT <= 256 && NSUB <= 16, "item code packs (slot << 4) | sub-block into 12 bits");
+    __shared__ float4 s_p[NSUB * kSubStride];  // px, py, pz, dN
+    __shared__ float4 s_c[NSUB * kSubStride];  // unclamped image v, dX, dD
+    __shared__ SubBounds s_sb[NSUB];
+    __shared__ float4 s_g0[NT], s_g1[NT], s_g2[NT];  // the batch's Gaussian records
+    __shared__ uint16_t s_item[MAXI];
+    __shared__ int s_wtot[NW];
+    __shared__ GradSums s_pend[MAXC];   // head part of a Gaussian that continues into the next chunk
+    __shared__ GradSums s_carry[MAXC];  // tail part of a Gaussian that began in the previous chunk
+    __shared__ int s_pend_slot[MAXC];
     if (stats->overflow) return;
 
     const int t = blockIdx.x;
@@ -410,6 +551,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     if (s0 == s1) return;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+    const bool ortho = axes_orthonormal(fr);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -436,75 +578,156 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
         const float dx = x - ox, dy = y - oy, dz = z - oz;
         const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
         const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        const SubBounds b = sub_bounds(valid, lat, ax);
+        const float ele = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+        const SubBounds b = sub_bounds(valid, lat, ax, ele);
         if (lane == 0) s_sb[sb] = b;
-        s_p[sb * 64 + lane] = make_float4(x, y, z, dN);
-        s_c[sb * 64 + lane] = make_float4(vimg, dX, dD, 0.f);
+        s_p[sb * kSubStride + lane] = make_float4(x, y, z, dN);
+        s_c[sb * kSubStride + lane] = make_float4(vimg, dX, dD, 0.f);
     }
-    __syncthreads();
 
     for (int base = s0; base < s1; base += NT) {
-        const int i = base + tid;
-        const bool active = i < s1;
-        float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
-        uint32_t mask = 0;
-        float e = 0.f;
-        if (active) {
-            const int g = (int)(uint32_t)seg_key[i];
-            r0 = rec[3 * (int64_t)g + 0];
-            r1 = rec[3 * (int64_t)g + 1];
-            r2 = rec[3 * (int64_t)g + 2];
-            const float vx = r0.x - ox, vy = r0.y - oy, vz = r0.z - oz;
-            e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+        const int n = min(NT, s1 - base);
+        __syncthreads();  // staged pixels are in place; the previous batch is done with the batch arrays
+        uint32_t m = 0;
+        if (tid < n) {
+            const int g = (int)(uint32_t)seg_key[base + tid];
+            const float4 q0 = rec[3 * (int64_t)g + 0];
+            const float4 q1 = rec[3 * (int64_t)g + 1];
+            const float4 q2 = rec[3 * (int64_t)g + 2];
+            const float vx = q0.x - ox, vy = q0.y - oy, vz = q0.z - oz;
+            const float ge = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            mask = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, fr, sqrt_chi2);
+            m = gauss_sub_mask<NSUB>(s_sb, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, lat_g, ax_g, ge, fr, sqrt_chi2, ortho);
+            s_g0[tid] = q0;
+            s_g1[tid] = q1;
+            s_g2[tid] = q2;
+            if (m == 0) {  // touches no pixel of the tile: its partial is zero
+                const int64_t ei = seg_val[base + tid];
+                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                partial[3 * ei + 0] = z4;
+                partial[3 * ei + 1] = z4;
+                partial[3 * ei + 2] = z4;
+            }
         }
-        const float mx = r0.x, my = r0.y, mz = r0.z, o = r0.w, c = r1.x;
-        const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
-        float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
-        float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
-        for (int sb = 0; sb < S::kSub; ++sb) {
-            const bool mine = (mask >> sb) & 1u;
-            if (!__any(mine)) continue;
-            if (!mine) continue;
-            for (int j = sb * 64; j < sb * 64 + 64; ++j) {
-                const float4 pp = s_p[j];
-                const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
-                float y0, y1, y2;
-                const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
-                if (q <= chi2) {
-                    const float w = __expf(-0.5f * q);
-                    const float ah = o * w;
-                    if (ah > alpha_min) {
-                        const float4 cc = s_c[j];
-                        // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
-                        // cancellation of dN c against dN v
-                        const float da = __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y));
-                        ac = __builtin_fmaf(pp.w, ah, ac);
-                        ao = __builtin_fmaf(da, w, ao);
-                        aed = __builtin_fmaf(cc.z, ah, aed);
-                        const float m = -da * ah;
-                        const float m0 = m * y0, m1 = m * y1, m2 = m * y2;
-                        Y0 += m0; Y1 += m1; Y2 += m2;
-                        g11 = __builtin_fmaf(m0, dx, g11);
-                        g12 = __builtin_fmaf(m0, dy, g12);
-                        g13 = __builtin_fmaf(m0, dz, g13);
-                        g22 = __builtin_fmaf(m1, dy, g22);
-                        g23 = __builtin_fmaf(m1, dz, g23);
-                        g33 = __builtin_fmaf(m2, dz, g33);
+        for (int c = tid; c < MAXC; c += NT) s_pend_slot[c] = -1;
+        // exclusive scan of the per-Gaussian item counts over the batch
+        const int pc = __popc(m);
+        int incl = pc;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const int v = __shfl_up(incl, off, kWave);
+            if (lane >= off) incl += v;
+        }
+        if (lane == kWave - 1) s_wtot[wave] = incl;
+        __syncthreads();
+        int pos = incl - pc, total = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            const int tw = s_wtot[w];
+            pos += (w < wave) ? tw : 0;
+            total += tw;
+        }
+        for (uint32_t mm = m; mm; mm &= mm - 1) s_item[pos++] = (uint16_t)((tid << 4) | __builtin_ctz(mm));
+        __syncthreads();
+
+        const int nchunks = (total + kWave - 1) / kWave;
+        for (int c = wave; c < nchunks; c += NW) {
+            const int it = c * kWave + lane;
+            const bool valid = it < total;
+            const int item = valid ? (int)s_item[it] : 0;
+            const int slot = valid ? (item >> 4) : -1 - lane;  // unique negative ids for empty lanes
+            const int sb = item & 15;
+            const int prev_slot = (valid && it > 0) ? ((int)s_item[it - 1] >> 4) : -1000;
+            const int next_slot = (it + 1 < total) ? ((int)s_item[it + 1] >> 4) : -1000;
+            const bool first = valid && prev_slot != slot;  // first item of its Gaussian
+            const bool last = valid && next_slot != slot;   // last item of its Gaussian
+
+            GradSums a;
+#pragma unroll
+            for (int k = 0; k < 12; ++k) a.v[k] = 0.f;
+            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
+            if (valid) {
+                r0 = s_g0[slot];
+                r1 = s_g1[slot];
+                r2 = s_g2[slot];
+                const float mx = r0.x, my = r0.y, mz = r0.z, o = r0.w, c = r1.x;
+                const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+                const float e = dot3(mx - ox, my - oy, mz - oz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+                const float4* __restrict__ sp = s_p + sb * kSubStride;
+                const float4* __restrict__ sc = s_c + sb * kSubStride;
+                float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
+                float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
+                for (int j = 0; j < kWave; ++j) {
+                    const float4 pp = sp[j];
+                    const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
+                    float y0, y1, y2;
+                    const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
+                    if (q <= chi2) {
+                        const float w = __expf(-0.5f * q);
+                        const float ah = o * w;
+                        if (ah > alpha_min) {
+                            const float4 cc = sc[j];
+                            // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
+                            // cancellation of dN c against dN v
+                            const float da = __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y));
+                            ac = __builtin_fmaf(pp.w, ah, ac);
+                            ao = __builtin_fmaf(da, w, ao);
+                            aed = __builtin_fmaf(cc.z, ah, aed);
+                            const float mm = -da * ah;
+                            const float m0 = mm * y0, m1 = mm * y1, m2 = mm * y2;
+                            Y0 += m0; Y1 += m1; Y2 += m2;
+                            g11 = __builtin_fmaf(m0, dx, g11);
+                            g12 = __builtin_fmaf(m0, dy, g12);
+                            g13 = __builtin_fmaf(m0, dz, g13);
+                            g22 = __builtin_fmaf(m1, dy, g22);
+                            g23 = __builtin_fmaf(m1, dz, g23);
+                            g33 = __builtin_fmaf(m2, dz, g33);
+                        }
                     }
+                }
+                a.v[0] = ao; a.v[1] = ac; a.v[2] = aed; a.v[3] = Y0; a.v[4] = Y1; a.v[5] = Y2;
+                a.v[6] = g11; a.v[7] = g12; a.v[8] = g13; a.v[9] = g22; a.v[10] = g23; a.v[11] = g33;
+            }
+            // segmented reduction: a Gaussian's lanes are adjacent; after the steps the first lane of each run
+            // holds the run's sum (tree order fixed by the lane positions)
+#pragma unroll
+            for (int off = 1; off < NSUB; off <<= 1) {
+                const int oslot = __shfl_down(slot, off, kWave);
+                const bool same = (lane + off < kWave) && oslot == slot;
+#pragma unroll
+                for (int k = 0; k < 12; ++k) {
+                    const float v = __shfl_down(a.v[k], off, kWave);
+                    a.v[k] = same ? a.v[k] + v : a.v[k];
+                }
+            }
+            const int up_slot = __shfl_up(slot, 1, kWave);
+            const bool head = valid && (lane == 0 || up_slot != slot);
+            const int slot63 = __shfl(slot, kWave - 1, kWave);
+            const bool last63 = __shfl(last ? 1 : 0, kWave - 1, kWave) != 0;
+            const bool to_next = (slot == slot63) && !last63;  // run reaches lane 63, Gaussian goes on
+            const bool from_prev = (lane == 0) && valid && !first;
+            if (head) {
+                if (from_prev) {
+                    s_carry[c] = a;
+                } else if (to_next) {
+                    s_pend[c] = a;
+                    s_pend_slot[c] = slot;
+                } else {
+                    write_partial(partial, seg_val[base + slot], a, r1, r2, fr);
                 }
             }
         }
-        if (active) {
-            const float dmx = __builtin_fmaf(aed, fr.axis_ele[0], -(l11 * Y0));
-            const float dmy = __builtin_fmaf(aed, fr.axis_ele[1], -__builtin_fmaf(l22, Y1, l12 * Y0));
-            const float dmz = __builtin_fmaf(aed, fr.axis_ele[2], -__builtin_fmaf(l33, Y2, __builtin_fmaf(l23, Y1, l13 * Y0)));
-            const int64_t ei = seg_val[i];
-            partial[3 * ei + 0] = make_float4(ao, ac, dmx, dmy);
-            partial[3 * ei + 1] = make_float4(dmz, g11, g12, g13);
-            partial[3 * ei + 2] = make_float4(g22, g23, g33, 0.f);
+        __syncthreads();
+        // Gaussians cut by a chunk boundary: head part + tail part
+        for (int c = tid; c + 1 < nchunks; c += NT) {
+            const int slot = s_pend_slot[c];
+            if (slot < 0) continue;
+            GradSums a = s_pend[c];
+            const GradSums& b = s_carry[c + 1];
+#pragma unroll
+            for (int k = 0; k < 12; ++k) a.v[k] += b.v[k];
+            write_partial(partial, seg_val[base + slot], a, s_g1[slot], s_g2[slot], fr);
         }
     }
 }
