@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
+    ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
+                    help="print a markdown summary of a rocprofv3 output directory: MODE 'trace' (per-kernel "
+                         "durations of --kernel-trace) or 'pmc' (per-dispatch means of --pmc counters), then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (reference dispatch for HBM counters)")
     return ap.parse_args()
@@ -129,8 +132,47 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read_per_dispatch": 1 << 30, "copy_bytes_written_per_dispatch": 1 << 30}))
 
 
+def rocprof_summary(mode: str, directory: str) -> None:
+    """Markdown tables from the SQLite databases rocprofv3 writes (kernel durations or PMC counter means)."""
+    import collections
+    import glob
+    import sqlite3
+
+    def short(name: str) -> str:
+        name = name.split("(")[0]
+        for k in ("void us3d::", "us3d::", "(anonymous namespace)::"):
+            name = name.replace(k, "")
+        return name[:70]
+
+    dbs = glob.glob(f"{directory}/**/*.db", recursive=True)
+    agg = collections.defaultdict(list)
+    for db in dbs:
+        con = sqlite3.connect(db)
+        if mode == "trace":
+            for n, t0, t1 in con.execute("select kernel_name, start, end from kernels"):
+                agg[short(n)].append(t1 - t0)
+        else:
+            for n, c, v in con.execute("select name, counter_name, counter_value from pmc_events"):
+                agg[(short(n), c)].append(v)
+    if mode == "trace":
+        tot = sum(sum(v) for v in agg.values()) or 1
+        print("| kernel | calls | total ms | avg ms | min ms | max ms | % |")
+        print("|---|---|---|---|---|---|---|")
+        for n, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:20]:
+            print(f"| {n} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e6:.4f} | {min(v) / 1e6:.4f} | "
+                  f"{max(v) / 1e6:.4f} | {100 * sum(v) / tot:.1f} |")
+    else:
+        print("| kernel | counter | dispatches | mean per dispatch |")
+        print("|---|---|---|---|")
+        for (n, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:24]:
+            print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
+
+
 def main():
     args = parse()
+    if args.rocprof_summary:
+        rocprof_summary(*args.rocprof_summary)
+        return
     if args.copy_calibration:
         copy_calibration()
         return
