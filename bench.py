@@ -46,6 +46,11 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
+    ap.add_argument("--workload", choices=("slice", "ray"), default="slice",
+                    help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
+                         "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples")
+    ap.add_argument("--ray-frames", type=int, default=128)
+    ap.add_argument("--cpu-sample-rays", type=int, default=8)
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
                     help="print a markdown summary of a rocprofv3 output directory: MODE 'trace' (per-kernel "
                          "durations of --kernel-trace) or 'pmc' (per-dispatch means of --pmc counters), then exit")
@@ -168,6 +173,151 @@ def rocprof_summary(mode: str, directory: str) -> None:
             print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
 
 
+def ray_cpu_baseline(scene, frame, cfg, n_rays: int):
+    """The oracle's CPU restatement of the reference B-mode renderer (FP32 forward + FP64 adjoint, one thread) on
+    n_rays scanlines of one frame: the mapper runs over the whole frame, the shader over the sampled rays."""
+    import numpy as np
+
+    from oracle import ur as our
+    from us3d.scenes import ray_geometry
+
+    pix, ro, rd = (t.detach().cpu().numpy().astype(np.float32) for t in ray_geometry(frame, "cpu"))
+    n = lambda t: t.detach().cpu().numpy().astype(np.float32)  # noqa: E731
+    W = frame.width
+    st = our.RaySettings(k_sigma=cfg.k_sigma, Rw=cfg.quadrature_window_radius_mm, grad_threshold=cfg.grad_threshold)
+    fr = our.RayFrame(origin=n(frame.origin), axis_lat=n(frame.axis_lat), axis_ax=n(frame.axis_ax),
+                      axis_ele=n(frame.axis_ele), W=W, H=frame.height, width_mm=float(cfg.width_mm),
+                      near_mm=float(cfg.near_mm), col_offset=float(cfg.lateral_col_offset), pixels=pix, ray_o=ro,
+                      ray_d=rd, delta=np.zeros(W, np.float32))
+    params = [n(t) for t in scene.params()]
+    t0 = time.perf_counter()
+    rm = our.map_frame(params[0], params[1], fr, st)
+    # keep the hit lists of the sampled rays only (rays W/2 - n/2 ...), empty lists elsewhere
+    r0 = W // 2 - n_rays // 2
+    keep = np.zeros(W, bool)
+    keep[r0:r0 + n_rays] = True
+    counts = np.where(keep, rm.counts, 0).astype(np.int32)
+    rm.counts = counts
+    hits = int(counts.sum())
+    img, _aux = our.forward(params, fr, rm, st)
+    d_img = np.sign(img).astype(np.float32)
+    our.backward_f64(params, fr, rm, st, d_img)
+    dt = time.perf_counter() - t0
+    return {"value": (n_rays / W) / dt, "unit": "frames/s", "cores": 1, "kind": "port",
+            "sample": f"{n_rays} of {W} scanlines x {frame.height} samples of the middle frame ({hits} ray-Gaussian "
+                      f"hits), all {params[0].shape[0]} Gaussians mapped: oracle FP32 forward + FP64 adjoint, one "
+                      f"thread, {dt:.1f} s; frames/s = sampled fraction of a frame / time"}
+
+
+def bench_ray(args):
+    """UltraG-Ray B-mode forward+backward over a batch of frames (200k Gaussians, 128 x 1024 scanline frames)."""
+    import torch
+
+    from us3d import _lib
+    from us3d.batch import CapacityHint, FrameSpec, SweepSpec
+    from us3d.scenes import config3, make_ray_scene, ray_geometry
+    from us3d.ultragray import URBatch, UltragrayRenderConfig
+
+    dev = torch.device("cuda:0")
+    F, G = args.ray_frames, args.gaussians
+    scene, frames = config3(n_frames=F, n_gauss=G)
+    target = make_ray_scene(G, 120.0, seed=1)
+    f0 = frames[0]
+    H, W = f0.height, f0.width
+    cfg = UltragrayRenderConfig(image_width=W, image_height=H, width_mm=W * f0.delta_lat, near_mm=f0.near_mm,
+                                lateral_col_offset=f0.col_offset, ops_max_offset_mm=0.0)
+    specs = [FrameSpec(f.origin, f.axis_lat, f.axis_ax, f.axis_ele, H, W, cfg.pixel_spacing(), 0.0, f.near_mm,
+                       f.col_offset, 0) for f in frames]
+    geos = [ray_geometry(f, dev) for f in frames]
+    pix = torch.cat([g[0].reshape(-1, 3) for g in geos]).contiguous()
+    ro = torch.cat([g[1] for g in geos]).contiguous()
+    rd = torch.cat([g[2] for g in geos]).contiguous()
+    off = torch.zeros(F * W, device=dev)
+    params = [t.to(dev).float().contiguous() for t in scene.params()]
+    tparams = [t.to(dev).float().contiguous() for t in target.params()]
+
+    def run(batch, p):
+        m = batch.map(*p)
+        image, aux = batch.forward(m, pix, ro, rd, off)
+        return m, image, aux
+
+    batch = URBatch(specs, [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
+    with torch.no_grad():
+        _m, gt, _a = run(batch, tparams)
+        gt = gt.clone()
+
+    def step():
+        m, image, aux = run(batch, params)
+        d_image = torch.sign(image - gt)  # L1 (sum) loss gradient
+        return batch.backward(m, pix, ro, rd, off, aux, d_image)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    dt_b = timed(step, args.steps, args.warmup)
+    fps_b = F * args.steps / dt_b
+    print(f"[bench] ray batched: {dt_b / args.steps * 1e3:.1f} ms/step, {fps_b:.1f} frames/s", file=sys.stderr,
+          flush=True)
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    m, image, aux = run(batch, params)
+    batch.backward(m, pix, ro, rd, off, aux, torch.sign(image - gt))
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    stages = _lib.stage_timing_read()
+    K = int(m.stats.n_instances)
+    P = F * H * W
+
+    fps_p = None
+    if not args.no_plain:
+        plain = [URBatch([specs[i]], [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint()) for i in range(F)]
+
+        def step_plain():
+            for i, b in enumerate(plain):
+                ps, rs = slice(i * W * H, (i + 1) * W * H), slice(i * W, (i + 1) * W)
+                mm = b.map(*params)
+                img, ax = b.forward(mm, pix[ps], ro[rs], rd[rs], off[rs])
+                b.backward(mm, pix[ps], ro[rs], rd[rs], off[rs], ax, torch.sign(img - gt[i * W * H:(i + 1) * W * H]))
+
+        dt_p = timed(step_plain, args.plain_steps, 1)
+        fps_p = F * args.plain_steps / dt_p
+        print(f"[bench] ray plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s",
+              file=sys.stderr, flush=True)
+
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    launch_bytes = {"ur_forward_kernel": 64 * K + 8 * P, "ur_backward_kernel": 64 * K + 8 * P}
+    alg = launch_bytes.get(dom)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = ray_cpu_baseline(scene, frames[F // 2], cfg, args.cpu_sample_rays)
+    line = {
+        "metric": "UltraG-Ray fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
+        "value": fps_b, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"UltraG-Ray B-mode map + forward + adjoint, {G} Gaussians in a 120 mm cube, {F} "
+                               f"freehand frames of {W} scanlines x {H} samples (0.8 mm x 0.1 mm), L1 (sum) loss "
+                               f"gradient, batched eager", "frames": F, "gaussians": G, "ray_hits": K},
+        "plain_frames_per_s": fps_p, "batched_over_plain": (fps_b / fps_p) if fps_p else None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": read_traffic(dom, f"ray_{F}x{W}x{H}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
+                     "algorithmic_bytes_per_launch": alg, "stage_ms_per_launch": stage_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.rocprof_summary:
@@ -180,6 +330,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
+    if args.workload == "ray":
+        bench_ray(args)
+        return
     import torch
 
     from us3d import _lib
