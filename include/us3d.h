@@ -113,12 +113,13 @@ typedef struct us3d_ug_settings {
 
 /* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by both renderers.
  * Arrays: pair_count i32[n_pairs], pair_off i32[n_pairs+1], pair_rect i32x4[n_pairs] (tile rectangle
- * [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)), pair_dk u32[n_pairs] (depth key), tile_count
- * i32[n_tiles], tile_off i32[n_tiles+1], inst_rank i32[K], seg_key u64[K] (depth key << 32 | Gaussian),
- * seg_val i32[K] (emit index), records (per Gaussian), partial (per instance). */
+ * [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)) and pair_dk u32[n_pairs] (depth key) -- both
+ * written only where pair_count > 0 --, tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor
+ * i32[n_tiles] (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (emit index),
+ * records (per Gaussian), partial (per instance). */
 typedef struct us3d_layout {
     int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
-    int64_t inst_rank, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp, total;
+    int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp, total;
 } us3d_layout;
 
 int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
@@ -216,7 +217,8 @@ int us3d_scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total_dev
 /* Fused loss of pipeline_god_file.py:606-637 over a batch of frames: per frame
  * loss_f = w_l1 * mean|pred - gt| + w_ssim * (1 - global SSIM(pred, gt)) (single-window SSIM, :628-637); writes
  * loss[n_frames] and d_pred = d(scale * sum_f loss_f)/d pred. Frame statistics are reduced in FP64 with a fixed
- * order (bitwise reproducible). frame_pixel_base [n_frames+1]; tmp holds us3d_image_loss_tmp_bytes bytes. */
+ * order (bitwise reproducible). frame_pixel_base [n_frames+1]; tmp holds us3d_image_loss_tmp_bytes bytes;
+ * 1 <= n_frames <= 65535. */
 int64_t us3d_image_loss_tmp_bytes(int32_t n_frames);
 int us3d_image_loss(const float* pred, const float* gt, const int32_t* frame_pixel_base, int32_t n_frames,
                     float w_l1, float w_ssim, float scale, float* loss, float* d_pred, double* tmp, void* stream);

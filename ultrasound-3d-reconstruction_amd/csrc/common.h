@@ -151,10 +151,54 @@ inline unsigned blocks_for(int64_t n, int threads) {
 
 int check_plan(const us3d_plan* plan, int64_t ws_needed);
 
-// After the per-pair counts/rects/depth keys are in the workspace: scan pairs (instance count into stats,
-// overflow flag), per-tile counts and offsets, emit instances into tile buckets, sort every bucket by
-// (depth key, Gaussian). Pairs carry tile rectangles in frame tile coordinates.
+// Before the preprocess: clear the per-tile instance counters the preprocess increments.
+int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+
+// After the preprocess has written per-pair counts (and, for pairs with a nonzero count, tile rectangles and
+// depth keys) and counted every tile's instances: scan pairs (instance count into stats, overflow flag), scan
+// the tile counts into offsets, emit instances into tile buckets, sort every bucket by (depth key, Gaussian).
+// Pairs carry tile rectangles in frame tile coordinates.
 int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+
+// Wave-cooperative walk over the tile instances of the pairs held by the 64 lanes of a wavefront. Lane l holds
+// a pair with `count` instances covering `rect` = [x0,x1)x[y0,y1) of a frame whose tiles start at tile_base,
+// tiles_x per row (count = 0 for no pair), plus NP payload words. The wave's instances are numbered by an
+// inclusive scan of the counts; lanes take instances k = lane, lane + 64, ..., so one pair's tiles are spread
+// over the lanes instead of walked serially by one. For instance j (row-major in the rectangle) of the pair
+// of lane `owner`: fn(owner payload, j, global tile). Every lane of the wave must call this together.
+template <int NP, typename Fn>
+__device__ __forceinline__ void for_each_wave_instance(int count, int4 rect, int tile_base, int tiles_x,
+                                                       const uint32_t (&payload)[NP], Fn&& fn) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int incl = count;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, kWave - 1, kWave);
+    const int rw = max(rect.z - rect.x, 1);
+    for (int base = 0; base < total; base += kWave) {
+        const int k = base + lane;
+        int owner = 0;  // first lane whose inclusive count exceeds k
+#pragma unroll
+        for (int step = kWave / 2; step >= 1; step >>= 1)
+            if (__shfl(incl, owner + step - 1, kWave) <= k) owner += step;
+        const int o_incl = __shfl(incl, owner, kWave);
+        const int o_cnt = __shfl(count, owner, kWave);
+        const int o_x = __shfl(rect.x, owner, kWave), o_y = __shfl(rect.y, owner, kWave);
+        const int o_w = __shfl(rw, owner, kWave);
+        const int o_tb = __shfl(tile_base, owner, kWave), o_tx = __shfl(tiles_x, owner, kWave);
+        uint32_t o_pay[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) o_pay[i] = (uint32_t)__shfl((int)payload[i], owner, kWave);
+        if (k < total) {
+            const int j = k - (o_incl - o_cnt);
+            const int jy = j / o_w;
+            fn(o_pay, j, o_tb + (o_y + jy) * o_tx + (o_x + j - jy * o_w));
+        }
+    }
+}
 
 __global__ void init_stats_kernel(us3d_stats* stats);
 

@@ -242,7 +242,9 @@ __global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* _
     const int seg = blockIdx.x;
     const int s0 = seg_off[seg];
     const int n = seg_off[seg + 1] - s0;
-    if (threadIdx.x == 0 && n > 0) atomicMax((unsigned long long*)&stats->max_segment, (unsigned long long)n);
+    // running maximum: an atomic only when this segment beats the value already seen (one hot address)
+    if (threadIdx.x == 0 && n > 0 && (int64_t)n > *(volatile int64_t*)&stats->max_segment)
+        atomicMax((unsigned long long*)&stats->max_segment, (unsigned long long)n);
     if (n <= 1) return;
     uint64_t* k = keys + s0;
     int32_t* v = vals + s0;

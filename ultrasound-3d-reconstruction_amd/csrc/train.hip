@@ -72,15 +72,20 @@ __global__ void loss_finish_kernel(const int32_t* __restrict__ frame_base, int n
     coeff[4 * f + 3] = k * (2.0 / B);
 }
 
-__global__ void loss_grad_kernel(const float* __restrict__ pred, const float* __restrict__ gt,
-                                 const int32_t* __restrict__ frame_base, int n_frames, const double* __restrict__ coeff,
-                                 float* __restrict__ d_pred) {
-    const int64_t n = frame_base[n_frames];
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int f = upper_index(frame_base, n_frames, (int)i);
+// Pass 3: d/dx_i for every pixel; blockIdx.y is the frame, so each block reads its frame's four coefficients once.
+__global__ __launch_bounds__(kLossThreads) void loss_grad_kernel(const float* __restrict__ pred,
+                                                                 const float* __restrict__ gt,
+                                                                 const int32_t* __restrict__ frame_base,
+                                                                 const double* __restrict__ coeff,
+                                                                 float* __restrict__ d_pred) {
+    const int f = blockIdx.y;
+    const int64_t p0 = frame_base[f], p1 = frame_base[f + 1];
+    const double cs = coeff[4 * f + 0], ca = coeff[4 * f + 1], cx = coeff[4 * f + 2], cy = coeff[4 * f + 3];
+    for (int64_t i = p0 + (int64_t)blockIdx.x * kLossThreads + threadIdx.x; i < p1;
+         i += (int64_t)kLossThreads * gridDim.x) {
         const double x = pred[i], y = gt[i];
         const double sg = (x > y) ? 1.0 : ((x < y) ? -1.0 : 0.0);
-        d_pred[i] = (float)(coeff[4 * f] * sg + coeff[4 * f + 1] + coeff[4 * f + 2] * x + coeff[4 * f + 3] * y);
+        d_pred[i] = (float)(cs * sg + ca + cx * x + cy * y);
     }
 }
 
@@ -125,6 +130,7 @@ int us3d_image_loss(const float* pred, const float* gt, const int32_t* frame_pix
                     float w_l1, float w_ssim, float scale, float* loss, float* d_pred, double* tmp, void* stream_) {
     if (!pred || !gt || !frame_pixel_base || !loss || !d_pred || !tmp || n_frames <= 0)
         return fail(US3D_ERR_ARGUMENT, "image_loss: bad argument");
+    if (n_frames > 65535) return fail(US3D_ERR_ARGUMENT, "image_loss: at most 65535 frames per call");
     hipStream_t stream = as_stream(stream_);
     double* partial = tmp;
     double* coeff = tmp + (int64_t)n_frames * kLossBlocksPerFrame * 8;
@@ -140,8 +146,8 @@ int us3d_image_loss(const float* pred, const float* gt, const int32_t* frame_pix
     if ((rc = check_launch("loss_finish_kernel"))) return rc;
     {
         StageTimer timer_("loss_grad_kernel", stream);
-        hipLaunchKernelGGL(loss_grad_kernel, dim3(2048), dim3(256), 0, stream, pred, gt, frame_pixel_base, n_frames,
-                           coeff, d_pred);
+        hipLaunchKernelGGL(loss_grad_kernel, dim3(kLossBlocksPerFrame, n_frames), dim3(kLossThreads), 0, stream, pred,
+                           gt, frame_pixel_base, coeff, d_pred);
     }
     return check_launch("loss_grad_kernel");
 }

@@ -34,7 +34,7 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     L.pair_dk = take(n_pairs * 4);
     L.tile_count = take(n_tiles * 4);
     L.tile_off = take((n_tiles + 1) * 4);
-    L.inst_rank = take(k_cap * 4);
+    L.tile_cursor = take(n_tiles * 4);
     L.seg_key = take(k_cap * 8);
     L.seg_val = take(k_cap * 4);
     L.seg_key_tmp = take(k_cap * 8);
@@ -80,54 +80,66 @@ __device__ __forceinline__ uint32_t depth_key(float v_pix) {
 // Per (frame, Gaussian) pair: probe-frame projection, chi2 marginal radii, elevation cull, tile rectangle
 // and depth key (compute_probe_bbox_radii + kernel_project_gauss_and_compute_tile_rects,
 // ops/ultragauss_renderer/src/mapper/core.cu:9-116). Non-finite projections are culled.
-__global__ void ug_preprocess_kernel(int64_t n_pairs, const float4* __restrict__ rec,
-                                     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
-                                     int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
-                                     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect,
-                                     uint32_t* __restrict__ pair_dk) {
+__global__ __launch_bounds__(256) void ug_preprocess_kernel(
+    int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
+    const us3d_sweep* __restrict__ sweeps, int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
+    int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
+    int32_t* __restrict__ tile_count) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int64_t local = pair - sw.pair_base;
-    const int gl = (int)(local / sw.f_count);
-    const int g = sw.g_begin + gl;
-    const int f = sw.f_begin + (int)(local - (int64_t)gl * sw.f_count);
-    const us3d_frame& fr = frames[f];
-    const float4 r0 = rec[3 * (int64_t)g + 0];
-    const float4 r1 = rec[3 * (int64_t)g + 1];
-    const float4 r2 = rec[3 * (int64_t)g + 2];
-    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
-    const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
-    const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-    const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-    const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-    const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
-    const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-    const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+    const bool in = pair < n_pairs;
+    if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
     int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
-    uint32_t dk = 0;
-    if (fabsf(ele) <= r_ele) {
-        const float u = lat / fr.delta_lat + fr.col_offset;
-        const float v = (ax - fr.near_mm) / fr.delta_ax;
-        const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
-        const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
-        const float tsf = (float)ts;
-        const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
-        const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
-        if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
-            const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
-            const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
-            if (x0 < x1 && y0 < y1) {
-                count = (x1 - x0) * (y1 - y0);
-                rect = make_int4(x0, y0, x1, y1);
-                dk = sort_by_depth ? depth_key(v) : 0u;
+    int tile_base = 0, tiles_x = 1;
+    if (in) {
+        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+        const int64_t local = pair - sw.pair_base;
+        const int gl = (int)(local / sw.f_count);
+        const int g = sw.g_begin + gl;
+        const int f = sw.f_begin + (int)(local - (int64_t)gl * sw.f_count);
+        const us3d_frame& fr = frames[f];
+        const float4 r0 = rec[3 * (int64_t)g + 0];
+        const float4 r1 = rec[3 * (int64_t)g + 1];
+        const float4 r2 = rec[3 * (int64_t)g + 2];
+        const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+        const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
+        const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+        const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+        const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+        const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+        const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+        const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+        uint32_t dk = 0;
+        if (fabsf(ele) <= r_ele) {
+            const float u = lat / fr.delta_lat + fr.col_offset;
+            const float v = (ax - fr.near_mm) / fr.delta_ax;
+            const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+            const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+            const float tsf = (float)ts;
+            const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
+            const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
+            if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
+                const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
+                const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
+                if (x0 < x1 && y0 < y1) {
+                    count = (x1 - x0) * (y1 - y0);
+                    rect = make_int4(x0, y0, x1, y1);
+                    dk = sort_by_depth ? depth_key(v) : 0u;
+                }
             }
         }
+        pair_count[pair] = count;
+        if (count > 0) {  // rect and key are read only for pairs with instances
+            pair_rect[pair] = rect;
+            pair_dk[pair] = dk;
+            tile_base = fr.tile_base;
+            tiles_x = fr.tiles_x;
+        }
     }
-    pair_count[pair] = count;
-    pair_rect[pair] = rect;
-    pair_dk[pair] = dk;
+    const uint32_t none[1] = {0u};
+    for_each_wave_instance<1>(count, rect, tile_base, tiles_x, none, [&](const uint32_t*, int, int tile) {
+        atomicAdd(&tile_count[tile], 1);
+    });
 }
 
 __global__ void init_stats_kernel(us3d_stats* stats) {
@@ -147,53 +159,38 @@ __global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* tot
     off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
 }
 
-__device__ __forceinline__ int frame_of_pair(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int64_t pair) {
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int64_t local = pair - sw.pair_base;
-    return sw.f_begin + (int)(local % sw.f_count);
-}
-
-// Per pair: rank of each of its instances inside its tile's bucket (atomic counter per tile).
-__global__ void ug_count_kernel(int64_t n_pairs, const us3d_frame* __restrict__ frames,
-                                const us3d_sweep* __restrict__ sweeps, int n_sweeps,
-                                const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off,
-                                const int4* __restrict__ pair_rect, int32_t* __restrict__ tile_count,
-                                int32_t* __restrict__ inst_rank, const us3d_stats* stats) {
+// Per pair: write (depth key << 32 | Gaussian, emit index) into each covered tile's bucket, at the bucket's
+// offset plus a slot taken from the tile's cursor. Slot order inside a bucket is arbitrary; the bucket sort by
+// the unique keys makes the final order exact.
+__global__ __launch_bounds__(256) void ug_emit_kernel(
+    int64_t n_pairs, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+    const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+    const uint32_t* __restrict__ pair_dk, const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor,
+    uint64_t* __restrict__ seg_key, int32_t* __restrict__ seg_val, const us3d_stats* stats) {
+    if (stats->overflow) return;
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs || stats->overflow) return;
-    if (pair_count[pair] == 0) return;
-    const us3d_frame& fr = frames[frame_of_pair(sweeps, n_sweeps, pair)];
-    const int4 r = pair_rect[pair];
-    int e = pair_off[pair];
-    for (int y = r.y; y < r.w; ++y)
-        for (int x = r.x; x < r.z; ++x) inst_rank[e++] = atomicAdd(&tile_count[fr.tile_base + y * fr.tiles_x + x], 1);
-}
-
-// Per pair: write (depth key << 32 | Gaussian, emit index) into each covered tile's bucket.
-__global__ void ug_emit_kernel(int64_t n_pairs, const us3d_frame* __restrict__ frames,
-                               const us3d_sweep* __restrict__ sweeps, int n_sweeps,
-                               const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off,
-                               const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
-                               const int32_t* __restrict__ tile_off, const int32_t* __restrict__ inst_rank,
-                               uint64_t* __restrict__ seg_key, int32_t* __restrict__ seg_val, const us3d_stats* stats) {
-    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs || stats->overflow) return;
-    if (pair_count[pair] == 0) return;
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int64_t local = pair - sw.pair_base;
-    const int gl = (int)(local / sw.f_count);
-    const int g = sw.g_begin + gl;
-    const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
-    const int4 r = pair_rect[pair];
-    const uint64_t key = ((uint64_t)pair_dk[pair] << 32) | (uint32_t)g;
-    int e = pair_off[pair];
-    for (int y = r.y; y < r.w; ++y)
-        for (int x = r.x; x < r.z; ++x) {
-            const int slot = tile_off[fr.tile_base + y * fr.tiles_x + x] + inst_rank[e];
-            seg_key[slot] = key;
-            seg_val[slot] = e;
-            ++e;
-        }
+    const int count = (pair < n_pairs) ? pair_count[pair] : 0;
+    if (__ballot(count > 0) == 0) return;  // wave-uniform
+    int4 rect = make_int4(0, 0, 0, 0);
+    int tile_base = 0, tiles_x = 1;
+    uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
+    if (count > 0) {
+        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+        const int64_t local = pair - sw.pair_base;
+        const int gl = (int)(local / sw.f_count);
+        const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+        rect = pair_rect[pair];
+        tile_base = fr.tile_base;
+        tiles_x = fr.tiles_x;
+        pay[0] = (uint32_t)(sw.g_begin + gl);
+        pay[1] = pair_dk[pair];
+        pay[2] = (uint32_t)pair_off[pair];
+    }
+    for_each_wave_instance<3>(count, rect, tile_base, tiles_x, pay, [&](const uint32_t* o, int j, int tile) {
+        const int slot = tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
+        seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
+        seg_val[slot] = (int32_t)o[2] + j;
+    });
 }
 
 // ---------------------------------------------------------------- splat geometry
@@ -846,6 +843,12 @@ static T* at(const us3d_plan* plan, int64_t off) {
     return ws_at<T>(plan, off);
 }
 
+int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream,
+                       at<int32_t>(plan, L.tile_count), (int64_t)plan->n_tiles, 0);
+    return check_launch("fill_i32_kernel");
+}
+
 int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
     int rc;
     us3d_stats* stats = at<us3d_stats>(plan, L.stats);
@@ -866,28 +869,20 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     // Close pair_off with the total (offsets have n_pairs+1 entries).
     hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, pair_off, np, &stats->n_instances,
                        (const int32_t*)nullptr);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream, tile_count,
-                       (int64_t)plan->n_tiles, 0);
-    if ((rc = check_launch("fill_i32_kernel"))) return rc;
-    if (np > 0) {
-        {
-            StageTimer timer_("ug_count_kernel", stream);
-            hipLaunchKernelGGL(ug_count_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
-                               plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, tile_count,
-                               at<int32_t>(plan, L.inst_rank), (const us3d_stats*)stats);
-        }
-        if ((rc = check_launch("ug_count_kernel"))) return rc;
-    }
     int64_t* tile_total = &stats->reserved[0];
     if ((rc = scan_i32(tile_count, tile_off, plan->n_tiles, tile_total, at<void>(plan, L.scan_tmp), stream))) return rc;
     hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, tile_off, (int64_t)plan->n_tiles,
                        tile_total, (const int32_t*)&stats->overflow);
+    int32_t* tile_cursor = at<int32_t>(plan, L.tile_cursor);
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream, tile_cursor,
+                       (int64_t)plan->n_tiles, 0);
+    if ((rc = check_launch("fill_i32_kernel"))) return rc;
     if (np > 0) {
         {
             StageTimer timer_("ug_emit_kernel", stream);
             hipLaunchKernelGGL(ug_emit_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
                                plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, pair_dk, tile_off,
-                               at<int32_t>(plan, L.inst_rank), at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
+                               tile_cursor, at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
                                (const us3d_stats*)stats);
         }
         if ((rc = check_launch("ug_emit_kernel"))) return rc;
@@ -932,6 +927,7 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
 
     hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
+    if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ug_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
                            means, tril, opacity, intensity, settings->diag_beta, rec);
@@ -943,7 +939,8 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
             StageTimer timer_("ug_preprocess_kernel", stream);
             hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
                                plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
-                               settings->sort_by_depth, pair_count, pair_rect, pair_dk);
+                               settings->sort_by_depth, pair_count, pair_rect, pair_dk,
+                               at<int32_t>(plan, L.tile_count));
         }
         if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
     }

@@ -40,46 +40,59 @@ __global__ void ur_pack_kernel(int n, const float* __restrict__ means, const flo
 // scanline range [floor(c - r/ps), ceil(c + r/ps)] clipped to [0, W-1] with r = k * max(s); culled when
 // z + r < near or |elev| > r + margin (and, defined here, when the projection is not finite). Depth key =
 // (uint32)(max(0, z) * 1000) (core.cu:86), saturating.
-__global__ void ur_preprocess_kernel(int64_t n_pairs, const float4* __restrict__ rec,
-                                     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
-                                     int n_sweeps, float k_sigma, float elev_margin, int32_t* __restrict__ pair_count,
-                                     int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk) {
+__global__ __launch_bounds__(256) void ur_preprocess_kernel(
+    int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
+    const us3d_sweep* __restrict__ sweeps, int n_sweeps, float k_sigma, float elev_margin,
+    int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
+    int32_t* __restrict__ tile_count) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int64_t local = pair - sw.pair_base;
-    const int gl = (int)(local / sw.f_count);
-    const int g = sw.g_begin + gl;
-    const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
-    const float4 r0 = rec[4 * (int64_t)g + 0];
-    const float4 r1 = rec[4 * (int64_t)g + 1];
-    const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
-    const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-    const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-    const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-    const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
-    const float r = ms * k_sigma;
+    const bool in = pair < n_pairs;
+    if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
     int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
-    uint32_t dk = 0;
-    if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
-        const float c = x / fr.delta_lat + fr.col_offset;
-        const float rp = r / fr.delta_lat;
-        const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
-        if (isfinite(f0) && isfinite(f1)) {
-            const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
-            const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
-            if (lo <= hi) {
-                count = hi - lo + 1;
-                rect = make_int4(0, lo, 1, hi + 1);
-                const float q = fmaxf(0.0f, z) * 1000.0f;
-                dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
+    int tile_base = 0, tiles_x = 1;
+    if (in) {
+        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+        const int64_t local = pair - sw.pair_base;
+        const int gl = (int)(local / sw.f_count);
+        const int g = sw.g_begin + gl;
+        const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+        const float4 r0 = rec[4 * (int64_t)g + 0];
+        const float4 r1 = rec[4 * (int64_t)g + 1];
+        const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
+        const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+        const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+        const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+        const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
+        const float r = ms * k_sigma;
+        uint32_t dk = 0;
+        if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
+            const float c = x / fr.delta_lat + fr.col_offset;
+            const float rp = r / fr.delta_lat;
+            const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
+            if (isfinite(f0) && isfinite(f1)) {
+                const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
+                const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
+                if (lo <= hi) {
+                    count = hi - lo + 1;
+                    rect = make_int4(0, lo, 1, hi + 1);
+                    const float q = fmaxf(0.0f, z) * 1000.0f;
+                    dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
+                }
             }
         }
+        pair_count[pair] = count;
+        if (count > 0) {  // rect and key are read only for pairs with instances
+            pair_rect[pair] = rect;
+            pair_dk[pair] = dk;
+            tile_base = fr.tile_base;
+            tiles_x = fr.tiles_x;
+        }
     }
-    pair_count[pair] = count;
-    pair_rect[pair] = rect;
-    pair_dk[pair] = dk;
+    const uint32_t none[1] = {0u};
+    for_each_wave_instance<1>(count, rect, tile_base, tiles_x, none, [&](const uint32_t*, int, int tile) {
+        atomicAdd(&tile_count[tile], 1);
+    });
 }
 
 // ---------------------------------------------------------------- per (ray, Gaussian) setup
@@ -623,6 +636,7 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
     us3d_stats* stats = ws_at<us3d_stats>(plan, L.stats);
     float4* rec = ws_at<float4>(plan, L.records);
     hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
+    if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ur_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
                            means, scales, quats, attenuations, sh, rec);
@@ -634,7 +648,8 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
             hipLaunchKernelGGL(ur_preprocess_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
                                plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
                                settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
-                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk));
+                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk),
+                               ws_at<int32_t>(plan, L.tile_count));
         }
         if ((rc = check_launch("ur_preprocess_kernel"))) return rc;
     }
