@@ -178,13 +178,116 @@ int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* t
 }
 
 // ---------------------------------------------------------------- per-segment sort
-// One workgroup per segment. Segments of up to kSortCap keys are bitonic-sorted in LDS. Longer segments
-// are sorted in kSortCap chunks in LDS and then merged in global memory (ping-pong with the tmp arrays):
-// with unique keys, an element's merged position is its index in its run plus the number of keys of
-// the partner run that are smaller (binary search), so a merge pass is fully parallel.
+// Segments of up to kWaveSortCap keys (the common case) are sorted by one wavefront each, four per
+// workgroup: a bitonic network over registers (up to 8 keys per lane) with cross-lane shuffles, no LDS. Longer segments get one workgroup each: kSortCap-key chunks are bitonic-sorted in
+// LDS and then merged in global memory (ping-pong with the tmp arrays): with unique keys, an element's
+// merged position is its index in its run plus the number of keys of the partner run that are smaller
+// (binary search), so a merge pass is fully parallel.
 
 constexpr int kSortThreads = 256;
 constexpr int kSortCap = 2048;
+constexpr int kWaveSortCap = 512;
+constexpr int kWaveSortWaves = 4;
+
+// Bitonic sort of 64*E (key, value) pairs held by one wavefront in registers: lane l holds elements
+// l*E .. l*E+E-1. Stages whose partner index differs in the low log2(E) bits exchange inside a lane; the
+// others exchange with lane l ^ (j / E) through cross-lane shuffles. Keys must be unique.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E]) {
+    constexpr int N = kWave * E;
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int kk = 2; kk <= N; kk <<= 1) {
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1) {
+            if (j < E) {
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int p = e ^ j;
+                    if (p > e) {
+                        const bool asc = (((lane * E + e) & kk) == 0);
+                        if ((k[e] > k[p]) == asc) {
+                            const uint64_t tk = k[e]; k[e] = k[p]; k[p] = tk;
+                            const int32_t tv = v[e]; v[e] = v[p]; v[p] = tv;
+                        }
+                    }
+                }
+            } else {
+                const int m = j / E;
+#pragma unroll
+                for (int e = 0; e < E; ++e) {
+                    const int i = lane * E + e;
+                    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k[e], m, kWave);
+                    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k[e] >> 32), m, kWave);
+                    const int32_t ov = __shfl_xor(v[e], m, kWave);
+                    const uint64_t ok = ((uint64_t)hi << 32) | lo;
+                    const bool asc = (i & kk) == 0;
+                    const bool lower = (i & j) == 0;
+                    // the lower element keeps the min when ascending, the max when descending
+                    const bool take = (lower == asc) ? (ok < k[e]) : (ok > k[e]);
+                    if (take) {
+                        k[e] = ok;
+                        v[e] = ov;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void wave_sort_segment(uint64_t* __restrict__ keys, int32_t* __restrict__ vals, int n) {
+    const int lane = threadIdx.x & (kWave - 1);
+    uint64_t k[E];
+    int32_t v[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        k[e] = (i < n) ? keys[i] : ~0ull;
+        v[e] = (i < n) ? vals[i] : -1;
+    }
+    wave_bitonic_regs<E>(k, v);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const int i = lane * E + e;
+        if (i < n) {
+            keys[i] = k[e];
+            vals[i] = v[e];
+        }
+    }
+}
+
+// Longest segment into stats->max_segment: a block-wide max, then one atomic per block.
+__global__ __launch_bounds__(256) void segment_max_kernel(const int32_t* __restrict__ seg_off, int n_segments,
+                                                          us3d_stats* stats) {
+    int m = 0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n_segments; i += gridDim.x * 256)
+        m = max(m, seg_off[i + 1] - seg_off[i]);
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) m = max(m, __shfl_xor(m, off, kWave));
+    __shared__ int red[256 / kWave];
+    if ((threadIdx.x & (kWave - 1)) == 0) red[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / kWave; ++w) m = max(m, red[w]);
+        if (m > 0) atomicMax((unsigned long long*)&stats->max_segment, (unsigned long long)m);
+    }
+}
+
+__global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_small_segments_kernel(
+    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
+    const us3d_stats* __restrict__ stats) {
+    if (stats->overflow) return;
+    const int seg = blockIdx.x * kWaveSortWaves + threadIdx.x / kWave;
+    if (seg >= n_segments) return;  // wave-uniform
+    const int s0 = seg_off[seg];
+    const int n = seg_off[seg + 1] - s0;
+    if (n <= 1 || n > kWaveSortCap) return;
+    if (n <= kWave) wave_sort_segment<1>(keys + s0, vals + s0, n);
+    else if (n <= 2 * kWave) wave_sort_segment<2>(keys + s0, vals + s0, n);
+    else if (n <= 4 * kWave) wave_sort_segment<4>(keys + s0, vals + s0, n);
+    else wave_sort_segment<8>(keys + s0, vals + s0, n);
+}
 
 __device__ __forceinline__ void lds_bitonic(uint64_t* sk, int32_t* sv, int n_pad) {
     for (int k = 2; k <= n_pad; k <<= 1) {
@@ -242,10 +345,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* _
     const int seg = blockIdx.x;
     const int s0 = seg_off[seg];
     const int n = seg_off[seg + 1] - s0;
-    // running maximum: an atomic only when this segment beats the value already seen (one hot address)
-    if (threadIdx.x == 0 && n > 0 && (int64_t)n > *(volatile int64_t*)&stats->max_segment)
-        atomicMax((unsigned long long*)&stats->max_segment, (unsigned long long)n);
-    if (n <= 1) return;
+    if (n <= kWaveSortCap) return;  // sorted by sort_small_segments_kernel
     uint64_t* k = keys + s0;
     int32_t* v = vals + s0;
     if (n <= kSortCap) {
@@ -290,6 +390,16 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
                   int n_segments, us3d_stats* stats, int64_t k_capacity, hipStream_t stream) {
     (void)k_capacity;
     if (n_segments <= 0) return US3D_OK;
+    hipLaunchKernelGGL(segment_max_kernel, dim3(std::min<unsigned>(blocks_for(n_segments, 256), 1024u)), dim3(256), 0,
+                       stream, seg_off, n_segments, stats);
+    if (int rc0 = check_launch("segment_max_kernel")) return rc0;
+    {
+        StageTimer timer("sort_small_segments_kernel", stream);
+        hipLaunchKernelGGL(sort_small_segments_kernel, dim3(blocks_for(n_segments, kWaveSortWaves)),
+                           dim3(kWaveSortWaves * kWave), 0, stream, keys, vals, seg_off, n_segments, stats);
+    }
+    int rc = check_launch("sort_small_segments_kernel");
+    if (rc) return rc;
     StageTimer timer("sort_segments_kernel", stream);
     hipLaunchKernelGGL(sort_segments_kernel, dim3(n_segments), dim3(kSortThreads), 0, stream, keys, vals, keys_tmp,
                        vals_tmp, seg_off, stats);
