@@ -46,9 +46,13 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
-    ap.add_argument("--workload", choices=("slice", "ray"), default="slice",
+    ap.add_argument("--workload", choices=("slice", "ray", "multisweep"), default="slice",
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
-                         "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples")
+                         "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
+                         "multisweep: 8 sweeps of uneven size in one captured training step, tile size from the "
+                         "autotuner")
+    ap.add_argument("--ms-scale", type=float, default=1.0,
+                    help="multisweep: scale factor on the sweeps' Gaussian and frame counts")
     ap.add_argument("--ray-frames", type=int, default=128)
     ap.add_argument("--cpu-sample-rays", type=int, default=8)
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
@@ -318,6 +322,97 @@ def bench_ray(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_multisweep(args):
+    """8 sweeps (Gaussians log-uniform in 50k..500k, 32..400 frames, frame size 256x256 / 512x512 / 640x480) in
+    ONE hipGraph-captured training step; the tile size is picked by the autotuner on a frame sample first."""
+    from dataclasses import replace
+
+    import torch
+
+    from us3d import _lib
+    from us3d.autotune import tune_tile_size
+    from us3d.batch import CapacityHint, read_stats
+    from us3d.scenes import config4, config4_specs
+    from us3d.train_step import SliceSplatTrainer
+    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
+
+    dev = torch.device("cuda:0")
+    specs = config4_specs(scale=args.ms_scale)
+    scene, target, frames, sweeps = config4(scale=args.ms_scale)
+    F = len(frames)
+    G = scene.means.shape[0]
+    P = sum(f.height * f.width for f in frames)
+    f0 = frames[0]
+    cfg = UltraGaussRenderConfig(H=f0.height, W=f0.width, lateral_spacing_mm_per_pix=0.2,
+                                 axial_spacing_mm_per_pix=0.2, axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=16)
+    with torch.no_grad():
+        gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
+                           frames, cfg, sweeps=sweeps, hint=CapacityHint())[0].clone()
+    del target
+    logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
+    init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
+    tune = tune_tile_size(*init, frames, gt, cfg, sample_frames=32, reps=2, sweeps=sweeps, use_cache=False)
+    cfg = replace(cfg, tile_size=int(tune["tile_size"]))
+    print(f"[bench] autotuner: {tune}", file=sys.stderr, flush=True)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=not args.eager)
+    dt = timed(trainer.step, args.steps, args.warmup)
+    fps = F * args.steps / dt
+    print(f"[bench] multisweep batched ({'eager' if args.eager else 'graph'}): {dt / args.steps * 1e3:.1f} ms/step, "
+          f"{fps:.1f} frames/s", file=sys.stderr, flush=True)
+    del trainer
+    torch.cuda.empty_cache()
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False)
+    tt.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    tt.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    stages = _lib.stage_timing_read()
+    st = read_stats(tt.workspace)
+    off = tt.batch.layout_offsets(tt.k_capacity)
+    lay = tt.batch.layout
+    g_vis = int((tt.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
+    K = int(st.n_instances)
+    del tt
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    launch_bytes = {"ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis, "ug_forward_kernel": 36 * K + 16 * P}
+    alg = launch_bytes.get(dom)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    line = {
+        "metric": "fwd+bwd frames/s of a multi-sweep batch in one captured training step on 1 GPU; % of HBM roofline",
+        "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"slice splat fwd+bwd+Adam over {len(sweeps)} sweeps (Gaussians, frames, HxW) = "
+                               f"{specs}, {G} Gaussians and {F} frames in total, L1+0.2(1-SSIM), batched "
+                               f"{'eager' if args.eager else 'hipGraph'} step, tile {cfg.tile_size} (autotuned)",
+                   "frames": F, "gaussians": G, "pixels": P, "tile_size": cfg.tile_size, "instances": K,
+                   "visible_pairs": g_vis, "autotune_ms_per_frame": tune["ms_per_frame"]},
+        "megapixels_per_s": P * args.steps / dt / 1e6,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
+                     "stage_ms_per_launch": stage_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.rocprof_summary:
@@ -332,6 +427,9 @@ def main():
         sys.exit(2)
     if args.workload == "ray":
         bench_ray(args)
+        return
+    if args.workload == "multisweep":
+        bench_multisweep(args)
         return
     import torch
 

@@ -131,3 +131,18 @@ def test_config_validation_mirrors_reference():
         UltragrayRenderConfig(image_width=4, image_height=4, width_mm=1.0, near_mm=0.0, ops_max_offset_mm=-1.0)
     c = UltragrayRenderConfig(image_width=128, image_height=4, width_mm=102.4, near_mm=0.0)
     assert c.pixel_spacing() == float(np.float32(102.4) / np.float32(128))
+
+
+def test_autotune_sample_renumbers_sweeps_and_slices_targets():
+    from us3d.autotune import _sample
+
+    sizes = [(4, 3), (4, 3), (2, 5), (2, 5), (6, 1)]
+    frames = [_frame(H=h, W=w, sweep=s) for (h, w), s in zip(sizes, [0, 0, 1, 1, 2])]
+    n = [h * w for h, w in sizes]
+    target = torch.cat([torch.full((k,), float(i)) for i, k in enumerate(n)])
+    sweeps = [SweepSpec(0, 10), SweepSpec(10, 5), SweepSpec(15, 7)]
+    sub, sub_sweeps, tgt = _sample(frames, sweeps, target, 3)
+    picked = [0, 1, 3]  # int(i * 5 / 3) for i in 0..2
+    assert [f.sweep for f in sub] == [0, 0, 1]
+    assert sub_sweeps == [sweeps[0], sweeps[1]]
+    assert torch.equal(tgt, torch.cat([torch.full((n[i],), float(i)) for i in picked]))

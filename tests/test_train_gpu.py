@@ -98,3 +98,48 @@ def test_capacity_overflow_is_skipped_and_retried(dev):
     for x, y in zip(t.state(), ref.state()):
         assert torch.equal(x, y)
     assert int(t.step_count.item()) == 1
+
+
+def test_multi_sweep_graph_step_equals_eager(dev):
+    """A captured step over several sweeps of uneven size (frame sizes differ too) replays bit-identically to
+    the eager launch sequence."""
+    from us3d.scenes import multi_sweep
+
+    specs = [(500, 2, 64, 48), (900, 3, 48, 48)]
+    scene, target, frames, sweeps = multi_sweep(specs, cube_mm=64.0, seed=3, length_mm=30.0, spacing=0.3)
+    cfg = cfg_for(frames[0])
+    with torch.no_grad():
+        gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
+                           frames, cfg, sweeps=sweeps)[0]
+    logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
+    init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
+    a = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=True)
+    b = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False)
+    for _ in range(3):
+        a.step()
+        torch.cuda.synchronize()
+        b.step()
+        torch.cuda.synchronize()
+        for x, y in zip(a.state(), b.state()):
+            assert torch.equal(x, y)
+    assert a.losses().shape[0] == len(frames)
+
+
+def test_autotuner_picks_and_caches_a_tile_size(dev, tmp_path, monkeypatch):
+    from us3d.autotune import CANDIDATES, tune_tile_size
+    from us3d.scenes import multi_sweep
+
+    monkeypatch.setenv("US3D_CACHE_DIR", str(tmp_path))
+    specs = [(600, 3, 64, 64), (400, 2, 48, 64)]
+    scene, target, frames, sweeps = multi_sweep(specs, cube_mm=64.0, seed=5, length_mm=30.0, spacing=0.3)
+    cfg = cfg_for(frames[0])
+    with torch.no_grad():
+        gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
+                           frames, cfg, sweeps=sweeps)[0]
+    logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
+    init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
+    first = tune_tile_size(*init, frames, gt, cfg, sample_frames=4, reps=1, sweeps=sweeps)
+    assert first["tile_size"] in CANDIDATES and not first["cached"]
+    assert set(first["ms_per_frame"]) == {str(c) for c in CANDIDATES}
+    again = tune_tile_size(*init, frames, gt, cfg, sample_frames=4, reps=1, sweeps=sweeps)
+    assert again["cached"] and again["tile_size"] == first["tile_size"]

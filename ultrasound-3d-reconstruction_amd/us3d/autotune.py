@@ -15,11 +15,11 @@ import os
 import time
 from dataclasses import replace
 from pathlib import Path
-from typing import Dict, Optional, Sequence
+from typing import Dict, List, Optional, Sequence
 
 import torch
 
-from .batch import FrameSpec
+from .batch import FrameSpec, SweepSpec
 from .ultragauss import UltraGaussRenderConfig
 
 CANDIDATES = (8, 16, 32)
@@ -30,8 +30,28 @@ def _cache_path() -> Path:
 
 
 def _key(frames: Sequence[FrameSpec], n_gauss: int, semantics: str) -> str:
-    f = frames[0]
-    return f"{f.height}x{f.width}|g2^{round(math.log2(max(n_gauss, 1)))}|f2^{round(math.log2(len(frames)))}|{semantics}"
+    sizes = sorted({(f.height, f.width) for f in frames})
+    shape = "+".join(f"{h}x{w}" for h, w in sizes)
+    return f"{shape}|g2^{round(math.log2(max(n_gauss, 1)))}|f2^{round(math.log2(len(frames)))}|{semantics}"
+
+
+def _sample(frames: Sequence[FrameSpec], sweeps: Optional[Sequence[SweepSpec]], target: torch.Tensor, n: int):
+    """n frames spread over the batch, their sweeps renumbered compactly, and their target pixels."""
+    idx = sorted({int(i * len(frames) / n) for i in range(n)})
+    offs = [0]
+    for f in frames:
+        offs.append(offs[-1] + f.height * f.width)
+    flat = target.reshape(-1)
+    used: List[int] = []
+    sub: List[FrameSpec] = []
+    for i in idx:
+        s = frames[i].sweep
+        if s not in used:
+            used.append(s)
+        sub.append(replace(frames[i], sweep=used.index(s)))
+    tgt = torch.cat([flat[offs[i]:offs[i + 1]] for i in idx])
+    sub_sweeps = None if sweeps is None else [sweeps[s] for s in used]
+    return sub, sub_sweeps, tgt
 
 
 def _load() -> Dict[str, dict]:
@@ -53,8 +73,9 @@ def _store(table: Dict[str, dict]) -> None:
 
 def tune_tile_size(means, tril, raw_opacity, raw_intensity, frames: Sequence[FrameSpec], target: torch.Tensor,
                    cfg: UltraGaussRenderConfig, sample_frames: int = 16, reps: int = 2, use_cache: bool = True,
-                   candidates=CANDIDATES) -> Dict[str, object]:
-    """Returns {"tile_size": best, "ms_per_frame": {ts: ms}, "cached": bool}."""
+                   candidates=CANDIDATES, sweeps: Optional[Sequence[SweepSpec]] = None) -> Dict[str, object]:
+    """Returns {"tile_size": best, "ms_per_frame": {ts: ms}, "cached": bool}. `sweeps` as for the trainer (frames
+    grouped by sweep); the sample keeps each sampled frame's sweep."""
     from .train_step import SliceSplatTrainer
 
     key = _key(frames, means.shape[0], cfg.semantics)
@@ -64,21 +85,19 @@ def tune_tile_size(means, tril, raw_opacity, raw_intensity, frames: Sequence[Fra
         hit["cached"] = True
         return hit
     n = min(sample_frames, len(frames))
-    idx = [int(i * len(frames) / n) for i in range(n)]
-    sub = [frames[i] for i in idx]
-    P = frames[0].height * frames[0].width
-    tgt = torch.cat([target.reshape(len(frames), P)[i] for i in idx])
+    sub, sub_sweeps, tgt = _sample(frames, sweeps, target, n)
     timings: Dict[int, float] = {}
     for ts in candidates:
         c = replace(cfg, tile_size=ts)
-        tr = SliceSplatTrainer(means, tril, raw_opacity, raw_intensity, sub, tgt, c, use_graph=False, check_every=1)
+        tr = SliceSplatTrainer(means, tril, raw_opacity, raw_intensity, sub, tgt, c, sweeps=sub_sweeps,
+                               use_graph=False, check_every=1)
         tr.step()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(reps):
             tr.step()
         torch.cuda.synchronize()
-        timings[ts] = (time.perf_counter() - t0) / (reps * n) * 1e3
+        timings[ts] = (time.perf_counter() - t0) / (reps * len(sub)) * 1e3
         del tr
     best = min(timings, key=timings.get)
     out = {"tile_size": best, "ms_per_frame": {str(k): round(v, 4) for k, v in timings.items()}, "cached": False}

@@ -20,7 +20,7 @@ from typing import List
 import numpy as np
 import torch
 
-from .batch import FrameSpec
+from .batch import FrameSpec, SweepSpec
 
 
 @dataclass
@@ -155,6 +155,48 @@ def config2(device="cpu", n_frames: int = 256, n_gauss: int = 200_000):
     target = make_scene(n_gauss, 120.0, seed=1).to(device)
     frames = freehand_sweep(n_frames, 512, 512, 0.2, 120.0, 150.0, seed=2)
     return scene, target, frames
+
+
+def multi_sweep(specs, cube_mm: float = 120.0, seed: int = 0, length_mm: float = 150.0, spacing: float = 0.2):
+    """Several sweeps in one batch. specs: [(n_gauss, n_frames, H, W), ...]. Sweep s gets its own Gaussians
+    (scene seed 100 s + seed, target seed 100 s + seed + 1) and its own freehand path (seed 100 s + seed + 2).
+    Returns the concatenated scene and target, the frames (grouped by sweep, f.sweep = s) and the sweeps'
+    Gaussian row ranges."""
+    scenes, targets, frames, sweeps = [], [], [], []
+    g0 = 0
+    for s, (G, F, H, W) in enumerate(specs):
+        base = 100 * s + seed
+        scenes.append(make_scene(G, cube_mm, seed=base))
+        targets.append(make_scene(G, cube_mm, seed=base + 1))
+        for f in freehand_sweep(F, H, W, spacing, cube_mm, length_mm, seed=base + 2):
+            f.sweep = s
+            frames.append(f)
+        sweeps.append(SweepSpec(g0, G))
+        g0 += G
+
+    def cat(sc):
+        return Scene(*(torch.cat([getattr(x, k) for x in sc]) for k in
+                       ("means", "scales", "quats", "opacity", "intensity", "tril")))
+
+    return cat(scenes), cat(targets), frames, sweeps
+
+
+def config4_specs(seed: int = 4, n_sweeps: int = 8, scale: float = 1.0):
+    """Sweep sizes of the batched multi-sweep workload: Gaussians log-uniform in [50k, 500k], frames uniform in
+    32..400, frame size from {256x256, 512x512, 640x480 (W x H)}; `scale` shrinks Gaussian and frame counts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_sweeps):
+        G = int(round(math.exp(rng.uniform(math.log(50_000), math.log(500_000))) * scale))
+        F = int(rng.integers(32, 401))
+        H, W = [(256, 256), (512, 512), (480, 640)][int(rng.integers(0, 3))]
+        out.append((max(G, 1), max(int(round(F * scale)), 1), H, W))
+    return out
+
+
+def config4(seed: int = 4, scale: float = 1.0):
+    """8 sweeps of uneven size (config4_specs) with config-2 Gaussian and pose distributions."""
+    return multi_sweep(config4_specs(seed, scale=scale), seed=seed)
 
 
 # ---------------------------------------------------------------- UltraG-Ray scenes and scanline geometry
