@@ -229,3 +229,39 @@ def test_multi_sweep_batch_equals_per_sweep(dev):
         for a, b in zip(sub, params):
             assert torch.equal(a.grad, b.grad[sw.g_begin:sw.g_begin + sw.g_count]), f"sweep {s}: gradient differs"
         off += n
+
+
+@pytest.mark.parametrize("ts", [16, 8])
+def test_mapper_long_tile_lists_bitexact(dev, ts):
+    """Dense clusters put ~700, ~2500 and ~6000 Gaussians into single tiles, so every tile-sort path runs
+    (register wavefront sorts, the workgroup register sort, the chunked merge); keys, order and ranges must
+    still equal the oracle's bit for bit."""
+    from us3d.scenes import Scene, precision_tril_from_scales_quats
+
+    fr = straight_sweep(1, 128, 128, 0.3, 64.0, 10.0, 0.0, seed=1)[0]
+    rng = np.random.default_rng(8)
+    parts = []
+    for n, (u, v) in zip((700, 2500, 6000), ((20, 30), (70, 64), (100, 100))):
+        lat = (u - fr.col_offset) * fr.delta_lat + rng.normal(0.0, 0.4, n)
+        ax = fr.near_mm + v * fr.delta_ax + rng.normal(0.0, 0.4, n)
+        ele = rng.normal(0.0, 0.1, n)
+        o, a_l, a_a, a_e = (f.numpy().astype(np.float64) for f in (fr.origin, fr.axis_lat, fr.axis_ax, fr.axis_ele))
+        parts.append(o + lat[:, None] * a_l + ax[:, None] * a_a + ele[:, None] * a_e)
+    means = torch.from_numpy(np.concatenate(parts).astype(np.float32))
+    G = means.shape[0]
+    g = torch.Generator().manual_seed(3)
+    scales = 0.3 * torch.exp(0.2 * torch.randn((G, 3), generator=g))
+    quats = torch.randn((G, 4), generator=g)
+    quats = quats / quats.norm(dim=1, keepdim=True)
+    scene = Scene(means, scales, quats, torch.full((G,), 0.5), torch.full((G,), 0.5),
+                  precision_tril_from_scales_quats(scales, quats, 1e-4))
+    out = gpu_render(scene, [fr], cfg_for(fr, tile_size=ts), dev, pixels="procedural")
+    view = out["batch"].sorted_view(out["batch"].map(*[t.to(dev) for t in (
+        scene.means, scene.tril, scene.opacity, scene.intensity)]))[0]
+    _g, _p, mr, _l = oracle_frame(scene, fr, oug.Semantics(), ts=ts)
+    seg = (mr.tile_end - mr.tile_start)[mr.tile_start >= 0]
+    assert seg.max() > 4096 and ((seg > 1024) & (seg <= 4096)).any() and ((seg > 512) & (seg <= 1024)).any()
+    np.testing.assert_array_equal(view["keys"].cpu().numpy().astype(np.uint64), mr.keys)
+    np.testing.assert_array_equal(view["values"].cpu().numpy(), mr.vals)
+    np.testing.assert_array_equal(view["tile_start"].cpu().numpy(), mr.tile_start)
+    np.testing.assert_array_equal(view["tile_end"].cpu().numpy(), mr.tile_end)

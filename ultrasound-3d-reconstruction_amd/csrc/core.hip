@@ -178,24 +178,27 @@ int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* t
 }
 
 // ---------------------------------------------------------------- per-segment sort
-// Segments of up to kWaveSortCap keys (the common case) are sorted by one wavefront each, four per
-// workgroup: a bitonic network over registers (up to 8 keys per lane) with cross-lane shuffles, no LDS. Longer segments get one workgroup each: kSortCap-key chunks are bitonic-sorted in
-// LDS and then merged in global memory (ping-pong with the tmp arrays): with unique keys, an element's
-// merged position is its index in its run plus the number of keys of the partner run that are smaller
-// (binary search), so a merge pass is fully parallel.
+// Segments of up to kWaveSortCapMid keys are sorted by one wavefront each, four per workgroup: a bitonic
+// network over registers (up to 16 keys per lane) with cross-lane shuffles, no LDS. Longer segments get
+// one workgroup each: kSortCap-key chunks are bitonic-sorted in LDS and then merged in global memory
+// (ping-pong with the tmp arrays): with unique keys, an element's merged position is its index in its run
+// plus the number of keys of the partner run that are smaller (binary search), so a merge pass is fully
+// parallel.
 
 constexpr int kSortThreads = 256;
 constexpr int kSortCap = 2048;
-constexpr int kWaveSortCap = 512;
+constexpr int kWaveSortCap = 512;     // segments sorted with <= 8 keys per lane
+constexpr int kWaveSortCapMid = 1024; // segments sorted with 16 keys per lane (a kernel of their own)
 constexpr int kWaveSortWaves = 4;
 
 // Bitonic sort of 64*E (key, value) pairs held by one wavefront in registers: lane l holds elements
-// l*E .. l*E+E-1. Stages whose partner index differs in the low log2(E) bits exchange inside a lane; the
-// others exchange with lane l ^ (j / E) through cross-lane shuffles. Keys must be unique.
+// l*E .. l*E+E-1. The network is unrolled completely (the partner distance and direction of every exchange
+// are compile-time constants, so shuffles and register indices are static). Stages whose partner index
+// differs only in the low log2(E) bits exchange inside a lane; the others exchange with lane ^ (j / E)
+// through cross-lane shuffles. Keys must be unique.
 template <int E>
-__device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E]) {
+__device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], int t) {
     constexpr int N = kWave * E;
-    const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
     for (int kk = 2; kk <= N; kk <<= 1) {
 #pragma unroll
@@ -205,7 +208,7 @@ __device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)
                 for (int e = 0; e < E; ++e) {
                     const int p = e ^ j;
                     if (p > e) {
-                        const bool asc = (((lane * E + e) & kk) == 0);
+                        const bool asc = (((t * E + e) & kk) == 0);
                         if ((k[e] > k[p]) == asc) {
                             const uint64_t tk = k[e]; k[e] = k[p]; k[p] = tk;
                             const int32_t tv = v[e]; v[e] = v[p]; v[p] = tv;
@@ -216,7 +219,7 @@ __device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)
                 const int m = j / E;
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
-                    const int i = lane * E + e;
+                    const int i = t * E + e;
                     const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k[e], m, kWave);
                     const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k[e] >> 32), m, kWave);
                     const int32_t ov = __shfl_xor(v[e], m, kWave);
@@ -233,6 +236,11 @@ __device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)
             }
         }
     }
+}
+
+template <int E>
+__device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E]) {
+    bitonic_regs<E>(k, v, (int)(threadIdx.x & (kWave - 1)));
 }
 
 template <int E>
@@ -287,6 +295,20 @@ __global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_small_segments_ke
     else if (n <= 2 * kWave) wave_sort_segment<2>(keys + s0, vals + s0, n);
     else if (n <= 4 * kWave) wave_sort_segment<4>(keys + s0, vals + s0, n);
     else wave_sort_segment<8>(keys + s0, vals + s0, n);
+}
+
+// Segments of kWaveSortCap+1 .. kWaveSortCapMid keys: the same register bitonic with 16 keys per lane
+// (kept out of the small-segment kernel so its register budget stays small).
+__global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_mid_segments_kernel(
+    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
+    const us3d_stats* __restrict__ stats) {
+    if (stats->overflow) return;
+    const int seg = blockIdx.x * kWaveSortWaves + threadIdx.x / kWave;
+    if (seg >= n_segments) return;  // wave-uniform
+    const int s0 = seg_off[seg];
+    const int n = seg_off[seg + 1] - s0;
+    if (n <= kWaveSortCap || n > kWaveSortCapMid) return;
+    wave_sort_segment<16>(keys + s0, vals + s0, n);
 }
 
 __device__ __forceinline__ void lds_bitonic(uint64_t* sk, int32_t* sv, int n_pad) {
@@ -345,7 +367,7 @@ __global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* _
     const int seg = blockIdx.x;
     const int s0 = seg_off[seg];
     const int n = seg_off[seg + 1] - s0;
-    if (n <= kWaveSortCap) return;  // sorted by sort_small_segments_kernel
+    if (n <= kWaveSortCapMid) return;  // sorted by sort_small_segments_kernel / sort_mid_segments_kernel
     uint64_t* k = keys + s0;
     int32_t* v = vals + s0;
     if (n <= kSortCap) {
@@ -400,6 +422,12 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
     }
     int rc = check_launch("sort_small_segments_kernel");
     if (rc) return rc;
+    {
+        StageTimer timer("sort_mid_segments_kernel", stream);
+        hipLaunchKernelGGL(sort_mid_segments_kernel, dim3(blocks_for(n_segments, kWaveSortWaves)),
+                           dim3(kWaveSortWaves * kWave), 0, stream, keys, vals, seg_off, n_segments, stats);
+    }
+    if ((rc = check_launch("sort_mid_segments_kernel"))) return rc;
     StageTimer timer("sort_segments_kernel", stream);
     hipLaunchKernelGGL(sort_segments_kernel, dim3(n_segments), dim3(kSortThreads), 0, stream, keys, vals, keys_tmp,
                        vals_tmp, seg_off, stats);
