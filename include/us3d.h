@@ -81,7 +81,7 @@ typedef struct us3d_stats {
 
 /* A batch: frames (possibly from several sweeps) that are mapped, rendered and differentiated together.
  * Host struct holding device pointers. n_tiles = sum of frames' tiles_x*tiles_y (UltraGauss) or of their
- * widths (UltraG-Ray); n_pairs = sum over sweeps of g_count*f_count. */
+ * widths (UltraG-Ray); n_pairs = sum over sweeps of g_count*f_count. n_pairs and n_pixels must be < 2^31. */
 typedef struct us3d_plan {
     int32_t n_gauss;                /* rows of the concatenated Gaussian arrays */
     int32_t n_frames;

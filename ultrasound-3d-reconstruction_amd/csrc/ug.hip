@@ -52,16 +52,27 @@ static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, 
 
 // ---------------------------------------------------------------- mapper kernels
 
+// Packs one record per Gaussian. The spare last word holds r_max = sqrt(chi2) ||M^-1||_F * 1.01 (M = the upper
+// triangular factor with q = |M d|^2): aT Sigma a = |M^-T a|^2 <= ||M^-1||_F^2 for any unit a, so r_max bounds
+// every marginal radius and a pair with |elevation| > r_max is culled without computing them (NaN for a
+// degenerate factor, which then takes the exact path).
 __global__ void ug_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ tril,
                                const float* __restrict__ opacity, const float* __restrict__ intensity, float beta,
-                               float4* __restrict__ rec) {
+                               float sqrt_chi2, float4* __restrict__ rec) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float* t = tril + (int64_t)g * 6;
     const float* m = means + (int64_t)g * 3;
+    const float l11 = __builtin_fmaf(t[0], t[0], beta), l12 = t[1], l13 = t[2];
+    const float l22 = __builtin_fmaf(t[3], t[3], beta), l23 = t[4], l33 = __builtin_fmaf(t[5], t[5], beta);
+    const double i11 = 1.0 / l11, i22 = 1.0 / l22, i33 = 1.0 / l33;
+    const double i12 = -(double)l12 * i11 * i22, i23 = -(double)l23 * i22 * i33;
+    const double i13 = ((double)l12 * l23 - (double)l13 * l22) * i11 * i22 * i33;
+    const double fro2 = i11 * i11 + i22 * i22 + i33 * i33 + i12 * i12 + i23 * i23 + i13 * i13;
+    const float r_max = (float)((double)sqrt_chi2 * sqrt(fro2) * 1.01);
     rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], opacity[g]);
-    rec[3 * (int64_t)g + 1] = make_float4(intensity[g], __builtin_fmaf(t[0], t[0], beta), t[1], t[2]);
-    rec[3 * (int64_t)g + 2] = make_float4(__builtin_fmaf(t[3], t[3], beta), t[4], __builtin_fmaf(t[5], t[5], beta), 0.f);
+    rec[3 * (int64_t)g + 1] = make_float4(intensity[g], l11, l12, l13);
+    rec[3 * (int64_t)g + 2] = make_float4(l22, l23, l33, r_max);
 }
 
 __device__ __forceinline__ int clamp_to_int(float x) {
@@ -93,10 +104,10 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
     int tile_base = 0, tiles_x = 1;
     if (in) {
         const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int64_t local = pair - sw.pair_base;
-        const int gl = (int)(local / sw.f_count);
+        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+        const int gl = local / sw.f_count;
         const int g = sw.g_begin + gl;
-        const int f = sw.f_begin + (int)(local - (int64_t)gl * sw.f_count);
+        const int f = sw.f_begin + (local - gl * sw.f_count);
         const us3d_frame& fr = frames[f];
         const float4 r0 = rec[3 * (int64_t)g + 0];
         const float4 r1 = rec[3 * (int64_t)g + 1];
@@ -106,25 +117,28 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
         const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
         const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
         const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
-        const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-        const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
         uint32_t dk = 0;
-        if (fabsf(ele) <= r_ele) {
-            const float u = lat / fr.delta_lat + fr.col_offset;
-            const float v = (ax - fr.near_mm) / fr.delta_ax;
-            const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
-            const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
-            const float tsf = (float)ts;
-            const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
-            const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
-            if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
-                const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
-                const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
-                if (x0 < x1 && y0 < y1) {
-                    count = (x1 - x0) * (y1 - y0);
-                    rect = make_int4(x0, y0, x1, y1);
-                    dk = sort_by_depth ? depth_key(v) : 0u;
+        const bool far = fabsf(ele) > r2.w;  // beyond every marginal radius (see ug_pack_kernel)
+        if (!far) {
+            const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+            const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+            const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+            if (fabsf(ele) <= r_ele) {
+                const float u = lat / fr.delta_lat + fr.col_offset;
+                const float v = (ax - fr.near_mm) / fr.delta_ax;
+                const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+                const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+                const float tsf = (float)ts;
+                const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
+                const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
+                if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
+                    const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
+                    const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
+                    if (x0 < x1 && y0 < y1) {
+                        count = (x1 - x0) * (y1 - y0);
+                        rect = make_int4(x0, y0, x1, y1);
+                        dk = sort_by_depth ? depth_key(v) : 0u;
+                    }
                 }
             }
         }
@@ -176,9 +190,9 @@ __global__ __launch_bounds__(256) void ug_emit_kernel(
     uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
     if (count > 0) {
         const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int64_t local = pair - sw.pair_base;
-        const int gl = (int)(local / sw.f_count);
-        const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+        const int gl = local / sw.f_count;
+        const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
         rect = pair_rect[pair];
         tile_base = fr.tile_base;
         tiles_x = fr.tiles_x;
@@ -820,6 +834,8 @@ int check_plan(const us3d_plan* plan, int64_t ws_needed) {
     if (!plan) return fail(US3D_ERR_ARGUMENT, "plan is NULL");
     if (plan->n_gauss < 0 || plan->n_frames <= 0 || plan->n_sweeps <= 0 || plan->n_tiles <= 0 || plan->n_pairs < 0)
         return fail(US3D_ERR_ARGUMENT, "plan sizes out of range");
+    if (plan->n_pairs >= (int64_t(1) << 31) || plan->n_pixels >= (int64_t(1) << 31))
+        return fail(US3D_ERR_ARGUMENT, "n_pairs and n_pixels must be < 2^31 (split the batch)");
     if (plan->k_capacity < 1 || plan->k_capacity >= (int64_t(1) << 31))
         return fail(US3D_ERR_ARGUMENT, "k_capacity must be in [1, 2^31)");
     if (!plan->workspace || plan->workspace_bytes < ws_needed)
@@ -930,7 +946,7 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ug_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
-                           means, tril, opacity, intensity, settings->diag_beta, rec);
+                           means, tril, opacity, intensity, settings->diag_beta, sqrt_chi2, rec);
         if ((rc = check_launch("ug_pack_kernel"))) return rc;
     }
     const int64_t np = plan->n_pairs;

@@ -53,10 +53,10 @@ __global__ __launch_bounds__(256) void ur_preprocess_kernel(
     int tile_base = 0, tiles_x = 1;
     if (in) {
         const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int64_t local = pair - sw.pair_base;
-        const int gl = (int)(local / sw.f_count);
+        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+        const int gl = local / sw.f_count;
         const int g = sw.g_begin + gl;
-        const us3d_frame& fr = frames[sw.f_begin + (int)(local - (int64_t)gl * sw.f_count)];
+        const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
         const float4 r0 = rec[4 * (int64_t)g + 0];
         const float4 r1 = rec[4 * (int64_t)g + 1];
         const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
