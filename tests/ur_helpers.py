@@ -32,8 +32,10 @@ def ray_spec(fr: FrameSpec, cfg: UltragrayRenderConfig) -> FrameSpec:
                      fr.near_mm, fr.col_offset, fr.sweep)
 
 
-def oracle_frame(fr: FrameSpec, cfg: UltragrayRenderConfig, delta=None):
+def oracle_frame(fr: FrameSpec, cfg: UltragrayRenderConfig, delta=None, pixels=None):
     pix, ro, rd = ray_geometry(fr)
+    if pixels is not None:
+        pix = pixels
     n = lambda t: t.detach().cpu().numpy().astype(np.float32)  # noqa: E731
     W = fr.width
     return our.RayFrame(origin=n(fr.origin), axis_lat=n(fr.axis_lat), axis_ax=n(fr.axis_ax), axis_ele=n(fr.axis_ele),
@@ -48,10 +50,14 @@ def ray_settings(cfg: UltragrayRenderConfig) -> our.RaySettings:
                            local_max=cfg.local_transmission_max_window_mm, grad_threshold=cfg.grad_threshold)
 
 
-def gpu_ray_render(scene, frames, cfg, dev, deltas=None, d_image=None, hint=None):
+def gpu_ray_render(scene, frames, cfg, dev, deltas=None, d_image=None, hint=None, pixels=None):
+    """pixels: optional list of per-frame (W, H, 3) sample positions replacing the linear grid."""
     params = [t.to(dev).clone().requires_grad_(True) for t in scene.params()]
     geos = [ray_geometry(f) for f in frames]
-    pix = torch.cat([g[0].reshape(-1, 3) for g in geos]).to(dev)
+    if pixels is None:
+        pix = torch.cat([g[0].reshape(-1, 3) for g in geos]).to(dev)
+    else:
+        pix = torch.cat([p.reshape(-1, 3) for p in pixels]).to(dev)
     ro = torch.cat([g[1] for g in geos]).to(dev)
     rd = torch.cat([g[2] for g in geos]).to(dev)
     W = sum(f.width for f in frames)

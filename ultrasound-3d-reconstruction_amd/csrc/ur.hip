@@ -363,8 +363,35 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
 //   echo: dL/dw = dNum I + dDen + dS; d mu, d Sigma^-1 accumulated as sums over samples;
 // then mapped once per (ray, Gaussian) to (mu, s, q, raw tau, sh) and written at the instance's emit index.
 // Samples with |dL/dB| < grad_threshold contribute nothing (backward.cu:170).
+// With the transmission window on and sample depths nondecreasing along the ray (checked per chunk), the
+// samples where a hit is in front and its window finished form a suffix of the chunk; their terms are
+// cT / f_final times constants, so they enter through one FP64 suffix sum of cT, and each thread walks only
+// the samples of its echo range and open window (located by binary search with the same comparisons as the
+// per-sample rules). Otherwise every thread walks the whole chunk.
 
 constexpr int kRayChunk = 1024;
+constexpr int kRayChunkPerThread = kRayChunk / kRayThreads;
+
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = min(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+// First index in [lo, hi) whose element satisfies pred (pred false then true along the range), else hi.
+template <typename Pred>
+__device__ __forceinline__ int first_true(int lo, int hi, Pred pred) {
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (pred(m)) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
 
 __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
@@ -375,7 +402,10 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     float4* __restrict__ partial) {
     __shared__ float4 s_p[kRayChunk];  // px, py, pz, tpx
     __shared__ float4 s_c[kRayChunk];  // cT, dNum, dS, E
+    __shared__ double s_S[kRayChunk + 1];  // s_S[j] = sum of cT over samples j.. of the chunk
+    __shared__ double s_wsum[kRayThreads / kWave];
     if (stats->overflow) return;
+    const bool window_mode = F.Rw > 0.0f && !F.local_only;
     const int t = blockIdx.x;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     if (s0 == s1) return;
@@ -424,11 +454,66 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                 s_c[j] = make_float4(cT, dNum, dS, E);
             }
             __syncthreads();
-            if (!active) continue;
-            for (int j = 0; j < n; ++j) {
+            // monotone sample depths, and the suffix sums of cT (thread t owns samples 4t .. 4t+3)
+            bool mono = true;
+            double loc[kRayChunkPerThread];
+            double tot = 0.0;
+#pragma unroll
+            for (int q = kRayChunkPerThread - 1; q >= 0; --q) {
+                const int j = tid * kRayChunkPerThread + q;
+                if (j < n) {
+                    tot += (double)s_c[j].x;
+                    if (j + 1 < n) mono = mono && (s_p[j].w <= s_p[j + 1].w);
+                }
+                loc[q] = tot;
+            }
+            double suf = tot;
+#pragma unroll
+            for (int off = 1; off < kWave; off <<= 1) {
+                const double o = __shfl_down(suf, off, kWave);
+                if ((tid & (kWave - 1)) + off < kWave) suf += o;
+            }
+            if ((tid & (kWave - 1)) == 0) s_wsum[tid / kWave] = suf;
+            mono = __syncthreads_and(mono) != 0;
+            double after = suf - tot;
+            for (int w = tid / kWave + 1; w < kRayThreads / kWave; ++w) after += s_wsum[w];
+#pragma unroll
+            for (int q = 0; q < kRayChunkPerThread; ++q) {
+                const int j = tid * kRayChunkPerThread + q;
+                if (j < n) s_S[j] = loc[q] + after;
+            }
+            if (tid == 0) s_S[n] = 0.0;
+            __syncthreads();
+            const bool fast = window_mode && mono;
+            // this thread's sample range [lo, hi) and the first finished sample jf (suffix handled by s_S)
+            int lo = n, hi = 0, jf = n;
+            if (active) {
+                if (fast) {
+                    const int jt = first_true(0, n, [&](int m) { return G.tg < s_p[m].w; });
+                    const int jb = first_true(0, n, [&](int m) { return s_p[m].w >= G.b; });
+                    jf = max(jt, jb);
+                    const int je0 = first_true(0, n, [&](int m) { return (G.tg - s_p[m].w) <= G.rd; });
+                    const int je1 = first_true(je0, n, [&](int m) { return (G.tg - s_p[m].w) < -G.rd; });
+                    lo = min(jt, je0);
+                    hi = max(jf, je1);
+                    const double S = s_S[jf];
+                    if (S != 0.0 && G.f_final > 1e-6f) {
+                        const float dTs = (float)(S / (double)G.f_final);
+                        dtau = __builtin_fmaf(dTs, 1.0f - G.e_final, dtau);
+                        fin -= dTs * (1.0f - G.tau) * G.e_final;
+                    }
+                } else {
+                    lo = 0;
+                    hi = n;
+                }
+            }
+            const int wlo = wave_min_i(lo), whi = wave_max_i(hi);
+            for (int j = wlo; j < whi; ++j) {
+                if (j < lo || j >= hi) continue;
                 const float4 pp = s_p[j];
                 bool trans, echo;
                 ray_terms(G, pp.w, F, trans, echo);
+                if (j >= jf) trans = false;  // in the finished suffix: counted through s_S
                 if (!trans && !echo) continue;
                 const float4 cc = s_c[j];
                 if (trans) {
