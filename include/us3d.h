@@ -116,10 +116,12 @@ typedef struct us3d_ug_settings {
  * [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)) and pair_dk u32[n_pairs] (depth key) -- both
  * written only where pair_count > 0 --, tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor
  * i32[n_tiles] (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (emit index),
- * records (per Gaussian), partial (per instance). */
+ * records (per Gaussian), partial (per instance), vis_list i32[n_pairs] (frame-major copy of the pair
+ * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1). */
 typedef struct us3d_layout {
     int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
-    int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp, total;
+    int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
+    int64_t vis_list, total;
 } us3d_layout;
 
 int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);

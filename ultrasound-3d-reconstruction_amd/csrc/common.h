@@ -151,11 +151,16 @@ inline unsigned blocks_for(int64_t n, int threads) {
 
 int check_plan(const us3d_plan* plan, int64_t ws_needed);
 
-// Before the preprocess: clear the per-tile instance counters the preprocess increments.
+// XCDs of the chip (workgroups are dispatched to them round-robin) and workgroups per frame of the emit.
+constexpr int kXcds = 8;
+constexpr int kEmitGroups = 32;
+
+// Before the preprocess: clear the per-tile instance counters the preprocess increments and mark every
+// frame-major visible-pair entry empty (-1).
 int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
 
-// After the preprocess has written per-pair counts (and, for pairs with a nonzero count, tile rectangles and
-// depth keys) and counted every tile's instances: scan pairs (instance count into stats, overflow flag), scan
+// After the preprocess has written per-pair counts (and, for pairs with a nonzero count, tile rectangles,
+// depth keys and their entry in the frame-major visible-pair list) and counted every tile's instances: scan pairs (instance count into stats, overflow flag), scan
 // the tile counts into offsets, emit instances into tile buckets, sort every bucket by (depth key, Gaussian).
 // Pairs carry tile rectangles in frame tile coordinates.
 int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);

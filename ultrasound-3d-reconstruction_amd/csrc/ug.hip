@@ -42,6 +42,7 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     L.records = take(n_gauss * record_bytes);
     L.partial = take(k_cap * partial_bytes);
     L.scan_tmp = take(std::max(scan_tmp_bytes(n_pairs + 1), scan_tmp_bytes(n_tiles + 1)));
+    L.vis_list = take(n_pairs * 4);
     L.total = off;
     return L;
 }
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
     int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
     const us3d_sweep* __restrict__ sweeps, int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
-    int32_t* __restrict__ tile_count) {
+    int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = pair < n_pairs;
     if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
@@ -148,6 +149,8 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
             pair_dk[pair] = dk;
             tile_base = fr.tile_base;
             tiles_x = fr.tiles_x;
+            // frame-major copy of the visible pairs: entry gl of the frame's g_count-sized region
+            vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
         }
     }
     const uint32_t none[1] = {0u};
@@ -173,38 +176,75 @@ __global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* tot
     off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
 }
 
-// Per pair: write (depth key << 32 | Gaussian, emit index) into each covered tile's bucket, at the bucket's
-// offset plus a slot taken from the tile's cursor. Slot order inside a bucket is arbitrary; the bucket sort by
-// the unique keys makes the final order exact.
+// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket, at the bucket's
+// offset plus a slot taken from the tile's cursor. Slot order inside a bucket is arbitrary; the bucket sort
+// by the unique keys makes the final order exact. Frame-major: workgroup w serves frame f = 8 (w / 8 /
+// kEmitGroups) + w % 8 and walks that frame's region of vis_list (one entry per Gaussian of its sweep, -1
+// where the pair is culled) in strides of kEmitGroups groups of 256. Workgroups are dispatched to the 8 XCDs
+// round-robin, so every bucket of a frame is written through one XCD's L2, where the scattered 8 B / 4 B
+// stores of a bucket merge into full lines before they are written back.
 __global__ __launch_bounds__(256) void ug_emit_kernel(
-    int64_t n_pairs, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, int n_sweeps,
-    const int32_t* __restrict__ pair_count, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
-    const uint32_t* __restrict__ pair_dk, const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor,
-    uint64_t* __restrict__ seg_key, int32_t* __restrict__ seg_val, const us3d_stats* stats) {
+    int n_frames, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const int32_t* __restrict__ vis_list, const int32_t* __restrict__ pair_count,
+    const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
+    const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor, uint64_t* __restrict__ seg_key,
+    int32_t* __restrict__ seg_val, const us3d_stats* stats) {
     if (stats->overflow) return;
-    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const int count = (pair < n_pairs) ? pair_count[pair] : 0;
-    if (__ballot(count > 0) == 0) return;  // wave-uniform
-    int4 rect = make_int4(0, 0, 0, 0);
-    int tile_base = 0, tiles_x = 1;
-    uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
-    if (count > 0) {
-        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
-        const int gl = local / sw.f_count;
-        const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
-        rect = pair_rect[pair];
-        tile_base = fr.tile_base;
-        tiles_x = fr.tiles_x;
-        pay[0] = (uint32_t)(sw.g_begin + gl);
-        pay[1] = pair_dk[pair];
-        pay[2] = (uint32_t)pair_off[pair];
+    const int w = blockIdx.x;
+    const int xcd = w % kXcds, j = w / kXcds;
+    const int f = (j / kEmitGroups) * kXcds + xcd;
+    if (f >= n_frames) return;
+    const us3d_frame& fr = frames[f];
+    const us3d_sweep sw = sweeps[fr.sweep];
+    const int32_t* list = vis_list + sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count;
+    // visible pairs are gathered into an LDS queue (ballot compaction, no global atomics) and expanded 256 at a
+    // time, so the lanes of the instance walk stay busy although most entries of the region are empty
+    __shared__ int32_t s_q[512];
+    __shared__ int s_n;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    for (int grp = j % kEmitGroups;; grp += kEmitGroups) {
+        const bool more = grp * 256 < sw.g_count;  // workgroup-uniform
+        if (more) {
+            const int idx = grp * 256 + (int)threadIdx.x;
+            const int pair = idx < sw.g_count ? list[idx] : -1;
+            const uint64_t bal = __ballot(pair >= 0);
+            int wbase = 0;
+            if (lane == 0 && bal) wbase = atomicAdd(&s_n, (int)__popcll(bal));
+            wbase = __shfl(wbase, 0, kWave);
+            if (pair >= 0) s_q[wbase + (int)__popcll(bal & ((1ull << lane) - 1ull))] = pair;
+        }
+        __syncthreads();
+        const int n = s_n;
+        if (n >= 256 || (!more && n > 0)) {
+            const int take = min(n, 256);
+            const int pair = (int)threadIdx.x < take ? s_q[threadIdx.x] : -1;
+            int count = 0;
+            int4 rect = make_int4(0, 0, 0, 0);
+            uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
+            if (pair >= 0) {
+                count = pair_count[pair];
+                rect = pair_rect[pair];
+                pay[0] = (uint32_t)(sw.g_begin + (int)(pair - sw.pair_base) / sw.f_count);
+                pay[1] = pair_dk[pair];
+                pay[2] = (uint32_t)pair_off[pair];
+            }
+            for_each_wave_instance<3>(count, rect, fr.tile_base, fr.tiles_x, pay,
+                                      [&](const uint32_t* o, int jj, int tile) {
+                                          const int slot = tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
+                                          seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
+                                          seg_val[slot] = (int32_t)o[2] + jj;
+                                      });
+            const int rest = n - take;
+            const int moved = (int)threadIdx.x < rest ? s_q[take + threadIdx.x] : 0;
+            __syncthreads();  // every read of the queue is done
+            if ((int)threadIdx.x < rest) s_q[threadIdx.x] = moved;
+            if (threadIdx.x == 0) s_n = rest;
+            __syncthreads();
+        }
+        if (!more) break;
     }
-    for_each_wave_instance<3>(count, rect, tile_base, tiles_x, pay, [&](const uint32_t* o, int j, int tile) {
-        const int slot = tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
-        seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
-        seg_val[slot] = (int32_t)o[2] + j;
-    });
 }
 
 // ---------------------------------------------------------------- splat geometry
@@ -862,6 +902,9 @@ static T* at(const us3d_plan* plan, int64_t off) {
 int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream,
                        at<int32_t>(plan, L.tile_count), (int64_t)plan->n_tiles, 0);
+    if (plan->n_pairs > 0)
+        hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
+                           at<int32_t>(plan, L.vis_list), plan->n_pairs, -1);
     return check_launch("fill_i32_kernel");
 }
 
@@ -896,10 +939,11 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     if (np > 0) {
         {
             StageTimer timer_("ug_emit_kernel", stream);
-            hipLaunchKernelGGL(ug_emit_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->frames,
-                               plan->sweeps, plan->n_sweeps, pair_count, pair_off, pair_rect, pair_dk, tile_off,
-                               tile_cursor, at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val),
-                               (const us3d_stats*)stats);
+            const unsigned grid = (unsigned)(((plan->n_frames + kXcds - 1) / kXcds) * kXcds * kEmitGroups);
+            hipLaunchKernelGGL(ug_emit_kernel, dim3(grid), dim3(256), 0, stream, plan->n_frames, plan->frames,
+                               plan->sweeps, at<int32_t>(plan, L.vis_list), pair_count, pair_off, pair_rect, pair_dk,
+                               tile_off, tile_cursor,
+                               at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), (const us3d_stats*)stats);
         }
         if ((rc = check_launch("ug_emit_kernel"))) return rc;
     }
@@ -956,7 +1000,7 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
             hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
                                plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
                                settings->sort_by_depth, pair_count, pair_rect, pair_dk,
-                               at<int32_t>(plan, L.tile_count));
+                               at<int32_t>(plan, L.tile_count), at<int32_t>(plan, L.vis_list));
         }
         if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
     }

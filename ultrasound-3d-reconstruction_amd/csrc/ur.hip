@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void ur_preprocess_kernel(
     int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
     const us3d_sweep* __restrict__ sweeps, int n_sweeps, float k_sigma, float elev_margin,
     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
-    int32_t* __restrict__ tile_count) {
+    int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = pair < n_pairs;
     if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
@@ -56,7 +56,8 @@ __global__ __launch_bounds__(256) void ur_preprocess_kernel(
         const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
         const int gl = local / sw.f_count;
         const int g = sw.g_begin + gl;
-        const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
+        const int f = sw.f_begin + (local - gl * sw.f_count);
+        const us3d_frame& fr = frames[f];
         const float4 r0 = rec[4 * (int64_t)g + 0];
         const float4 r1 = rec[4 * (int64_t)g + 1];
         const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(256) void ur_preprocess_kernel(
             pair_dk[pair] = dk;
             tile_base = fr.tile_base;
             tiles_x = fr.tiles_x;
+            vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
         }
     }
     const uint32_t none[1] = {0u};
@@ -734,7 +736,7 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
                                plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
                                settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
                                ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk),
-                               ws_at<int32_t>(plan, L.tile_count));
+                               ws_at<int32_t>(plan, L.tile_count), ws_at<int32_t>(plan, L.vis_list));
         }
         if ((rc = check_launch("ur_preprocess_kernel"))) return rc;
     }
