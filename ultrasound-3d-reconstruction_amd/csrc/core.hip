@@ -196,7 +196,23 @@ constexpr int kWaveSortWaves = 4;
 // are compile-time constants, so shuffles and register indices are static). Stages whose partner index
 // differs only in the low log2(E) bits exchange inside a lane; the others exchange with lane ^ (j / E)
 // through cross-lane shuffles. Keys must be unique.
-template <int E>
+// v of lane ^ m (m a power of two < 64, a constant once the caller is unrolled): DPP quad_perm for m = 1, 2,
+// DPP row_ror:8 for m = 8 (rotation by 8 inside a 16-lane row is xor 8), ds_swizzle in bitmask mode for m = 4,
+// 16 (source lane ((lane & 31) ^ m) inside each 32-lane half), ds_bpermute for m = 32.
+__device__ __forceinline__ int shfl_xor_pow2(int v, int m) {
+    switch (m) {
+        case 1: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+        case 2: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+        case 4: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (4 << 10));
+        case 8: return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (16 << 10));
+        default: return __shfl_xor(v, m, kWave);
+    }
+}
+
+// kDpp: shuffles through shfl_xor_pow2 (only where the network unrolls completely, so every distance is a
+// constant); otherwise ds_bpermute.
+template <int E, bool kDpp>
 __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], int t) {
     constexpr int N = kWave * E;
 #pragma unroll
@@ -220,9 +236,10 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], 
 #pragma unroll
                 for (int e = 0; e < E; ++e) {
                     const int i = t * E + e;
-                    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)k[e], m, kWave);
-                    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(k[e] >> 32), m, kWave);
-                    const int32_t ov = __shfl_xor(v[e], m, kWave);
+                    const int klo = (int)(uint32_t)k[e], khi = (int)(uint32_t)(k[e] >> 32);
+                    const uint32_t lo = (uint32_t)(kDpp ? shfl_xor_pow2(klo, m) : __shfl_xor(klo, m, kWave));
+                    const uint32_t hi = (uint32_t)(kDpp ? shfl_xor_pow2(khi, m) : __shfl_xor(khi, m, kWave));
+                    const int32_t ov = kDpp ? shfl_xor_pow2(v[e], m) : __shfl_xor(v[e], m, kWave);
                     const uint64_t ok = ((uint64_t)hi << 32) | lo;
                     const bool asc = (i & kk) == 0;
                     const bool lower = (i & j) == 0;
@@ -240,7 +257,7 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], 
 
 template <int E>
 __device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E]) {
-    bitonic_regs<E>(k, v, (int)(threadIdx.x & (kWave - 1)));
+    bitonic_regs<E, (E <= 8)>(k, v, (int)(threadIdx.x & (kWave - 1)));
 }
 
 template <int E>
