@@ -109,6 +109,13 @@ __device__ __forceinline__ int sweep_of_gauss(const us3d_sweep* __restrict__ swe
     return s;
 }
 
+// exp(-q/2) exactly as __expf(-0.5f * q) evaluates it (v_exp_f32 of (-0.5 q) * log2(e)_f32), with the two
+// scalings folded into one multiply: -0.5 * log2(e)_f32 is exactly representable, so q * (-0.5 log2(e))
+// rounds the same real number once and the result is bit-identical, one VALU operation cheaper.
+__device__ __forceinline__ float exp_neg_half(float q) {
+    return __builtin_amdgcn_exp2f(q * -0.721347510814666748046875f);
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);

@@ -500,7 +500,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
                     const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z,
                                               y0, y1, y2);
                     if (q <= chi2) {
-                        const float ah = a.w * __expf(-0.5f * q);
+                        const float ah = a.w * exp_neg_half(q);
                         if (ah > alpha_min) {
                             const double ahd = (double)ah;
                             S_[k] += ahd;
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                     float y0, y1, y2;
                     const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
                     if (q <= chi2) {
-                        const float w = __expf(-0.5f * q);
+                        const float w = exp_neg_half(q);
                         const float ah = o * w;
                         if (ah > alpha_min) {
                             const float4 cc = sc[j];
