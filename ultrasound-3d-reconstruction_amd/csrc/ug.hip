@@ -270,6 +270,18 @@ struct TileShape {
     static constexpr int kSubPerWave = kSub / kWaves;  // pixels per thread
 };
 
+// Launch shape of the forward splat: a wavefront owns kSubPerWave sub-blocks (wave w: w, w + kWaves, ...),
+// two per wavefront for 16-pixel tiles, so every Gaussian record read from LDS serves two pixels per lane
+// whenever the Gaussian touches both of the wavefront's (vertically adjacent) sub-blocks.
+template <int TS>
+struct FwdShape {
+    static constexpr int kSubPerRow = TS / 8;
+    static constexpr int kSub = kSubPerRow * kSubPerRow;
+    static constexpr int kThreads = (TS == 8) ? 64 : (TS == 16 ? 128 : 256);
+    static constexpr int kWaves = kThreads / kWave;
+    static constexpr int kSubPerWave = kSub / kWaves;
+};
+
 struct SubBounds {
     float lat_lo, lat_hi, ax_lo, ax_hi, ele_lo, ele_hi;
 };
@@ -417,13 +429,13 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // accuracy wherever c is close to it.
 
 template <int TS>
-__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
+__global__ __launch_bounds__(FwdShape<TS>::kThreads) void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
     float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux) {
-    using S = TileShape<TS>;
+    using S = FwdShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
     __shared__ float4 s_r0[NT], s_r1[NT], s_r2[NT];
@@ -482,20 +494,29 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_forward_kernel(
             s_r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
         }
         __syncthreads();
-        // The Gaussians that touch this wave's sub-block, 64 list entries at a time: a ballot of their mask
-        // bits gives a wave-uniform 64-bit set that is walked lowest-first with scalar bit scans, so skipped
-        // entries cost nothing and each pixel still sums its Gaussians in list order.
+        // The Gaussians that touch this wave's sub-blocks, 64 list entries at a time: ballots of their mask
+        // bits give wave-uniform 64-bit sets; their union is walked lowest-first with scalar bit scans, each
+        // record is read from LDS once and evaluated for every sub-block whose bit is set. Skipped entries
+        // cost nothing and each pixel still sums its Gaussians in list order.
         for (int jb = 0; jb < n; jb += kWave) {
             const uint32_t mm = (jb + lane < n) ? s_mask[jb + lane] : 0u;
+            uint64_t bk[PPT];
+            uint64_t any = 0;
 #pragma unroll
             for (int k = 0; k < PPT; ++k) {
-                uint64_t bits = __ballot((mm >> (wave + k * S::kWaves)) & 1u);
-                while (bits) {
-                    const int j = jb + (int)__builtin_ctzll(bits);
-                    bits &= bits - 1;
-                    const float4 a = s_r0[j];
-                    const float4 b = s_r1[j];
-                    const float4 c = s_r2[j];
+                bk[k] = __ballot((mm >> (wave + k * S::kWaves)) & 1u);
+                any |= bk[k];
+            }
+            while (any) {
+                const int bit = (int)__builtin_ctzll(any);
+                any &= any - 1;
+                const int j = jb + bit;
+                const float4 a = s_r0[j];
+                const float4 b = s_r1[j];
+                const float4 c = s_r2[j];
+#pragma unroll
+                for (int k = 0; k < PPT; ++k) {
+                    if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
                     float y0, y1, y2;
                     const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z,
                                               y0, y1, y2);
@@ -1018,7 +1039,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
 #define UG_FWD(TS)                                                                                                    \
-    hipLaunchKernelGGL(ug_forward_kernel<TS>, dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream,          \
+    hipLaunchKernelGGL(ug_forward_kernel<TS>, dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0, stream,           \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        plan->frames, plan->frame_tile_base, plan->n_frames, pixels, settings->chi2_threshold,         \
                        sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
