@@ -148,17 +148,16 @@ def rocprof_summary(mode: str, directory: str) -> None:
     import sqlite3
 
     def short(name: str) -> str:
-        name = name.split("(")[0]
         for k in ("void us3d::", "us3d::", "(anonymous namespace)::"):
             name = name.replace(k, "")
-        return name[:70]
+        return name.split("(")[0][:70]
 
     dbs = glob.glob(f"{directory}/**/*.db", recursive=True)
     agg = collections.defaultdict(list)
     for db in dbs:
         con = sqlite3.connect(db)
         if mode == "trace":
-            for n, t0, t1 in con.execute("select kernel_name, start, end from kernels"):
+            for n, t0, t1 in con.execute("select name, start, end from kernels"):
                 agg[short(n)].append(t1 - t0)
         else:
             for n, c, v in con.execute("select name, counter_name, counter_value from pmc_events"):
