@@ -193,8 +193,9 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
 /* B-mode forward: replaces ShaderForwardCUDA of ops/ultragray_renderer (src/shader/wrapper_shader.cu:7-58,
  * forward.cu:22-119). pixels: per frame (W,H,3) world mm (pixel (x,y) at ((x*H)+y)*3), concatenated;
  * ray_origins/ray_dirs [n_rays,3]; ray_offsets [n_rays] elevational OPS offsets (mm).
- * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, E = num/(den+1e-6), den, 0),
- * accumulated in FP64 and stored as FP32. */
+ * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, E = num/(den+1e-6), den, 0) per
+ * sample in the pixels' ray-major order (sample (x,y) at x*H+y of its frame), accumulated in FP64 and stored
+ * as FP32; it is the backward's input, not an image. */
 int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray_origins,
                     const float* ray_dirs, const float* ray_offsets, const us3d_ur_settings* settings,
                     float* image, float* aux, void* stream);

@@ -129,3 +129,26 @@ def test_ray_samples_in_decreasing_depth_take_the_general_path(dev):
     for name, g, r in zip(("means", "scales", "quats", "atten", "sh"), out["grads"], ref):
         err = rel_l2(g, r)
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("H", [1000, 1300])
+@pytest.mark.parametrize("window", [2.0, 0.0])
+def test_ray_long_rays_with_several_hit_batches(dev, H, window):
+    """About 580 hits per scanline (three batches of 256 in the adjoint) over 1000 samples (one staged chunk
+    shared by every batch) or 1300 samples (two chunks, restaged per batch); forward and adjoint must match the
+    oracle either way."""
+    scene, frames = small_ray_case(n=3000, n_frames=1, W=4, H=H, d_ax=0.04)
+    fr = frames[0]
+    cfg = ray_cfg(fr, quadrature_window_radius_mm=window)
+    P = fr.width * fr.height
+    dB = (np.random.default_rng(5).normal(size=P) * 1e-2).astype(np.float32)
+    out = gpu_ray_render(scene, [fr], cfg, dev, d_image=dB)
+    geo, st, params, rm = _oracle(scene, fr, cfg)
+    assert rm.counts.min() > 2 * 256
+    img32, _aux = our.forward(params, geo, rm, st)
+    assert rel_l2(out["image"].detach().cpu().numpy(), img32) < IMG_TOL
+    ref = our.backward_f64(params, geo, rm, st, dB)
+    for name, g, r in zip(("means", "scales", "quats", "atten", "sh"), out["grads"], ref):
+        assert np.abs(r).max() > 0, name
+        err = rel_l2(g, r)
+        assert err < GRAD_TOL, f"H={H} Rw={window} {name}: rel L2 {err:.3e}"

@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
                 B = T[k] * (gamma * E);
             }
             image[o] = (float)B;
-            aux[o] = make_float4((float)T[k], (float)E, (float)den[k], 0.f);
+            aux[(int64_t)fr.pixel_base + (int64_t)x * H + y] = make_float4((float)T[k], (float)E, (float)den[k], 0.f);
         }
         __syncthreads();
     }
@@ -419,6 +419,68 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
     const int tid = threadIdx.x;
 
+    // Stage samples [c0, c0 + n) of the ray: positions and depths, the per-sample adjoint coefficients, and the
+    // suffix sums of cT. Returns whether the sample depths are nondecreasing over the chunk.
+    auto stage = [&](int c0, int n) -> bool {
+        __syncthreads();
+        for (int j = tid; j < n; j += kRayThreads) {
+            const int y = c0 + j;
+            const int64_t r = (int64_t)fr.pixel_base + (int64_t)x * H + y;  // ray-major sample index
+            const float* p = pixels + 3 * r;
+            const float tp = fmaxf(0.0f, dot3(p[0] - ro[0], p[1] - ro[1], p[2] - ro[2], rd[0], rd[1], rd[2]));
+            const float gB = d_image[(int64_t)fr.pixel_base + (int64_t)y * W + x];
+            float cT = 0.f, dNum = 0.f, dS = 0.f, E = 0.f;
+            if (!(fabsf(gB) < F.grad_threshold)) {
+                const float4 a = aux[r];  // T, E = num / (den + 1e-6), den
+                const float ex = __expf(-a.z);
+                const float gamma = 1.0f - ex;
+                E = a.y;
+                const float dE = gB * a.x;           // dL/dE_val
+                const float dEraw = dE * gamma;
+                dS = dE * E * ex;                      // through gamma = 1 - e^-S
+                dNum = dEraw / (a.z + 1e-6f);          // dL/dw = dNum (I - E) + dS (= dNum I + dDen + dS)
+                cT = gB * (gamma * E) * a.x;           // dL/dT * T
+            }
+            s_p[j] = make_float4(p[0], p[1], p[2], tp);
+            s_c[j] = make_float4(cT, dNum, dS, E);
+        }
+        __syncthreads();
+        // monotone sample depths, and the suffix sums of cT (thread t owns samples 4t .. 4t+3)
+        bool mono = true;
+        double loc[kRayChunkPerThread];
+        double tot = 0.0;
+#pragma unroll
+        for (int q = kRayChunkPerThread - 1; q >= 0; --q) {
+            const int j = tid * kRayChunkPerThread + q;
+            if (j < n) {
+                tot += (double)s_c[j].x;
+                if (j + 1 < n) mono = mono && (s_p[j].w <= s_p[j + 1].w);
+            }
+            loc[q] = tot;
+        }
+        double suf = tot;
+#pragma unroll
+        for (int off = 1; off < kWave; off <<= 1) {
+            const double o = __shfl_down(suf, off, kWave);
+            if ((tid & (kWave - 1)) + off < kWave) suf += o;
+        }
+        if ((tid & (kWave - 1)) == 0) s_wsum[tid / kWave] = suf;
+        mono = __syncthreads_and(mono) != 0;
+        double after = suf - tot;
+        for (int w = tid / kWave + 1; w < kRayThreads / kWave; ++w) after += s_wsum[w];
+#pragma unroll
+        for (int q = 0; q < kRayChunkPerThread; ++q) {
+            const int j = tid * kRayChunkPerThread + q;
+            if (j < n) s_S[j] = loc[q] + after;
+        }
+        if (tid == 0) s_S[n] = 0.0;
+        __syncthreads();
+        return mono;
+    };
+    // A ray of at most one chunk is staged once for all its hits; longer rays restage every chunk per batch.
+    const bool one_chunk = H <= kRayChunk;
+    bool mono = one_chunk ? stage(0, H) : true;
+
     for (int base = s0; base < s1; base += kRayThreads) {
         const int i = base + tid;
         const bool active = i < s1;
@@ -433,59 +495,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
 
         for (int c0 = 0; c0 < H; c0 += kRayChunk) {
             const int n = min(kRayChunk, H - c0);
-            __syncthreads();
-            for (int j = tid; j < n; j += kRayThreads) {
-                const int y = c0 + j;
-                const float* p = pixels + 3 * ((int64_t)fr.pixel_base + (int64_t)x * H + y);
-                const float tp = fmaxf(0.0f, dot3(p[0] - ro[0], p[1] - ro[1], p[2] - ro[2], rd[0], rd[1], rd[2]));
-                const int64_t o = (int64_t)fr.pixel_base + (int64_t)y * W + x;
-                const float gB = d_image[o];
-                float cT = 0.f, dNum = 0.f, dS = 0.f, E = 0.f;
-                if (!(fabsf(gB) < F.grad_threshold)) {
-                    const float4 a = aux[o];  // T, E = num / (den + 1e-6), den
-                    const float ex = __expf(-a.z);
-                    const float gamma = 1.0f - ex;
-                    E = a.y;
-                    const float dE = gB * a.x;           // dL/dE_val
-                    const float dEraw = dE * gamma;
-                    dS = dE * E * ex;                      // through gamma = 1 - e^-S
-                    dNum = dEraw / (a.z + 1e-6f);          // dL/dw = dNum (I - E) + dS (= dNum I + dDen + dS)
-                    cT = gB * (gamma * E) * a.x;           // dL/dT * T
-                }
-                s_p[j] = make_float4(p[0], p[1], p[2], tp);
-                s_c[j] = make_float4(cT, dNum, dS, E);
-            }
-            __syncthreads();
-            // monotone sample depths, and the suffix sums of cT (thread t owns samples 4t .. 4t+3)
-            bool mono = true;
-            double loc[kRayChunkPerThread];
-            double tot = 0.0;
-#pragma unroll
-            for (int q = kRayChunkPerThread - 1; q >= 0; --q) {
-                const int j = tid * kRayChunkPerThread + q;
-                if (j < n) {
-                    tot += (double)s_c[j].x;
-                    if (j + 1 < n) mono = mono && (s_p[j].w <= s_p[j + 1].w);
-                }
-                loc[q] = tot;
-            }
-            double suf = tot;
-#pragma unroll
-            for (int off = 1; off < kWave; off <<= 1) {
-                const double o = __shfl_down(suf, off, kWave);
-                if ((tid & (kWave - 1)) + off < kWave) suf += o;
-            }
-            if ((tid & (kWave - 1)) == 0) s_wsum[tid / kWave] = suf;
-            mono = __syncthreads_and(mono) != 0;
-            double after = suf - tot;
-            for (int w = tid / kWave + 1; w < kRayThreads / kWave; ++w) after += s_wsum[w];
-#pragma unroll
-            for (int q = 0; q < kRayChunkPerThread; ++q) {
-                const int j = tid * kRayChunkPerThread + q;
-                if (j < n) s_S[j] = loc[q] + after;
-            }
-            if (tid == 0) s_S[n] = 0.0;
-            __syncthreads();
+            if (!one_chunk) mono = stage(c0, n);
             const bool fast = window_mode && mono;
             // this thread's sample range [lo, hi) and the first finished sample jf (suffix handled by s_S)
             int lo = n, hi = 0, jf = n;
