@@ -216,7 +216,7 @@ def test_multi_sweep_batch_equals_per_sweep(dev):
     params = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity,
                                                                  scene.intensity)]
     image, _alpha, _depth = render_frames(*params, None, frames, cfg, sweeps=sweeps)
-    assert float(image.abs().max()) > 0
+    assert float(image.detach().abs().max()) > 0
     (image * dI).sum().backward()
     off = 0
     for s, sw in enumerate(sweeps):
@@ -265,3 +265,36 @@ def test_mapper_long_tile_lists_bitexact(dev, ts):
     np.testing.assert_array_equal(view["values"].cpu().numpy(), mr.vals)
     np.testing.assert_array_equal(view["tile_start"].cpu().numpy(), mr.tile_start)
     np.testing.assert_array_equal(view["tile_end"].cpu().numpy(), mr.tile_end)
+
+
+@pytest.mark.parametrize("shadow_delta", [0.0, 0.05])
+def test_shadow_branch_attenuates_rows_by_running_transmittance(dev, shadow_delta):
+    """UltraGaussRenderer with enable_shadow (api.py:37-42): image * T with T = exp(-cumsum(alpha * delta))
+    down the rows, delta = shadow_delta or 1/H. Checked against a running product of per-row factors in
+    FP64 on the CPU from the unshadowed render, and the shadowed render must stay differentiable."""
+    from us3d.ultragauss import SliceGeometry, UltraGaussRenderer
+
+    scene, frames = small_case(n=2048, n_frames=1, H=64, W=48)
+    fr = frames[0]
+    geo = SliceGeometry(fr.origin.to(dev), fr.axis_lat.to(dev), fr.axis_ax.to(dev), fr.axis_ele.to(dev))
+    pix = linear_pixels(fr).to(dev)
+    args = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    plain = UltraGaussRenderer(cfg_for(fr))
+    shadow = UltraGaussRenderer(cfg_for(fr, enable_shadow=True, shadow_delta=shadow_delta))
+    img0, alpha0, _d0, _ = plain(*args, pix, geo)
+    img1, alpha1, _d1, _ = shadow(*args, pix, geo)
+    a = alpha0.detach().cpu().double().numpy()
+    assert a.max() > 0.5
+    delta = shadow_delta if shadow_delta > 0 else 1.0 / fr.height
+    T = np.ones(fr.width)
+    want = np.empty_like(a)
+    for r in range(fr.height):
+        T = T * np.exp(-a[r] * delta)
+        want[r] = img0.detach().cpu().double().numpy()[r] * T
+    np.testing.assert_array_equal(alpha1.detach().cpu().numpy(), alpha0.detach().cpu().numpy())
+    assert rel_l2(img1.detach().cpu().numpy(), want) < 1e-6
+    assert float((img0 - img1).abs().max()) > 1e-3
+    img1.sum().backward()
+    for t in args:
+        assert t.grad is not None and torch.isfinite(t.grad).all()
+    assert float(args[2].grad.abs().max()) > 0

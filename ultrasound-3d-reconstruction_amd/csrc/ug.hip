@@ -2,7 +2,7 @@
 // gradient reduction and the per-pixel candidate lists.
 //
 // Data layout in HBM (one batch = frames of one or more sweeps):
-//   records   f32x12 per Gaussian: (mu.xyz, opacity | intensity, l11, l12, l13 | l22, l23, l33, 0),
+//   records   f32x12 per Gaussian: (mu.xyz, opacity | intensity, l11, l12, l13 | l22, l23, l33, r_max),
 //             l_ii = raw_ii^2 + beta, packed once per call so every gather is three aligned 16 B loads.
 //   pairs     (frame, Gaussian), numbered Gaussian-major per sweep: count, tile rect, depth key.
 //   instances (tile, Gaussian) in emit order e (pair order, then tile rows, then columns) -- the
@@ -284,27 +284,6 @@ struct FwdShape {
 
 struct SubBounds {
     float lat_lo, lat_hi, ax_lo, ax_hi, ele_lo, ele_hi;
-};
-
-// Grouping key of a sub-block mask: the mask itself for up to 4 sub-blocks; for a 4x4 grid of sub-blocks,
-// the 4 bits "touches quadrant q" (16 classes).
-template <int NSUB>
-struct MaskClass {
-    static constexpr int kClasses = (NSUB <= 4) ? (1 << NSUB) : 16;
-    __device__ static __forceinline__ uint32_t of(uint32_t m) {
-        if constexpr (NSUB <= 4) {
-            return m;
-        } else {
-            static_assert(NSUB == 16, "4x4 sub-block grid expected");
-            uint32_t c = 0;
-#pragma unroll
-            for (int sb = 0; sb < 16; ++sb) {
-                const int q = ((sb >> 3) << 1) | ((sb & 3) >> 1);  // row/2 * 2 + col/2 of the 4x4 grid
-                c |= ((m >> sb) & 1u) << q;
-            }
-            return c;
-        }
-    }
 };
 
 // Pixel of sub-block sb handled by `lane`.
@@ -730,8 +709,13 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                 const float4* __restrict__ sc = s_c + sb * kSubStride;
                 float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
                 float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
+                // the next pixel's position and coefficients are read before this pixel's work, so the LDS
+                // latency overlaps it (the branches below would otherwise expose it on every pixel)
+                float4 pn = sp[0], cn = sc[0];
                 for (int j = 0; j < kWave; ++j) {
-                    const float4 pp = sp[j];
+                    const float4 pp = pn, cc = cn;
+                    pn = sp[(j + 1) & (kWave - 1)];
+                    cn = sc[(j + 1) & (kWave - 1)];
                     const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
                     float y0, y1, y2;
                     const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
@@ -739,7 +723,6 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                         const float w = exp_neg_half(q);
                         const float ah = o * w;
                         if (ah > alpha_min) {
-                            const float4 cc = sc[j];
                             // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
                             // cancellation of dN c against dN v
                             const float da = __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y));

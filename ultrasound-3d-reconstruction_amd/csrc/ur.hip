@@ -266,9 +266,59 @@ __device__ __forceinline__ void unpack_rg(const RGPack& p, RayGauss& G) {
 // tau + (1 - tau) e^-psi, echo = (1 - e^-S) sum(I w) / (sum w + 1e-6), B = T * echo (forward.cu:22-119).
 // T and the echo sums are accumulated in FP64; aux keeps (T, E = num / (den + 1e-6), den) so the adjoint can
 // form dL/dw = dNum (I - E) + dS without cancelling dNum I against dNum E.
+//
+// With the transmission window on and the chunk's sample depths nondecreasing, every staged hit gets the
+// chunk-sample range [jlo, jhi) outside which its terms are fixed: before jlo it is neither in front nor in
+// echo range, from jhi on it is in front with its window finished and out of echo range (factor f_final). The
+// range is found by binary search with the same comparisons as the per-sample rules. A wavefront covers 64
+// consecutive samples per k; hits finished for all 64 enter as one FP64 product over the wavefront's 64 hits
+// (a fixed butterfly, the same on every lane), and only hits whose range meets the 64 samples are evaluated
+// per lane, walked lowest-first from ballots with scalar bit scans. Otherwise every hit is evaluated for
+// every sample.
 
 constexpr int kRayThreads = 256;
 constexpr int kPPT = 4;
+constexpr int kRaySamples = kRayThreads * kPPT;  // samples of one forward chunk
+
+// First index in [lo, hi) whose element satisfies pred (pred false then true along the range), else hi.
+template <typename Pred>
+__device__ __forceinline__ int first_true(int lo, int hi, Pred pred) {
+    while (lo < hi) {
+        const int m = (lo + hi) >> 1;
+        if (pred(m)) hi = m; else lo = m + 1;
+    }
+    return lo;
+}
+
+// Product over the wavefront; the butterfly pairs the same operands on every lane, so all lanes get the same
+// bits.
+__device__ __forceinline__ double wave_prod(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v *= __shfl_xor(v, off, kWave);
+    return v;
+}
+
+// Terms of hit G at one sample under the per-sample rules (forward.cu:76-96).
+__device__ __forceinline__ void ray_accumulate(const RayGauss& G, const RayFlags& F, float px, float py, float pz,
+                                               float tpx, double& T, double& num, double& den) {
+    bool trans, echo;
+    ray_terms(G, tpx, F, trans, echo);
+    if (trans) {
+        if (F.Rw > 0.0f && tpx >= G.b) {
+            T *= (double)G.f_final;
+        } else {
+            float ts, te, psi = 0.f;
+            if (ray_window(G, tpx, F.Rw, ts, te)) psi = gl3(G.og, G.dg, ts, te);
+            T *= (double)__builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
+        }
+    }
+    if (echo) {
+        float dx, dy, dz, v0, v1, v2;
+        const float w = echo_weight(G, px, py, pz, dx, dy, dz, v0, v1, v2);
+        num = __builtin_fma((double)G.I, (double)w, num);
+        den += (double)w;
+    }
+}
 
 __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
@@ -277,6 +327,8 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
     const float* __restrict__ ray_off, RayFlags F, const us3d_stats* __restrict__ stats, float* __restrict__ image,
     float4* __restrict__ aux) {
     __shared__ RGPack s_g[kRayThreads];
+    __shared__ float s_tp[kRaySamples];  // sample depths of the chunk
+    __shared__ int2 s_rng[kRayThreads];  // per staged hit: [jlo, jhi) in chunk samples
     if (stats->overflow) return;
     const int t = blockIdx.x;
     const int f = upper_index(frame_tile_base, n_frames, t);
@@ -286,9 +338,11 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     float ro[3], rd[3];
     ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    const bool window_mode = F.Rw > 0.0f && !F.local_only;
 
-    for (int c0 = 0; c0 < H; c0 += kRayThreads * kPPT) {
+    for (int c0 = 0; c0 < H; c0 += kRaySamples) {
+        const int ns = min(kRaySamples, H - c0);
         float px[kPPT], py[kPPT], pz[kPPT], tpx[kPPT];
         double T[kPPT], num[kPPT], den[kPPT];
 #pragma unroll
@@ -300,8 +354,17 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
                 px[k] = p[0]; py[k] = p[1]; pz[k] = p[2];
                 tpx[k] = fmaxf(0.0f, dot3(px[k] - ro[0], py[k] - ro[1], pz[k] - ro[2], rd[0], rd[1], rd[2]));
             }
+            s_tp[tid + k * kRayThreads] = tpx[k];
             T[k] = 1.0; num[k] = 0.0; den[k] = 0.0;
         }
+        __syncthreads();
+        bool mono = true;
+#pragma unroll
+        for (int q = 0; q < kPPT; ++q) {
+            const int j = tid * kPPT + q;
+            if (j + 1 < ns) mono = mono && (s_tp[j] <= s_tp[j + 1]);
+        }
+        const bool fast = window_mode && __syncthreads_and(mono) != 0;
         for (int base = s0; base < s1; base += kRayThreads) {
             __syncthreads();
             const int n = min(kRayThreads, s1 - base);
@@ -311,30 +374,50 @@ __global__ __launch_bounds__(kRayThreads) void ur_forward_kernel(
                 float R[9];
                 ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
                 s_g[tid] = pack_rg(G);
+                if (fast) {
+                    const int jt = first_true(0, ns, [&](int m) { return G.tg < s_tp[m]; });
+                    const int jb = first_true(0, ns, [&](int m) { return s_tp[m] >= G.b; });
+                    const int je0 = first_true(0, ns, [&](int m) { return (G.tg - s_tp[m]) <= G.rd; });
+                    const int je1 = first_true(je0, ns, [&](int m) { return (G.tg - s_tp[m]) < -G.rd; });
+                    s_rng[tid] = make_int2(min(jt, je0), max(max(jt, jb), je1));
+                }
             }
             __syncthreads();
-            for (int j = 0; j < n; ++j) {
-                RayGauss G;
-                unpack_rg(s_g[j], G);
+            if (fast) {
+                for (int jb0 = 0; jb0 < n; jb0 += kWave) {
+                    const int h = jb0 + lane;
+                    const bool hv = h < n;
+                    const int2 rg = hv ? s_rng[h] : make_int2(ns, ns);
+                    const float ff = hv ? s_g[h].a.w : 1.0f;  // f_final
+                    uint64_t bk[kPPT];
+                    uint64_t any = 0;
 #pragma unroll
-                for (int k = 0; k < kPPT; ++k) {
-                    bool trans, echo;
-                    ray_terms(G, tpx[k], F, trans, echo);
-                    if (trans) {
-                        if (F.Rw > 0.0f && tpx[k] >= G.b) {
-                            T[k] *= (double)G.f_final;
-                        } else {
-                            float ts, te, psi = 0.f;
-                            if (ray_window(G, tpx[k], F.Rw, ts, te)) psi = gl3(G.og, G.dg, ts, te);
-                            T[k] *= (double)__builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
+                    for (int k = 0; k < kPPT; ++k) {
+                        const int y0 = wave * kWave + k * kRayThreads;  // the wavefront's first sample for k
+                        const bool fin = hv && y0 < ns && rg.y <= y0;
+                        const bool ovl = hv && y0 < ns && rg.y > y0 && rg.x < y0 + kWave;
+                        if (__ballot(fin) != 0) T[k] *= wave_prod(fin ? (double)ff : 1.0);
+                        bk[k] = __ballot(ovl);
+                        any |= bk[k];
+                    }
+                    while (any) {
+                        const int bit = (int)__builtin_ctzll(any);
+                        any &= any - 1;
+                        RayGauss G;
+                        unpack_rg(s_g[jb0 + bit], G);
+#pragma unroll
+                        for (int k = 0; k < kPPT; ++k) {
+                            if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
+                            ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
                         }
                     }
-                    if (echo) {
-                        float dx, dy, dz, v0, v1, v2;
-                        const float w = echo_weight(G, px[k], py[k], pz[k], dx, dy, dz, v0, v1, v2);
-                        num[k] = __builtin_fma((double)G.I, (double)w, num[k]);
-                        den[k] += (double)w;
-                    }
+                }
+            } else {
+                for (int j = 0; j < n; ++j) {
+                    RayGauss G;
+                    unpack_rg(s_g[j], G);
+#pragma unroll
+                    for (int k = 0; k < kPPT; ++k) ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
                 }
             }
         }
@@ -383,16 +466,6 @@ __device__ __forceinline__ int wave_max_i(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
     return v;
-}
-
-// First index in [lo, hi) whose element satisfies pred (pred false then true along the range), else hi.
-template <typename Pred>
-__device__ __forceinline__ int first_true(int lo, int hi, Pred pred) {
-    while (lo < hi) {
-        const int m = (lo + hi) >> 1;
-        if (pred(m)) hi = m; else lo = m + 1;
-    }
-    return lo;
 }
 
 __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
