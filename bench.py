@@ -128,6 +128,40 @@ def read_traffic(kernel: str, workload_tag: str):
     return float(m.group(1)) if m else None
 
 
+def read_valu_issue(kernel: str):
+    """VALU issue fraction (VALU instructions x 4 cycles / SIMD cycles) of a kernel from the committed rocprofv3
+    counter summary of the default workloads, if it lists the kernel."""
+    p = ROOT / "profiles" / "splat_and_ray_kernels_valu_issue_pmc.md"
+    if not p.exists():
+        return None
+    for line in p.read_text().splitlines():
+        c = [x.strip() for x in line.strip("|").split("|")]
+        if len(c) == 7 and c[0] == kernel and c[6].endswith("%"):
+            return float(c[6].rstrip(" %")) / 100.0
+    return None
+
+
+def measure_copy_peak():
+    """Device-to-device copy bandwidth (read + write bytes / time) of a 1 GiB buffer, timed with HIP events on
+    the current stream: the measured HBM figure to set beside the nominal 8 TB/s."""
+    import torch
+
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(5):
+        dst.copy_(src)
+    t1.record()
+    torch.cuda.synchronize()
+    gbs = 5 * 2 * (1 << 30) / (t0.elapsed_time(t1) * 1e-3) / 1e9
+    del src, dst
+    torch.cuda.empty_cache()
+    return round(gbs, 1)
+
+
 def copy_calibration():
     """A plain copy kernel of known size: reads and writes exactly 2^30 bytes per dispatch."""
     import torch
@@ -170,9 +204,9 @@ def rocprof_summary(mode: str, directory: str) -> None:
             print(f"| {n} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e6:.4f} | {min(v) / 1e6:.4f} | "
                   f"{max(v) / 1e6:.4f} | {100 * sum(v) / tot:.1f} |")
     else:
-        print("| kernel | counter | dispatches | mean per dispatch |")
+        print("| kernel | counter | rows (dispatches x counter instances) | mean per row |")
         print("|---|---|---|---|")
-        for (n, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:24]:
+        for (n, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:60]:
             print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
 
 
@@ -315,7 +349,9 @@ def bench_ray(args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "traffic": read_traffic(dom, f"ray_{F}x{W}x{H}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
-                     "algorithmic_bytes_per_launch": alg, "stage_ms_per_launch": stage_ms},
+                     "algorithmic_bytes_per_launch": alg, "stage_ms_per_launch": stage_ms,
+                     "measured_copy_GBs": measure_copy_peak(),
+                     "valu_issue_frac": read_valu_issue(dom) if (F, G, W, H) == (128, 200_000, 128, 1024) else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -519,13 +555,13 @@ def main():
     dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
     avg_ms = total_ms / max(launches, 1)
     launch_bytes = {
-        # algorithmic HBM bytes per launch over the batch (formulas: DESIGN.md "Roofline accounting")
+        # algorithmic HBM bytes per launch over the batch: the per-frame terms 96 G + 80 G_vis + 96 K + 24 P of
+        # the slice splat split by the kernel that moves them (K instances, P pixels, G_vis visible pairs)
         "ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis,
         "ug_forward_kernel": 36 * K + 16 * P,
-        "sort_segments_kernel": 24 * K,
-        "ug_emit_kernel": 12 * K + 28 * g_vis,
-        "ug_count_kernel": 8 * K,
-        "ug_preprocess_kernel": 48 * G + 28 * F * G,
+        "sort_small_segments_kernel": 24 * K,
+        "ug_emit_kernel": 4 * F * G + 28 * g_vis + 12 * K,
+        "ug_preprocess_kernel": 48 * G + 32 * F * G + 4 * K,
         "ug_reduce_kernel": 48 * K + 48 * G,
     }
     alg_bytes = launch_bytes.get(dom)
@@ -534,7 +570,10 @@ def main():
     roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                 "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": alg_bytes, "stage_ms_per_launch": stage_ms}
+                "algorithmic_bytes_per_launch": alg_bytes, "stage_ms_per_launch": stage_ms,
+                "measured_copy_GBs": measure_copy_peak(),
+                "valu_issue_frac": read_valu_issue(f"{dom}<{ts}>") if (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
+                else None}
 
     cpu = None
     if not args.no_cpu_baseline:
