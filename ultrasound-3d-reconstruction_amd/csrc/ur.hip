@@ -593,51 +593,64 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                 }
             }
             const int wlo = wave_min_i(lo), whi = wave_max_i(hi);
+            // the next sample's position and coefficients are read before this sample's work, so the LDS
+            // latency overlaps it
+            float4 pn = s_p[min(wlo, n - 1)], cn = s_c[min(wlo, n - 1)];
             for (int j = wlo; j < whi; ++j) {
+                const float4 pp = pn, cc = cn;
+                const int jn = (j + 1 < whi) ? j + 1 : j;
+                pn = s_p[jn];
+                cn = s_c[jn];
                 if (j < lo || j >= hi) continue;
-                const float4 pp = s_p[j];
                 bool trans, echo;
                 ray_terms(G, pp.w, F, trans, echo);
                 if (j >= jf) trans = false;  // in the finished suffix: counted through s_S
                 if (!trans && !echo) continue;
-                const float4 cc = s_c[j];
                 if (trans) {
-                    float ts = 0.f, te = 0.f, ep, Ti;
                     const bool finished = F.Rw > 0.0f && pp.w >= G.b;
-                    bool win = false;
                     if (finished) {
-                        ep = G.e_final;
-                        Ti = G.f_final;
+                        const float ep = G.e_final, Ti = G.f_final;
+                        const float dTi = Ti > 1e-6f ? cc.x * __builtin_amdgcn_rcpf(Ti) : 0.f;
+                        dtau = __builtin_fmaf(dTi, 1.0f - ep, dtau);
+                        fin += -dTi * (1.0f - G.tau) * ep;
                     } else {
-                        win = ray_window(G, pp.w, F.Rw, ts, te);
-                        ep = win ? __expf(-gl3(G.og, G.dg, ts, te)) : 1.0f;
-                        Ti = __builtin_fmaf(1.0f - G.tau, ep, G.tau);
-                    }
-                    const float dTi = Ti > 1e-6f ? cc.x / Ti : 0.f;
-                    dtau = __builtin_fmaf(dTi, 1.0f - ep, dtau);
-                    const float dpsi = -dTi * (1.0f - G.tau) * ep;
-                    if (finished) {
-                        fin += dpsi;
-                    } else if (win) {
-                        const float half = 0.5f * (te - ts), c = 0.5f * (te + ts);
-                        const float coeff = dpsi * half;
-                        const float xi[3] = {-kXi, 0.0f, kXi};
-                        const float wq[3] = {kW0, kW1, kW0};
+                        float ts = 0.f, te = 0.f;
+                        if (ray_window(G, pp.w, F.Rw, ts, te)) {
+                            // the GL3 nodes and their exponentials serve both psi (as gl3 evaluates it) and
+                            // its VJP, with the window endpoints held fixed (backward.cu:57-72)
+                            const float half = 0.5f * (te - ts), c = 0.5f * (te + ts);
+                            const float xi[3] = {-kXi, 0.0f, kXi};
+                            const float wq[3] = {kW0, kW1, kW0};
+                            float tq[3], x0[3], x1[3], x2[3], eq[3];
+                            float acc = 0.f;
 #pragma unroll
-                        for (int q = 0; q < 3; ++q) {
-                            const float tq = __builtin_fmaf(half, xi[q], c);
-                            const float x0 = __builtin_fmaf(G.dg[0], tq, G.og[0]);
-                            const float x1 = __builtin_fmaf(G.dg[1], tq, G.og[1]);
-                            const float x2 = __builtin_fmaf(G.dg[2], tq, G.og[2]);
-                            const float a = -coeff * wq[q] * __expf(-0.5f * dot3(x0, x1, x2, x0, x1, x2));
-                            dog[0] = __builtin_fmaf(a, x0, dog[0]);
-                            dog[1] = __builtin_fmaf(a, x1, dog[1]);
-                            dog[2] = __builtin_fmaf(a, x2, dog[2]);
-                            const float at = a * tq;
-                            ddg[0] = __builtin_fmaf(at, x0, ddg[0]);
-                            ddg[1] = __builtin_fmaf(at, x1, ddg[1]);
-                            ddg[2] = __builtin_fmaf(at, x2, ddg[2]);
+                            for (int q = 0; q < 3; ++q) {
+                                tq[q] = __builtin_fmaf(half, xi[q], c);
+                                x0[q] = __builtin_fmaf(G.dg[0], tq[q], G.og[0]);
+                                x1[q] = __builtin_fmaf(G.dg[1], tq[q], G.og[1]);
+                                x2[q] = __builtin_fmaf(G.dg[2], tq[q], G.og[2]);
+                                eq[q] = __expf(-0.5f * dot3(x0[q], x1[q], x2[q], x0[q], x1[q], x2[q]));
+                                acc = __builtin_fmaf(wq[q], eq[q], acc);
+                            }
+                            const float ep = __expf(-(half * acc));
+                            const float Ti = __builtin_fmaf(1.0f - G.tau, ep, G.tau);
+                            const float dTi = Ti > 1e-6f ? cc.x * __builtin_amdgcn_rcpf(Ti) : 0.f;
+                            dtau = __builtin_fmaf(dTi, 1.0f - ep, dtau);
+                            const float coeff = -dTi * (1.0f - G.tau) * ep * half;  // d psi * half
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                const float a = -coeff * wq[q] * eq[q];
+                                dog[0] = __builtin_fmaf(a, x0[q], dog[0]);
+                                dog[1] = __builtin_fmaf(a, x1[q], dog[1]);
+                                dog[2] = __builtin_fmaf(a, x2[q], dog[2]);
+                                const float at = a * tq[q];
+                                ddg[0] = __builtin_fmaf(at, x0[q], ddg[0]);
+                                ddg[1] = __builtin_fmaf(at, x1[q], ddg[1]);
+                                ddg[2] = __builtin_fmaf(at, x2[q], ddg[2]);
+                            }
                         }
+                        // an empty window (depth <= 0 or before the window starts) has psi = 0, T_i = 1 and
+                        // contributes nothing to d raw tau or the window VJP
                     }
                 }
                 if (echo) {
