@@ -280,6 +280,10 @@ struct FwdShape {
     static constexpr int kThreads = (TS == 8) ? 64 : (TS == 16 ? 128 : 256);
     static constexpr int kWaves = kThreads / kWave;
     static constexpr int kSubPerWave = kSub / kWaves;
+    // Minimum waves per SIMD asked of the compiler: for 16-pixel tiles eight (at most 64 VGPRs) measured
+    // faster than its own choice of five, the few values it then spills being live only at the ends of the
+    // kernel; 1 leaves the choice to the compiler.
+    static constexpr int kMinWavesPerSimd = (TS == 16) ? 8 : 1;
 };
 
 struct SubBounds {
@@ -408,7 +412,8 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // accuracy wherever c is close to it.
 
 template <int TS>
-__global__ __launch_bounds__(FwdShape<TS>::kThreads) void ug_forward_kernel(
+__global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
+void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
