@@ -109,6 +109,10 @@ typedef struct us3d_ug_settings {
     int32_t sort_by_depth;  /* 1: key = (tile, depth key); 0: key = (tile) */
     int32_t tile_size;      /* 8, 16 or 32 pixels */
     int32_t alpha_excludes_bg; /* alpha output = sum alpha_hat (1) or sum alpha_hat + bg_alpha (0) */
+    int32_t raw_activations;   /* 1: the opacity and intensity arrays hold logits. us3d_ug_map applies the
+                                * sigmoid 1/(1+exp(-x)) of the training model's activations
+                                * (pipeline_god_file.py:369-373) and us3d_ug_backward returns the gradients
+                                * w.r.t. the logits (chain s(1-s)); 0: they hold opacity and intensity. */
 } us3d_ug_settings;
 
 /* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by both renderers.
@@ -174,7 +178,10 @@ typedef struct us3d_ur_settings {
     float local_transmission_max_window_mm;
     float grad_threshold;               /* pixels with |dL/dB| < grad_threshold are skipped (1e-6) */
     int32_t local_transmission_window_only;
-    int32_t reserved0;
+    int32_t scales_are_log;             /* 1: the scales array holds log-scales. us3d_ur_map applies
+                                         * max(exp(x), 1e-5) (pipeline_god_file.py:360-361) and us3d_ur_backward
+                                         * returns the gradients w.r.t. the log-scales (chain exp(x) where
+                                         * exp(x) >= 1e-5, else 0); 0: it holds the scales. */
     int32_t reserved1;
 } us3d_ur_settings;
 

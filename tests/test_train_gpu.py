@@ -143,3 +143,31 @@ def test_autotuner_picks_and_caches_a_tile_size(dev, tmp_path, monkeypatch):
     assert set(first["ms_per_frame"]) == {str(c) for c in CANDIDATES}
     again = tune_tile_size(*init, frames, gt, cfg, sample_frames=4, reps=1, sweeps=sweeps)
     assert again["cached"] and again["tile_size"] == first["tile_size"]
+
+
+def test_trainer_gradients_match_torch_autograd_through_the_activations(dev):
+    """The step passes the opacity/intensity logits to the library: the pack applies the sigmoid and the reduce
+    its chain (pipeline_god_file.py:369-373). Its gradients equal torch autograd of the same loss through
+    torch.sigmoid and the differentiable renderer (which is given the activated values)."""
+    from us3d.pipeline import simple_ssim
+
+    frames, gt, cfg, init = _setup(dev)
+    tr = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, lr=0.0)
+    tr.step()
+    torch.cuda.synchronize()
+    ps = [t.detach().clone().requires_grad_(True) for t in init]
+    img = render_frames(ps[0], ps[1], torch.sigmoid(ps[2]), torch.sigmoid(ps[3]), None, frames, cfg)[0]
+    F = len(frames)
+    P = img.numel() // F
+    per = []
+    for f in range(F):
+        x, y = img[f * P:(f + 1) * P], gt.reshape(-1)[f * P:(f + 1) * P]
+        per.append((x - y).abs().mean() + 0.2 * (1 - simple_ssim(x, y)))
+    loss = torch.stack(per).mean()
+    loss.backward()
+    assert abs(float(tr.losses().mean()) - float(loss.detach())) < 1e-5 * max(1.0, abs(float(loss.detach())))
+    for name, g, p in zip(("means", "tril", "raw_opacity", "raw_intensity"), tr.grads, ps):
+        ref = p.grad.detach().cpu().numpy()
+        assert np.abs(ref).max() > 0, name
+        err = rel_l2(g.detach().cpu().numpy(), ref)
+        assert err < 1e-4, f"{name}: rel L2 {err:.3e}"

@@ -152,3 +152,47 @@ def test_ray_long_rays_with_several_hit_batches(dev, H, window):
         assert np.abs(r).max() > 0, name
         err = rel_l2(g, r)
         assert err < GRAD_TOL, f"H={H} Rw={window} {name}: rel L2 {err:.3e}"
+
+
+def test_ray_log_scales_are_activated_in_the_pack(dev):
+    """scales_are_log: the pack applies max(exp(x), 1e-5) to log-scales (pipeline_god_file.py:360-361) and the
+    reduce returns d/dx = d/ds * exp(x) where exp(x) >= 1e-5, else 0. Same image as passing the activated scales,
+    gradients equal to the chain applied on the host; one clamped axis gets a zero gradient."""
+    import torch
+
+    from us3d.batch import CapacityHint, SweepSpec
+    from us3d.scenes import ray_geometry
+    from us3d.ultragray import URBatch
+    from tests.ur_helpers import ray_spec
+
+    scene, frames = small_ray_case(n_frames=1)
+    fr = frames[0]
+    cfg = ray_cfg(fr)
+    means, scales, quats, atten, sh = [t.to(dev).float().contiguous() for t in scene.params()]
+    log_s = torch.log(scales)
+    log_s[0, 1] = -20.0  # exp = 2e-9: clamped to 1e-5
+    act = torch.exp(log_s).clamp_min(1e-5)
+    pix, ro, rd = (t.to(dev).contiguous() for t in ray_geometry(fr))
+    pix = pix.reshape(-1, 3).contiguous()
+    off = torch.zeros(fr.width, device=dev)
+    P = fr.width * fr.height
+    dB = torch.from_numpy((np.random.default_rng(8).normal(size=P) * 1e-2).astype(np.float32)).to(dev)
+    G = means.shape[0]
+    out = {}
+    for log_mode in (0, 1):
+        st = cfg.settings()
+        st.scales_are_log = log_mode
+        batch = URBatch([ray_spec(fr, cfg)], [SweepSpec(0, G)], G, st, dev, CapacityHint())
+        m = batch.map(means, log_s if log_mode else act, quats, atten, sh)
+        image, aux = batch.forward(m, pix, ro, rd, off)
+        grads = batch.backward(m, pix, ro, rd, off, aux, dB)
+        out[log_mode] = (image.clone(), [g.clone() for g in grads])
+    img_lin, g_lin = out[0]
+    img_log, g_log = out[1]
+    assert float(img_lin.abs().max()) > 1e-3
+    assert rel_l2(img_log.cpu().numpy(), img_lin.cpu().numpy()) < 1e-6
+    want = g_lin[1] * act * (torch.exp(log_s) >= 1e-5).float()
+    assert float(g_log[1][0, 1]) == 0.0
+    assert rel_l2(g_log[1].cpu().numpy(), want.cpu().numpy()) < 1e-5
+    for a, b in zip(g_log[:1] + g_log[2:], g_lin[:1] + g_lin[2:]):
+        assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-5

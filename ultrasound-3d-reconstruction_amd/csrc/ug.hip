@@ -56,10 +56,13 @@ static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, 
 // Packs one record per Gaussian. The spare last word holds r_max = sqrt(chi2) ||M^-1||_F * 1.01 (M = the upper
 // triangular factor with q = |M d|^2): aT Sigma a = |M^-T a|^2 <= ||M^-1||_F^2 for any unit a, so r_max bounds
 // every marginal radius and a pair with |elevation| > r_max is culled without computing them (NaN for a
-// degenerate factor, which then takes the exact path).
+// degenerate factor, which then takes the exact path). With `logits`, opacity and intensity are the training
+// model's logits and get its sigmoid here (pipeline_god_file.py:369-373), so no separate activation pass runs.
+__device__ __forceinline__ float sigmoidf(float x) { return 1.0f / (1.0f + expf(-x)); }
+
 __global__ void ug_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ tril,
                                const float* __restrict__ opacity, const float* __restrict__ intensity, float beta,
-                               float sqrt_chi2, float4* __restrict__ rec) {
+                               float sqrt_chi2, int logits, float4* __restrict__ rec) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float* t = tril + (int64_t)g * 6;
@@ -71,8 +74,10 @@ __global__ void ug_pack_kernel(int n, const float* __restrict__ means, const flo
     const double i13 = ((double)l12 * l23 - (double)l13 * l22) * i11 * i22 * i33;
     const double fro2 = i11 * i11 + i22 * i22 + i33 * i33 + i12 * i12 + i23 * i23 + i13 * i13;
     const float r_max = (float)((double)sqrt_chi2 * sqrt(fro2) * 1.01);
-    rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], opacity[g]);
-    rec[3 * (int64_t)g + 1] = make_float4(intensity[g], l11, l12, l13);
+    const float o = logits ? sigmoidf(opacity[g]) : opacity[g];
+    const float c = logits ? sigmoidf(intensity[g]) : intensity[g];
+    rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], o);
+    rec[3 * (int64_t)g + 1] = make_float4(c, l11, l12, l13);
     rec[3 * (int64_t)g + 2] = make_float4(l22, l23, l33, r_max);
 }
 
@@ -799,6 +804,7 @@ __global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_
                                                         int n_sweeps, const int32_t* __restrict__ pair_off,
                                                         const float4* __restrict__ partial,
                                                         const float* __restrict__ tril,
+                                                        const float4* __restrict__ rec, int logits,
                                                         const us3d_stats* __restrict__ stats,
                                                         float* __restrict__ d_means, float* __restrict__ d_tril,
                                                         float* __restrict__ d_opacity,
@@ -824,8 +830,14 @@ __global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_
     for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
     if (lane == 0) {
         const float* t = tril + (int64_t)g * 6;
-        d_opacity[g] = acc[0];
-        d_intensity[g] = acc[1];
+        if (logits) {  // through the sigmoids of the pack: d x = d s * s (1 - s)
+            const float o = rec[3 * (int64_t)g + 0].w, c = rec[3 * (int64_t)g + 1].x;
+            d_opacity[g] = acc[0] * (o * (1.0f - o));
+            d_intensity[g] = acc[1] * (c * (1.0f - c));
+        } else {
+            d_opacity[g] = acc[0];
+            d_intensity[g] = acc[1];
+        }
         d_means[3 * (int64_t)g + 0] = acc[2];
         d_means[3 * (int64_t)g + 1] = acc[3];
         d_means[3 * (int64_t)g + 2] = acc[4];
@@ -999,7 +1011,8 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ug_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
-                           means, tril, opacity, intensity, settings->diag_beta, sqrt_chi2, rec);
+                           means, tril, opacity, intensity, settings->diag_beta, sqrt_chi2,
+                           settings->raw_activations ? 1 : 0, rec);
         if ((rc = check_launch("ug_pack_kernel"))) return rc;
     }
     const int64_t np = plan->n_pairs;
@@ -1079,7 +1092,8 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
             StageTimer timer_("ug_reduce_kernel", stream);
             hipLaunchKernelGGL(ug_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
                                plan->sweeps, plan->n_sweeps, at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
-                               tril, at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
+                               tril, at<float4>(plan, L.records), settings->raw_activations ? 1 : 0,
+                               at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
         }
         if ((rc = check_launch("ug_reduce_kernel"))) return rc;
     }

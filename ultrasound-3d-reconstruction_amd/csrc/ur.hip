@@ -24,14 +24,32 @@ us3d_layout ur_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 64, 64);
 }
 
+// One record per Gaussian: (mu, raw attenuation | scales, m | quaternion | SH). With `log_scales` the scales
+// array holds the training model's log-scales and gets its activation max(exp(x), 1e-5) here
+// (pipeline_god_file.py:360-361); m has bit c set where exp(x_c) >= 1e-5, i.e. where the gradient passes the
+// clamp (torch's clamp_min passes it where x >= min). A NaN log-scale stays NaN, as under clamp_min.
 __global__ void ur_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ scales,
                                const float* __restrict__ quats, const float* __restrict__ atten,
-                               const float* __restrict__ sh, float4* __restrict__ rec) {
+                               const float* __restrict__ sh, int log_scales, float4* __restrict__ rec) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const int64_t G = g;
+    float sc[3];
+    int pass = 7;
+    if (log_scales) {
+        pass = 0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float e = expf(scales[3 * G + c]);
+            sc[c] = (e < 1e-5f) ? 1e-5f : e;
+            pass |= (e >= 1e-5f ? 1 : 0) << c;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) sc[c] = scales[3 * G + c];
+    }
     rec[4 * G + 0] = make_float4(means[3 * G], means[3 * G + 1], means[3 * G + 2], atten[G]);
-    rec[4 * G + 1] = make_float4(scales[3 * G], scales[3 * G + 1], scales[3 * G + 2], 0.f);
+    rec[4 * G + 1] = make_float4(sc[0], sc[1], sc[2], (float)pass);
     rec[4 * G + 2] = make_float4(quats[4 * G], quats[4 * G + 1], quats[4 * G + 2], quats[4 * G + 3]);
     rec[4 * G + 3] = make_float4(sh[4 * G], sh[4 * G + 1], sh[4 * G + 2], sh[4 * G + 3]);
 }
@@ -741,6 +759,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
 __global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
                                                         int n_sweeps, const int32_t* __restrict__ pair_off,
                                                         const float4* __restrict__ partial,
+                                                        const float4* __restrict__ rec, int log_scales,
                                                         const us3d_stats* __restrict__ stats, float* __restrict__ d_means,
                                                         float* __restrict__ d_scales, float* __restrict__ d_quats,
                                                         float* __restrict__ d_atten, float* __restrict__ d_sh) {
@@ -765,7 +784,15 @@ __global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_
     if (lane == 0) {
         const int64_t G = g;
         d_means[3 * G] = acc[0]; d_means[3 * G + 1] = acc[1]; d_means[3 * G + 2] = acc[2];
-        d_scales[3 * G] = acc[3]; d_scales[3 * G + 1] = acc[4]; d_scales[3 * G + 2] = acc[5];
+        if (log_scales) {  // through max(exp(x), 1e-5): d x = d s * exp(x) where the clamp passes, else 0
+            const float4 r1 = rec[4 * G + 1];
+            const int pass = (int)r1.w;
+            d_scales[3 * G] = (pass & 1) ? acc[3] * r1.x : 0.f;
+            d_scales[3 * G + 1] = (pass & 2) ? acc[4] * r1.y : 0.f;
+            d_scales[3 * G + 2] = (pass & 4) ? acc[5] * r1.z : 0.f;
+        } else {
+            d_scales[3 * G] = acc[3]; d_scales[3 * G + 1] = acc[4]; d_scales[3 * G + 2] = acc[5];
+        }
         d_atten[G] = acc[6];
         d_quats[4 * G] = acc[8]; d_quats[4 * G + 1] = acc[9]; d_quats[4 * G + 2] = acc[10]; d_quats[4 * G + 3] = acc[11];
         d_sh[4 * G] = acc[12]; d_sh[4 * G + 1] = acc[13]; d_sh[4 * G + 2] = acc[14]; d_sh[4 * G + 3] = acc[15];
@@ -824,7 +851,7 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
     if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ur_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
-                           means, scales, quats, attenuations, sh, rec);
+                           means, scales, quats, attenuations, sh, settings->scales_are_log ? 1 : 0, rec);
         if ((rc = check_launch("ur_pack_kernel"))) return rc;
     }
     if (plan->n_pairs > 0) {
@@ -887,6 +914,7 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
             StageTimer timer_("ur_reduce_kernel", stream);
             hipLaunchKernelGGL(ur_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
                                plan->sweeps, plan->n_sweeps, ws_at<int32_t>(plan, L.pair_off), ws_at<float4>(plan, L.partial),
+                               ws_at<float4>(plan, L.records), settings->scales_are_log ? 1 : 0,
                                ws_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_attenuations, d_sh);
         }
         if ((rc = check_launch("ur_reduce_kernel"))) return rc;

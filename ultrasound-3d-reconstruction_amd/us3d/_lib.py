@@ -90,6 +90,7 @@ class UGSettings(ctypes.Structure):
         ("sort_by_depth", ctypes.c_int32),
         ("tile_size", ctypes.c_int32),
         ("alpha_excludes_bg", ctypes.c_int32),
+        ("raw_activations", ctypes.c_int32),
     ]
 
 
@@ -101,7 +102,7 @@ class URSettings(ctypes.Structure):
         ("local_transmission_max_window_mm", ctypes.c_float),
         ("grad_threshold", ctypes.c_float),
         ("local_transmission_window_only", ctypes.c_int32),
-        ("reserved0", ctypes.c_int32),
+        ("scales_are_log", ctypes.c_int32),
         ("reserved1", ctypes.c_int32),
     ]
 

@@ -1,9 +1,10 @@
 """A whole slice-splat training step over a batch of frames as one captured hipGraph.
 
 Per step (all on one stream, no host synchronisation, no allocation):
-  sigmoid activations -> us3d_ug_map -> us3d_ug_forward -> us3d_image_loss (L1 + w (1 - global SSIM) and its
-  gradient, pipeline_god_file.py:606-637) -> us3d_ug_backward -> chain through the sigmoids -> us3d_adam_step
-  for each parameter group (torch.optim.Adam semantics, :740,764) -> us3d_adam_advance.
+  us3d_ug_map (its pack applies the opacity/intensity sigmoids, pipeline_god_file.py:369-373) ->
+  us3d_ug_forward -> us3d_image_loss (L1 + w (1 - global SSIM) and its gradient, :606-637) ->
+  us3d_ug_backward (its reduce returns the gradients w.r.t. the logits) -> us3d_adam_step for each parameter
+  group (torch.optim.Adam semantics, :740,764) -> us3d_adam_advance.
 The instance arrays have a fixed capacity (from an eager mapping of the same batch, with headroom). If a step's
 mapping overflows it, the Adam kernels skip the update (they read the overflow flag), `step()` notices it when it
 checks the stats, grows the capacity, captures again and repeats the step -- so an overflow never corrupts the
@@ -42,13 +43,13 @@ class SliceSplatTrainer:
         self.lr, self.betas, self.eps = float(lr), betas, float(eps)
         self.w_l1, self.w_ssim = float(w_l1), float(w_ssim)
         self.cfg = cfg
-        self.batch = UGBatch(frames, sweeps or [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
+        settings = cfg.settings()
+        settings.raw_activations = 1  # opacity and intensity are passed as logits
+        self.batch = UGBatch(frames, sweeps or [SweepSpec(0, G)], G, settings, dev, CapacityHint())
         lay = self.batch.layout
         self.F = lay.n_frames
         P = lay.n_pixels
         self.target = target.detach().to(**f32).reshape(P).contiguous()
-        self.opac = torch.empty(G, **f32)
-        self.inten = torch.empty(G, **f32)
         self.image = torch.empty(P, **f32)
         self.alpha = torch.empty(P, **f32)
         self.depth = torch.empty(P, **f32)
@@ -58,8 +59,6 @@ class SliceSplatTrainer:
         self.loss_tmp = torch.empty(int(_lib.lib().us3d_image_loss_tmp_bytes(self.F)) // 8, dtype=torch.float64,
                                     device=dev)
         self.grads = [torch.empty_like(p) for p in self.params]
-        self.d_opac = torch.empty(G, **f32)
-        self.d_inten = torch.empty(G, **f32)
         self.headroom = headroom
         self.use_graph = use_graph
         self.check_every = max(1, int(check_every))
@@ -67,7 +66,7 @@ class SliceSplatTrainer:
         self.graph = None
         # size the instance arrays from an eager mapping of the current parameters
         with torch.no_grad():
-            m = self.batch.map(self.means, self.tril, torch.sigmoid(self.raw_o), torch.sigmoid(self.raw_c))
+            m = self.batch.map(self.means, self.tril, self.raw_o, self.raw_c)
         self._allocate(int(m.stats.n_instances * headroom) + 4096)
 
     # ------------------------------------------------------------------ plumbing
@@ -84,10 +83,8 @@ class SliceSplatTrainer:
         s = _lib.stream_ptr(self.dev)
         st = self.batch.settings
         lay = self.batch.layout
-        torch.sigmoid(self.raw_o, out=self.opac)
-        torch.sigmoid(self.raw_c, out=self.inten)
-        _lib.check(L.us3d_ug_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.opac),
-                                 _lib.ptr(self.inten), st, s), "us3d_ug_map")
+        _lib.check(L.us3d_ug_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.raw_o),
+                                 _lib.ptr(self.raw_c), st, s), "us3d_ug_map")
         _lib.check(L.us3d_ug_forward(self.plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
                                      _lib.ptr(self.depth), _lib.ptr(self.aux), s), "us3d_ug_forward")
         _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
@@ -95,11 +92,8 @@ class SliceSplatTrainer:
                                      _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
         _lib.check(L.us3d_ug_backward(self.plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
                                       _lib.ptr(self.d_image), None, None, _lib.ptr(self.grads[0]),
-                                      _lib.ptr(self.grads[1]), _lib.ptr(self.d_opac), _lib.ptr(self.d_inten), s),
+                                      _lib.ptr(self.grads[1]), _lib.ptr(self.grads[2]), _lib.ptr(self.grads[3]), s),
                    "us3d_ug_backward")
-        # d raw = d sigmoid * s (1 - s)
-        torch.mul(self.d_opac, self.opac * (1.0 - self.opac), out=self.grads[2])
-        torch.mul(self.d_inten, self.inten * (1.0 - self.inten), out=self.grads[3])
         b1, b2 = self.betas
         for p, g, m, v in zip(self.params, self.grads, self.exp_avg, self.exp_avg_sq):
             _lib.check(L.us3d_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v), p.numel(),
