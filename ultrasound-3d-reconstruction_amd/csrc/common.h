@@ -158,9 +158,6 @@ inline unsigned blocks_for(int64_t n, int threads) {
 
 int check_plan(const us3d_plan* plan, int64_t ws_needed);
 
-// XCDs of the chip (workgroups are dispatched to them round-robin) and workgroups per frame of the emit.
-constexpr int kXcds = 8;
-constexpr int kEmitGroups = 32;
 
 // Before the preprocess: clear the per-tile instance counters the preprocess increments and mark every
 // frame-major visible-pair entry empty (-1).

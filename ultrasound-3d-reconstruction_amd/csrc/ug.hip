@@ -181,50 +181,64 @@ __global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* tot
     off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
 }
 
-// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket, at the bucket's
-// offset plus a slot taken from the tile's cursor. Slot order inside a bucket is arbitrary; the bucket sort
-// by the unique keys makes the final order exact. Frame-major: workgroup w serves frame f = 8 (w / 8 /
-// kEmitGroups) + w % 8 and walks that frame's region of vis_list (one entry per Gaussian of its sweep, -1
-// where the pair is culled) in strides of kEmitGroups groups of 256. Workgroups are dispatched to the 8 XCDs
-// round-robin, so every bucket of a frame is written through one XCD's L2, where the scattered 8 B / 4 B
-// stores of a bucket merge into full lines before they are written back.
-__global__ __launch_bounds__(256) void ug_emit_kernel(
-    int n_frames, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket. One workgroup per
+// frame walks the frame's region of vis_list (one entry per Gaussian of its sweep, -1 where the pair is culled)
+// kEmitPerThread entries per thread at a time, gathers the visible pairs into an LDS queue (ballot compaction)
+// and expands them kEmitThreads at a time with the wave-cooperative instance walk. The frame's bucket cursors
+// live in LDS, seeded with the bucket offsets, so a slot costs one LDS atomic instead of a returning atomic in
+// L2; a frame with more than kEmitLdsTiles tiles takes its slots from the global cursors instead. Slot order
+// inside a bucket is arbitrary; the bucket sort by the unique keys makes the final order exact. Workgroups go
+// to the XCDs round-robin, so all buckets of a frame are written through one XCD's L2.
+constexpr int kEmitThreads = 1024;
+constexpr int kEmitPerThread = 8;
+constexpr int kEmitChunk = kEmitThreads * kEmitPerThread;
+constexpr int kEmitQueue = kEmitChunk + kEmitThreads;
+constexpr int kEmitLdsTiles = 8192;
+
+__global__ __launch_bounds__(kEmitThreads) void ug_emit_kernel(
+    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
     const int32_t* __restrict__ vis_list, const int32_t* __restrict__ pair_count,
     const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
     const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor, uint64_t* __restrict__ seg_key,
     int32_t* __restrict__ seg_val, const us3d_stats* stats) {
+    __shared__ int32_t s_cur[kEmitLdsTiles];  // next free slot of each bucket of the frame
+    __shared__ int32_t s_q[kEmitQueue];       // visible pairs not yet expanded
+    __shared__ int s_n;
     if (stats->overflow) return;
-    const int w = blockIdx.x;
-    const int xcd = w % kXcds, j = w / kXcds;
-    const int f = (j / kEmitGroups) * kXcds + xcd;
-    if (f >= n_frames) return;
+    const int f = blockIdx.x;
     const us3d_frame& fr = frames[f];
     const us3d_sweep sw = sweeps[fr.sweep];
     const int32_t* list = vis_list + sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count;
-    // visible pairs are gathered into an LDS queue (ballot compaction, no global atomics) and expanded 256 at a
-    // time, so the lanes of the instance walk stay busy although most entries of the region are empty
-    __shared__ int32_t s_q[512];
-    __shared__ int s_n;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (threadIdx.x == 0) s_n = 0;
+    const int n_tiles = fr.tiles_x * fr.tiles_y;
+    const bool local = n_tiles <= kEmitLdsTiles;  // workgroup-uniform
+    const int tid = threadIdx.x, lane = tid & (kWave - 1);
+    if (local)
+        for (int i = tid; i < n_tiles; i += kEmitThreads) s_cur[i] = tile_off[fr.tile_base + i];
+    if (tid == 0) s_n = 0;
     __syncthreads();
-    for (int grp = j % kEmitGroups;; grp += kEmitGroups) {
-        const bool more = grp * 256 < sw.g_count;  // workgroup-uniform
-        if (more) {
-            const int idx = grp * 256 + (int)threadIdx.x;
-            const int pair = idx < sw.g_count ? list[idx] : -1;
-            const uint64_t bal = __ballot(pair >= 0);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int c0 = 0; c0 < sw.g_count; c0 += kEmitChunk) {
+        int pr[kEmitPerThread];
+#pragma unroll
+        for (int k = 0; k < kEmitPerThread; ++k) {
+            const int idx = c0 + k * kEmitThreads + tid;
+            pr[k] = idx < sw.g_count ? list[idx] : -1;
+        }
+#pragma unroll
+        for (int k = 0; k < kEmitPerThread; ++k) {
+            const uint64_t bal = __ballot(pr[k] >= 0);
             int wbase = 0;
             if (lane == 0 && bal) wbase = atomicAdd(&s_n, (int)__popcll(bal));
             wbase = __shfl(wbase, 0, kWave);
-            if (pair >= 0) s_q[wbase + (int)__popcll(bal & ((1ull << lane) - 1ull))] = pair;
+            if (pr[k] >= 0) s_q[wbase + (int)__popcll(bal & below)] = pr[k];
         }
         __syncthreads();
-        const int n = s_n;
-        if (n >= 256 || (!more && n > 0)) {
-            const int take = min(n, 256);
-            const int pair = (int)threadIdx.x < take ? s_q[threadIdx.x] : -1;
+        int n = s_n;  // workgroup-uniform
+        const bool last = c0 + kEmitChunk >= sw.g_count;
+        while (n >= kEmitThreads || (last && n > 0)) {
+            const int take = min(n, kEmitThreads);
+            n -= take;  // expand the queue's last `take` entries, so the rest stays in place
+            const int pair = tid < take ? s_q[n + tid] : -1;
             int count = 0;
             int4 rect = make_int4(0, 0, 0, 0);
             uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
@@ -237,18 +251,15 @@ __global__ __launch_bounds__(256) void ug_emit_kernel(
             }
             for_each_wave_instance<3>(count, rect, fr.tile_base, fr.tiles_x, pay,
                                       [&](const uint32_t* o, int jj, int tile) {
-                                          const int slot = tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
+                                          const int slot = local ? atomicAdd(&s_cur[tile - fr.tile_base], 1)
+                                                                 : tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
                                           seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
                                           seg_val[slot] = (int32_t)o[2] + jj;
                                       });
-            const int rest = n - take;
-            const int moved = (int)threadIdx.x < rest ? s_q[take + threadIdx.x] : 0;
-            __syncthreads();  // every read of the queue is done
-            if ((int)threadIdx.x < rest) s_q[threadIdx.x] = moved;
-            if (threadIdx.x == 0) s_n = rest;
-            __syncthreads();
         }
-        if (!more) break;
+        __syncthreads();  // every read of the queue is done
+        if (tid == 0) s_n = n;
+        __syncthreads();
     }
 }
 
@@ -960,8 +971,7 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     if (np > 0) {
         {
             StageTimer timer_("ug_emit_kernel", stream);
-            const unsigned grid = (unsigned)(((plan->n_frames + kXcds - 1) / kXcds) * kXcds * kEmitGroups);
-            hipLaunchKernelGGL(ug_emit_kernel, dim3(grid), dim3(256), 0, stream, plan->n_frames, plan->frames,
+            hipLaunchKernelGGL(ug_emit_kernel, dim3(plan->n_frames), dim3(kEmitThreads), 0, stream, plan->frames,
                                plan->sweeps, at<int32_t>(plan, L.vis_list), pair_count, pair_off, pair_rect, pair_dk,
                                tile_off, tile_cursor,
                                at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), (const us3d_stats*)stats);
