@@ -252,7 +252,7 @@ def bench_ray(args):
 
     from us3d import _lib
     from us3d.batch import CapacityHint, FrameSpec, SweepSpec
-    from us3d.scenes import config3, make_ray_scene, ray_geometry
+    from us3d.scenes import config3, make_ray_scene
     from us3d.ultragray import URBatch, UltragrayRenderConfig
 
     dev = torch.device("cuda:0")
@@ -263,12 +263,10 @@ def bench_ray(args):
     H, W = f0.height, f0.width
     cfg = UltragrayRenderConfig(image_width=W, image_height=H, width_mm=W * f0.delta_lat, near_mm=f0.near_mm,
                                 lateral_col_offset=f0.col_offset, ops_max_offset_mm=0.0)
-    specs = [FrameSpec(f.origin, f.axis_lat, f.axis_ax, f.axis_ele, H, W, cfg.pixel_spacing(), 0.0, f.near_mm,
+    specs = [FrameSpec(f.origin, f.axis_lat, f.axis_ax, f.axis_ele, H, W, cfg.pixel_spacing(), f.delta_ax, f.near_mm,
                        f.col_offset, 0) for f in frames]
-    geos = [ray_geometry(f, dev) for f in frames]
-    pix = torch.cat([g[0].reshape(-1, 3) for g in geos]).contiguous()
-    ro = torch.cat([g[1] for g in geos]).contiguous()
-    rd = torch.cat([g[2] for g in geos]).contiguous()
+    # the linear probe's samples and rays are evaluated inside the kernels from the frame table (no (P,3) array)
+    pix = ro = rd = None
     off = torch.zeros(F * W, device=dev)
     params = [t.to(dev).float().contiguous() for t in scene.params()]
     tparams = [t.to(dev).float().contiguous() for t in target.params()]
@@ -318,10 +316,10 @@ def bench_ray(args):
 
         def step_plain():
             for i, b in enumerate(plain):
-                ps, rs = slice(i * W * H, (i + 1) * W * H), slice(i * W, (i + 1) * W)
+                rs = slice(i * W, (i + 1) * W)
                 mm = b.map(*params)
-                img, ax = b.forward(mm, pix[ps], ro[rs], rd[rs], off[rs])
-                b.backward(mm, pix[ps], ro[rs], rd[rs], off[rs], ax, torch.sign(img - gt[i * W * H:(i + 1) * W * H]))
+                img, ax = b.forward(mm, None, None, None, off[rs])
+                b.backward(mm, None, None, None, off[rs], ax, torch.sign(img - gt[i * W * H:(i + 1) * W * H]))
 
         dt_p = timed(step_plain, args.plain_steps, 1)
         fps_p = F * args.plain_steps / dt_p

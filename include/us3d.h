@@ -199,7 +199,12 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
 
 /* B-mode forward: replaces ShaderForwardCUDA of ops/ultragray_renderer (src/shader/wrapper_shader.cu:7-58,
  * forward.cu:22-119). pixels: per frame (W,H,3) world mm (pixel (x,y) at ((x*H)+y)*3), concatenated;
- * ray_origins/ray_dirs [n_rays,3]; ray_offsets [n_rays] elevational OPS offsets (mm).
+ * ray_origins/ray_dirs [n_rays,3]; ray_offsets [n_rays] elevational OPS offsets (mm) or NULL for none.
+ * Each of pixels, ray_origins and ray_dirs may be NULL: the kernels then use the linear probe's geometry
+ * described by the frame -- sample (x,y) at origin + ((x - col_offset) delta_lat) a_lat +
+ * (near_mm + y delta_ax) a_ax, ray origin origin + ((x - col_offset) delta_lat) a_lat, direction a_ax --
+ * evaluated with the FP32 operations of the pipeline's grid (pipeline_god_file.py:292-333), so the results
+ * equal those of the explicit arrays that grid produces, without the (P,3) array in HBM.
  * image [n_pixels] in (H,W) row-major per frame; aux [n_pixels,4] = (T, E = num/(den+1e-6), den, 0) per
  * sample in the pixels' ray-major order (sample (x,y) at x*H+y of its frame), accumulated in FP64 and stored
  * as FP32; it is the backward's input, not an image. */
@@ -208,7 +213,7 @@ int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray
                     float* image, float* aux, void* stream);
 
 /* B-mode adjoint: replaces ShaderBackwardCUDA of ops/ultragray_renderer (wrapper_shader.cu:60-123,
- * backward.cu:26-325). Outputs (overwritten): d_means [G,3], d_scales [G,3], d_quats [G,4],
+ * backward.cu:26-325). Geometry arguments as for the forward (NULL = the frame's linear probe). Outputs (overwritten): d_means [G,3], d_scales [G,3], d_quats [G,4],
  * d_attenuations [G], d_sh [G,4]. */
 int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ray_origins,
                      const float* ray_dirs, const float* ray_offsets, const us3d_ur_settings* settings,

@@ -196,3 +196,41 @@ def test_ray_log_scales_are_activated_in_the_pack(dev):
     assert rel_l2(g_log[1].cpu().numpy(), want.cpu().numpy()) < 1e-5
     for a, b in zip(g_log[:1] + g_log[2:], g_lin[:1] + g_lin[2:]):
         assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-5
+
+
+def test_ray_linear_probe_geometry_in_kernel_equals_explicit_arrays(dev):
+    """pixels / ray_origins / ray_dirs = NULL: the kernels evaluate the linear probe's samples and rays from the
+    frame table with the FP32 operations of the pipeline's grid (pipeline_god_file.py:292-333). Image, aux and
+    every gradient equal those computed from the explicit arrays of that grid bit for bit (two frames, nonzero
+    elevational offsets)."""
+    import torch
+
+    from us3d.batch import CapacityHint, SweepSpec
+    from us3d.scenes import ray_geometry
+    from us3d.ultragray import URBatch
+    from tests.ur_helpers import ray_spec
+
+    scene, frames = small_ray_case(n_frames=2)
+    cfg = ray_cfg(frames[0])
+    specs = [ray_spec(f, cfg) for f in frames]
+    geos = [ray_geometry(s, dev) for s in specs]
+    pix = torch.cat([g[0].reshape(-1, 3) for g in geos]).contiguous()
+    ro = torch.cat([g[1] for g in geos]).contiguous()
+    rd = torch.cat([g[2] for g in geos]).contiguous()
+    W = sum(s.width for s in specs)
+    off = torch.linspace(-0.4, 0.4, W, device=dev)
+    params = [t.to(dev).float().contiguous() for t in scene.params()]
+    G = params[0].shape[0]
+    batch = URBatch(specs, [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
+    m = batch.map(*params)
+    img_e, aux_e = batch.forward(m, pix, ro, rd, off)
+    img_p, aux_p = batch.forward(m, None, None, None, off)
+    assert float(img_e.abs().max()) > 1e-3
+    assert torch.equal(img_e, img_p)
+    assert torch.equal(aux_e, aux_p)
+    dB = torch.from_numpy((np.random.default_rng(3).normal(size=img_e.numel()) * 1e-2).astype(np.float32)).to(dev)
+    g_e = batch.backward(m, pix, ro, rd, off, aux_e, dB)
+    g_p = batch.backward(m, None, None, None, off, aux_p, dB)
+    for a, b in zip(g_e, g_p):
+        assert float(a.abs().max()) > 0
+        assert torch.equal(a, b)

@@ -28,7 +28,7 @@ def ray_cfg(fr: FrameSpec, **kw) -> UltragrayRenderConfig:
 
 
 def ray_spec(fr: FrameSpec, cfg: UltragrayRenderConfig) -> FrameSpec:
-    return FrameSpec(fr.origin, fr.axis_lat, fr.axis_ax, fr.axis_ele, fr.height, fr.width, cfg.pixel_spacing(), 0.0,
+    return FrameSpec(fr.origin, fr.axis_lat, fr.axis_ax, fr.axis_ele, fr.height, fr.width, cfg.pixel_spacing(), fr.delta_ax,
                      fr.near_mm, fr.col_offset, fr.sweep)
 
 

@@ -238,14 +238,33 @@ __device__ __forceinline__ float echo_weight(const RayGauss& G, float px, float 
     return __expf(-0.5f * dot3(v0, v1, v2, dx, dy, dz));
 }
 
-__device__ __forceinline__ void ray_origin(const us3d_frame& fr, int ray, const float* __restrict__ ray_o,
+// Origin (with the elevational offset) and direction of scanline x of a frame. With ray_o == NULL the ray is
+// the linear probe's: origin + lateral * a_lat with lateral = (x - c) * d_lat, direction a_ax, evaluated with
+// the FP32 operations of NpzUltrasoundDataset._ray_origins/_ray_dirs (pipeline_god_file.py:320-333), so it
+// equals those tensors bit for bit.
+__device__ __forceinline__ void ray_origin(const us3d_frame& fr, int x, const float* __restrict__ ray_o,
                                            const float* __restrict__ ray_d, const float* __restrict__ ray_off,
                                            float ro[3], float rd[3]) {
+    const int ray = fr.ray_base + x;
     const float delta = ray_off ? ray_off[ray] : 0.0f;
+    const float lat = ((float)x - fr.col_offset) * fr.delta_lat;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        ro[k] = __builtin_fmaf(fr.axis_ele[k], delta, ray_o[3 * (int64_t)ray + k]);
-        rd[k] = ray_d[3 * (int64_t)ray + k];
+        const float o = ray_o ? ray_o[3 * (int64_t)ray + k] : fr.origin[k] + lat * fr.axis_lat[k];
+        ro[k] = __builtin_fmaf(fr.axis_ele[k], delta, o);
+        rd[k] = ray_d ? ray_d[3 * (int64_t)ray + k] : fr.axis_ax[k];
+    }
+}
+
+// Position of sample y of scanline x: from the (W, H, 3) pixel array, or with pixels == NULL the linear
+// probe's grid point (pixel_position, the pipeline's _pixel_positions grid permuted to scanline-major).
+__device__ __forceinline__ void ray_sample(const us3d_frame& fr, const float* __restrict__ pixels, int x, int y,
+                                           float& px, float& py, float& pz) {
+    if (pixels) {
+        const float* p = pixels + 3 * ((int64_t)fr.pixel_base + (int64_t)x * fr.height + y);
+        px = p[0]; py = p[1]; pz = p[2];
+    } else {
+        pixel_position(fr, nullptr, x, y, px, py, pz);
     }
 }
 
@@ -357,7 +376,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
     const int W = fr.width, H = fr.height;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     float ro[3], rd[3];
-    ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
+    ray_origin(fr, x, ray_o, ray_d, ray_off, ro, rd);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const bool window_mode = F.Rw > 0.0f && !F.local_only;
 
@@ -370,8 +389,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
             const int y = c0 + tid + k * kRayThreads;
             px[k] = py[k] = pz[k] = tpx[k] = 0.f;
             if (y < H) {
-                const float* p = pixels + 3 * ((int64_t)fr.pixel_base + (int64_t)x * H + y);
-                px[k] = p[0]; py[k] = p[1]; pz[k] = p[2];
+                ray_sample(fr, pixels, x, y, px[k], py[k], pz[k]);
                 tpx[k] = fmaxf(0.0f, dot3(px[k] - ro[0], py[k] - ro[1], pz[k] - ro[2], rd[0], rd[1], rd[2]));
             }
             s_tp[tid + k * kRayThreads] = tpx[k];
@@ -509,7 +527,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     const int x = t - fr.tile_base;
     const int W = fr.width, H = fr.height;
     float ro[3], rd[3];
-    ray_origin(fr, fr.ray_base + x, ray_o, ray_d, ray_off, ro, rd);
+    ray_origin(fr, x, ray_o, ray_d, ray_off, ro, rd);
     const int tid = threadIdx.x;
 
     // Stage samples [c0, c0 + n) of the ray: positions and depths, the per-sample adjoint coefficients, and the
@@ -519,7 +537,8 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
         for (int j = tid; j < n; j += kRayThreads) {
             const int y = c0 + j;
             const int64_t r = (int64_t)fr.pixel_base + (int64_t)x * H + y;  // ray-major sample index
-            const float* p = pixels + 3 * r;
+            float p[3];
+            ray_sample(fr, pixels, x, y, p[0], p[1], p[2]);
             const float tp = fmaxf(0.0f, dot3(p[0] - ro[0], p[1] - ro[1], p[2] - ro[2], rd[0], rd[1], rd[2]));
             const float gB = d_image[(int64_t)fr.pixel_base + (int64_t)y * W + x];
             float cT = 0.f, dNum = 0.f, dS = 0.f, E = 0.f;
@@ -876,7 +895,7 @@ int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
-    if (!pixels || !ray_origins || !ray_dirs || !image || !aux) return fail(US3D_ERR_ARGUMENT, "ur_forward: NULL array");
+    if (!image || !aux) return fail(US3D_ERR_ARGUMENT, "ur_forward: NULL array");
     {
         StageTimer timer_("ur_forward_kernel", as_stream(stream_));
         hipLaunchKernelGGL(ur_forward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
@@ -896,8 +915,7 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
-    if (!pixels || !ray_origins || !ray_dirs || !aux || !d_image || !d_means || !d_scales || !d_quats ||
-        !d_attenuations || !d_sh)
+    if (!aux || !d_image || !d_means || !d_scales || !d_quats || !d_attenuations || !d_sh)
         return fail(US3D_ERR_ARGUMENT, "ur_backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     {
