@@ -46,14 +46,17 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
-    ap.add_argument("--workload", choices=("slice", "ray", "multisweep"), default="slice",
+    ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan"), default="slice",
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
                          "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
                          "multisweep: 8 sweeps of uneven size in one captured training step, tile size from the "
-                         "autotuner")
+                         "autotuner; fan: --fan-frames frames of a 1000-frame fan-probe sweep over 2M Gaussians")
     ap.add_argument("--ms-scale", type=float, default=1.0,
                     help="multisweep: scale factor on the sweeps' Gaussian and frame counts")
     ap.add_argument("--ray-frames", type=int, default=128)
+    ap.add_argument("--fan-frames", type=int, default=100)
+    ap.add_argument("--fan-gaussians", type=int, default=2_000_000)
+    ap.add_argument("--fan-cpu-rows", type=int, default=2)
     ap.add_argument("--cpu-sample-rays", type=int, default=8)
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
                     help="print a markdown summary of a rocprofv3 output directory: MODE 'trace' (per-kernel "
@@ -355,6 +358,118 @@ def bench_ray(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_fan(args):
+    """Config 5: 2M Gaussians in a 200 x 150 x 150 mm box, a 1000-frame sweep of a 60-degree fan probe (640 beams x
+    480 depth samples over 40-160 mm). One step = one captured training step over --fan-frames consecutive
+    frames of the sweep (pixel grid evaluated in the kernels); reports frames/s and peak HBM."""
+    import torch
+
+    from us3d import _lib
+    from us3d.batch import CapacityHint, read_stats
+    from us3d.scenes import config5, fan_pixels
+    from us3d.train_step import SliceSplatTrainer
+    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
+
+    dev = torch.device("cuda:0")
+    scene, target, sweep = config5(n_gauss=args.fan_gaussians)
+    F = args.fan_frames
+    f0 = len(sweep) // 2 - F // 2
+    frames = sweep[f0:f0 + F]
+    G = scene.means.shape[0]
+    H, W = frames[0].height, frames[0].width
+    P = F * H * W
+    cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=frames[0].delta_lat,
+                                 axial_spacing_mm_per_pix=frames[0].delta_ax, axial_near_mm=frames[0].near_mm,
+                                 fixed_bg_alpha=1e-6, tile_size=16)
+    torch.zeros(1, device=dev)  # creates the context, so the peak counter can be reset
+    torch.cuda.reset_peak_memory_stats(dev)
+    with torch.no_grad():
+        gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
+                           frames, cfg, hint=CapacityHint())[0].clone()
+    logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
+    init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager)
+    dt = timed(trainer.step, args.steps, args.warmup)
+    if trainer.overflowed():
+        raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
+    fps = F * args.steps / dt
+    peak = torch.cuda.max_memory_allocated(dev)
+    print(f"[bench] fan batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s, peak HBM "
+          f"{peak / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+    del trainer
+    torch.cuda.empty_cache()
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False)
+    tt.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    tt.step()
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    stages = _lib.stage_timing_read()
+    st = read_stats(tt.workspace)
+    off = tt.batch.layout_offsets(tt.k_capacity)
+    lay = tt.batch.layout
+    g_vis = int((tt.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
+    K = int(st.n_instances)
+    del tt
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    launch_bytes = {"ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis, "ug_forward_kernel": 36 * K + 16 * P}
+    alg = launch_bytes.get(dom)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import fallback as ofb
+
+        torch.set_num_threads(os.cpu_count() or 1)
+        fr = frames[F // 2]
+        rows = args.fan_cpu_rows
+        pix = fan_pixels(fr).reshape(H, W, 3)[H // 2 - rows // 2:H // 2 - rows // 2 + rows].reshape(-1, 3)
+        ts = [t.clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+        with torch.no_grad():
+            ref, _ = ofb.render(target.means, target.tril, target.opacity, target.intensity, pix, rows, W)
+        t0 = time.perf_counter()
+        img, _ = ofb.render(*ts, pix, rows, W)
+        (img - ref).abs().mean().backward()
+        dtc = time.perf_counter() - t0
+        cpu = {"value": rows * W / (H * W) / dtc, "unit": "frames/s", "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"{rows} rows x {W} pixels of fan frame {f0 + F // 2} against all {G} Gaussians, forward + "
+                         f"autograd backward of the restated PyTorch fallback renderer, {dtc:.1f} s "
+                         f"({rows * W * G / dtc / 1e6:.0f} M pair evaluations/s); frames/s = sampled fraction of a "
+                         f"{W}x{H} frame / time"}
+    line = {
+        "metric": "fwd+bwd frames/s of a fan-probe batch in one captured training step on 1 GPU; % of HBM roofline",
+        "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 200x150x150 mm box, frames "
+                               f"{f0}..{f0 + F - 1} of a 1000-frame 60-degree fan sweep ({W} beams x {H} samples, "
+                               f"40-160 mm), L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} step, "
+                               f"tile 16, pixel grid evaluated in the kernels",
+                   "frames": F, "gaussians": G, "pixels": P, "instances": K, "visible_pairs": g_vis},
+        "peak_hbm_bytes": int(peak),
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
+                     "stage_ms_per_launch": stage_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_multisweep(args):
     """8 sweeps (Gaussians log-uniform in 50k..500k, 32..400 frames, frame size 256x256 / 512x512 / 640x480) in
     ONE hipGraph-captured training step; the tile size is picked by the autotuner on a frame sample first."""
@@ -463,6 +578,9 @@ def main():
         return
     if args.workload == "multisweep":
         bench_multisweep(args)
+        return
+    if args.workload == "fan":
+        bench_fan(args)
         return
     import torch
 

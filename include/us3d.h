@@ -38,8 +38,13 @@ extern "C" {
 #define US3D_ERR_HIP 2
 #define US3D_ERR_CAPACITY 3
 
-/* One probe frame. 96 bytes, all fields 4 bytes, no padding. Axis vectors are used as given
- * (the training loop normalises them). */
+/* One probe frame. 100 bytes, all fields 4 bytes, no padding. Axis vectors are used as given
+ * (the training loop normalises them).
+ * geometry 0, linear probe: pixel (x,y) at origin + ((x - col_offset) delta_lat) a_lat + (near_mm + y delta_ax) a_ax.
+ * geometry 1, fan (sector) probe, UltraGauss only: origin is the apex, delta_lat the angular step (radians per
+ * column), col_offset the column at angle 0, near_mm the radius of row 0 and delta_ax the radial step; pixel
+ * (x,y) at origin + r sin(t) a_lat + r cos(t) a_ax with t = (x - col_offset) delta_lat, r = near_mm + y delta_ax.
+ * Explicit pixel positions of a fan frame must lie on that grid to within half a pixel. */
 typedef struct us3d_frame {
     float origin[3];        /* probe origin, world mm */
     float axis_lat[3];      /* lateral unit axis */
@@ -57,6 +62,7 @@ typedef struct us3d_frame {
     int32_t pixel_base;     /* first pixel of this frame in the concatenated pixel arrays */
     int32_t sweep;          /* index into the sweep table */
     int32_t ray_base;       /* first ray of this frame in concatenated per-ray arrays (UltraG-Ray) */
+    int32_t geometry;       /* 0: linear probe grid; 1: fan (see above) */
 } us3d_frame;
 
 /* A sweep: a set of frames rendered from one Gaussian set. Gaussians [g_begin, g_begin+g_count) of the

@@ -94,13 +94,17 @@ def test_plan_layout_numbering():
 
 def test_frame_table_bits():
     frames = [_frame(), _frame(32, 40)]
+    frames[1].geometry = 1
     lay = plan_layout(frames, [SweepSpec(0, 10)], 10, 16)
     tab = frame_table(frames, lay, torch.device("cpu"))
-    assert tab.shape == (2, 24) and tab.dtype == torch.int32
+    n = ctypes.sizeof(_lib.Frame)
+    assert n == 100 and tab.shape == (2, n // 4) and tab.dtype == torch.int32
     buf = tab.numpy().tobytes()
-    f1 = _lib.Frame.from_buffer_copy(buf[96:192])
+    f0 = _lib.Frame.from_buffer_copy(buf[:n])
+    f1 = _lib.Frame.from_buffer_copy(buf[n:2 * n])
     assert (f1.height, f1.width, f1.tiles_x, f1.tiles_y, f1.tile_base, f1.pixel_base) == (32, 40, 3, 2, 12, 64 * 48)
     assert abs(f1.delta_ax - 0.3) < 1e-7 and abs(f1.near_mm - 1.5) < 1e-7 and list(f1.origin) == [1.0, 2.0, 3.0]
+    assert (f0.geometry, f1.geometry) == (0, 1)
 
 
 def test_render_without_gpu_fails_loudly():
@@ -146,3 +150,13 @@ def test_autotune_sample_renumbers_sweeps_and_slices_targets():
     assert [f.sweep for f in sub] == [0, 0, 1]
     assert sub_sweeps == [sweeps[0], sweeps[1]]
     assert torch.equal(tgt, torch.cat([torch.full((n[i],), float(i)) for i in picked]))
+
+
+def test_ray_renderer_rejects_fan_frames():
+    """Fan frames (geometry 1) are a slice-splat geometry; the ray renderer refuses them before any device work."""
+    from us3d.ultragray import URBatch
+
+    fr = _frame()
+    fr.geometry = 1
+    with pytest.raises(ValueError):
+        URBatch([fr], [SweepSpec(0, 10)], 10, _lib.URSettings(), torch.device("cpu"))

@@ -65,10 +65,11 @@ __device__ __forceinline__ float quad_form(float l11, float l12, float l13, floa
     return __builtin_fmaf(y2, y2, __builtin_fmaf(y1, y1, y0 * y0));
 }
 
-// World position of pixel (ix, iy) of a frame. With pixels == NULL the position is the linear-probe grid of
-// NpzUltrasoundDataset._pixel_positions (pipeline_god_file.py:299-306), evaluated with the same FP32
-// operations in the same order (lateral = (u - c) * d_lat; axial = near + v * d_ax;
-// p = (o + lateral * a_lat) + axial * a_ax), so it equals that tensor bit for bit.
+// World position of pixel (ix, iy) of a frame. With pixels == NULL the position is the frame's grid. Linear
+// probe: the grid of NpzUltrasoundDataset._pixel_positions (pipeline_god_file.py:299-306), evaluated with the
+// same FP32 operations in the same order (lateral = (u - c) * d_lat; axial = near + v * d_ax;
+// p = (o + lateral * a_lat) + axial * a_ax), so it equals that tensor bit for bit. Fan probe: angle
+// t = (u - c) * d_lat, radius r = near + v * d_ax, lateral = r sin t, axial = r cos t, then the same sum.
 __device__ __forceinline__ void pixel_position(const us3d_frame& fr, const float* __restrict__ pixels, int ix, int iy,
                                                float& x, float& y, float& z) {
     if (pixels) {
@@ -77,8 +78,15 @@ __device__ __forceinline__ void pixel_position(const us3d_frame& fr, const float
         y = pixels[3 * p + 1];
         z = pixels[3 * p + 2];
     } else {
-        const float lat = ((float)ix - fr.col_offset) * fr.delta_lat;
-        const float ax = fr.near_mm + (float)iy * fr.delta_ax;
+        float lat = ((float)ix - fr.col_offset) * fr.delta_lat;
+        float ax = fr.near_mm + (float)iy * fr.delta_ax;
+        if (fr.geometry == 1) {
+            float s, c;
+            sincosf(lat, &s, &c);
+            const float r = ax;
+            lat = r * s;
+            ax = r * c;
+        }
         x = (fr.origin[0] + lat * fr.axis_lat[0]) + ax * fr.axis_ax[0];
         y = (fr.origin[1] + lat * fr.axis_lat[1]) + ax * fr.axis_ax[1];
         z = (fr.origin[2] + lat * fr.axis_lat[2]) + ax * fr.axis_ax[2];

@@ -94,6 +94,37 @@ __device__ __forceinline__ uint32_t depth_key(float v_pix) {
     return (uint32_t)x;
 }
 
+// Tile rectangle [fx0, fx1) x [fy0, fy1) (unclipped, as floats) and radial row coordinate v of a Gaussian in a
+// fan frame, from the polar extent of its in-plane marginal box [lat +- r_lat] x [ax +- r_ax] about the apex:
+// radius from the box's distance to the apex to its farthest corner; angle t = atan2(lateral, axial) from the
+// box's extreme corners (a convex region clear of the apex and of the half-line behind it has its angular
+// extremes at vertices), or every angle when the box holds the apex or crosses that half-line. Any pixel
+// within the box lies in the rectangle; one pixel of margin on each side covers the rounding of the grid.
+__device__ __forceinline__ void fan_tile_bounds(const us3d_frame& fr, float lat, float ax, float r_lat, float r_ax,
+                                                float tsf, float& fx0, float& fy0, float& fx1, float& fy1,
+                                                float& v) {
+    const float L0 = lat - r_lat, L1 = lat + r_lat, A0 = ax - r_ax, A1 = ax + r_ax;
+    const float dl = L0 > 0.0f ? L0 : (L1 < 0.0f ? -L1 : 0.0f);
+    const float da = A0 > 0.0f ? A0 : (A1 < 0.0f ? -A1 : 0.0f);
+    const float rmin = sqrtf(dl * dl + da * da);
+    const float rmax = sqrtf(fmaxf(L0 * L0, L1 * L1) + fmaxf(A0 * A0, A1 * A1));
+    float t0 = -3.14159274f, t1 = 3.14159274f;
+    const bool apex = dl == 0.0f && da == 0.0f;
+    const bool behind = A0 < 0.0f && L0 <= 0.0f && L1 >= 0.0f;
+    if (!apex && !behind) {
+        const float c0 = atan2f(L0, A0), c1 = atan2f(L0, A1), c2 = atan2f(L1, A0), c3 = atan2f(L1, A1);
+        t0 = fminf(fminf(c0, c1), fminf(c2, c3));
+        t1 = fmaxf(fmaxf(c0, c1), fmaxf(c2, c3));
+    }
+    const float u0 = t0 / fr.delta_lat + fr.col_offset, u1 = t1 / fr.delta_lat + fr.col_offset;
+    const float v0 = (rmin - fr.near_mm) / fr.delta_ax, v1 = (rmax - fr.near_mm) / fr.delta_ax;
+    fx0 = floorf((u0 - 1.0f) / tsf);
+    fx1 = ceilf((u1 + 1.0f) / tsf);
+    fy0 = floorf((v0 - 1.0f) / tsf);
+    fy1 = ceilf((v1 + 1.0f) / tsf);
+    v = (sqrtf(lat * lat + ax * ax) - fr.near_mm) / fr.delta_ax;
+}
+
 // Per (frame, Gaussian) pair: probe-frame projection, chi2 marginal radii, elevation cull, tile rectangle
 // and depth key (compute_probe_bbox_radii + kernel_project_gauss_and_compute_tile_rects,
 // ops/ultragauss_renderer/src/mapper/core.cu:9-116). Non-finite projections are culled.
@@ -130,13 +161,20 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
             const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
             const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
             if (fabsf(ele) <= r_ele) {
-                const float u = lat / fr.delta_lat + fr.col_offset;
-                const float v = (ax - fr.near_mm) / fr.delta_ax;
-                const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
-                const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
                 const float tsf = (float)ts;
-                const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
-                const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
+                float fx0, fy0, fx1, fy1, v;
+                if (fr.geometry == 1) {
+                    fan_tile_bounds(fr, lat, ax, r_lat, r_ax, tsf, fx0, fy0, fx1, fy1, v);
+                } else {
+                    const float u = lat / fr.delta_lat + fr.col_offset;
+                    v = (ax - fr.near_mm) / fr.delta_ax;
+                    const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+                    const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+                    fx0 = floorf((u - ru) / tsf);
+                    fy0 = floorf((v - rv) / tsf);
+                    fx1 = ceilf((u + ru) / tsf);
+                    fy1 = ceilf((v + rv) / tsf);
+                }
                 if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
                     const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
                     const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
@@ -181,48 +219,63 @@ __global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* tot
     off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
 }
 
-// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket. One workgroup per
-// frame walks the frame's region of vis_list (one entry per Gaussian of its sweep, -1 where the pair is culled)
-// kEmitPerThread entries per thread at a time, gathers the visible pairs into an LDS queue (ballot compaction)
-// and expands them kEmitThreads at a time with the wave-cooperative instance walk. The frame's bucket cursors
-// live in LDS, seeded with the bucket offsets, so a slot costs one LDS atomic instead of a returning atomic in
-// L2; a frame with more than kEmitLdsTiles tiles takes its slots from the global cursors instead. Slot order
-// inside a bucket is arbitrary; the bucket sort by the unique keys makes the final order exact. Workgroups go
-// to the XCDs round-robin, so all buckets of a frame are written through one XCD's L2.
+// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket. Workgroup w serves
+// frame w / (bands * parts), band b and part p of it (w % (bands * parts) = b * parts + p): the tile rows
+// [b * rows, (b + 1) * rows), rows = ceil(tiles_y / bands), and the chunks p, p + parts, ... of the frame's
+// region of vis_list (one entry per Gaussian of its sweep, -1 where the pair is culled). bands and parts exceed
+// 1 only when the batch has few frames, so the launch still fills the chip: bands split the frame's buckets
+// (every band reads the whole region), parts split the region (their buckets are shared). A workgroup reads
+// kEmitPerThread entries per thread at a time, gathers the visible pairs whose rectangle meets its rows into
+// an LDS queue (ballot compaction) and expands them, clipped to its rows, kEmitThreads at a time with the
+// wave-cooperative instance walk; the emit index stays that of the full rectangle. With one part the band's
+// bucket cursors live in LDS, seeded with the bucket offsets, so a slot costs one LDS atomic instead of a
+// returning atomic in L2; with several parts, or more than kEmitLdsTiles tiles in the band, slots come from
+// the global cursors. Slot order inside a bucket is arbitrary; the bucket sort by the unique keys makes the
+// final order exact.
 constexpr int kEmitThreads = 1024;
 constexpr int kEmitPerThread = 8;
 constexpr int kEmitChunk = kEmitThreads * kEmitPerThread;
 constexpr int kEmitQueue = kEmitChunk + kEmitThreads;
 constexpr int kEmitLdsTiles = 8192;
+constexpr int kEmitTargetGroups = 256;
 
 __global__ __launch_bounds__(kEmitThreads) void ug_emit_kernel(
     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
     const int32_t* __restrict__ vis_list, const int32_t* __restrict__ pair_count,
     const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
     const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor, uint64_t* __restrict__ seg_key,
-    int32_t* __restrict__ seg_val, const us3d_stats* stats) {
-    __shared__ int32_t s_cur[kEmitLdsTiles];  // next free slot of each bucket of the frame
+    int32_t* __restrict__ seg_val, int bands, int parts, const us3d_stats* stats) {
+    __shared__ int32_t s_cur[kEmitLdsTiles];  // next free slot of each bucket of the band
     __shared__ int32_t s_q[kEmitQueue];       // visible pairs not yet expanded
     __shared__ int s_n;
     if (stats->overflow) return;
-    const int f = blockIdx.x;
+    const int f = blockIdx.x / (bands * parts), band = (blockIdx.x % (bands * parts)) / parts;
+    const int part = blockIdx.x % parts;
     const us3d_frame& fr = frames[f];
+    const int rows = (fr.tiles_y + bands - 1) / bands;
+    const int y_lo = band * rows, y_hi = min(fr.tiles_y, y_lo + rows);
+    if (y_lo >= y_hi) return;  // workgroup-uniform: a band past the frame's last tile row
     const us3d_sweep sw = sweeps[fr.sweep];
     const int32_t* list = vis_list + sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count;
-    const int n_tiles = fr.tiles_x * fr.tiles_y;
-    const bool local = n_tiles <= kEmitLdsTiles;  // workgroup-uniform
+    const int band_base = fr.tile_base + y_lo * fr.tiles_x;  // first tile of the band
+    const int n_tiles = (y_hi - y_lo) * fr.tiles_x;
+    const bool local = parts == 1 && n_tiles <= kEmitLdsTiles;  // workgroup-uniform
     const int tid = threadIdx.x, lane = tid & (kWave - 1);
     if (local)
-        for (int i = tid; i < n_tiles; i += kEmitThreads) s_cur[i] = tile_off[fr.tile_base + i];
+        for (int i = tid; i < n_tiles; i += kEmitThreads) s_cur[i] = tile_off[band_base + i];
     if (tid == 0) s_n = 0;
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-    for (int c0 = 0; c0 < sw.g_count; c0 += kEmitChunk) {
+    for (int c0 = part * kEmitChunk; c0 < sw.g_count; c0 += parts * kEmitChunk) {
         int pr[kEmitPerThread];
 #pragma unroll
         for (int k = 0; k < kEmitPerThread; ++k) {
             const int idx = c0 + k * kEmitThreads + tid;
             pr[k] = idx < sw.g_count ? list[idx] : -1;
+            if (bands > 1 && pr[k] >= 0) {  // keep only pairs whose rectangle meets the band's rows
+                const int4 r = pair_rect[pr[k]];
+                if (r.w <= y_lo || r.y >= y_hi) pr[k] = -1;
+            }
         }
 #pragma unroll
         for (int k = 0; k < kEmitPerThread; ++k) {
@@ -234,7 +287,7 @@ __global__ __launch_bounds__(kEmitThreads) void ug_emit_kernel(
         }
         __syncthreads();
         int n = s_n;  // workgroup-uniform
-        const bool last = c0 + kEmitChunk >= sw.g_count;
+        const bool last = c0 + parts * kEmitChunk >= sw.g_count;
         while (n >= kEmitThreads || (last && n > 0)) {
             const int take = min(n, kEmitThreads);
             n -= take;  // expand the queue's last `take` entries, so the rest stays in place
@@ -243,15 +296,17 @@ __global__ __launch_bounds__(kEmitThreads) void ug_emit_kernel(
             int4 rect = make_int4(0, 0, 0, 0);
             uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
             if (pair >= 0) {
-                count = pair_count[pair];
-                rect = pair_rect[pair];
+                const int4 full = pair_rect[pair];
+                rect = make_int4(full.x, max(full.y, y_lo), full.z, min(full.w, y_hi));  // clipped to the band
+                count = (rect.z - rect.x) * (rect.w - rect.y);
                 pay[0] = (uint32_t)(sw.g_begin + (int)(pair - sw.pair_base) / sw.f_count);
                 pay[1] = pair_dk[pair];
-                pay[2] = (uint32_t)pair_off[pair];
+                // instance (x, y) is number (y - y0) * width + (x - x0) of the full rectangle
+                pay[2] = (uint32_t)(pair_off[pair] + (rect.y - full.y) * (full.z - full.x));
             }
             for_each_wave_instance<3>(count, rect, fr.tile_base, fr.tiles_x, pay,
                                       [&](const uint32_t* o, int jj, int tile) {
-                                          const int slot = local ? atomicAdd(&s_cur[tile - fr.tile_base], 1)
+                                          const int slot = local ? atomicAdd(&s_cur[tile - band_base], 1)
                                                                  : tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
                                           seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
                                           seg_val[slot] = (int32_t)o[2] + jj;
@@ -971,10 +1026,17 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     if (np > 0) {
         {
             StageTimer timer_("ug_emit_kernel", stream);
-            hipLaunchKernelGGL(ug_emit_kernel, dim3(plan->n_frames), dim3(kEmitThreads), 0, stream, plan->frames,
+            // enough workgroups for the chip when the batch has few frames: bands (LDS cursors) down to 16
+            // frames, below that parts of the visible list (global cursors), since every band rereads the list
+            const int split = std::min(32, (kEmitTargetGroups + plan->n_frames - 1) / plan->n_frames);
+            const int bands = plan->n_frames >= 16 ? std::max(1, split) : 1;
+            const int parts = plan->n_frames >= 16 ? 1 : std::max(1, split);
+            hipLaunchKernelGGL(ug_emit_kernel, dim3((unsigned)(plan->n_frames * bands * parts)), dim3(kEmitThreads), 0,
+                               stream, plan->frames,
                                plan->sweeps, at<int32_t>(plan, L.vis_list), pair_count, pair_off, pair_rect, pair_dk,
                                tile_off, tile_cursor,
-                               at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), (const us3d_stats*)stats);
+                               at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), bands, parts,
+                               (const us3d_stats*)stats);
         }
         if ((rc = check_launch("ug_emit_kernel"))) return rc;
     }

@@ -39,6 +39,7 @@ class Frame(ctypes.Structure):
         ("pixel_base", ctypes.c_int32),
         ("sweep", ctypes.c_int32),
         ("ray_base", ctypes.c_int32),
+        ("geometry", ctypes.c_int32),
     ]
 
 

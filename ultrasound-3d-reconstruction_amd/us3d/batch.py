@@ -34,6 +34,9 @@ class FrameSpec:
     near_mm: float
     col_offset: float
     sweep: int = 0
+    # 0: linear probe grid; 1: fan probe (UltraGauss only): origin = apex, delta_lat = radians per column,
+    # col_offset = column at angle 0, near_mm = radius of row 0, delta_ax = radial step
+    geometry: int = 0
 
 
 @dataclass
@@ -106,7 +109,7 @@ def plan_layout(frames: Sequence[FrameSpec], sweeps: Sequence[SweepSpec], n_gaus
 
 
 def frame_table(frames: Sequence[FrameSpec], lay: BatchLayout, device: torch.device) -> torch.Tensor:
-    """(n_frames, 24) 4-byte words laid out as us3d_frame, assembled on `device` without a host sync."""
+    """(n_frames, 25) 4-byte words laid out as us3d_frame, assembled on `device` without a host sync."""
     f32 = torch.float32
     vec = torch.stack(
         [torch.cat([f.origin.reshape(3), f.axis_lat.reshape(3), f.axis_ax.reshape(3), f.axis_ele.reshape(3)]).to(
@@ -115,7 +118,7 @@ def frame_table(frames: Sequence[FrameSpec], lay: BatchLayout, device: torch.dev
     scal = torch.tensor([[f.delta_lat, f.delta_ax, f.near_mm, f.col_offset] for f in frames], dtype=f32)
     ints = np.array(
         [[f.height, f.width, lay.tiles_x[i], lay.tiles_y[i], lay.frame_tile_base[i], lay.frame_pixel_base[i], f.sweep,
-          lay.frame_ray_base[i]] for i, f in enumerate(frames)],
+          lay.frame_ray_base[i], int(f.geometry)] for i, f in enumerate(frames)],
         dtype=np.int32,
     )
     # concatenate as raw 32-bit words (int32 views) so the integer fields keep their bits exactly

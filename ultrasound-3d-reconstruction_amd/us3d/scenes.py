@@ -139,6 +139,45 @@ def linear_pixels(frame: FrameSpec, device=None) -> torch.Tensor:
     return pix.reshape(-1, 3).contiguous()
 
 
+def fan_pixels(frame: FrameSpec, device=None) -> torch.Tensor:
+    """(H*W, 3) pixel grid of a fan frame (geometry 1): angle t = (u - c) * d_t, radius r = near + v * d_r,
+    p = apex + r sin(t) a_lat + r cos(t) a_ax -- the grid the kernels evaluate for such frames when no pixel
+    array is given, up to the last-bit rounding of sin/cos."""
+    dev = device or frame.origin.device
+    u = torch.arange(frame.width, device=dev, dtype=torch.float32)
+    v = torch.arange(frame.height, device=dev, dtype=torch.float32)
+    t = (u - float(frame.col_offset)) * float(frame.delta_lat)
+    r = float(frame.near_mm) + v * float(frame.delta_ax)
+    rr, tt = torch.meshgrid(r, t, indexing="ij")
+    lat_mm, ax_mm = rr * torch.sin(tt), rr * torch.cos(tt)
+    o, lat, ax = (x.to(dev, torch.float32) for x in (frame.origin, frame.axis_lat, frame.axis_ax))
+    pix = o.view(1, 1, 3) + lat_mm[..., None] * lat.view(1, 1, 3) + ax_mm[..., None] * ax.view(1, 1, 3)
+    return pix.reshape(-1, 3).contiguous()
+
+
+def fan_sweep(n_frames: int, H: int, W: int, fan_deg: float, r0: float, r1: float, box_mm, length_mm: float,
+              seed: int, jitter_mm: float = 0.1, rot_sigma_deg: float = 1.0) -> List[FrameSpec]:
+    """Freehand sweep of a fan (sector) probe: W beams over fan_deg, H depth samples over [r0, r1) mm from the
+    apex; the apex rides above the box's y = 0 face so the sector's centre line crosses the box's middle, and
+    moves along elevation (z) with jitter and small cumulative rotations."""
+    rng = np.random.default_rng(seed)
+    bx, by, bz = (float(b) for b in box_mm)
+    pos = np.array([bx / 2, by / 2 - (r0 + r1) / 2, bz / 2 - length_mm / 2])
+    R = np.eye(3)
+    step = length_mm / n_frames
+    frames = []
+    for _ in range(n_frames):
+        f = _frame(pos.copy(), R.copy(), H, W, math.radians(fan_deg) / W, (r1 - r0) / H, near=r0)
+        f.geometry = 1
+        frames.append(f)
+        pos = pos + R[:, 2] * step + rng.normal(0.0, jitter_mm, 3)
+        for a in range(3):
+            R = R @ _rot(np.eye(3)[a], math.radians(rng.normal(0.0, rot_sigma_deg)))
+        u_, _, vt = np.linalg.svd(R)
+        R = u_ @ vt
+    return frames
+
+
 # ---------------------------------------------------------------- benchmark workloads
 
 def config1(device="cpu"):
@@ -197,6 +236,17 @@ def config4_specs(seed: int = 4, n_sweeps: int = 8, scale: float = 1.0):
 def config4(seed: int = 4, scale: float = 1.0):
     """8 sweeps of uneven size (config4_specs) with config-2 Gaussian and pose distributions."""
     return multi_sweep(config4_specs(seed, scale=scale), seed=seed)
+
+
+def config5(n_frames: int = 1000, n_gauss: int = 2_000_000, H: int = 480, W: int = 640):
+    """Fan-probe workload: Gaussians (config-2 distributions) in a 200 x 150 x 150 mm box; n_frames fan frames of
+    W beams over 60 degrees and H depth samples over 40-160 mm, swept 150 mm along elevation; target scene from
+    seed 1."""
+    box = [200.0, 150.0, 150.0]
+    scene = make_scene(n_gauss, 150.0, seed=0, box_mm=box)
+    target = make_scene(n_gauss, 150.0, seed=1, box_mm=box)
+    frames = fan_sweep(n_frames, H, W, 60.0, 40.0, 160.0, box, 150.0, seed=2)
+    return scene, target, frames
 
 
 # ---------------------------------------------------------------- UltraG-Ray scenes and scanline geometry

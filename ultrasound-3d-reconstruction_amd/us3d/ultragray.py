@@ -130,6 +130,8 @@ class URBatch:
 
     def __init__(self, frames: Sequence[FrameSpec], sweeps: Sequence[SweepSpec], n_gauss: int,
                  settings: _lib.URSettings, device: torch.device, hint: Optional[CapacityHint] = None):
+        if any(int(f.geometry) != 0 for f in frames):
+            raise ValueError("the UltraG-Ray renderer takes linear-probe frames only (geometry 0)")
         self.settings = settings
         self.device = device
         self.layout = plan_layout(frames, sweeps, n_gauss, None)
