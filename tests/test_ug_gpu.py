@@ -346,3 +346,45 @@ def test_fan_frames_match_brute_force_oracle(dev, semantics):
         assert np.abs(r).max() > 0, name
         err = rel_l2(t.grad.detach().cpu().numpy(), r)
         assert err < GRAD_TOL, f"{semantics} {name}: rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("n_frames", [1, 20, 260])
+def test_emit_split_modes_bitexact(dev, n_frames):
+    """The emit splits a batch by its frame count: parts of the visible list with global cursors below 16 frames,
+    bands of tile rows with LDS cursors below 256, one workgroup per frame beyond. In every mode the sorted keys
+    and tile ranges equal the oracle's for sampled frames, and every instance's emit index is a distinct slot of
+    its pair's range whose row-major position in the pair's tile rectangle is exactly the tile holding it."""
+    scene = make_scene(2000, 64.0, seed=13)
+    frames = straight_sweep(n_frames, 128, 48, 0.3, 64.0, 40.0, 5.0, seed=14)
+    cfg = cfg_for(frames[0])
+    out = gpu_render(scene, frames, cfg, dev, pixels="procedural")
+    b = out["batch"]
+    m = b.map(*[t.to(dev) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)])
+    views = b.sorted_view(m)
+    for f in sorted({0, n_frames // 2, n_frames - 1}):
+        _g, _p, mr, _l = oracle_frame(scene, frames[f], oug.Semantics())
+        np.testing.assert_array_equal(views[f]["keys"].cpu().numpy().astype(np.uint64), mr.keys)
+        np.testing.assert_array_equal(views[f]["values"].cpu().numpy(), mr.vals)
+        np.testing.assert_array_equal(views[f]["tile_start"].cpu().numpy(), mr.tile_start)
+        np.testing.assert_array_equal(views[f]["tile_end"].cpu().numpy(), mr.tile_end)
+    lay = b.layout
+    off = b.layout_offsets(m.k_capacity)
+    ws = m.workspace.cpu()
+    K = int(ws[:8].view(torch.int64)[0])
+    key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64).numpy()
+    val = ws[off.seg_val: off.seg_val + 4 * K].view(torch.int32).numpy().astype(np.int64)
+    tile_off = ws[off.tile_off: off.tile_off + 4 * (lay.n_tiles + 1)].view(torch.int32).numpy().astype(np.int64)
+    pair_off = ws[off.pair_off: off.pair_off + 4 * (lay.n_pairs + 1)].view(torch.int32).numpy().astype(np.int64)
+    rect = ws[off.pair_rect: off.pair_rect + 16 * lay.n_pairs].view(torch.int32).numpy().reshape(-1, 4)
+    assert K > 300 * min(n_frames, 3) and np.array_equal(np.sort(val), np.arange(K))
+    tile = np.repeat(np.arange(lay.n_tiles), np.diff(tile_off))  # global tile of each instance
+    frame = np.searchsorted(np.asarray(lay.frame_tile_base), tile, side="right") - 1
+    g = key & 0xFFFFFFFF
+    pair = g * n_frames + frame  # one sweep: pairs are numbered Gaussian-major
+    j = val - pair_off[pair]
+    r = rect[pair]
+    w = r[:, 2] - r[:, 0]
+    assert np.all((j >= 0) & (j < w * (r[:, 3] - r[:, 1])))
+    tiles_x = np.asarray(lay.tiles_x)[frame]
+    local = (r[:, 1] + j // w) * tiles_x + r[:, 0] + j % w
+    np.testing.assert_array_equal(local, tile - np.asarray(lay.frame_tile_base)[frame])
