@@ -82,7 +82,7 @@ typedef struct us3d_stats {
     int64_t max_segment;    /* longest per-tile / per-ray list */
     int32_t overflow;       /* 1 if n_instances > k_capacity (outputs are then invalid) */
     int32_t n_sorted_big;   /* segments that took the multi-chunk merge path */
-    int64_t reserved[5];
+    int64_t reserved[5];    /* mapper scratch (reserved[0]: tile total); zeroed by every map call */
 } us3d_stats;
 
 /* A batch: frames (possibly from several sweeps) that are mapped, rendered and differentiated together.

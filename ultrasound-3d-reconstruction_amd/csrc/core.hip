@@ -179,17 +179,18 @@ int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* t
 
 // ---------------------------------------------------------------- per-segment sort
 // Segments of up to kWaveSortCapMid keys are sorted by one wavefront each, four per workgroup: a bitonic
-// network over registers (up to 16 keys per lane) with cross-lane shuffles, no LDS. Longer segments get
-// one workgroup each: kSortCap-key chunks are bitonic-sorted in LDS and then merged in global memory
-// (ping-pong with the tmp arrays): with unique keys, an element's merged position is its index in its run
-// plus the number of keys of the partner run that are smaller (binary search), so a merge pass is fully
-// parallel.
+// network over registers (up to 16 keys per lane) with cross-lane shuffles, no LDS. Longer segments get one
+// workgroup each (a workgroup examines kLongSortSegments consecutive segments and sorts the long ones): runs of kWaveSortCap keys are sorted by the wavefronts in registers the same way and joined by a
+// bitonic merge whose long-distance stages run in LDS and the rest in registers (kMergeCap keys); beyond
+// kMergeCap, chunks sorted that way are merged in global memory (ping-pong with the tmp arrays), where an
+// element's place is its index in its run plus its rank in the partner run, so a pass is fully parallel.
 
 constexpr int kSortThreads = 256;
-constexpr int kSortCap = 2048;
-constexpr int kWaveSortCap = 512;     // segments sorted with <= 8 keys per lane
-constexpr int kWaveSortCapMid = 1024; // segments sorted with 16 keys per lane (a kernel of their own)
+constexpr int kWaveSortCap = 512;      // segments sorted by one wavefront, <= 8 keys per lane
+constexpr int kWaveSortCapMid = 1024;  // segments sorted by one wavefront with 16 keys per lane
+constexpr int kMergeCap = 4096;        // segments (and chunks) sorted in one workgroup's LDS
 constexpr int kWaveSortWaves = 4;
+constexpr int kLongSortSegments = 4;   // segments examined by one workgroup of the long-segment sort
 
 // Bitonic sort of 64*E (key, value) pairs held by one wavefront in registers: lane l holds elements
 // l*E .. l*E+E-1. The network is unrolled completely (the partner distance and direction of every exchange
@@ -328,76 +329,141 @@ __global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_mid_segments_kern
     wave_sort_segment<16>(keys + s0, vals + s0, n);
 }
 
-__device__ __forceinline__ void lds_bitonic(uint64_t* sk, int32_t* sv, int n_pad) {
-    for (int k = 2; k <= n_pad; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < (n_pad >> 1); i += kSortThreads) {
-                const int lo = ((i & ~(j - 1)) << 1) | (i & (j - 1));
-                const int hi = lo + j;
-                const bool asc = (lo & k) == 0;
-                const uint64_t a = sk[lo];
-                const uint64_t b = sk[hi];
-                if ((a > b) == asc) {
-                    sk[lo] = b;
-                    sk[hi] = a;
-                    const int32_t t = sv[lo];
-                    sv[lo] = sv[hi];
-                    sv[hi] = t;
+// Ascending bitonic merge of a bitonic sequence of 64*E (key, value) pairs held by a wavefront (lane l holds
+// elements l*E .. l*E+E-1): the stages j = 32E .. 1 of the network's last level, in registers, exchanges across
+// lanes through the same shuffles as bitonic_regs.
+template <int E>
+__device__ __forceinline__ void wave_bitonic_merge_regs(uint64_t (&k)[E], int32_t (&v)[E], int t) {
+    constexpr int N = kWave * E;
+#pragma unroll
+    for (int j = N >> 1; j > 0; j >>= 1) {
+        if (j < E) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int p = e ^ j;
+                if (p > e && k[e] > k[p]) {
+                    const uint64_t tk = k[e]; k[e] = k[p]; k[p] = tk;
+                    const int32_t tv = v[e]; v[e] = v[p]; v[p] = tv;
                 }
+            }
+        } else {
+            const int m = j / E;
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                const int i = t * E + e;
+                const int klo = (int)(uint32_t)k[e], khi = (int)(uint32_t)(k[e] >> 32);
+                const uint32_t lo = (uint32_t)shfl_xor_pow2(klo, m);
+                const uint32_t hi = (uint32_t)shfl_xor_pow2(khi, m);
+                const int32_t ov = shfl_xor_pow2(v[e], m);
+                const uint64_t ok = ((uint64_t)hi << 32) | lo;
+                const bool take = ((i & j) == 0) ? (ok < k[e]) : (ok > k[e]);
+                if (take) {
+                    k[e] = ok;
+                    v[e] = ov;
+                }
+            }
+        }
+    }
+}
+
+// Compare-exchange of LDS entries a < b, ascending.
+__device__ __forceinline__ void lds_cas(uint64_t* sk, int32_t* sv, int a, int b) {
+    const uint64_t x = sk[a], y = sk[b];
+    if (x > y) {
+        sk[a] = y;
+        sk[b] = x;
+        const int32_t t = sv[a];
+        sv[a] = sv[b];
+        sv[b] = t;
+    }
+}
+
+// Sorts n (1 <= n <= kMergeCap) keys with their values in LDS. The data is padded with ~0 keys to 1024, 2048 or
+// 4096 entries; the wavefronts sort runs of kWaveSortCap ascending in registers; then every level of the
+// bitonic merge doubles the sorted blocks: a first stage compares element i of a block's first half with
+// element kk-1-i (two ascending runs become two bitonic halves, the smaller elements in the first), then the
+// half-cleaners with partner distance >= kWaveSortCap run in LDS and the remaining ones in registers, each
+// wavefront merging whole 512-entry blocks. Ends with the sorted entries in sk/sv (the first n are the data).
+__device__ __forceinline__ void lds_merge_sort(const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
+                                               int n, uint64_t* sk, int32_t* sv) {
+    constexpr int kWaves = kSortThreads / kWave;
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+    int padded = 2 * kWaveSortCap;
+    while (padded < n) padded <<= 1;
+    const int runs = padded / kWaveSortCap;
+    for (int r = wave; r < runs; r += kWaves) {
+        uint64_t k[8];
+        int32_t v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int i = r * kWaveSortCap + lane * 8 + e;
+            k[e] = (i < n) ? keys[i] : ~0ull;
+            v[e] = (i < n) ? vals[i] : -1;
+        }
+        if (r * kWaveSortCap < n) wave_bitonic_regs<8>(k, v);  // wave-uniform
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            sk[r * kWaveSortCap + lane * 8 + e] = k[e];
+            sv[r * kWaveSortCap + lane * 8 + e] = v[e];
+        }
+    }
+    __syncthreads();
+    const int half = padded >> 1;  // compare-exchanges per stage
+    for (int kk = 2 * kWaveSortCap; kk <= padded; kk <<= 1) {
+        for (int q = tid; q < half; q += kSortThreads) {  // first stage: i against kk-1-i within the block
+            const int base = (q / (kk >> 1)) * kk, off = q % (kk >> 1);
+            lds_cas(sk, sv, base + off, base + kk - 1 - off);
+        }
+        __syncthreads();
+        for (int j = kk >> 2; j >= kWaveSortCap; j >>= 1) {
+            for (int q = tid; q < half; q += kSortThreads) {
+                const int lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
+                lds_cas(sk, sv, lo, lo + j);
             }
             __syncthreads();
         }
-    }
-}
-
-__device__ __forceinline__ void lds_sort_chunk(uint64_t* __restrict__ keys, int32_t* __restrict__ vals, int n,
-                                               uint64_t* sk, int32_t* sv) {
-    int n_pad = 1;
-    while (n_pad < n) n_pad <<= 1;
-    for (int i = threadIdx.x; i < n_pad; i += kSortThreads) {
-        if (i < n) {
-            sk[i] = keys[i];
-            sv[i] = vals[i];
-        } else {
-            sk[i] = ~0ull;
-            sv[i] = -1;
+        for (int r = wave; r < runs; r += kWaves) {  // distances < kWaveSortCap inside each 512-entry block
+            uint64_t k[8];
+            int32_t v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                k[e] = sk[r * kWaveSortCap + lane * 8 + e];
+                v[e] = sv[r * kWaveSortCap + lane * 8 + e];
+            }
+            wave_bitonic_merge_regs<8>(k, v, lane);
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                sk[r * kWaveSortCap + lane * 8 + e] = k[e];
+                sv[r * kWaveSortCap + lane * 8 + e] = v[e];
+            }
         }
+        __syncthreads();
     }
-    __syncthreads();
-    lds_bitonic(sk, sv, n_pad);
-    for (int i = threadIdx.x; i < n; i += kSortThreads) {
-        keys[i] = sk[i];
-        vals[i] = sv[i];
-    }
-    __syncthreads();
 }
 
-__global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* __restrict__ keys,
-                                                                     int32_t* __restrict__ vals,
-                                                                     uint64_t* __restrict__ keys_tmp,
-                                                                     int32_t* __restrict__ vals_tmp,
-                                                                     const int32_t* __restrict__ seg_off,
-                                                                     us3d_stats* stats) {
-    __shared__ uint64_t sk[kSortCap];
-    __shared__ int32_t sv[kSortCap];
-    if (stats->overflow) return;
-    const int seg = blockIdx.x;
-    const int s0 = seg_off[seg];
-    const int n = seg_off[seg + 1] - s0;
-    if (n <= kWaveSortCapMid) return;  // sorted by sort_small_segments_kernel / sort_mid_segments_kernel
+// Sorts segment [s0, s0 + n) (n > kWaveSortCapMid) with the whole workgroup.
+__device__ __forceinline__ void sort_long_segment_body(uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
+                                                       uint64_t* __restrict__ keys_tmp, int32_t* __restrict__ vals_tmp,
+                                                       int s0, int n, us3d_stats* stats, uint64_t* sk, int32_t* sv) {
     uint64_t* k = keys + s0;
     int32_t* v = vals + s0;
-    if (n <= kSortCap) {
-        lds_sort_chunk(k, v, n, sk, sv);
-        return;
+    if (n > kMergeCap && threadIdx.x == 0) atomicAdd(&stats->n_sorted_big, 1);
+    // chunks of kMergeCap sorted in LDS (one chunk when n <= kMergeCap)
+    for (int c = 0; c < n; c += kMergeCap) {
+        const int m = min(kMergeCap, n - c);
+        lds_merge_sort(k + c, v + c, m, sk, sv);
+        for (int i = threadIdx.x; i < m; i += kSortThreads) {
+            k[c + i] = sk[i];
+            v[c + i] = sv[i];
+        }
+        __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&stats->n_sorted_big, 1);
-    for (int c = 0; c < n; c += kSortCap) lds_sort_chunk(k + c, v + c, min(kSortCap, n - c), sk, sv);
+    if (n <= kMergeCap) return;
     uint64_t* src_k = k;
     int32_t* src_v = v;
     uint64_t* dst_k = keys_tmp + s0;
     int32_t* dst_v = vals_tmp + s0;
-    for (int width = kSortCap; width < n; width <<= 1) {
+    for (int width = kMergeCap; width < n; width <<= 1) {
         for (int i = threadIdx.x; i < n; i += kSortThreads) {
             const int run = i / width;
             const int pair_start = (run & ~1) * width;
@@ -425,6 +491,27 @@ __global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* _
     }
 }
 
+// Segments kLongSortSegments * blockIdx.x + (0 .. kLongSortSegments-1): each one longer than kWaveSortCapMid
+// is sorted by the whole workgroup.
+__global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* __restrict__ keys,
+                                                                     int32_t* __restrict__ vals,
+                                                                     uint64_t* __restrict__ keys_tmp,
+                                                                     int32_t* __restrict__ vals_tmp,
+                                                                     const int32_t* __restrict__ seg_off,
+                                                                     int n_segments, us3d_stats* stats) {
+    __shared__ uint64_t sk[kMergeCap];
+    __shared__ int32_t sv[kMergeCap];
+    if (stats->overflow) return;
+    for (int i = 0; i < kLongSortSegments; ++i) {
+        const int seg = blockIdx.x * kLongSortSegments + i;
+        if (seg >= n_segments) return;  // workgroup-uniform
+        const int s0 = seg_off[seg], n = seg_off[seg + 1] - s0;
+        if (n <= kWaveSortCapMid) continue;  // sorted by the wavefront kernels
+        __syncthreads();  // a previous segment's last reads of sk/sv are done
+        sort_long_segment_body(keys, vals, keys_tmp, vals_tmp, s0, n, stats, sk, sv);
+    }
+}
+
 int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp, const int32_t* seg_off,
                   int n_segments, us3d_stats* stats, int64_t k_capacity, hipStream_t stream) {
     (void)k_capacity;
@@ -446,8 +533,8 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
     }
     if ((rc = check_launch("sort_mid_segments_kernel"))) return rc;
     StageTimer timer("sort_segments_kernel", stream);
-    hipLaunchKernelGGL(sort_segments_kernel, dim3(n_segments), dim3(kSortThreads), 0, stream, keys, vals, keys_tmp,
-                       vals_tmp, seg_off, stats);
+    hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks_for(n_segments, kLongSortSegments)), dim3(kSortThreads), 0,
+                       stream, keys, vals, keys_tmp, vals_tmp, seg_off, n_segments, stats);
     return check_launch("sort_segments_kernel");
 }
 
