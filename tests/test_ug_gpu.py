@@ -234,8 +234,8 @@ def test_multi_sweep_batch_equals_per_sweep(dev):
 @pytest.mark.parametrize("ts", [16, 8])
 def test_mapper_long_tile_lists_bitexact(dev, ts):
     """Dense clusters put ~700, ~2500 and ~6000 Gaussians into single tiles, so every tile-sort path runs
-    (register wavefront sorts, the workgroup register sort, the chunked merge); keys, order and ranges must
-    still equal the oracle's bit for bit."""
+    (one-wavefront register sorts, the two-wavefront sort of 513..1024 keys, the workgroup merge sort of up to
+    4096 keys, the chunked merge beyond); keys, order and ranges must still equal the oracle's bit for bit."""
     from us3d.scenes import Scene, precision_tril_from_scales_quats
 
     fr = straight_sweep(1, 128, 128, 0.3, 64.0, 10.0, 0.0, seed=1)[0]

@@ -178,16 +178,17 @@ int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* t
 }
 
 // ---------------------------------------------------------------- per-segment sort
-// Segments of up to kWaveSortCapMid keys are sorted by one wavefront each, four per workgroup: a bitonic
-// network over registers (up to 16 keys per lane) with cross-lane shuffles, no LDS. Longer segments get one
-// workgroup each (a workgroup examines kLongSortSegments consecutive segments and sorts the long ones): runs of kWaveSortCap keys are sorted by the wavefronts in registers the same way and joined by a
+// Segments of up to kWaveSortCap keys are sorted by one wavefront each, four per workgroup: a bitonic network
+// over registers (up to 8 keys per lane) with cross-lane shuffles, no LDS. Segments of up to kWaveSortCapMid
+// keys get two wavefronts (two register-sorted halves, one LDS stage, a register merge). Longer segments get
+// one workgroup each (a workgroup examines kLongSortSegments consecutive segments and sorts the long ones): runs of kWaveSortCap keys are sorted by the wavefronts in registers the same way and joined by a
 // bitonic merge whose long-distance stages run in LDS and the rest in registers (kMergeCap keys); beyond
 // kMergeCap, chunks sorted that way are merged in global memory (ping-pong with the tmp arrays), where an
 // element's place is its index in its run plus its rank in the partner run, so a pass is fully parallel.
 
 constexpr int kSortThreads = 256;
 constexpr int kWaveSortCap = 512;      // segments sorted by one wavefront, <= 8 keys per lane
-constexpr int kWaveSortCapMid = 1024;  // segments sorted by one wavefront with 16 keys per lane
+constexpr int kWaveSortCapMid = 1024;  // segments sorted by two wavefronts
 constexpr int kMergeCap = 4096;        // segments (and chunks) sorted in one workgroup's LDS
 constexpr int kWaveSortWaves = 4;
 constexpr int kLongSortSegments = 4;   // segments examined by one workgroup of the long-segment sort
@@ -315,20 +316,6 @@ __global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_small_segments_ke
     else wave_sort_segment<8>(keys + s0, vals + s0, n);
 }
 
-// Segments of kWaveSortCap+1 .. kWaveSortCapMid keys: the same register bitonic with 16 keys per lane
-// (kept out of the small-segment kernel so its register budget stays small).
-__global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_mid_segments_kernel(
-    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
-    const us3d_stats* __restrict__ stats) {
-    if (stats->overflow) return;
-    const int seg = blockIdx.x * kWaveSortWaves + threadIdx.x / kWave;
-    if (seg >= n_segments) return;  // wave-uniform
-    const int s0 = seg_off[seg];
-    const int n = seg_off[seg + 1] - s0;
-    if (n <= kWaveSortCap || n > kWaveSortCapMid) return;
-    wave_sort_segment<16>(keys + s0, vals + s0, n);
-}
-
 // Ascending bitonic merge of a bitonic sequence of 64*E (key, value) pairs held by a wavefront (lane l holds
 // elements l*E .. l*E+E-1): the stages j = 32E .. 1 of the network's last level, in registers, exchanges across
 // lanes through the same shuffles as bitonic_regs.
@@ -375,6 +362,60 @@ __device__ __forceinline__ void lds_cas(uint64_t* sk, int32_t* sv, int a, int b)
         const int32_t t = sv[a];
         sv[a] = sv[b];
         sv[b] = t;
+    }
+}
+
+// Segments of kWaveSortCap+1 .. kWaveSortCapMid keys, kLongSortSegments consecutive ones per two-wavefront
+// workgroup: each wavefront sorts a 512-key half in registers, one stage in LDS compares element i of the first
+// half with element 1023-i (the two sorted halves become bitonic halves, the smaller keys in the first), and
+// each wavefront finishes its half with the register merge and writes it out.
+constexpr int kMidThreads = 2 * kWave;
+
+__global__ __launch_bounds__(kMidThreads) void sort_mid_segments_kernel(
+    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
+    const us3d_stats* __restrict__ stats) {
+    __shared__ uint64_t sk[kWaveSortCapMid];
+    __shared__ int32_t sv[kWaveSortCapMid];
+    if (stats->overflow) return;
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+    for (int it = 0; it < kLongSortSegments; ++it) {
+        const int seg = blockIdx.x * kLongSortSegments + it;
+        if (seg >= n_segments) return;  // workgroup-uniform
+        const int s0 = seg_off[seg], n = seg_off[seg + 1] - s0;
+        if (n <= kWaveSortCap || n > kWaveSortCapMid) continue;
+        __syncthreads();  // a previous segment's reads of sk/sv are done
+        uint64_t k[8];
+        int32_t v[8];
+        const int b = wave * kWaveSortCap + lane * 8;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int i = b + e;
+            k[e] = (i < n) ? keys[s0 + i] : ~0ull;
+            v[e] = (i < n) ? vals[s0 + i] : -1;
+        }
+        wave_bitonic_regs<8>(k, v);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            sk[b + e] = k[e];
+            sv[b + e] = v[e];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < kWaveSortCap; q += kMidThreads) lds_cas(sk, sv, q, kWaveSortCapMid - 1 - q);
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            k[e] = sk[b + e];
+            v[e] = sv[b + e];
+        }
+        wave_bitonic_merge_regs<8>(k, v, lane);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int i = b + e;
+            if (i < n) {
+                keys[s0 + i] = k[e];
+                vals[s0 + i] = v[e];
+            }
+        }
     }
 }
 
@@ -528,8 +569,8 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
     if (rc) return rc;
     {
         StageTimer timer("sort_mid_segments_kernel", stream);
-        hipLaunchKernelGGL(sort_mid_segments_kernel, dim3(blocks_for(n_segments, kWaveSortWaves)),
-                           dim3(kWaveSortWaves * kWave), 0, stream, keys, vals, seg_off, n_segments, stats);
+        hipLaunchKernelGGL(sort_mid_segments_kernel, dim3(blocks_for(n_segments, kLongSortSegments)),
+                           dim3(kMidThreads), 0, stream, keys, vals, seg_off, n_segments, stats);
     }
     if ((rc = check_launch("sort_mid_segments_kernel"))) return rc;
     StageTimer timer("sort_segments_kernel", stream);
