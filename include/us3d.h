@@ -153,7 +153,10 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
  * linear-probe grid origin + (x - col_offset)*delta_lat*a_lat + (near + y*delta_ax)*a_ax computed in-kernel
  * with the FP32 operations of pipeline_god_file.py:299-306 (bit-identical to that tensor).
  * Outputs [n_pixels]: image, alpha, depth; aux [n_pixels,4] = (sum alpha_hat, unclamped image,
- * sum alpha_hat*elev, 0) kept for the adjoint (sums accumulated in FP64, stored as FP32). Requires a prior us3d_ug_map on the same workspace. */
+ * sum alpha_hat*elev, 0) kept for the adjoint (sums accumulated in FP64, stored as FP32). Requires a prior us3d_ug_map on the same workspace.
+ * depth may be NULL: the depth numerator is then not accumulated, aux holds 0 in its place, and a backward on
+ * this aux must pass d_depth = NULL. The backward with d_alpha = d_depth = NULL (image gradients only) skips
+ * their terms as well. */
 int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
                     float* image, float* alpha, float* depth, float* aux, void* stream);
 

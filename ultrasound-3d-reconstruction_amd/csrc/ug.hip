@@ -482,7 +482,9 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // adjoint's coefficient dL/dalpha_hat = dN (c - image) depends on the image to far better than FP32-sum
 // accuracy wherever c is close to it.
 
-template <int TS>
+// kDepth: the depth output is requested (out_depth != NULL); without it the depth numerator is not accumulated
+// and aux holds 0 in its place.
+template <int TS, bool kDepth>
 __global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
 void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
@@ -581,7 +583,7 @@ void ug_forward_kernel(
                             const double ahd = (double)ah;
                             S_[k] += ahd;
                             N_[k] = __builtin_fma(ahd, (double)b.x, N_[k]);
-                            D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
+                            if constexpr (kDepth) D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
                         }
                     }
                 }
@@ -597,8 +599,8 @@ void ug_forward_kernel(
         const double img = num / fmax(A, 1e-8);
         out_image[pidx[k]] = (float)fmax(0.0, fmin(1.0, img));
         out_alpha[pidx[k]] = (float)(alpha_excludes_bg ? S_[k] : A);
-        out_depth[pidx[k]] = (float)(D_[k] / fmax(S_[k], 1e-8));
-        out_aux[pidx[k]] = make_float4((float)S_[k], (float)img, (float)D_[k], 0.f);
+        if constexpr (kDepth) out_depth[pidx[k]] = (float)(D_[k] / fmax(S_[k], 1e-8));
+        out_aux[pidx[k]] = make_float4((float)S_[k], (float)img, kDepth ? (float)D_[k] : 0.f, 0.f);
     }
 }
 
@@ -642,7 +644,9 @@ __device__ __forceinline__ void write_partial(float4* __restrict__ partial, int6
     partial[3 * ei + 2] = make_float4(a.v[9], a.v[10], a.v[11], 0.f);
 }
 
-template <int TS>
+// kFull: alpha or depth gradients are given (d_alpha or d_depth != NULL); without them dL/dalpha_hat is
+// dN (c - v) alone and the depth term is not accumulated.
+template <int TS, bool kFull>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
@@ -694,8 +698,8 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             const float A = sm.x + bg_alpha;
             const float inv = 1.0f / fmaxf(A, 1e-8f);
             const float gi = d_image[p];
-            const float ga = d_alpha ? d_alpha[p] : 0.f;
-            const float gd = d_depth ? d_depth[p] : 0.f;
+            const float ga = (kFull && d_alpha) ? d_alpha[p] : 0.f;
+            const float gd = (kFull && d_depth) ? d_depth[p] : 0.f;
             const float invS = 1.0f / fmaxf(sm.x, 1e-8f);
             dN = gi * inv;
             vimg = sm.y;
@@ -801,10 +805,11 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                         if (ah > alpha_min) {
                             // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
                             // cancellation of dN c against dN v
-                            const float da = __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y));
+                            const float da = kFull ? __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y))
+                                                   : pp.w * (c - cc.x);
                             ac = __builtin_fmaf(pp.w, ah, ac);
                             ao = __builtin_fmaf(da, w, ao);
-                            aed = __builtin_fmaf(cc.z, ah, aed);
+                            if constexpr (kFull) aed = __builtin_fmaf(cc.z, ah, aed);
                             const float mm = -da * ah;
                             const float m0 = mm * y0, m1 = mm * y1, m2 = mm * y2;
                             Y0 += m0; Y1 += m1; Y2 += m2;
@@ -1108,11 +1113,11 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_settings(settings))) return rc;
-    if (!image || !alpha || !depth || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
+    if (!image || !alpha || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_FWD(TS)                                                                                                    \
-    hipLaunchKernelGGL(ug_forward_kernel<TS>, dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0, stream,           \
+#define UG_FWD(TS, DEPTH)                                                                                             \
+    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0, stream,  \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        plan->frames, plan->frame_tile_base, plan->n_frames, pixels, settings->chi2_threshold,         \
                        sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
@@ -1121,9 +1126,9 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     {
         StageTimer timer_("ug_forward_kernel", stream);
         switch (settings->tile_size) {
-            case 8: UG_FWD(8); break;
-            case 16: UG_FWD(16); break;
-            default: UG_FWD(32); break;
+            case 8: if (depth) UG_FWD(8, true); else UG_FWD(8, false); break;
+            case 16: if (depth) UG_FWD(16, true); else UG_FWD(16, false); break;
+            default: if (depth) UG_FWD(32, true); else UG_FWD(32, false); break;
         }
     }
 #undef UG_FWD
@@ -1142,19 +1147,20 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_BWD(TS)                                                                                                    \
-    hipLaunchKernelGGL(ug_backward_kernel<TS>, dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream,         \
+#define UG_BWD(TS, FULL)                                                                                              \
+    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream, \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
                        settings->chi2_threshold, sqrt_chi2, settings->alpha_min, settings->bg_alpha,                  \
                        settings->bg_intensity, reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,      \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial))
+    const bool full = d_alpha || d_depth;
     {
         StageTimer timer_("ug_backward_kernel", stream);
         switch (settings->tile_size) {
-            case 8: UG_BWD(8); break;
-            case 16: UG_BWD(16); break;
-            default: UG_BWD(32); break;
+            case 8: if (full) UG_BWD(8, true); else UG_BWD(8, false); break;
+            case 16: if (full) UG_BWD(16, true); else UG_BWD(16, false); break;
+            default: if (full) UG_BWD(32, true); else UG_BWD(32, false); break;
         }
     }
 #undef UG_BWD

@@ -52,7 +52,6 @@ class SliceSplatTrainer:
         self.target = target.detach().to(**f32).reshape(P).contiguous()
         self.image = torch.empty(P, **f32)
         self.alpha = torch.empty(P, **f32)
-        self.depth = torch.empty(P, **f32)
         self.aux = torch.empty((P, 4), **f32)
         self.d_image = torch.empty(P, **f32)
         self.loss = torch.empty(self.F, **f32)
@@ -86,7 +85,7 @@ class SliceSplatTrainer:
         _lib.check(L.us3d_ug_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.raw_o),
                                  _lib.ptr(self.raw_c), st, s), "us3d_ug_map")
         _lib.check(L.us3d_ug_forward(self.plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
-                                     _lib.ptr(self.depth), _lib.ptr(self.aux), s), "us3d_ug_forward")
+                                     None, _lib.ptr(self.aux), s), "us3d_ug_forward")
         _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
                                      lay.n_frames, self.w_l1, self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
                                      _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
