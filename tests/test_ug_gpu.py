@@ -388,3 +388,30 @@ def test_emit_split_modes_bitexact(dev, n_frames):
     tiles_x = np.asarray(lay.tiles_x)[frame]
     local = (r[:, 1] + j // w) * tiles_x + r[:, 0] + j % w
     np.testing.assert_array_equal(local, tile - np.asarray(lay.frame_tile_base)[frame])
+
+
+def test_fan_grid_in_kernel_matches_explicit_positions(dev):
+    """pixels = NULL on fan frames: the kernels evaluate the polar grid (sincos of the beam angle) themselves.
+    It matches rendering from the host's restatement of that grid (fan_pixels) to within the last-bit rounding
+    of sin/cos (image and gradients, relative L2 < 1e-5)."""
+    from us3d.scenes import fan_pixels, fan_sweep
+    from us3d.ultragauss import render_frames
+
+    box = [64.0, 64.0, 64.0]
+    scene = make_scene(1500, 64.0, seed=23, box_mm=box)
+    frames = fan_sweep(3, 64, 80, 60.0, 8.0, 56.0, box, 20.0, seed=7)
+    cfg = cfg_for(frames[0])
+    dI = torch.from_numpy(np.random.default_rng(2).normal(size=sum(f.height * f.width for f in frames))
+                          .astype(np.float32)).to(dev)
+    outs = []
+    for pix in (torch.cat([fan_pixels(f) for f in frames]).to(dev), None):
+        ts = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+        image, _a, _d = render_frames(*ts, pix, frames, cfg, sweeps=[SweepSpec(0, scene.means.shape[0])],
+                                      hint=CapacityHint())
+        (image * dI).sum().backward()
+        outs.append((image.detach().cpu().numpy(), [t.grad.detach().cpu().numpy() for t in ts]))
+    (img_e, g_e), (img_p, g_p) = outs
+    assert np.abs(img_e).max() > 1e-2
+    assert rel_l2(img_p, img_e) < 1e-5
+    for a, b in zip(g_p, g_e):
+        assert rel_l2(a, b) < 1e-5
