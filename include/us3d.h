@@ -127,11 +127,13 @@ typedef struct us3d_ug_settings {
  * written only where pair_count > 0 --, tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor
  * i32[n_tiles] (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (emit index),
  * records (per Gaussian), partial (per instance), vis_list i32[n_pairs] (frame-major copy of the pair
- * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1). */
+ * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1), inst_mask u16[K]
+ * (UltraGauss: the sub-block mask of each sorted instance, written by the forward, read by the backward;
+ * empty for UltraG-Ray). */
 typedef struct us3d_layout {
     int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
     int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
-    int64_t vis_list, total;
+    int64_t vis_list, inst_mask, total;
 } us3d_layout;
 
 int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
@@ -153,7 +155,8 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
  * linear-probe grid origin + (x - col_offset)*delta_lat*a_lat + (near + y*delta_ax)*a_ax computed in-kernel
  * with the FP32 operations of pipeline_god_file.py:299-306 (bit-identical to that tensor).
  * Outputs [n_pixels]: image, alpha, depth; aux [n_pixels,4] = (sum alpha_hat, unclamped image,
- * sum alpha_hat*elev, 0) kept for the adjoint (sums accumulated in FP64, stored as FP32). Requires a prior us3d_ug_map on the same workspace.
+ * sum alpha_hat*elev, 0) kept for the adjoint (sums accumulated in FP64, stored as FP32). Requires a prior us3d_ug_map on the same workspace;
+ * leaves each instance's sub-block mask in the workspace (layout inst_mask) for the adjoint.
  * depth may be NULL: the depth numerator is then not accumulated, aux holds 0 in its place, and a backward on
  * this aux must pass d_depth = NULL. The backward with d_alpha = d_depth = NULL (image gradients only) skips
  * their terms as well. */
@@ -161,7 +164,9 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                     float* image, float* alpha, float* depth, float* aux, void* stream);
 
 /* Adjoint: replaces ShaderBackwardCUDA (ops/ultragauss_renderer/src/shader/wrapper_shader.cu:68-122,
- * backward.cu:17-131). Recomputes per-pair weights from the stored per-pixel sums, accumulates per
+ * backward.cu:17-131). Requires the map and forward of the same call sequence on the same workspace, with
+ * nothing else run on it in between (it reads their sorted lists and sub-block masks).
+ * Recomputes per-pair weights from the stored per-pixel sums, accumulates per
  * (tile, Gaussian) in registers (no atomics) and reduces per Gaussian in a fixed order, so the result
  * is bitwise reproducible. d_alpha / d_depth may be NULL (zero). Outputs (overwritten): d_means [G,3],
  * d_tril [G,6] (w.r.t. the raw tril parameters), d_opacity [G], d_intensity [G]. */

@@ -19,7 +19,7 @@ namespace us3d {
 // ---------------------------------------------------------------- workspace layout
 
 us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
-                      int64_t partial_bytes) {
+                      int64_t partial_bytes, int64_t mask_bytes) {
     us3d_layout L{};
     int64_t off = 0;
     auto take = [&](int64_t bytes) {
@@ -43,12 +43,13 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     L.partial = take(k_cap * partial_bytes);
     L.scan_tmp = take(std::max(scan_tmp_bytes(n_pairs + 1), scan_tmp_bytes(n_tiles + 1)));
     L.vis_list = take(n_pairs * 4);
+    L.inst_mask = take(k_cap * mask_bytes);
     L.total = off;
     return L;
 }
 
 static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2);
 }
 
 // ---------------------------------------------------------------- mapper kernels
@@ -491,7 +492,8 @@ void ug_forward_kernel(
     const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
-    float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux) {
+    float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
+    uint16_t* __restrict__ inst_mask) {
     using S = FwdShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
@@ -529,8 +531,9 @@ void ug_forward_kernel(
         const float ele = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
         const SubBounds b = sub_bounds(valid[k], lat, ax, ele);
         if (lane == 0) s_sb[sb] = b;
-        S_[k] = N_[k] = D_[k] = 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) S_[k] = N_[k] = D_[k] = 0.0;
 
     for (int base = s0; base < s1; base += NT) {
         __syncthreads();
@@ -544,8 +547,10 @@ void ug_forward_kernel(
             const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            s_mask[tid] = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, e, fr,
-                                                  sqrt_chi2, ortho);
+            const uint32_t m = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, e, fr,
+                                                       sqrt_chi2, ortho);
+            s_mask[tid] = m;
+            inst_mask[base + tid] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             s_r0[tid] = r0;                                  // mu, opacity
             s_r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
             s_r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
@@ -653,7 +658,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels, float chi2,
     float sqrt_chi2, float alpha_min, float bg_alpha, float bg_intensity, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
-    const us3d_stats* __restrict__ stats, float4* __restrict__ partial) {
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
     using S = TileShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
@@ -662,9 +667,9 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     constexpr int MAXI = NT * NSUB;     // items of one batch, at most
     constexpr int MAXC = MAXI / kWave;  // chunks of one batch, at most
     static_assert(NT <= 256 && NSUB <= 16, "item code packs (slot << 4) | sub-block into 12 bits");
+    static_assert(FwdShape<TS>::kSub == NSUB, "the forward's instance masks number the same sub-blocks");
     __shared__ float4 s_p[NSUB * kSubStride];  // px, py, pz, dN
     __shared__ float4 s_c[NSUB * kSubStride];  // unclamped image v, dX, dD
-    __shared__ SubBounds s_sb[NSUB];
     __shared__ float4 s_g0[NT], s_g1[NT], s_g2[NT];  // the batch's Gaussian records
     __shared__ uint16_t s_item[MAXI];
     __shared__ int s_wtot[NW];
@@ -682,7 +687,6 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     if (s0 == s1) return;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
-    const bool ortho = axes_orthonormal(fr);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -706,12 +710,6 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             dD = gd * invS;
             dX = ga - gd * sm.z * invS * invS;
         }
-        const float dx = x - ox, dy = y - oy, dz = z - oz;
-        const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-        const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        const float ele = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-        const SubBounds b = sub_bounds(valid, lat, ax, ele);
-        if (lane == 0) s_sb[sb] = b;
         s_p[sb * kSubStride + lane] = make_float4(x, y, z, dN);
         s_c[sb * kSubStride + lane] = make_float4(vimg, dX, dD, 0.f);
     }
@@ -725,11 +723,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             const float4 q0 = rec[3 * (int64_t)g + 0];
             const float4 q1 = rec[3 * (int64_t)g + 1];
             const float4 q2 = rec[3 * (int64_t)g + 2];
-            const float vx = q0.x - ox, vy = q0.y - oy, vz = q0.z - oz;
-            const float ge = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-            const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-            const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            m = gauss_sub_mask<NSUB>(s_sb, q1.y, q1.z, q1.w, q2.x, q2.y, q2.z, lat_g, ax_g, ge, fr, sqrt_chi2, ortho);
+            m = inst_mask[base + tid];  // the sub-block mask the forward left for this instance
             s_g0[tid] = q0;
             s_g1[tid] = q1;
             s_g2[tid] = q2;
@@ -1122,7 +1116,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                        plan->frames, plan->frame_tile_base, plan->n_frames, pixels, settings->chi2_threshold,         \
                        sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
                        settings->alpha_excludes_bg, at<us3d_stats>(plan, L.stats), image, alpha, depth,              \
-                       reinterpret_cast<float4*>(aux))
+                       reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask))
     {
         StageTimer timer_("ug_forward_kernel", stream);
         switch (settings->tile_size) {
@@ -1153,7 +1147,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                        at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
                        settings->chi2_threshold, sqrt_chi2, settings->alpha_min, settings->bg_alpha,                  \
                        settings->bg_intensity, reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,      \
-                       at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial))
+                       at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask))
     const bool full = d_alpha || d_depth;
     {
         StageTimer timer_("ug_backward_kernel", stream);
