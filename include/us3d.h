@@ -129,11 +129,12 @@ typedef struct us3d_ug_settings {
  * records (per Gaussian), partial (per instance), vis_list i32[n_pairs] (frame-major copy of the pair
  * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1), inst_mask u16[K]
  * (UltraGauss: the sub-block mask of each sorted instance, written by the forward, read by the backward;
- * empty for UltraG-Ray). */
+ * empty for UltraG-Ray), pair_sect f32[n_pairs][8] (UltraGauss: per visible pair, the pair's part of the
+ * sub-block test, written by the map, read by the forward; empty for UltraG-Ray). */
 typedef struct us3d_layout {
     int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
     int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
-    int64_t vis_list, inst_mask, total;
+    int64_t vis_list, inst_mask, pair_sect, total;
 } us3d_layout;
 
 int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);

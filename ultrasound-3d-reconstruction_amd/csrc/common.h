@@ -153,7 +153,7 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
 
 // Workspace layout shared by both renderers (record and partial sizes differ).
 us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
-                      int64_t partial_bytes, int64_t mask_bytes);
+                      int64_t partial_bytes, int64_t mask_bytes, int64_t sect_bytes);
 
 template <typename T>
 inline T* ws_at(const us3d_plan* plan, int64_t off) {

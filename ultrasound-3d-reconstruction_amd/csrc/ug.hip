@@ -19,7 +19,7 @@ namespace us3d {
 // ---------------------------------------------------------------- workspace layout
 
 us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
-                      int64_t partial_bytes, int64_t mask_bytes) {
+                      int64_t partial_bytes, int64_t mask_bytes, int64_t sect_bytes) {
     us3d_layout L{};
     int64_t off = 0;
     auto take = [&](int64_t bytes) {
@@ -44,12 +44,13 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     L.scan_tmp = take(std::max(scan_tmp_bytes(n_pairs + 1), scan_tmp_bytes(n_tiles + 1)));
     L.vis_list = take(n_pairs * 4);
     L.inst_mask = take(k_cap * mask_bytes);
+    L.pair_sect = take(n_pairs * sect_bytes);
     L.total = off;
     return L;
 }
 
 static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2, 32);
 }
 
 // ---------------------------------------------------------------- mapper kernels
@@ -129,11 +130,74 @@ __device__ __forceinline__ void fan_tile_bounds(const us3d_frame& fr, float lat,
 // Per (frame, Gaussian) pair: probe-frame projection, chi2 marginal radii, elevation cull, tile rectangle
 // and depth key (compute_probe_bbox_radii + kernel_project_gauss_and_compute_tile_rects,
 // ops/ultragauss_renderer/src/mapper/core.cu:9-116). Non-finite projections are culled.
+// Per-pair part of the sub-block test (the tests themselves are described with the splat kernels): computed
+// once per visible (frame, Gaussian) pair by the preprocess, read by the forward for each of the pair's
+// instances. s0 = (wl, wa, u_lat, u_ax): padded marginal half-widths and the in-plane centre's drift per unit
+// elevation; s1 = (i00, i11, kk, pad): diagonal of the in-plane inverse (widened), the elevational Schur
+// complement and the rounding pad. kk < 0 switches the section test off (probe axes not orthonormal, or a
+// degenerate section); a degenerate factor (some l_ii <= 1e-10) gets infinite half-widths, so every sub-block.
+// The section setup is in FP64 (no cancellation in det or the Schur complement).
+__device__ __forceinline__ bool axes_orthonormal(const us3d_frame& fr) {
+    const float* a = fr.axis_lat;
+    const float* b = fr.axis_ax;
+    const float* c = fr.axis_ele;
+    const float ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const float ac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+    const float bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
+    const float aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    const float bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
+    const float cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+    return fabsf(ab) < 1e-5f && fabsf(ac) < 1e-5f && fabsf(bc) < 1e-5f && fabsf(aa - 1.f) < 2e-5f &&
+           fabsf(bb - 1.f) < 2e-5f && fabsf(cc - 1.f) < 2e-5f;
+}
+
+__device__ __forceinline__ void pair_section(float l11, float l12, float l13, float l22, float l23, float l33,
+                                             float lat_g, float ax_g, float r_lat, float r_ax,
+                                             const us3d_frame& fr, float4& s0, float4& s1) {
+    if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) {
+        s0 = make_float4(INFINITY, INFINITY, 0.f, 0.f);
+        s1 = make_float4(0.f, 0.f, -1.f, 0.f);
+        return;
+    }
+    const float pad = 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
+    const float wl = r_lat * 1.001f + pad;
+    const float wa = r_ax * 1.001f + pad;
+    float u_lat = 0.f, u_ax = 0.f, i00 = 0.f, i11 = 0.f, kk = -1.f;
+    if (axes_orthonormal(fr)) {
+        // rows of A = R^T L, L = [[l11 0 0] [l12 l22 0] [l13 l23 l33]] so that q = |L^T d|^2
+        const float* a = fr.axis_lat;
+        const float* c = fr.axis_ax;
+        const float* e = fr.axis_ele;
+        const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
+        const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
+        const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
+        const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
+        const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
+        const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
+        const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
+        const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
+        const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
+        const double cz = C0 * C0 + C1 * C1 + C2 * C2;
+        const double det = M00 * M11 - M01 * M01;
+        if (det > 1e-8 * M00 * M11) {
+            const double j00 = M11 / det, j11 = M00 / det, j01 = -M01 / det;
+            const double w0 = j00 * b0 + j01 * b1, w1 = j01 * b0 + j11 * b1;
+            u_lat = (float)(-w0);
+            u_ax = (float)(-w1);
+            i00 = (float)j00 * 1.0001f;
+            i11 = (float)j11 * 1.0001f;
+            kk = (float)fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
+        }
+    }
+    s0 = make_float4(wl, wa, u_lat, u_ax);
+    s1 = make_float4(i00, i11, kk, pad);
+}
+
 __global__ __launch_bounds__(256) void ug_preprocess_kernel(
     int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
     const us3d_sweep* __restrict__ sweeps, int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
-    int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
+    float4* __restrict__ pair_sect, int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool in = pair < n_pairs;
     if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
@@ -183,6 +247,10 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
                         count = (x1 - x0) * (y1 - y0);
                         rect = make_int4(x0, y0, x1, y1);
                         dk = sort_by_depth ? depth_key(v) : 0u;
+                        float4 s0, s1;
+                        pair_section(l11, l12, l13, l22, l23, l33, lat, ax, r_lat, r_ax, fr, s0, s1);
+                        pair_sect[2 * pair + 0] = s0;
+                        pair_sect[2 * pair + 1] = s1;
                     }
                 }
             }
@@ -386,73 +454,23 @@ __device__ __forceinline__ SubBounds sub_bounds(bool valid, float lat, float ax,
     return b;
 }
 
-// Probe axes orthonormal to within 1e-5 (the section test needs it; the marginal test does not).
-__device__ __forceinline__ bool axes_orthonormal(const us3d_frame& fr) {
-    const float* a = fr.axis_lat;
-    const float* b = fr.axis_ax;
-    const float* c = fr.axis_ele;
-    const float ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
-    const float ac = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
-    const float bc = b[0] * c[0] + b[1] * c[1] + b[2] * c[2];
-    const float aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
-    const float bb = b[0] * b[0] + b[1] * b[1] + b[2] * b[2];
-    const float cc = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
-    return fabsf(ab) < 1e-5f && fabsf(ac) < 1e-5f && fabsf(bc) < 1e-5f && fabsf(aa - 1.f) < 2e-5f &&
-           fabsf(bb - 1.f) < 2e-5f && fabsf(cc - 1.f) < 2e-5f;
-}
-
-// Sub-block mask of one Gaussian (see above). Degenerate factors (l_ii <= 1e-10) get a full mask.
+// Sub-block mask of one instance from its pair's section setup (pair_section): the marginal box test, then,
+// where the setup allows it (kk >= 0), the section test over the sub-block's elevation range.
+// chi2w = sqrt_chi2^2 * 1.002.
 template <int NSUB>
-__device__ __forceinline__ uint32_t gauss_sub_mask(const SubBounds* __restrict__ sb, float l11, float l12, float l13,
-                                                   float l22, float l23, float l33, float lat_g, float ax_g,
-                                                   float ele_g, const us3d_frame& fr, float sqrt_chi2, bool ortho) {
-    constexpr uint32_t kAll = (NSUB >= 32) ? 0xffffffffu : ((1u << NSUB) - 1u);
-    if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) return kAll;
-    const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
-    const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-    const float pad = 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
-    const float wl = r_lat * 1.001f + pad;
-    const float wa = r_ax * 1.001f + pad;
-    // section test setup in FP64 (no cancellation in det or the Schur complement): rows of A = R^T L,
-    // L = [[l11 0 0] [l12 l22 0] [l13 l23 l33]] so that q = |L^T d|^2
-    bool sect = ortho;
-    float u_lat = 0.f, u_ax = 0.f, i00 = 0.f, i11 = 0.f, kk = 0.f;
-    if (sect) {
-        const float* a = fr.axis_lat;
-        const float* c = fr.axis_ax;
-        const float* e = fr.axis_ele;
-        const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
-        const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
-        const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
-        const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
-        const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
-        const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
-        const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
-        const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
-        const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
-        const double cz = C0 * C0 + C1 * C1 + C2 * C2;
-        const double det = M00 * M11 - M01 * M01;
-        sect = det > 1e-8 * M00 * M11;
-        if (sect) {
-            const double j00 = M11 / det, j11 = M00 / det, j01 = -M01 / det;
-            const double w0 = j00 * b0 + j01 * b1, w1 = j01 * b0 + j11 * b1;
-            u_lat = (float)(-w0);
-            u_ax = (float)(-w1);
-            i00 = (float)j00 * 1.0001f;
-            i11 = (float)j11 * 1.0001f;
-            kk = (float)fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
-        }
-    }
-    const float chi2 = sqrt_chi2 * sqrt_chi2 * 1.002f;
+__device__ __forceinline__ uint32_t sub_mask(const SubBounds* __restrict__ sb, float4 s0, float4 s1, float lat_g,
+                                             float ax_g, float ele_g, float chi2w) {
+    const float wl = s0.x, wa = s0.y, u_lat = s0.z, u_ax = s0.w;
+    const float i00 = s1.x, i11 = s1.y, kk = s1.z, pad = s1.w;
     uint32_t m = 0;
 #pragma unroll
     for (int s = 0; s < NSUB; ++s) {
         bool hit = (lat_g + wl >= sb[s].lat_lo) && (lat_g - wl <= sb[s].lat_hi) && (ax_g + wa >= sb[s].ax_lo) &&
                    (ax_g - wa <= sb[s].ax_hi);
-        if (hit && sect) {
+        if (hit && kk >= 0.0f) {
             const float d_lo = sb[s].ele_lo - ele_g, d_hi = sb[s].ele_hi - ele_g;
             const float d0 = fminf(fmaxf(0.0f, d_lo), d_hi);
-            const float rem = chi2 - kk * d0 * d0;
+            const float rem = chi2w - kk * d0 * d0;
             if (rem <= 0.0f) {
                 hit = false;
             } else {
@@ -489,8 +507,9 @@ template <int TS, bool kDepth>
 __global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
 void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
-    const float* __restrict__ pixels, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
+    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
+    const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
     float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
     uint16_t* __restrict__ inst_mask) {
@@ -510,7 +529,10 @@ void ug_forward_kernel(
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
-    const bool ortho = axes_orthonormal(fr);
+    const us3d_sweep& sw = sweeps[fr.sweep];
+    // pair index of Gaussian g in this frame: pair_g0 + g * f_count
+    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
+    const float chi2w = sqrt_chi2 * sqrt_chi2 * 1.002f;
 
     float px[PPT], py[PPT], pz[PPT];
     bool valid[PPT];
@@ -547,8 +569,9 @@ void ug_forward_kernel(
             const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            const uint32_t m = gauss_sub_mask<S::kSub>(s_sb, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, lat_g, ax_g, e, fr,
-                                                       sqrt_chi2, ortho);
+            const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
+            const uint32_t m = sub_mask<S::kSub>(s_sb, pair_sect[2 * pair + 0], pair_sect[2 * pair + 1], lat_g, ax_g, e,
+                                                 chi2w);
             s_mask[tid] = m;
             inst_mask[base + tid] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             s_r0[tid] = r0;                                  // mu, opacity
@@ -1092,7 +1115,7 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
             StageTimer timer_("ug_preprocess_kernel", stream);
             hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
                                plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
-                               settings->sort_by_depth, pair_count, pair_rect, pair_dk,
+                               settings->sort_by_depth, pair_count, pair_rect, pair_dk, at<float4>(plan, L.pair_sect),
                                at<int32_t>(plan, L.tile_count), at<int32_t>(plan, L.vis_list));
         }
         if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
@@ -1113,9 +1136,10 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
 #define UG_FWD(TS, DEPTH)                                                                                             \
     hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0, stream,  \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
-                       plan->frames, plan->frame_tile_base, plan->n_frames, pixels, settings->chi2_threshold,         \
-                       sqrt_chi2, settings->alpha_min, settings->bg_alpha, settings->bg_intensity,                    \
-                       settings->alpha_excludes_bg, at<us3d_stats>(plan, L.stats), image, alpha, depth,              \
+                       plan->frames, plan->sweeps, plan->frame_tile_base, plan->n_frames, pixels,                     \
+                       at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
+                       settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
+                       at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
                        reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask))
     {
         StageTimer timer_("ug_forward_kernel", stream);

@@ -21,7 +21,7 @@ constexpr float kW0 = 0.5555555556f;
 constexpr float kW1 = 0.8888888889f;
 
 us3d_layout ur_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 64, 64, 0);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 64, 64, 0, 0);
 }
 
 // One record per Gaussian: (mu, raw attenuation | scales, m | quaternion | SH). With `log_scales` the scales

@@ -110,7 +110,7 @@ class URSettings(ctypes.Structure):
 
 _LAYOUT_FIELDS = [
     "stats", "pair_count", "pair_off", "pair_rect", "pair_dk", "tile_count", "tile_off", "tile_cursor",
-    "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp", "vis_list", "inst_mask", "total",
+    "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp", "vis_list", "inst_mask", "pair_sect", "total",
 ]
 
 
