@@ -132,11 +132,14 @@ __device__ __forceinline__ void fan_tile_bounds(const us3d_frame& fr, float lat,
 // ops/ultragauss_renderer/src/mapper/core.cu:9-116). Non-finite projections are culled.
 // Per-pair part of the sub-block test (the tests themselves are described with the splat kernels): computed
 // once per visible (frame, Gaussian) pair by the preprocess, read by the forward for each of the pair's
-// instances. s0 = (wl, wa, u_lat, u_ax): padded marginal half-widths and the in-plane centre's drift per unit
-// elevation; s1 = (i00, i11, kk, pad): diagonal of the in-plane inverse (widened), the elevational Schur
-// complement and the rounding pad. kk < 0 switches the section test off (probe axes not orthonormal, or a
-// degenerate section); a degenerate factor (some l_ii <= 1e-10) gets infinite half-widths, so every sub-block.
-// The section setup is in FP64 (no cancellation in det or the Schur complement).
+// instances. With orthonormal probe axes and a regular section (kk >= 0):
+//   s0 = (u_lat, u_ax, kk, t_x): in-plane centre drift per unit elevation, elevational Schur complement, and
+//        t_x = M01 / M11;
+//   s1 = (p, q, s, t_y): the in-plane precision M = [[M00 M01] [M01 M11]] as f(x, y) = (p x + q y)^2 + (s y)^2
+//        (p = sqrt(M00), q = M01 / p, s = sqrt(det M / M00)), and t_y = M01 / M00.
+// Otherwise kk = -1 and s0.xy = (wl, wa), the padded marginal half-widths (infinite for a degenerate factor,
+// some l_ii <= 1e-10, so every sub-block). The section setup is in FP64 (no cancellation in det or the Schur
+// complement).
 __device__ __forceinline__ bool axes_orthonormal(const us3d_frame& fr) {
     const float* a = fr.axis_lat;
     const float* b = fr.axis_ax;
@@ -151,46 +154,42 @@ __device__ __forceinline__ bool axes_orthonormal(const us3d_frame& fr) {
            fabsf(bb - 1.f) < 2e-5f && fabsf(cc - 1.f) < 2e-5f;
 }
 
+__device__ __forceinline__ float sub_pad(float lat_g, float ax_g) {
+    return 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
+}
+
 __device__ __forceinline__ void pair_section(float l11, float l12, float l13, float l22, float l23, float l33,
                                              float lat_g, float ax_g, float r_lat, float r_ax,
                                              const us3d_frame& fr, float4& s0, float4& s1) {
+    s1 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) {
-        s0 = make_float4(INFINITY, INFINITY, 0.f, 0.f);
-        s1 = make_float4(0.f, 0.f, -1.f, 0.f);
+        s0 = make_float4(INFINITY, INFINITY, -1.f, 0.f);
         return;
     }
-    const float pad = 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
-    const float wl = r_lat * 1.001f + pad;
-    const float wa = r_ax * 1.001f + pad;
-    float u_lat = 0.f, u_ax = 0.f, i00 = 0.f, i11 = 0.f, kk = -1.f;
-    if (axes_orthonormal(fr)) {
-        // rows of A = R^T L, L = [[l11 0 0] [l12 l22 0] [l13 l23 l33]] so that q = |L^T d|^2
-        const float* a = fr.axis_lat;
-        const float* c = fr.axis_ax;
-        const float* e = fr.axis_ele;
-        const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
-        const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
-        const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
-        const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
-        const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
-        const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
-        const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
-        const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
-        const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
-        const double cz = C0 * C0 + C1 * C1 + C2 * C2;
-        const double det = M00 * M11 - M01 * M01;
-        if (det > 1e-8 * M00 * M11) {
-            const double j00 = M11 / det, j11 = M00 / det, j01 = -M01 / det;
-            const double w0 = j00 * b0 + j01 * b1, w1 = j01 * b0 + j11 * b1;
-            u_lat = (float)(-w0);
-            u_ax = (float)(-w1);
-            i00 = (float)j00 * 1.0001f;
-            i11 = (float)j11 * 1.0001f;
-            kk = (float)fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
-        }
-    }
-    s0 = make_float4(wl, wa, u_lat, u_ax);
-    s1 = make_float4(i00, i11, kk, pad);
+    const float pad = sub_pad(lat_g, ax_g);
+    s0 = make_float4(r_lat * 1.001f + pad, r_ax * 1.001f + pad, -1.f, 0.f);
+    if (!axes_orthonormal(fr)) return;
+    // rows of A = R^T L, L = [[l11 0 0] [l12 l22 0] [l13 l23 l33]] so that q = |L^T d|^2
+    const float* a = fr.axis_lat;
+    const float* c = fr.axis_ax;
+    const float* e = fr.axis_ele;
+    const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
+    const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
+    const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
+    const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
+    const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
+    const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
+    const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
+    const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
+    const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
+    const double cz = C0 * C0 + C1 * C1 + C2 * C2;
+    const double det = M00 * M11 - M01 * M01;
+    if (!(det > 1e-8 * M00 * M11)) return;
+    const double w0 = (M11 * b0 - M01 * b1) / det, w1 = (M00 * b1 - M01 * b0) / det;
+    const double kk = fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
+    const double pp = sqrt(M00);
+    s0 = make_float4((float)(-w0), (float)(-w1), (float)kk, (float)(M01 / M11));
+    s1 = make_float4((float)pp, (float)(M01 / pp), (float)sqrt(det / M00), (float)(M01 / M00));
 }
 
 __global__ __launch_bounds__(256) void ug_preprocess_kernel(
@@ -454,34 +453,48 @@ __device__ __forceinline__ SubBounds sub_bounds(bool valid, float lat, float ax,
     return b;
 }
 
-// Sub-block mask of one instance from its pair's section setup (pair_section): the marginal box test, then,
-// where the setup allows it (kk >= 0), the section test over the sub-block's elevation range.
-// chi2w = sqrt_chi2^2 * 1.002.
+// Sub-block mask of one instance from its pair's setup (pair_section); chi2w = sqrt_chi2^2 * 1.002.
+// Section test: at elevation offset d the pixels with q <= chi2 form the ellipse f(x - c(d)) <= chi2 - kk d^2,
+// c(d) = (lat_g, ax_g) + (u_lat, u_ax) d. Over the sub-block's elevation range all of them lie in the ellipse of
+// the largest level (d* = the offset closest to 0) moved along the centre segment, so the sub-block can hold
+// such a pixel only if that ellipse meets its rectangle shifted by the segment (and widened by the pad). The
+// minimum of the convex f over a rectangle not containing its centre lies on the edge x = x_n or the edge
+// y = y_n nearest the centre (x_n, y_n = 0 clamped into the rectangle; a minimiser on a far edge or corner would
+// make f <= 0 there by the optimality conditions), and along an edge at the clamped 1-D minimiser, so
+// min(f(x_n, y_a), f(x_b, y_n)) is the exact minimum; it is compared with the level widened by 0.4 %.
 template <int NSUB>
 __device__ __forceinline__ uint32_t sub_mask(const SubBounds* __restrict__ sb, float4 s0, float4 s1, float lat_g,
                                              float ax_g, float ele_g, float chi2w) {
-    const float wl = s0.x, wa = s0.y, u_lat = s0.z, u_ax = s0.w;
-    const float i00 = s1.x, i11 = s1.y, kk = s1.z, pad = s1.w;
+    const float kk = s0.z;
     uint32_t m = 0;
+    if (kk < 0.0f) {  // marginal box only
+#pragma unroll
+        for (int s = 0; s < NSUB; ++s) {
+            const bool hit = (lat_g + s0.x >= sb[s].lat_lo) && (lat_g - s0.x <= sb[s].lat_hi) &&
+                             (ax_g + s0.y >= sb[s].ax_lo) && (ax_g - s0.y <= sb[s].ax_hi);
+            m |= (hit ? 1u : 0u) << s;
+        }
+        return m;
+    }
+    const float u_lat = s0.x, u_ax = s0.y, t_x = s0.w;
+    const float p = s1.x, q = s1.y, sy = s1.z, t_y = s1.w;
+    const float pad = sub_pad(lat_g, ax_g);
 #pragma unroll
     for (int s = 0; s < NSUB; ++s) {
-        bool hit = (lat_g + wl >= sb[s].lat_lo) && (lat_g - wl <= sb[s].lat_hi) && (ax_g + wa >= sb[s].ax_lo) &&
-                   (ax_g - wa <= sb[s].ax_hi);
-        if (hit && kk >= 0.0f) {
-            const float d_lo = sb[s].ele_lo - ele_g, d_hi = sb[s].ele_hi - ele_g;
-            const float d0 = fminf(fmaxf(0.0f, d_lo), d_hi);
-            const float rem = chi2w - kk * d0 * d0;
-            if (rem <= 0.0f) {
-                hit = false;
-            } else {
-                const float hl = sqrtf(rem * i00) * 1.002f + pad;
-                const float ha = sqrtf(rem * i11) * 1.002f + pad;
-                const float cl0 = lat_g + u_lat * d_lo, cl1 = lat_g + u_lat * d_hi;
-                const float ca0 = ax_g + u_ax * d_lo, ca1 = ax_g + u_ax * d_hi;
-                hit = (fmaxf(cl0, cl1) + hl >= sb[s].lat_lo) && (fminf(cl0, cl1) - hl <= sb[s].lat_hi) &&
-                      (fmaxf(ca0, ca1) + ha >= sb[s].ax_lo) && (fminf(ca0, ca1) - ha <= sb[s].ax_hi);
-            }
-        }
+        const float d_lo = sb[s].ele_lo - ele_g, d_hi = sb[s].ele_hi - ele_g;
+        const float d0 = fminf(fmaxf(0.0f, d_lo), d_hi);
+        const float rem = chi2w - kk * d0 * d0;
+        const float cl0 = lat_g + u_lat * d_lo, cl1 = lat_g + u_lat * d_hi;
+        const float ca0 = ax_g + u_ax * d_lo, ca1 = ax_g + u_ax * d_hi;
+        const float x0 = sb[s].lat_lo - fmaxf(cl0, cl1) - pad, x1 = sb[s].lat_hi - fminf(cl0, cl1) + pad;
+        const float y0 = sb[s].ax_lo - fmaxf(ca0, ca1) - pad, y1 = sb[s].ax_hi - fminf(ca0, ca1) + pad;
+        const float xn = fminf(fmaxf(0.0f, x0), x1), yn = fminf(fmaxf(0.0f, y0), y1);
+        const float ya = fminf(fmaxf(-t_x * xn, y0), y1);
+        const float xb = fminf(fmaxf(-t_y * yn, x0), x1);
+        const float u1 = p * xn + q * ya, v1 = sy * ya;
+        const float u2 = p * xb + q * yn, v2 = sy * yn;
+        const float f = fminf(u1 * u1 + v1 * v1, u2 * u2 + v2 * v2);
+        const bool hit = sb[s].lat_lo <= sb[s].lat_hi && rem > 0.0f && f <= rem * 1.004f;
         m |= (hit ? 1u : 0u) << s;
     }
     return m;
