@@ -415,3 +415,66 @@ def test_fan_grid_in_kernel_matches_explicit_positions(dev):
     assert rel_l2(img_p, img_e) < 1e-5
     for a, b in zip(g_p, g_e):
         assert rel_l2(a, b) < 1e-5
+
+
+@pytest.mark.parametrize("ts", [16, 32, 8])
+def test_sub_block_masks_keep_every_contributing_pair(dev, ts):
+    """The splat kernels skip, per 8x8 sub-block, the listed Gaussians whose plane section cannot reach the
+    sub-block's pixels (the section ellipse of largest level over the sub-block's elevation range, moved along
+    its centre segment, tested exactly against the pixels' bounding rectangle). Strongly anisotropic Gaussians
+    (scales ~0.03-5 mm) packed around two oblique frames, rendered from explicit pixel positions jittered in
+    and out of the plane: every pixel must match the oracle's evaluation of its candidate list, so no pair with
+    q <= chi2 at a pixel may be skipped (a skipped pair moves alpha by its alpha_hat)."""
+    from us3d.scenes import Scene, precision_tril_from_scales_quats
+
+    frames = straight_sweep(2, 96, 80, 0.1, 64.0, 6.0, 25.0, seed=13)
+    rng = np.random.default_rng(11)
+    G = 2500
+    base = make_scene(G, 64.0, seed=12, log_scale_mean=float(np.log(0.4)), log_scale_std=1.0)
+    fr0 = frames[0]
+    o, al, aa, ae = (np.asarray(v, np.float64) for v in (fr0.origin, fr0.axis_lat, fr0.axis_ax, fr0.axis_ele))
+    lat = rng.uniform(-fr0.col_offset * fr0.delta_lat - 1.0, (fr0.width - fr0.col_offset) * fr0.delta_lat + 1.0, G)
+    ax = rng.uniform(fr0.near_mm - 1.0, fr0.near_mm + fr0.height * fr0.delta_ax + 1.0, G)
+    ele = rng.uniform(-4.0, 4.0, G)
+    means = torch.from_numpy((o + lat[:, None] * al + ax[:, None] * aa + ele[:, None] * ae).astype(np.float32))
+    scales = base.scales.clamp(0.03, 5.0)
+    scene = Scene(means, scales, base.quats, base.opacity, base.intensity,
+                  precision_tril_from_scales_quats(scales, base.quats))
+    pix = []
+    for f in frames:
+        n = f.height * f.width
+        jit = rng.uniform(-1.0, 1.0, size=(n, 3)) * np.array([0.03, 0.03, 0.15])
+        a3 = np.stack([np.asarray(v, np.float64) for v in (f.axis_lat, f.axis_ax, f.axis_ele)])
+        pix.append((linear_pixels(f).double() + torch.from_numpy(jit @ a3)).float())
+    sem = oug.Semantics()
+    cfg = cfg_for(frames[0], tile_size=ts)
+    P = [f.height * f.width for f in frames]
+    dI = rng.normal(size=sum(P)).astype(np.float32)
+    dA = (rng.normal(size=sum(P)) * 0.1).astype(np.float32)
+    from us3d.ultragauss import render_frames
+
+    tens = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    image, alpha, _depth = render_frames(*tens, torch.cat(pix).to(dev), frames, cfg,
+                                         sweeps=[SweepSpec(0, G)], hint=CapacityHint())
+    ((image * torch.from_numpy(dI).to(dev)).sum() + (alpha * torch.from_numpy(dA).to(dev)).sum()).backward()
+    params = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy())
+    refs = [np.zeros((G, 3)), np.zeros((G, 6)), np.zeros(G), np.zeros(G)]
+    off = 0
+    for f, px in zip(frames, pix):
+        n = f.height * f.width
+        geom = oug.FrameGeom.from_spec(f)
+        mr = oug.map_frame(params[0], params[1], geom, chi2=sem.chi2, beta=sem.beta, ts=ts)
+        lists = oug.pixel_lists(params[0], params[1], px.numpy(), geom, mr, chi2=sem.chi2, beta=sem.beta, ts=ts)
+        assert int(lists[0].sum()) > 20 * n  # a dense case: many candidates per pixel
+        args = params + (px.numpy(), geom, lists, sem)
+        img32, a32, _d, _s = oug.forward(*args)
+        got_i = image[off:off + n].detach().cpu().numpy()
+        got_a = alpha[off:off + n].detach().cpu().numpy()
+        assert np.all(np.abs(got_a - a32) <= 2e-5 * np.maximum(1.0, np.abs(a32)))
+        assert np.abs(got_i - img32).max() <= 2e-5
+        for acc, r in zip(refs, oug.backward_f64(*args, dI[off:off + n], dA[off:off + n], None, gate=True)):
+            acc += r
+        off += n
+    for name, t, r in zip(("means", "tril", "opacity", "intensity"), tens, refs):
+        err = rel_l2(t.grad.detach().cpu().numpy(), r)
+        assert err < GRAD_TOL, f"ts {ts} {name}: rel L2 {err:.3e}"
