@@ -84,6 +84,7 @@ __global__ void fill_f32_kernel(float* dst, int64_t n, float v) {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanBlock = kScanThreads * kScanItems;
+static_assert(kScanItems == 8, "the vector path of scan_block_kernel moves 8 int32 items as two int4");
 
 template <typename TIn, typename TOut>
 __global__ __launch_bounds__(kScanThreads) void scan_block_kernel(const TIn* __restrict__ in, TOut* __restrict__ out,
@@ -94,12 +95,26 @@ __global__ __launch_bounds__(kScanThreads) void scan_block_kernel(const TIn* __r
     const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanItems;
     int64_t v[kScanItems];
     int64_t sum = 0;
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        v[k] = (i < n) ? (int64_t)in[i] : 0;
-        sum += v[k];
+    // int32 items of a full chunk are read as two 16 B loads (chunks start 32 B apart from a 16 B aligned base)
+    bool vec_in = false;
+    if constexpr (sizeof(TIn) == 4) {
+        vec_in = base + kScanItems <= n && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+        if (vec_in) {
+            const int4 a = *reinterpret_cast<const int4*>(in + base);
+            const int4 b = *reinterpret_cast<const int4*>(in + base + 4);
+            v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+            v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+        }
     }
+    if (!vec_in) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t i = base + k;
+            v[k] = (i < n) ? (int64_t)in[i] : 0;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) sum += v[k];
     // inclusive wave scan of per-thread sums
     const int lane = tid & (kWave - 1);
     const int wave = tid / kWave;
@@ -114,11 +129,27 @@ __global__ __launch_bounds__(kScanThreads) void scan_block_kernel(const TIn* __r
     int64_t wave_base = 0;
     for (int w = 0; w < wave; ++w) wave_base += wave_tot[w];
     int64_t run = wave_base + incl - sum;  // exclusive prefix of this thread
+    bool vec_out = false;
+    if constexpr (sizeof(TOut) == 4) {
+        vec_out = base + kScanItems <= n && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+        if (vec_out) {
+            int32_t o[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int64_t i = base + k;
-        if (i < n) out[i] = (TOut)run;
-        run += v[k];
+            for (int k = 0; k < kScanItems; ++k) {
+                o[k] = (int32_t)run;
+                run += v[k];
+            }
+            *reinterpret_cast<int4*>(out + base) = make_int4(o[0], o[1], o[2], o[3]);
+            *reinterpret_cast<int4*>(out + base + 4) = make_int4(o[4], o[5], o[6], o[7]);
+        }
+    }
+    if (!vec_out) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t i = base + k;
+            if (i < n) out[i] = (TOut)run;
+            run += v[k];
+        }
     }
     if (tid == kScanThreads - 1) {
         int64_t block_total = 0;
