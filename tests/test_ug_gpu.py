@@ -478,3 +478,68 @@ def test_sub_block_masks_keep_every_contributing_pair(dev, ts):
     for name, t, r in zip(("means", "tril", "opacity", "intensity"), tens, refs):
         err = rel_l2(t.grad.detach().cpu().numpy(), r)
         assert err < GRAD_TOL, f"ts {ts} {name}: rel L2 {err:.3e}"
+
+
+def test_config5_fan_gradients_vs_fallback_on_64_frames(dev):
+    """Config 5 at full size: 2M Gaussians in a 200 x 150 x 150 mm box, 640 x 480 fan frames (60 degrees,
+    40-160 mm), fallback semantics (chi2 = 120, the reference fallback renderer's; the only reference semantics
+    defined for arbitrary pixel positions). A fixed 64-frame subset of the 1000-frame sweep (every 15th frame)
+    is rendered on the GPU in batches of 4 frames (30-60M instances per frame) with the loss gradient nonzero
+    on 16 fixed pixels per frame. The oracle evaluates those pixels against every Gaussian with q <= 120; the
+    candidates come from an FP64 prefilter with a 1 % margin, and a pair beyond it contributes at most
+    o * exp(-60) in the fallback renderer and nothing in the kernels. Image at the sampled pixels and the
+    accumulated gradients of all four parameter groups must match."""
+    from oracle.fallback import build_precision
+    from us3d.scenes import fan_pixels, fan_sweep
+    from us3d.ultragauss import render_frames
+
+    box = [200.0, 150.0, 150.0]
+    scene = make_scene(2_000_000, 150.0, seed=0, box_mm=box)
+    frames = fan_sweep(1000, 480, 640, 60.0, 40.0, 160.0, box, 150.0, seed=2)[::15][:64]
+    G, n_pix, n_samp = scene.means.shape[0], 480 * 640, 16
+    sem = oug.Semantics.fallback()
+    cfg = cfg_for(frames[0], semantics="fallback")
+    rng = np.random.default_rng(7)
+    samples = [np.sort(rng.choice(n_pix, n_samp, replace=False)) for _ in frames]
+    dI = [rng.normal(size=n_samp).astype(np.float32) for _ in frames]
+    tens = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    hint = CapacityHint()
+    got_img = []
+    for b0 in range(0, len(frames), 4):
+        fb = frames[b0:b0 + 4]
+        image, _alpha, _depth = render_frames(*tens, None, fb, cfg, sweeps=[SweepSpec(0, G)], hint=hint)
+        g = torch.zeros(len(fb) * n_pix, dtype=torch.float32)
+        for k in range(len(fb)):
+            g[k * n_pix + samples[b0 + k]] = torch.from_numpy(dI[b0 + k])
+        (image * g.to(dev)).sum().backward()
+        img = image.detach().cpu().numpy()
+        got_img += [img[k * n_pix + samples[b0 + k]] for k in range(len(fb))]
+        del image, _alpha, _depth
+    lam = build_precision(scene.tril.to(dev).double(), sem.beta)
+    means64 = scene.means.to(dev).double()
+    params = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy())
+    refs = [np.zeros((G, 3)), np.zeros((G, 6)), np.zeros(G), np.zeros(G)]
+    ref_img, gpu_img = [], []
+    for k, f in enumerate(frames):
+        px = fan_pixels(f)[samples[k]]
+        d = px.to(dev).double()[:, None, :] - means64[None]
+        q = (torch.einsum("pgi,gij->pgj", d, lam) * d).sum(-1)
+        keep = q <= sem.chi2 * 1.01
+        counts = keep.sum(1).to(torch.int32).cpu().numpy()
+        indices = keep.nonzero()[:, 1].to(torch.int32).cpu().numpy()
+        offsets = np.concatenate([[0], np.cumsum(counts)[:-1]]).astype(np.int32)
+        args = params + (px.numpy(), oug.FrameGeom.from_spec(f), (counts, offsets, indices), sem)
+        img32, _a, _d, _s = oug.forward(*args)
+        ref_img.append(img32)
+        gpu_img.append(got_img[k])
+        for acc, r in zip(refs, oug.backward_f64(*args, dI[k], None, None, gate=True)):
+            acc += r
+    img_err = rel_l2(np.concatenate(gpu_img), np.concatenate(ref_img))
+    errs = {name: rel_l2(t.grad.detach().cpu().numpy(), r)
+            for name, t, r in zip(("means", "tril", "opacity", "intensity"), tens, refs)}
+    print(f"[config 5, 64 frames] image rel L2 {img_err:.2e}; gradients rel L2 "
+          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    assert img_err < IMG_TOL
+    for name, r in zip(errs, refs):
+        assert np.abs(r).max() > 0, name
+        assert errs[name] < GRAD_TOL, f"config 5 {name}: rel L2 {errs[name]:.3e}"
