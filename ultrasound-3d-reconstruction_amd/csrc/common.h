@@ -217,6 +217,25 @@ __device__ __forceinline__ void for_each_wave_instance(int count, int4 rect, int
     }
 }
 
+// Counts a pair's instances into its frame's tiles without one atomic per instance: the rectangle
+// [r.x, r.z) x [r.y, r.w) of tiles goes into a 2-D difference array over the frame's tiles (+1 at the first
+// corner, -1 at the two corners past it along one axis, +1 past both; corners beyond the frame's last column or
+// row are dropped, the prefix sums never reach them). prefix_tile_counts_kernel then turns each frame's array
+// into per-tile counts.
+__device__ __forceinline__ void add_rect_corners(int32_t* __restrict__ tile_diff, int tile_base, int tiles_x,
+                                                 int tiles_y, int4 r) {
+    int32_t* a = tile_diff + tile_base;
+    atomicAdd(&a[r.y * tiles_x + r.x], 1);
+    if (r.z < tiles_x) atomicAdd(&a[r.y * tiles_x + r.z], -1);
+    if (r.w < tiles_y) {
+        atomicAdd(&a[r.w * tiles_x + r.x], -1);
+        if (r.z < tiles_x) atomicAdd(&a[r.w * tiles_x + r.z], 1);
+    }
+}
+
+// One workgroup per frame: inclusive 2-D prefix sum of the frame's tile difference array, in place.
+__global__ void prefix_tile_counts_kernel(const us3d_frame* __restrict__ frames, int32_t* __restrict__ tile_count);
+
 __global__ void init_stats_kernel(us3d_stats* stats);
 
 __global__ void fill_i32_kernel(int32_t* dst, int64_t n, int32_t v);

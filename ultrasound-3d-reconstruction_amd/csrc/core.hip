@@ -72,6 +72,43 @@ __global__ void fill_i32_kernel(int32_t* dst, int64_t n, int32_t v) {
         dst[i] = v;
 }
 
+constexpr int kPrefixThreads = 256;
+constexpr int kPrefixLds = 16384;  // tiles of a frame summed in LDS; larger frames are summed in place
+
+__global__ __launch_bounds__(kPrefixThreads) void prefix_tile_counts_kernel(const us3d_frame* __restrict__ frames,
+                                                                            int32_t* __restrict__ tile_count) {
+    __shared__ int32_t s_a[kPrefixLds];
+    const us3d_frame& fr = frames[blockIdx.x];
+    const int tx = fr.tiles_x, ty = fr.tiles_y;
+    const int n = tx * ty;
+    int32_t* g = tile_count + fr.tile_base;
+    const bool lds = n <= kPrefixLds;
+    int32_t* a = lds ? s_a : g;
+    if (lds) {
+        for (int i = threadIdx.x; i < n; i += kPrefixThreads) s_a[i] = g[i];
+        __syncthreads();
+    }
+    for (int r = threadIdx.x; r < ty; r += kPrefixThreads) {  // along each row
+        int32_t s = 0;
+        for (int x = 0; x < tx; ++x) {
+            s += a[r * tx + x];
+            a[r * tx + x] = s;
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < tx; c += kPrefixThreads) {  // then down each column
+        int32_t s = 0;
+        for (int y = 0; y < ty; ++y) {
+            s += a[y * tx + c];
+            a[y * tx + c] = s;
+        }
+    }
+    if (lds) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += kPrefixThreads) g[i] = s_a[i];
+    }
+}
+
 __global__ void fill_f32_kernel(float* dst, int64_t n, float v) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = v;

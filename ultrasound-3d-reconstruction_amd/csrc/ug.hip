@@ -198,76 +198,67 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
     float4* __restrict__ pair_sect, int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool in = pair < n_pairs;
-    if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
+    if (pair >= n_pairs) return;
     int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
-    int tile_base = 0, tiles_x = 1;
-    if (in) {
-        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
-        const int gl = local / sw.f_count;
-        const int g = sw.g_begin + gl;
-        const int f = sw.f_begin + (local - gl * sw.f_count);
-        const us3d_frame& fr = frames[f];
-        const float4 r0 = rec[3 * (int64_t)g + 0];
-        const float4 r1 = rec[3 * (int64_t)g + 1];
-        const float4 r2 = rec[3 * (int64_t)g + 2];
-        const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
-        const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
-        const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-        const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-        const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        uint32_t dk = 0;
-        const bool far = fabsf(ele) > r2.w;  // beyond every marginal radius (see ug_pack_kernel)
-        if (!far) {
-            const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
-            const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-            const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
-            if (fabsf(ele) <= r_ele) {
-                const float tsf = (float)ts;
-                float fx0, fy0, fx1, fy1, v;
-                if (fr.geometry == 1) {
-                    fan_tile_bounds(fr, lat, ax, r_lat, r_ax, tsf, fx0, fy0, fx1, fy1, v);
-                } else {
-                    const float u = lat / fr.delta_lat + fr.col_offset;
-                    v = (ax - fr.near_mm) / fr.delta_ax;
-                    const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
-                    const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
-                    fx0 = floorf((u - ru) / tsf);
-                    fy0 = floorf((v - rv) / tsf);
-                    fx1 = ceilf((u + ru) / tsf);
-                    fy1 = ceilf((v + rv) / tsf);
-                }
-                if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
-                    const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
-                    const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
-                    if (x0 < x1 && y0 < y1) {
-                        count = (x1 - x0) * (y1 - y0);
-                        rect = make_int4(x0, y0, x1, y1);
-                        dk = sort_by_depth ? depth_key(v) : 0u;
-                        float4 s0, s1;
-                        pair_section(l11, l12, l13, l22, l23, l33, lat, ax, r_lat, r_ax, fr, s0, s1);
-                        pair_sect[2 * pair + 0] = s0;
-                        pair_sect[2 * pair + 1] = s1;
-                    }
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+    const int gl = local / sw.f_count;
+    const int g = sw.g_begin + gl;
+    const int f = sw.f_begin + (local - gl * sw.f_count);
+    const us3d_frame& fr = frames[f];
+    const float4 r0 = rec[3 * (int64_t)g + 0];
+    const float4 r1 = rec[3 * (int64_t)g + 1];
+    const float4 r2 = rec[3 * (int64_t)g + 2];
+    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+    const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
+    const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+    const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+    const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+    uint32_t dk = 0;
+    const bool far = fabsf(ele) > r2.w;  // beyond every marginal radius (see ug_pack_kernel)
+    if (!far) {
+        const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+        const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+        const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+        if (fabsf(ele) <= r_ele) {
+            const float tsf = (float)ts;
+            float fx0, fy0, fx1, fy1, v;
+            if (fr.geometry == 1) {
+                fan_tile_bounds(fr, lat, ax, r_lat, r_ax, tsf, fx0, fy0, fx1, fy1, v);
+            } else {
+                const float u = lat / fr.delta_lat + fr.col_offset;
+                v = (ax - fr.near_mm) / fr.delta_ax;
+                const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+                const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+                fx0 = floorf((u - ru) / tsf);
+                fy0 = floorf((v - rv) / tsf);
+                fx1 = ceilf((u + ru) / tsf);
+                fy1 = ceilf((v + rv) / tsf);
+            }
+            if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
+                const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
+                const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
+                if (x0 < x1 && y0 < y1) {
+                    count = (x1 - x0) * (y1 - y0);
+                    rect = make_int4(x0, y0, x1, y1);
+                    dk = sort_by_depth ? depth_key(v) : 0u;
+                    float4 s0, s1;
+                    pair_section(l11, l12, l13, l22, l23, l33, lat, ax, r_lat, r_ax, fr, s0, s1);
+                    pair_sect[2 * pair + 0] = s0;
+                    pair_sect[2 * pair + 1] = s1;
                 }
             }
         }
-        pair_count[pair] = count;
-        if (count > 0) {  // rect and key are read only for pairs with instances
-            pair_rect[pair] = rect;
-            pair_dk[pair] = dk;
-            tile_base = fr.tile_base;
-            tiles_x = fr.tiles_x;
-            // frame-major copy of the visible pairs: entry gl of the frame's g_count-sized region
-            vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
-        }
     }
-    const uint32_t none[1] = {0u};
-    for_each_wave_instance<1>(count, rect, tile_base, tiles_x, none, [&](const uint32_t*, int, int tile) {
-        atomicAdd(&tile_count[tile], 1);
-    });
+    pair_count[pair] = count;
+    if (count > 0) {  // rect and key are read only for pairs with instances
+        pair_rect[pair] = rect;
+        pair_dk[pair] = dk;
+        add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
+        // frame-major copy of the visible pairs: entry gl of the frame's g_count-sized region
+        vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
+    }
 }
 
 __global__ void init_stats_kernel(us3d_stats* stats) {
@@ -1041,6 +1032,9 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     int32_t* tile_off = at<int32_t>(plan, L.tile_off);
     const int64_t np = plan->n_pairs;
     if (np > 0) {
+        hipLaunchKernelGGL(prefix_tile_counts_kernel, dim3(plan->n_frames), dim3(256), 0, stream, plan->frames,
+                           tile_count);
+        if ((rc = check_launch("prefix_tile_counts_kernel"))) return rc;
         if ((rc = scan_i32(pair_count, pair_off, np, &stats->n_instances, at<void>(plan, L.scan_tmp), stream))) return rc;
     } else {
         hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, pair_off, (int64_t)1, 0);

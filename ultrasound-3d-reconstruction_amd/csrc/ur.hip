@@ -64,55 +64,46 @@ __global__ __launch_bounds__(256) void ur_preprocess_kernel(
     int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
     int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    const bool in = pair < n_pairs;
-    if (__ballot(in) == 0) return;  // whole wavefront past the end (wave-uniform)
+    if (pair >= n_pairs) return;
     int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
-    int tile_base = 0, tiles_x = 1;
-    if (in) {
-        const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
-        const int gl = local / sw.f_count;
-        const int g = sw.g_begin + gl;
-        const int f = sw.f_begin + (local - gl * sw.f_count);
-        const us3d_frame& fr = frames[f];
-        const float4 r0 = rec[4 * (int64_t)g + 0];
-        const float4 r1 = rec[4 * (int64_t)g + 1];
-        const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
-        const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
-        const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-        const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-        const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
-        const float r = ms * k_sigma;
-        uint32_t dk = 0;
-        if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
-            const float c = x / fr.delta_lat + fr.col_offset;
-            const float rp = r / fr.delta_lat;
-            const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
-            if (isfinite(f0) && isfinite(f1)) {
-                const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
-                const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
-                if (lo <= hi) {
-                    count = hi - lo + 1;
-                    rect = make_int4(0, lo, 1, hi + 1);
-                    const float q = fmaxf(0.0f, z) * 1000.0f;
-                    dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
-                }
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+    const int gl = local / sw.f_count;
+    const int g = sw.g_begin + gl;
+    const int f = sw.f_begin + (local - gl * sw.f_count);
+    const us3d_frame& fr = frames[f];
+    const float4 r0 = rec[4 * (int64_t)g + 0];
+    const float4 r1 = rec[4 * (int64_t)g + 1];
+    const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
+    const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
+    const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+    const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
+    const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
+    const float r = ms * k_sigma;
+    uint32_t dk = 0;
+    if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
+        const float c = x / fr.delta_lat + fr.col_offset;
+        const float rp = r / fr.delta_lat;
+        const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
+        if (isfinite(f0) && isfinite(f1)) {
+            const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
+            const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
+            if (lo <= hi) {
+                count = hi - lo + 1;
+                rect = make_int4(0, lo, 1, hi + 1);
+                const float q = fmaxf(0.0f, z) * 1000.0f;
+                dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
             }
         }
-        pair_count[pair] = count;
-        if (count > 0) {  // rect and key are read only for pairs with instances
-            pair_rect[pair] = rect;
-            pair_dk[pair] = dk;
-            tile_base = fr.tile_base;
-            tiles_x = fr.tiles_x;
-            vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
-        }
     }
-    const uint32_t none[1] = {0u};
-    for_each_wave_instance<1>(count, rect, tile_base, tiles_x, none, [&](const uint32_t*, int, int tile) {
-        atomicAdd(&tile_count[tile], 1);
-    });
+    pair_count[pair] = count;
+    if (count > 0) {  // rect and key are read only for pairs with instances
+        pair_rect[pair] = rect;
+        pair_dk[pair] = dk;
+        add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
+        vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
+    }
 }
 
 // ---------------------------------------------------------------- per (ray, Gaussian) setup
