@@ -688,7 +688,7 @@ def main():
                 "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
                 "algorithmic_bytes_per_launch": alg_bytes, "stage_ms_per_launch": stage_ms,
                 "measured_copy_GBs": measure_copy_peak(),
-                "valu_issue_frac": read_valu_issue(f"{dom}<{ts}>") if (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
+                "valu_issue_frac": read_valu_issue(f"{dom}<{ts}, false>") if (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
                 else None}
 
     cpu = None
