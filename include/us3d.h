@@ -235,6 +235,49 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
                      const float* aux, const float* d_image, float* d_means, float* d_scales,
                      float* d_quats, float* d_attenuations, float* d_sh, void* stream);
 
+/* ---------------------------------------------------------------- GaussianPile us_focus slice splat */
+
+/* Settings of the GaussianPile renderer's "ours_focus" backend as its facade configures the reference's
+ * ProjectedSliceForwardCUDA (ops/gaussianplie_renderer/gaussianplie_renderer/autograd.py:17-55): shader weight
+ * US_FOCUS, tile mapper, additive compositing, constant elevational focus width, det_gamma 0.5. */
+typedef struct us3d_gp_settings {
+    float k_sigma;      /* radius_sigma: box half-widths k * sqrt(Sigma2_ii + |Sigma2_01|) */
+    float sigma_ele;    /* sigma_z_mm: elevational focus width */
+    float det_eps;      /* det_eps */
+    float weight_min;   /* max(opacity_cull_threshold, 1e-12): smaller weights are dropped */
+    int32_t tile_size;  /* 8, 16 or 32 (tile_size_px) */
+    int32_t reserved[3];
+} us3d_gp_settings;
+
+/* Workspace for a batch (same plan structure as UltraGauss: tiles of tile_size px per frame). */
+int64_t us3d_gp_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
+int us3d_gp_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                             us3d_layout* out);
+
+/* Mapper: replaces the tile mapper of ProjectedSliceForwardCUDA (src/shader/wrapper_shader.cu:107-129 ->
+ * OursTileGaussMapperCUDA, src/mapper/core_tile.cu:25-379) for shader weight US_FOCUS. means/scales [G,3],
+ * quats [G,4] (w, x, y, z, normalised inside), opacity [G], intensity [G] (NULL: all ones, as the facade
+ * passes). Frames use the linear-probe grid of us3d_frame (axes unit length). Per (frame, Gaussian) pair the
+ * focus footprint (us_focus_device.cuh:176-300); pairs whose peak weight is below weight_min are not listed
+ * (they contribute nothing); each tile lists its Gaussians in index order. */
+int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, const float* quats,
+                const float* opacity, const float* intensity, const us3d_gp_settings* settings, void* stream);
+
+/* Forward: replaces kernel_us_focus_tile2d_additive_forward (src/shader/forward.cu:114-210). image [n_pixels]
+ * = per pixel the FP32 sum, in list order, of intensity * o * slice_response * det2^-1/2 * exp(-quad/2) over
+ * the listed Gaussians inside their k-sigma box with weight >= weight_min. Requires us3d_gp_map on the same
+ * workspace. */
+int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, void* stream);
+
+/* Adjoint: replaces the US_FOCUS branch of ProjectedSliceBackwardCUDA (src/shader/wrapper_shader.cu; kernels in
+ * src/shader/us_focus.cu). The exact derivative of the forward for every parameter (the reference takes the
+ * scale and quaternion gradients by finite differences of its adjoint, us_focus.cu:695-743). Outputs
+ * (overwritten): d_means [G,3], d_scales [G,3], d_quats [G,4], d_opacity [G], d_intensity [G] (may be NULL).
+ * Requires the map of the same call sequence on the same workspace; deterministic (no atomics). */
+int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
+                     float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
+                     void* stream);
+
 /* ---------------------------------------------------------------- training-step helpers */
 
 /* Exclusive scan of n int32 values into out[0..n) and the int64 total into *total_dev (device).

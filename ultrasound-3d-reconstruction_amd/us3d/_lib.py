@@ -108,6 +108,17 @@ class URSettings(ctypes.Structure):
     ]
 
 
+class GPSettings(ctypes.Structure):
+    _fields_ = [
+        ("k_sigma", ctypes.c_float),
+        ("sigma_ele", ctypes.c_float),
+        ("det_eps", ctypes.c_float),
+        ("weight_min", ctypes.c_float),
+        ("tile_size", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 3),
+    ]
+
+
 _LAYOUT_FIELDS = [
     "stats", "pair_count", "pair_off", "pair_rect", "pair_dk", "tile_count", "tile_off", "tile_cursor",
     "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp", "vis_list", "inst_mask", "pair_sect", "total",
@@ -143,6 +154,11 @@ SIGNATURES = {
         ctypes.c_int,
         [_PP, _VP, _VP, _VP, _VP, ctypes.POINTER(URSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
+    "us3d_gp_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64]),
+    "us3d_gp_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
+    "us3d_gp_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(GPSettings), _VP]),
+    "us3d_gp_forward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP]),
+    "us3d_gp_backward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     "us3d_scan_tmp_bytes": (_I64, [_I64]),
     "us3d_scan_i32": (ctypes.c_int, [_VP, _VP, _I64, _VP, _VP, _VP]),
     "us3d_image_loss_tmp_bytes": (_I64, [_I32]),
