@@ -1,6 +1,6 @@
 """The training-loop counterpart end to end on the reference's own sweep data (3-frame fixture cut from
 data/synthetic_liver_l1_single_sweep.npz), with the smoke configuration of
-configs/synthetic_liver_l1_ultragauss_smoke.json, for both renderer methods."""
+configs/synthetic_liver_l1_ultragauss_smoke.json, for each renderer method this build provides."""
 
 import json
 from pathlib import Path
@@ -22,7 +22,7 @@ SMOKE = dict(data_npz=str(FIXTURE), device="cuda", seed=42, eval_every=0, save_e
              init_scale_mm=0.8, init_opacity=0.1, init_intensity=0.5)
 
 
-@pytest.mark.parametrize("method", ["ultragauss", "ultragray"])
+@pytest.mark.parametrize("method", ["ultragauss", "ultragray", "gaussianpile"])
 def test_pipeline_trains_on_reference_sweep(dev, tmp_path, method):
     cfg = GodConfig(method=method, output_dir=str(tmp_path / method), iterations=12, **SMOKE)
     ctx = train(cfg)
@@ -46,4 +46,4 @@ def test_pipeline_trains_on_reference_sweep(dev, tmp_path, method):
 
 def test_unsupported_method_raises(dev):
     with pytest.raises(ValueError):
-        RenderDispatcher(GodConfig(method="gaussianpile", **SMOKE))
+        RenderDispatcher(GodConfig(method="slice_adapted_3dgs", **SMOKE))

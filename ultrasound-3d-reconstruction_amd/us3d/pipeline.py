@@ -4,9 +4,9 @@ Same configuration keys (GodConfig, :60-132) and overrides (--config X.json --se
 same .npz data contract (DATA_FORMAT.md; NpzUltrasoundDataset, :172-333), same parameter container and
 activations (GaussianParameters, :340-399), same per-iteration loop (one random frame, L1 + w * (1 - global
 SSIM), Adam with one learning rate, CSV/eval/checkpoint callbacks, :640-770). Rendering goes through the
-dispatcher, whose `ultragauss` and `ultragray` methods are the MI355X kernels of this package (the drop-in
+dispatcher, whose `ultragauss`, `ultragray` and `gaussianpile` methods are the MI355X kernels of this package (the drop-in
 selected by `method=`); there is no CPU fallback renderer -- a machine without the HIP library fails loudly.
-Methods of the reference outside this build (gaussianpile, slice_adapted_3dgs, ultrags_baseline) raise.
+Methods of the reference outside this build (slice_adapted_3dgs, ultrags_baseline) raise.
 """
 
 from __future__ import annotations
@@ -169,9 +169,18 @@ class NpzUltrasoundDataset:
             u = torch.arange(self.w, device=device, dtype=torch.float32)
             ray_o = origin.view(1, 3) + ((u - centre) * float(cfg.delta_lateral_mm))[:, None] * lat.view(1, 3)
         ray_d = t(self.raw["ray_dirs"][idx]) if "ray_dirs" in self.raw else ax.view(1, 3).expand(self.w, 3).contiguous()
+        if "pose_world_to_probe" in self.raw:  # pipeline_god_file.py:308-318
+            arr = self.raw["pose_world_to_probe"]
+            pose = t(arr[idx] if arr.ndim == 3 else arr)
+        else:
+            rot = torch.stack([lat, ax, ele], dim=0)
+            pose = torch.eye(4, device=device, dtype=torch.float32)
+            pose[:3, :3] = rot
+            pose[:3, 3] = -rot.matmul(origin)
         return {"image": t(self.images[idx]), "frame_index": torch.tensor(idx, device=device),
                 "pixel_positions_world": pix, "probe_origin_world": origin, "axis_lateral_world": lat,
-                "axis_axial_world": ax, "axis_elevational_world": ele, "ray_origins": ray_o, "ray_dirs": ray_d}
+                "axis_axial_world": ax, "axis_elevational_world": ele, "ray_origins": ray_o, "ray_dirs": ray_d,
+                "pose_world_to_probe": pose}
 
     def initial_points(self, device: torch.device, num_points: int) -> Tuple[torch.Tensor, torch.Tensor]:
         if "init_means" in self.raw:
@@ -237,17 +246,18 @@ class RenderDispatcher:
     """Maps GaussianParameters + a frame dict to a renderer call, selected by cfg.method
     (RenderDispatcher, pipeline_god_file.py:402-536)."""
 
-    SUPPORTED = ("ultragauss", "ultragray")
+    SUPPORTED = ("ultragauss", "ultragray", "gaussianpile")
 
     def __init__(self, cfg: GodConfig):
         self.cfg = cfg
         self.method = cfg.method
         if self.method not in self.SUPPORTED:
             raise ValueError(f"method {self.method!r} is not provided by this build (supported: {self.SUPPORTED})")
+        from .baselines.gaussianpile import render as gp_render
         from .baselines.ultragauss import render as ug_render
         from .baselines.ultragray import render as ur_render
 
-        self.mod = ug_render if self.method == "ultragauss" else ur_render
+        self.mod = {"ultragauss": ug_render, "ultragray": ur_render, "gaussianpile": gp_render}[self.method]
 
     def render(self, params: GaussianParameters, frame: Dict[str, torch.Tensor]):
         cfg, m = self.cfg, self.mod
@@ -264,6 +274,19 @@ class RenderDispatcher:
                     lateral_spacing_mm_per_pix=cfg.delta_lateral_mm, axial_spacing_mm_per_pix=cfg.delta_axial_mm,
                     axial_near_mm=cfg.near_mm, lateral_col_offset=col, chi2_threshold=cfg.ultragauss_chi2_threshold,
                     diag_beta=cfg.ultragauss_diag_beta, return_pixel_lists=False),
+            )
+        if self.method == "gaussianpile":  # pipeline_god_file.py:435-455
+            return m.render_gaussianpile(
+                m.GaussianPileInput(params.means, params.scales_3d(), params.quats(), params.opacities().reshape(-1)),
+                m.SliceGeometryInput(frame["pixel_positions_world"], frame["probe_origin_world"],
+                                     frame["axis_lateral_world"], frame["axis_axial_world"],
+                                     frame["axis_elevational_world"], frame["pose_world_to_probe"]),
+                m.GaussianPileRenderSettings(
+                    image_height=cfg.image_height, image_width=cfg.image_width,
+                    lateral_spacing_mm_per_pix=cfg.delta_lateral_mm, axial_spacing_mm_per_pix=cfg.delta_axial_mm,
+                    axial_near_mm=cfg.near_mm, lateral_col_offset=col,
+                    # load_config's default (pipeline_god_file.py:146-147) for configs built directly
+                    sigma_z_mm=float(cfg.sigma_z_mm if cfg.sigma_z_mm is not None else cfg.delta_elevational_mm)),
             )
         pix = frame["pixel_positions_world"].reshape(cfg.image_height, cfg.image_width, 3).permute(1, 0, 2).contiguous()
         return m.render_ultragray(
