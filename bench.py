@@ -46,11 +46,12 @@ def parse():
     ap.add_argument("--cpu-sample-rows", type=int, default=16)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
-    ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan"), default="slice",
+    ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan", "gaussianpile"), default="slice",
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
                          "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
                          "multisweep: 8 sweeps of uneven size in one captured training step, tile size from the "
-                         "autotuner; fan: --fan-frames frames of a 1000-frame fan-probe sweep over 2M Gaussians")
+                         "autotuner; fan: --fan-frames frames of a 1000-frame fan-probe sweep over 2M Gaussians; "
+                         "gaussianpile: the GaussianPile us_focus splat on the config-2 scene and sweep")
     ap.add_argument("--ms-scale", type=float, default=1.0,
                     help="multisweep: scale factor on the sweeps' Gaussian and frame counts")
     ap.add_argument("--ray-frames", type=int, default=128)
@@ -470,6 +471,131 @@ def bench_fan(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_gaussianpile(args):
+    """The GaussianPile us_focus slice splat (the second slice model) on the config-2 scene: 200k Gaussians
+    (means, scales, quaternions, opacities) and the 256-frame 512x512 freehand sweep. One step = map + forward of
+    all frames in one batch, L1 loss against a target rendered from the seed-1 scene, adjoint (eager launches)."""
+    import torch
+
+    from us3d import _lib
+    from us3d.batch import CapacityHint, read_stats
+    from us3d.gaussianpile import GaussianPlieRenderConfig, render_frames
+    from us3d.scenes import config2
+
+    dev = torch.device("cuda:0")
+    scene, target, frames = config2(n_frames=args.frames, n_gauss=args.gaussians)
+    F = len(frames)
+    G = scene.means.shape[0]
+    H, W = frames[0].height, frames[0].width
+    P = F * H * W
+    fr0 = frames[0]
+    cfg = GaussianPlieRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=fr0.delta_lat,
+                                   axial_spacing_mm_per_pix=fr0.delta_ax, axial_near_mm=fr0.near_mm,
+                                   lateral_col_offset=fr0.col_offset, tile_size_px=args.tile_size)
+    hint = CapacityHint()
+    with torch.no_grad():
+        gt = render_frames(*[t.to(dev) for t in (target.means, target.scales, target.quats, target.opacity)], frames,
+                           cfg, hint=hint).clone()
+    params = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.scales, scene.quats, scene.opacity)]
+
+    def step(fr=frames, ref=gt):
+        for t in params:
+            t.grad = None
+        img = render_frames(*params, fr, cfg, hint=hint)
+        (img - ref).abs().mean().backward()
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0
+
+    dt = timed(step, args.steps, args.warmup)
+    fps = F * args.steps / dt
+    print(f"[bench] gaussianpile batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s", file=sys.stderr,
+          flush=True)
+    fps_plain = None
+    if not args.no_plain:
+        n_plain = min(8, F)
+        n = H * W
+
+        def plain():
+            for k in range(n_plain):
+                step([frames[k]], gt[k * n:(k + 1) * n])
+
+        dtp = timed(plain, 1, 1)
+        fps_plain = n_plain / dtp
+        print(f"[bench] gaussianpile plain (per frame): {fps_plain:.1f} frames/s", file=sys.stderr, flush=True)
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    for t in params:
+        t.grad = None
+    img, batch = render_frames(*params, frames, cfg, hint=hint, return_batch=True)
+    (img - gt).abs().mean().backward()
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    stages = _lib.stage_timing_read()
+    m = batch.map(*[t.detach() for t in params])
+    st = read_stats(m.workspace)
+    off = _lib.Layout()
+    lay = batch.layout
+    _lib.check(_lib.lib().us3d_gp_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, m.k_capacity, off),
+               "us3d_gp_workspace_layout")
+    g_vis = int((m.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
+    K = int(st.n_instances)
+    del m
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    # algorithmic bytes per launch: per instance its 4 B key and 32 B footprint gathered, per pixel the image
+    # written (forward) or its gradient read (adjoint), per visible pair its record-space gradient
+    launch_bytes = {"gp_forward_kernel": 36 * K + 4 * P, "gp_backward_kernel": 36 * K + 4 * P + 32 * g_vis}
+    alg = launch_bytes.get(dom)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import gp as ogp
+
+        torch.set_num_threads(os.cpu_count() or 1)
+        fr = frames[F // 2]
+        rows = 32
+        band = ogp.GPFrame(origin=fr.origin, axis_lat=fr.axis_lat, axis_ax=fr.axis_ax, axis_ele=fr.axis_ele, H=rows,
+                           W=W, delta_lat=fr.delta_lat, delta_ax=fr.delta_ax,
+                           near=fr.near_mm + (H // 2) * fr.delta_ax, col_offset=fr.col_offset)
+        ps = [t.double().clone().requires_grad_(True) for t in (scene.means, scene.scales, scene.quats, scene.opacity)]
+        t0 = time.perf_counter()
+        imgc = ogp.render(*ps, torch.ones(G, dtype=torch.float64), band, ogp.GPSettings(tile_size=args.tile_size),
+                          fp64=True)
+        imgc.sum().backward()
+        dtc = time.perf_counter() - t0
+        cpu = {"value": rows / H / dtc, "unit": "frames/s", "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"{rows} rows x {W} pixels (two tile rows) of frame {F // 2} against all {G} Gaussians: the "
+                         f"oracle's tile lists and forward, FP64 with torch autograd for the adjoint, {dtc:.1f} s; "
+                         f"frames/s = sampled fraction of a frame / time"}
+    line = {
+        "metric": "GaussianPile us_focus fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
+        "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"GaussianPile us_focus map + forward + adjoint, {G} Gaussians in a 120 mm cube, {F} "
+                               f"freehand frames {H}x{W} at 0.2 mm, radius_sigma 3, sigma_z 1 mm, L1 loss, batched "
+                               f"eager, tile {args.tile_size}",
+                   "frames": F, "gaussians": G, "instances": K, "visible_pairs": g_vis},
+        "plain_frames_per_s": fps_plain,
+        "batched_over_plain": (fps / fps_plain) if fps_plain else None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
+                     "stage_ms_per_launch": stage_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_multisweep(args):
     """8 sweeps (Gaussians log-uniform in 50k..500k, 32..400 frames, frame size 256x256 / 512x512 / 640x480) in
     ONE hipGraph-captured training step; the tile size is picked by the autotuner on a frame sample first."""
@@ -581,6 +707,9 @@ def main():
         return
     if args.workload == "fan":
         bench_fan(args)
+        return
+    if args.workload == "gaussianpile":
+        bench_gaussianpile(args)
         return
     import torch
 
