@@ -14,8 +14,10 @@ det_gamma = 0.5 and nothing is inflated.
 
 Two restatements share the formulas:
   * footprint_f32 / render_f32: torch float32, elementwise operations in the order the kernels use (no fused
-    multiply-adds), so the structural decisions (footprint validity, tile rectangles, box tests) match the
-    kernels bit for bit;
+    multiply-adds in the footprint), so the structural decisions (footprint validity, tile rectangles, box
+    tests) match the kernels bit for bit; the per-pixel weight b exp(-quad/2) is formed as the kernels form
+    it, b exp2(e) with e from the inverse covariance pre-scaled by -log2(e)/2 and three fused multiply-adds,
+    so a weight can differ from the kernels' only by exp2's last-ulp rounding;
   * render_f64: torch float64 and differentiable, the same decisions taken from the FP32 pass (a pixel-pair
     either contributes or not), so torch autograd gives the reference gradients.
 """
@@ -177,6 +179,17 @@ def tile_lists(fp, fr: GPFrame, st: GPSettings):
     return lists, tw, th
 
 
+# -log2(e)/2 in FP32: the kernels evaluate exp(-quad/2) as exp2(e) with e formed from the pre-scaled inverse
+EXP2_SCALE = -0.72134752044448170
+
+
+def _fma32(a, b, c):
+    """FP32 fused multiply-add: the exact product and sum in float64, one rounding to float32 (a second
+    rounding after float64's can differ from a single one only at an exact float64 tie, which these
+    operands practically never reach)."""
+    return (a.double() * b.double() + c.double()).float()
+
+
 def _pixel_grid(fr: GPFrame, dtype):
     cols = torch.arange(fr.W, dtype=torch.float32)
     rows = torch.arange(fr.H, dtype=torch.float32)
@@ -211,6 +224,8 @@ def render(means, scales, quats, opacity, intensity, fr: GPFrame, st: GPSettings
         out = torch.zeros((fr.H, fr.W), dtype=f32)
     ts = st.tile_size
     wmin = torch.tensor(st.weight_min, dtype=f32)
+    C = torch.tensor(EXP2_SCALE, dtype=f32)
+    c00, c01, c11 = fp32["i00"] * C, fp32["i01"] * C, fp32["i11"] * C
     tiles = []
     for t, lst in enumerate(lists):
         if not lst:
@@ -226,9 +241,14 @@ def render(means, scales, quats, opacity, intensity, fr: GPFrame, st: GPSettings
             dl32 = L32 - fp32["m0"][g]
             da32 = A32 - fp32["m1"][g]
             box = (torch.abs(dl32) <= fp32["r_lat"][g]) & (torch.abs(da32) <= fp32["r_ax"][g])
-            q32 = dl32 * (fp32["i00"][g] * dl32 + fp32["i01"][g] * da32) + da32 * (fp32["i01"][g] * dl32 + fp32["i11"][g] * da32)
-            w32 = fp32["b"][g] * torch.exp(-0.5 * q32)
-            m = box & (q32 >= 0) & torch.isfinite(q32) & torch.isfinite(w32) & (w32 >= wmin)
+            # e = C quad with (l, q) = C inv2 d, in the kernels' order: l = fma(c00, dl, c01 da),
+            # q = fma(c01, dl, c11 da), e = fma(dl, l, da q); w = b exp2(e). e <= 0 is quad >= 0 (NaN fails
+            # it); an infinite quad gives w = 0 < weight_min.
+            l32 = _fma32(c00[g], dl32, c01[g] * da32)
+            k32 = _fma32(c01[g], dl32, c11[g] * da32)
+            e32 = _fma32(dl32, l32, da32 * k32)
+            w32 = fp32["b"][g] * torch.exp2(e32)
+            m = box & (e32 <= 0) & (w32 >= wmin)
             if not bool(m.any()):
                 continue
             if fp64:

@@ -16,7 +16,10 @@
 //
 // The footprint is evaluated with the reference's FP32 operation order (no fused multiply-adds; the library
 // is built with -ffp-contract=off), so its decisions are those of the CPU restatement bit for bit. A pair
-// whose base weight is below weight_min is not listed: every weight of it is below weight_min.
+// whose base weight is below weight_min is not listed: every weight of it is below weight_min. Per pixel the
+// weight is b exp2(e) with e = dl l + da q, (l, q) = C inv2 (dl, da), C = -log2(e)/2 stored pre-multiplied
+// into the inverse covariance: three fused multiply-adds and one hardware exp2 instead of exp(-quad/2)
+// (the same value to a few FP32 ulps; the CPU restatement follows this order).
 //
 // The adjoint is the exact derivative of this forward for every parameter: per (tile, Gaussian) the
 // record-space gradients (base weight, intensity, 2-D mean, inverse covariance) without atomics, then per
@@ -35,6 +38,12 @@ template <typename T>
 static T* gp_at(const us3d_plan* plan, int64_t off) {
     return reinterpret_cast<T*>(reinterpret_cast<char*>(plan->workspace) + off);
 }
+
+// The pixel weight b exp(-quad/2) is evaluated as b exp2(e), e = C quad, C = -log2(e)/2 folded into the stored
+// inverse covariance: one hardware exp2 per (pixel, Gaussian). Its decisions (e <= 0, weight >= weight_min) are
+// made on the same FP32 numbers by the forward and the adjoint.
+constexpr float kGpExp2Scale = -0.72134752044448170f;
+constexpr float kGpLn2 = 0.69314718055994531f;
 
 __device__ __forceinline__ int gp_clamp_to_int(float x) {
     return (int)fminf(fmaxf(x, -1073741824.0f), 1073741824.0f);
@@ -224,8 +233,8 @@ __global__ __launch_bounds__(256) void gp_preprocess_kernel(
     if (count > 0) {
         pair_rect[pair] = rect;
         pair_dk[pair] = 0u;
-        pair_foot[2 * pair + 0] = make_float4(F.m0, F.m1, F.i00, F.i01);
-        pair_foot[2 * pair + 1] = make_float4(F.i11, F.r_lat, F.r_ax, F.b);
+        pair_foot[2 * pair + 0] = make_float4(F.m0, F.m1, F.i00 * kGpExp2Scale, F.i01 * kGpExp2Scale);
+        pair_foot[2 * pair + 1] = make_float4(F.i11 * kGpExp2Scale, F.r_lat, F.r_ax, F.b);
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
         vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
     }
@@ -269,24 +278,27 @@ __global__ __launch_bounds__(TS * TS) void gp_forward_kernel(
         }
         __syncthreads();
         for (int k = 0; k < n; ++k) {
-            const float4 a = s_f0[k], c = s_f1[k];  // (m0, m1, i00, i01), (i11, r_lat, r_ax, b)
+            const float4 a = s_f0[k], c = s_f1[k];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
             const float dl = xl - a.x, da = xa - a.y;
-            if (fabsf(dl) > c.y || fabsf(da) > c.z) continue;
-            const float ql = a.z * dl + a.w * da;
-            const float qa = a.w * dl + c.x * da;
-            const float quad = dl * ql + da * qa;
-            if (!(quad >= 0.f) || !isfinite(quad)) continue;
-            const float wgt = c.w * expf(-0.5f * quad);
-            if (!isfinite(wgt) || wgt < weight_min) continue;
-            acc += s_I[k] * wgt;
+            const float l = __builtin_fmaf(a.z, dl, a.w * da);
+            const float q = __builtin_fmaf(a.w, dl, c.x * da);
+            const float e = __builtin_fmaf(dl, l, da * q);
+            const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+            // box, quad >= 0 (a NaN fails it), weight >= weight_min (an infinite quad gives weight 0)
+            const bool in = fabsf(dl) <= c.y && fabsf(da) <= c.z && e <= 0.f && wgt >= weight_min;
+            acc += in ? s_I[k] * wgt : 0.f;
         }
     }
     if (valid) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc;
 }
 
 // ---------------------------------------------------------------- adjoint
-// Record-space gradients per (tile, Gaussian): a thread per instance walks the tile's pixels inside its box
-// and accumulates dL/db, dL/dI, dL/dmu2 and dL/dinv2; written at the instance's emit index (no atomics).
+// Record-space gradients per (tile, Gaussian). The work item is one half of the tile's rows for one instance:
+// the two items of an instance sit on adjacent lanes, each walks its rows of the instance's box and the two
+// sums are added with one lane swap (a fixed order), then the even lane writes the instance's partial at its
+// emit index (no atomics). With t = dL/dimage * w summed over the pixels, the intensity I factors out:
+//   dL/dI = sum t, dL/db = I sum t / b, dL/dinv2 = -I/2 sum t (dl^2, 2 dl da, da^2),
+//   dL/dmu2 = -2 ln2 I sum t (l, q) with (l, q) = C inv2 d, the pre-scaled half-gradient of e = C quad.
 template <int TS>
 __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
@@ -295,6 +307,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     float weight_min, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
     float4* __restrict__ partial) {
     constexpr int NT = TS * TS;
+    constexpr int HALF = TS / 2;
     __shared__ float s_g[NT];
     if (stats->overflow) return;
     const int t = blockIdx.x;
@@ -315,47 +328,57 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const int cx0 = tx * TS, cy0 = ty * TS;
     const int cw = min(TS, fr.width - cx0), ch = min(TS, fr.height - cy0);
-    for (int i = s0 + tid; i < s1; i += NT) {
+    const int n_items = 2 * (s1 - s0);
+    // NT is even, so both items of an instance run in the same iteration, on lanes tid and tid ^ 1
+    for (int it = tid; it < n_items; it += NT) {
+        const int i = s0 + (it >> 1);
+        const int half = it & 1;
         const int g = (int)(uint32_t)seg_key[i];
         const int64_t p = pair_g0 + (int64_t)g * sw.f_count;
-        const float4 a = pair_foot[2 * p + 0], c = pair_foot[2 * p + 1];
-        const float I = rec[3 * (int64_t)g + 1].w;
-        // pixel range of the box, widened by one pixel (the exact test below decides)
+        const float4 a = pair_foot[2 * p + 0], c = pair_foot[2 * p + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
+        // pixel range of the box, widened by one pixel (the exact test below decides), rows cut to this half
         const float u0 = (a.x - c.y) / fr.delta_lat + fr.col_offset, u1 = (a.x + c.y) / fr.delta_lat + fr.col_offset;
         const float v0 = (a.y - c.z - fr.near_mm) / fr.delta_ax, v1 = (a.y + c.z - fr.near_mm) / fr.delta_ax;
         const int xa0 = max(0, gp_clamp_to_int(floorf(u0)) - 1 - cx0), xa1 = min(cw - 1, gp_clamp_to_int(ceilf(u1)) + 1 - cx0);
-        const int ya0 = max(0, gp_clamp_to_int(floorf(v0)) - 1 - cy0), ya1 = min(ch - 1, gp_clamp_to_int(ceilf(v1)) + 1 - cy0);
-        float gb = 0.f, gI = 0.f, gm0 = 0.f, gm1 = 0.f, g00 = 0.f, g01 = 0.f, g11 = 0.f;
+        const int ya0 = max(half * HALF, gp_clamp_to_int(floorf(v0)) - 1 - cy0);
+        const int ya1 = min(min(ch - 1, half * HALF + HALF - 1), gp_clamp_to_int(ceilf(v1)) + 1 - cy0);
+        float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sP = 0.f, sQ = 0.f;
         for (int yy = ya0; yy <= ya1; ++yy) {
             const float xa_ = (float)(cy0 + yy) * fr.delta_ax + fr.near_mm;
             const float da = xa_ - a.y;
             if (fabsf(da) > c.z) continue;
+            const float c01da = a.w * da, c11da = c.x * da;
             for (int xx = xa0; xx <= xa1; ++xx) {
                 const float xl = ((float)(cx0 + xx) - fr.col_offset) * fr.delta_lat;
                 const float dl = xl - a.x;
-                if (fabsf(dl) > c.y) continue;
-                const float ql = a.z * dl + a.w * da;
-                const float qa = a.w * dl + c.x * da;
-                const float quad = dl * ql + da * qa;
-                if (!(quad >= 0.f) || !isfinite(quad)) continue;
-                const float phi = expf(-0.5f * quad);
-                const float wgt = c.w * phi;
-                if (!isfinite(wgt) || wgt < weight_min) continue;
-                const float go = s_g[yy * TS + xx];
-                const float gw = go * I;          // dL/dw
-                gI += go * wgt;
-                gb += gw * phi;
-                const float gq = -0.5f * gw * wgt;  // dL/dquad
-                g00 += gq * dl * dl;
-                g01 += gq * 2.f * dl * da;
-                g11 += gq * da * da;
-                gm0 -= gq * 2.f * ql;             // d quad / d dl = 2 ql, d dl / d m0 = -1
-                gm1 -= gq * 2.f * qa;
+                const float l = __builtin_fmaf(a.z, dl, c01da);
+                const float q = __builtin_fmaf(a.w, dl, c11da);
+                const float e = __builtin_fmaf(dl, l, da * q);
+                const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+                const bool in = fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
+                const float tw = in ? s_g[yy * TS + xx] * wgt : 0.f;
+                sT += tw;
+                const float td = tw * dl, ta = tw * da;
+                sA = __builtin_fmaf(td, dl, sA);
+                sB = __builtin_fmaf(td, da, sB);
+                sC = __builtin_fmaf(ta, da, sC);
+                sP = __builtin_fmaf(tw, l, sP);
+                sQ = __builtin_fmaf(tw, q, sQ);
             }
         }
-        const int64_t ei = seg_val[i];
-        partial[2 * ei + 0] = make_float4(gb, gI, gm0, gm1);
-        partial[2 * ei + 1] = make_float4(g00, g01, g11, 0.f);
+        sT += __shfl_xor(sT, 1, kWave);
+        sA += __shfl_xor(sA, 1, kWave);
+        sB += __shfl_xor(sB, 1, kWave);
+        sC += __shfl_xor(sC, 1, kWave);
+        sP += __shfl_xor(sP, 1, kWave);
+        sQ += __shfl_xor(sQ, 1, kWave);
+        if (half == 0) {
+            const float I = rec[3 * (int64_t)g + 1].w;
+            const float mI = -2.f * kGpLn2 * I;
+            const int64_t ei = seg_val[i];
+            partial[2 * ei + 0] = make_float4(I * sT / c.w, sT, mI * sP, mI * sQ);
+            partial[2 * ei + 1] = make_float4(-0.5f * I * sA, -I * sB, -0.5f * I * sC, 0.f);
+        }
     }
 }
 
