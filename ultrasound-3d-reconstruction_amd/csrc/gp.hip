@@ -241,14 +241,25 @@ __global__ __launch_bounds__(256) void gp_preprocess_kernel(
 }
 
 // ---------------------------------------------------------------- forward
-// One workgroup per (frame, tile), a thread per pixel; the tile's list staged through LDS in batches.
+// One workgroup per (frame, tile); a thread owns GpFwd<TS>::kPix pixels of one column (rows kRowStep apart),
+// so the instance's LDS reads and its lateral offset are shared by them. The tile's list is staged through
+// LDS in batches of one instance per thread.
 template <int TS>
-__global__ __launch_bounds__(TS * TS) void gp_forward_kernel(
+struct GpFwd {
+    static constexpr int kPix = (TS == 8) ? 1 : (TS == 16 ? 2 : 4);
+    static constexpr int kThreads = TS * TS / kPix;
+    static constexpr int kRowStep = TS / kPix;
+};
+
+template <int TS>
+__global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
     const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min,
     const us3d_stats* __restrict__ stats, float* __restrict__ image) {
-    constexpr int NT = TS * TS;
+    using S = GpFwd<TS>;
+    constexpr int NT = S::kThreads;
+    constexpr int NP = S::kPix;
     __shared__ float4 s_f0[NT], s_f1[NT];
     __shared__ float s_I[NT];
     if (stats->overflow) return;
@@ -258,14 +269,17 @@ __global__ __launch_bounds__(TS * TS) void gp_forward_kernel(
     const int lt = t - fr.tile_base;
     const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
     const int tid = threadIdx.x;
-    const int ix = tx * TS + tid % TS, iy = ty * TS + tid / TS;
-    const bool valid = ix < fr.width && iy < fr.height;
+    const int ix = tx * TS + tid % TS, iy0 = ty * TS + tid / TS;
     const float xl = ((float)ix - fr.col_offset) * fr.delta_lat;
-    const float xa = (float)iy * fr.delta_ax + fr.near_mm;
+    float xa[NP], acc[NP];
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+        xa[r] = (float)(iy0 + r * S::kRowStep) * fr.delta_ax + fr.near_mm;
+        acc[r] = 0.f;
+    }
     const us3d_sweep& sw = sweeps[fr.sweep];
     const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    float acc = 0.f;
     for (int base = s0; base < s1; base += NT) {
         const int n = min(NT, s1 - base);
         __syncthreads();
@@ -279,17 +293,27 @@ __global__ __launch_bounds__(TS * TS) void gp_forward_kernel(
         __syncthreads();
         for (int k = 0; k < n; ++k) {
             const float4 a = s_f0[k], c = s_f1[k];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
-            const float dl = xl - a.x, da = xa - a.y;
-            const float l = __builtin_fmaf(a.z, dl, a.w * da);
-            const float q = __builtin_fmaf(a.w, dl, c.x * da);
-            const float e = __builtin_fmaf(dl, l, da * q);
-            const float wgt = c.w * __builtin_amdgcn_exp2f(e);
-            // box, quad >= 0 (a NaN fails it), weight >= weight_min (an infinite quad gives weight 0)
-            const bool in = fabsf(dl) <= c.y && fabsf(da) <= c.z && e <= 0.f && wgt >= weight_min;
-            acc += in ? s_I[k] * wgt : 0.f;
+            const float I = s_I[k];
+            const float dl = xl - a.x;
+            const bool lat_in = fabsf(dl) <= c.y;
+#pragma unroll
+            for (int r = 0; r < NP; ++r) {
+                const float da = xa[r] - a.y;
+                const float l = __builtin_fmaf(a.z, dl, a.w * da);
+                const float q = __builtin_fmaf(a.w, dl, c.x * da);
+                const float e = __builtin_fmaf(dl, l, da * q);
+                const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+                // box, quad >= 0 (a NaN fails it), weight >= weight_min (an infinite quad gives weight 0)
+                const bool in = lat_in && fabsf(da) <= c.z && e <= 0.f && wgt >= weight_min;
+                acc[r] += in ? I * wgt : 0.f;
+            }
         }
     }
-    if (valid) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc;
+#pragma unroll
+    for (int r = 0; r < NP; ++r) {
+        const int iy = iy0 + r * S::kRowStep;
+        if (ix < fr.width && iy < fr.height) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc[r];
+    }
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -308,7 +332,9 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     float4* __restrict__ partial) {
     constexpr int NT = TS * TS;
     constexpr int HALF = TS / 2;
-    __shared__ float s_g[NT];
+    static_assert(TS % 4 == 0, "rows of the staged gradient are read as float4");
+    __shared__ __align__(16) float s_g[NT];  // dL/dimage of the tile's pixels
+    __shared__ __align__(16) float s_xl[TS];  // lateral position of each column
     if (stats->overflow) return;
     const int t = blockIdx.x;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
@@ -318,16 +344,16 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const int lt = t - fr.tile_base;
     const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
     const int tid = threadIdx.x;
+    const int cx0 = tx * TS, cy0 = ty * TS;
     {
-        const int ix = tx * TS + tid % TS, iy = ty * TS + tid / TS;
+        const int ix = cx0 + tid % TS, iy = cy0 + tid / TS;
         s_g[tid] = (ix < fr.width && iy < fr.height) ? d_image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix]
                                                      : 0.f;
+        if (tid < TS) s_xl[tid] = ((float)(cx0 + tid) - fr.col_offset) * fr.delta_lat;
     }
     __syncthreads();
     const us3d_sweep& sw = sweeps[fr.sweep];
     const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
-    const int cx0 = tx * TS, cy0 = ty * TS;
-    const int cw = min(TS, fr.width - cx0), ch = min(TS, fr.height - cy0);
     const int n_items = 2 * (s1 - s0);
     // NT is even, so both items of an instance run in the same iteration, on lanes tid and tid ^ 1
     for (int it = tid; it < n_items; it += NT) {
@@ -336,34 +362,40 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         const int g = (int)(uint32_t)seg_key[i];
         const int64_t p = pair_g0 + (int64_t)g * sw.f_count;
         const float4 a = pair_foot[2 * p + 0], c = pair_foot[2 * p + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
-        // pixel range of the box, widened by one pixel (the exact test below decides), rows cut to this half
-        const float u0 = (a.x - c.y) / fr.delta_lat + fr.col_offset, u1 = (a.x + c.y) / fr.delta_lat + fr.col_offset;
-        const float v0 = (a.y - c.z - fr.near_mm) / fr.delta_ax, v1 = (a.y + c.z - fr.near_mm) / fr.delta_ax;
-        const int xa0 = max(0, gp_clamp_to_int(floorf(u0)) - 1 - cx0), xa1 = min(cw - 1, gp_clamp_to_int(ceilf(u1)) + 1 - cx0);
-        const int ya0 = max(half * HALF, gp_clamp_to_int(floorf(v0)) - 1 - cy0);
-        const int ya1 = min(min(ch - 1, half * HALF + HALF - 1), gp_clamp_to_int(ceilf(v1)) + 1 - cy0);
+        // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
+        // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
         float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sP = 0.f, sQ = 0.f;
-        for (int yy = ya0; yy <= ya1; ++yy) {
+#pragma unroll 1
+        for (int r = 0; r < HALF; ++r) {
+            const int yy = half * HALF + r;
             const float xa_ = (float)(cy0 + yy) * fr.delta_ax + fr.near_mm;
             const float da = xa_ - a.y;
-            if (fabsf(da) > c.z) continue;
+            const bool row_in = fabsf(da) <= c.z;
             const float c01da = a.w * da, c11da = c.x * da;
-            for (int xx = xa0; xx <= xa1; ++xx) {
-                const float xl = ((float)(cx0 + xx) - fr.col_offset) * fr.delta_lat;
-                const float dl = xl - a.x;
-                const float l = __builtin_fmaf(a.z, dl, c01da);
-                const float q = __builtin_fmaf(a.w, dl, c11da);
-                const float e = __builtin_fmaf(dl, l, da * q);
-                const float wgt = c.w * __builtin_amdgcn_exp2f(e);
-                const bool in = fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
-                const float tw = in ? s_g[yy * TS + xx] * wgt : 0.f;
-                sT += tw;
-                const float td = tw * dl, ta = tw * da;
-                sA = __builtin_fmaf(td, dl, sA);
-                sB = __builtin_fmaf(td, da, sB);
-                sC = __builtin_fmaf(ta, da, sC);
-                sP = __builtin_fmaf(tw, l, sP);
-                sQ = __builtin_fmaf(tw, q, sQ);
+            const float4* grow = reinterpret_cast<const float4*>(s_g + yy * TS);
+#pragma unroll 2
+            for (int x4 = 0; x4 < TS / 4; ++x4) {
+                const float4 go4 = grow[x4];
+                const float4 xl4 = reinterpret_cast<const float4*>(s_xl)[x4];
+                const float gov[4] = {go4.x, go4.y, go4.z, go4.w};
+                const float xlv[4] = {xl4.x, xl4.y, xl4.z, xl4.w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const float dl = xlv[j] - a.x;
+                    const float l = __builtin_fmaf(a.z, dl, c01da);
+                    const float q = __builtin_fmaf(a.w, dl, c11da);
+                    const float e = __builtin_fmaf(dl, l, da * q);
+                    const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+                    const bool in = row_in && fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
+                    const float tw = in ? gov[j] * wgt : 0.f;
+                    sT += tw;
+                    const float td = tw * dl, ta = tw * da;
+                    sA = __builtin_fmaf(td, dl, sA);
+                    sB = __builtin_fmaf(td, da, sB);
+                    sC = __builtin_fmaf(ta, da, sC);
+                    sP = __builtin_fmaf(tw, l, sP);
+                    sQ = __builtin_fmaf(tw, q, sQ);
+                }
             }
         }
         sT += __shfl_xor(sT, 1, kWave);
@@ -652,7 +684,7 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
     if (plan->n_tiles <= 0) return US3D_OK;
     hipStream_t stream = as_stream(stream_);
 #define GP_FWD(TS)                                                                                                    \
-    hipLaunchKernelGGL((gp_forward_kernel<TS>), dim3(plan->n_tiles), dim3(TS * TS), 0, stream,                        \
+    hipLaunchKernelGGL((gp_forward_kernel<TS>), dim3(plan->n_tiles), dim3(GpFwd<TS>::kThreads), 0, stream,            \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
                        gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key), plan->frames,             \
                        plan->sweeps, plan->frame_tile_base, plan->n_frames, settings->weight_min,                    \
