@@ -134,14 +134,12 @@ def read_traffic(kernel: str, workload_tag: str):
 
 def read_valu_issue(kernel: str):
     """VALU issue fraction (VALU instructions x 4 cycles / SIMD cycles) of a kernel from the committed rocprofv3
-    counter summary of the default workloads, if it lists the kernel."""
-    p = ROOT / "profiles" / "splat_and_ray_kernels_valu_issue_pmc.md"
-    if not p.exists():
-        return None
-    for line in p.read_text().splitlines():
-        c = [x.strip() for x in line.strip("|").split("|")]
-        if len(c) == 7 and c[0] == kernel and c[6].endswith("%"):
-            return float(c[6].rstrip(" %")) / 100.0
+    counter summaries (profiles/*valu_issue_pmc.md), if one lists the kernel."""
+    for p in sorted((ROOT / "profiles").glob("*valu_issue_pmc.md")):
+        for line in p.read_text().splitlines():
+            c = [x.strip() for x in line.strip("|").split("|")]
+            if len(c) == 7 and c[0] == kernel and c[6].endswith("%"):
+                return float(c[6].rstrip(" %")) / 100.0
     return None
 
 
@@ -478,8 +476,8 @@ def bench_gaussianpile(args):
     import torch
 
     from us3d import _lib
-    from us3d.batch import CapacityHint, read_stats
-    from us3d.gaussianpile import GaussianPlieRenderConfig, render_frames
+    from us3d.batch import CapacityHint, SweepSpec, read_stats
+    from us3d.gaussianpile import GaussianPlieRenderConfig, GPBatch, render_frames
     from us3d.scenes import config2
 
     dev = torch.device("cuda:0")
@@ -497,11 +495,13 @@ def bench_gaussianpile(args):
         gt = render_frames(*[t.to(dev) for t in (target.means, target.scales, target.quats, target.opacity)], frames,
                            cfg, hint=hint).clone()
     params = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.scales, scene.quats, scene.opacity)]
+    # the sweep's frame tables are built once, as a training loop over a fixed sweep keeps them
+    batch_all = GPBatch(frames, [SweepSpec(0, G)], G, cfg.settings(), dev, hint)
 
-    def step(fr=frames, ref=gt):
+    def step(fr=frames, ref=gt, batch=batch_all):
         for t in params:
             t.grad = None
-        img = render_frames(*params, fr, cfg, hint=hint)
+        img = render_frames(*params, fr, cfg, hint=hint, batch=batch)
         (img - ref).abs().mean().backward()
 
     def timed(fn, steps, warmup):
@@ -523,9 +523,11 @@ def bench_gaussianpile(args):
         n_plain = min(8, F)
         n = H * W
 
+        singles = [GPBatch([frames[k]], [SweepSpec(0, G)], G, cfg.settings(), dev, hint) for k in range(n_plain)]
+
         def plain():
             for k in range(n_plain):
-                step([frames[k]], gt[k * n:(k + 1) * n])
+                step([frames[k]], gt[k * n:(k + 1) * n], singles[k])
 
         dtp = timed(plain, 1, 1)
         fps_plain = n_plain / dtp
@@ -534,7 +536,7 @@ def bench_gaussianpile(args):
     _lib.stage_timing(True)
     for t in params:
         t.grad = None
-    img, batch = render_frames(*params, frames, cfg, hint=hint, return_batch=True)
+    img, batch = render_frames(*params, frames, cfg, hint=hint, return_batch=True, batch=batch_all)
     (img - gt).abs().mean().backward()
     torch.cuda.synchronize()
     _lib.stage_timing(False)
@@ -588,7 +590,9 @@ def bench_gaussianpile(args):
         "plain_frames_per_s": fps_plain,
         "batched_over_plain": (fps / fps_plain) if fps_plain else None,
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}") if dom in launch_bytes else None,
+                     "valu_issue_frac": read_valu_issue(f"{dom}<{args.tile_size}>"),
                      "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
                      "stage_ms_per_launch": stage_ms},
         "cpu_baseline": cpu,

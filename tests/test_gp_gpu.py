@@ -100,9 +100,15 @@ def test_gp_batch_equals_per_frame_and_is_deterministic(dev):
     params = _scene(frames[1], G=600, seed=5)
     cfg = _cfg(frames[0])
     tens = [p.to(dev) for p in params]
-    batched = render_frames(tens[0], tens[1], tens[2], tens[3], frames, cfg, intensity=tens[4])
+    batched, batch = render_frames(tens[0], tens[1], tens[2], tens[3], frames, cfg, intensity=tens[4],
+                                   return_batch=True)
     again = render_frames(tens[0], tens[1], tens[2], tens[3], frames, cfg, intensity=tens[4])
     assert torch.equal(batched, again)
+    # a batch kept from an earlier call (its frame tables already on the device) renders the same image
+    reused = render_frames(tens[0], tens[1], tens[2], tens[3], frames, cfg, intensity=tens[4], batch=batch)
+    assert torch.equal(batched, reused)
+    with pytest.raises(ValueError):
+        render_frames(tens[0], tens[1], tens[2], tens[3], frames[:2], cfg, intensity=tens[4], batch=batch)
     n = frames[0].height * frames[0].width
     for k, fr in enumerate(frames):
         one = render_frames(tens[0], tens[1], tens[2], tens[3], [fr], cfg, intensity=tens[4])

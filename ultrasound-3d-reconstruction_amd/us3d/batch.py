@@ -126,15 +126,50 @@ def frame_table(frames: Sequence[FrameSpec], lay: BatchLayout, device: torch.dev
     return torch.cat([vec.contiguous().view(torch.int32), host.to(device)], dim=1).contiguous()
 
 
+def _frame_rows_host(frames: Sequence[FrameSpec], lay: BatchLayout) -> np.ndarray:
+    """(n_frames, 25) int32 words laid out as us3d_frame, from frames whose vectors are host tensors."""
+    n = len(frames)
+    vec = np.empty((n, 16), dtype=np.float32)
+    for i, f in enumerate(frames):
+        for k, v in enumerate((f.origin, f.axis_lat, f.axis_ax, f.axis_ele)):
+            vec[i, 3 * k:3 * k + 3] = v.detach().reshape(3).to(torch.float32).numpy()
+        vec[i, 12:16] = (f.delta_lat, f.delta_ax, f.near_mm, f.col_offset)
+    ints = np.array(
+        [[f.height, f.width, lay.tiles_x[i], lay.tiles_y[i], lay.frame_tile_base[i], lay.frame_pixel_base[i], f.sweep,
+          lay.frame_ray_base[i], int(f.geometry)] for i, f in enumerate(frames)],
+        dtype=np.int32,
+    ).reshape(n, 9)
+    return np.concatenate([vec.view(np.int32), ints], axis=1)
+
+
 class DeviceTables:
-    """Device copies of the frame/sweep/prefix tables of one batch (kept alive by the owner)."""
+    """Device copies of the frame/sweep/prefix tables of one batch (kept alive by the owner). When every frame
+    vector is a host tensor the four tables travel in ONE host-to-device copy of a packed int32 buffer: sweep
+    rows first (us3d_sweep holds an int64, so they sit at offset 0), then the frame rows, the tile bases and
+    the pixel bases; each table is a view of that buffer."""
 
     def __init__(self, frames: Sequence[FrameSpec], lay: BatchLayout, device: torch.device):
-        self.frames = frame_table(frames, lay, device)
         i32 = torch.int32
-        self.tile_base = torch.tensor(lay.frame_tile_base, dtype=i32).to(device)
-        self.pixel_base = torch.tensor(lay.frame_pixel_base, dtype=i32).to(device)
-        self.sweeps = torch.from_numpy(lay.sweep_rows.copy()).to(device)
+        on_host = all(v.device.type == "cpu" for f in frames for v in (f.origin, f.axis_lat, f.axis_ax, f.axis_ele))
+        if on_host:
+            rows = _frame_rows_host(frames, lay)
+            parts = [lay.sweep_rows.reshape(-1).astype(np.int32), rows.reshape(-1),
+                     np.asarray(lay.frame_tile_base, dtype=np.int32).reshape(-1),
+                     np.asarray(lay.frame_pixel_base, dtype=np.int32).reshape(-1)]
+            buf = torch.from_numpy(np.concatenate(parts)).to(device)
+            o1 = parts[0].size
+            o2 = o1 + parts[1].size
+            o3 = o2 + parts[2].size
+            self.sweeps = buf[:o1].view(lay.sweep_rows.shape)
+            self.frames = buf[o1:o2].view(len(frames), 25)
+            self.tile_base = buf[o2:o3]
+            self.pixel_base = buf[o3:]
+            self._buf = buf
+        else:
+            self.frames = frame_table(frames, lay, device)
+            self.tile_base = torch.tensor(lay.frame_tile_base, dtype=i32).to(device)
+            self.pixel_base = torch.tensor(lay.frame_pixel_base, dtype=i32).to(device)
+            self.sweeps = torch.from_numpy(lay.sweep_rows.copy()).to(device)
         self.lay = lay
 
     def plan(self, workspace: torch.Tensor, k_capacity: int) -> _lib.Plan:

@@ -175,16 +175,21 @@ def _f32(t: torch.Tensor, device) -> torch.Tensor:
 
 def render_frames(means, scales, quats, opacity, frames: Sequence[FrameSpec], cfg: GaussianPlieRenderConfig,
                   intensity: Optional[torch.Tensor] = None, sweeps: Optional[Sequence[SweepSpec]] = None,
-                  hint: Optional[CapacityHint] = None, return_batch: bool = False):
+                  hint: Optional[CapacityHint] = None, return_batch: bool = False, batch: Optional[GPBatch] = None):
     """Batched us_focus render: every frame of `frames` (over one or more sweeps) in one map/forward/backward.
-    Returns the concatenated images (sum of H*W over frames,), differentiable w.r.t. the Gaussian tensors."""
+    Returns the concatenated images (sum of H*W over frames,), differentiable w.r.t. the Gaussian tensors.
+    `batch`: a GPBatch built earlier for these same frames, sweeps and settings (one returned with
+    return_batch=True, or constructed directly); reusing it skips rebuilding and uploading the frame tables."""
     dev = means.device
     if dev.type != "cuda":
         raise RuntimeError("us3d.gaussianpile renders on the GPU only; there is no CPU path.")
     G = means.shape[0]
     if sweeps is None:
         sweeps = [SweepSpec(0, G)]
-    batch = GPBatch(frames, sweeps, G, cfg.settings(), dev, hint)
+    if batch is None:
+        batch = GPBatch(frames, sweeps, G, cfg.settings(), dev, hint)
+    elif batch.layout.n_gauss != G or batch.layout.n_frames != len(frames) or batch.device != dev:
+        raise ValueError("render_frames: the given batch was built for other frames, Gaussians or device")
     if intensity is None:
         intensity = torch.ones(G, dtype=torch.float32, device=dev)
     image = _GPRenderFunction.apply(batch, _f32(means, dev), _f32(scales, dev), _f32(quats, dev),
