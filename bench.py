@@ -552,9 +552,10 @@ def bench_gaussianpile(args):
     del m
     dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
     avg_ms = total_ms / max(launches, 1)
-    # algorithmic bytes per launch: per instance its 4 B key and 32 B footprint gathered, per pixel the image
-    # written (forward) or its gradient read (adjoint), per visible pair its record-space gradient
-    launch_bytes = {"gp_forward_kernel": 36 * K + 4 * P, "gp_backward_kernel": 36 * K + 4 * P + 32 * g_vis}
+    # algorithmic bytes per launch: per instance its 8 B sort key, 32 B footprint and 4 B intensity gathered
+    # (adjoint: also its 4 B emit index read and its 32 B record-space gradient written), per pixel the 4 B
+    # image written (forward) or its 4 B gradient read (adjoint)
+    launch_bytes = {"gp_forward_kernel": 44 * K + 4 * P, "gp_backward_kernel": 80 * K + 4 * P}
     alg = launch_bytes.get(dom)
     achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
     stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
