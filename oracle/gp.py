@@ -99,7 +99,8 @@ def footprint(means, scales, quats, opacity, fr: GPFrame, st: GPSettings):
     mu_ele = d[0] * ae[0] + d[1] * ae[1] + d[2] * ae[2]
     R = _quat_rot(quats)
     s = [torch.clamp(scales[:, i], min=1e-6) for i in range(3)]
-    # Sigma_w = R diag(s^2) R^T (plane_cond2d_device.cuh:48-69), M = (R s) s column-wise
+    # Sigma_w as plane_cond2d_device.cuh:48-69 forms it from the row-major rotation entries: M[i][k] = R[i][k]
+    # s_k^2, Sigma_w[i][j] = sum_k M[i][k] R[k][j], i.e. R diag(s^2) R (not R diag(s^2) R^T), kept as written
     Mx = [R[k] * s[k % 3] * s[k % 3] for k in range(9)]
     Sw = [None] * 9
     for i in range(3):

@@ -253,6 +253,10 @@ class GaussianPlieRenderer(nn.Module):
                                        axis_elevational_world=axis_elevational_world),
                 return_intermediates=False)
             return pred
+        if torch.is_tensor(pose_probe_to_world) and pose_probe_to_world.requires_grad:
+            # the reference's backward can return d_pose (autograd.py:173-211); this build does not compute it
+            raise NotImplementedError("GaussianPlieRenderer: gradients w.r.t. the probe pose are not computed; "
+                                      "pass a pose that does not require grad")
         fr = frame_from_pose(pose_probe_to_world, cfg)
         image = render_frames(gauss_means, gauss_scales, gauss_quats, gauss_alphas, [fr], cfg, hint=self._hint)
         return image.view(cfg.H, cfg.W)
