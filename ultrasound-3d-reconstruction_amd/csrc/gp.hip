@@ -575,12 +575,14 @@ __global__ __launch_bounds__(256) void gp_reduce_kernel(int n_gauss, const us3d_
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = 0.f;
     float accI = 0.f;
-    for (int fl = lane; fl < sw.f_count; fl += kWave) {
+    // One visible pair per lane: lane j of a full queue holds the j-th visible frame, so the footprint VJP
+    // runs once per 64 visible pairs instead of once per 64 frames (a Gaussian is seen by a few frames).
+    auto process = [&](int fl) {
         const int64_t p = p0 + fl;
         const int cnt = pair_count[p];
-        if (cnt == 0) continue;
         const int e0 = pair_off[p];
         float gb = 0.f, gI = 0.f, gm0 = 0.f, gm1 = 0.f, g00 = 0.f, g01 = 0.f, g11 = 0.f;
+#pragma unroll 4
         for (int e = e0; e < e0 + cnt; ++e) {
             const float4 a = partial[2 * (int64_t)e + 0], b = partial[2 * (int64_t)e + 1];
             gb += a.x; gI += a.y; gm0 += a.z; gm1 += a.w;
@@ -592,7 +594,31 @@ __global__ __launch_bounds__(256) void gp_reduce_kernel(int n_gauss, const us3d_
         GPInter I;
         gp_footprint<true>(r0, r1, r2, fr, st.k_sigma, st.sigma_ele, st.det_eps, F, &I);
         gp_footprint_vjp(I, fr, gb, gm0, gm1, g00, g01, g11, acc);
+    };
+    const uint64_t below = (1ull << lane) - 1ull;
+    int q = -1;  // queued visible frame of this lane (lanes [0, nq))
+    int nq = 0;  // wave-uniform
+    for (int base = 0; base < sw.f_count; base += kWave) {
+        const int fl = base + lane;
+        const bool vis = fl < sw.f_count && pair_count[p0 + fl] > 0;
+        const uint64_t m = __ballot(vis);
+        const int nv = __popcll(m);
+        if (nv == 0) continue;
+        const int r = __popcll(m & below);
+        // a permutation of the lanes: visible ones to queue slots nq + r (mod 64), in frame order
+        const int dst = (nq + (vis ? r : nv + (lane - r))) & (kWave - 1);
+        const int got = __builtin_amdgcn_ds_permute(dst << 2, fl);
+        const int lo = nq + nv;  // queue length after appending; entries past 64 wrap into slots [0, lo - 64)
+        if (lane >= nq && lane < lo) q = got;
+        const int q2 = (lane < lo - kWave) ? got : -1;
+        nq = lo;
+        if (nq >= kWave) {
+            process(q);
+            q = q2;
+            nq -= kWave;
+        }
     }
+    if (lane < nq) process(q);
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
     accI = wave_sum(accI);
