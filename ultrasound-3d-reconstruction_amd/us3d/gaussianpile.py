@@ -197,6 +197,21 @@ def render_frames(means, scales, quats, opacity, frames: Sequence[FrameSpec], cf
     return (image, batch) if return_batch else image
 
 
+def validate_render_inputs(gauss_means, gauss_scales, gauss_quats, gauss_opacities, pose_probe_to_world) -> None:
+    """check.py:10-29: the shape checks the facade runs when config.enable_input_checks is set."""
+    n = gauss_means.shape[0]
+    if gauss_means.ndim != 2 or tuple(gauss_means.shape) != (n, 3):
+        raise ValueError(f"gauss_means must be (N, 3), got {tuple(gauss_means.shape)}.")
+    if tuple(gauss_scales.shape) != (n, 3):
+        raise ValueError(f"gauss_scales must be (N, 3), got {tuple(gauss_scales.shape)}.")
+    if tuple(gauss_quats.shape) != (n, 4):
+        raise ValueError(f"gauss_quats must be (N, 4), got {tuple(gauss_quats.shape)}.")
+    if tuple(gauss_opacities.shape) not in ((n,), (n, 1)):
+        raise ValueError(f"gauss_opacities must be (N,) or (N, 1), got {tuple(gauss_opacities.shape)}.")
+    if tuple(pose_probe_to_world.shape) != (4, 4):
+        raise ValueError(f"pose_probe_to_world must be (4, 4), got {tuple(pose_probe_to_world.shape)}.")
+
+
 class GaussianPlieRenderer(nn.Module):
     """api.py:10-154: GaussianPile facade taking (mean, scale, quat, alpha) and a probe-to-world pose."""
 
@@ -232,6 +247,9 @@ class GaussianPlieRenderer(nn.Module):
                 pixel_positions_world=None, origin_world=None, axis_lateral_world=None, axis_axial_world=None,
                 axis_elevational_world=None) -> torch.Tensor:
         cfg = self.config
+        if cfg.enable_input_checks:
+            validate_render_inputs(gauss_means, gauss_scales, gauss_quats, gauss_alphas,
+                                   torch.as_tensor(pose_probe_to_world))
         if cfg.backend == "ultragauss_lowmem":
             if (pixel_positions_world is None or origin_world is None or axis_lateral_world is None
                     or axis_axial_world is None or axis_elevational_world is None):
