@@ -107,7 +107,8 @@ __device__ __forceinline__ float gp_inv3(const float M[9], float Mi[9]) {
 
 template <bool kInter>
 __device__ __forceinline__ void gp_footprint(float4 r0, float4 r1, float4 r2, const us3d_frame& fr, float k_sigma,
-                                             float sigma_ele, float det_eps, GPFoot& F, GPInter* I) {
+                                             float sigma_ele, float det_eps, GPFoot& F, GPInter* I,
+                                             float early_weight_min = 0.f) {
     const float* al = fr.axis_lat;
     const float* aa = fr.axis_ax;
     const float* ae = fr.axis_ele;
@@ -150,9 +151,20 @@ __device__ __forceinline__ void gp_footprint(float4 r0, float4 r1, float4 r2, co
     Sc[0] += 1e-6f;
     Sc[4] += 1e-6f;
     Sc[8] += 1e-6f;
+    const float sig2 = sigma_ele * sigma_ele;
+    const float denom_raw = Sc[8] + sig2 + det_eps;
+    const float denom = fmaxf(denom_raw, 1e-12f);
+    const float sr = expf(-0.5f * mu_ele * mu_ele / denom);
+    // Exact early cull (early_weight_min > 0): a valid footprint has det2 > det_eps, so its normaliser
+    // 1/sqrt(det2 + det_eps) is below 1/sqrt(2 det_eps) (the 0.1 % margin covers FP32 rounding); when even that
+    // bound gives a base weight below weight_min, the pair would not be listed and the inversions are skipped.
+    if (early_weight_min > 0.f && r0.w * sr * (1.001f / sqrtf(2.f * det_eps)) < early_weight_min) {
+        F.ok = false;
+        F.b = 0.f;
+        return;
+    }
     float Lc[9], Le[9], Se[9];
     const float det_c = gp_inv3(Sc, Lc);
-    const float sig2 = sigma_ele * sigma_ele;
 #pragma unroll
     for (int k = 0; k < 9; ++k) Le[k] = Lc[k];
     Le[8] = Le[8] + 1.f / fmaxf(sig2, 1e-12f);
@@ -171,9 +183,6 @@ __device__ __forceinline__ void gp_footprint(float4 r0, float4 r1, float4 r2, co
     const float cross = fabsf(s01);
     F.r_lat = k_sigma * sqrtf(fmaxf(s00 + cross, 1e-12f));
     F.r_ax = k_sigma * sqrtf(fmaxf(s11 + cross, 1e-12f));
-    const float denom_raw = Sc[8] + sig2 + det_eps;
-    const float denom = fmaxf(denom_raw, 1e-12f);
-    const float sr = expf(-0.5f * mu_ele * mu_ele / denom);
     const float norm = 1.f / sqrtf(det2 + det_eps);
     F.b = r0.w * sr * norm;
     F.ok = det_c > 1e-12f && isfinite(det_c) && det_e > 1e-12f && isfinite(det_e) && det2 > det_eps &&
@@ -209,7 +218,7 @@ __global__ __launch_bounds__(256) void gp_preprocess_kernel(
     const us3d_frame& fr = frames[f];
     GPFoot F;
     gp_footprint<false>(rec[3 * (int64_t)g + 0], rec[3 * (int64_t)g + 1], rec[3 * (int64_t)g + 2], fr, st.k_sigma,
-                        st.sigma_ele, st.det_eps, F, nullptr);
+                        st.sigma_ele, st.det_eps, F, nullptr, st.weight_min);
     int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
     if (F.ok && F.b >= st.weight_min) {
@@ -364,7 +373,10 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         const float4 a = pair_foot[2 * p + 0], c = pair_foot[2 * p + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
         // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
         // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
-        float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sP = 0.f, sQ = 0.f;
+        // da is constant along a row, so per row only sum t, sum t dl and sum t dl^2 are accumulated per pixel;
+        // the sums over da (and the pre-scaled half-gradients l, q, which are linear in dl and da) follow
+        // from them once per row
+        float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sD = 0.f, sE = 0.f;  // sD = sum t dl, sE = sum t da
 #pragma unroll 1
         for (int r = 0; r < HALF; ++r) {
             const int yy = half * HALF + r;
@@ -373,6 +385,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
             const bool row_in = fabsf(da) <= c.z;
             const float c01da = a.w * da, c11da = c.x * da;
             const float4* grow = reinterpret_cast<const float4*>(s_g + yy * TS);
+            float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll 2
             for (int x4 = 0; x4 < TS / 4; ++x4) {
                 const float4 go4 = grow[x4];
@@ -388,22 +401,28 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
                     const float wgt = c.w * __builtin_amdgcn_exp2f(e);
                     const bool in = row_in && fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
                     const float tw = in ? gov[j] * wgt : 0.f;
-                    sT += tw;
-                    const float td = tw * dl, ta = tw * da;
-                    sA = __builtin_fmaf(td, dl, sA);
-                    sB = __builtin_fmaf(td, da, sB);
-                    sC = __builtin_fmaf(ta, da, sC);
-                    sP = __builtin_fmaf(tw, l, sP);
-                    sQ = __builtin_fmaf(tw, q, sQ);
+                    r0 += tw;
+                    const float td = tw * dl;
+                    r1 += td;
+                    r2 = __builtin_fmaf(td, dl, r2);
                 }
             }
+            sT += r0;
+            sA += r2;
+            sD += r1;
+            sE = __builtin_fmaf(da, r0, sE);
+            sB = __builtin_fmaf(da, r1, sB);
+            sC = __builtin_fmaf(da * da, r0, sC);
         }
         sT += __shfl_xor(sT, 1, kWave);
         sA += __shfl_xor(sA, 1, kWave);
         sB += __shfl_xor(sB, 1, kWave);
         sC += __shfl_xor(sC, 1, kWave);
-        sP += __shfl_xor(sP, 1, kWave);
-        sQ += __shfl_xor(sQ, 1, kWave);
+        sD += __shfl_xor(sD, 1, kWave);
+        sE += __shfl_xor(sE, 1, kWave);
+        // sum t l = C i00 sum t dl + C i01 sum t da, sum t q = C i01 sum t dl + C i11 sum t da
+        const float sP = __builtin_fmaf(a.z, sD, a.w * sE);
+        const float sQ = __builtin_fmaf(a.w, sD, c.x * sE);
         if (half == 0) {
             const float I = rec[3 * (int64_t)g + 1].w;
             const float mI = -2.f * kGpLn2 * I;
