@@ -312,3 +312,50 @@ def render_reference_math(means, scales, quats, opacity, fr: GPFrame, sigma_ele:
         q = (dd * (dd @ inv2[g].T)).sum(-1)
         out = out + alpha[g] * torch.exp(-0.5 * q)
     return out.reshape(fr.H, fr.W)
+
+
+def render_pixels(means, scales, quats, opacity, intensity, fr: GPFrame, st: GPSettings, rows, cols,
+                  fp64=False):
+    """Values at the pixels (rows[k], cols[k]) of the frame by brute force over EVERY Gaussian, without tile
+    lists: a pair contributes where the mapper would list it (valid footprint, b >= weight_min) and the
+    pixel passes the shader's tests (box, quad >= 0, weight >= weight_min, decided in FP32 as the kernels
+    decide). A tile mapper that drops a contributing pair disagrees with this at some pixel. FP32 sums in
+    index order (the tile lists' order); fp64=True: float64 values from the float64 inputs on the same pair
+    set, differentiable by torch autograd. Returns (n,)."""
+    f32 = torch.float32
+    rows = torch.as_tensor(rows, dtype=torch.long)
+    cols = torch.as_tensor(cols, dtype=torch.long)
+    p32 = [t.detach().to(f32) for t in (means, scales, quats, opacity)]
+    fp = footprint(*p32, fr, st)
+    keep = fp["ok"] & (fp["b"] >= st.weight_min)
+    idx = torch.nonzero(keep).flatten()
+    xl32 = (cols.to(f32) - torch.tensor(fr.col_offset, dtype=f32)) * torch.tensor(fr.delta_lat, dtype=f32)
+    xa32 = rows.to(f32) * torch.tensor(fr.delta_ax, dtype=f32) + torch.tensor(fr.near, dtype=f32)
+    C = torch.tensor(EXP2_SCALE, dtype=f32)
+    g = {k: v[idx][None, :] for k, v in fp.items() if k != "ok"}
+    c00, c01, c11 = g["i00"] * C, g["i01"] * C, g["i11"] * C
+    dl32 = xl32[:, None] - g["m0"]
+    da32 = xa32[:, None] - g["m1"]
+    box = (torch.abs(dl32) <= g["r_lat"]) & (torch.abs(da32) <= g["r_ax"])
+    l32 = _fma32(c00, dl32, c01 * da32)
+    k32 = _fma32(c01, dl32, c11 * da32)
+    e32 = _fma32(dl32, l32, da32 * k32)
+    w32 = g["b"] * torch.exp2(e32)
+    m = box & (e32 <= 0) & (w32 >= torch.tensor(st.weight_min, dtype=f32))  # (n, len(idx))
+    if not fp64:
+        inten = intensity.detach().to(f32)[idx][None, :]
+        terms = torch.where(m, inten * w32, torch.zeros_like(w32))
+        out = torch.zeros(rows.numel(), dtype=f32)
+        for j in range(terms.shape[1]):  # sequential FP32 sum in index order
+            out = out + terms[:, j]
+        return out
+    used = torch.nonzero(m.any(dim=0)).flatten()
+    gi = idx[used]
+    sub = footprint(means[gi], scales[gi], quats[gi], opacity[gi], fr, st)
+    mm = m[:, used]
+    # pixel positions as the kernels form them (FP32), then float64
+    dl = xl32.double()[:, None] - sub["m0"][None, :]
+    da = xa32.double()[:, None] - sub["m1"][None, :]
+    q = dl * (sub["i00"] * dl + sub["i01"] * da) + da * (sub["i01"] * dl + sub["i11"] * da)
+    w = sub["b"][None, :] * torch.exp(-0.5 * torch.where(mm, q, torch.zeros_like(q)))
+    return torch.where(mm, intensity[gi][None, :] * w, torch.zeros_like(w)).sum(dim=1)

@@ -83,3 +83,24 @@ def test_oracle_fp64_gradients_match_central_differences():
             fd = (float(loss(*xs_p)) - float(loss(*xs_m))) / (2 * h)
             an = float(gr.reshape(-1)[idx])
             assert abs(fd - an) <= 1e-5 * max(1.0, abs(fd)), (k, int(idx), fd, an)
+
+
+def test_brute_force_pixels_equal_the_tile_list_render():
+    """render_pixels (every Gaussian, no tile lists) against render (tile lists): the same FP32 image bit for
+    bit and the same FP64 image, so the brute force can stand in for the tile-list oracle at sizes where only
+    sampled pixels are affordable."""
+    G = 150
+    sc = make_scene(G, 20.0, seed=4)
+    fr = _frame()
+    st = gp.GPSettings()
+    args = (sc.means, sc.scales, sc.quats, sc.opacity)
+    rows = torch.arange(fr.H).repeat_interleave(fr.W)
+    cols = torch.arange(fr.W).repeat(fr.H)
+    img = gp.render(*args, torch.ones(G), fr, st)
+    assert float(img.max()) > 0.05
+    assert torch.equal(gp.render_pixels(*args, torch.ones(G), fr, st, rows, cols).reshape(fr.H, fr.W), img)
+    a64 = [t.double() for t in args]
+    one = torch.ones(G, dtype=torch.float64)
+    img64 = gp.render(*a64, one, fr, st, fp64=True)
+    bf64 = gp.render_pixels(*a64, one, fr, st, rows, cols, fp64=True).reshape(fr.H, fr.W)
+    assert float((bf64 - img64).abs().max()) < 1e-12

@@ -137,3 +137,50 @@ def test_gp_facade_backends(dev):
                                         axis_axial_world=fr.axis_ax.to(dev),
                                         axis_elevational_world=fr.axis_ele.to(dev))
     assert img_u.shape == (fr.height, fr.width) and float(img_u.max()) > 0.05
+
+
+def test_gp_config2_full_size_sampled_pixels_and_gradients(dev):
+    """The bench workload at full size (200k Gaussians, 256 frames of 512x512, one batch): 4 frames x 32
+    sampled pixels against the brute-force oracle over every Gaussian (no tile lists, so a pair the tile
+    mapper dropped would show), and the gradients of a weighted sum of those pixels against torch autograd
+    through the FP64 brute force. Bars: 2e-5 * max(1, |ref|) per pixel, 1e-4 rel. L2 per parameter group."""
+    from us3d.scenes import config2
+
+    scene, _target, frames = config2()
+    G = scene.means.shape[0]
+    fr0 = frames[0]
+    H, W = fr0.height, fr0.width
+    cfg = GaussianPlieRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=fr0.delta_lat,
+                                   axial_spacing_mm_per_pix=fr0.delta_ax, axial_near_mm=fr0.near_mm,
+                                   lateral_col_offset=fr0.col_offset)
+    base = [t.float() for t in (scene.means, scene.scales, scene.quats, scene.opacity)]
+    params = [t.to(dev).clone().requires_grad_(True) for t in base]
+    img = render_frames(*params, frames, cfg)
+    assert img.numel() == len(frames) * H * W
+    rng = np.random.default_rng(11)
+    sel = [0, 85, 170, 255]
+    n = 32
+    rows = torch.tensor(rng.integers(0, H, size=(len(sel), n)))
+    cols = torch.tensor(rng.integers(0, W, size=(len(sel), n)))
+    wts = torch.tensor(rng.normal(size=(len(sel), n)), dtype=torch.float32)
+    idx = torch.cat([f * H * W + rows[k] * W + cols[k] for k, f in enumerate(sel)])
+    (img[idx.to(dev)] * wts.reshape(-1).to(dev)).sum().backward()
+    got = img.detach()[idx.to(dev)].cpu()
+    st = gp.GPSettings(tile_size=16)
+    one32 = torch.ones(G)
+    p64 = [t.double().clone().requires_grad_(True) for t in base]
+    one64 = torch.ones(G, dtype=torch.float64)
+    ref32, loss64 = [], 0.0
+    for k, f in enumerate(sel):
+        of = _oframe(frames[f])
+        ref32.append(gp.render_pixels(*base, one32, of, st, rows[k], cols[k]))
+        loss64 = loss64 + (gp.render_pixels(*p64, one64, of, st, rows[k], cols[k], fp64=True) * wts[k].double()).sum()
+    ref32 = torch.cat(ref32)
+    assert float(ref32.abs().max()) > 0.05
+    assert bool(torch.all((got - ref32).abs() <= 2e-5 * torch.clamp(ref32.abs(), min=1.0)))
+    loss64.backward()
+    for name, t, r in zip(("means", "scales", "quats", "opacity"), params, p64):
+        ref = r.grad
+        assert float(ref.abs().max()) > 0, name
+        err = float((t.grad.detach().cpu().double() - ref).norm() / ref.norm())
+        assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
