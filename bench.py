@@ -471,14 +471,17 @@ def bench_fan(args):
 
 def bench_gaussianpile(args):
     """The GaussianPile us_focus slice splat (the second slice model) on the config-2 scene: 200k Gaussians
-    (means, scales, quaternions, opacities) and the 256-frame 512x512 freehand sweep. One step = map + forward of
-    all frames in one batch, L1 loss against a target rendered from the seed-1 scene, adjoint (eager launches)."""
+    (means, log-scales, quaternions, opacity logits) and the 256-frame 512x512 freehand sweep. One step = the
+    captured training step of GaussianPileTrainer over all frames: map + forward, L1 + 0.2 (1 - SSIM) against
+    a target rendered from the seed-1 scene, adjoint, Adam. "plain": the same API frame by frame with torch
+    activations, torch autograd and torch.optim.Adam."""
     import torch
 
     from us3d import _lib
     from us3d.batch import CapacityHint, SweepSpec, read_stats
     from us3d.gaussianpile import GaussianPlieRenderConfig, GPBatch, render_frames
     from us3d.scenes import config2
+    from us3d.train_step import GaussianPileTrainer
 
     dev = torch.device("cuda:0")
     scene, target, frames = config2(n_frames=args.frames, n_gauss=args.gaussians)
@@ -494,15 +497,10 @@ def bench_gaussianpile(args):
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.scales, target.quats, target.opacity)], frames,
                            cfg, hint=hint).clone()
-    params = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.scales, scene.quats, scene.opacity)]
-    # the sweep's frame tables are built once, as a training loop over a fixed sweep keeps them
-    batch_all = GPBatch(frames, [SweepSpec(0, G)], G, cfg.settings(), dev, hint)
-
-    def step(fr=frames, ref=gt, batch=batch_all):
-        for t in params:
-            t.grad = None
-        img = render_frames(*params, fr, cfg, hint=hint, batch=batch)
-        (img - ref).abs().mean().backward()
+    op = scene.opacity.clamp(1e-6, 1 - 1e-6)
+    raw = [t.to(dev).float().contiguous() for t in (scene.means, torch.log(scene.scales), scene.quats,
+                                                     torch.log(op / (1 - op)))]
+    tr = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=not args.eager, check_every=args.steps)
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -514,7 +512,7 @@ def bench_gaussianpile(args):
         torch.cuda.synchronize()
         return time.perf_counter() - t0
 
-    dt = timed(step, args.steps, args.warmup)
+    dt = timed(tr.step, args.steps, args.warmup)
     fps = F * args.steps / dt
     print(f"[bench] gaussianpile batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s", file=sys.stderr,
           flush=True)
@@ -522,34 +520,38 @@ def bench_gaussianpile(args):
     if not args.no_plain:
         n_plain = min(8, F)
         n = H * W
-
+        ps = [t.clone().requires_grad_(True) for t in raw]
+        opt = torch.optim.Adam(ps, lr=1e-3)
         singles = [GPBatch([frames[k]], [SweepSpec(0, G)], G, cfg.settings(), dev, hint) for k in range(n_plain)]
 
         def plain():
             for k in range(n_plain):
-                step([frames[k]], gt[k * n:(k + 1) * n], singles[k])
+                opt.zero_grad(set_to_none=True)
+                img = render_frames(ps[0], torch.exp(ps[1]).clamp_min(1e-5),
+                                    torch.nn.functional.normalize(ps[2], dim=-1, eps=1e-8), torch.sigmoid(ps[3]),
+                                    [frames[k]], cfg, hint=hint, batch=singles[k])
+                frame_losses(img.view(1, H, W), gt[k * n:(k + 1) * n].view(1, H, W)).mean().backward()
+                opt.step()
 
         dtp = timed(plain, 1, 1)
         fps_plain = n_plain / dtp
         print(f"[bench] gaussianpile plain (per frame): {fps_plain:.1f} frames/s", file=sys.stderr, flush=True)
+    # stage timing: one eager step of a second trainer (HIP events around each launch)
+    tr_t = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=False)
+    tr_t.step()
+    torch.cuda.synchronize()
     _lib.stage_timing_read()
     _lib.stage_timing(True)
-    for t in params:
-        t.grad = None
-    img, batch = render_frames(*params, frames, cfg, hint=hint, return_batch=True, batch=batch_all)
-    (img - gt).abs().mean().backward()
+    tr_t.step()
     torch.cuda.synchronize()
     _lib.stage_timing(False)
     stages = _lib.stage_timing_read()
-    m = batch.map(*[t.detach() for t in params])
-    st = read_stats(m.workspace)
-    off = _lib.Layout()
-    lay = batch.layout
-    _lib.check(_lib.lib().us3d_gp_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, m.k_capacity, off),
-               "us3d_gp_workspace_layout")
-    g_vis = int((m.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
+    st = read_stats(tr_t.workspace)
+    off = tr_t.batch.layout_offsets(tr_t.k_capacity)
+    lay = tr_t.batch.layout
+    g_vis = int((tr_t.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
     K = int(st.n_instances)
-    del m
+    del tr_t
     dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
     avg_ms = total_ms / max(launches, 1)
     # algorithmic bytes per launch: per instance its 8 B sort key, 32 B footprint and 4 B intensity gathered
@@ -580,13 +582,14 @@ def bench_gaussianpile(args):
                          f"oracle's tile lists and forward, FP64 with torch autograd for the adjoint, {dtc:.1f} s; "
                          f"frames/s = sampled fraction of a frame / time"}
     line = {
-        "metric": "GaussianPile us_focus fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
+        "metric": "GaussianPile us_focus training-step frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
         "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"GaussianPile us_focus map + forward + adjoint, {G} Gaussians in a 120 mm cube, {F} "
-                               f"freehand frames {H}x{W} at 0.2 mm, radius_sigma 3, sigma_z 1 mm, L1 loss, batched "
-                               f"eager, tile {args.tile_size}",
+        "config": {"workload": f"GaussianPile us_focus training step (map + forward + L1+0.2(1-SSIM) + adjoint + "
+                               f"Adam, raw activations fused), {G} Gaussians in a 120 mm cube, {F} freehand frames "
+                               f"{H}x{W} at 0.2 mm, radius_sigma 3, sigma_z 1 mm, batched "
+                               f"{'eager' if args.eager else 'hipGraph'}, tile {args.tile_size}",
                    "frames": F, "gaussians": G, "instances": K, "visible_pairs": g_vis},
         "plain_frames_per_s": fps_plain,
         "batched_over_plain": (fps / fps_plain) if fps_plain else None,

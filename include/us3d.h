@@ -246,7 +246,12 @@ typedef struct us3d_gp_settings {
     float det_eps;      /* det_eps */
     float weight_min;   /* max(opacity_cull_threshold, 1e-12): smaller weights are dropped */
     int32_t tile_size;  /* 8, 16 or 32 (tile_size_px) */
-    int32_t reserved[3];
+    int32_t raw_activations; /* 1: opacity is a logit (the map applies the sigmoid) and scales are log-scales
+                                (the map applies max(exp(x), 1e-5)), as GaussianParameters activates them
+                                (pipeline_god_file.py:360-370); the backward then returns the gradients w.r.t.
+                                those raw values. 0: both are used as given. Quaternions are always normalised
+                                by the kernels. Pass the same value to the map, forward and backward. */
+    int32_t reserved[2];
 } us3d_gp_settings;
 
 /* Workspace for a batch (same plan structure as UltraGauss: tiles of tile_size px per frame). */

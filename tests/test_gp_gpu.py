@@ -184,3 +184,80 @@ def test_gp_config2_full_size_sampled_pixels_and_gradients(dev):
         assert float(ref.abs().max()) > 0, name
         err = float((t.grad.detach().cpu().double() - ref).norm() / ref.norm())
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
+
+
+def _trainer_setup(dev):
+    from us3d.train_step import GaussianPileTrainer
+
+    frames = _frames(2, seed=21)
+    means, scales, quats, opac, _ = _scene(frames[0], G=500, seed=22)
+    cfg = _cfg(frames[0])
+    tm, ts_, tq, to_, _ = _scene(frames[0], G=500, seed=23)
+    with torch.no_grad():
+        gt = render_frames(tm.to(dev), ts_.to(dev), tq.to(dev), to_.to(dev), frames, cfg).clone()
+    raw = [means, torch.log(scales), quats * 1.7, torch.log(opac / (1 - opac))]
+    return GaussianPileTrainer, frames, cfg, gt, [t.to(dev) for t in raw]
+
+
+def test_gp_trainer_gradients_match_torch_autograd_through_the_activations(dev):
+    """The GaussianPile training step passes raw values: the pack applies sigmoid(opacity) and
+    max(exp(scale), 1e-5), the kernels normalise the quaternions, the reduce applies the chain rules
+    (pipeline_god_file.py:360-370). The packed values agree with torch's activations to a few ulps. Taking the
+    packed values themselves (an ulp of difference can move a pixel across a box edge), loss and gradients equal
+    torch autograd of the same per-frame loss through the renderer at those values, followed by the chain rules
+    d raw_s = d s * s (s > 1e-5) and d raw_o = d o * o (1 - o) in torch."""
+    from us3d.pipeline import simple_ssim
+
+    Trainer, frames, cfg, gt, raw = _trainer_setup(dev)
+    tr = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=0.0)
+    tr.step()
+    torch.cuda.synchronize()
+    G = raw[0].shape[0]
+    off = tr.batch.layout_offsets(tr.k_capacity)
+    rec = tr.workspace[off.records: off.records + 48 * G].view(torch.float32).view(G, 12).clone()
+    o_pk, s_pk = rec[:, 3], rec[:, 4:7]
+    s_t = torch.exp(raw[1]).clamp_min(1e-5)
+    o_t = torch.sigmoid(raw[3])
+    assert float(((s_pk - s_t).abs() / s_t).max()) < 1e-6
+    assert float(((o_pk - o_t).abs() / o_t).max()) < 1e-6
+    ps = [raw[0].clone().requires_grad_(True), s_pk.clone().requires_grad_(True),
+          raw[2].clone().requires_grad_(True), o_pk.clone().requires_grad_(True)]
+    img = render_frames(ps[0], ps[1], ps[2], ps[3], frames, cfg)
+    # the fused loss: its value and dL/dimage against torch autograd of the per-frame loss
+    x_all = img.detach().clone().requires_grad_(True)
+    P = img.numel() // len(frames)
+    per = []
+    for f in range(len(frames)):
+        x, y = x_all[f * P:(f + 1) * P], gt[f * P:(f + 1) * P]
+        per.append((x - y).abs().mean() + 0.2 * (1 - simple_ssim(x, y)))
+    loss = torch.stack(per).mean()
+    loss.backward()
+    assert abs(float(tr.losses().mean()) - float(loss.detach())) < 1e-5 * max(1.0, abs(float(loss.detach())))
+    d_img = tr.d_image.detach().clone()
+    assert float((d_img.double() - x_all.grad.double()).norm() / x_all.grad.double().norm()) < 1e-4
+    # the chain rules: the renderer's adjoint at the packed values, driven by the same dL/dimage
+    img.backward(d_img)
+    refs = [ps[0].grad, torch.where(s_pk > 1e-5, ps[1].grad * s_pk, torch.zeros_like(s_pk)), ps[2].grad,
+            ps[3].grad * o_pk * (1 - o_pk)]
+    for name, g, ref in zip(("means", "raw_scales", "raw_quats", "raw_opacity"), tr.grads, refs):
+        ref = ref.detach().double()
+        assert float(ref.abs().max()) > 0, name
+        err = float((g.detach().double() - ref).norm() / ref.norm())
+        assert err < 1e-5, f"{name}: rel L2 {err:.3e}"
+
+
+def test_gp_trainer_graph_replays_equal_eager_steps(dev):
+    """Three captured steps (replays, with eager reads between them) against three eager steps: the same
+    parameters bit for bit after every step, and the loss decreases with lr 1e-2."""
+    Trainer, frames, cfg, gt, raw = _trainer_setup(dev)
+    eager = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=1e-2)
+    graph = Trainer(*raw, frames, gt, cfg, use_graph=True, lr=1e-2)
+    losses = []
+    for _ in range(3):
+        eager.step()
+        graph.step()
+        torch.cuda.synchronize()
+        for a, b in zip(eager.state(), graph.state()):
+            assert torch.equal(a, b)
+        losses.append(float(graph.losses().mean()))
+    assert losses[-1] < losses[0]

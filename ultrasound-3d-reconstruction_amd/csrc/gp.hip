@@ -49,17 +49,23 @@ __device__ __forceinline__ int gp_clamp_to_int(float x) {
     return (int)fminf(fmaxf(x, -1073741824.0f), 1073741824.0f);
 }
 
-// Record per Gaussian: (mu, opacity), (scale, intensity), (quaternion w, x, y, z).
+// Record per Gaussian: (mu, opacity), (scale, intensity), (quaternion w, x, y, z). With raw activations the
+// opacity is sigmoid(x) and each scale max(exp(x), 1e-5) (pipeline_god_file.py:360-370).
+__device__ __forceinline__ float gp_scale_act(float x) { return fmaxf(expf(x), 1e-5f); }
+
 __global__ void gp_pack_kernel(int n, const float* __restrict__ means, const float* __restrict__ scales,
                                const float* __restrict__ quats, const float* __restrict__ opacity,
-                               const float* __restrict__ intensity, float4* __restrict__ rec) {
+                               const float* __restrict__ intensity, int raw, float4* __restrict__ rec) {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const float* m = means + 3 * (int64_t)g;
     const float* s = scales + 3 * (int64_t)g;
     const float* q = quats + 4 * (int64_t)g;
-    rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], opacity[g]);
-    rec[3 * (int64_t)g + 1] = make_float4(s[0], s[1], s[2], intensity ? intensity[g] : 1.0f);
+    const float o = raw ? 1.0f / (1.0f + expf(-opacity[g])) : opacity[g];
+    const float s0 = raw ? gp_scale_act(s[0]) : s[0], s1 = raw ? gp_scale_act(s[1]) : s[1];
+    const float s2 = raw ? gp_scale_act(s[2]) : s[2];
+    rec[3 * (int64_t)g + 0] = make_float4(m[0], m[1], m[2], o);
+    rec[3 * (int64_t)g + 1] = make_float4(s0, s1, s2, intensity ? intensity[g] : 1.0f);
     rec[3 * (int64_t)g + 2] = make_float4(q[0], q[1], q[2], q[3]);
 }
 
@@ -645,6 +651,13 @@ __global__ __launch_bounds__(256) void gp_reduce_kernel(int n_gauss, const us3d_
         d_means[3 * (int64_t)g + 0] = acc[0];
         d_means[3 * (int64_t)g + 1] = acc[1];
         d_means[3 * (int64_t)g + 2] = acc[2];
+        if (st.raw_activations) {
+            // chain of max(exp(x), 1e-5): d x = d s * s where the clamp is inactive; of the sigmoid: o (1 - o)
+            const float sv[3] = {r1.x, r1.y, r1.z};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) acc[3 + k] = sv[k] > 1e-5f ? acc[3 + k] * sv[k] : 0.f;
+            acc[10] = acc[10] * (r0.w * (1.f - r0.w));
+        }
         d_scales[3 * (int64_t)g + 0] = acc[3];
         d_scales[3 * (int64_t)g + 1] = acc[4];
         d_scales[3 * (int64_t)g + 2] = acc[5];
@@ -701,7 +714,7 @@ int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, 
     if ((rc = begin_instances(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(gp_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
-                           means, scales, quats, opacity, intensity, rec);
+                           means, scales, quats, opacity, intensity, settings->raw_activations, rec);
         if ((rc = check_launch("gp_pack_kernel"))) return rc;
     }
     const int64_t np = plan->n_pairs;

@@ -115,7 +115,8 @@ class GPSettings(ctypes.Structure):
         ("det_eps", ctypes.c_float),
         ("weight_min", ctypes.c_float),
         ("tile_size", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 3),
+        ("raw_activations", ctypes.c_int32),
+        ("reserved", ctypes.c_int32 * 2),
     ]
 
 

@@ -112,6 +112,13 @@ class GPBatch:
         lay = self.layout
         return int(_lib.lib().us3d_gp_workspace_bytes(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap))
 
+    def layout_offsets(self, k_cap: int) -> _lib.Layout:
+        lay = self.layout
+        out = _lib.Layout()
+        _lib.check(_lib.lib().us3d_gp_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap, out),
+                   "us3d_gp_workspace_layout")
+        return out
+
     def map(self, means, scales, quats, opacity, intensity=None) -> Mapped:
         L = _lib.lib()
         k_cap = int(self.hint.capacity(self.layout.n_pairs))

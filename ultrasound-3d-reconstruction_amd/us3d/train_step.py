@@ -137,3 +137,90 @@ class SliceSplatTrainer:
 
     def losses(self) -> torch.Tensor:
         return self.loss.detach().clone()
+
+
+class GaussianPileTrainer:
+    """The same captured step for the GaussianPile us_focus splat (method "gaussianpile",
+    pipeline_god_file.py:435-452): us3d_gp_map with raw activations (its pack applies the opacity sigmoid and
+    the scales' max(exp(x), 1e-5), :360-370; the kernels normalise the quaternions as :367 does) ->
+    us3d_gp_forward -> us3d_image_loss -> us3d_gp_backward (gradients w.r.t. the raw values) -> us3d_adam_step
+    per group -> us3d_adam_advance. Intensities are one, as the facade passes them. Overflow handling and
+    `check_every` as in SliceSplatTrainer."""
+
+    def __init__(self, means: torch.Tensor, raw_scales: torch.Tensor, raw_quats: torch.Tensor,
+                 raw_opacity: torch.Tensor, frames: Sequence[FrameSpec], target: torch.Tensor, cfg,
+                 sweeps: Optional[Sequence[SweepSpec]] = None, lr: float = 1e-3, betas=(0.9, 0.999),
+                 eps: float = 1e-8, w_l1: float = 1.0, w_ssim: float = 0.2, headroom: float = 1.3,
+                 use_graph: bool = True, check_every: int = 1):
+        from .gaussianpile import GPBatch
+
+        dev = means.device
+        self.dev = dev
+        G = means.shape[0]
+        f32 = dict(dtype=torch.float32, device=dev)
+        self.means = means.detach().to(**f32).contiguous().clone()
+        self.raw_s = raw_scales.detach().to(**f32).contiguous().clone()
+        self.raw_q = raw_quats.detach().to(**f32).contiguous().clone()
+        self.raw_o = raw_opacity.detach().to(**f32).reshape(G).contiguous().clone()
+        self.params = [self.means, self.raw_s, self.raw_q, self.raw_o]
+        self.exp_avg = [torch.zeros_like(p) for p in self.params]
+        self.exp_avg_sq = [torch.zeros_like(p) for p in self.params]
+        self.step_count = torch.zeros(1, dtype=torch.int32, device=dev)
+        self.lr, self.betas, self.eps = float(lr), betas, float(eps)
+        self.w_l1, self.w_ssim = float(w_l1), float(w_ssim)
+        settings = cfg.settings()
+        settings.raw_activations = 1
+        self.batch = GPBatch(frames, sweeps or [SweepSpec(0, G)], G, settings, dev, CapacityHint())
+        lay = self.batch.layout
+        self.F = lay.n_frames
+        P = lay.n_pixels
+        self.target = target.detach().to(**f32).reshape(P).contiguous()
+        self.image = torch.empty(P, **f32)
+        self.d_image = torch.empty(P, **f32)
+        self.loss = torch.empty(self.F, **f32)
+        self.loss_tmp = torch.empty(int(_lib.lib().us3d_image_loss_tmp_bytes(self.F)) // 8, dtype=torch.float64,
+                                    device=dev)
+        self.grads = [torch.empty_like(p) for p in self.params]
+        self.headroom = headroom
+        self.use_graph = use_graph
+        self.check_every = max(1, int(check_every))
+        self.n_steps = 0
+        self.graph = None
+        with torch.no_grad():
+            m = self.batch.map(self.means, self.raw_s, self.raw_q, self.raw_o)
+        self._allocate(int(m.stats.n_instances * headroom) + 4096)
+
+    def _allocate(self, k_cap: int) -> None:
+        self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+        self.workspace = torch.empty(self.batch.workspace_bytes(self.k_capacity), dtype=torch.uint8, device=self.dev)
+        self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
+        off = self.batch.layout_offsets(self.k_capacity)
+        self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
+        self.graph = None
+
+    def _body(self) -> None:
+        L = _lib.lib()
+        s = _lib.stream_ptr(self.dev)
+        st = self.batch.settings
+        lay = self.batch.layout
+        _lib.check(L.us3d_gp_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.raw_s), _lib.ptr(self.raw_q),
+                                 _lib.ptr(self.raw_o), None, st, s), "us3d_gp_map")
+        _lib.check(L.us3d_gp_forward(self.plan, st, _lib.ptr(self.image), s), "us3d_gp_forward")
+        _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
+                                     lay.n_frames, self.w_l1, self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
+                                     _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
+        _lib.check(L.us3d_gp_backward(self.plan, st, _lib.ptr(self.d_image), _lib.ptr(self.grads[0]),
+                                      _lib.ptr(self.grads[1]), _lib.ptr(self.grads[2]), _lib.ptr(self.grads[3]), None,
+                                      s), "us3d_gp_backward")
+        b1, b2 = self.betas
+        for p, g, m, v in zip(self.params, self.grads, self.exp_avg, self.exp_avg_sq):
+            _lib.check(L.us3d_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v), p.numel(),
+                                        _lib.ptr(self.step_count), self.overflow_ptr, self.lr, b1, b2, self.eps, s),
+                       "us3d_adam_step")
+        _lib.check(L.us3d_adam_advance(_lib.ptr(self.step_count), self.overflow_ptr, s), "us3d_adam_advance")
+
+    _capture = SliceSplatTrainer._capture
+    overflowed = SliceSplatTrainer.overflowed
+    step = SliceSplatTrainer.step
+    state = SliceSplatTrainer.state
+    losses = SliceSplatTrainer.losses
