@@ -34,7 +34,8 @@ def test_library_loads_and_exports_every_declared_symbol():
 def test_struct_layouts_match_header(tmp_path):
     """sizeof/offsetof of every ABI struct, from a C program compiled against us3d.h, equal the ctypes mirror."""
     structs = {"us3d_frame": _lib.Frame, "us3d_sweep": _lib.Sweep, "us3d_stats": _lib.Stats, "us3d_plan": _lib.Plan,
-               "us3d_ug_settings": _lib.UGSettings, "us3d_ur_settings": _lib.URSettings, "us3d_layout": _lib.Layout}
+               "us3d_ug_settings": _lib.UGSettings, "us3d_ur_settings": _lib.URSettings, "us3d_layout": _lib.Layout,
+               "us3d_gp_settings": _lib.GPSettings}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "us3d.h"', "int main(void) {"]
     for cname, cls in structs.items():
         lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
