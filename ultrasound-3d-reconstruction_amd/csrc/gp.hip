@@ -580,7 +580,7 @@ __device__ __forceinline__ void gp_footprint_vjp(const GPInter& I, const us3d_fr
 
 // One wavefront per Gaussian: its lanes take its pairs (frames); each sums the pair's instance partials in
 // emit order and applies the footprint's vector-Jacobian product; a fixed butterfly sums the lanes.
-__global__ __launch_bounds__(256) void gp_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
+__global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
                                                         int n_sweeps, const us3d_frame* __restrict__ frames,
                                                         us3d_gp_settings st, const int32_t* __restrict__ pair_count,
                                                         const int32_t* __restrict__ pair_off,
