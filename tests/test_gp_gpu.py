@@ -261,3 +261,29 @@ def test_gp_trainer_graph_replays_equal_eager_steps(dev):
             assert torch.equal(a, b)
         losses.append(float(graph.losses().mean()))
     assert losses[-1] < losses[0]
+
+
+def test_gp_two_sweeps_in_one_batch_equal_separate_batches(dev):
+    """Two sweeps with their own Gaussians and frames (3 and 2 frames, 400 and 300 Gaussians) in one batch:
+    each sweep's image equals its own batch bit for bit, and each Gaussian's gradient equals the one from its
+    own sweep's batch (the reduce walks the pairs of the Gaussian's sweep only)."""
+    fa = _frames(3, seed=31)
+    fb = [FrameSpec(**{**f.__dict__, "sweep": 1}) for f in _frames(2, seed=32)]
+    pa = _scene(fa[1], G=400, seed=33)
+    pb = _scene(fb[0], G=300, seed=34)
+    cfg = _cfg(fa[0])
+    both = [torch.cat([a, b]).to(dev).requires_grad_(True) for a, b in zip(pa[:4], pb[:4])]
+    img = render_frames(*both, fa + fb, cfg, sweeps=[SweepSpec(0, 400), SweepSpec(400, 300)])
+    n = fa[0].height * fa[0].width
+    w = torch.linspace(-1.0, 1.0, img.numel(), device=dev)
+    (img * w).sum().backward()
+    sep_a = [t.to(dev).requires_grad_(True) for t in pa[:4]]
+    sep_b = [t.to(dev).requires_grad_(True) for t in pb[:4]]
+    ia = render_frames(*sep_a, fa, cfg)
+    ib = render_frames(*[t for t in sep_b], [FrameSpec(**{**f.__dict__, "sweep": 0}) for f in fb], cfg)
+    assert torch.equal(img[:3 * n], ia) and torch.equal(img[3 * n:], ib)
+    assert float(ia.detach().abs().max()) > 0.05 and float(ib.detach().abs().max()) > 0.05
+    (ia * w[:3 * n]).sum().backward()
+    (ib * w[3 * n:]).sum().backward()
+    for t, a, b in zip(both, sep_a, sep_b):
+        assert torch.equal(t.grad[:400], a.grad) and torch.equal(t.grad[400:], b.grad)
