@@ -261,7 +261,7 @@ __global__ __launch_bounds__(256) void gp_preprocess_kernel(
 // LDS in batches of one instance per thread.
 template <int TS>
 struct GpFwd {
-    static constexpr int kPix = (TS == 8) ? 1 : (TS == 16 ? 2 : 4);
+    static constexpr int kPix = (TS == 8) ? 1 : 4;
     static constexpr int kThreads = TS * TS / kPix;
     static constexpr int kRowStep = TS / kPix;
 };
