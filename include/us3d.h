@@ -76,13 +76,17 @@ typedef struct us3d_sweep {
     int64_t pair_base;
 } us3d_sweep;
 
-/* Data-dependent results, at offset 0 of every workspace. */
+/* Data-dependent results, at offset 0 of every workspace. Every map call resets the block except
+ * overflow_any, which only the caller clears (write 0 to it after allocating the workspace, e.g. with
+ * us3d_fill_i32): a run of replayed steps can then be checked for any overflowed step with one read. */
 typedef struct us3d_stats {
     int64_t n_instances;    /* K: (tile, Gaussian) or (ray, Gaussian) instances of the batch */
     int64_t max_segment;    /* longest per-tile / per-ray list */
     int32_t overflow;       /* 1 if n_instances > k_capacity (outputs are then invalid) */
     int32_t n_sorted_big;   /* segments that took the multi-chunk merge path */
-    int64_t reserved[5];    /* mapper scratch (reserved[0]: tile total); zeroed by every map call */
+    int32_t overflow_any;   /* sticky: set to 1 by every map call that overflows, never cleared by the library */
+    int32_t reserved_i32;
+    int64_t reserved[4];    /* mapper scratch (reserved[0]: tile total); zeroed by every map call */
 } us3d_stats;
 
 /* A batch: frames (possibly from several sweeps) that are mapped, rendered and differentiated together.
@@ -244,7 +248,7 @@ typedef struct us3d_gp_settings {
     float k_sigma;      /* radius_sigma: box half-widths k * sqrt(Sigma2_ii + |Sigma2_01|) */
     float sigma_ele;    /* sigma_z_mm: elevational focus width */
     float det_eps;      /* det_eps */
-    float weight_min;   /* max(opacity_cull_threshold, 1e-12): smaller weights are dropped */
+    float weight_min;   /* max(opacity_cull_threshold, 1e-12): smaller weights are dropped; must be > 0 */
     int32_t tile_size;  /* 8, 16 or 32 (tile_size_px) */
     int32_t raw_activations; /* 1: opacity is a logit (the map applies the sigmoid) and scales are log-scales
                                 (the map applies max(exp(x), 1e-5)), as GaussianParameters activates them
@@ -276,9 +280,12 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
 
 /* Adjoint: replaces the US_FOCUS branch of ProjectedSliceBackwardCUDA (src/shader/wrapper_shader.cu; kernels in
  * src/shader/us_focus.cu). The exact derivative of the forward for every parameter (the reference takes the
- * scale and quaternion gradients by finite differences of its adjoint, us_focus.cu:695-743). Outputs
- * (overwritten): d_means [G,3], d_scales [G,3], d_quats [G,4], d_opacity [G], d_intensity [G] (may be NULL).
- * Requires the map of the same call sequence on the same workspace; deterministic (no atomics). */
+ * scale and quaternion gradients by finite differences of its adjoint, us_focus.cu:695-743). Each (frame, Gaussian)
+ * pair's gradients are sanitised before the frames are summed, as the reference sanitises every render call's
+ * (us_focus.cu:129-137): non-finite -> 0, magnitudes clamped to 1e3 (kGradClamp, plane_cond2d_device.cuh:12);
+ * with raw_activations the chain rules apply after. Outputs (overwritten): d_means [G,3], d_scales [G,3],
+ * d_quats [G,4], d_opacity [G], d_intensity [G] (may be NULL). Requires the map of the same call sequence on the
+ * same workspace; deterministic (no atomics). */
 int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
                      float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
                      void* stream);

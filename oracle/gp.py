@@ -30,6 +30,7 @@ from dataclasses import dataclass
 import torch
 
 JITTER = 1e-6  # plane_cond2d_device.cuh:9, added to the probe-frame covariance diagonal
+GRAD_CLAMP = 1e3  # kGradClamp, plane_cond2d_device.cuh:12
 
 
 @dataclass
@@ -359,3 +360,24 @@ def render_pixels(means, scales, quats, opacity, intensity, fr: GPFrame, st: GPS
     q = dl * (sub["i00"] * dl + sub["i01"] * da) + da * (sub["i01"] * dl + sub["i11"] * da)
     w = sub["b"][None, :] * torch.exp(-0.5 * torch.where(mm, q, torch.zeros_like(q)))
     return torch.where(mm, intensity[gi][None, :] * w, torch.zeros_like(w)).sum(dim=1)
+
+
+def sanitize_grad(g: torch.Tensor) -> torch.Tensor:
+    """us_focus_sanitize_grad (us_focus.cu:129-137): a non-finite value becomes 0, a magnitude above kGradClamp
+    becomes +-kGradClamp (copysign). The reference applies it to every per-Gaussian gradient of each render call,
+    i.e. of each frame (us_focus.cu:694-752)."""
+    g = torch.where(torch.isfinite(g), g, torch.zeros_like(g))
+    return torch.where(g.abs() > GRAD_CLAMP, torch.sign(g) * GRAD_CLAMP, g)
+
+
+def frame_gradients(params64, frame_losses):
+    """The reference's gradients of a multi-frame loss: sum over frames of the sanitised gradient of each frame's
+    own loss (one render call per frame in the reference). params64: the float64 leaf tensors; frame_losses: a
+    list of callables, each returning one frame's scalar loss from params64. Returns one tensor per parameter."""
+    total = [torch.zeros_like(p) for p in params64]
+    for fl in frame_losses:
+        gs = torch.autograd.grad(fl(), params64, allow_unused=True)
+        for t, g in zip(total, gs):
+            if g is not None:
+                t += sanitize_grad(g)
+    return total

@@ -11,6 +11,7 @@ from oracle import fallback as ofb
 from oracle import ug as oug
 from oracle import ur as our
 from tests.ug_helpers import rel_l2
+from tests.ur_helpers import axis_quat as _axis_quat, composed_rotation as _composed_rotation, rodrigues as _rodrigues
 from us3d.scenes import linear_pixels, make_scene, precision_tril_from_scales_quats, quat_to_rot, straight_sweep
 
 
@@ -251,3 +252,122 @@ def test_gl3_transmission_of_one_gaussian():
     assert aux[1, 0] == 1.0
     exact = math.sqrt(2 * math.pi) * math.erf(math.sqrt(2))
     assert abs(psi - exact) / exact < 0.03
+
+
+# ---------------------------------------------------------------- UltraG-Ray rotation / covariance conventions
+# The reference's covariance.cuh (quaternion -> R, Sigma^-1 from (s, R)) is absent from its tree. What fixes the
+# conventions: the repo-wide quaternion layout (w, x, y, z) of gaussianplie_renderer/include/shader/shared.cuh:15-34
+# and reference.py:7-23; the transmission integral's whitening o_local = R^T (o - mu) / s
+# (ultragray_renderer/include/shader/shared.cuh:37-48), which makes R's columns the principal axes; and the echo
+# weight must describe the same Gaussian. These checks build R independently (Rodrigues' formula and the Hamilton
+# product) and use a rotation that is not symmetric, so an R <-> R^T swap or an echo covariance that disagrees with
+# the whitening fails them.
+
+
+def _one_ray_frame(pixels, ro, rd, axes):
+    al, aa, ae = axes
+    H = pixels.shape[0]
+    return our.RayFrame(origin=np.zeros(3, np.float32), axis_lat=al.astype(np.float32),
+                        axis_ax=aa.astype(np.float32), axis_ele=ae.astype(np.float32), W=1, H=H, width_mm=1.0,
+                        near_mm=0.0, col_offset=0.0, pixels=pixels.reshape(1, H, 3).astype(np.float32),
+                        ray_o=ro.reshape(1, 3).astype(np.float32), ray_d=rd.reshape(1, 3).astype(np.float32),
+                        delta=np.zeros(1, np.float32))
+
+
+def test_ray_echo_weight_is_the_gaussian_with_principal_axes_the_columns_of_R():
+    """One anisotropic Gaussian (s = 3, 1, 0.5 mm) under the composed rotation, tau = 1 (transmission 1): the echo
+    weight the oracle accumulates (aux den) at sample p equals exp(-1/2 |S^-1 R^T (p - mu)|^2) -- the whitening of
+    shared.cuh:37-48 -- at points along each principal axis R e_k (where it is exp(-t^2 / 2 s_k^2)) and at
+    random points within the echo range."""
+    q, R = _composed_rotation()
+    s = np.array([3.0, 1.0, 0.5])
+    mu = np.array([0.0, 30.0, 0.0])
+    rng = np.random.default_rng(3)
+    pts = [mu + t * R[:, k] for k in range(3) for t in (-1.5 * s[k], -0.5 * s[k], 0.7 * s[k], 1.2 * s[k])]
+    pts += [mu + rng.uniform(-2.0, 2.0, 3) for _ in range(12)]
+    pts = np.array(pts)
+    axes = (np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0]))
+    geo = _one_ray_frame(pts, np.zeros(3), axes[1], axes)
+    params = [mu[None], s[None], q[None], np.array([60.0]), np.array([[0.5, 0, 0, 0]])]
+    st = our.RaySettings(k_sigma=3.0, Rw=2.0)
+    rm = our.map_frame(params[0], params[1], geo, st)
+    assert rm.indices.tolist() == [0]
+    _img, aux = our.forward(params, geo, rm, st)
+    z = (pts - mu) @ R / s  # S^-1 R^T (p - mu), row per point
+    want = np.exp(-0.5 * (z * z).sum(1))
+    assert np.allclose(aux[:, 0], 1.0)
+    np.testing.assert_allclose(aux[:, 2], want, rtol=2e-6, atol=1e-12)
+    for k in range(3):  # on the principal axes: the 1-D Gaussian of that axis's scale
+        t = np.array([-1.5, -0.5, 0.7, 1.2]) * s[k]
+        np.testing.assert_allclose(aux[4 * k:4 * k + 4, 2], np.exp(-0.5 * (t / s[k]) ** 2), rtol=2e-6)
+    # the transposed rotation describes a different Gaussian at these points
+    zt = (pts - mu) @ R.T / s
+    assert np.abs(np.exp(-0.5 * (zt * zt).sum(1)) - want).max() > 0.1
+
+
+def test_ray_transmission_whitening_uses_the_same_principal_axes():
+    """A ray through the centre of the same Gaussian along its long principal axis R e_0 (s_0 = 3 mm), tau = 1/2,
+    window R_w = 2 mm: along the ray the whitened distance is (t - t_c) / s_0, so a sample past the window sees
+    T = 1/2 + 1/2 e^-psi with psi the 3-point Gauss-Legendre rule 2 (5/9 e^{-(2 xi / 3)^2 / 2} 2 + 8/9); a sample
+    in front of the centre is not attenuated."""
+    q, R = _composed_rotation()
+    s = np.array([3.0, 1.0, 0.5])
+    d = R[:, 0]
+    e1 = np.cross(d, [0.0, 0.0, 1.0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(e1, d)
+    mu = 30.0 * d
+    geo = _one_ray_frame(np.array([40.0 * d, 29.0 * d]), np.zeros(3), d, (e1, d, e2))
+    params = [mu[None], s[None], q[None], np.array([0.0]), np.array([[0.5, 0, 0, 0]])]
+    st = our.RaySettings(k_sigma=3.0, Rw=2.0)
+    rm = our.map_frame(params[0], params[1], geo, st)
+    _img, aux = our.forward(params, geo, rm, st)
+    xi = 0.7745966692
+    psi = 2.0 * (0.5555555556 * math.exp(-0.5 * (2 * xi / 3.0) ** 2) * 2 + 0.8888888889)
+    assert aux[0, 0] == pytest.approx(0.5 + 0.5 * math.exp(-psi), rel=1e-6)
+    assert aux[1, 0] == 1.0
+    # whitened with R in place of R^T, the ray's direction would be S^-1 R d, whose length is not 1/s_0, so psi
+    # would differ
+    assert abs(np.linalg.norm(R @ d / s) - 1.0 / 3.0) > 0.01
+
+
+def test_ray_quaternion_sign_and_scale_invariance_and_gradient_orthogonality():
+    """R(q) = R(-q) = R(2q) (a rotation is fixed by the unit quaternion up to sign): the FP64 images are identical,
+    and since the image is invariant under q -> lambda q its gradient satisfies dL/dq . q = 0,
+    dL/d(-q) = -dL/dq and dL/d(2q) = dL/dq / 2."""
+    params, geo, st, rm = _ray_case(n=48)
+    dB = np.random.default_rng(5).normal(size=geo.W * geo.H).astype(np.float32)
+    base = our.forward(params, geo, rm, st, fp64=True)
+    g = our.backward_f64(params, geo, rm, st, dB)
+    assert np.abs(g[2]).max() > 0
+    for lam in (-1.0, 2.0):
+        p2 = [p.copy() for p in params]
+        p2[2] = p2[2] * lam
+        assert np.array_equal(our.forward(p2, geo, rm, st, fp64=True), base)
+        g2 = our.backward_f64(p2, geo, rm, st, dB)
+        np.testing.assert_allclose(g2[2], g[2] / lam, rtol=1e-9, atol=1e-12 * np.abs(g[2]).max())
+    dots = np.abs((g[2] * params[2]).sum(1))
+    assert dots.max() <= 1e-9 * np.abs(g[2]).max() * np.abs(params[2]).max()
+
+
+def test_oracle_quaternion_matrix_is_the_rodrigues_rotation():
+    """The echo weight along many directions pins the full matrix: for the composed rotation the oracle's weight
+    equals the closed form built from Rodrigues' R only (checked above); here the same holds for 20 random
+    rotations given as unnormalised quaternions (q = 1.7 * (cos a/2, sin a/2 n)) against R = exp(a [n]x)."""
+    rng = np.random.default_rng(8)
+    axes = (np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0]))
+    s = np.array([2.5, 1.2, 0.6])
+    mu = np.array([0.0, 30.0, 0.0])
+    st = our.RaySettings(k_sigma=3.0, Rw=2.0)
+    for _ in range(20):
+        n = rng.normal(size=3)
+        ang = rng.uniform(10, 170)
+        q = 1.7 * _axis_quat(n, ang)
+        R = _rodrigues(n, ang)
+        pts = mu + rng.uniform(-2.0, 2.0, (8, 3))
+        geo = _one_ray_frame(pts, np.zeros(3), axes[1], axes)
+        params = [mu[None], s[None], q[None], np.array([60.0]), np.array([[0.5, 0, 0, 0]])]
+        rm = our.map_frame(params[0], params[1], geo, st)
+        _img, aux = our.forward(params, geo, rm, st)
+        z = (pts - mu) @ R / s
+        np.testing.assert_allclose(aux[:, 2], np.exp(-0.5 * (z * z).sum(1)), rtol=2e-5, atol=1e-12)

@@ -2,7 +2,8 @@
 
 Bars: the image within 1e-5 relative L2 of the FP32 oracle (same footprints, tile lists and decisions; FP32 sums
 in list order) and within 2e-5 * max(1, |ref|) at every pixel; the gradients of all five parameter groups within
-1e-4 relative L2 of torch autograd through the FP64 oracle.
+1e-4 relative L2 of torch autograd through the FP64 oracle, each frame's gradient sanitised as the reference
+sanitises every render call's (non-finite -> 0, magnitudes clamped to 1e3; us_focus.cu:129-137).
 """
 
 import numpy as np
@@ -69,9 +70,12 @@ def _cfg(fr, ts=16, **kw):
                                     lateral_col_offset=fr.col_offset, tile_size_px=ts, **kw)
 
 
-@pytest.mark.parametrize("ts", [16, 8, 32])
-def test_gp_forward_and_gradients_match_oracle(dev, ts):
-    fr = _frames(1)[0]
+@pytest.mark.parametrize("ts, H, W", [(16, 64, 80), (8, 64, 80), (32, 64, 80), (8, 70, 84), (16, 70, 84),
+                                       (32, 70, 84)])
+def test_gp_forward_and_gradients_match_oracle(dev, ts, H, W):
+    """H = 70, W = 84 leave the last tile row and column partly outside the frame at every tile size (the
+    forward's multi-row threads guard those rows)."""
+    fr = _frames(1, H=H, W=W)[0]
     params = _scene(fr)
     cfg = _cfg(fr, ts, radius_sigma=3.0, sigma_z_mm=1.2)
     st = gp.GPSettings(k_sigma=3.0, sigma_ele=1.2, det_eps=1e-6, weight_min=1e-6, tile_size=ts)
@@ -88,11 +92,48 @@ def test_gp_forward_and_gradients_match_oracle(dev, ts):
     assert np.all(np.abs(got.numpy() - ref32.numpy()) <= 2e-5 * np.maximum(1.0, np.abs(ref32.numpy())))
     p64 = [p.double().clone().requires_grad_(True) for p in params]
     img64 = gp.render(*p64[:4], p64[4], ofr, st, fp64=True)
-    refs = torch.autograd.grad((img64 * dI.double()).sum(), p64)
+    refs = [gp.sanitize_grad(r) for r in torch.autograd.grad((img64 * dI.double()).sum(), p64)]
     for name, t, r in zip(("means", "scales", "quats", "opacity", "intensity"), tens, refs):
         assert float(r.abs().max()) > 0, name
         err = rel_l2(t.grad.detach().cpu().numpy(), r.numpy())
         assert err < GRAD_TOL, f"ts {ts} {name}: rel L2 {err:.3e}"
+
+
+def test_gp_gradients_are_sanitised_per_frame(dev):
+    """Small Gaussians (0.06-0.25 mm, so det2^-1/2 makes them bright) over two frames: some per-frame gradients
+    exceed the reference's clamp of 1e3, and the renderer's gradient equals the sum over frames of each frame's
+    clamped gradient (the reference clamps every render call's gradients, us_focus.cu:129-137, 694-752), not the
+    clamp of the sum; within 1e-4 rel. L2 of the FP64 oracle per parameter group."""
+    frames = _frames(2, H=48, W=64, seed=41)
+    frames[1] = FrameSpec(**{**frames[0].__dict__, "origin": frames[0].origin + 0.05 * frames[0].axis_ele})
+    rng = np.random.default_rng(42)
+    G = 60
+    means, _s, quats, opac, inten = _scene(frames[0], G=G, seed=43)
+    scales = torch.tensor(np.exp(rng.uniform(np.log(0.06), np.log(0.25), (G, 3))), dtype=torch.float32)
+    params = [means, scales, quats, opac, inten]
+    cfg = _cfg(frames[0], 16, radius_sigma=3.0, sigma_z_mm=1.0)
+    st = gp.GPSettings(k_sigma=3.0, sigma_ele=1.0, det_eps=1e-6, weight_min=1e-6, tile_size=16)
+    n = frames[0].height * frames[0].width
+    dI = torch.tensor(rng.normal(size=(2, n)), dtype=torch.float32)
+    tens = [p.to(dev).clone().requires_grad_(True) for p in params]
+    image = render_frames(tens[0], tens[1], tens[2], tens[3], frames, cfg, intensity=tens[4])
+    (image * dI.reshape(-1).to(dev)).sum().backward()
+    p64 = [p.double().clone().requires_grad_(True) for p in params]
+    H, W = frames[0].height, frames[0].width
+
+    def frame_loss(k):
+        return lambda: (gp.render(*p64[:4], p64[4], _oframe(frames[k]), st, fp64=True)
+                        * dI[k].double().reshape(H, W)).sum()
+
+    refs = gp.frame_gradients(p64, [frame_loss(0), frame_loss(1)])
+    raw = [torch.autograd.grad(frame_loss(k)(), p64) for k in range(2)]
+    clamped = sum(int((r.abs() > gp.GRAD_CLAMP).sum()) for rk in raw for r in rk)
+    assert clamped > 0, "no per-frame gradient reached the clamp"
+    summed = [gp.sanitize_grad(a + b) for a, b in zip(*raw)]
+    assert any(float((a - b).abs().max()) > 1.0 for a, b in zip(refs, summed)), "per-frame clamp not exercised"
+    for name, t, r in zip(("means", "scales", "quats", "opacity", "intensity"), tens, refs):
+        err = rel_l2(t.grad.detach().cpu().numpy(), r.numpy())
+        assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
 
 
 def test_gp_batch_equals_per_frame_and_is_deterministic(dev):
@@ -170,17 +211,17 @@ def test_gp_config2_full_size_sampled_pixels_and_gradients(dev):
     one32 = torch.ones(G)
     p64 = [t.double().clone().requires_grad_(True) for t in base]
     one64 = torch.ones(G, dtype=torch.float64)
-    ref32, loss64 = [], 0.0
+    ref32, losses64 = [], []
     for k, f in enumerate(sel):
         of = _oframe(frames[f])
         ref32.append(gp.render_pixels(*base, one32, of, st, rows[k], cols[k]))
-        loss64 = loss64 + (gp.render_pixels(*p64, one64, of, st, rows[k], cols[k], fp64=True) * wts[k].double()).sum()
+        losses64.append((lambda of, k: lambda: (gp.render_pixels(*p64, one64, of, st, rows[k], cols[k], fp64=True)
+                                                 * wts[k].double()).sum())(of, k))
     ref32 = torch.cat(ref32)
     assert float(ref32.abs().max()) > 0.05
     assert bool(torch.all((got - ref32).abs() <= 2e-5 * torch.clamp(ref32.abs(), min=1.0)))
-    loss64.backward()
-    for name, t, r in zip(("means", "scales", "quats", "opacity"), params, p64):
-        ref = r.grad
+    refs = gp.frame_gradients(p64, losses64)
+    for name, t, ref in zip(("means", "scales", "quats", "opacity"), params, refs):
         assert float(ref.abs().max()) > 0, name
         err = float((t.grad.detach().cpu().double() - ref).norm() / ref.norm())
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"

@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import torch
 
@@ -71,3 +73,35 @@ def gpu_ray_render(scene, frames, cfg, dev, deltas=None, d_image=None, hint=None
         (image * torch.as_tensor(d_image, dtype=torch.float32, device=dev)).sum().backward()
         out["grads"] = [p.grad.detach().cpu().numpy().astype(np.float64) for p in params]
     return out
+
+
+# Rotations built independently of the oracle: Rodrigues' formula and the Hamilton product of (w, x, y, z)
+# quaternions (a unit quaternion (cos a/2, sin a/2 n) is the rotation by a about n; q_a (x) q_b is R_a R_b).
+
+
+def rodrigues(axis, deg):
+    n = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    K = np.array([[0, -n[2], n[1]], [n[2], 0, -n[0]], [-n[1], n[0], 0]])
+    a = math.radians(deg)
+    return np.eye(3) + math.sin(a) * K + (1 - math.cos(a)) * K @ K
+
+
+def axis_quat(axis, deg):
+    n = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    a = math.radians(deg) / 2
+    return np.array([math.cos(a), *(math.sin(a) * n)])
+
+
+def hamilton(p, q):
+    w1, x1, y1, z1 = p
+    w2, x2, y2, z2 = q
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def composed_rotation():
+    """30 degrees about (0, 0, 1) after 20 degrees about (1, 1, 0): q = q_a (x) q_b <-> R = R_a R_b."""
+    qa, qb = axis_quat([0, 0, 1], 30), axis_quat([1, 1, 0], 20)
+    R = rodrigues([0, 0, 1], 30) @ rodrigues([1, 1, 0], 20)
+    assert np.abs(R - R.T).max() > 0.3  # far from symmetric: R and R^T are different rotations
+    return hamilton(qa, qb), R

@@ -578,8 +578,21 @@ __device__ __forceinline__ void gp_footprint_vjp(const GPInter& I, const us3d_fr
     for (int r = 0; r < 3; ++r) acc[r] += g_muc[0] * ax3[0][r] + g_muc[1] * ax3[1][r] + g_muc[2] * ax3[2][r];
 }
 
+// Per (frame, Gaussian) pair, each parameter gradient is sanitised before the pairs are summed, as the reference
+// sanitises the gradients of every render call (one frame) per Gaussian (us_focus_sanitize_grad, us_focus.cu:129-137,
+// applied at :694-752, with kGradClamp = 1e3 of plane_cond2d_device.cuh:12): a non-finite value becomes 0, a
+// magnitude above 1e3 is clamped to +-1e3. The raw-activation chain rules come after, as torch autograd applies them
+// to the reference's returned gradients.
+constexpr float kGpGradClamp = 1e3f;
+
+__device__ __forceinline__ float gp_sanitize_grad(float g) {
+    if (!isfinite(g)) return 0.f;
+    return fminf(fmaxf(g, -kGpGradClamp), kGpGradClamp);
+}
+
 // One wavefront per Gaussian: its lanes take its pairs (frames); each sums the pair's instance partials in
-// emit order and applies the footprint's vector-Jacobian product; a fixed butterfly sums the lanes.
+// emit order, applies the footprint's vector-Jacobian product and sanitises the pair's gradients; a fixed
+// butterfly sums the lanes.
 __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
                                                         int n_sweeps, const us3d_frame* __restrict__ frames,
                                                         us3d_gp_settings st, const int32_t* __restrict__ pair_count,
@@ -613,12 +626,17 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
             gb += a.x; gI += a.y; gm0 += a.z; gm1 += a.w;
             g00 += b.x; g01 += b.y; g11 += b.z;
         }
-        accI += gI;
+        accI += gp_sanitize_grad(gI);
         const us3d_frame& fr = frames[sw.f_begin + fl];
         GPFoot F;
         GPInter I;
         gp_footprint<true>(r0, r1, r2, fr, st.k_sigma, st.sigma_ele, st.det_eps, F, &I);
-        gp_footprint_vjp(I, fr, gb, gm0, gm1, g00, g01, g11, acc);
+        float pacc[11];
+#pragma unroll
+        for (int k = 0; k < 11; ++k) pacc[k] = 0.f;
+        gp_footprint_vjp(I, fr, gb, gm0, gm1, g00, g01, g11, pacc);
+#pragma unroll
+        for (int k = 0; k < 11; ++k) acc[k] += gp_sanitize_grad(pacc[k]);
     };
     const uint64_t below = (1ull << lane) - 1ull;
     int q = -1;  // queued visible frame of this lane (lanes [0, nq))
@@ -677,7 +695,9 @@ static int gp_check_settings(const us3d_gp_settings* s) {
     if (!(s->k_sigma > 0.0f)) return fail(US3D_ERR_ARGUMENT, "k_sigma must be > 0");
     if (!(s->sigma_ele > 0.0f)) return fail(US3D_ERR_ARGUMENT, "sigma_ele must be > 0");
     if (!(s->det_eps > 0.0f)) return fail(US3D_ERR_ARGUMENT, "det_eps must be > 0");
-    if (!(s->weight_min >= 0.0f)) return fail(US3D_ERR_ARGUMENT, "weight_min must be >= 0");
+    // the reference floors it at 1e-12 (max(threshold, 1e-12)); at 0 a listed pair whose base weight underflowed
+    // to 0 would make the adjoint divide 0 by 0
+    if (!(s->weight_min > 0.0f)) return fail(US3D_ERR_ARGUMENT, "weight_min must be > 0");
     return US3D_OK;
 }
 

@@ -266,11 +266,14 @@ __global__ void init_stats_kernel(us3d_stats* stats) {
     stats->max_segment = 0;
     stats->overflow = 0;
     stats->n_sorted_big = 0;
-    for (int i = 0; i < 5; ++i) stats->reserved[i] = 0;
+    stats->reserved_i32 = 0;
+    for (int i = 0; i < 4; ++i) stats->reserved[i] = 0;  // overflow_any is the caller's: left as it is
 }
 
 __global__ void capacity_kernel(us3d_stats* stats, int64_t k_cap) {
-    stats->overflow = stats->n_instances > k_cap ? 1 : 0;
+    const int over = stats->n_instances > k_cap ? 1 : 0;
+    stats->overflow = over;
+    if (over) stats->overflow_any = 1;
 }
 
 // off[n] = total (or 0 when the batch overflowed), closing an (n+1)-entry offset array.

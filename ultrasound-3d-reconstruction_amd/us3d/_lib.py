@@ -59,7 +59,9 @@ class Stats(ctypes.Structure):
         ("max_segment", ctypes.c_int64),
         ("overflow", ctypes.c_int32),
         ("n_sorted_big", ctypes.c_int32),
-        ("reserved", ctypes.c_int64 * 5),
+        ("overflow_any", ctypes.c_int32),
+        ("reserved_i32", ctypes.c_int32),
+        ("reserved", ctypes.c_int64 * 4),
     ]
 
 

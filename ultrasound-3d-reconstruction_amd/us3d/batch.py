@@ -191,6 +191,14 @@ class DeviceTables:
         return p
 
 
+def new_workspace(nbytes: int, device) -> torch.Tensor:
+    """A workspace of `nbytes` bytes with its us3d_stats block zeroed, so the sticky overflow_any flag starts
+    clear (every map call resets the rest of the block itself)."""
+    ws = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+    ws[: ctypes.sizeof(_lib.Stats)].zero_()
+    return ws
+
+
 def read_stats(workspace: torch.Tensor) -> _lib.Stats:
     """Copy the us3d_stats block at workspace offset 0 to the host (synchronises the stream)."""
     raw = workspace[: ctypes.sizeof(_lib.Stats)].cpu().numpy().tobytes()

@@ -23,7 +23,7 @@ import torch.nn as nn
 import torch.nn.functional as F
 
 from . import _lib
-from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, plan_layout, read_stats
+from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, new_workspace, plan_layout, read_stats
 
 
 @dataclass
@@ -102,6 +102,12 @@ class GPBatch:
 
     def __init__(self, frames: Sequence[FrameSpec], sweeps: Sequence[SweepSpec], n_gauss: int,
                  settings: _lib.GPSettings, device: torch.device, hint: Optional[CapacityHint] = None):
+        # the GaussianPile kernels evaluate the linear-probe grid only (the reference's slice grid,
+        # slice_grid_geometry.cuh:38-45); fan frames are an UltraGauss geometry
+        if any(int(f.geometry) != 0 for f in frames):
+            raise ValueError("the GaussianPile renderer takes linear-probe frames only (geometry 0)")
+        if not (float(settings.weight_min) > 0.0):
+            raise ValueError("weight_min must be > 0 (the reference floors it at 1e-12)")
         self.settings = settings
         self.device = device
         self.layout = plan_layout(frames, sweeps, n_gauss, settings.tile_size)
@@ -123,7 +129,7 @@ class GPBatch:
         L = _lib.lib()
         k_cap = int(self.hint.capacity(self.layout.n_pairs))
         while True:
-            ws = torch.empty(self.workspace_bytes(k_cap), dtype=torch.uint8, device=self.device)
+            ws = new_workspace(self.workspace_bytes(k_cap), self.device)
             m = Mapped(self.tables, ws, k_cap)
             _lib.check(L.us3d_gp_map(m.plan(), _lib.ptr(means), _lib.ptr(scales), _lib.ptr(quats), _lib.ptr(opacity),
                                      _lib.ptr(intensity), self.settings, _lib.stream_ptr(self.device)), "us3d_gp_map")

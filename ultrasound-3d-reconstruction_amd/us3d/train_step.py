@@ -13,12 +13,13 @@ parameters. The overflow check synchronises; `check_every` sets how often it run
 
 from __future__ import annotations
 
+import ctypes
 from typing import List, Optional, Sequence
 
 import torch
 
 from . import _lib
-from .batch import CapacityHint, FrameSpec, SweepSpec, read_stats
+from .batch import CapacityHint, FrameSpec, SweepSpec, new_workspace, read_stats
 from .ultragauss import UGBatch, UltraGaussRenderConfig
 
 
@@ -71,7 +72,7 @@ class SliceSplatTrainer:
     # ------------------------------------------------------------------ plumbing
     def _allocate(self, k_cap: int) -> None:
         self.k_capacity = min(int(k_cap), (1 << 31) - 1)
-        self.workspace = torch.empty(self.batch.workspace_bytes(self.k_capacity), dtype=torch.uint8, device=self.dev)
+        self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
         self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
         off = self.batch.layout_offsets(self.k_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
@@ -112,7 +113,12 @@ class SliceSplatTrainer:
         self.graph = g
 
     def overflowed(self) -> bool:
-        return bool(read_stats(self.workspace).overflow)
+        """True if any step since the workspace was allocated (or since clear_overflow) overflowed: the sticky
+        us3d_stats.overflow_any flag, which every overflowing map sets and none clears (synchronises)."""
+        return bool(read_stats(self.workspace).overflow_any)
+
+    def clear_overflow(self) -> None:
+        self.workspace[: ctypes.sizeof(_lib.Stats)].zero_()
 
     # ------------------------------------------------------------------ public
     def step(self) -> None:
@@ -192,7 +198,7 @@ class GaussianPileTrainer:
 
     def _allocate(self, k_cap: int) -> None:
         self.k_capacity = min(int(k_cap), (1 << 31) - 1)
-        self.workspace = torch.empty(self.batch.workspace_bytes(self.k_capacity), dtype=torch.uint8, device=self.dev)
+        self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
         self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
         off = self.batch.layout_offsets(self.k_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
@@ -221,6 +227,7 @@ class GaussianPileTrainer:
 
     _capture = SliceSplatTrainer._capture
     overflowed = SliceSplatTrainer.overflowed
+    clear_overflow = SliceSplatTrainer.clear_overflow
     step = SliceSplatTrainer.step
     state = SliceSplatTrainer.state
     losses = SliceSplatTrainer.losses

@@ -21,7 +21,7 @@ import torch
 import torch.nn as nn
 
 from . import _lib
-from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, plan_layout, read_stats
+from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, new_workspace, plan_layout, read_stats
 
 FALLBACK_CHI2 = 120.0  # exp(clamp(-q/2, min=-60)) saturates at q = 120
 
@@ -122,7 +122,7 @@ class UGBatch:
         L = _lib.lib()
         k_cap = int(k_capacity or self.hint.capacity(self.layout.n_pairs))
         while True:
-            ws = torch.empty(self.workspace_bytes(k_cap), dtype=torch.uint8, device=self.device)
+            ws = new_workspace(self.workspace_bytes(k_cap), self.device)
             m = Mapped(self.tables, ws, k_cap)
             plan = m.plan()
             _lib.check(L.us3d_ug_map(plan, _lib.ptr(means), _lib.ptr(tril), _lib.ptr(opacity), _lib.ptr(intensity),

@@ -21,7 +21,7 @@ import torch
 import torch.nn as nn
 
 from . import _lib
-from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, plan_layout, read_stats
+from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, new_workspace, plan_layout, read_stats
 
 
 @dataclass
@@ -146,7 +146,7 @@ class URBatch:
         L = _lib.lib()
         k_cap = self.hint.capacity(self.layout.n_pairs)
         while True:
-            ws = torch.empty(self.workspace_bytes(k_cap), dtype=torch.uint8, device=self.device)
+            ws = new_workspace(self.workspace_bytes(k_cap), self.device)
             m = Mapped(self.tables, ws, k_cap)
             _lib.check(L.us3d_ur_map(m.plan(), _lib.ptr(means), _lib.ptr(scales), _lib.ptr(quats), _lib.ptr(atten),
                                      _lib.ptr(sh), self.settings, _lib.stream_ptr(self.device)), "us3d_ur_map")
