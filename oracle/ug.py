@@ -38,11 +38,13 @@ def lib() -> ctypes.CDLL:
         L.ug_oracle_preprocess.argtypes = [_I, _P, _P, _P, _P, _P, _P, _I, _I, _F, _F, _F, _F, _F, _F, _I, _I, _P, _P, _P]
         L.ug_oracle_emit_sort.argtypes = [_I, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P]
         L.ug_oracle_pixel_lists.argtypes = [_I, _P, _P, _P, _I, _I, _P, _P, _F, _F, _I, _P, _P, _P, _I, _P, _P, _P]
-        fwd = [_I, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _F, _I, _P, _P, _P, _P]
+        grid = [_P, _I, _F, _F, _F, _F]  # plane form (or NULL), W, col_offset, delta_lat, near, delta_ax
+        fwd = [_I, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _F, _I, _P, _P, _P, _P] + grid
         L.ug_oracle_forward_f32.argtypes = fwd
         L.ug_oracle_forward_f64.argtypes = fwd
         L.ug_oracle_backward_f64.argtypes = [_I, _P, _P, _P, _P, _P, _I64, _P, _P, _P, _P, _P, _F, _F, _F, _F, _F, _I,
-                                             _P, _P, _P, _P, _P, _P, _P]
+                                             _P, _P, _P, _P, _P, _P, _P] + grid
+        L.ug_oracle_plane_form.argtypes = [_I, _P, _P, _P, _P, _P, _P, _F, _P]
         L.ug_oracle_backward_f32.argtypes = L.ug_oracle_backward_f64.argtypes
         _lib = L
     return _lib
@@ -75,6 +77,26 @@ class FrameGeom:
         return FrameGeom(n(spec.origin), n(spec.axis_lat), n(spec.axis_ax), n(spec.axis_ele), int(spec.height),
                          int(spec.width), float(spec.delta_lat), float(spec.delta_ax), float(spec.near_mm),
                          float(spec.col_offset))
+
+
+def plane_form(means, tril, fr: FrameGeom, beta=1e-4) -> np.ndarray:
+    """(G, 6) float32 (cx, cy, K, p, qq, s): each Gaussian's plane form of q on the frame's own pixel grid, as the
+    kernels set it up per visible pair. Passing it to forward / backward_f64 (with the frame's grid) makes their
+    q <= chi2 decisions, and the FP32 values of q, those of a frame rendered on its own grid (pixels = NULL)."""
+    L = lib()
+    means = _c(means, np.float32)
+    tril = _c(tril, np.float32)
+    out = np.zeros((means.shape[0], 6), np.float32)
+    L.ug_oracle_plane_form(means.shape[0], _p(means), _p(tril), _p(_c(fr.origin, np.float32)),
+                           _p(_c(fr.axis_lat, np.float32)), _p(_c(fr.axis_ax, np.float32)),
+                           _p(_c(fr.axis_ele, np.float32)), beta, _p(out))
+    return out
+
+
+def _grid_args(plane, fr: "FrameGeom"):
+    if plane is None:
+        return [None, 0, 0.0, 0.0, 0.0, 0.0]
+    return [_p(plane), fr.W, fr.col_offset, fr.delta_lat, fr.near_mm, fr.delta_ax]
 
 
 @dataclass
@@ -143,8 +165,9 @@ class Semantics:
         return Semantics(chi2=120.0, beta=beta, alpha_min=-1.0, bg=1e-6, bg_intensity=0.0, alpha_excludes_bg=True)
 
 
-def forward(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semantics, fp64=False):
-    """FP32 (canonical) or FP64 forward; the FP64 path takes the parameter arrays in float64."""
+def forward(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semantics, fp64=False, plane=None):
+    """FP32 (canonical) or FP64 forward; the FP64 path takes the parameter arrays in float64. plane: the frame's
+    plane forms (plane_form) when it is rendered on its own grid."""
     L = lib()
     counts, offsets, indices = lists
     P = counts.shape[0]
@@ -152,16 +175,17 @@ def forward(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semanti
     image, alpha, depth = np.zeros(P, dt), np.zeros(P, dt), np.zeros(P, dt)
     sums = np.zeros((P, 3), dt)
     fn = L.ug_oracle_forward_f64 if fp64 else L.ug_oracle_forward_f32
+    plane_c = None if plane is None else _c(plane, np.float32)  # kept alive across the call
     fn(int(np.asarray(means).shape[0]), _p(_c(means, dt)), _p(_c(tril, dt)), _p(_c(opac, dt)),
        _p(_c(inten, dt)), _p(_c(pixels, dt)), P, _p(_c(fr.origin, np.float32)),
        _p(_c(fr.axis_ele, np.float32)), _p(_c(counts, np.int32)), _p(_c(offsets, np.int32)), _p(_c(indices, np.int32)),
        sem.chi2, sem.beta, sem.alpha_min, sem.bg, sem.bg_intensity, int(sem.alpha_excludes_bg), _p(image), _p(alpha),
-       _p(depth), _p(sums))
+       _p(depth), _p(sums), *_grid_args(plane_c, fr))
     return image, alpha, depth, sums
 
 
 def backward_f64(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Semantics, d_image, d_alpha=None,
-                 d_depth=None, gate=True, fp32=False):
+                 d_depth=None, gate=True, fp32=False, plane=None):
     """FP64 restatement of the reference backward (FP32 with fp32=True: the reference's precision, naive
     per-pair accumulation in pixel order)."""
     L = lib()
@@ -173,9 +197,11 @@ def backward_f64(means, tril, opac, inten, pixels, fr: FrameGeom, lists, sem: Se
     opt = lambda a: None if a is None else _c(a, np.float32)  # noqa: E731
     gA, gD = opt(d_alpha), opt(d_depth)
     fn = L.ug_oracle_backward_f32 if fp32 else L.ug_oracle_backward_f64
+    plane_c = None if plane is None else _c(plane, np.float32)  # kept alive across the call
     fn(G, _p(_c(means, f8)), _p(_c(tril, f8)), _p(_c(opac, f8)),
                              _p(_c(inten, f8)), _p(_c(pixels, f8)), P, _p(_c(fr.origin, np.float32)),
                              _p(_c(fr.axis_ele, np.float32)), _p(_c(counts, np.int32)), _p(_c(offsets, np.int32)),
                              _p(_c(indices, np.int32)), sem.chi2, sem.beta, sem.alpha_min, sem.bg, sem.bg_intensity,
-                             int(gate), _p(_c(d_image, np.float32)), _p(gA), _p(gD), _p(dm), _p(dt), _p(do), _p(di))
+                             int(gate), _p(_c(d_image, np.float32)), _p(gA), _p(gD), _p(dm), _p(dt), _p(do), _p(di),
+                             *_grid_args(plane_c, fr))
     return dm, dt, do, di

@@ -99,9 +99,85 @@ bool keep_pair(const In* raw, float beta, In px, In py, In pz, In mx, In my, In 
     return !(q > chi2);
 }
 
+// Plane form of q for a frame's own linear-probe pixel grid, restating the kernels' pair setup (ug.hip,
+// pair_plane_form: FP64 from the FP32 record values) and evaluation (plane_q: FP32, fixed order). The pixel at probe
+// coordinates (l, a) = (lat_g + x, ax_g + y) has L^T (p - mu) = c0 + x A + y B (A = L^T a_lat, B = L^T a_ax, c0 =
+// L^T g0, g0 = (o - mu) + lat_g a_lat + ax_g a_ax), so q = (p (l - cx) + qq (a - cy))^2 + (s (a - cy))^2 + K about
+// its minimum. This is the renderer's canonical FP32 q for frames rendered on their own grid; explicit pixel
+// positions keep q = |L^T (p - mu)|^2 (quad_form).
+struct Plane {
+    float cx, cy, K, p, qq, s;
+};
+
+Plane plane_form(const float* mu, const float l[6], const Frame& fr) {
+    const float vx = mu[0] - fr.o[0], vy = mu[1] - fr.o[1], vz = mu[2] - fr.o[2];
+    const float lat_g = dot3(vx, vy, vz, fr.lat[0], fr.lat[1], fr.lat[2]);
+    const float ax_g = dot3(vx, vy, vz, fr.ax[0], fr.ax[1], fr.ax[2]);
+    const double L11 = l[0], L12 = l[1], L13 = l[2], L22 = l[3], L23 = l[4], L33 = l[5];
+    const float* a = fr.lat;
+    const float* c = fr.ax;
+    const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
+    const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
+    const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
+    const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
+    const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
+    const double lg = lat_g, ag = ax_g;
+    const double g0 = ((double)fr.o[0] - (double)mu[0]) + lg * a[0] + ag * c[0];
+    const double g1 = ((double)fr.o[1] - (double)mu[1]) + lg * a[1] + ag * c[1];
+    const double g2 = ((double)fr.o[2] - (double)mu[2]) + lg * a[2] + ag * c[2];
+    const double c0 = L11 * g0 + L12 * g1 + L13 * g2, c1 = L22 * g1 + L23 * g2, c2 = L33 * g2;
+    const double h0 = A0 * c0 + A1 * c1 + A2 * c2, h1 = B0 * c0 + B1 * c1 + B2 * c2;
+    const double cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const double det = M00 * M11 - M01 * M01;
+    double xs = 0.0, ys = 0.0, K = cc, pp = std::sqrt(std::max(M00, 0.0)), qq = 0.0, ss = 0.0;
+    if (M00 > 0.0 && det > 0.0) {
+        xs = -(M11 * h0 - M01 * h1) / det;
+        ys = -(M00 * h1 - M01 * h0) / det;
+        K = std::max(0.0, cc + h0 * xs + h1 * ys);
+        qq = M01 / pp;
+        ss = std::sqrt(det / M00);
+    } else if (M00 > 0.0) {
+        qq = M01 / pp;
+    }
+    return Plane{(float)(lg + xs), (float)(ag + ys), (float)K, (float)pp, (float)qq, (float)ss};
+}
+
+// The pixel grid of a frame rendered on its own grid: pixel index p -> probe coordinates, as the kernels form them.
+struct Grid {
+    int W;
+    float col_offset, dlat, near_mm, dax;
+};
+
+inline void grid_lat_ax(const Grid& gr, int64_t p, float& lat, float& ax) {
+    const int ix = (int)(p % gr.W), iy = (int)(p / gr.W);
+    lat = ((float)ix - gr.col_offset) * gr.dlat;
+    ax = gr.near_mm + (float)iy * gr.dax;
+}
+
+inline float plane_q(const Plane& P, float lat, float ax) {
+    const float x = lat - P.cx, y = ax - P.cy;
+    const float u = std::fmaf(P.p, x, P.qq * y);
+    const float v = P.s * y;
+    return std::fmaf(u, u, std::fmaf(v, v, P.K));
+}
+
 }  // namespace
 
 extern "C" {
+
+// The plane form (cx, cy, K, p, qq, s) of every Gaussian for one frame (out[6 G]).
+int ug_oracle_plane_form(int G, const float* means, const float* tril, const float* origin, const float* a_lat,
+                         const float* a_ax, const float* a_ele, float beta, float* out) {
+    const Frame fr = make_frame(origin, a_lat, a_ax, a_ele);
+    for (int g = 0; g < G; ++g) {
+        float l[6];
+        tril_factor<float>(tril + 6 * (int64_t)g, beta, l);
+        const Plane P = plane_form(means + 3 * (int64_t)g, l, fr);
+        const float v[6] = {P.cx, P.cy, P.K, P.p, P.qq, P.s};
+        std::memcpy(out + 6 * (int64_t)g, v, sizeof(v));
+    }
+    return 0;
+}
 
 // kernel_project_gauss_and_compute_tile_rects (core.cu:46-116) for every Gaussian of one frame.
 // Outputs count[G], rect[G*4] (min_tx, min_ty, max_tx, max_ty; zeros when culled), dk[G].
@@ -245,23 +321,33 @@ int ug_oracle_pixel_lists(int G, const float* means, const float* tril, const fl
 namespace {
 
 // Forward (forward.cu:17-76) for one frame over per-pixel candidate lists.
+// With `plane` (one Plane per Gaussian) the frame is rendered on its own grid `grid`: the q <= chi2 decisions are
+// the plane form's, and so is the FP32 value of q; the FP64 value is |L^T (p - mu)|^2 from the given pixels.
 template <typename T>
 void forward_impl(int G, const T* means, const T* tril, const T* opac, const T* inten,
                   const T* pixels, int64_t P, const float* origin, const float* a_ele, const int32_t* counts,
                   const int32_t* offsets, const int32_t* indices, float chi2, float beta, float alpha_min, float bg,
-                  float bgI, int excl, T* image, T* alpha, T* depth, T* sums) {
+                  float bgI, int excl, T* image, T* alpha, T* depth, T* sums, const Plane* plane, Grid grid) {
     (void)G;
     for (int64_t p = 0; p < P; ++p) {
         T S = 0, N = 0, D = 0;
         const T px = pixels[3 * p], py = pixels[3 * p + 1], pz = pixels[3 * p + 2];
+        float glat = 0.f, gax = 0.f;
+        if (plane) grid_lat_ax(grid, p, glat, gax);
         for (int k = 0; k < counts[p]; ++k) {
             const int g = indices[offsets[p] + k];
             T l[6];
             tril_factor<T>(tril + 6 * (int64_t)g, T(beta), l);
             const T mx = means[3 * g], my = means[3 * g + 1], mz = means[3 * g + 2];
             T y[3];
-            const T q = quad_form<T>(l, px - mx, py - my, pz - mz, y);
-            if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, mx, my, mz, chi2)) continue;
+            T q = quad_form<T>(l, px - mx, py - my, pz - mz, y);
+            if (plane) {
+                const float q32 = plane_q(plane[g], glat, gax);
+                if (q32 > chi2) continue;
+                if (sizeof(T) == sizeof(float)) q = T(q32);
+            } else if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, mx, my, mz, chi2)) {
+                continue;
+            }
             const T ah = T(opac[g]) * exp_(T(-0.5) * q);
             if (!(ah > T(alpha_min))) continue;
             const T e = dot3<T>(mx - T(origin[0]), my - T(origin[1]), mz - T(origin[2]), T(a_ele[0]), T(a_ele[1]), T(a_ele[2]));
@@ -292,7 +378,7 @@ void backward_impl(int G, const T* means, const T* tril, const T* opac, const T*
                    const T* pixels, int64_t P, const float* origin, const float* a_ele, const int32_t* counts,
                    const int32_t* offsets, const int32_t* indices, float chi2, float beta, float alpha_min, float bg,
                    float bgI, int gate, const float* gI, const float* gA, const float* gD, T* d_means, T* d_tril,
-                   T* d_opac, T* d_inten) {
+                   T* d_opac, T* d_inten, const Plane* plane, Grid grid) {
     std::fill(d_means, d_means + 3 * (int64_t)G, T(0));
     std::fill(d_tril, d_tril + 6 * (int64_t)G, T(0));
     std::fill(d_opac, d_opac + G, T(0));
@@ -300,6 +386,8 @@ void backward_impl(int G, const T* means, const T* tril, const T* opac, const T*
     const T ex = a_ele[0], ey = a_ele[1], ez = a_ele[2];
     for (int64_t p = 0; p < P; ++p) {
         const T px = pixels[3 * p], py = pixels[3 * p + 1], pz = pixels[3 * p + 2];
+        float glat = 0.f, gax = 0.f;
+        if (plane) grid_lat_ax(grid, p, glat, gax);
         T S = 0, N = 0, D = 0;
         auto weight = [&](int g, T& w, T& dx, T& dy, T& dz, T lam[9], T l[6]) -> bool {
             tril_factor<T>(tril + 6 * (int64_t)g, T(beta), l);
@@ -316,10 +404,15 @@ void backward_impl(int G, const T* means, const T* tril, const T* opac, const T*
             dy = py - T(means[3 * g + 1]);
             dz = pz - T(means[3 * g + 2]);
             T y[3];
-            const T q = quad_form<T>(l, dx, dy, dz, y);
-            if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, T(means[3 * g]), T(means[3 * g + 1]),
-                              T(means[3 * g + 2]), chi2))
+            T q = quad_form<T>(l, dx, dy, dz, y);
+            if (plane) {
+                const float q32 = plane_q(plane[g], glat, gax);
+                if (q32 > chi2) return false;
+                if (sizeof(T) == sizeof(float)) q = T(q32);
+            } else if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, T(means[3 * g]), T(means[3 * g + 1]),
+                                     T(means[3 * g + 2]), chi2)) {
                 return false;
+            }
             w = exp_(T(-0.5) * q);
             return true;
         };
@@ -389,18 +482,22 @@ extern "C" {
 int ug_oracle_forward_f32(int G, const float* means, const float* tril, const float* opac, const float* inten,
                           const float* pixels, int64_t P, const float* origin, const float* a_ele, const int32_t* counts,
                           const int32_t* offsets, const int32_t* indices, float chi2, float beta, float alpha_min,
-                          float bg, float bgI, int excl, float* image, float* alpha, float* depth, float* sums) {
+                          float bg, float bgI, int excl, float* image, float* alpha, float* depth, float* sums,
+                          const float* plane, int W, float col_offset, float dlat, float near_mm, float dax) {
     forward_impl<float>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
-                        alpha_min, bg, bgI, excl, image, alpha, depth, sums);
+                        alpha_min, bg, bgI, excl, image, alpha, depth, sums, reinterpret_cast<const Plane*>(plane),
+                        Grid{W, col_offset, dlat, near_mm, dax});
     return 0;
 }
 
 int ug_oracle_forward_f64(int G, const double* means, const double* tril, const double* opac, const double* inten,
                           const double* pixels, int64_t P, const float* origin, const float* a_ele, const int32_t* counts,
                           const int32_t* offsets, const int32_t* indices, float chi2, float beta, float alpha_min,
-                          float bg, float bgI, int excl, double* image, double* alpha, double* depth, double* sums) {
+                          float bg, float bgI, int excl, double* image, double* alpha, double* depth, double* sums,
+                          const float* plane, int W, float col_offset, float dlat, float near_mm, float dax) {
     forward_impl<double>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
-                         alpha_min, bg, bgI, excl, image, alpha, depth, sums);
+                         alpha_min, bg, bgI, excl, image, alpha, depth, sums, reinterpret_cast<const Plane*>(plane),
+                         Grid{W, col_offset, dlat, near_mm, dax});
     return 0;
 }
 
@@ -408,9 +505,11 @@ int ug_oracle_backward_f32(int G, const float* means, const float* tril, const f
                            const float* pixels, int64_t P, const float* origin, const float* a_ele,
                            const int32_t* counts, const int32_t* offsets, const int32_t* indices, float chi2,
                            float beta, float alpha_min, float bg, float bgI, int gate, const float* gI, const float* gA,
-                           const float* gD, float* d_means, float* d_tril, float* d_opac, float* d_inten) {
+                           const float* gD, float* d_means, float* d_tril, float* d_opac, float* d_inten,
+                           const float* plane, int W, float col_offset, float dlat, float near_mm, float dax) {
     backward_impl<float>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
-                         alpha_min, bg, bgI, gate, gI, gA, gD, d_means, d_tril, d_opac, d_inten);
+                         alpha_min, bg, bgI, gate, gI, gA, gD, d_means, d_tril, d_opac, d_inten,
+                         reinterpret_cast<const Plane*>(plane), Grid{W, col_offset, dlat, near_mm, dax});
     return 0;
 }
 
@@ -418,9 +517,11 @@ int ug_oracle_backward_f64(int G, const double* means, const double* tril, const
                            const double* pixels, int64_t P, const float* origin, const float* a_ele,
                            const int32_t* counts, const int32_t* offsets, const int32_t* indices, float chi2,
                            float beta, float alpha_min, float bg, float bgI, int gate, const float* gI, const float* gA,
-                           const float* gD, double* d_means, double* d_tril, double* d_opac, double* d_inten) {
+                           const float* gD, double* d_means, double* d_tril, double* d_opac, double* d_inten,
+                           const float* plane, int W, float col_offset, float dlat, float near_mm, float dax) {
     backward_impl<double>(G, means, tril, opac, inten, pixels, P, origin, a_ele, counts, offsets, indices, chi2, beta,
-                          alpha_min, bg, bgI, gate, gI, gA, gD, d_means, d_tril, d_opac, d_inten);
+                          alpha_min, bg, bgI, gate, gI, gA, gD, d_means, d_tril, d_opac, d_inten,
+                          reinterpret_cast<const Plane*>(plane), Grid{W, col_offset, dlat, near_mm, dax});
     return 0;
 }
 

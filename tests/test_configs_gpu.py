@@ -99,32 +99,39 @@ def test_config4_captured_step_equals_eager_and_sampled_frames_match_the_oracle(
         gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
                            frames, cfg, sweeps=sweeps, hint=CapacityHint())[0].clone()
     del target
+    torch.cuda.empty_cache()
     logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
-    eager = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False, lr=1e-3)
-    eager.step()
-    torch.cuda.synchronize()
-    # the first step's images are the render of the initial parameters; its packed records hold the activated
-    # opacity and intensity the kernels used
-    image0 = eager.image.detach().clone()
-    off = eager.batch.layout_offsets(eager.k_capacity)
-    rec = eager.workspace[off.records: off.records + 48 * G].view(torch.float32).view(G, 12).cpu()
-    o_pk, c_pk = rec[:, 3].numpy(), rec[:, 4].numpy()
-    graph = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=True, lr=1e-3)
-    graph.step()
-    torch.cuda.synchronize()
-    for x, y in zip(eager.state(), graph.state()):
-        assert torch.equal(x, y)
-    for _ in range(2):
+    # eager steps first (states kept on the host), then the captured steps: one workspace on the device at a time
+    eager = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False, lr=1e-3, check_every=10 ** 9)
+    states, losses = [], []
+    for step in range(3):
         eager.step()
+        torch.cuda.synchronize()
+        if step == 0:
+            # the first step's images render the initial parameters; its packed records hold the activated
+            # opacity and intensity the kernels used
+            image0 = eager.image.detach().cpu()
+            off = eager.batch.layout_offsets(eager.k_capacity)
+            rec = eager.workspace[off.records: off.records + 48 * G].view(torch.float32).view(G, 12).cpu()
+            o_pk, c_pk = rec[:, 3].numpy(), rec[:, 4].numpy()
+        states.append([t.cpu() for t in eager.state()])
+        losses.append(eager.losses().cpu())
+    assert not eager.overflowed()
+    base = eager.batch.layout.frame_pixel_base
+    ws_gib = eager.workspace.numel() / 2 ** 30
+    del eager
+    torch.cuda.empty_cache()
+    graph = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=True, lr=1e-3, check_every=10 ** 9)
+    for step in range(3):
         graph.step()
         torch.cuda.synchronize()
-        for x, y in zip(eager.state(), graph.state()):
-            assert torch.equal(x, y)
-        assert torch.equal(eager.losses(), graph.losses())
-    assert not eager.overflowed() and not graph.overflowed()
-    base = eager.batch.layout.frame_pixel_base
-    del eager, graph
+        for x, y in zip(states[step], graph.state()):
+            assert torch.equal(x, y.cpu())
+        assert torch.equal(losses[step], graph.losses().cpu())
+    assert not graph.overflowed()
+    print(f"[config4] 3 graph replays equal 3 eager steps bit for bit; workspace {ws_gib:.1f} GiB", flush=True)
+    del graph
     torch.cuda.empty_cache()
 
     sem = oug.Semantics()
@@ -138,7 +145,9 @@ def test_config4_captured_step_equals_eager_and_sampled_frames_match_the_oracle(
         pix = linear_pixels(fr).numpy()
         mr = oug.map_frame(means[g0:g1], tril[g0:g1], geom)
         lists = oug.pixel_lists(means[g0:g1], tril[g0:g1], pix, geom, mr)
-        img, _a, _d, _s = oug.forward(means[g0:g1], tril[g0:g1], o_pk[g0:g1], c_pk[g0:g1], pix, geom, lists, sem)
+        plane = oug.plane_form(means[g0:g1], tril[g0:g1], geom)  # the trainer renders the frames' own grids
+        img, _a, _d, _s = oug.forward(means[g0:g1], tril[g0:g1], o_pk[g0:g1], c_pk[g0:g1], pix, geom, lists, sem,
+                                      plane=plane)
         got = image0[base[k]:base[k + 1]].cpu().numpy()
         err = rel_l2(got, img)
         print(f"[config4] sweep {s}: frame {k} ({fr.width}x{fr.height}), {sw.g_count} Gaussians, "

@@ -371,3 +371,72 @@ def test_oracle_quaternion_matrix_is_the_rodrigues_rotation():
         _img, aux = our.forward(params, geo, rm, st)
         z = (pts - mu) @ R / s
         np.testing.assert_allclose(aux[:, 2], np.exp(-0.5 * (z * z).sum(1)), rtol=2e-5, atol=1e-12)
+
+
+# ---------------------------------------------------------------- plane form of q (frames on their own grids)
+
+
+def test_plane_form_is_the_mahalanobis_distance_on_the_plane():
+    """The oracle's plane form (cx, cy, K, p, qq, s) restates the kernels' reparametrisation of q on a frame's pixel
+    plane. Pinned by closed forms of a Gaussian restricted to a plane with unit normal n = a_ele: the minimum of the
+    Mahalanobis distance over the plane is K = ele^2 / (n^T Sigma n) (ele = (mu - o).n), attained at the conditional
+    point x* = mu - Sigma n ele / (n^T Sigma n), so (cx, cy) = ((x* - o).a_lat, (x* - o).a_ax); and at any point of
+    the plane q equals (x - mu)^T Lambda (x - mu) computed directly in FP64."""
+    rng = np.random.default_rng(17)
+    scene = make_scene(300, 30.0, seed=19)
+    fr = straight_sweep(3, 64, 64, 0.3, 30.0, 10.0, 8.0, seed=20)[1]
+    geom = oug.FrameGeom.from_spec(fr)
+    means, tril = scene.means.numpy(), scene.tril.numpy()
+    plane = oug.plane_form(means, tril, geom, beta=1e-4)
+    o, al, aa, ae = (np.asarray(v, np.float64) for v in (geom.origin, geom.axis_lat, geom.axis_ax, geom.axis_ele))
+    t = tril.astype(np.float64)
+    L = np.zeros((len(means), 3, 3))
+    L[:, 0, 0], L[:, 1, 0], L[:, 2, 0] = t[:, 0] ** 2 + 1e-4, t[:, 1], t[:, 2]
+    L[:, 1, 1], L[:, 2, 1], L[:, 2, 2] = t[:, 3] ** 2 + 1e-4, t[:, 4], t[:, 5] ** 2 + 1e-4
+    lam = L @ L.transpose(0, 2, 1)
+    sig = np.linalg.inv(lam)
+    mu = means.astype(np.float64)
+    ele = (mu - o) @ ae
+    sn = np.einsum("i,gij,j->g", ae, sig, ae)
+    K = ele ** 2 / sn
+    xs = mu - (sig @ ae) * (ele / sn)[:, None]
+    near = np.abs(ele) < 3.0 * np.sqrt(sn)  # the visible pairs: the plane crosses the Gaussian
+    assert near.sum() > 20
+    np.testing.assert_allclose(plane[near, 2], K[near], rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(plane[near, 0], ((xs - o) @ al)[near], rtol=0, atol=2e-4)
+    np.testing.assert_allclose(plane[near, 1], ((xs - o) @ aa)[near], rtol=0, atol=2e-4)
+    # q at random plane points: the plane form (FP32 fixed order) against the direct quadratic form in FP64
+    for g in np.nonzero(near)[0][:40]:
+        cx, cy, Kg, p, qq, s = (float(v) for v in plane[g])
+        for _ in range(5):
+            lat = float(np.float32(cx + rng.normal() * 2.0))
+            ax = float(np.float32(cy + rng.normal() * 2.0))
+            x = o + lat * al + ax * aa - mu[g]
+            q_ref = float(x @ lam[g] @ x)
+            xf, yf = np.float32(lat) - np.float32(cx), np.float32(ax) - np.float32(cy)
+            u = np.float32(p) * xf + np.float32(qq) * yf
+            v = np.float32(s) * yf
+            q = float(u * u + v * v + np.float32(Kg))
+            assert q == pytest.approx(q_ref, rel=2e-5, abs=2e-5)
+
+
+def test_plane_form_decisions_follow_the_grid_and_change_only_boundary_pairs():
+    """Rendered on its own grid, a frame's FP32 oracle image (plane-form decisions and values) agrees with its FP64
+    evaluation on the same pair set within 1e-6, and with the explicit-pixel semantics (q from the world positions)
+    up to the few pairs whose q lies within rounding of chi2."""
+    fr = straight_sweep(2, 64, 64, 0.3, 40.0, 10.0, 5.0, seed=3)[1]
+    sc = make_scene(2000, 40.0, seed=4)
+    geom = oug.FrameGeom.from_spec(fr)
+    pix = linear_pixels(fr).numpy()
+    m, t = sc.means.numpy(), sc.tril.numpy()
+    mr = oug.map_frame(m, t, geom)
+    lists = oug.pixel_lists(m, t, pix, geom, mr)
+    plane = oug.plane_form(m, t, geom)
+    sem = oug.Semantics()
+    args = (m, t, sc.opacity.numpy(), sc.intensity.numpy(), pix, geom, lists, sem)
+    img_p, _, _, _ = oug.forward(*args, plane=plane)
+    img_p64, _, _, _ = oug.forward(*args, fp64=True, plane=plane)
+    img_e, _, _, _ = oug.forward(*args)
+    assert np.abs(img_p).max() > 0.05
+    assert rel_l2(img_p, img_p64) < 1e-6
+    assert rel_l2(img_p, img_e) < 1e-4

@@ -98,28 +98,70 @@ def test_fallback_semantics_vs_torch_fallback(dev):
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
 
 
-def test_batch_equals_per_frame_and_procedural_pixels(dev):
+def test_batch_equals_per_frame_for_explicit_and_grid_pixels(dev):
+    """Batched equals per frame bit for bit, with explicit pixel positions and with the frames' own grids."""
     scene, frames = small_case(n_frames=4)
     cfg = cfg_for(frames[0])
     rng = np.random.default_rng(2)
     P = sum(f.height * f.width for f in frames)
     dI = rng.normal(size=P).astype(np.float32)
-    batched = gpu_render(scene, frames, cfg, dev, grads=(dI, None, None))
-    proc = gpu_render(scene, frames, cfg, dev, pixels="procedural", grads=(dI, None, None))
-    np.testing.assert_array_equal(batched["image"].detach().cpu().numpy(), proc["image"].detach().cpu().numpy())
-    for a, b in zip(batched["grads"], proc["grads"]):
-        np.testing.assert_array_equal(a, b)
-    off = 0
-    acc = [np.zeros_like(g) for g in batched["grads"]]
-    for fr in frames:
-        n = fr.height * fr.width
-        single = gpu_render(scene, [fr], cfg, dev, grads=(dI[off:off + n], None, None))
-        np.testing.assert_array_equal(single["image"].detach().cpu().numpy(),
-                                      batched["image"].detach().cpu().numpy()[off:off + n])
-        acc = [a + g for a, g in zip(acc, single["grads"])]
-        off += n
-    for a, b in zip(acc, batched["grads"]):
-        assert rel_l2(b, a) < 1e-6
+    for pixels in ("explicit", "procedural"):
+        batched = gpu_render(scene, frames, cfg, dev, pixels=pixels, grads=(dI, None, None))
+        off = 0
+        acc = [np.zeros_like(g) for g in batched["grads"]]
+        for fr in frames:
+            n = fr.height * fr.width
+            single = gpu_render(scene, [fr], cfg, dev, pixels=pixels, grads=(dI[off:off + n], None, None))
+            np.testing.assert_array_equal(single["image"].detach().cpu().numpy(),
+                                          batched["image"].detach().cpu().numpy()[off:off + n])
+            acc = [a + g for a, g in zip(acc, single["grads"])]
+            off += n
+        for a, b in zip(acc, batched["grads"]):
+            assert rel_l2(b, a) < 1e-6
+
+
+@pytest.mark.parametrize("ts", [16, 8, 32])
+def test_grid_pixels_plane_form_parity(dev, ts):
+    """Frames rendered on their own grids (pixels = NULL) evaluate q in the plane form of each pair. The plane
+    coefficients the preprocess leaves in the workspace equal the oracle's bit for bit for every visible pair; the
+    image, alpha and depth match the oracle's plane-form forward within 1e-5 (FP32 and FP64), and the gradients
+    its FP64 adjoint within 1e-4, with and without alpha/depth gradients."""
+    scene, frames = small_case(n_frames=2)
+    rng = np.random.default_rng(15)
+    sem = oug.Semantics()
+    G = scene.means.shape[0]
+    for k, fr in enumerate(frames):
+        P = fr.height * fr.width
+        dI = rng.normal(size=P).astype(np.float32)
+        dA = rng.normal(size=P).astype(np.float32) * 0.1
+        dD = rng.normal(size=P).astype(np.float32) * 0.01
+        full = k == 1
+        cfg = cfg_for(fr, tile_size=ts)
+        out = gpu_render(scene, [fr], cfg, dev, pixels="procedural", grads=(dI, dA if full else None,
+                                                                             dD if full else None))
+        b = out["batch"]
+        m = b.map(*[t.to(dev) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)])
+        geom, pix, mr, lists = oracle_frame(scene, fr, sem, ts=ts)
+        plane = oug.plane_form(scene.means.numpy(), scene.tril.numpy(), geom)
+        off = b.layout_offsets(m.k_capacity)
+        sect = m.workspace[off.pair_sect: off.pair_sect + 48 * G].view(torch.float32).view(G, 12).cpu().numpy()
+        vis = mr.count > 0
+        assert vis.sum() > 100
+        got_plane = np.stack([sect[:, 8], sect[:, 9], sect[:, 10], sect[:, 4], sect[:, 5], sect[:, 6]], 1)
+        np.testing.assert_array_equal(got_plane[vis].view(np.uint32), plane[vis].view(np.uint32))
+        args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom,
+                lists, sem)
+        img32, a32, d32, _ = oug.forward(*args, plane=plane)
+        img64, a64, d64, _ = oug.forward(*args, fp64=True, plane=plane)
+        got = {key: out[key].detach().cpu().numpy() for key in ("image", "alpha", "depth")}
+        assert np.abs(img32).max() > 0.05
+        for key, r32, r64 in (("image", img32, img64), ("alpha", a32, a64), ("depth", d32, d64)):
+            assert rel_l2(got[key], r32) < IMG_TOL, key
+            assert rel_l2(got[key], r64) < IMG_TOL, key
+        ref = oug.backward_f64(*args, dI, dA if full else None, dD if full else None, gate=True, plane=plane)
+        for name, g, r in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref):
+            err = rel_l2(g, r)
+            assert err < GRAD_TOL, f"ts {ts} full {full} {name}: rel L2 {err:.3e}"
 
 
 def test_bitwise_deterministic(dev):
@@ -182,17 +224,18 @@ def test_config2_frame_parity(dev):
     out = gpu_render(scene, [fr], cfg_for(fr), dev, pixels="procedural", grads=(dI, None, None))
     sem = oug.Semantics()
     geom, pix, mr, lists = oracle_frame(scene, fr, sem)
+    plane = oug.plane_form(scene.means.numpy(), scene.tril.numpy(), geom)  # rendered on the frame's own grid
     args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom, lists,
             sem)
-    img, _a, _d, _s = oug.forward(*args)
-    img64, _a, _d, _s = oug.forward(*args, fp64=True)
+    img, _a, _d, _s = oug.forward(*args, plane=plane)
+    img64, _a, _d, _s = oug.forward(*args, fp64=True, plane=plane)
     assert mr.keys.size > 100_000
     assert rel_l2(out["image"].detach().cpu().numpy(), img) < IMG_TOL
     print(f"image rel L2: vs FP32 oracle {rel_l2(out['image'].detach().cpu().numpy(), img):.2e}, "
           f"vs FP64 {rel_l2(out['image'].detach().cpu().numpy(), img64):.2e}")
-    ref = oug.backward_f64(*args, dI, None, None, gate=True)
+    ref = oug.backward_f64(*args, dI, None, None, gate=True, plane=plane)
     # the reference's own precision: its backward restated in FP32 (naive per-pair accumulation)
-    ref32 = oug.backward_f64(*args, dI, None, None, gate=True, fp32=True)
+    ref32 = oug.backward_f64(*args, dI, None, None, gate=True, fp32=True, plane=plane)
     for name, g, r, r32 in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref, ref32):
         err = rel_l2(g, r)
         print(f"{name}: HIP vs FP64 {err:.2e}; FP32 reference restatement vs FP64 {rel_l2(r32, r):.2e}")

@@ -93,6 +93,21 @@ __device__ __forceinline__ void pixel_position(const us3d_frame& fr, const float
     }
 }
 
+// XCD-aware work order. Workgroups are dispatched to the 8 XCDs round-robin (workgroup b to XCD b % 8), and each
+// XCD has its own L2, while neighbouring tiles of a frame share most of their Gaussians. Workgroup b therefore
+// takes tile remap_xcd(b): within each super-block of 8 * kXcdRun tiles, XCD r walks the run of kXcdRun
+// consecutive tiles [r kXcdRun, (r + 1) kXcdRun), so the records a tile gathers are in its XCD's L2 from the tiles
+// before it, and the runs stay short enough that every XCD sees the whole batch (work per tile varies along a
+// sweep). Tiles past the last whole super-block keep their own index: a bijection of [0, n).
+constexpr int kXcds = 8;
+constexpr int kXcdRun = 32;
+__device__ __forceinline__ int remap_xcd(int b, int n) {
+    constexpr int kSuper = kXcds * kXcdRun;
+    if (b >= (n / kSuper) * kSuper) return b;
+    const int r = b % kXcds, q = b / kXcds;
+    return (q / kXcdRun) * kSuper + r * kXcdRun + (q % kXcdRun);
+}
+
 // Binary search: largest i in [0, n) with base[i] <= v (base ascending, base[0] <= v).
 __device__ __forceinline__ int upper_index(const int32_t* __restrict__ base, int n, int v) {
     int lo = 0, hi = n - 1;

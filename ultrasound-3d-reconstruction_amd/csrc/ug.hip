@@ -50,7 +50,7 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
 }
 
 static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2, 32);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2, 48);
 }
 
 // ---------------------------------------------------------------- mapper kernels
@@ -158,10 +158,55 @@ __device__ __forceinline__ float sub_pad(float lat_g, float ax_g) {
     return 1e-3f + 4e-6f * (fabsf(lat_g) + fabsf(ax_g));
 }
 
+// Plane form of q for the linear probe's pixel grid (every frame geometry with an affine grid, whatever the axes):
+// the pixel at probe coordinates (l, a) sits at o + l a_lat + a a_ax, so with (l, a) = (lat_g + x, ax_g + y),
+// L^T (p - mu) = c0 + x A + y B with A = L^T a_lat, B = L^T a_ax, c0 = L^T g0, g0 = (o - mu) + lat_g a_lat +
+// ax_g a_ax, and q = |c0 + x A + y B|^2, a 2-D quadratic. Written about its minimum (x*, y*) with value K:
+//   q = (p (l - cx) + qq (a - cy))^2 + (s (a - cy))^2 + K,   cx = lat_g + x*, cy = ax_g + y*,
+// p = sqrt(M00), qq = M01 / p, s = sqrt(det M / M00), M = [[A.A, A.B], [A.B, B.B]] -- a sum of squares, so the
+// evaluation per pixel is two subtractions and five multiply-adds without cancellation. Set up in FP64 per
+// visible pair (the preprocess), evaluated in FP32 per (pixel, instance) by the splat kernels. With det M <= 0
+// (only for parallel axes or a singular factor) the minimum is not unique: x* = y* = 0, K = |c0|^2, s = 0.
+// Returns s1 = (p, qq, s, t_y = M01 / M00) and s2 = (cx, cy, K, 0).
+__device__ __forceinline__ void pair_plane_form(double L11, double L12, double L13, double L22, double L23,
+                                                double L33, float mx, float my, float mz, float lat_g, float ax_g,
+                                                const us3d_frame& fr, double& M00, double& M01, double& M11,
+                                                float4& s1, float4& s2) {
+    const float* a = fr.axis_lat;
+    const float* c = fr.axis_ax;
+    const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
+    const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
+    M00 = A0 * A0 + A1 * A1 + A2 * A2;
+    M01 = A0 * B0 + A1 * B1 + A2 * B2;
+    M11 = B0 * B0 + B1 * B1 + B2 * B2;
+    const double lg = lat_g, ag = ax_g;
+    const double g0 = ((double)fr.origin[0] - (double)mx) + lg * a[0] + ag * c[0];
+    const double g1 = ((double)fr.origin[1] - (double)my) + lg * a[1] + ag * c[1];
+    const double g2 = ((double)fr.origin[2] - (double)mz) + lg * a[2] + ag * c[2];
+    const double c0 = L11 * g0 + L12 * g1 + L13 * g2, c1 = L22 * g1 + L23 * g2, c2 = L33 * g2;
+    const double h0 = A0 * c0 + A1 * c1 + A2 * c2, h1 = B0 * c0 + B1 * c1 + B2 * c2;
+    const double cc = c0 * c0 + c1 * c1 + c2 * c2;
+    const double det = M00 * M11 - M01 * M01;
+    double xs = 0.0, ys = 0.0, K = cc, pp = sqrt(fmax(M00, 0.0)), qq = 0.0, ss = 0.0;
+    if (M00 > 0.0 && det > 0.0) {
+        xs = -(M11 * h0 - M01 * h1) / det;
+        ys = -(M00 * h1 - M01 * h0) / det;
+        K = fmax(0.0, cc + h0 * xs + h1 * ys);
+        qq = M01 / pp;
+        ss = sqrt(det / M00);
+    } else if (M00 > 0.0) {
+        qq = M01 / pp;
+    }
+    s1 = make_float4((float)pp, (float)qq, (float)ss, M00 > 0.0 ? (float)(M01 / M00) : 0.f);
+    s2 = make_float4((float)(lg + xs), (float)(ag + ys), (float)K, 0.f);
+}
+
 __device__ __forceinline__ void pair_section(float l11, float l12, float l13, float l22, float l23, float l33,
-                                             float lat_g, float ax_g, float r_lat, float r_ax,
-                                             const us3d_frame& fr, float4& s0, float4& s1) {
-    s1 = make_float4(0.f, 0.f, 0.f, 0.f);
+                                             float mx, float my, float mz, float lat_g, float ax_g, float r_lat,
+                                             float r_ax, const us3d_frame& fr, float4& s0, float4& s1, float4& s2) {
+    const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
+    double M00, M01, M11;
+    pair_plane_form(L11, L12, L13, L22, L23, L33, mx, my, mz, lat_g, ax_g, fr, M00, M01, M11, s1, s2);
     if (!(l11 > 1e-10f && l22 > 1e-10f && l33 > 1e-10f)) {
         s0 = make_float4(INFINITY, INFINITY, -1.f, 0.f);
         return;
@@ -173,13 +218,9 @@ __device__ __forceinline__ void pair_section(float l11, float l12, float l13, fl
     const float* a = fr.axis_lat;
     const float* c = fr.axis_ax;
     const float* e = fr.axis_ele;
-    const double L11 = l11, L12 = l12, L13 = l13, L22 = l22, L23 = l23, L33 = l33;
     const double A0 = a[0] * L11 + a[1] * L12 + a[2] * L13, A1 = a[1] * L22 + a[2] * L23, A2 = a[2] * L33;
     const double B0 = c[0] * L11 + c[1] * L12 + c[2] * L13, B1 = c[1] * L22 + c[2] * L23, B2 = c[2] * L33;
     const double C0 = e[0] * L11 + e[1] * L12 + e[2] * L13, C1 = e[1] * L22 + e[2] * L23, C2 = e[2] * L33;
-    const double M00 = A0 * A0 + A1 * A1 + A2 * A2;
-    const double M01 = A0 * B0 + A1 * B1 + A2 * B2;
-    const double M11 = B0 * B0 + B1 * B1 + B2 * B2;
     const double b0 = A0 * C0 + A1 * C1 + A2 * C2;
     const double b1 = B0 * C0 + B1 * C1 + B2 * C2;
     const double cz = C0 * C0 + C1 * C1 + C2 * C2;
@@ -187,9 +228,7 @@ __device__ __forceinline__ void pair_section(float l11, float l12, float l13, fl
     if (!(det > 1e-8 * M00 * M11)) return;
     const double w0 = (M11 * b0 - M01 * b1) / det, w1 = (M00 * b1 - M01 * b0) / det;
     const double kk = fmax(0.0, (cz - (b0 * w0 + b1 * w1)) * (1.0 - 1e-5));
-    const double pp = sqrt(M00);
     s0 = make_float4((float)(-w0), (float)(-w1), (float)kk, (float)(M01 / M11));
-    s1 = make_float4((float)pp, (float)(M01 / pp), (float)sqrt(det / M00), (float)(M01 / M00));
 }
 
 __global__ __launch_bounds__(256) void ug_preprocess_kernel(
@@ -243,10 +282,12 @@ __global__ __launch_bounds__(256) void ug_preprocess_kernel(
                     count = (x1 - x0) * (y1 - y0);
                     rect = make_int4(x0, y0, x1, y1);
                     dk = sort_by_depth ? depth_key(v) : 0u;
-                    float4 s0, s1;
-                    pair_section(l11, l12, l13, l22, l23, l33, lat, ax, r_lat, r_ax, fr, s0, s1);
-                    pair_sect[2 * pair + 0] = s0;
-                    pair_sect[2 * pair + 1] = s1;
+                    float4 s0, s1, s2;
+                    pair_section(l11, l12, l13, l22, l23, l33, r0.x, r0.y, r0.z, lat, ax, r_lat, r_ax, fr, s0, s1,
+                                 s2);
+                    pair_sect[3 * pair + 0] = s0;
+                    pair_sect[3 * pair + 1] = s1;
+                    pair_sect[3 * pair + 2] = s2;
                 }
             }
         }
@@ -508,40 +549,50 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // adjoint's coefficient dL/dalpha_hat = dN (c - image) depends on the image to far better than FP32-sum
 // accuracy wherever c is close to it.
 
-// kDepth: the depth output is requested (out_depth != NULL); without it the depth numerator is not accumulated
-// and aux holds 0 in its place.
-template <int TS, bool kDepth>
-__global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
-void ug_forward_kernel(
-    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
-    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
+// Plane form of q (pair_plane_form) at a pixel with probe coordinates (lat, ax): two subtractions and five
+// multiply-adds in one fixed order, shared by the forward and the adjoint so that their q <= chi2 decisions are
+// the same bits. pa = (cx, cy, K, .), pb = (p, qq, s, .).
+__device__ __forceinline__ float plane_q(float lat, float ax, float cx, float cy, float K, float p, float qq, float s) {
+    const float x = lat - cx, y = ax - cy;
+    const float u = __builtin_fmaf(p, x, qq * y);
+    const float v = s * y;
+    return __builtin_fmaf(u, u, __builtin_fmaf(v, v, K));
+}
+
+// Probe coordinates (lat, ax) of pixel (ix, iy) of a linear-probe frame, with the FP32 operations of
+// pixel_position (so the plane form sees the lateral and axial values the world position is built from).
+__device__ __forceinline__ void grid_lat_ax(const us3d_frame& fr, int ix, int iy, float& lat, float& ax) {
+    lat = ((float)ix - fr.col_offset) * fr.delta_lat;
+    ax = fr.near_mm + (float)iy * fr.delta_ax;
+}
+
+// LDS of one forward workgroup.
+template <int TS>
+struct FwdLds {
+    float4 r0[FwdShape<TS>::kThreads], r1[FwdShape<TS>::kThreads], r2[FwdShape<TS>::kThreads];
+    uint32_t mask[FwdShape<TS>::kThreads];
+    SubBounds sb[FwdShape<TS>::kSub];
+};
+
+// The forward of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_forward_kernel).
+template <int TS, bool kDepth, bool kPlaneBody>
+__device__ __forceinline__ void ug_forward_tile(
+    FwdLds<TS>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key, const us3d_frame& fr,
+    const us3d_sweep& sw, int f, int tx, int ty, int s0, int s1, const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
-    float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
-    float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
-    uint16_t* __restrict__ inst_mask) {
+    float bg_intensity, int alpha_excludes_bg, float* __restrict__ out_image, float* __restrict__ out_alpha,
+    float* __restrict__ out_depth, float4* __restrict__ out_aux, uint16_t* __restrict__ inst_mask) {
     using S = FwdShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
-    __shared__ float4 s_r0[NT], s_r1[NT], s_r2[NT];
-    __shared__ uint32_t s_mask[NT];
-    __shared__ SubBounds s_sb[S::kSub];
-    if (stats->overflow) return;
-
-    const int t = blockIdx.x;
-    const int f = frame_of_tile(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    constexpr int NC = kPlaneBody ? 2 : 3;  // per-pixel coordinates held: (lat, ax) or the world position
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
-    const us3d_sweep& sw = sweeps[fr.sweep];
     // pair index of Gaussian g in this frame: pair_g0 + g * f_count
     const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const float chi2w = sqrt_chi2 * sqrt_chi2 * 1.002f;
 
-    float px[PPT], py[PPT], pz[PPT];
+    float pc[NC][PPT];
     bool valid[PPT];
     int pidx[PPT];
     double S_[PPT], N_[PPT], D_[PPT];
@@ -552,17 +603,23 @@ void ug_forward_kernel(
         sub_pixel<TS>(tx, ty, sb, lane, ix, iy);
         valid[k] = ix < fr.width && iy < fr.height;
         pidx[k] = fr.pixel_base + iy * fr.width + ix;
-        px[k] = py[k] = pz[k] = 0.f;
-        if (valid[k]) pixel_position(fr, pixels, ix, iy, px[k], py[k], pz[k]);
-        const float dx = px[k] - ox, dy = py[k] - oy, dz = pz[k] - oz;
+        float x = 0.f, y = 0.f, z = 0.f;
+        if (valid[k]) pixel_position(fr, pixels, ix, iy, x, y, z);
+        const float dx = x - ox, dy = y - oy, dz = z - oz;
         const float lat = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
         const float ax = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
         const float ele = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
         const SubBounds b = sub_bounds(valid[k], lat, ax, ele);
-        if (lane == 0) s_sb[sb] = b;
+        if (lane == 0) L.sb[sb] = b;
+        if constexpr (kPlaneBody) {
+            grid_lat_ax(fr, ix, iy, pc[0][k], pc[1][k]);
+        } else {
+            pc[0][k] = x;
+            pc[1][k] = y;
+            pc[NC - 1][k] = z;
+        }
+        S_[k] = N_[k] = D_[k] = 0.0;
     }
-#pragma unroll
-    for (int k = 0; k < PPT; ++k) S_[k] = N_[k] = D_[k] = 0.0;
 
     for (int base = s0; base < s1; base += NT) {
         __syncthreads();
@@ -577,13 +634,20 @@ void ug_forward_kernel(
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
             const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
-            const uint32_t m = sub_mask<S::kSub>(s_sb, pair_sect[2 * pair + 0], pair_sect[2 * pair + 1], lat_g, ax_g, e,
-                                                 chi2w);
-            s_mask[tid] = m;
+            const float4 ps0 = pair_sect[3 * pair + 0], ps1 = pair_sect[3 * pair + 1];
+            const uint32_t m = sub_mask<S::kSub>(L.sb, ps0, ps1, lat_g, ax_g, e, chi2w);
+            L.mask[tid] = m;
             inst_mask[base + tid] = (uint16_t)m;  // the adjoint reads it instead of computing it again
-            s_r0[tid] = r0;                                  // mu, opacity
-            s_r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
-            s_r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
+            if constexpr (kPlaneBody) {
+                const float4 ps2 = pair_sect[3 * pair + 2];
+                L.r0[tid] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
+                L.r1[tid] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
+                L.r2[tid] = make_float4(e, 0.f, 0.f, 0.f);
+            } else {
+                L.r0[tid] = r0;                                  // mu, opacity
+                L.r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
+                L.r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
+            }
         }
         __syncthreads();
         // The Gaussians that touch this wave's sub-blocks, 64 list entries at a time: ballots of their mask
@@ -591,7 +655,7 @@ void ug_forward_kernel(
         // record is read from LDS once and evaluated for every sub-block whose bit is set. Skipped entries
         // cost nothing and each pixel still sums its Gaussians in list order.
         for (int jb = 0; jb < n; jb += kWave) {
-            const uint32_t mm = (jb + lane < n) ? s_mask[jb + lane] : 0u;
+            const uint32_t mm = (jb + lane < n) ? L.mask[jb + lane] : 0u;
             uint64_t bk[PPT];
             uint64_t any = 0;
 #pragma unroll
@@ -603,22 +667,28 @@ void ug_forward_kernel(
                 const int bit = (int)__builtin_ctzll(any);
                 any &= any - 1;
                 const int j = jb + bit;
-                const float4 a = s_r0[j];
-                const float4 b = s_r1[j];
-                const float4 c = s_r2[j];
+                const float4 a = L.r0[j];
+                const float4 b = L.r1[j];
+                const float4 c = L.r2[j];
+                const double cd = (double)(kPlaneBody ? b.w : b.x);
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
-                    float y0, y1, y2;
-                    const float q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, px[k] - a.x, py[k] - a.y, pz[k] - a.z,
-                                              y0, y1, y2);
+                    float q;
+                    if constexpr (kPlaneBody) {
+                        q = plane_q(pc[0][k], pc[1][k], a.x, a.y, a.z, b.x, b.y, b.z);
+                    } else {
+                        float y0, y1, y2;
+                        q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, pc[0][k] - a.x, pc[1][k] - a.y,
+                                      pc[NC - 1][k] - a.z, y0, y1, y2);
+                    }
                     if (q <= chi2) {
                         const float ah = a.w * exp_neg_half(q);
                         if (ah > alpha_min) {
                             const double ahd = (double)ah;
                             S_[k] += ahd;
-                            N_[k] = __builtin_fma(ahd, (double)b.x, N_[k]);
-                            if constexpr (kDepth) D_[k] = __builtin_fma(ahd, (double)b.y, D_[k]);
+                            N_[k] = __builtin_fma(ahd, cd, N_[k]);
+                            if constexpr (kDepth) D_[k] = __builtin_fma(ahd, (double)(kPlaneBody ? c.x : b.y), D_[k]);
                         }
                     }
                 }
@@ -637,6 +707,40 @@ void ug_forward_kernel(
         if constexpr (kDepth) out_depth[pidx[k]] = (float)(D_[k] / fmax(S_[k], 1e-8));
         out_aux[pidx[k]] = make_float4((float)S_[k], (float)img, kDepth ? (float)D_[k] : 0.f, 0.f);
     }
+}
+
+// kDepth: the depth output is requested (out_depth != NULL); without it the depth numerator is not accumulated
+// and aux holds 0 in its place. kPlane: the pixels are the frames' own grids (pixels == NULL); then the tiles of
+// linear-probe frames evaluate q in the plane form of their pairs (pair_plane_form, read from pair_sect) from the
+// pixels' probe coordinates, and fan-probe tiles take the general form q = |L^T (p - mu)|^2 from the pixels'
+// world positions, as every tile does with explicit pixels.
+template <int TS, bool kDepth, bool kPlane>
+__global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
+void ug_forward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
+    const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
+    float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
+    float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
+    uint16_t* __restrict__ inst_mask) {
+    __shared__ FwdLds<TS> lds;
+    if (stats->overflow) return;
+    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
+    const int f = frame_of_tile(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int lt = t - fr.tile_base;
+    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    const us3d_sweep& sw = sweeps[fr.sweep];
+    if (kPlane && fr.geometry == 0)  // workgroup-uniform
+        ug_forward_tile<TS, kDepth, true>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+                                          sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
+                                          out_alpha, out_depth, out_aux, inst_mask);
+    else
+        ug_forward_tile<TS, kDepth, false>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+                                           sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
+                                           out_alpha, out_depth, out_aux, inst_mask);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -679,44 +783,109 @@ __device__ __forceinline__ void write_partial(float4* __restrict__ partial, int6
     partial[3 * ei + 2] = make_float4(a.v[9], a.v[10], a.v[11], 0.f);
 }
 
-// kFull: alpha or depth gradients are given (d_alpha or d_depth != NULL); without them dL/dalpha_hat is
-// dN (c - v) alone and the depth term is not accumulated.
-template <int TS, bool kFull>
-__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
-    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
-    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels, float chi2,
-    float sqrt_chi2, float alpha_min, float bg_alpha, float bg_intensity, const float4* __restrict__ aux,
-    const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
-    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
+// Plane form (linear-probe tiles with the frames' own grids): an item's lane walks its sub-block row by row. Along
+// a row the axial offset is fixed, so q = (p x + qq y)^2 + (s y)^2 + K costs one subtraction and two multiply-adds
+// per pixel (the same operations as plane_q, so the decisions equal the forward's), and the gradient needs only
+// six moments of m = -dah ah over the pixel offsets (X, Y) from the tile centre: with d = p - mu and y = L^T d
+// affine in (X, Y), d = d_c + X U + Y V, sum(m y_k) and sum(m y_k d_i) follow from sum(m), sum(m X), sum(m Y),
+// sum(m X^2), sum(m X Y), sum(m Y^2) once per instance (plane_sums_to_grad). Per pixel the moments cost three
+// multiply-adds (row sums of m, m X, m X^2; the Y terms are added per row).
+constexpr int kPlaneSums = 9;  // do, dc, sum(dD ah), M0, MX, MY, MXX, MXY, MYY
+
+// The twelve record-space sums (GradSums) of an instance from its plane-form moments about the tile centre.
+__device__ __forceinline__ GradSums plane_sums_to_grad(const GradSums& a, const float4& r0, const float4& r1,
+                                                       const float4& r2, const us3d_frame& fr, int TS, int tx,
+                                                       int ty) {
+    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+    const float half = 0.5f * (float)(TS - 1);
+    const float lat_c = ((float)(tx * TS) + half - fr.col_offset) * fr.delta_lat;
+    const float ax_c = fr.near_mm + ((float)(ty * TS) + half) * fr.delta_ax;
+    float dc[3], U[3], V[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        dc[i] = __builtin_fmaf(ax_c, fr.axis_ax[i], __builtin_fmaf(lat_c, fr.axis_lat[i], fr.origin[i] - (&r0.x)[i]));
+        U[i] = fr.delta_lat * fr.axis_lat[i];
+        V[i] = fr.delta_ax * fr.axis_ax[i];
+    }
+    auto lt = [&](const float v[3], float y[3]) {  // y = L^T v
+        y[0] = __builtin_fmaf(l13, v[2], __builtin_fmaf(l12, v[1], l11 * v[0]));
+        y[1] = __builtin_fmaf(l23, v[2], l22 * v[1]);
+        y[2] = l33 * v[2];
+    };
+    float yc[3], yU[3], yV[3];
+    lt(dc, yc);
+    lt(U, yU);
+    lt(V, yV);
+    const float M0 = a.v[3], MX = a.v[4], MY = a.v[5], MXX = a.v[6], MXY = a.v[7], MYY = a.v[8];
+    GradSums o;
+    o.v[0] = a.v[0];
+    o.v[1] = a.v[1];
+    o.v[2] = a.v[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o.v[3 + k] = __builtin_fmaf(yV[k], MY, __builtin_fmaf(yU[k], MX, yc[k] * M0));
+    // sum(m y_k d_i) for (k, i) = (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+    const int kk[6] = {0, 0, 0, 1, 1, 2}, ii[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int e = 0; e < 6; ++e) {
+        const int k = kk[e], i = ii[e];
+        float g = yc[k] * dc[i] * M0;
+        g = __builtin_fmaf(__builtin_fmaf(yc[k], U[i], yU[k] * dc[i]), MX, g);
+        g = __builtin_fmaf(__builtin_fmaf(yc[k], V[i], yV[k] * dc[i]), MY, g);
+        g = __builtin_fmaf(yU[k] * U[i], MXX, g);
+        g = __builtin_fmaf(__builtin_fmaf(yU[k], V[i], yV[k] * U[i]), MXY, g);
+        g = __builtin_fmaf(yV[k] * V[i], MYY, g);
+        o.v[6 + e] = g;
+    }
+    return o;
+}
+
+// Plane-form staging inside the adjoint's pixel arrays: per sub-block kPlanePixStride float2 of (dN, v) in p and
+// of (dX, dD) in c (66: 16-byte aligned rows, sub-blocks in different banks), then from float4 NSUB *
+// kPlaneCoordBase of c, per sub-block 8 column lateral and 8 row axial coordinates.
+constexpr int kPlanePixStride = 66;
+constexpr int kPlaneCoordBase = 33;
+static_assert(kPlanePixStride * 8 <= kSubStride * 16 && (kPlaneCoordBase + 4) <= kSubStride, "plane staging fits");
+
+// LDS of one adjoint workgroup (the plane form's two extra record arrays only where a tile can take it).
+template <int TS, bool kPlane>
+struct BwdLds {
+    static constexpr int NT = TileShape<TS>::kThreads;
+    static constexpr int NSUB = TileShape<TS>::kSub;
+    static constexpr int MAXI = NT * NSUB;     // items of one batch, at most
+    static constexpr int MAXC = MAXI / kWave;  // chunks of one batch, at most
+    float4 p[NSUB * kSubStride];  // general form: px, py, pz, dN; plane form: lat, ax, dN, v
+    float4 c[NSUB * kSubStride];  // general form: v, dX, dD; plane form: dX, dD
+    float4 g0[NT], g1[NT], g2[NT];                     // the batch's Gaussian records
+    float4 g3[kPlane ? NT : 1], g4[kPlane ? NT : 1];   // plane form: (cx, cy, K, o), (p, qq, s, c)
+    uint16_t item[MAXI];
+    int wtot[TileShape<TS>::kWaves];
+    GradSums pend[MAXC];   // head part of a Gaussian that continues into the next chunk
+    GradSums carry[MAXC];  // tail part of a Gaussian that began in the previous chunk
+    int pend_slot[MAXC];
+};
+
+// The adjoint of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_backward_kernel).
+template <int TS, bool kFull, bool kPlaneBody, bool kPlane>
+__device__ __forceinline__ void ug_backward_tile(
+    BwdLds<TS, kPlane>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
+    const int32_t* __restrict__ seg_val, const us3d_frame& fr, const us3d_sweep& sw, int f, int tx, int ty, int s0,
+    int s1, const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
+    float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
+    const float* __restrict__ d_alpha, const float* __restrict__ d_depth, float4* __restrict__ partial,
+    const uint16_t* __restrict__ inst_mask) {
     using S = TileShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
     constexpr int NSUB = S::kSub;
     constexpr int NW = S::kWaves;
-    constexpr int MAXI = NT * NSUB;     // items of one batch, at most
-    constexpr int MAXC = MAXI / kWave;  // chunks of one batch, at most
+    constexpr int MAXC = BwdLds<TS, kPlane>::MAXC;
+    static_assert(kPlane || !kPlaneBody, "the plane form needs the plane record arrays");
+    constexpr int NS = kPlaneBody ? kPlaneSums : 12;
     static_assert(NT <= 256 && NSUB <= 16, "item code packs (slot << 4) | sub-block into 12 bits");
     static_assert(FwdShape<TS>::kSub == NSUB, "the forward's instance masks number the same sub-blocks");
-    __shared__ float4 s_p[NSUB * kSubStride];  // px, py, pz, dN
-    __shared__ float4 s_c[NSUB * kSubStride];  // unclamped image v, dX, dD
-    __shared__ float4 s_g0[NT], s_g1[NT], s_g2[NT];  // the batch's Gaussian records
-    __shared__ uint16_t s_item[MAXI];
-    __shared__ int s_wtot[NW];
-    __shared__ GradSums s_pend[MAXC];   // head part of a Gaussian that continues into the next chunk
-    __shared__ GradSums s_carry[MAXC];  // tail part of a Gaussian that began in the previous chunk
-    __shared__ int s_pend_slot[MAXC];
-    if (stats->overflow) return;
-
-    const int t = blockIdx.x;
-    const int f = frame_of_tile(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    if (s0 == s1) return;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -727,7 +896,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
         float x = 0.f, y = 0.f, z = 0.f, dN = 0.f, dX = 0.f, dD = 0.f, vimg = 0.f;
         if (valid) {
             const int64_t p = (int64_t)fr.pixel_base + iy * fr.width + ix;
-            pixel_position(fr, pixels, ix, iy, x, y, z);
+            if constexpr (!kPlaneBody) pixel_position(fr, pixels, ix, iy, x, y, z);
             const float4 sm = aux[p];  // sum alpha_hat, unclamped image, sum alpha_hat * elev
             const float A = sm.x + bg_alpha;
             const float inv = 1.0f / fmaxf(A, 1e-8f);
@@ -740,8 +909,19 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             dD = gd * invS;
             dX = ga - gd * sm.z * invS * invS;
         }
-        s_p[sb * kSubStride + lane] = make_float4(x, y, z, dN);
-        s_c[sb * kSubStride + lane] = make_float4(vimg, dX, dD, 0.f);
+        if constexpr (kPlaneBody) {
+            // per pixel (dN, v) and (dX, dD); per sub-block the lateral coordinate of each column and the axial
+            // coordinate of each row (every pixel of the sub-block: dN = 0 outside the frame)
+            grid_lat_ax(fr, ix, iy, x, y);
+            reinterpret_cast<float2*>(L.p)[sb * kPlanePixStride + lane] = make_float2(dN, vimg);
+            reinterpret_cast<float2*>(L.c)[sb * kPlanePixStride + lane] = make_float2(dX, dD);
+            float* la = reinterpret_cast<float*>(L.c + NSUB * kPlaneCoordBase) + sb * 16;
+            if (lane < 8) la[lane] = x;
+            if ((lane & 7) == 0) la[8 + (lane >> 3)] = y;
+        } else {
+            L.p[sb * kSubStride + lane] = make_float4(x, y, z, dN);
+            L.c[sb * kSubStride + lane] = make_float4(vimg, dX, dD, 0.f);
+        }
     }
 
     for (int base = s0; base < s1; base += NT) {
@@ -754,9 +934,15 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             const float4 q1 = rec[3 * (int64_t)g + 1];
             const float4 q2 = rec[3 * (int64_t)g + 2];
             m = inst_mask[base + tid];  // the sub-block mask the forward left for this instance
-            s_g0[tid] = q0;
-            s_g1[tid] = q1;
-            s_g2[tid] = q2;
+            L.g0[tid] = q0;
+            L.g1[tid] = q1;
+            L.g2[tid] = q2;
+            if constexpr (kPlaneBody) {
+                const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
+                const float4 ps1 = pair_sect[3 * pair + 1], ps2 = pair_sect[3 * pair + 2];
+                L.g3[tid] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
+                L.g4[tid] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
+            }
             if (m == 0) {  // touches no pixel of the tile: its partial is zero
                 const int64_t ei = seg_val[base + tid];
                 const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -765,7 +951,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                 partial[3 * ei + 2] = z4;
             }
         }
-        for (int c = tid; c < MAXC; c += NT) s_pend_slot[c] = -1;
+        for (int c = tid; c < MAXC; c += NT) L.pend_slot[c] = -1;
         // exclusive scan of the per-Gaussian item counts over the batch
         const int pc = __popc(m);
         int incl = pc;
@@ -774,80 +960,149 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             const int v = __shfl_up(incl, off, kWave);
             if (lane >= off) incl += v;
         }
-        if (lane == kWave - 1) s_wtot[wave] = incl;
+        if (lane == kWave - 1) L.wtot[wave] = incl;
         __syncthreads();
         int pos = incl - pc, total = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) {
-            const int tw = s_wtot[w];
+            const int tw = L.wtot[w];
             pos += (w < wave) ? tw : 0;
             total += tw;
         }
-        for (uint32_t mm = m; mm; mm &= mm - 1) s_item[pos++] = (uint16_t)((tid << 4) | __builtin_ctz(mm));
+        for (uint32_t mm = m; mm; mm &= mm - 1) L.item[pos++] = (uint16_t)((tid << 4) | __builtin_ctz(mm));
         __syncthreads();
 
         const int nchunks = (total + kWave - 1) / kWave;
         for (int c = wave; c < nchunks; c += NW) {
             const int it = c * kWave + lane;
             const bool valid = it < total;
-            const int item = valid ? (int)s_item[it] : 0;
+            const int item = valid ? (int)L.item[it] : 0;
             const int slot = valid ? (item >> 4) : -1 - lane;  // unique negative ids for empty lanes
             const int sb = item & 15;
-            const int prev_slot = (valid && it > 0) ? ((int)s_item[it - 1] >> 4) : -1000;
-            const int next_slot = (it + 1 < total) ? ((int)s_item[it + 1] >> 4) : -1000;
+            const int prev_slot = (valid && it > 0) ? ((int)L.item[it - 1] >> 4) : -1000;
+            const int next_slot = (it + 1 < total) ? ((int)L.item[it + 1] >> 4) : -1000;
             const bool first = valid && prev_slot != slot;  // first item of its Gaussian
             const bool last = valid && next_slot != slot;   // last item of its Gaussian
 
             GradSums a;
 #pragma unroll
             for (int k = 0; k < 12; ++k) a.v[k] = 0.f;
-            float4 r0 = make_float4(0.f, 0.f, 0.f, 0.f), r1 = r0, r2 = r0;
             if (valid) {
-                r0 = s_g0[slot];
-                r1 = s_g1[slot];
-                r2 = s_g2[slot];
-                const float mx = r0.x, my = r0.y, mz = r0.z, o = r0.w, c = r1.x;
-                const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
-                const float e = dot3(mx - ox, my - oy, mz - oz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
-                const float4* __restrict__ sp = s_p + sb * kSubStride;
-                const float4* __restrict__ sc = s_c + sb * kSubStride;
-                float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
-                float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
-                // the next pixel's position and coefficients are read before this pixel's work, so the LDS
-                // latency overlaps it (the branches below would otherwise expose it on every pixel)
-                float4 pn = sp[0], cn = sc[0];
-                for (int j = 0; j < kWave; ++j) {
-                    const float4 pp = pn, cc = cn;
-                    pn = sp[(j + 1) & (kWave - 1)];
-                    cn = sc[(j + 1) & (kWave - 1)];
-                    const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
-                    float y0, y1, y2;
-                    const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
-                    if (q <= chi2) {
-                        const float w = exp_neg_half(q);
-                        const float ah = o * w;
-                        if (ah > alpha_min) {
-                            // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
-                            // cancellation of dN c against dN v
-                            const float da = kFull ? __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y))
-                                                   : pp.w * (c - cc.x);
-                            ac = __builtin_fmaf(pp.w, ah, ac);
-                            ao = __builtin_fmaf(da, w, ao);
-                            if constexpr (kFull) aed = __builtin_fmaf(cc.z, ah, aed);
-                            const float mm = -da * ah;
-                            const float m0 = mm * y0, m1 = mm * y1, m2 = mm * y2;
-                            Y0 += m0; Y1 += m1; Y2 += m2;
-                            g11 = __builtin_fmaf(m0, dx, g11);
-                            g12 = __builtin_fmaf(m0, dy, g12);
-                            g13 = __builtin_fmaf(m0, dz, g13);
-                            g22 = __builtin_fmaf(m1, dy, g22);
-                            g23 = __builtin_fmaf(m1, dz, g23);
-                            g33 = __builtin_fmaf(m2, dz, g33);
+                const float4 r0 = L.g0[slot];
+                const float e = dot3(r0.x - ox, r0.y - oy, r0.z - oz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+                const float4* __restrict__ sp = L.p + sb * kSubStride;
+                const float4* __restrict__ sc = L.c + sb * kSubStride;
+                if constexpr (kPlaneBody) {
+                    const float4 A = L.g3[slot];  // cx, cy, K, o
+                    const float4 B = L.g4[slot];  // p, qq, s, c
+                    const float* la = reinterpret_cast<const float*>(L.c + NSUB * kPlaneCoordBase) + sb * 16;
+                    const float4 la0 = *reinterpret_cast<const float4*>(la), la1 = *reinterpret_cast<const float4*>(la + 4);
+                    const float lat[8] = {la0.x, la0.y, la0.z, la0.w, la1.x, la1.y, la1.z, la1.w};
+                    // (dN, v) of the sub-block's pixels, two per float4, a row = four float4 read together at the
+                    // start of the row, so the LDS latency is paid once per row
+                    const float4* __restrict__ dv = reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(L.p) +
+                                                                                    sb * kPlanePixStride);
+                    const float2* __restrict__ dxd = reinterpret_cast<const float2*>(L.c) + sb * kPlanePixStride;
+                    float ao = 0.f, ac = 0.f, aed = 0.f;
+                    float M0 = 0.f, MX = 0.f, MY = 0.f, MXX = 0.f, MXY = 0.f, MYY = 0.f;
+#pragma unroll
+                    for (int r = 0; r < 8; ++r) {
+                        float4 cur[4];
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) cur[i] = dv[4 * r + i];
+                        const float y = la[8 + r] - A.y;
+                        const float qy = B.y * y;
+                        const float v = B.z * y;
+                        const float vv = __builtin_fmaf(v, v, A.z);
+                        float R0 = 0.f, RX = 0.f, RXX = 0.f;
+#pragma unroll
+                        for (int jx = 0; jx < 8; ++jx) {
+                            const float dN = (jx & 1) ? cur[jx >> 1].z : cur[jx >> 1].x;
+                            const float vi = (jx & 1) ? cur[jx >> 1].w : cur[jx >> 1].y;
+                            // branch-free: a pixel that fails the forward's tests (q <= chi2, ah > alpha_min)
+                            // enters every sum with a zero factor; the coefficients of a listed pair are finite,
+                            // so q, w and ah are finite and the masked terms are exact zeros
+                            const float u = __builtin_fmaf(B.x, lat[jx] - A.x, qy);
+                            const float q = __builtin_fmaf(u, u, vv);
+                            const float w = exp_neg_half(q);
+                            const float ah = A.w * w;
+                            const bool pass = (q <= chi2) && (ah > alpha_min);
+                            float da;
+                            if constexpr (kFull) {
+                                const float2 cc = dxd[8 * r + jx];
+                                da = __builtin_fmaf(dN, B.w - vi, __builtin_fmaf(cc.y, e, cc.x));
+                                aed = __builtin_fmaf(pass ? cc.y : 0.f, ah, aed);
+                            } else {
+                                da = dN * (B.w - vi);
+                            }
+                            const float dNp = pass ? dN : 0.f, dap = pass ? da : 0.f;
+                            ac = __builtin_fmaf(dNp, ah, ac);
+                            ao = __builtin_fmaf(dap, w, ao);
+                            const float mm = -dap * ah;
+                            const float X = (float)jx - 3.5f;  // constants once the loops are unrolled
+                            R0 += mm;
+                            RX = __builtin_fmaf(mm, X, RX);
+                            RXX = __builtin_fmaf(mm, X * X, RXX);
+                        }
+                        const float Y = (float)r - 3.5f;
+                        M0 += R0;
+                        MX += RX;
+                        MXX += RXX;
+                        MY = __builtin_fmaf(R0, Y, MY);
+                        MYY = __builtin_fmaf(R0, Y * Y, MYY);
+                        MXY = __builtin_fmaf(RX, Y, MXY);
+                    }
+                    // moments about the sub-block centre -> about the tile centre (offset (ox_, oy_) pixels)
+                    const float ox_ = (float)((sb % S::kSubPerRow) * 8) + 3.5f - 0.5f * (float)(TS - 1);
+                    const float oy_ = (float)((sb / S::kSubPerRow) * 8) + 3.5f - 0.5f * (float)(TS - 1);
+                    a.v[0] = ao; a.v[1] = ac; a.v[2] = aed; a.v[3] = M0;
+                    a.v[6] = __builtin_fmaf(2.f * ox_, MX, __builtin_fmaf(ox_ * ox_, M0, MXX));
+                    a.v[7] = __builtin_fmaf(ox_, MY, __builtin_fmaf(oy_, MX, __builtin_fmaf(ox_ * oy_, M0, MXY)));
+                    a.v[8] = __builtin_fmaf(2.f * oy_, MY, __builtin_fmaf(oy_ * oy_, M0, MYY));
+                    a.v[4] = __builtin_fmaf(ox_, M0, MX);
+                    a.v[5] = __builtin_fmaf(oy_, M0, MY);
+                } else {
+                    const float4 r1 = L.g1[slot], r2 = L.g2[slot];
+                    const float mx = r0.x, my = r0.y, mz = r0.z, o = r0.w, c = r1.x;
+                    const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
+                    float ao = 0.f, ac = 0.f, aed = 0.f, Y0 = 0.f, Y1 = 0.f, Y2 = 0.f;
+                    float g11 = 0.f, g12 = 0.f, g13 = 0.f, g22 = 0.f, g23 = 0.f, g33 = 0.f;
+                    // the next pixel's position and coefficients are read before this pixel's work, so the LDS
+                    // latency overlaps it (the branches below would otherwise expose it on every pixel)
+                    float4 pn = sp[0], cn = sc[0];
+                    for (int j = 0; j < kWave; ++j) {
+                        const float4 pp = pn, cc = cn;
+                        pn = sp[(j + 1) & (kWave - 1)];
+                        cn = sc[(j + 1) & (kWave - 1)];
+                        const float dx = pp.x - mx, dy = pp.y - my, dz = pp.z - mz;
+                        float y0, y1, y2;
+                        const float q = quad_form(l11, l12, l13, l22, l23, l33, dx, dy, dz, y0, y1, y2);
+                        if (q <= chi2) {
+                            const float w = exp_neg_half(q);
+                            const float ah = o * w;
+                            if (ah > alpha_min) {
+                                // dL/dah = dN c + dA + ... = dN (c - v) + dX + dD elev: the (c - v) form avoids the
+                                // cancellation of dN c against dN v
+                                const float da = kFull ? __builtin_fmaf(pp.w, c - cc.x, __builtin_fmaf(cc.z, e, cc.y))
+                                                       : pp.w * (c - cc.x);
+                                ac = __builtin_fmaf(pp.w, ah, ac);
+                                ao = __builtin_fmaf(da, w, ao);
+                                if constexpr (kFull) aed = __builtin_fmaf(cc.z, ah, aed);
+                                const float mm = -da * ah;
+                                const float m0 = mm * y0, m1 = mm * y1, m2 = mm * y2;
+                                Y0 += m0; Y1 += m1; Y2 += m2;
+                                g11 = __builtin_fmaf(m0, dx, g11);
+                                g12 = __builtin_fmaf(m0, dy, g12);
+                                g13 = __builtin_fmaf(m0, dz, g13);
+                                g22 = __builtin_fmaf(m1, dy, g22);
+                                g23 = __builtin_fmaf(m1, dz, g23);
+                                g33 = __builtin_fmaf(m2, dz, g33);
+                            }
                         }
                     }
+                    a.v[0] = ao; a.v[1] = ac; a.v[2] = aed; a.v[3] = Y0; a.v[4] = Y1; a.v[5] = Y2;
+                    a.v[6] = g11; a.v[7] = g12; a.v[8] = g13; a.v[9] = g22; a.v[10] = g23; a.v[11] = g33;
                 }
-                a.v[0] = ao; a.v[1] = ac; a.v[2] = aed; a.v[3] = Y0; a.v[4] = Y1; a.v[5] = Y2;
-                a.v[6] = g11; a.v[7] = g12; a.v[8] = g13; a.v[9] = g22; a.v[10] = g23; a.v[11] = g33;
             }
             // segmented reduction: a Gaussian's lanes are adjacent; after the steps the first lane of each run
             // holds the run's sum (tree order fixed by the lane positions)
@@ -856,7 +1111,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
                 const int oslot = __shfl_down(slot, off, kWave);
                 const bool same = (lane + off < kWave) && oslot == slot;
 #pragma unroll
-                for (int k = 0; k < 12; ++k) {
+                for (int k = 0; k < NS; ++k) {
                     const float v = __shfl_down(a.v[k], off, kWave);
                     a.v[k] = same ? a.v[k] + v : a.v[k];
                 }
@@ -869,27 +1124,61 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
             const bool from_prev = (lane == 0) && valid && !first;
             if (head) {
                 if (from_prev) {
-                    s_carry[c] = a;
+                    L.carry[c] = a;
                 } else if (to_next) {
-                    s_pend[c] = a;
-                    s_pend_slot[c] = slot;
+                    L.pend[c] = a;
+                    L.pend_slot[c] = slot;
                 } else {
-                    write_partial(partial, seg_val[base + slot], a, r1, r2, fr);
+                    if constexpr (kPlaneBody)
+                        a = plane_sums_to_grad(a, L.g0[slot], L.g1[slot], L.g2[slot], fr, TS, tx, ty);
+                    write_partial(partial, seg_val[base + slot], a, L.g1[slot], L.g2[slot], fr);
                 }
             }
         }
         __syncthreads();
         // Gaussians cut by a chunk boundary: head part + tail part
         for (int c = tid; c + 1 < nchunks; c += NT) {
-            const int slot = s_pend_slot[c];
+            const int slot = L.pend_slot[c];
             if (slot < 0) continue;
-            GradSums a = s_pend[c];
-            const GradSums& b = s_carry[c + 1];
+            GradSums a = L.pend[c];
+            const GradSums& b = L.carry[c + 1];
 #pragma unroll
-            for (int k = 0; k < 12; ++k) a.v[k] += b.v[k];
-            write_partial(partial, seg_val[base + slot], a, s_g1[slot], s_g2[slot], fr);
+            for (int k = 0; k < NS; ++k) a.v[k] += b.v[k];
+            if constexpr (kPlaneBody) a = plane_sums_to_grad(a, L.g0[slot], L.g1[slot], L.g2[slot], fr, TS, tx, ty);
+            write_partial(partial, seg_val[base + slot], a, L.g1[slot], L.g2[slot], fr);
         }
     }
+}
+
+// kFull: alpha or depth gradients are given (d_alpha or d_depth != NULL); without them dL/dalpha_hat is
+// dN (c - v) alone and the depth term is not accumulated. kPlane: the pixels are the frames' own grids (pixels ==
+// NULL); linear-probe tiles then take the plane form of q (as in the forward) and fan tiles the general form.
+template <int TS, bool kFull, bool kPlane>
+__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
+    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
+    const float4* __restrict__ pair_sect, float chi2, float alpha_min, float bg_alpha, const float4* __restrict__ aux,
+    const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
+    __shared__ BwdLds<TS, kPlane> lds;
+    if (stats->overflow) return;
+    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
+    const int f = frame_of_tile(frame_tile_base, n_frames, t);
+    const us3d_frame& fr = frames[f];
+    const int lt = t - fr.tile_base;
+    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+    const int s0 = tile_off[t], s1 = tile_off[t + 1];
+    if (s0 == s1) return;
+    const us3d_sweep& sw = sweeps[fr.sweep];
+    if (kPlane && fr.geometry == 0)  // workgroup-uniform
+        ug_backward_tile<TS, kFull, kPlane, kPlane>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+                                          chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
+                                          inst_mask);
+    else
+        ug_backward_tile<TS, kFull, false, kPlane>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+                                           chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
+                                           inst_mask);
 }
 
 // One wavefront per Gaussian: lanes stride over the Gaussian's instances (contiguous in emit order across
@@ -1143,14 +1432,19 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     if (!image || !alpha || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_FWD(TS, DEPTH)                                                                                             \
-    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0, stream,  \
+#define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
+    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0,   \
+                       stream,                                                                                        \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        plan->frames, plan->sweeps, plan->frame_tile_base, plan->n_frames, pixels,                     \
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
                        reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask))
+#define UG_FWD(TS, DEPTH)                                                                                             \
+    do {                                                                                                              \
+        if (pixels) UG_FWD_(TS, DEPTH, false); else UG_FWD_(TS, DEPTH, true);                                         \
+    } while (0)
     {
         StageTimer timer_("ug_forward_kernel", stream);
         switch (settings->tile_size) {
@@ -1160,6 +1454,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
         }
     }
 #undef UG_FWD
+#undef UG_FWD_
     return check_launch("ug_forward_kernel");
 }
 
@@ -1175,13 +1470,18 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_BWD(TS, FULL)                                                                                              \
-    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, stream, \
-                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
-                       at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
-                       settings->chi2_threshold, sqrt_chi2, settings->alpha_min, settings->bg_alpha,                  \
-                       settings->bg_intensity, reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,      \
+#define UG_BWD_(TS, FULL, PLANE)                                                                                      \
+    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, \
+                       stream, at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off),                            \
+                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), plan->frames, plan->sweeps,       \
+                       plan->frame_tile_base, plan->n_frames, pixels, at<float4>(plan, L.pair_sect),                  \
+                       settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
+                       reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask))
+#define UG_BWD(TS, FULL)                                                                                              \
+    do {                                                                                                              \
+        if (pixels) UG_BWD_(TS, FULL, false); else UG_BWD_(TS, FULL, true);                                           \
+    } while (0)
     const bool full = d_alpha || d_depth;
     {
         StageTimer timer_("ug_backward_kernel", stream);
@@ -1192,6 +1492,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         }
     }
 #undef UG_BWD
+#undef UG_BWD_
     if ((rc = check_launch("ug_backward_kernel"))) return rc;
     if (plan->n_gauss > 0) {
         {

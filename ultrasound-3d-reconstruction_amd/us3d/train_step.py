@@ -66,12 +66,14 @@ class SliceSplatTrainer:
         self.graph = None
         # size the instance arrays from an eager mapping of the current parameters
         with torch.no_grad():
-            m = self.batch.map(self.means, self.tril, self.raw_o, self.raw_c)
-        self._allocate(int(m.stats.n_instances * headroom) + 4096)
+            k = int(self.batch.map(self.means, self.tril, self.raw_o, self.raw_c).stats.n_instances)
+        self._allocate(int(k * headroom) + 4096)  # the sizing map's workspace is released first
 
     # ------------------------------------------------------------------ plumbing
     def _allocate(self, k_cap: int) -> None:
         self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+        self.graph = None
+        self.workspace = None  # the old workspace goes before the new one is allocated
         self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
         self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
         off = self.batch.layout_offsets(self.k_capacity)
@@ -193,11 +195,13 @@ class GaussianPileTrainer:
         self.n_steps = 0
         self.graph = None
         with torch.no_grad():
-            m = self.batch.map(self.means, self.raw_s, self.raw_q, self.raw_o)
-        self._allocate(int(m.stats.n_instances * headroom) + 4096)
+            k = int(self.batch.map(self.means, self.raw_s, self.raw_q, self.raw_o).stats.n_instances)
+        self._allocate(int(k * headroom) + 4096)
 
     def _allocate(self, k_cap: int) -> None:
         self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+        self.graph = None
+        self.workspace = None  # the old workspace goes before the new one is allocated
         self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
         self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
         off = self.batch.layout_offsets(self.k_capacity)
