@@ -10,7 +10,8 @@
  * Conventions
  *   - Every pointer inside a plan, and every array argument, is a DEVICE pointer unless stated.
  *   - Nothing is allocated inside the library. All scratch lives in a caller-provided workspace whose
- *     size comes from the matching *_workspace_bytes() query. Every entry point is stream-ordered on
+ *     size comes from the matching *_workspace_bytes() query (the one exception, us3d_ug_render_static, uses a
+ *     fixed static per-GPU array instead). Every entry point is stream-ordered on
  *     `stream` (a hipStream_t passed as void*; NULL = legacy default stream), performs no host
  *     synchronisation and issues no hipMemset, so a call sequence can be captured into a hipGraph.
  *   - Results that depend on data (instance counts, overflow) are written to the us3d_stats block at
@@ -187,6 +188,20 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
 int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
                         int pass, int32_t* counts, const int32_t* offsets, int32_t* indices,
                         void* stream);
+
+/* No-allocation render. No counterpart in the reference: for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Runs us3d_ug_map then
+ * us3d_ug_forward on the same arguments, with the workspace taken from a static per-GPU array of
+ * US3D_STATIC_SCRATCH_BYTES instead of the plan's: plan->workspace, workspace_bytes and k_capacity are ignored (may
+ * be 0), the instance capacity is the largest that fits. A batch whose workspace cannot fit is rejected with
+ * US3D_ERR_CAPACITY before any device work. overflow_dev (device int, may be NULL) receives 1 if the instances
+ * exceeded that capacity (the outputs are then invalid), else 0. Allocates nothing (no hipMalloc/hipFree); the first
+ * call on a device looks the array up with hipGetSymbolAddress. Each GPU's array keeps the last call's contents, so
+ * calls that overlap on one GPU (several streams or threads) race; serialise them. */
+#define US3D_STATIC_SCRATCH_BYTES (64ll << 20)
+int us3d_ug_render_static(const us3d_plan* plan, const float* means, const float* tril, const float* opacity,
+                          const float* intensity, const float* pixels, const us3d_ug_settings* settings, float* image,
+                          float* alpha, float* depth, float* aux, int32_t* overflow_dev, void* stream);
 
 /* ---------------------------------------------------------------- UltraG-Ray B-mode ray renderer */
 

@@ -161,3 +161,17 @@ def test_ray_renderer_rejects_fan_frames():
     fr.geometry = 1
     with pytest.raises(ValueError):
         URBatch([fr], [SweepSpec(0, 10)], 10, _lib.URSettings(), torch.device("cpu"))
+
+
+def test_render_static_refuses_an_oversized_batch_on_the_host():
+    """The no-allocation render checks that the batch's workspace fits its static scratch before touching the
+    device (US3D_ERR_CAPACITY); plan pointers are placeholders that are never dereferenced."""
+    p = _lib.Plan()
+    p.n_gauss, p.n_frames, p.n_sweeps, p.n_tiles, p.n_pairs, p.n_pixels = 1 << 20, 512, 1, 1 << 18, 1 << 29, 1 << 27
+    dummy = ctypes.create_string_buffer(64)
+    for f in ("frames", "frame_tile_base", "frame_pixel_base", "sweeps"):
+        setattr(p, f, ctypes.addressof(dummy))
+    st = _lib.UGSettings(chi2_threshold=7.815, diag_beta=1e-4, tile_size=16)
+    L = _lib.lib()
+    assert L.us3d_ug_render_static(p, None, None, None, None, None, st, None, None, None, None, None, None) == 3
+    assert b"static scratch" in L.us3d_last_error()

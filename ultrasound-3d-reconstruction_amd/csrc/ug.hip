@@ -11,6 +11,7 @@
 //   partial   f32x12 per instance, written at e by the adjoint, so each Gaussian's partials across all
 //             frames are contiguous and reduce in a fixed order.
 #include <algorithm>
+#include <mutex>
 
 #include "common.h"
 
@@ -1365,6 +1366,47 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
                          at<int32_t>(plan, L.seg_val_tmp), tile_off, plan->n_tiles, stats, plan->k_capacity, stream);
 }
 
+// ---------------------------------------------------------------- static scratch (no-allocation render)
+// us3d_ug_render_static takes its workspace from this array instead of the caller. Every GPU the library runs on
+// holds its own copy until the process ends. In host code `scratch` names a host-side stand-in, not a device
+// address: the device address comes from hipGetSymbolAddress, once per device (static_scratch_ptr), and only that
+// pointer is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[US3D_STATIC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+
+static int static_scratch_ptr(void** out) {
+    static std::mutex mu;
+    static void* ptrs[kMaxDevices] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+        return fail(US3D_ERR_HIP, "render_static: no current device");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(US3D_ERR_HIP, std::string("render_static: hipGetSymbolAddress(scratch) failed: ") +
+                                          hipGetErrorString(e));
+        ptrs[dev] = p;
+    }
+    *out = ptrs[dev];
+    return US3D_OK;
+}
+
+// Largest instance capacity whose workspace fits `bytes` (0 if none does); the layout grows with k.
+static int64_t static_k_capacity(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t bytes) {
+    if (ug_layout(n_gauss, n_pairs, n_tiles, 1).total > bytes) return 0;
+    int64_t lo = 1, hi = (int64_t(1) << 31) - 1;
+    while (lo < hi) {
+        const int64_t mid = lo + (hi - lo + 1) / 2;
+        if (ug_layout(n_gauss, n_pairs, n_tiles, mid).total <= bytes) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void copy_overflow_kernel(const us3d_stats* stats, int32_t* out) { *out = stats->overflow; }
+
 }  // namespace us3d
 
 using namespace us3d;
@@ -1528,6 +1570,33 @@ int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_u
                            at<us3d_stats>(plan, L.stats));
     }
     return check_launch("ug_pixel_lists_kernel");
+}
+
+int us3d_ug_render_static(const us3d_plan* plan_in, const float* means, const float* tril, const float* opacity,
+                          const float* intensity, const float* pixels, const us3d_ug_settings* settings, float* image,
+                          float* alpha, float* depth, float* aux, int32_t* overflow_dev, void* stream_) {
+    if (!plan_in) return fail(US3D_ERR_ARGUMENT, "render_static: plan is NULL");
+    us3d_plan plan = *plan_in;
+    const int64_t k_cap = static_k_capacity(plan.n_gauss, plan.n_pairs, plan.n_tiles, US3D_STATIC_SCRATCH_BYTES);
+    if (k_cap < 1)
+        return fail(US3D_ERR_CAPACITY, "render_static: the batch's workspace does not fit the " +
+                                           std::to_string(US3D_STATIC_SCRATCH_BYTES >> 20) +
+                                           " MiB static scratch (use us3d_ug_map / us3d_ug_forward with a workspace)");
+    void* ws = nullptr;
+    int rc = static_scratch_ptr(&ws);
+    if (rc) return rc;
+    plan.workspace = ws;
+    plan.workspace_bytes = US3D_STATIC_SCRATCH_BYTES;
+    plan.k_capacity = k_cap;
+    if ((rc = us3d_ug_map(&plan, means, tril, opacity, intensity, settings, stream_))) return rc;
+    if ((rc = us3d_ug_forward(&plan, pixels, settings, image, alpha, depth, aux, stream_))) return rc;
+    if (overflow_dev) {
+        const us3d_layout L = ug_layout(plan.n_gauss, plan.n_pairs, plan.n_tiles, plan.k_capacity);
+        hipLaunchKernelGGL(copy_overflow_kernel, dim3(1), dim3(1), 0, as_stream(stream_), at<us3d_stats>(&plan, L.stats),
+                           overflow_dev);
+        if ((rc = check_launch("copy_overflow_kernel"))) return rc;
+    }
+    return US3D_OK;
 }
 
 }  // extern "C"

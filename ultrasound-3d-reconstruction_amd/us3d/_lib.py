@@ -149,6 +149,10 @@ SIGNATURES = {
         [_PP, _VP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
     "us3d_ug_pixel_lists": (ctypes.c_int, [_PP, _VP, ctypes.POINTER(UGSettings), ctypes.c_int, _VP, _VP, _VP, _VP]),
+    "us3d_ug_render_static": (
+        ctypes.c_int,
+        [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _VP],
+    ),
     "us3d_ur_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64]),
     "us3d_ur_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
     "us3d_ur_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(URSettings), _VP]),
