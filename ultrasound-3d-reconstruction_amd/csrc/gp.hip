@@ -278,7 +278,7 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     __shared__ float4 s_f0[NT], s_f1[NT];
     __shared__ float s_I[NT];
     if (stats->overflow) return;
-    const int t = blockIdx.x;
+    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
     const int f = upper_index(frame_tile_base, n_frames, t);
     const us3d_frame& fr = frames[f];
     const int lt = t - fr.tile_base;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     __shared__ __align__(16) float s_g[NT];  // dL/dimage of the tile's pixels
     __shared__ __align__(16) float s_xl[TS];  // lateral position of each column
     if (stats->overflow) return;
-    const int t = blockIdx.x;
+    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     if (s0 == s1) return;
     const int f = upper_index(frame_tile_base, n_frames, t);
