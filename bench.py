@@ -1,23 +1,26 @@
 #!/usr/bin/env python
 """Forward+backward frames/s of the differentiable slice splat on one MI355X.
 
-Workload: 200k anisotropic Gaussians in a 120 mm cube; 256 frames of 512x512 at 0.2 mm along a 150 mm freehand
-sweep; loss = L1 + 0.2 (1 - global SSIM) against the same sweep rendered from a second seeded scene; Adam step
-(lr 1e-3) over means, raw precision factors and opacity/intensity logits. One step = all 256 frames forward +
-backward + Adam. "batched" renders the whole sweep as one batch (one map / forward / adjoint launch sequence);
-"plain" renders it frame by frame through the same API (one launch sequence per frame). `value` is the batched
-frames/s; the plain rate, the roofline of the dominant kernel and the CPU baseline are reported beside it.
+Headline workload: 200k anisotropic Gaussians in a 120 mm cube; 256 frames of 512x512 at 0.2 mm along a 150 mm
+freehand sweep; loss = L1 + 0.2 (1 - global SSIM) against the same sweep rendered from a second seeded scene; Adam
+step (lr 1e-3) over means, raw precision factors and opacity/intensity logits. One step = all 256 frames forward +
+backward + Adam. "batched" renders the whole sweep as one batch (one captured launch sequence per step); "plain"
+renders it frame by frame through the same API with torch autograd and torch.optim.Adam. `value` is the batched
+frames/s; the plain rate, the roofline of the dominant kernel and the CPU baseline are reported beside it, and the
+other workloads (UltraG-Ray, multi-sweep, fan probe, GaussianPile) as nested objects of the same line.
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--frames 256] [--gaussians 200000] [--tile-size 16]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload slice|ray|multisweep|fan|gaussianpile]
 """
 
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import platform
 import re
+import statistics
 import subprocess
 import sys
 import time
@@ -27,12 +30,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path[:0] = [str(ROOT / "ultrasound-3d-reconstruction_amd"), str(ROOT)]
 
 HBM_PEAK_GBS = 8000.0
-FP32_VALU_PEAK_TFLOPS = 157.3
+# LDS: 128 bytes per clock per CU (the CDNA3 LDS data path), 256 CUs at 2.4 GHz
+LDS_PEAK_GBS = 128 * 256 * 2.4
+CLOCK_GHZ = 2.4
 
 METRIC = "fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline"
 
 
-def parse():
+def parse(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -43,7 +48,10 @@ def parse():
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-rows", type=int, default=16)
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="headline only: skip the nested ray / multisweep / fan / gaussianpile measurements")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed CPU-baseline runs (median), after one warm-up")
+    ap.add_argument("--cpu-sample-rows", type=int, default=2)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
     ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan", "gaussianpile"), default="slice",
@@ -62,9 +70,29 @@ def parse():
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
                     help="print a markdown summary of a rocprofv3 output directory: MODE 'trace' (per-kernel "
                          "durations of --kernel-trace) or 'pmc' (per-dispatch means of --pmc counters), then exit")
+    ap.add_argument("--rocprof-derive", nargs="+", metavar="ARG",
+                    help="KERNEL DIR [DIR ...]: print the derived figures of one kernel (traffic, LDS bytes, VALU "
+                         "issue and active lanes) from rocprofv3 --pmc directories of the same command, then exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy 5 times (reference dispatch for HBM counters)")
-    return ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def timed(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
 
 
 def ssim_global(x, y):
@@ -82,15 +110,41 @@ def frame_losses(pred, gt):
     return l1 + 0.2 * (1.0 - ssim_global(pred, gt))
 
 
-def cpu_baseline(n_rows: int):
-    """The oracle's restatement of the reference PyTorch fallback renderer (pipeline_god_file.py:556-603: every
-    pixel against every Gaussian, chunked, autograd backward) on a bounded sample of the benchmark workload:
-    n_rows full rows (512 pixels each) of the middle frame, all 200k Gaussians, L1 against the target scene's
-    render of the same pixels, all host threads. frames/s = sampled pixels / 512^2 / time."""
+def host_cpu() -> str:
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        m = re.search(r"Model name:\s*(.+)", out)
+        return m.group(1).strip() if m else platform.processor()
+    except OSError:
+        return platform.processor() or "unknown"
+
+
+# ---------------------------------------------------------------- CPU baseline (the oracle's fallback restatement)
+
+def _fallback_fwd_bwd(scene, target, pix, rows, width):
+    """One forward + autograd backward of the restated PyTorch fallback renderer (pipeline_god_file.py:556-603)
+    over `pix`, L1 against the target scene's render of the same pixels; returns the seconds it took."""
+    from oracle import fallback as ofb
+
+    ts = [t.clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    t0 = time.perf_counter()
+    img, _ = ofb.render(*ts, pix, rows, width)
+    (img - target).abs().mean().backward()
+    return time.perf_counter() - t0
+
+
+def cpu_baseline(n_rows: int, runs: int):
+    """The oracle's restatement of the reference's PyTorch fallback renderer (every pixel against every Gaussian,
+    2048 x 512 chunks, autograd backward), on all host threads, one warm-up then the median of `runs` timed runs:
+      - the benchmark workload, bounded: n_rows full rows of the middle frame of the 256-frame sweep against all
+        200k Gaussians; value = sampled fraction of a 512x512 frame / median time (the fallback's cost is pixels x
+        Gaussians, so this is the rate of the whole workload);
+      - the reference's CPU-sized case complete (4,096 Gaussians, 16 frames of 128x128): frames/s, and the per-pair
+        rate it gives with the benchmark workload's frames/s extrapolated from it."""
     import torch
 
     from oracle import fallback as ofb
-    from us3d.scenes import config2, linear_pixels
+    from us3d.scenes import config1, config2, linear_pixels
 
     torch.set_num_threads(os.cpu_count() or 1)
     scene, target, frames = config2()
@@ -98,49 +152,148 @@ def cpu_baseline(n_rows: int):
     pix = linear_pixels(fr).reshape(fr.height, fr.width, 3)
     r0 = fr.height // 2 - n_rows // 2
     pix = pix[r0:r0 + n_rows].reshape(-1, 3).contiguous()
-    ts = [t.clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
-    tgt = [t.clone() for t in (target.means, target.tril, target.opacity, target.intensity)]
     with torch.no_grad():
-        gt, _ = ofb.render(*tgt, pix, n_rows, fr.width)
-    t0 = time.perf_counter()
-    img, _ = ofb.render(*ts, pix, n_rows, fr.width)
-    (img - gt).abs().mean().backward()
-    dt = time.perf_counter() - t0
-    n_pix = pix.shape[0]
-    pairs = n_pix * scene.means.shape[0]
-    cpu = "unknown"
-    try:
-        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-        m = re.search(r"Model name:\s*(.+)", out)
-        cpu = m.group(1).strip() if m else platform.processor()
-    except OSError:
-        pass
-    return {"value": n_pix / (fr.height * fr.width) / dt, "unit": "frames/s", "cores": torch.get_num_threads(),
-            "kind": "port",
-            "sample": f"{n_rows} rows x {fr.width} pixels of frame {len(frames) // 2} of the benchmark sweep, all "
-                      f"{scene.means.shape[0]} Gaussians, forward + autograd backward of the restated PyTorch "
-                      f"fallback renderer, {dt:.1f} s ({pairs / dt / 1e6:.0f} M pair evaluations/s); frames/s = "
-                      f"sampled fraction of a 512x512 frame / time; host CPU: {cpu}"}
+        gt, _ = ofb.render(target.means, target.tril, target.opacity, target.intensity, pix, n_rows, fr.width)
+    _fallback_fwd_bwd(scene, gt, pix, n_rows, fr.width)
+    t2 = statistics.median(_fallback_fwd_bwd(scene, gt, pix, n_rows, fr.width) for _ in range(runs))
+    G2 = scene.means.shape[0]
+    pairs2 = pix.shape[0] * G2
+
+    s1, t1s, f1 = config1()
+    pix1 = torch.cat([linear_pixels(f) for f in f1]).contiguous()
+    H1, W1 = f1[0].height, f1[0].width
+    with torch.no_grad():
+        gt1, _ = ofb.render(t1s.means, t1s.tril, t1s.opacity, t1s.intensity, pix1, len(f1) * H1, W1)
+    _fallback_fwd_bwd(s1, gt1, pix1, len(f1) * H1, W1)
+    times1 = [_fallback_fwd_bwd(s1, gt1, pix1, len(f1) * H1, W1) for _ in range(runs)]
+    t1 = statistics.median(times1)
+    pairs1 = pix1.shape[0] * s1.means.shape[0]
+    rate1 = pairs1 / t1
+    full2 = fr.height * fr.width * G2
+    return {
+        "value": n_rows / fr.height / t2, "unit": "frames/s", "cores": torch.get_num_threads(), "kind": "port",
+        "sample": f"{n_rows} rows x {fr.width} pixels of frame {len(frames) // 2} of the benchmark sweep against all "
+                  f"{G2} Gaussians: the restated PyTorch fallback renderer, forward + autograd backward, one warm-up "
+                  f"then the median of {runs} runs ({t2:.2f} s, {pairs2 / t2 / 1e6:.0f} M pair evaluations/s); "
+                  f"frames/s = sampled fraction of a 512x512 frame / time; host CPU: {host_cpu()}",
+        "config1": {"frames_per_s": len(f1) / t1, "median_s": t1, "runs_s": [round(t, 3) for t in times1],
+                    "workload": f"{s1.means.shape[0]} Gaussians, {len(f1)} frames {H1}x{W1}, complete",
+                    "pairs_per_s": rate1,
+                    "benchmark_frames_per_s_extrapolated": rate1 / full2},
+    }
 
 
-def read_traffic(kernel: str, workload_tag: str):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary, if one exists for this kernel/workload."""
-    p = ROOT / "profiles" / f"{kernel}_{workload_tag}_pmc.md"
+# ---------------------------------------------------------------- rocprof summaries and the figures read from them
+
+def _short(name: str) -> str:
+    for k in ("void us3d::", "us3d::", "(anonymous namespace)::"):
+        name = name.replace(k, "")
+    return name.split("(")[0][:70]
+
+
+def _pmc_rows(directory: str):
+    import collections
+    import glob
+    import sqlite3
+
+    agg = collections.defaultdict(list)
+    for db in glob.glob(f"{directory}/**/*.db", recursive=True):
+        con = sqlite3.connect(db)
+        for n, c, v in con.execute("select name, counter_name, counter_value from pmc_events"):
+            agg[(_short(n), c)].append(v)
+    return agg
+
+
+def rocprof_summary(mode: str, directory: str) -> None:
+    """Markdown tables from the SQLite databases rocprofv3 writes (kernel durations or PMC counter means)."""
+    import collections
+    import glob
+    import sqlite3
+
+    if mode == "trace":
+        agg = collections.defaultdict(list)
+        for db in glob.glob(f"{directory}/**/*.db", recursive=True):
+            con = sqlite3.connect(db)
+            for n, t0, t1 in con.execute("select name, start, end from kernels"):
+                agg[_short(n)].append(t1 - t0)
+        tot = sum(sum(v) for v in agg.values()) or 1
+        print("| kernel | calls | total ms | avg ms | min ms | max ms | % |")
+        print("|---|---|---|---|---|---|---|")
+        for n, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:20]:
+            print(f"| {n} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e6:.4f} | {min(v) / 1e6:.4f} | "
+                  f"{max(v) / 1e6:.4f} | {100 * sum(v) / tot:.1f} |")
+    else:
+        agg = _pmc_rows(directory)
+        print("| kernel | counter | rows (dispatches x counter instances) | mean per row |")
+        print("|---|---|---|---|")
+        for (n, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:60]:
+            print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
+
+
+def rocprof_derive(kernel: str, dirs) -> None:
+    """Per-dispatch figures of one kernel from --pmc runs of the same command. SQ counters are summed over their
+    instances per dispatch (rows / dispatches instances each); dispatches = the rows of a GRBM_GUI_ACTIVE or
+    FETCH_SIZE counter divided by its instances. Prints `key: value` lines that bench.py reads back:
+      fetch_kib, write_kib            FETCH_SIZE / WRITE_SIZE (KiB per dispatch, raw)
+      valu_insts, lds_insts           wave instructions per dispatch
+      valu_issue_frac                 VALU instructions x 4 cycles / (1024 SIMDs x active cycles)
+      valu_active_threads             SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU: lanes busy per VALU instruction
+      lds_bytes                       (SQ_INSTS_LDS_LOAD_BANDWIDTH + _STORE_BANDWIDTH) x 64 bytes per dispatch
+      lds_bank_conflict_frac          SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE
+      active_cycles                   GRBM_GUI_ACTIVE (mean over XCDs)"""
+    per = {}
+    dispatches = None
+    for d in dirs:
+        for (n, c), v in _pmc_rows(d).items():
+            if n != kernel:
+                continue
+            per[c] = v
+    if "GRBM_GUI_ACTIVE" in per:
+        dispatches = len(per["GRBM_GUI_ACTIVE"]) // 8
+    elif "FETCH_SIZE" in per:
+        dispatches = len(per["FETCH_SIZE"])
+    if not dispatches:
+        raise SystemExit(f"no dispatch count for {kernel}")
+
+    def total(c):
+        return sum(per[c]) / dispatches if c in per else None
+
+    out = {"dispatches": dispatches}
+    if "FETCH_SIZE" in per:
+        out["fetch_kib"] = total("FETCH_SIZE")
+    if "WRITE_SIZE" in per:
+        out["write_kib"] = total("WRITE_SIZE")
+    if "GRBM_GUI_ACTIVE" in per:
+        out["active_cycles"] = sum(per["GRBM_GUI_ACTIVE"]) / len(per["GRBM_GUI_ACTIVE"])
+    if "SQ_INSTS_VALU" in per:
+        out["valu_insts"] = total("SQ_INSTS_VALU")
+        if "active_cycles" in out:
+            out["valu_issue_frac"] = out["valu_insts"] * 4 / (1024 * out["active_cycles"])
+    if "SQ_INSTS_LDS" in per:
+        out["lds_insts"] = total("SQ_INSTS_LDS")
+    if "SQ_THREAD_CYCLES_VALU" in per and "SQ_ACTIVE_INST_VALU" in per:
+        out["valu_active_threads"] = total("SQ_THREAD_CYCLES_VALU") / total("SQ_ACTIVE_INST_VALU")
+    if "SQ_INSTS_LDS_LOAD_BANDWIDTH" in per and "SQ_INSTS_LDS_STORE_BANDWIDTH" in per:
+        out["lds_bytes"] = 64 * (total("SQ_INSTS_LDS_LOAD_BANDWIDTH") + total("SQ_INSTS_LDS_STORE_BANDWIDTH"))
+    if "SQ_LDS_BANK_CONFLICT" in per and "SQ_LDS_IDX_ACTIVE" in per:
+        out["lds_bank_conflict_frac"] = total("SQ_LDS_BANK_CONFLICT") / max(total("SQ_LDS_IDX_ACTIVE"), 1.0)
+    for k, v in out.items():
+        print(f"{k}: {v:.6g}" if isinstance(v, float) else f"{k}: {v}")
+
+
+def profile_figures(kernel: str, workload_tag: str):
+    """Figures of a kernel from its committed rocprofv3 summary (profiles/<kernel>_<workload>_pmc.md), if one exists:
+    every `key: value` line of the file."""
+    name = re.sub(r"[<>, ]+", "_", kernel).strip("_")
+    p = ROOT / "profiles" / f"{name}_{workload_tag}_pmc.md"
     if not p.exists():
-        return None
-    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
-    return float(m.group(1)) if m else None
-
-
-def read_valu_issue(kernel: str):
-    """VALU issue fraction (VALU instructions x 4 cycles / SIMD cycles) of a kernel from the committed rocprofv3
-    counter summaries (profiles/*valu_issue_pmc.md), if one lists the kernel."""
-    for p in sorted((ROOT / "profiles").glob("*valu_issue_pmc.md")):
-        for line in p.read_text().splitlines():
-            c = [x.strip() for x in line.strip("|").split("|")]
-            if len(c) == 7 and c[0] == kernel and c[6].endswith("%"):
-                return float(c[6].rstrip(" %")) / 100.0
-    return None
+        return {}
+    out = {}
+    for line in p.read_text().splitlines():
+        m = re.match(r"^([a-z_]+):\s*([-0-9.eE+]+)\s*$", line.strip())
+        if m:
+            out[m.group(1)] = float(m.group(2))
+    return out
 
 
 def measure_copy_peak():
@@ -177,39 +330,163 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read_per_dispatch": 1 << 30, "copy_bytes_written_per_dispatch": 1 << 30}))
 
 
-def rocprof_summary(mode: str, directory: str) -> None:
-    """Markdown tables from the SQLite databases rocprofv3 writes (kernel durations or PMC counter means)."""
-    import collections
-    import glob
-    import sqlite3
+def stage_timing(run):
+    """Per-kernel {name: (launches, total_ms)} of `run()` with a HIP event pair around every launch."""
+    import torch
 
-    def short(name: str) -> str:
-        for k in ("void us3d::", "us3d::", "(anonymous namespace)::"):
-            name = name.replace(k, "")
-        return name.split("(")[0][:70]
+    from us3d import _lib
 
-    dbs = glob.glob(f"{directory}/**/*.db", recursive=True)
-    agg = collections.defaultdict(list)
-    for db in dbs:
-        con = sqlite3.connect(db)
-        if mode == "trace":
-            for n, t0, t1 in con.execute("select name, start, end from kernels"):
-                agg[short(n)].append(t1 - t0)
-        else:
-            for n, c, v in con.execute("select name, counter_name, counter_value from pmc_events"):
-                agg[(short(n), c)].append(v)
-    if mode == "trace":
-        tot = sum(sum(v) for v in agg.values()) or 1
-        print("| kernel | calls | total ms | avg ms | min ms | max ms | % |")
-        print("|---|---|---|---|---|---|---|")
-        for n, v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:20]:
-            print(f"| {n} | {len(v)} | {sum(v) / 1e6:.3f} | {sum(v) / len(v) / 1e6:.4f} | {min(v) / 1e6:.4f} | "
-                  f"{max(v) / 1e6:.4f} | {100 * sum(v) / tot:.1f} |")
-    else:
-        print("| kernel | counter | rows (dispatches x counter instances) | mean per row |")
-        print("|---|---|---|---|")
-        for (n, c), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:60]:
-            print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
+    torch.cuda.synchronize()
+    _lib.stage_timing_read()
+    _lib.stage_timing(True)
+    run()
+    torch.cuda.synchronize()
+    _lib.stage_timing(False)
+    return _lib.stage_timing_read()
+
+
+def roofline(stages, launch_bytes, profile_key, workload_tag):
+    """HBM roofline of the dominant kernel by total time: algorithmic bytes per launch / the average launch duration
+    measured with HIP events on the launch stream; measured HBM traffic, VALU issue and lanes, and LDS throughput from
+    the kernel's committed rocprofv3 summary (profile_key: its name there) when there is one."""
+    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
+    avg_ms = total_ms / max(launches, 1)
+    alg = launch_bytes.get(dom)
+    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
+    fig = profile_figures(profile_key(dom), workload_tag) if profile_key else {}
+    out = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+           "traffic": fig.get("traffic_bytes_per_launch"), "avg_launch_ms": round(avg_ms, 4),
+           "algorithmic_bytes_per_launch": alg,
+           "stage_ms_per_launch": {k: round(v[1] / max(v[0], 1), 3)
+                                   for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}}
+    if "valu_issue_frac" in fig:
+        out["valu_issue_frac"] = round(fig["valu_issue_frac"], 3)
+    if "valu_active_threads" in fig:
+        out["valu_active_threads"] = round(fig["valu_active_threads"], 1)
+    if "lds_bytes" in fig:
+        lds_gbs = fig["lds_bytes"] / (avg_ms * 1e-3) / 1e9
+        out["lds_GBs"] = round(lds_gbs, 1)
+        out["lds_peak_GBs"] = round(LDS_PEAK_GBS, 1)
+        out["lds_frac"] = round(lds_gbs / LDS_PEAK_GBS, 4)
+    if "lds_bank_conflict_frac" in fig:
+        out["lds_bank_conflict_frac"] = round(fig["lds_bank_conflict_frac"], 4)
+    return out
+
+
+def visible_pairs(trainer) -> int:
+    import torch
+
+    off = trainer.batch.layout_offsets(trainer.k_capacity)
+    lay = trainer.batch.layout
+    pc = trainer.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32)
+    return int((pc > 0).sum().item())
+
+
+# ---------------------------------------------------------------- workloads
+
+def bench_slice(args):
+    """The headline: the slice splat training step over the 256-frame config-2 sweep."""
+    import torch
+
+    from us3d.batch import CapacityHint, read_stats
+    from us3d.scenes import config2
+    from us3d.train_step import SliceSplatTrainer
+    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
+
+    dev = torch.device("cuda:0")
+    F, G, ts = args.frames, args.gaussians, args.tile_size
+    scene, target, frames = config2(n_frames=F, n_gauss=G)
+    H, W = frames[0].height, frames[0].width
+    cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=0.2, axial_spacing_mm_per_pix=0.2,
+                                 axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=ts)
+    hint = CapacityHint()
+    with torch.no_grad():
+        tgt = [t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)]
+        gt = render_frames(*tgt, None, frames, cfg, hint=CapacityHint())[0].view(F, H, W).clone()
+    del tgt
+
+    def logit(p):
+        return torch.log(p / (1 - p))
+
+    means = torch.nn.Parameter(scene.means.to(dev))
+    tril = torch.nn.Parameter(scene.tril.to(dev))
+    raw_o = torch.nn.Parameter(logit(scene.opacity.clamp(1e-6, 1 - 1e-6)).to(dev))
+    raw_c = torch.nn.Parameter(logit(scene.intensity.clamp(1e-6, 1 - 1e-6)).to(dev))
+    init = [p.detach() for p in (means, tril, raw_o, raw_c)]
+
+    torch.cuda.reset_peak_memory_stats(dev)
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9)
+    dt_b = timed(trainer.step, args.steps, args.warmup)
+    if trainer.overflowed():  # sticky over every replay since the workspace was allocated
+        raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
+    fps_b = F * args.steps / dt_b
+    peak = torch.cuda.max_memory_allocated(dev)
+    ws_bytes = trainer.workspace.numel()
+    log(f"slice batched ({'eager' if args.eager else 'graph'}): {dt_b / args.steps * 1e3:.2f} ms/step, "
+        f"{fps_b:.1f} frames/s, loss {float(trainer.losses().mean()):.5f}, workspace {ws_bytes / 2**30:.2f} GiB")
+    del trainer
+    torch.cuda.empty_cache()
+
+    # per-kernel durations of the same launch sequence (eager, HIP event pair around every launch)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9)
+    tt.step()
+
+    def run():
+        for _ in range(args.timing_steps):
+            tt.step()
+
+    stages = stage_timing(run)
+    K = int(read_stats(tt.workspace).n_instances)
+    g_vis = visible_pairs(tt)
+    P = F * H * W
+    del tt
+    torch.cuda.empty_cache()
+
+    fps_p = None
+    if not args.no_plain:
+        opt = torch.optim.Adam([means, tril, raw_o, raw_c], lr=1e-3)
+
+        def step_plain():
+            opt.zero_grad(set_to_none=True)
+            for i, fr in enumerate(frames):
+                img = render_frames(means, tril, torch.sigmoid(raw_o), torch.sigmoid(raw_c), None, [fr], cfg,
+                                    hint=hint)[0]
+                (frame_losses(img.view(1, H, W), gt[i:i + 1]).sum() / F).backward()
+            opt.step()
+
+        dt_p = timed(step_plain, args.plain_steps, 1)
+        fps_p = F * args.plain_steps / dt_p
+        log(f"slice plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s")
+
+    launch_bytes = {
+        # algorithmic HBM bytes per launch over the batch: the per-frame terms 96 G + 80 G_vis + 96 K + 24 P of
+        # the slice splat split by the kernel that moves them (K tile instances, P pixels, G_vis visible pairs)
+        "ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis,
+        "ug_forward_kernel": 36 * K + 16 * P,
+        "sort_small_segments_kernel": 24 * K,
+        "ug_emit_kernel": 4 * F * G + 28 * g_vis + 12 * K,
+        "ug_preprocess_kernel": 48 * G + 32 * F * G + 4 * K,
+        "ug_reduce_kernel": 48 * K + 48 * G,
+    }
+    tag = f"{F}x{H}x{W}_g{G}"
+    full = (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
+    key = (lambda k: {"ug_backward_kernel": f"ug_backward_kernel<{ts}, false, true>",
+                      "ug_forward_kernel": f"ug_forward_kernel<{ts}, false, true>"}.get(k, k)) if full else None
+    roof = roofline(stages, launch_bytes, key, tag)
+    roof["measured_copy_GBs"] = measure_copy_peak()
+    return {
+        "metric": METRIC, "value": fps_b, "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": dt_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 120 mm cube, {F} freehand frames "
+                               f"{H}x{W} at 0.2 mm, L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} "
+                               f"step, tile {ts}",
+                   "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis},
+        "plain_frames_per_s": fps_p, "batched_over_plain": (fps_b / fps_p) if fps_p else None,
+        "iters_per_s": args.steps / dt_b, "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes),
+        "roofline": roof,
+    }
 
 
 def ray_cpu_baseline(scene, frame, cfg, n_rays: int):
@@ -231,16 +508,13 @@ def ray_cpu_baseline(scene, frame, cfg, n_rays: int):
     params = [n(t) for t in scene.params()]
     t0 = time.perf_counter()
     rm = our.map_frame(params[0], params[1], fr, st)
-    # keep the hit lists of the sampled rays only (rays W/2 - n/2 ...), empty lists elsewhere
     r0 = W // 2 - n_rays // 2
     keep = np.zeros(W, bool)
     keep[r0:r0 + n_rays] = True
-    counts = np.where(keep, rm.counts, 0).astype(np.int32)
-    rm.counts = counts
-    hits = int(counts.sum())
+    rm.counts = np.where(keep, rm.counts, 0).astype(np.int32)
+    hits = int(rm.counts.sum())
     img, _aux = our.forward(params, fr, rm, st)
-    d_img = np.sign(img).astype(np.float32)
-    our.backward_f64(params, fr, rm, st, d_img)
+    our.backward_f64(params, fr, rm, st, np.sign(img).astype(np.float32))
     dt = time.perf_counter() - t0
     return {"value": (n_rays / W) / dt, "unit": "frames/s", "cores": 1, "kind": "port",
             "sample": f"{n_rays} of {W} scanlines x {frame.height} samples of the middle frame ({hits} ray-Gaussian "
@@ -252,7 +526,6 @@ def bench_ray(args):
     """UltraG-Ray B-mode forward+backward over a batch of frames (200k Gaussians, 128 x 1024 scanline frames)."""
     import torch
 
-    from us3d import _lib
     from us3d.batch import CapacityHint, FrameSpec, SweepSpec
     from us3d.scenes import config3, make_ray_scene
     from us3d.ultragray import URBatch, UltragrayRenderConfig
@@ -278,6 +551,7 @@ def bench_ray(args):
         image, aux = batch.forward(m, pix, ro, rd, off)
         return m, image, aux
 
+    torch.cuda.reset_peak_memory_stats(dev)
     batch = URBatch(specs, [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
     with torch.no_grad():
         _m, gt, _a = run(batch, tparams)
@@ -285,31 +559,21 @@ def bench_ray(args):
 
     def step():
         m, image, aux = run(batch, params)
-        d_image = torch.sign(image - gt)  # L1 (sum) loss gradient
-        return batch.backward(m, pix, ro, rd, off, aux, d_image)
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
+        return batch.backward(m, pix, ro, rd, off, aux, torch.sign(image - gt))  # L1 (sum) loss gradient
 
     dt_b = timed(step, args.steps, args.warmup)
     fps_b = F * args.steps / dt_b
-    print(f"[bench] ray batched: {dt_b / args.steps * 1e3:.1f} ms/step, {fps_b:.1f} frames/s", file=sys.stderr,
-          flush=True)
-    _lib.stage_timing_read()
-    _lib.stage_timing(True)
-    m, image, aux = run(batch, params)
-    batch.backward(m, pix, ro, rd, off, aux, torch.sign(image - gt))
-    torch.cuda.synchronize()
-    _lib.stage_timing(False)
-    stages = _lib.stage_timing_read()
-    K = int(m.stats.n_instances)
+    peak = torch.cuda.max_memory_allocated(dev)
+    log(f"ray batched: {dt_b / args.steps * 1e3:.1f} ms/step, {fps_b:.1f} frames/s")
+    holder = {}
+
+    def once():
+        m, image, aux = run(batch, params)
+        batch.backward(m, pix, ro, rd, off, aux, torch.sign(image - gt))
+        holder["K"] = int(m.stats.n_instances)
+
+    stages = stage_timing(once)
+    K = holder["K"]
     P = F * H * W
 
     fps_p = None
@@ -325,19 +589,13 @@ def bench_ray(args):
 
         dt_p = timed(step_plain, args.plain_steps, 1)
         fps_p = F * args.plain_steps / dt_p
-        print(f"[bench] ray plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s",
-              file=sys.stderr, flush=True)
+        log(f"ray plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s")
 
-    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / max(launches, 1)
     launch_bytes = {"ur_forward_kernel": 64 * K + 8 * P, "ur_backward_kernel": 64 * K + 8 * P}
-    alg = launch_bytes.get(dom)
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
-    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = ray_cpu_baseline(scene, frames[F // 2], cfg, args.cpu_sample_rays)
-    line = {
+    full = (F, G, W, H) == (128, 200_000, 128, 1024)
+    roof = roofline(stages, launch_bytes, (lambda k: k) if full else None, f"ray_{F}x{W}x{H}_g{G}")
+    cpu = None if args.no_cpu_baseline else ray_cpu_baseline(scene, frames[F // 2], cfg, args.cpu_sample_rays)
+    return {
         "metric": "UltraG-Ray fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
         "value": fps_b, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt_b / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -346,15 +604,8 @@ def bench_ray(args):
                                f"freehand frames of {W} scanlines x {H} samples (0.8 mm x 0.1 mm), L1 (sum) loss "
                                f"gradient, batched eager", "frames": F, "gaussians": G, "ray_hits": K},
         "plain_frames_per_s": fps_p, "batched_over_plain": (fps_b / fps_p) if fps_p else None,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": read_traffic(dom, f"ray_{F}x{W}x{H}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
-                     "algorithmic_bytes_per_launch": alg, "stage_ms_per_launch": stage_ms,
-                     "measured_copy_GBs": measure_copy_peak(),
-                     "valu_issue_frac": read_valu_issue(dom) if (F, G, W, H) == (128, 200_000, 128, 1024) else None},
-        "cpu_baseline": cpu,
+        "peak_hbm_bytes": int(peak), "roofline": roof, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
 
 
 def bench_fan(args):
@@ -363,7 +614,6 @@ def bench_fan(args):
     frames of the sweep (pixel grid evaluated in the kernels); reports frames/s and peak HBM."""
     import torch
 
-    from us3d import _lib
     from us3d.batch import CapacityHint, read_stats
     from us3d.scenes import config5, fan_pixels
     from us3d.train_step import SliceSplatTrainer
@@ -385,18 +635,9 @@ def bench_fan(args):
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
                            frames, cfg, hint=CapacityHint())[0].clone()
+    torch.cuda.empty_cache()
     logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
 
     trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9)
     dt = timed(trainer.step, args.steps, args.warmup)
@@ -404,31 +645,19 @@ def bench_fan(args):
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
     fps = F * args.steps / dt
     peak = torch.cuda.max_memory_allocated(dev)
-    print(f"[bench] fan batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s, peak HBM "
-          f"{peak / 2**30:.1f} GiB", file=sys.stderr, flush=True)
+    ws_bytes = trainer.workspace.numel()
+    log(f"fan batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s, peak HBM {peak / 2**30:.1f} GiB")
     del trainer
     torch.cuda.empty_cache()
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9)
     tt.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing_read()
-    _lib.stage_timing(True)
-    tt.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing(False)
-    stages = _lib.stage_timing_read()
-    st = read_stats(tt.workspace)
-    off = tt.batch.layout_offsets(tt.k_capacity)
-    lay = tt.batch.layout
-    g_vis = int((tt.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
-    K = int(st.n_instances)
+    stages = stage_timing(tt.step)
+    K = int(read_stats(tt.workspace).n_instances)
+    g_vis = visible_pairs(tt)
     del tt
-    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / max(launches, 1)
+    torch.cuda.empty_cache()
     launch_bytes = {"ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis, "ug_forward_kernel": 36 * K + 16 * P}
-    alg = launch_bytes.get(dom)
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
-    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    roof = roofline(stages, launch_bytes, None, "")
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import fallback as ofb
@@ -437,19 +666,16 @@ def bench_fan(args):
         fr = frames[F // 2]
         rows = args.fan_cpu_rows
         pix = fan_pixels(fr).reshape(H, W, 3)[H // 2 - rows // 2:H // 2 - rows // 2 + rows].reshape(-1, 3)
-        ts = [t.clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
         with torch.no_grad():
             ref, _ = ofb.render(target.means, target.tril, target.opacity, target.intensity, pix, rows, W)
-        t0 = time.perf_counter()
-        img, _ = ofb.render(*ts, pix, rows, W)
-        (img - ref).abs().mean().backward()
-        dtc = time.perf_counter() - t0
-        cpu = {"value": rows * W / (H * W) / dtc, "unit": "frames/s", "cores": torch.get_num_threads(), "kind": "port",
+        dtc = _fallback_fwd_bwd(scene, ref, pix, rows, W)
+        cpu = {"value": rows * W / (H * W) / dtc, "unit": "frames/s", "cores": torch.get_num_threads(),
+               "kind": "port",
                "sample": f"{rows} rows x {W} pixels of fan frame {f0 + F // 2} against all {G} Gaussians, forward + "
                          f"autograd backward of the restated PyTorch fallback renderer, {dtc:.1f} s "
                          f"({rows * W * G / dtc / 1e6:.0f} M pair evaluations/s); frames/s = sampled fraction of a "
                          f"{W}x{H} frame / time"}
-    line = {
+    return {
         "metric": "fwd+bwd frames/s of a fan-probe batch in one captured training step on 1 GPU; % of HBM roofline",
         "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -459,14 +685,8 @@ def bench_fan(args):
                                f"40-160 mm), L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} step, "
                                f"tile 16, pixel grid evaluated in the kernels",
                    "frames": F, "gaussians": G, "pixels": P, "instances": K, "visible_pairs": g_vis},
-        "peak_hbm_bytes": int(peak),
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
-                     "stage_ms_per_launch": stage_ms},
-        "cpu_baseline": cpu,
+        "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes), "roofline": roof, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
 
 
 def bench_gaussianpile(args):
@@ -477,7 +697,6 @@ def bench_gaussianpile(args):
     activations, torch autograd and torch.optim.Adam."""
     import torch
 
-    from us3d import _lib
     from us3d.batch import CapacityHint, SweepSpec, read_stats
     from us3d.gaussianpile import GaussianPlieRenderConfig, GPBatch, render_frames
     from us3d.scenes import config2
@@ -494,6 +713,7 @@ def bench_gaussianpile(args):
                                    axial_spacing_mm_per_pix=fr0.delta_ax, axial_near_mm=fr0.near_mm,
                                    lateral_col_offset=fr0.col_offset, tile_size_px=args.tile_size)
     hint = CapacityHint()
+    torch.cuda.reset_peak_memory_stats(dev)
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.scales, target.quats, target.opacity)], frames,
                            cfg, hint=hint).clone()
@@ -501,23 +721,14 @@ def bench_gaussianpile(args):
     raw = [t.to(dev).float().contiguous() for t in (scene.means, torch.log(scene.scales), scene.quats,
                                                      torch.log(op / (1 - op)))]
     tr = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=not args.eager, check_every=10 ** 9)
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
     dt = timed(tr.step, args.steps, args.warmup)
     if tr.overflowed():  # sticky over every replay since the workspace was allocated
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
     fps = F * args.steps / dt
-    print(f"[bench] gaussianpile batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s", file=sys.stderr,
-          flush=True)
+    peak = torch.cuda.max_memory_allocated(dev)
+    log(f"gaussianpile batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s")
+    del tr
+    torch.cuda.empty_cache()
     fps_plain = None
     if not args.no_plain:
         n_plain = min(8, F)
@@ -537,32 +748,20 @@ def bench_gaussianpile(args):
 
         dtp = timed(plain, 1, 1)
         fps_plain = n_plain / dtp
-        print(f"[bench] gaussianpile plain (per frame): {fps_plain:.1f} frames/s", file=sys.stderr, flush=True)
-    # stage timing: one eager step of a second trainer (HIP events around each launch)
-    tr_t = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=False)
+        log(f"gaussianpile plain (per frame): {fps_plain:.1f} frames/s")
+    tr_t = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=False, check_every=10 ** 9)
     tr_t.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing_read()
-    _lib.stage_timing(True)
-    tr_t.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing(False)
-    stages = _lib.stage_timing_read()
-    st = read_stats(tr_t.workspace)
-    off = tr_t.batch.layout_offsets(tr_t.k_capacity)
-    lay = tr_t.batch.layout
-    g_vis = int((tr_t.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
-    K = int(st.n_instances)
+    stages = stage_timing(tr_t.step)
+    K = int(read_stats(tr_t.workspace).n_instances)
+    g_vis = visible_pairs(tr_t)
     del tr_t
-    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / max(launches, 1)
+    torch.cuda.empty_cache()
     # algorithmic bytes per launch: per instance its 8 B sort key, 32 B footprint and 4 B intensity gathered
     # (adjoint: also its 4 B emit index read and its 32 B record-space gradient written), per pixel the 4 B
     # image written (forward) or its 4 B gradient read (adjoint)
     launch_bytes = {"gp_forward_kernel": 44 * K + 4 * P, "gp_backward_kernel": 80 * K + 4 * P}
-    alg = launch_bytes.get(dom)
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
-    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
+    full = (F, G, H, W) == (256, 200_000, 512, 512)
+    roof = roofline(stages, launch_bytes, (lambda k: f"{k}<{args.tile_size}>") if full else None, f"{F}x{H}x{W}_g{G}")
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import gp as ogp
@@ -583,7 +782,7 @@ def bench_gaussianpile(args):
                "sample": f"{rows} rows x {W} pixels (two tile rows) of frame {F // 2} against all {G} Gaussians: the "
                          f"oracle's tile lists and forward, FP64 with torch autograd for the adjoint, {dtc:.1f} s; "
                          f"frames/s = sampled fraction of a frame / time"}
-    line = {
+    return {
         "metric": "GaussianPile us_focus training-step frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
         "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -593,17 +792,9 @@ def bench_gaussianpile(args):
                                f"{H}x{W} at 0.2 mm, radius_sigma 3, sigma_z 1 mm, batched "
                                f"{'eager' if args.eager else 'hipGraph'}, tile {args.tile_size}",
                    "frames": F, "gaussians": G, "instances": K, "visible_pairs": g_vis},
-        "plain_frames_per_s": fps_plain,
-        "batched_over_plain": (fps / fps_plain) if fps_plain else None,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}") if dom in launch_bytes else None,
-                     "valu_issue_frac": read_valu_issue(f"{dom}<{args.tile_size}>"),
-                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
-                     "stage_ms_per_launch": stage_ms},
-        "cpu_baseline": cpu,
+        "plain_frames_per_s": fps_plain, "batched_over_plain": (fps / fps_plain) if fps_plain else None,
+        "peak_hbm_bytes": int(peak), "roofline": roof, "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
 
 
 def bench_multisweep(args):
@@ -613,7 +804,6 @@ def bench_multisweep(args):
 
     import torch
 
-    from us3d import _lib
     from us3d.autotune import tune_tile_size
     from us3d.batch import CapacityHint, read_stats
     from us3d.scenes import config4, config4_specs
@@ -629,57 +819,39 @@ def bench_multisweep(args):
     f0 = frames[0]
     cfg = UltraGaussRenderConfig(H=f0.height, W=f0.width, lateral_spacing_mm_per_pix=0.2,
                                  axial_spacing_mm_per_pix=0.2, axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=16)
+    torch.cuda.reset_peak_memory_stats(dev)
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
                            frames, cfg, sweeps=sweeps, hint=CapacityHint())[0].clone()
     del target
+    torch.cuda.empty_cache()
     logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
     tune = tune_tile_size(*init, frames, gt, cfg, sample_frames=32, reps=2, sweeps=sweeps, use_cache=False)
     cfg = replace(cfg, tile_size=int(tune["tile_size"]))
-    print(f"[bench] autotuner: {tune}", file=sys.stderr, flush=True)
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
+    log(f"multisweep autotuner: {tune}")
+    torch.cuda.empty_cache()
     trainer = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=not args.eager, check_every=10 ** 9)
     dt = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
     fps = F * args.steps / dt
-    print(f"[bench] multisweep batched ({'eager' if args.eager else 'graph'}): {dt / args.steps * 1e3:.1f} ms/step, "
-          f"{fps:.1f} frames/s", file=sys.stderr, flush=True)
+    peak = torch.cuda.max_memory_allocated(dev)
+    ws_bytes = trainer.workspace.numel()
+    log(f"multisweep batched ({'eager' if args.eager else 'graph'}): {dt / args.steps * 1e3:.1f} ms/step, "
+        f"{fps:.1f} frames/s, workspace {ws_bytes / 2**30:.1f} GiB")
     del trainer
     torch.cuda.empty_cache()
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False, check_every=10 ** 9)
     tt.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing_read()
-    _lib.stage_timing(True)
-    tt.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing(False)
-    stages = _lib.stage_timing_read()
-    st = read_stats(tt.workspace)
-    off = tt.batch.layout_offsets(tt.k_capacity)
-    lay = tt.batch.layout
-    g_vis = int((tt.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32) > 0).sum().item())
-    K = int(st.n_instances)
+    stages = stage_timing(tt.step)
+    K = int(read_stats(tt.workspace).n_instances)
+    g_vis = visible_pairs(tt)
     del tt
-    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / max(launches, 1)
+    torch.cuda.empty_cache()
     launch_bytes = {"ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis, "ug_forward_kernel": 36 * K + 16 * P}
-    alg = launch_bytes.get(dom)
-    achieved = alg / (avg_ms * 1e-3) / 1e9 if alg else None
-    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
-    line = {
+    roof = roofline(stages, launch_bytes, None, "")
+    return {
         "metric": "fwd+bwd frames/s of a multi-sweep batch in one captured training step on 1 GPU; % of HBM roofline",
         "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -689,20 +861,49 @@ def bench_multisweep(args):
                                f"{'eager' if args.eager else 'hipGraph'} step, tile {cfg.tile_size} (autotuned)",
                    "frames": F, "gaussians": G, "pixels": P, "tile_size": cfg.tile_size, "instances": K,
                    "visible_pairs": g_vis, "autotune_ms_per_frame": tune["ms_per_frame"]},
-        "megapixels_per_s": P * args.steps / dt / 1e6,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "avg_launch_ms": round(avg_ms, 4), "algorithmic_bytes_per_launch": alg,
-                     "stage_ms_per_launch": stage_ms},
-        "cpu_baseline": None,
+        "megapixels_per_s": P * args.steps / dt / 1e6, "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes),
+        "roofline": roof, "cpu_baseline": None,
     }
-    print(json.dumps(line), flush=True)
+
+
+WORKLOADS = {"ray": bench_ray, "multisweep": bench_multisweep, "fan": bench_fan, "gaussianpile": bench_gaussianpile}
+
+
+def nested(name, args):
+    """One of the other workloads, short (1 warm-up, 3 timed steps, no plain or CPU legs), as a compact object."""
+    import torch
+
+    sub = copy.copy(args)
+    sub.steps, sub.warmup, sub.no_plain, sub.no_cpu_baseline = 3, 1, True, True
+    t0 = time.perf_counter()
+    try:
+        d = WORKLOADS[name](sub)
+    except Exception as e:  # reported in the line; the headline stands on its own
+        torch.cuda.empty_cache()
+        return {"error": f"{type(e).__name__}: {e}"}
+    torch.cuda.empty_cache()
+    r = d["roofline"]
+    out = {"metric": d["metric"], "value": d["value"], "unit": d["unit"], "ms_per_step": d["ms_per_step"],
+           "steps": sub.steps, "warmup": sub.warmup, "workload": d["config"]["workload"],
+           "frames": d["config"].get("frames"), "gaussians": d["config"].get("gaussians"),
+           "dominant_kernel": r["kernel"], "dominant_avg_launch_ms": r["avg_launch_ms"], "roofline_frac": r["frac"],
+           "stage_ms_per_launch": r["stage_ms_per_launch"], "peak_hbm_bytes": d.get("peak_hbm_bytes"),
+           "wall_s": round(time.perf_counter() - t0, 1)}
+    for k in ("workspace_bytes", "megapixels_per_s"):
+        if k in d:
+            out[k] = d[k]
+    if "tile_size" in d["config"]:
+        out["tile_size"] = d["config"]["tile_size"]
+    return out
 
 
 def main():
     args = parse()
     if args.rocprof_summary:
         rocprof_summary(*args.rocprof_summary)
+        return
+    if args.rocprof_derive:
+        rocprof_derive(args.rocprof_derive[0], args.rocprof_derive[1:])
         return
     if args.copy_calibration:
         copy_calibration()
@@ -711,154 +912,16 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU figures are not produced",
                           "n_gpus_requested": args.gpus}))
         sys.exit(2)
-    if args.workload == "ray":
-        bench_ray(args)
+    if args.workload != "slice":
+        print(json.dumps(WORKLOADS[args.workload](args)), flush=True)
         return
-    if args.workload == "multisweep":
-        bench_multisweep(args)
-        return
-    if args.workload == "fan":
-        bench_fan(args)
-        return
-    if args.workload == "gaussianpile":
-        bench_gaussianpile(args)
-        return
-    import torch
-
-    from us3d import _lib
-    from us3d.batch import CapacityHint, read_stats
-    from us3d.scenes import config2
-    from us3d.train_step import SliceSplatTrainer
-    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
-
-    dev = torch.device("cuda:0")
-    F, G, ts = args.frames, args.gaussians, args.tile_size
-    scene, target, frames = config2(n_frames=F, n_gauss=G)
-    H, W = frames[0].height, frames[0].width
-    cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=0.2, axial_spacing_mm_per_pix=0.2,
-                                 axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=ts)
-    hint = CapacityHint()
-    with torch.no_grad():
-        tgt = [t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)]
-        gt = render_frames(*tgt, None, frames, cfg, hint=CapacityHint())[0].view(F, H, W).clone()
-
-    def logit(p):
-        return torch.log(p / (1 - p))
-
-    means = torch.nn.Parameter(scene.means.to(dev))
-    tril = torch.nn.Parameter(scene.tril.to(dev))
-    raw_o = torch.nn.Parameter(logit(scene.opacity.clamp(1e-6, 1 - 1e-6)).to(dev))
-    raw_c = torch.nn.Parameter(logit(scene.intensity.clamp(1e-6, 1 - 1e-6)).to(dev))
-
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return time.perf_counter() - t0
-
-    # batched: the whole sweep per step, one captured hipGraph (map, forward, loss, adjoint, Adam)
-    trainer = SliceSplatTrainer(means.detach(), tril.detach(), raw_o.detach(), raw_c.detach(), frames, gt, cfg,
-                                use_graph=not args.eager, check_every=10 ** 9)
-    dt_b = timed(trainer.step, args.steps, args.warmup)
-    if trainer.overflowed():
-        raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
-    fps_b = F * args.steps / dt_b
-    print(f"[bench] batched ({'eager' if args.eager else 'graph'}): {dt_b / args.steps * 1e3:.1f} ms/step, "
-          f"{fps_b:.1f} frames/s, loss {float(trainer.losses().mean()):.5f}", file=sys.stderr, flush=True)
-
-    # per-kernel durations of the same launch sequence (eager, HIP event pair around every launch)
-    timing_trainer = SliceSplatTrainer(means.detach(), tril.detach(), raw_o.detach(), raw_c.detach(), frames, gt, cfg,
-                                       use_graph=False, check_every=10 ** 9)
-    timing_trainer.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing_read()
-    _lib.stage_timing(True)
-    for _ in range(args.timing_steps):
-        timing_trainer.step()
-    torch.cuda.synchronize()
-    _lib.stage_timing(False)
-    stages = _lib.stage_timing_read()
-    st = read_stats(timing_trainer.workspace)
-    off = timing_trainer.batch.layout_offsets(timing_trainer.k_capacity)
-    lay = timing_trainer.batch.layout
-    pair_count = timing_trainer.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32)
-    g_vis = int((pair_count > 0).sum().item())
-    K = int(st.n_instances)
-    P = F * H * W
-    del timing_trainer
-
-    fps_p = None
-    if not args.no_plain:
-        opt = torch.optim.Adam([means, tril, raw_o, raw_c], lr=1e-3)
-
-        def step_plain():
-            opt.zero_grad(set_to_none=True)
-            for i, fr in enumerate(frames):
-                img = render_frames(means, tril, torch.sigmoid(raw_o), torch.sigmoid(raw_c), None, [fr], cfg,
-                                    hint=hint)[0]
-                (frame_losses(img.view(1, H, W), gt[i:i + 1]).sum() / F).backward()
-            opt.step()
-
-        dt_p = timed(step_plain, args.plain_steps, 1)
-        fps_p = F * args.plain_steps / dt_p
-        print(f"[bench] plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s", file=sys.stderr,
-              flush=True)
-
-    # roofline of the dominant kernel (largest total time in the timed batched region)
-    dom, (launches, total_ms) = max(stages.items(), key=lambda kv: kv[1][1])
-    avg_ms = total_ms / max(launches, 1)
-    launch_bytes = {
-        # algorithmic HBM bytes per launch over the batch: the per-frame terms 96 G + 80 G_vis + 96 K + 24 P of
-        # the slice splat split by the kernel that moves them (K instances, P pixels, G_vis visible pairs)
-        "ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis,
-        "ug_forward_kernel": 36 * K + 16 * P,
-        "sort_small_segments_kernel": 24 * K,
-        "ug_emit_kernel": 4 * F * G + 28 * g_vis + 12 * K,
-        "ug_preprocess_kernel": 48 * G + 32 * F * G + 4 * K,
-        "ug_reduce_kernel": 48 * K + 48 * G,
-    }
-    alg_bytes = launch_bytes.get(dom)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9 if alg_bytes else None
-    stage_ms = {k: round(v[1] / max(v[0], 1), 3) for k, v in sorted(stages.items(), key=lambda kv: -kv[1][1])}
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                "traffic": read_traffic(dom, f"{F}x{H}x{W}_g{G}"), "avg_launch_ms": round(avg_ms, 4),
-                "algorithmic_bytes_per_launch": alg_bytes, "stage_ms_per_launch": stage_ms,
-                "measured_copy_GBs": measure_copy_peak(),
-                "valu_issue_frac": read_valu_issue(f"{dom}<{ts}, false>") if (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
-                else None}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = cpu_baseline(args.cpu_sample_rows)
-
-    line = {
-        "metric": METRIC,
-        "value": fps_b,
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt_b / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 120 mm cube, {F} freehand frames "
-                               f"{H}x{W} at 0.2 mm, L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} "
-                               f"step, tile {ts}",
-                   "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis},
-        "plain_frames_per_s": fps_p,
-        "batched_over_plain": (fps_b / fps_p) if fps_p else None,
-        "iters_per_s": args.steps / dt_b,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    line = bench_slice(args)
+    if not args.no_workloads:
+        line["workloads"] = {}
+        for name in ("ray", "multisweep", "fan", "gaussianpile"):
+            log(f"nested workload {name} ...")
+            line["workloads"][name] = nested(name, args)
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_rows, args.cpu_runs)
     print(json.dumps(line), flush=True)
 
 
