@@ -68,7 +68,9 @@ typedef struct us3d_frame {
 
 /* A sweep: a set of frames rendered from one Gaussian set. Gaussians [g_begin, g_begin+g_count) of the
  * concatenated parameter arrays; frames [f_begin, f_begin+f_count). (frame, Gaussian) pairs of a sweep are
- * numbered Gaussian-major from pair_base: pair = pair_base + (g - g_begin) * f_count + (f - f_begin). */
+ * numbered Gaussian-major from pair_base: pair = pair_base + (g - g_begin) * f_count + (f - f_begin).
+ * The sweep table is ordered by g_begin and the Gaussian ranges are disjoint; they need not cover all n_gauss
+ * Gaussians: one outside every range is rendered in no frame of the batch and its gradients are zero. */
 typedef struct us3d_sweep {
     int32_t g_begin;
     int32_t g_count;

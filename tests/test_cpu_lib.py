@@ -91,6 +91,30 @@ def test_plan_layout_numbering():
         plan_layout([_frame(sweep=1), _frame(sweep=0)], sweeps, 150, 16)
     with pytest.raises(ValueError):
         plan_layout(frames, [SweepSpec(0, 100), SweepSpec(100, 51)], 150, 16)
+    # the sweep table is ordered by g_begin with disjoint ranges; gaps are allowed (those Gaussians are in no frame)
+    with pytest.raises(ValueError):
+        plan_layout(frames, [SweepSpec(100, 50), SweepSpec(0, 100)], 150, 16)
+    with pytest.raises(ValueError):
+        plan_layout(frames, [SweepSpec(0, 100), SweepSpec(99, 50)], 150, 16)
+    gap = plan_layout(frames, [SweepSpec(10, 80), SweepSpec(120, 20)], 150, 16)
+    assert gap.n_pairs == 80 * 2 + 20 * 1
+
+
+def test_chunks_of_a_step_keep_their_sweeps_in_order():
+    """frame_chunks splits a step's frames into runs; each run keeps the sweeps its frames use, renumbered in the
+    original order, so its sweep table is ordered by g_begin (Gaussians of the other sweeps are in no frame of the
+    run)."""
+    from us3d.train_step import frame_chunks
+
+    frames = [_frame(sweep=0)] * 3 + [_frame(sweep=1)] * 2 + [_frame(sweep=2)]
+    sweeps = [SweepSpec(0, 100), SweepSpec(100, 50), SweepSpec(150, 30)]
+    out = frame_chunks(frames, sweeps, 2)
+    assert [(a, b) for a, b, _f, _s in out] == [(0, 2), (2, 4), (4, 6)]
+    assert [[f.sweep for f in fs] for _a, _b, fs, _s in out] == [[0, 0], [0, 1], [0, 1]]
+    assert [[s.g_begin for s in ss] for _a, _b, _f, ss in out] == [[0], [0, 100], [100, 150]]
+    for _a, _b, fs, ss in out:
+        plan_layout(fs, ss, 180, 16)
+    assert [(a, b) for a, b, _f, _s in frame_chunks(frames, sweeps, None)] == [(0, 6)]
 
 
 def test_frame_table_bits():

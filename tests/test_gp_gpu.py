@@ -328,3 +328,25 @@ def test_gp_two_sweeps_in_one_batch_equal_separate_batches(dev):
     (ib * w[3 * n:]).sum().backward()
     for t, a, b in zip(both, sep_a, sep_b):
         assert torch.equal(t.grad[:400], a.grad) and torch.equal(t.grad[400:], b.grad)
+
+
+def test_gp_sweep_table_that_leaves_gaussians_out(dev):
+    """Only the second sweep's frames and Gaussian range [400, 700) in the batch: rows 0..399 belong to no sweep
+    of it. The image equals that sweep's own batch bit for bit, its Gaussians' gradients equal that batch's, and
+    the other rows get exactly zero."""
+    fb = [FrameSpec(**{**f.__dict__, "sweep": 0}) for f in _frames(2, seed=32)]
+    pa = _scene(_frames(3, seed=31)[1], G=400, seed=33)
+    pb = _scene(fb[0], G=300, seed=34)
+    cfg = _cfg(fb[0])
+    both = [torch.cat([a, b]).to(dev).requires_grad_(True) for a, b in zip(pa[:4], pb[:4])]
+    img = render_frames(*both, fb, cfg, sweeps=[SweepSpec(400, 300)])
+    w = torch.linspace(-1.0, 1.0, img.numel(), device=dev)
+    (img * w).sum().backward()
+    sep = [t.to(dev).requires_grad_(True) for t in pb[:4]]
+    ib = render_frames(*sep, fb, cfg)
+    (ib * w).sum().backward()
+    assert float(ib.detach().abs().max()) > 0.05
+    assert torch.equal(img, ib)
+    for t, b in zip(both, sep):
+        assert float(b.grad.abs().max()) > 0
+        assert torch.equal(t.grad[400:], b.grad) and not bool(t.grad[:400].any())

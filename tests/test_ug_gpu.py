@@ -274,6 +274,38 @@ def test_multi_sweep_batch_equals_per_sweep(dev):
         off += n
 
 
+def test_sweeps_that_leave_gaussians_out(dev):
+    """A batch whose sweep table leaves Gaussians out (here only the middle sweep's frames and range, so rows
+    before and after it belong to no sweep of the batch, as in a chunk of a training step): the image equals that
+    sweep rendered on its own bit for bit, its Gaussians' gradients equal that render's, and every Gaussian outside
+    the range gets exactly zero (the per-Gaussian reduction finds no sweep for it and never indexes the pair
+    table)."""
+    from dataclasses import replace
+
+    from us3d.scenes import multi_sweep
+    from us3d.ultragauss import render_frames
+
+    specs = [(700, 2, 64, 48), (300, 2, 32, 80), (500, 1, 48, 48)]
+    scene, _target, frames, sweeps = multi_sweep(specs, cube_mm=64.0, seed=9, length_mm=30.0, spacing=0.3)
+    cfg = cfg_for(frames[0])
+    sw = sweeps[1]
+    fs = [replace(f, sweep=0) for f in frames if f.sweep == 1]
+    P = sum(f.height * f.width for f in fs)
+    dI = torch.from_numpy(np.random.default_rng(5).normal(size=P).astype(np.float32)).to(dev)
+    full = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    image, _a, _d = render_frames(*full, None, fs, cfg, sweeps=[sw])
+    (image * dI).sum().backward()
+    sub = [t.detach()[sw.g_begin:sw.g_begin + sw.g_count].clone().requires_grad_(True) for t in full]
+    img_s, _a, _d = render_frames(*sub, None, fs, cfg, sweeps=[SweepSpec(0, sw.g_count)])
+    (img_s * dI).sum().backward()
+    assert float(img_s.detach().abs().max()) > 0.05
+    assert torch.equal(image.detach(), img_s.detach())
+    for a, b in zip(full, sub):
+        assert float(b.grad.abs().max()) > 0
+        assert torch.equal(a.grad[sw.g_begin:sw.g_begin + sw.g_count], b.grad)
+        assert not bool(a.grad[:sw.g_begin].any()) and not bool(a.grad[sw.g_begin + sw.g_count:].any())
+
+
 @pytest.mark.parametrize("ts", [16, 8])
 def test_mapper_long_tile_lists_bitexact(dev, ts):
     """Dense clusters put ~700, ~2500 and ~6000 Gaussians into single tiles, so every tile-sort path runs

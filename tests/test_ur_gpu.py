@@ -198,6 +198,41 @@ def test_ray_log_scales_are_activated_in_the_pack(dev):
         assert rel_l2(a.cpu().numpy(), b.cpu().numpy()) < 1e-5
 
 
+def test_ray_sweep_table_that_leaves_gaussians_out(dev):
+    """The scene's Gaussians sit at rows [100, 100 + G) of arrays padded with 100 rows before and 60 after, and
+    the sweep table names that range only: the image equals the unpadded batch's bit for bit, the scene rows'
+    gradients equal its gradients, and the padding rows (in no sweep of the batch) get exactly zero."""
+    import torch
+
+    from us3d.batch import CapacityHint, SweepSpec
+    from us3d.ultragray import URBatch
+    from tests.ur_helpers import ray_spec
+
+    scene, frames = small_ray_case(n_frames=2)
+    cfg = ray_cfg(frames[0])
+    specs = [ray_spec(f, cfg) for f in frames]
+    params = [t.to(dev).float().contiguous() for t in scene.params()]
+    G = params[0].shape[0]
+    pads = [(torch.rand((100,) + t.shape[1:], device=dev), torch.rand((60,) + t.shape[1:], device=dev))
+            for t in params]
+    padded = [torch.cat([a, t, b]).contiguous() for t, (a, b) in zip(params, pads)]
+    off = torch.zeros(sum(s.width for s in specs), device=dev)
+    plain = URBatch(specs, [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
+    gapped = URBatch(specs, [SweepSpec(100, G)], G + 160, cfg.settings(), dev, CapacityHint())
+    m0, m1 = plain.map(*params), gapped.map(*padded)
+    img0, aux0 = plain.forward(m0, None, None, None, off)
+    img1, aux1 = gapped.forward(m1, None, None, None, off)
+    assert float(img0.abs().max()) > 1e-3
+    assert torch.equal(img0, img1)
+    dB = torch.from_numpy((np.random.default_rng(6).normal(size=img0.numel()) * 1e-2).astype(np.float32)).to(dev)
+    g0 = plain.backward(m0, None, None, None, off, aux0, dB)
+    g1 = gapped.backward(m1, None, None, None, off, aux1, dB)
+    for a, b in zip(g0, g1):
+        assert float(a.abs().max()) > 0
+        assert torch.equal(a, b[100:100 + G])
+        assert not bool(b[:100].any()) and not bool(b[100 + G:].any())
+
+
 def test_ray_linear_probe_geometry_in_kernel_equals_explicit_arrays(dev):
     """pixels / ray_origins / ray_dirs = NULL: the kernels evaluate the linear probe's samples and rays from the
     frame table with the FP32 operations of the pipeline's grid (pipeline_god_file.py:292-333). Image, aux and

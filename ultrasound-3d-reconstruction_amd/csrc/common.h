@@ -129,7 +129,20 @@ __device__ __forceinline__ int sweep_of_pair(const us3d_sweep* __restrict__ swee
 __device__ __forceinline__ int sweep_of_gauss(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int g) {
     int s = 0;
     while (s + 1 < n_sweeps && sweeps[s + 1].g_begin <= g) ++s;
-    return s;
+    return (g >= sweeps[s].g_begin && g - sweeps[s].g_begin < sweeps[s].g_count) ? s : -1;
+}
+
+// The sweep of Gaussian g for a per-Gaussian reduction, with f_count = 0 when no sweep of the batch holds g
+// (a batch of some sweeps' frames, or ranges with gaps): such a Gaussian has no pairs, so its sums stay zero and
+// pair_base + (g - g_begin) * f_count is never formed outside the pair table.
+__device__ __forceinline__ us3d_sweep sweep_for_reduce(const us3d_sweep* __restrict__ sweeps, int n_sweeps, int g) {
+    const int s = sweep_of_gauss(sweeps, n_sweeps, g);
+    us3d_sweep sw = sweeps[s < 0 ? 0 : s];
+    if (s < 0) {
+        sw.g_begin = g;
+        sw.f_count = 0;
+    }
+    return sw;
 }
 
 // exp(-q/2) exactly as __expf(-0.5f * q) evaluates it (v_exp_f32 of (-0.5 q) * log2(e)_f32), with the two

@@ -606,7 +606,7 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
     const int g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (g >= n_gauss || stats->overflow) return;
-    const us3d_sweep sw = sweeps[sweep_of_gauss(sweeps, n_sweeps, g)];
+    const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
     const float4 r0 = rec[3 * (int64_t)g + 0], r1 = rec[3 * (int64_t)g + 1], r2 = rec[3 * (int64_t)g + 2];
     float acc[11];

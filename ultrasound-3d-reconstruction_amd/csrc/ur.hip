@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_
     const int g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     if (g >= n_gauss || stats->overflow) return;
-    const us3d_sweep sw = sweeps[sweep_of_gauss(sweeps, n_sweeps, g)];
+    const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
     const int e0 = pair_off[p0], e1 = pair_off[p0 + sw.f_count];
     float acc[16];

@@ -79,9 +79,15 @@ def plan_layout(frames: Sequence[FrameSpec], sweeps: Sequence[SweepSpec], n_gaus
         f_count[s] = f_count.get(s, 0) + 1
     rows = np.zeros((len(sweeps), 6), dtype=np.int32)
     pair_base = 0
+    g_end = 0
     for s, sw in enumerate(sweeps):
         if sw.g_begin < 0 or sw.g_count < 0 or sw.g_begin + sw.g_count > n_gauss:
             raise ValueError(f"sweep {s}: Gaussian range out of bounds")
+        # the kernels find a Gaussian's sweep by its position in the table; Gaussians in no range get zero
+        # gradients
+        if sw.g_begin < g_end:
+            raise ValueError(f"sweep {s}: Gaussian ranges must be disjoint and in ascending order")
+        g_end = sw.g_begin + sw.g_count
         fb, fc = f_begin.get(s, 0), f_count.get(s, 0)
         rows[s, :4] = (sw.g_begin, sw.g_count, fb, fc)
         rows[s, 4:6] = np.array([pair_base], dtype=np.int64).view(np.int32)

@@ -23,12 +23,38 @@ from .batch import CapacityHint, FrameSpec, SweepSpec, new_workspace, read_stats
 from .ultragauss import UGBatch, UltraGaussRenderConfig
 
 
+def frame_chunks(frames: Sequence[FrameSpec], sweeps: Sequence[SweepSpec], chunk_frames: Optional[int]):
+    """Consecutive runs of at most chunk_frames frames (all frames when None), each with its sweeps renumbered
+    compactly: [(frame begin, frame end, frames, sweeps)]."""
+    from dataclasses import replace
+
+    n = len(frames)
+    step = n if not chunk_frames else max(1, int(chunk_frames))
+    out = []
+    for a in range(0, n, step):
+        b = min(n, a + step)
+        used: List[int] = []
+        sub = []
+        for f in frames[a:b]:
+            if f.sweep not in used:
+                used.append(f.sweep)
+            sub.append(replace(f, sweep=used.index(f.sweep)))
+        out.append((a, b, sub, [sweeps[s] for s in used]))
+    return out
+
+
 class SliceSplatTrainer:
+    """The captured training step. With chunk_frames, the frames are rendered chunk by chunk inside the same step
+    (map -> forward -> loss -> adjoint per chunk of at most chunk_frames frames, the chunks' gradients summed, then
+    one Adam update), so the workspace holds one chunk's instances and pairs: memory is bounded by the chunk, not
+    the sweep. The loss is the mean over all frames either way, so the update is the same."""
+
     def __init__(self, means: torch.Tensor, tril: torch.Tensor, raw_opacity: torch.Tensor,
                  raw_intensity: torch.Tensor, frames: Sequence[FrameSpec], target: torch.Tensor,
                  cfg: UltraGaussRenderConfig, sweeps: Optional[Sequence[SweepSpec]] = None, lr: float = 1e-3,
                  betas=(0.9, 0.999), eps: float = 1e-8, w_l1: float = 1.0, w_ssim: float = 0.2,
-                 headroom: float = 1.3, use_graph: bool = True, check_every: int = 1):
+                 headroom: float = 1.3, use_graph: bool = True, check_every: int = 1,
+                 chunk_frames: Optional[int] = None):
         dev = means.device
         self.dev = dev
         G = means.shape[0]
@@ -46,62 +72,91 @@ class SliceSplatTrainer:
         self.cfg = cfg
         settings = cfg.settings()
         settings.raw_activations = 1  # opacity and intensity are passed as logits
-        self.batch = UGBatch(frames, sweeps or [SweepSpec(0, G)], G, settings, dev, CapacityHint())
-        lay = self.batch.layout
-        self.F = lay.n_frames
-        P = lay.n_pixels
+        sweeps = list(sweeps or [SweepSpec(0, G)])
+        self.F = len(frames)
+        self.chunks = []  # (frame begin, frame end, pixel begin, pixel end, UGBatch)
+        px0 = 0
+        for a, b, sub, sub_sweeps in frame_chunks(frames, sweeps, chunk_frames):
+            batch = UGBatch(sub, sub_sweeps, G, settings, dev, CapacityHint())
+            n = batch.layout.n_pixels
+            self.chunks.append((a, b, px0, px0 + n, batch))
+            px0 += n
+        self.batch = self.chunks[0][4]
+        P = px0
+        Pc = max(c[3] - c[2] for c in self.chunks)
         self.target = target.detach().to(**f32).reshape(P).contiguous()
-        self.image = torch.empty(P, **f32)
-        self.alpha = torch.empty(P, **f32)
-        self.aux = torch.empty((P, 4), **f32)
-        self.d_image = torch.empty(P, **f32)
+        self.image = torch.empty(Pc, **f32)
+        self.alpha = torch.empty(Pc, **f32)
+        self.aux = torch.empty((Pc, 4), **f32)
+        self.d_image = torch.empty(Pc, **f32)
         self.loss = torch.empty(self.F, **f32)
-        self.loss_tmp = torch.empty(int(_lib.lib().us3d_image_loss_tmp_bytes(self.F)) // 8, dtype=torch.float64,
+        fc = max(c[1] - c[0] for c in self.chunks)
+        self.loss_tmp = torch.empty(int(_lib.lib().us3d_image_loss_tmp_bytes(fc)) // 8, dtype=torch.float64,
                                     device=dev)
         self.grads = [torch.empty_like(p) for p in self.params]
+        self.grads_chunk = [torch.empty_like(p) for p in self.params] if len(self.chunks) > 1 else self.grads
+        self.skip = torch.zeros(1, dtype=torch.int32, device=dev)  # any chunk of the step overflowed
         self.headroom = headroom
         self.use_graph = use_graph
         self.check_every = max(1, int(check_every))
         self.n_steps = 0
         self.graph = None
-        # size the instance arrays from an eager mapping of the current parameters
+        self._allocate(self._sizing_capacity())
+
+    def _sizing_capacity(self) -> int:
+        # size the instance arrays from eager mappings of the current parameters (largest chunk), releasing each
+        # mapping's workspace before the next
+        k = 0
         with torch.no_grad():
-            k = int(self.batch.map(self.means, self.tril, self.raw_o, self.raw_c).stats.n_instances)
-        self._allocate(int(k * headroom) + 4096)  # the sizing map's workspace is released first
+            for c in self.chunks:
+                k = max(k, int(c[4].map(self.means, self.tril, self.raw_o, self.raw_c).stats.n_instances))
+        return int(k * self.headroom) + 4096
 
     # ------------------------------------------------------------------ plumbing
     def _allocate(self, k_cap: int) -> None:
         self.k_capacity = min(int(k_cap), (1 << 31) - 1)
         self.graph = None
         self.workspace = None  # the old workspace goes before the new one is allocated
-        self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
-        self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
+        self.workspace = new_workspace(max(c[4].workspace_bytes(self.k_capacity) for c in self.chunks), self.dev)
+        self.plans = [c[4].tables.plan(self.workspace, self.k_capacity) for c in self.chunks]
         off = self.batch.layout_offsets(self.k_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
-        self.graph = None
+        self.overflow_view = self.workspace[off.stats + _lib.Stats.overflow.offset:
+                                            off.stats + _lib.Stats.overflow.offset + 4].view(torch.int32)
 
     def _body(self) -> None:
         L = _lib.lib()
         s = _lib.stream_ptr(self.dev)
-        st = self.batch.settings
-        lay = self.batch.layout
-        _lib.check(L.us3d_ug_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.raw_o),
-                                 _lib.ptr(self.raw_c), st, s), "us3d_ug_map")
-        _lib.check(L.us3d_ug_forward(self.plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
-                                     None, _lib.ptr(self.aux), s), "us3d_ug_forward")
-        _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
-                                     lay.n_frames, self.w_l1, self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
-                                     _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
-        _lib.check(L.us3d_ug_backward(self.plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
-                                      _lib.ptr(self.d_image), None, None, _lib.ptr(self.grads[0]),
-                                      _lib.ptr(self.grads[1]), _lib.ptr(self.grads[2]), _lib.ptr(self.grads[3]), s),
-                   "us3d_ug_backward")
+        chunked = len(self.chunks) > 1
+        if chunked:
+            self.skip.zero_()
+        for ci, ((a, b, p0, p1, batch), plan) in enumerate(zip(self.chunks, self.plans)):
+            st = batch.settings
+            nf = b - a
+            _lib.check(L.us3d_ug_map(plan, _lib.ptr(self.means), _lib.ptr(self.tril), _lib.ptr(self.raw_o),
+                                     _lib.ptr(self.raw_c), st, s), "us3d_ug_map")
+            if chunked:
+                torch.maximum(self.skip, self.overflow_view, out=self.skip)
+            _lib.check(L.us3d_ug_forward(plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
+                                         None, _lib.ptr(self.aux), s), "us3d_ug_forward")
+            _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target) + 4 * p0,
+                                         _lib.ptr(batch.tables.pixel_base), nf, self.w_l1, self.w_ssim, 1.0 / self.F,
+                                         _lib.ptr(self.loss) + 4 * a, _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp),
+                                         s), "us3d_image_loss")
+            out = self.grads if ci == 0 else self.grads_chunk
+            _lib.check(L.us3d_ug_backward(plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
+                                          _lib.ptr(self.d_image), None, None, _lib.ptr(out[0]), _lib.ptr(out[1]),
+                                          _lib.ptr(out[2]), _lib.ptr(out[3]), s), "us3d_ug_backward")
+            if ci > 0:
+                for g, gc in zip(self.grads, self.grads_chunk):
+                    g.add_(gc)
+        skip = self.skip.data_ptr() if chunked else self.overflow_ptr
         b1, b2 = self.betas
         for p, g, m, v in zip(self.params, self.grads, self.exp_avg, self.exp_avg_sq):
             _lib.check(L.us3d_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v), p.numel(),
-                                        _lib.ptr(self.step_count), self.overflow_ptr, self.lr, b1, b2, self.eps, s),
+                                        _lib.ptr(self.step_count), skip, self.lr, b1, b2, self.eps, s),
                        "us3d_adam_step")
-        _lib.check(L.us3d_adam_advance(_lib.ptr(self.step_count), self.overflow_ptr, s), "us3d_adam_advance")
+        _lib.check(L.us3d_adam_advance(_lib.ptr(self.step_count), skip, s), "us3d_adam_advance")
 
     def _capture(self) -> None:
         side = torch.cuda.Stream(device=self.dev)
@@ -135,10 +190,15 @@ class SliceSplatTrainer:
             self.n_steps += 1
             if self.n_steps % self.check_every != 0:
                 return
-            st = read_stats(self.workspace)
-            if not st.overflow:
-                return
-            self._allocate(int(st.n_instances * self.headroom) + 4096)
+            if len(self.chunks) == 1:
+                st = read_stats(self.workspace)
+                if not st.overflow:
+                    return
+                self._allocate(int(st.n_instances * self.headroom) + 4096)
+            else:
+                if not int(self.skip.item()):
+                    return
+                self._allocate(self._sizing_capacity())
 
     def state(self) -> List[torch.Tensor]:
         return [p.detach() for p in self.params]
@@ -182,6 +242,7 @@ class GaussianPileTrainer:
         lay = self.batch.layout
         self.F = lay.n_frames
         P = lay.n_pixels
+        self.chunks = [(0, self.F, 0, P, self.batch)]  # one chunk: the whole batch
         self.target = target.detach().to(**f32).reshape(P).contiguous()
         self.image = torch.empty(P, **f32)
         self.d_image = torch.empty(P, **f32)
