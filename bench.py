@@ -296,6 +296,15 @@ def profile_figures(kernel: str, workload_tag: str):
     return out
 
 
+def reset_peak(dev):
+    """Start the peak-memory count; the allocator's statistics exist once the device is initialised, which a
+    workload run on its own has not done yet."""
+    import torch
+
+    torch.cuda.init()
+    torch.cuda.reset_peak_memory_stats(dev)
+
+
 def measure_copy_peak():
     """Device-to-device copy bandwidth (read + write bytes / time) of a 1 GiB buffer, timed with HIP events on
     the current stream: the measured HBM figure to set beside the nominal 8 TB/s."""
@@ -415,7 +424,7 @@ def bench_slice(args):
     raw_c = torch.nn.Parameter(logit(scene.intensity.clamp(1e-6, 1 - 1e-6)).to(dev))
     init = [p.detach() for p in (means, tril, raw_o, raw_c)]
 
-    torch.cuda.reset_peak_memory_stats(dev)
+    reset_peak(dev)
     trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9)
     dt_b = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():  # sticky over every replay since the workspace was allocated
@@ -551,7 +560,7 @@ def bench_ray(args):
         image, aux = batch.forward(m, pix, ro, rd, off)
         return m, image, aux
 
-    torch.cuda.reset_peak_memory_stats(dev)
+    reset_peak(dev)
     batch = URBatch(specs, [SweepSpec(0, G)], G, cfg.settings(), dev, CapacityHint())
     with torch.no_grad():
         _m, gt, _a = run(batch, tparams)
@@ -631,7 +640,7 @@ def bench_fan(args):
                                  axial_spacing_mm_per_pix=frames[0].delta_ax, axial_near_mm=frames[0].near_mm,
                                  fixed_bg_alpha=1e-6, tile_size=16)
     torch.zeros(1, device=dev)  # creates the context, so the peak counter can be reset
-    torch.cuda.reset_peak_memory_stats(dev)
+    reset_peak(dev)
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
                            frames, cfg, hint=CapacityHint())[0].clone()
@@ -713,7 +722,7 @@ def bench_gaussianpile(args):
                                    axial_spacing_mm_per_pix=fr0.delta_ax, axial_near_mm=fr0.near_mm,
                                    lateral_col_offset=fr0.col_offset, tile_size_px=args.tile_size)
     hint = CapacityHint()
-    torch.cuda.reset_peak_memory_stats(dev)
+    reset_peak(dev)
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.scales, target.quats, target.opacity)], frames,
                            cfg, hint=hint).clone()
@@ -819,7 +828,7 @@ def bench_multisweep(args):
     f0 = frames[0]
     cfg = UltraGaussRenderConfig(H=f0.height, W=f0.width, lateral_spacing_mm_per_pix=0.2,
                                  axial_spacing_mm_per_pix=0.2, axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=16)
-    torch.cuda.reset_peak_memory_stats(dev)
+    reset_peak(dev)
     with torch.no_grad():
         gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
                            frames, cfg, sweeps=sweeps, hint=CapacityHint())[0].clone()
