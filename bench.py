@@ -58,12 +58,15 @@ def parse(argv=None):
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
                          "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
                          "multisweep: 8 sweeps of uneven size in one captured training step, tile size from the "
-                         "autotuner; fan: --fan-frames frames of a 1000-frame fan-probe sweep over 2M Gaussians; "
+                         "autotuner; fan: --fan-frames frames of a 1000-frame fan-probe sweep over 2M Gaussians, trained "
+                         "in chunks of --fan-chunk frames inside one step; "
                          "gaussianpile: the GaussianPile us_focus splat on the config-2 scene and sweep")
     ap.add_argument("--ms-scale", type=float, default=1.0,
                     help="multisweep: scale factor on the sweeps' Gaussian and frame counts")
     ap.add_argument("--ray-frames", type=int, default=128)
-    ap.add_argument("--fan-frames", type=int, default=100)
+    ap.add_argument("--fan-frames", type=int, default=1000, help="fan: frames of the sweep in one step (all 1000)")
+    ap.add_argument("--fan-chunk", type=int, default=100,
+                    help="fan: frames rendered per chunk inside the step (bounds the workspace; 0 = one batch)")
     ap.add_argument("--fan-gaussians", type=int, default=2_000_000)
     ap.add_argument("--fan-cpu-rows", type=int, default=2)
     ap.add_argument("--cpu-sample-rays", type=int, default=8)
@@ -619,53 +622,70 @@ def bench_ray(args):
 
 def bench_fan(args):
     """Config 5: 2M Gaussians in a 200 x 150 x 150 mm box, a 1000-frame sweep of a 60-degree fan probe (640 beams x
-    480 depth samples over 40-160 mm). One step = one captured training step over --fan-frames consecutive
-    frames of the sweep (pixel grid evaluated in the kernels); reports frames/s and peak HBM."""
+    480 depth samples over 40-160 mm). One step = one captured training step over --fan-frames consecutive frames
+    of the sweep (all 1000 by default; pixel grid evaluated in the kernels), rendered in chunks of --fan-chunk
+    frames inside the step (map -> forward -> loss -> adjoint per chunk, gradients summed, one Adam update), so the
+    workspace holds one chunk's pairs and instances; reports frames/s, iterations/s and peak HBM."""
     import torch
 
-    from us3d.batch import CapacityHint, read_stats
+    from us3d.batch import CapacityHint
     from us3d.scenes import config5, fan_pixels
     from us3d.train_step import SliceSplatTrainer
     from us3d.ultragauss import UltraGaussRenderConfig, render_frames
 
     dev = torch.device("cuda:0")
     scene, target, sweep = config5(n_gauss=args.fan_gaussians)
-    F = args.fan_frames
+    F = min(args.fan_frames, len(sweep))
     f0 = len(sweep) // 2 - F // 2
     frames = sweep[f0:f0 + F]
+    chunk = args.fan_chunk if args.fan_chunk and args.fan_chunk < F else None
     G = scene.means.shape[0]
     H, W = frames[0].height, frames[0].width
     P = F * H * W
     cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=frames[0].delta_lat,
                                  axial_spacing_mm_per_pix=frames[0].delta_ax, axial_near_mm=frames[0].near_mm,
                                  fixed_bg_alpha=1e-6, tile_size=16)
-    torch.zeros(1, device=dev)  # creates the context, so the peak counter can be reset
     reset_peak(dev)
-    with torch.no_grad():
-        gt = render_frames(*[t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)], None,
-                           frames, cfg, hint=CapacityHint())[0].clone()
+    step_gt = chunk or F
+    with torch.no_grad():  # the target, rendered chunk by chunk as well
+        tgt = [t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)]
+        gt = torch.cat([render_frames(*tgt, None, frames[a:a + step_gt], cfg, hint=CapacityHint())[0].clone()
+                        for a in range(0, F, step_gt)])
+        del tgt
     torch.cuda.empty_cache()
     logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
 
-    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9)
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9,
+                                chunk_frames=chunk)
     dt = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
     fps = F * args.steps / dt
     peak = torch.cuda.max_memory_allocated(dev)
     ws_bytes = trainer.workspace.numel()
-    log(f"fan batched: {dt / args.steps * 1e3:.1f} ms/step, {fps:.1f} frames/s, peak HBM {peak / 2**30:.1f} GiB")
+    n_chunks = len(trainer.chunks)
+    log(f"fan batched: {dt / args.steps * 1e3:.1f} ms/step over {F} frames in {n_chunks} chunk(s), {fps:.1f} "
+        f"frames/s, peak HBM {peak / 2**30:.1f} GiB, workspace {ws_bytes / 2**30:.1f} GiB")
     del trainer
     torch.cuda.empty_cache()
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9, chunk_frames=chunk)
     tt.step()
     stages = stage_timing(tt.step)
-    K = int(read_stats(tt.workspace).n_instances)
-    g_vis = visible_pairs(tt)
+    K, g_vis = 0, 0  # instances and visible pairs summed over the chunks
+    with torch.no_grad():
+        for c in tt.chunks:
+            m = c[4].map(tt.means, tt.tril, tt.raw_o, tt.raw_c)
+            K += int(m.stats.n_instances)
+            off = c[4].layout_offsets(m.k_capacity)
+            pc = m.workspace[off.pair_count: off.pair_count + 4 * c[4].layout.n_pairs].view(torch.int32)
+            g_vis += int((pc > 0).sum().item())
+            del m
     del tt
     torch.cuda.empty_cache()
-    launch_bytes = {"ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis, "ug_forward_kernel": 36 * K + 16 * P}
+    # algorithmic bytes per launch: one launch renders one chunk, so the per-launch figure is the chunk average
+    launch_bytes = {"ug_backward_kernel": (36 * K + 8 * P + 32 * g_vis) // n_chunks,
+                    "ug_forward_kernel": (36 * K + 16 * P) // n_chunks}
     roof = roofline(stages, launch_bytes, None, "")
     cpu = None
     if not args.no_cpu_baseline:
@@ -685,15 +705,17 @@ def bench_fan(args):
                          f"({rows * W * G / dtc / 1e6:.0f} M pair evaluations/s); frames/s = sampled fraction of a "
                          f"{W}x{H} frame / time"}
     return {
-        "metric": "fwd+bwd frames/s of a fan-probe batch in one captured training step on 1 GPU; % of HBM roofline",
+        "metric": "fwd+bwd frames/s of a fan-probe sweep in one captured training step on 1 GPU; % of HBM roofline",
         "value": fps, "unit": "frames/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
+        "ms_per_step": dt / args.steps * 1e3, "iters_per_s": args.steps / dt, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 200x150x150 mm box, frames "
                                f"{f0}..{f0 + F - 1} of a 1000-frame 60-degree fan sweep ({W} beams x {H} samples, "
-                               f"40-160 mm), L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} step, "
-                               f"tile 16, pixel grid evaluated in the kernels",
-                   "frames": F, "gaussians": G, "pixels": P, "instances": K, "visible_pairs": g_vis},
+                               f"40-160 mm), L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} step "
+                               f"in {n_chunks} chunk(s) of {chunk or F} frames, tile 16, pixel grid evaluated in "
+                               f"the kernels",
+                   "frames": F, "chunk_frames": chunk or F, "gaussians": G, "pixels": P, "instances": K,
+                   "visible_pairs": g_vis},
         "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes), "roofline": roof, "cpu_baseline": cpu,
     }
 
@@ -898,9 +920,11 @@ def nested(name, args):
            "dominant_kernel": r["kernel"], "dominant_avg_launch_ms": r["avg_launch_ms"], "roofline_frac": r["frac"],
            "stage_ms_per_launch": r["stage_ms_per_launch"], "peak_hbm_bytes": d.get("peak_hbm_bytes"),
            "wall_s": round(time.perf_counter() - t0, 1)}
-    for k in ("workspace_bytes", "megapixels_per_s"):
+    for k in ("workspace_bytes", "megapixels_per_s", "iters_per_s"):
         if k in d:
             out[k] = d[k]
+    if "chunk_frames" in d["config"]:
+        out["chunk_frames"] = d["config"]["chunk_frames"]
     if "tile_size" in d["config"]:
         out["tile_size"] = d["config"]["tile_size"]
     return out
