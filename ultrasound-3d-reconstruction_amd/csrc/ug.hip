@@ -998,14 +998,19 @@ __device__ __forceinline__ void ug_backward_tile(
                     const float4 B = L.g4[slot];  // p, qq, s, c
                     const float* la = reinterpret_cast<const float*>(L.c + NSUB * kPlaneCoordBase) + sb * 16;
                     const float4 la0 = *reinterpret_cast<const float4*>(la), la1 = *reinterpret_cast<const float4*>(la + 4);
-                    const float lat[8] = {la0.x, la0.y, la0.z, la0.w, la1.x, la1.y, la1.z, la1.w};
+                    // lateral offsets from the minimum, once per item (the subtraction of plane_q, same bits)
+                    const float xs[8] = {la0.x - A.x, la0.y - A.x, la0.z - A.x, la0.w - A.x,
+                                         la1.x - A.x, la1.y - A.x, la1.z - A.x, la1.w - A.x};
                     // (dN, v) of the sub-block's pixels, two per float4, a row = four float4 read together at the
                     // start of the row, so the LDS latency is paid once per row
                     const float4* __restrict__ dv = reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(L.p) +
                                                                                     sb * kPlanePixStride);
                     const float2* __restrict__ dxd = reinterpret_cast<const float2*>(L.c) + sb * kPlanePixStride;
-                    float ao = 0.f, ac = 0.f, aed = 0.f;
-                    float M0 = 0.f, MX = 0.f, MY = 0.f, MXX = 0.f, MXY = 0.f, MYY = 0.f;
+                    // With wp = w at a passing pixel and 0 elsewhere and t = dL/dah * wp, every sum is one of
+                    // t's or wp's: do = sum t, dc = o sum dN wp, sum(dD ah) = o sum dD wp, and m = -dL/dah ah =
+                    // -o t, so the moments of m are -o times those of t (applied once per item below).
+                    float acw = 0.f, aedw = 0.f;
+                    float T0 = 0.f, TX = 0.f, TY = 0.f, TXX = 0.f, TXY = 0.f, TYY = 0.f;
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
                         float4 cur[4];
@@ -1021,42 +1026,43 @@ __device__ __forceinline__ void ug_backward_tile(
                             const float dN = (jx & 1) ? cur[jx >> 1].z : cur[jx >> 1].x;
                             const float vi = (jx & 1) ? cur[jx >> 1].w : cur[jx >> 1].y;
                             // branch-free: a pixel that fails the forward's tests (q <= chi2, ah > alpha_min)
-                            // enters every sum with a zero factor; the coefficients of a listed pair are finite,
-                            // so q, w and ah are finite and the masked terms are exact zeros
-                            const float u = __builtin_fmaf(B.x, lat[jx] - A.x, qy);
+                            // enters every sum through wp = 0; the coefficients of a listed pair and the pixel
+                            // coefficients are finite, so the masked terms are exact zeros
+                            const float u = __builtin_fmaf(B.x, xs[jx], qy);
                             const float q = __builtin_fmaf(u, u, vv);
                             const float w = exp_neg_half(q);
-                            const float ah = A.w * w;
-                            const bool pass = (q <= chi2) && (ah > alpha_min);
+                            const bool pass = (q <= chi2) && (A.w * w > alpha_min);
+                            const float wp = pass ? w : 0.f;
                             float da;
                             if constexpr (kFull) {
                                 const float2 cc = dxd[8 * r + jx];
                                 da = __builtin_fmaf(dN, B.w - vi, __builtin_fmaf(cc.y, e, cc.x));
-                                aed = __builtin_fmaf(pass ? cc.y : 0.f, ah, aed);
+                                aedw = __builtin_fmaf(cc.y, wp, aedw);
                             } else {
                                 da = dN * (B.w - vi);
                             }
-                            const float dNp = pass ? dN : 0.f, dap = pass ? da : 0.f;
-                            ac = __builtin_fmaf(dNp, ah, ac);
-                            ao = __builtin_fmaf(dap, w, ao);
-                            const float mm = -dap * ah;
+                            acw = __builtin_fmaf(dN, wp, acw);
+                            const float t = da * wp;
                             const float X = (float)jx - 3.5f;  // constants once the loops are unrolled
-                            R0 += mm;
-                            RX = __builtin_fmaf(mm, X, RX);
-                            RXX = __builtin_fmaf(mm, X * X, RXX);
+                            R0 += t;
+                            RX = __builtin_fmaf(t, X, RX);
+                            RXX = __builtin_fmaf(t, X * X, RXX);
                         }
                         const float Y = (float)r - 3.5f;
-                        M0 += R0;
-                        MX += RX;
-                        MXX += RXX;
-                        MY = __builtin_fmaf(R0, Y, MY);
-                        MYY = __builtin_fmaf(R0, Y * Y, MYY);
-                        MXY = __builtin_fmaf(RX, Y, MXY);
+                        T0 += R0;
+                        TX += RX;
+                        TXX += RXX;
+                        TY = __builtin_fmaf(R0, Y, TY);
+                        TYY = __builtin_fmaf(R0, Y * Y, TYY);
+                        TXY = __builtin_fmaf(RX, Y, TXY);
                     }
+                    const float mo = -A.w;
+                    const float M0 = mo * T0, MX = mo * TX, MY = mo * TY, MXX = mo * TXX, MXY = mo * TXY,
+                                MYY = mo * TYY;
                     // moments about the sub-block centre -> about the tile centre (offset (ox_, oy_) pixels)
                     const float ox_ = (float)((sb % S::kSubPerRow) * 8) + 3.5f - 0.5f * (float)(TS - 1);
                     const float oy_ = (float)((sb / S::kSubPerRow) * 8) + 3.5f - 0.5f * (float)(TS - 1);
-                    a.v[0] = ao; a.v[1] = ac; a.v[2] = aed; a.v[3] = M0;
+                    a.v[0] = T0; a.v[1] = A.w * acw; a.v[2] = A.w * aedw; a.v[3] = M0;
                     a.v[6] = __builtin_fmaf(2.f * ox_, MX, __builtin_fmaf(ox_ * ox_, M0, MXX));
                     a.v[7] = __builtin_fmaf(ox_, MY, __builtin_fmaf(oy_, MX, __builtin_fmaf(ox_ * oy_, M0, MXY)));
                     a.v[8] = __builtin_fmaf(2.f * oy_, MY, __builtin_fmaf(oy_ * oy_, M0, MYY));
