@@ -830,12 +830,13 @@ def bench_gaussianpile(args):
 
 def bench_multisweep(args):
     """8 sweeps (Gaussians log-uniform in 50k..500k, 32..400 frames, frame size 256x256 / 512x512 / 640x480) in
-    ONE hipGraph-captured training step; the tile size is picked by the autotuner on a frame sample first."""
+    ONE hipGraph-captured training step; the tile size and the forward's and adjoint's LDS batch sizes are picked by
+    the autotuner on a frame sample first."""
     from dataclasses import replace
 
     import torch
 
-    from us3d.autotune import tune_tile_size
+    from us3d.autotune import apply, tune_tile_size
     from us3d.batch import CapacityHint, read_stats
     from us3d.scenes import config4, config4_specs
     from us3d.train_step import SliceSplatTrainer
@@ -859,7 +860,7 @@ def bench_multisweep(args):
     logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
     tune = tune_tile_size(*init, frames, gt, cfg, sample_frames=32, reps=2, sweeps=sweeps, use_cache=False)
-    cfg = replace(cfg, tile_size=int(tune["tile_size"]))
+    cfg = apply(cfg, tune)
     log(f"multisweep autotuner: {tune}")
     torch.cuda.empty_cache()
     trainer = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=not args.eager, check_every=10 ** 9)
@@ -889,9 +890,11 @@ def bench_multisweep(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"slice splat fwd+bwd+Adam over {len(sweeps)} sweeps (Gaussians, frames, HxW) = "
                                f"{specs}, {G} Gaussians and {F} frames in total, L1+0.2(1-SSIM), batched "
-                               f"{'eager' if args.eager else 'hipGraph'} step, tile {cfg.tile_size} (autotuned)",
+                               f"{'eager' if args.eager else 'hipGraph'} step, tile {cfg.tile_size}, LDS batches "
+                               f"{cfg.fwd_lds_batch} (forward) / {cfg.bwd_lds_batch} (adjoint) Gaussians (autotuned)",
                    "frames": F, "gaussians": G, "pixels": P, "tile_size": cfg.tile_size, "instances": K,
-                   "visible_pairs": g_vis, "autotune_ms_per_frame": tune["ms_per_frame"]},
+                   "visible_pairs": g_vis, "autotune_ms_per_frame": tune["ms_per_frame"],
+                   "autotune_lds_batch_ms": tune["lds_batch_ms"]},
         "megapixels_per_s": P * args.steps / dt / 1e6, "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes),
         "roofline": roof, "cpu_baseline": None,
     }

@@ -126,6 +126,11 @@ typedef struct us3d_ug_settings {
                                 * sigmoid 1/(1+exp(-x)) of the training model's activations
                                 * (pipeline_god_file.py:369-373) and us3d_ug_backward returns the gradients
                                 * w.r.t. the logits (chain s(1-s)); 0: they hold opacity and intensity. */
+    int32_t fwd_lds_batch;  /* Gaussians the forward splat stages in LDS per batch: 64, 128 or 256, or 0 for its
+                             * thread count (64/128/256 at tiles 8/16/32). No counterpart in the reference (its
+                             * splat reads the Gaussians one by one); the outputs do not depend on it. */
+    int32_t bwd_lds_batch;  /* the same for the adjoint (0: 64/256/256 at tiles 8/16/32); the gradients depend on
+                             * it only through the FP32 rounding of the per-Gaussian sums. */
 } us3d_ug_settings;
 
 /* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by both renderers.
@@ -136,8 +141,9 @@ typedef struct us3d_ug_settings {
  * records (per Gaussian), partial (per instance), vis_list i32[n_pairs] (frame-major copy of the pair
  * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1), inst_mask u16[K]
  * (UltraGauss: the sub-block mask of each sorted instance, written by the forward, read by the backward;
- * empty for UltraG-Ray), pair_sect f32[n_pairs][8] (UltraGauss: per visible pair, the pair's part of the
- * sub-block test, written by the map, read by the forward; empty for UltraG-Ray). */
+ * empty for UltraG-Ray), pair_sect f32[n_pairs][12] (UltraGauss: per visible pair, the pair's part of the
+ * sub-block test and the plane form of q on the frame's grid, written by the map, read by the forward and the
+ * backward; empty for UltraG-Ray). */
 typedef struct us3d_layout {
     int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
     int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;

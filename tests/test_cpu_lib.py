@@ -154,6 +154,12 @@ def test_config_validation_mirrors_reference():
                                diag_beta=0.0)
     assert UltraGaussRenderConfig(H=4, W=5, lateral_spacing_mm_per_pix=1, axial_spacing_mm_per_pix=1,
                                   axial_near_mm=0).lateral_col_offset == 2.0
+    with pytest.raises(ValueError):  # the splat kernels stage 64, 128 or 256 Gaussians per LDS batch
+        UltraGaussRenderConfig(H=4, W=4, lateral_spacing_mm_per_pix=1, axial_spacing_mm_per_pix=1, axial_near_mm=0,
+                               bwd_lds_batch=96)
+    st = UltraGaussRenderConfig(H=4, W=4, lateral_spacing_mm_per_pix=1, axial_spacing_mm_per_pix=1,
+                                axial_near_mm=0, fwd_lds_batch=128, bwd_lds_batch=64).settings()
+    assert (st.fwd_lds_batch, st.bwd_lds_batch) == (128, 64)
     with pytest.raises(ValueError):
         UltragrayRenderConfig(image_width=4, image_height=4, width_mm=0.0, near_mm=0.0)
     with pytest.raises(ValueError):

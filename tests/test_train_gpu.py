@@ -2,6 +2,8 @@
 loss and its gradient match torch autograd of the reference loss; the Adam kernel matches torch.optim.Adam;
 an overflowing capacity is detected, skipped by Adam and retried."""
 
+from dataclasses import replace
+
 import numpy as np
 import pytest
 import torch
@@ -167,7 +169,10 @@ def test_chunked_step_equals_the_whole_batch_step(dev):
 
 
 def test_autotuner_picks_and_caches_a_tile_size(dev, tmp_path, monkeypatch):
-    from us3d.autotune import CANDIDATES, tune_tile_size
+    """The autotuner times every tile size at every LDS batch size (per kernel), keeps a tile size and a batch size
+    for the forward and for the adjoint, caches the choice, and its choice trains like the defaults: the step with
+    the tuned configuration gives the same per-frame losses as the default configuration's on its first step."""
+    from us3d.autotune import CANDIDATES, LDS_BATCHES, apply, tune_tile_size
     from us3d.scenes import multi_sweep
 
     monkeypatch.setenv("US3D_CACHE_DIR", str(tmp_path))
@@ -181,9 +186,17 @@ def test_autotuner_picks_and_caches_a_tile_size(dev, tmp_path, monkeypatch):
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
     first = tune_tile_size(*init, frames, gt, cfg, sample_frames=4, reps=1, sweeps=sweeps)
     assert first["tile_size"] in CANDIDATES and not first["cached"]
+    assert first["fwd_lds_batch"] in LDS_BATCHES and first["bwd_lds_batch"] in LDS_BATCHES
     assert set(first["ms_per_frame"]) == {str(c) for c in CANDIDATES}
+    assert all(set(v) == {str(b) for b in LDS_BATCHES} for v in first["lds_batch_ms"].values())
     again = tune_tile_size(*init, frames, gt, cfg, sample_frames=4, reps=1, sweeps=sweeps)
-    assert again["cached"] and again["tile_size"] == first["tile_size"]
+    assert again["cached"] and all(again[k] == first[k] for k in ("tile_size", "fwd_lds_batch", "bwd_lds_batch"))
+    tuned = apply(cfg, first)
+    a = SliceSplatTrainer(*init, frames, gt, replace(cfg, tile_size=tuned.tile_size), sweeps=sweeps, use_graph=False)
+    b = SliceSplatTrainer(*init, frames, gt, tuned, sweeps=sweeps, use_graph=False)
+    a.step()
+    b.step()
+    assert torch.equal(a.losses(), b.losses())
 
 
 def test_trainer_gradients_match_torch_autograd_through_the_activations(dev):

@@ -542,8 +542,9 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 
 // ---------------------------------------------------------------- forward splat
 // One workgroup per (frame, tile) work item. Pixel-parallel: each thread accumulates its pixel's sums over
-// the tile's sorted list, which is staged through LDS in batches of kThreads Gaussians (records gathered
-// with 16 B loads, read back as wavefront broadcasts). Follows kernel_pixel_ultragauss_forward,
+// the tile's sorted list, which is staged through LDS in batches of kBatch Gaussians (records gathered
+// with 16 B loads, read back as wavefront broadcasts; kBatch is the work-group's thread count unless the settings
+// choose 64, 128 or 256, and the sums do not depend on it). Follows kernel_pixel_ultragauss_forward,
 // ops/ultragauss_renderer/src/shader/forward.cu:17-76: alpha_hat = o * exp(-q/2) for q <= chi2 and
 // alpha_hat > alpha_min; A = sum alpha_hat + bg; image = clamp((sum alpha_hat c + bg I_bg) / max(A,1e-8));
 // depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8). The per-pixel sums are accumulated in FP64: the
@@ -568,17 +569,18 @@ __device__ __forceinline__ void grid_lat_ax(const us3d_frame& fr, int ix, int iy
 }
 
 // LDS of one forward workgroup.
-template <int TS>
+template <int TS, int kBatch>
 struct FwdLds {
-    float4 r0[FwdShape<TS>::kThreads], r1[FwdShape<TS>::kThreads], r2[FwdShape<TS>::kThreads];
-    uint32_t mask[FwdShape<TS>::kThreads];
+    float4 r0[kBatch], r1[kBatch], r2[kBatch];
+    uint32_t mask[kBatch];
     SubBounds sb[FwdShape<TS>::kSub];
 };
 
-// The forward of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_forward_kernel).
-template <int TS, bool kDepth, bool kPlaneBody>
+// The forward of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_forward_kernel); the
+// tile's list is staged kBatch Gaussians at a time.
+template <int TS, bool kDepth, bool kPlaneBody, int kBatch>
 __device__ __forceinline__ void ug_forward_tile(
-    FwdLds<TS>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key, const us3d_frame& fr,
+    FwdLds<TS, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key, const us3d_frame& fr,
     const us3d_sweep& sw, int f, int tx, int ty, int s0, int s1, const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, float* __restrict__ out_image, float* __restrict__ out_alpha,
@@ -622,11 +624,11 @@ __device__ __forceinline__ void ug_forward_tile(
         S_[k] = N_[k] = D_[k] = 0.0;
     }
 
-    for (int base = s0; base < s1; base += NT) {
+    for (int base = s0; base < s1; base += kBatch) {
         __syncthreads();
-        const int n = min(NT, s1 - base);
-        if (tid < n) {
-            const int g = (int)(uint32_t)seg_key[base + tid];
+        const int n = min(kBatch, s1 - base);
+        for (int j = tid; j < n; j += NT) {
+            const int g = (int)(uint32_t)seg_key[base + j];
             const float4 r0 = rec[3 * (int64_t)g + 0];
             const float4 r1 = rec[3 * (int64_t)g + 1];
             const float4 r2 = rec[3 * (int64_t)g + 2];
@@ -637,17 +639,17 @@ __device__ __forceinline__ void ug_forward_tile(
             const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
             const float4 ps0 = pair_sect[3 * pair + 0], ps1 = pair_sect[3 * pair + 1];
             const uint32_t m = sub_mask<S::kSub>(L.sb, ps0, ps1, lat_g, ax_g, e, chi2w);
-            L.mask[tid] = m;
-            inst_mask[base + tid] = (uint16_t)m;  // the adjoint reads it instead of computing it again
+            L.mask[j] = m;
+            inst_mask[base + j] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             if constexpr (kPlaneBody) {
                 const float4 ps2 = pair_sect[3 * pair + 2];
-                L.r0[tid] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
-                L.r1[tid] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
-                L.r2[tid] = make_float4(e, 0.f, 0.f, 0.f);
+                L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
+                L.r1[j] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
+                L.r2[j] = make_float4(e, 0.f, 0.f, 0.f);
             } else {
-                L.r0[tid] = r0;                                  // mu, opacity
-                L.r1[tid] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
-                L.r2[tid] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
+                L.r0[j] = r0;                                  // mu, opacity
+                L.r1[j] = make_float4(r1.x, e, r1.y, r1.z);   // intensity, elev, l11, l12
+                L.r2[j] = make_float4(r1.w, r2.x, r2.y, r2.z); // l13, l22, l23, l33
             }
         }
         __syncthreads();
@@ -715,7 +717,7 @@ __device__ __forceinline__ void ug_forward_tile(
 // linear-probe frames evaluate q in the plane form of their pairs (pair_plane_form, read from pair_sect) from the
 // pixels' probe coordinates, and fan-probe tiles take the general form q = |L^T (p - mu)|^2 from the pixels'
 // world positions, as every tile does with explicit pixels.
-template <int TS, bool kDepth, bool kPlane>
+template <int TS, bool kDepth, bool kPlane, int kBatch>
 __global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
 void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
@@ -725,7 +727,7 @@ void ug_forward_kernel(
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
     float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
     uint16_t* __restrict__ inst_mask) {
-    __shared__ FwdLds<TS> lds;
+    __shared__ FwdLds<TS, kBatch> lds;
     if (stats->overflow) return;
     const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
     const int f = frame_of_tile(frame_tile_base, n_frames, t);
@@ -735,11 +737,11 @@ void ug_forward_kernel(
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     const us3d_sweep& sw = sweeps[fr.sweep];
     if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_forward_tile<TS, kDepth, true>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+        ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
                                           sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
                                           out_alpha, out_depth, out_aux, inst_mask);
     else
-        ug_forward_tile<TS, kDepth, false>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+        ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
                                            sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
                                            out_alpha, out_depth, out_aux, inst_mask);
 }
@@ -747,7 +749,7 @@ void ug_forward_kernel(
 // ---------------------------------------------------------------- adjoint
 // One workgroup per (frame, tile). Per-pixel adjoint coefficients are computed once from the stored sums
 // and staged in LDS (sub-block-major). The work unit is an ITEM: one Gaussian of the tile's list paired with
-// one 8x8 sub-block its mask keeps. A batch of up to kThreads Gaussians is loaded, its items are listed
+// one 8x8 sub-block its mask keeps. A batch of up to kBatch Gaussians is loaded, its items are listed
 // Gaussian-major (a Gaussian's items are consecutive, in sub-block order), and the wavefronts take chunks of
 // 64 items: each lane loops over its item's 64 pixels (LDS reads with at most one address per sub-block),
 // accumulating the 12 gradient sums in registers with no atomics. The lanes of one Gaussian are adjacent, so
@@ -847,17 +849,17 @@ constexpr int kPlanePixStride = 66;
 constexpr int kPlaneCoordBase = 33;
 static_assert(kPlanePixStride * 8 <= kSubStride * 16 && (kPlaneCoordBase + 4) <= kSubStride, "plane staging fits");
 
-// LDS of one adjoint workgroup (the plane form's two extra record arrays only where a tile can take it).
-template <int TS, bool kPlane>
+// LDS of one adjoint workgroup staging kBatch Gaussians at a time (the plane form's two extra record arrays only
+// where a tile can take it).
+template <int TS, bool kPlane, int kBatch>
 struct BwdLds {
-    static constexpr int NT = TileShape<TS>::kThreads;
     static constexpr int NSUB = TileShape<TS>::kSub;
-    static constexpr int MAXI = NT * NSUB;     // items of one batch, at most
-    static constexpr int MAXC = MAXI / kWave;  // chunks of one batch, at most
+    static constexpr int MAXI = kBatch * NSUB;                  // items of one batch, at most
+    static constexpr int MAXC = (MAXI + kWave - 1) / kWave;     // chunks of one batch, at most
     float4 p[NSUB * kSubStride];  // general form: px, py, pz, dN; plane form: lat, ax, dN, v
     float4 c[NSUB * kSubStride];  // general form: v, dX, dD; plane form: dX, dD
-    float4 g0[NT], g1[NT], g2[NT];                     // the batch's Gaussian records
-    float4 g3[kPlane ? NT : 1], g4[kPlane ? NT : 1];   // plane form: (cx, cy, K, o), (p, qq, s, c)
+    float4 g0[kBatch], g1[kBatch], g2[kBatch];                     // the batch's Gaussian records
+    float4 g3[kPlane ? kBatch : 1], g4[kPlane ? kBatch : 1];       // plane form: (cx, cy, K, o), (p, qq, s, c)
     uint16_t item[MAXI];
     int wtot[TileShape<TS>::kWaves];
     GradSums pend[MAXC];   // head part of a Gaussian that continues into the next chunk
@@ -865,10 +867,11 @@ struct BwdLds {
     int pend_slot[MAXC];
 };
 
-// The adjoint of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_backward_kernel).
-template <int TS, bool kFull, bool kPlaneBody, bool kPlane>
+// The adjoint of one (frame, tile) work item; kPlaneBody selects the form of q (see ug_backward_kernel). The list
+// is staged kBatch Gaussians at a time, slot j of a batch by thread j mod kThreads.
+template <int TS, bool kFull, bool kPlaneBody, bool kPlane, int kBatch>
 __device__ __forceinline__ void ug_backward_tile(
-    BwdLds<TS, kPlane>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
+    BwdLds<TS, kPlane, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
     const int32_t* __restrict__ seg_val, const us3d_frame& fr, const us3d_sweep& sw, int f, int tx, int ty, int s0,
     int s1, const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
@@ -879,10 +882,11 @@ __device__ __forceinline__ void ug_backward_tile(
     constexpr int PPT = S::kSubPerWave;
     constexpr int NSUB = S::kSub;
     constexpr int NW = S::kWaves;
-    constexpr int MAXC = BwdLds<TS, kPlane>::MAXC;
+    constexpr int MAXC = BwdLds<TS, kPlane, kBatch>::MAXC;
+    constexpr int SPT = (kBatch + NT - 1) / NT;  // batch slots per thread
     static_assert(kPlane || !kPlaneBody, "the plane form needs the plane record arrays");
     constexpr int NS = kPlaneBody ? kPlaneSums : 12;
-    static_assert(NT <= 256 && NSUB <= 16, "item code packs (slot << 4) | sub-block into 12 bits");
+    static_assert(kBatch <= 256 && NSUB <= 16, "item code packs (slot << 4) | sub-block into 12 bits");
     static_assert(FwdShape<TS>::kSub == NSUB, "the forward's instance masks number the same sub-blocks");
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
@@ -925,36 +929,43 @@ __device__ __forceinline__ void ug_backward_tile(
         }
     }
 
-    for (int base = s0; base < s1; base += NT) {
-        const int n = min(NT, s1 - base);
+    for (int base = s0; base < s1; base += kBatch) {
+        const int n = min(kBatch, s1 - base);
         __syncthreads();  // staged pixels are in place; the previous batch is done with the batch arrays
-        uint32_t m = 0;
-        if (tid < n) {
-            const int g = (int)(uint32_t)seg_key[base + tid];
-            const float4 q0 = rec[3 * (int64_t)g + 0];
-            const float4 q1 = rec[3 * (int64_t)g + 1];
-            const float4 q2 = rec[3 * (int64_t)g + 2];
-            m = inst_mask[base + tid];  // the sub-block mask the forward left for this instance
-            L.g0[tid] = q0;
-            L.g1[tid] = q1;
-            L.g2[tid] = q2;
-            if constexpr (kPlaneBody) {
-                const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
-                const float4 ps1 = pair_sect[3 * pair + 1], ps2 = pair_sect[3 * pair + 2];
-                L.g3[tid] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
-                L.g4[tid] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
+        uint32_t m[SPT];
+        int pc = 0;
+#pragma unroll
+        for (int i = 0; i < SPT; ++i) {
+            const int j = tid + i * NT;
+            m[i] = 0;
+            if (j < n) {
+                const int g = (int)(uint32_t)seg_key[base + j];
+                const float4 q0 = rec[3 * (int64_t)g + 0];
+                const float4 q1 = rec[3 * (int64_t)g + 1];
+                const float4 q2 = rec[3 * (int64_t)g + 2];
+                m[i] = inst_mask[base + j];  // the sub-block mask the forward left for this instance
+                L.g0[j] = q0;
+                L.g1[j] = q1;
+                L.g2[j] = q2;
+                if constexpr (kPlaneBody) {
+                    const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
+                    const float4 ps1 = pair_sect[3 * pair + 1], ps2 = pair_sect[3 * pair + 2];
+                    L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
+                    L.g4[j] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
+                }
+                if (m[i] == 0) {  // touches no pixel of the tile: its partial is zero
+                    const int64_t ei = seg_val[base + j];
+                    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    partial[3 * ei + 0] = z4;
+                    partial[3 * ei + 1] = z4;
+                    partial[3 * ei + 2] = z4;
+                }
             }
-            if (m == 0) {  // touches no pixel of the tile: its partial is zero
-                const int64_t ei = seg_val[base + tid];
-                const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-                partial[3 * ei + 0] = z4;
-                partial[3 * ei + 1] = z4;
-                partial[3 * ei + 2] = z4;
-            }
+            pc += __popc(m[i]);
         }
         for (int c = tid; c < MAXC; c += NT) L.pend_slot[c] = -1;
-        // exclusive scan of the per-Gaussian item counts over the batch
-        const int pc = __popc(m);
+        // exclusive scan of the per-thread item counts over the batch; a thread lists its slots' items in slot
+        // order, so each Gaussian's items are consecutive
         int incl = pc;
 #pragma unroll
         for (int off = 1; off < kWave; off <<= 1) {
@@ -970,7 +981,10 @@ __device__ __forceinline__ void ug_backward_tile(
             pos += (w < wave) ? tw : 0;
             total += tw;
         }
-        for (uint32_t mm = m; mm; mm &= mm - 1) L.item[pos++] = (uint16_t)((tid << 4) | __builtin_ctz(mm));
+#pragma unroll
+        for (int i = 0; i < SPT; ++i)
+            for (uint32_t mm = m[i]; mm; mm &= mm - 1)
+                L.item[pos++] = (uint16_t)(((tid + i * NT) << 4) | __builtin_ctz(mm));
         __syncthreads();
 
         const int nchunks = (total + kWave - 1) / kWave;
@@ -1160,7 +1174,7 @@ __device__ __forceinline__ void ug_backward_tile(
 // kFull: alpha or depth gradients are given (d_alpha or d_depth != NULL); without them dL/dalpha_hat is
 // dN (c - v) alone and the depth term is not accumulated. kPlane: the pixels are the frames' own grids (pixels ==
 // NULL); linear-probe tiles then take the plane form of q (as in the forward) and fan tiles the general form.
-template <int TS, bool kFull, bool kPlane>
+template <int TS, bool kFull, bool kPlane, int kBatch>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
@@ -1168,7 +1182,7 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float4* __restrict__ pair_sect, float chi2, float alpha_min, float bg_alpha, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
-    __shared__ BwdLds<TS, kPlane> lds;
+    __shared__ BwdLds<TS, kPlane, kBatch> lds;
     if (stats->overflow) return;
     const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
     const int f = frame_of_tile(frame_tile_base, n_frames, t);
@@ -1179,11 +1193,11 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     if (s0 == s1) return;
     const us3d_sweep& sw = sweeps[fr.sweep];
     if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_backward_tile<TS, kFull, kPlane, kPlane>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+        ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
                                           chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
                                           inst_mask);
     else
-        ug_backward_tile<TS, kFull, false, kPlane>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+        ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
                                            chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
                                            inst_mask);
 }
@@ -1303,8 +1317,14 @@ static int check_settings(const us3d_ug_settings* s) {
         return fail(US3D_ERR_ARGUMENT, "tile_size must be 8, 16 or 32");
     if (!(s->chi2_threshold > 0.0f)) return fail(US3D_ERR_ARGUMENT, "chi2_threshold must be > 0");
     if (!(s->diag_beta > 0.0f)) return fail(US3D_ERR_ARGUMENT, "diag_beta must be > 0");
+    for (const int b : {s->fwd_lds_batch, s->bwd_lds_batch})
+        if (!(b == 0 || b == 64 || b == 128 || b == 256))
+            return fail(US3D_ERR_ARGUMENT, "fwd_lds_batch and bwd_lds_batch must be 0, 64, 128 or 256");
     return US3D_OK;
 }
+
+// Gaussians a splat kernel stages in LDS per batch: the settings' choice, or the kernel's thread count.
+static int lds_batch(int chosen, int threads) { return chosen ? chosen : threads; }
 
 template <typename T>
 static T* at(const us3d_plan* plan, int64_t off) {
@@ -1480,15 +1500,23 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     if (!image || !alpha || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
-    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads), 0,   \
-                       stream,                                                                                        \
+#define UG_FWD_B(TS, DEPTH, PLANE, B)                                                                                 \
+    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE, B>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads),   \
+                       0, stream,                                                                                     \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        plan->frames, plan->sweeps, plan->frame_tile_base, plan->n_frames, pixels,                     \
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
                        reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask))
+#define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
+    do {                                                                                                              \
+        switch (lds_batch(settings->fwd_lds_batch, FwdShape<TS>::kThreads)) {                                                        \
+            case 64: UG_FWD_B(TS, DEPTH, PLANE, 64); break;                                                           \
+            case 128: UG_FWD_B(TS, DEPTH, PLANE, 128); break;                                                         \
+            default: UG_FWD_B(TS, DEPTH, PLANE, 256); break;                                                          \
+        }                                                                                                             \
+    } while (0)
 #define UG_FWD(TS, DEPTH)                                                                                             \
     do {                                                                                                              \
         if (pixels) UG_FWD_(TS, DEPTH, false); else UG_FWD_(TS, DEPTH, true);                                         \
@@ -1503,6 +1531,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     }
 #undef UG_FWD
 #undef UG_FWD_
+#undef UG_FWD_B
     return check_launch("ug_forward_kernel");
 }
 
@@ -1518,14 +1547,22 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
-#define UG_BWD_(TS, FULL, PLANE)                                                                                      \
-    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads), 0, \
-                       stream, at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off),                            \
+#define UG_BWD_B(TS, FULL, PLANE, B)                                                                                  \
+    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE, B>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads),  \
+                       0, stream, at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off),                         \
                        at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), plan->frames, plan->sweeps,       \
                        plan->frame_tile_base, plan->n_frames, pixels, at<float4>(plan, L.pair_sect),                  \
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
                        reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask))
+#define UG_BWD_(TS, FULL, PLANE)                                                                                      \
+    do {                                                                                                              \
+        switch (lds_batch(settings->bwd_lds_batch, TileShape<TS>::kThreads)) {                                                       \
+            case 64: UG_BWD_B(TS, FULL, PLANE, 64); break;                                                            \
+            case 128: UG_BWD_B(TS, FULL, PLANE, 128); break;                                                          \
+            default: UG_BWD_B(TS, FULL, PLANE, 256); break;                                                           \
+        }                                                                                                             \
+    } while (0)
 #define UG_BWD(TS, FULL)                                                                                              \
     do {                                                                                                              \
         if (pixels) UG_BWD_(TS, FULL, false); else UG_BWD_(TS, FULL, true);                                           \
@@ -1541,6 +1578,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
     }
 #undef UG_BWD
 #undef UG_BWD_
+#undef UG_BWD_B
     if ((rc = check_launch("ug_backward_kernel"))) return rc;
     if (plan->n_gauss > 0) {
         {

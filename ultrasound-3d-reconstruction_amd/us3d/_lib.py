@@ -94,6 +94,8 @@ class UGSettings(ctypes.Structure):
         ("tile_size", ctypes.c_int32),
         ("alpha_excludes_bg", ctypes.c_int32),
         ("raw_activations", ctypes.c_int32),
+        ("fwd_lds_batch", ctypes.c_int32),
+        ("bwd_lds_batch", ctypes.c_int32),
     ]
 
 

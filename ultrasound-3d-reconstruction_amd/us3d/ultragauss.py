@@ -38,7 +38,10 @@ class SliceGeometry:
 
 @dataclass
 class UltraGaussRenderConfig:
-    """ops/ultragauss_renderer/ultragauss_renderer/types.py:14-40 (+ tile_size and semantics)."""
+    """ops/ultragauss_renderer/ultragauss_renderer/types.py:14-40 (+ tile_size, the LDS batch sizes and semantics).
+    fwd_lds_batch / bwd_lds_batch: Gaussians the forward splat / the adjoint stage in LDS per batch, 64, 128 or
+    256, or 0 for the kernel's thread count; the image does not depend on them and the gradients only through
+    FP32 rounding (the autotuner picks them with the tile size)."""
 
     H: int
     W: int
@@ -55,6 +58,8 @@ class UltraGaussRenderConfig:
     lateral_col_offset: Optional[float] = None
     tile_size: int = 16
     semantics: str = "cuda"
+    fwd_lds_batch: int = 0
+    bwd_lds_batch: int = 0
 
     def __post_init__(self) -> None:
         if self.H <= 0 or self.W <= 0:
@@ -69,6 +74,8 @@ class UltraGaussRenderConfig:
             raise ValueError("tile_size must be 8, 16 or 32.")
         if self.semantics not in ("cuda", "fallback"):
             raise ValueError("semantics must be 'cuda' or 'fallback'.")
+        if self.fwd_lds_batch not in (0, 64, 128, 256) or self.bwd_lds_batch not in (0, 64, 128, 256):
+            raise ValueError("fwd_lds_batch and bwd_lds_batch must be 0, 64, 128 or 256.")
         if self.lateral_col_offset is None:
             self.lateral_col_offset = float(self.W - 1) * 0.5
 
@@ -89,6 +96,8 @@ class UltraGaussRenderConfig:
         s.diag_beta = float(self.diag_beta)
         s.sort_by_depth = 1 if self.mapper_sort_by_depth else 0
         s.tile_size = int(self.tile_size)
+        s.fwd_lds_batch = int(self.fwd_lds_batch)
+        s.bwd_lds_batch = int(self.bwd_lds_batch)
         return s
 
 
