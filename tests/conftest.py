@@ -20,3 +20,14 @@ def dev():
     if not torch.cuda.is_available():
         pytest.skip("no GPU")
     return torch.device("cuda:0")
+
+
+def pytest_terminal_summary(terminalreporter):
+    """One table of the parity figures the GPU tests recorded (tests/parity.py), if any ran."""
+    mod = sys.modules.get("tests.parity")
+    if mod is None or not mod.RESULTS:
+        return
+    tr = terminalreporter
+    tr.section("parity against the oracle: largest relative L2 error per output")
+    for (case, output), (err, bar, n) in sorted(mod.RESULTS.items()):
+        tr.write_line(f"{case:<52} {output:<26} {err:9.2e}  bar {bar:.0e}  ({n} checks)")

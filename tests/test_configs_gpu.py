@@ -19,6 +19,7 @@ import torch
 
 from oracle import ug as oug
 from oracle import ur as our
+from tests.parity import note
 from tests.ug_helpers import rel_l2
 from tests.ur_helpers import oracle_frame, ray_settings, ray_spec
 
@@ -74,8 +75,11 @@ def test_config3_frame_ray_lists_image_and_sampled_gradients(dev):
     err64 = rel_l2(got[:, rays], img64[:, rays])
     ref = our.backward_f64(p_np, geo, rm, st, dB.reshape(-1))
     errs = {name: rel_l2(g, r) for name, g, r in zip(("means", "scales", "quats", "atten", "sh"), grads, ref)}
-    print(f"[config3] image rel L2 vs FP32 {err32:.2e}, vs FP64 (4 rays) {err64:.2e}; gradients vs FP64: "
-          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    case = "UltraG-Ray config-3 frame (200k Gaussians, 128x1024)"
+    note(case, "image vs FP32", err32, 1e-5)
+    note(case, "image vs FP64 (4 scanlines)", err64, 1e-5)
+    for k, v in errs.items():
+        note(case, f"d{k} vs FP64 (4 scanlines)", v, 1e-4)
     assert err32 < 1e-5 and err64 < 1e-5
     for name, r in zip(errs, ref):
         assert np.abs(r).max() > 0, name
@@ -149,8 +153,6 @@ def test_config4_captured_step_equals_eager_and_sampled_frames_match_the_oracle(
         img, _a, _d, _s = oug.forward(means[g0:g1], tril[g0:g1], o_pk[g0:g1], c_pk[g0:g1], pix, geom, lists, sem,
                                       plane=plane)
         got = image0[base[k]:base[k + 1]].cpu().numpy()
-        err = rel_l2(got, img)
-        print(f"[config4] sweep {s}: frame {k} ({fr.width}x{fr.height}), {sw.g_count} Gaussians, "
-              f"{int(mr.count.sum())} instances, image rel L2 {err:.2e}", flush=True)
+        err = note("UltraGauss config 4: one frame per sweep, 8 sweeps", "image vs FP32", rel_l2(got, img), 1e-5)
         assert np.abs(img).max() > 0.05
         assert err < 1e-5

@@ -11,6 +11,7 @@ import pytest
 import torch
 
 from oracle import gp
+from tests.parity import note
 from tests.ug_helpers import rel_l2
 from us3d.batch import CapacityHint, FrameSpec, SweepSpec
 from us3d.gaussianpile import GaussianPlieRenderConfig, GaussianPlieRenderer, frame_from_pose, render_frames
@@ -88,14 +89,16 @@ def test_gp_forward_and_gradients_match_oracle(dev, ts, H, W):
     got = image.detach().cpu().reshape(fr.height, fr.width)
     ref32 = gp.render(*params, ofr, st)
     assert float(ref32.abs().max()) > 0.05
-    assert rel_l2(got.numpy(), ref32.numpy()) < IMG_TOL
+    case = "GaussianPile us_focus, tiles 8/16/32"
+    assert note(case, "image vs FP32", rel_l2(got.numpy(), ref32.numpy()), IMG_TOL) < IMG_TOL
     assert np.all(np.abs(got.numpy() - ref32.numpy()) <= 2e-5 * np.maximum(1.0, np.abs(ref32.numpy())))
     p64 = [p.double().clone().requires_grad_(True) for p in params]
     img64 = gp.render(*p64[:4], p64[4], ofr, st, fp64=True)
+    note(case, "image vs FP64", rel_l2(got.numpy(), img64.detach().numpy()), IMG_TOL)
     refs = [gp.sanitize_grad(r) for r in torch.autograd.grad((img64 * dI.double()).sum(), p64)]
     for name, t, r in zip(("means", "scales", "quats", "opacity", "intensity"), tens, refs):
         assert float(r.abs().max()) > 0, name
-        err = rel_l2(t.grad.detach().cpu().numpy(), r.numpy())
+        err = note(case, f"d{name} vs FP64", rel_l2(t.grad.detach().cpu().numpy(), r.numpy()), GRAD_TOL)
         assert err < GRAD_TOL, f"ts {ts} {name}: rel L2 {err:.3e}"
 
 
@@ -220,10 +223,13 @@ def test_gp_config2_full_size_sampled_pixels_and_gradients(dev):
     ref32 = torch.cat(ref32)
     assert float(ref32.abs().max()) > 0.05
     assert bool(torch.all((got - ref32).abs() <= 2e-5 * torch.clamp(ref32.abs(), min=1.0)))
+    case = "GaussianPile config 2: 4 frames x 32 pixels, 200k"
+    note(case, "pixels vs FP32 brute force", rel_l2(got.numpy(), ref32.numpy()), IMG_TOL)
     refs = gp.frame_gradients(p64, losses64)
     for name, t, ref in zip(("means", "scales", "quats", "opacity"), params, refs):
         assert float(ref.abs().max()) > 0, name
-        err = float((t.grad.detach().cpu().double() - ref).norm() / ref.norm())
+        err = note(case, f"d{name} vs FP64", float((t.grad.detach().cpu().double() - ref).norm() / ref.norm()),
+                   GRAD_TOL)
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
 
 

@@ -10,6 +10,7 @@ import torch
 
 from oracle import fallback as ofb
 from oracle import ug as oug
+from tests.parity import note
 from tests.ug_helpers import cfg_for, gpu_render, oracle_frame, rel_l2, small_case
 from us3d.batch import CapacityHint, SweepSpec
 from us3d.scenes import linear_pixels, make_scene, straight_sweep
@@ -66,13 +67,16 @@ def test_forward_backward_parity_cuda_semantics(dev, ts):
         img32, a32, d32, _ = oug.forward(*args)
         img64, a64, d64, _ = oug.forward(*args, fp64=True)
         got = {k: out[k].detach().cpu().numpy() for k in ("image", "alpha", "depth")}
-        assert rel_l2(got["image"], img32) < IMG_TOL
-        assert rel_l2(got["alpha"], a32) < IMG_TOL
-        assert rel_l2(got["depth"], d32) < IMG_TOL
-        assert rel_l2(got["image"], img64) < IMG_TOL
+        case = "UltraGauss, explicit pixels, tiles 8/16/32"
+        assert note(case, "image vs FP32", rel_l2(got["image"], img32), IMG_TOL) < IMG_TOL
+        assert note(case, "alpha vs FP32", rel_l2(got["alpha"], a32), IMG_TOL) < IMG_TOL
+        assert note(case, "depth vs FP32", rel_l2(got["depth"], d32), IMG_TOL) < IMG_TOL
+        assert note(case, "image vs FP64", rel_l2(got["image"], img64), IMG_TOL) < IMG_TOL
+        note(case, "alpha vs FP64", rel_l2(got["alpha"], a64), IMG_TOL)
+        note(case, "depth vs FP64", rel_l2(got["depth"], d64), IMG_TOL)
         ref = oug.backward_f64(*args, dI, dA, dD, gate=True)
         for name, g, r in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref):
-            err = rel_l2(g, r)
+            err = note(case, f"d{name} vs FP64", rel_l2(g, r), GRAD_TOL)
             assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
         # the reference's backward omits the alpha_hat > 1e-8 gate; the difference is below the bar too
         ref_q = oug.backward_f64(*args, dI, dA, dD, gate=False)
@@ -91,10 +95,13 @@ def test_fallback_semantics_vs_torch_fallback(dev):
     ts = [t.double().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
     img, alpha = ofb.render(*ts, linear_pixels(fr).double(), fr.height, fr.width)
     (img.reshape(-1) * torch.from_numpy(dI).double()).sum().backward()
-    assert rel_l2(out["image"].detach().cpu().numpy(), img.detach().numpy()) < IMG_TOL
-    assert rel_l2(out["alpha"].detach().cpu().numpy(), alpha.detach().numpy()) < IMG_TOL
+    case = "UltraGauss fallback semantics vs torch fallback"
+    assert note(case, "image vs FP64", rel_l2(out["image"].detach().cpu().numpy(), img.detach().numpy()),
+                IMG_TOL) < IMG_TOL
+    assert note(case, "alpha vs FP64", rel_l2(out["alpha"].detach().cpu().numpy(), alpha.detach().numpy()),
+                IMG_TOL) < IMG_TOL
     for name, g, t in zip(("means", "tril", "opacity", "intensity"), out["grads"], ts):
-        err = rel_l2(g, t.grad.numpy())
+        err = note(case, f"d{name} vs FP64", rel_l2(g, t.grad.numpy()), GRAD_TOL)
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
 
 
@@ -155,12 +162,13 @@ def test_grid_pixels_plane_form_parity(dev, ts):
         img64, a64, d64, _ = oug.forward(*args, fp64=True, plane=plane)
         got = {key: out[key].detach().cpu().numpy() for key in ("image", "alpha", "depth")}
         assert np.abs(img32).max() > 0.05
+        case = "UltraGauss, frames' grids (plane form), tiles 8/16/32"
         for key, r32, r64 in (("image", img32, img64), ("alpha", a32, a64), ("depth", d32, d64)):
-            assert rel_l2(got[key], r32) < IMG_TOL, key
-            assert rel_l2(got[key], r64) < IMG_TOL, key
+            assert note(case, f"{key} vs FP32", rel_l2(got[key], r32), IMG_TOL) < IMG_TOL, key
+            assert note(case, f"{key} vs FP64", rel_l2(got[key], r64), IMG_TOL) < IMG_TOL, key
         ref = oug.backward_f64(*args, dI, dA if full else None, dD if full else None, gate=True, plane=plane)
         for name, g, r in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref):
-            err = rel_l2(g, r)
+            err = note(case, f"d{name} vs FP64", rel_l2(g, r), GRAD_TOL)
             assert err < GRAD_TOL, f"ts {ts} full {full} {name}: rel L2 {err:.3e}"
 
 
@@ -273,15 +281,16 @@ def test_config2_frame_parity(dev):
     img, _a, _d, _s = oug.forward(*args, plane=plane)
     img64, _a, _d, _s = oug.forward(*args, fp64=True, plane=plane)
     assert mr.keys.size > 100_000
-    assert rel_l2(out["image"].detach().cpu().numpy(), img) < IMG_TOL
-    print(f"image rel L2: vs FP32 oracle {rel_l2(out['image'].detach().cpu().numpy(), img):.2e}, "
-          f"vs FP64 {rel_l2(out['image'].detach().cpu().numpy(), img64):.2e}")
+    case = "UltraGauss config-2 frame (200k Gaussians, 512x512)"
+    got = out["image"].detach().cpu().numpy()
+    assert note(case, "image vs FP32", rel_l2(got, img), IMG_TOL) < IMG_TOL
+    assert note(case, "image vs FP64", rel_l2(got, img64), IMG_TOL) < IMG_TOL
     ref = oug.backward_f64(*args, dI, None, None, gate=True, plane=plane)
     # the reference's own precision: its backward restated in FP32 (naive per-pair accumulation)
     ref32 = oug.backward_f64(*args, dI, None, None, gate=True, fp32=True, plane=plane)
     for name, g, r, r32 in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref, ref32):
-        err = rel_l2(g, r)
-        print(f"{name}: HIP vs FP64 {err:.2e}; FP32 reference restatement vs FP64 {rel_l2(r32, r):.2e}")
+        err = note(case, f"d{name} vs FP64", rel_l2(g, r), GRAD_TOL)
+        note("UltraGauss config-2 frame: FP32 restatement", f"d{name} vs FP64", rel_l2(r32, r), GRAD_TOL)
         assert err < GRAD_TOL, f"{name}: rel L2 {err:.3e}"
 
 
@@ -455,14 +464,15 @@ def test_fan_frames_match_brute_force_oracle(dev, semantics):
         assert np.abs(img32).max() > 1e-2
         got_i = image[off:off + n].detach().cpu().numpy()
         got_a = alpha[off:off + n].detach().cpu().numpy()
-        assert rel_l2(got_i, img32) < IMG_TOL
-        assert rel_l2(got_a, a32) < IMG_TOL
+        case = f"UltraGauss fan frames vs all-pairs oracle ({semantics})"
+        assert note(case, "image vs FP32", rel_l2(got_i, img32), IMG_TOL) < IMG_TOL
+        assert note(case, "alpha vs FP32", rel_l2(got_a, a32), IMG_TOL) < IMG_TOL
         for acc, r in zip(refs, oug.backward_f64(*args, dI[off:off + n], dA[off:off + n], None, gate=True)):
             acc += r
         off += n
     for name, t, r in zip(("means", "tril", "opacity", "intensity"), ts, refs):
         assert np.abs(r).max() > 0, name
-        err = rel_l2(t.grad.detach().cpu().numpy(), r)
+        err = note(case, f"d{name} vs FP64", rel_l2(t.grad.detach().cpu().numpy(), r), GRAD_TOL)
         assert err < GRAD_TOL, f"{semantics} {name}: rel L2 {err:.3e}"
 
 
@@ -655,8 +665,10 @@ def test_config5_fan_gradients_vs_fallback_on_64_frames(dev):
     img_err = rel_l2(np.concatenate(gpu_img), np.concatenate(ref_img))
     errs = {name: rel_l2(t.grad.detach().cpu().numpy(), r)
             for name, t, r in zip(("means", "tril", "opacity", "intensity"), tens, refs)}
-    print(f"[config 5, 64 frames] image rel L2 {img_err:.2e}; gradients rel L2 "
-          + ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    case = "UltraGauss config 5: 64 fan frames, 2M Gaussians"
+    note(case, "image vs FP32 fallback", img_err, IMG_TOL)
+    for k, v in errs.items():
+        note(case, f"d{k} vs FP64 fallback", v, GRAD_TOL)
     assert img_err < IMG_TOL
     for name, r in zip(errs, refs):
         assert np.abs(r).max() > 0, name
