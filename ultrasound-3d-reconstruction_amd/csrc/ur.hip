@@ -160,6 +160,14 @@ struct RayGauss {
     float dkf;                  // sorted depth (key / 1000)
 };
 
+// Sigma^-1 = R diag(1/s^2) R^T as (S00, S01, S02, S11, S12, S22).
+__device__ __forceinline__ void sigma_inv6(const float R[9], const float s[3], float S[6]) {
+    const float d0 = 1.0f / (s[0] * s[0]), d1 = 1.0f / (s[1] * s[1]), d2 = 1.0f / (s[2] * s[2]);
+    auto sij = [&](int i, int j) { return R[3 * i] * d0 * R[3 * j] + R[3 * i + 1] * d1 * R[3 * j + 1] + R[3 * i + 2] * d2 * R[3 * j + 2]; };
+    S[0] = sij(0, 0); S[1] = sij(0, 1); S[2] = sij(0, 2);
+    S[3] = sij(1, 1); S[4] = sij(1, 2); S[5] = sij(2, 2);
+}
+
 __device__ __forceinline__ void ray_gauss(const float4* __restrict__ rec, int g, uint32_t dkey, const float ro[3],
                                           const float rdir[3], float k_sigma, float Rw, RayGauss& G, float R[9]) {
     const float4 r0 = rec[4 * (int64_t)g + 0];
@@ -189,10 +197,10 @@ __device__ __forceinline__ void ray_gauss(const float4* __restrict__ rec, int g,
     G.e_final = __expf(-psi);
     G.f_final = __builtin_fmaf(1.0f - G.tau, G.e_final, G.tau);
     G.I = r3.x + r3.y * rdir[0] + r3.z * rdir[1] + r3.w * rdir[2];
-    const float d0 = 1.0f / (s[0] * s[0]), d1 = 1.0f / (s[1] * s[1]), d2 = 1.0f / (s[2] * s[2]);
-    auto sij = [&](int i, int j) { return R[3 * i] * d0 * R[3 * j] + R[3 * i + 1] * d1 * R[3 * j + 1] + R[3 * i + 2] * d2 * R[3 * j + 2]; };
-    G.S00 = sij(0, 0); G.S01 = sij(0, 1); G.S02 = sij(0, 2);
-    G.S11 = sij(1, 1); G.S12 = sij(1, 2); G.S22 = sij(2, 2);
+    float S6[6];
+    sigma_inv6(R, s, S6);
+    G.S00 = S6[0]; G.S01 = S6[1]; G.S02 = S6[2];
+    G.S11 = S6[3]; G.S12 = S6[4]; G.S22 = S6[5];
     G.dkf = (float)dkey * 0.001f;
 }
 
@@ -482,9 +490,63 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
 // cT / f_final times constants, so they enter through one FP64 suffix sum of cT, and each thread walks only
 // the samples of its echo range and open window (located by binary search with the same comparisons as the
 // per-sample rules). Otherwise every thread walks the whole chunk.
+// kGrid (pixels == NULL): the scanline's samples are the linear probe's grid points P(y) = P(0) + y e, e =
+// delta_ax a_ax, so along the ray the echo quadratic form is one-dimensional: with d(y) = P(y) - mu,
+// d^T Sigma^-1 d = Qc + 2 c1 tau + c2 tau^2 in tau = y - J about the closest approach J (set up in FP64 per hit;
+// c1 is then the FP64 residual, ~0), two multiply-adds per sample instead of a 3-vector difference and a 3x3
+// product, and the echo sums sum(dQ d) and sum(dQ d d^T) follow from three moments sum(dQ tau^k), k = 0..2,
+// with d = A + tau e, A = d(J) (ur_echo_moments_to_sums). The decisions (which terms a sample enters) are those
+// of the general form: they use the sample depths only.
 
 constexpr int kRayChunk = 1024;
 constexpr int kRayChunkPerThread = kRayChunk / kRayThreads;
+// Grid, transmission window on, monotone depths, one staged chunk ("itemized"): a hit's open window [jt, jf) and its
+// echo range [je0, je1) are cut into segments of kWinSeg and kEchoSeg samples, and each wavefront walks its 64
+// hits' segments 64 at a time, one segment per lane, so the lanes stay busy however unequal the hits' ranges are
+// (a thread walking its own hit's ranges kept 20 % of the lanes busy on the open windows of the config-3 scene,
+// 58 % on the echo ranges). The owning lane's parameters reach a segment's lane by cross-lane reads, and a hit's
+// per-segment sums return to it through a fixed-order segmented lane reduction: no LDS, deterministic.
+constexpr int kWinSeg = 8;
+constexpr int kEchoSeg = 32;
+
+// Wave-local segment numbering: lane l owns nseg segments; incl is the inclusive scan of nseg over the wave.
+__device__ __forceinline__ int wave_incl_scan(int nseg) {
+    const int lane = threadIdx.x & (kWave - 1);
+    int incl = nseg;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int v = __shfl_up(incl, off, kWave);
+        if (lane >= off) incl += v;
+    }
+    return incl;
+}
+
+// Lane owning segment k: the first lane whose inclusive count exceeds k (k < the wave's total).
+__device__ __forceinline__ int wave_seg_owner(int incl, int k) {
+    int owner = 0;
+#pragma unroll
+    for (int step = kWave / 2; step >= 1; step >>= 1)
+        if (__shfl(incl, owner + step - 1, kWave) <= k) owner += step;
+    return owner;
+}
+
+// Sums of NS values over runs of adjacent lanes with the same id; afterwards the first lane of each run holds the
+// run's total (runs of at most 64 lanes; the tree order is fixed by the lane positions).
+template <int NS>
+__device__ __forceinline__ void wave_run_sums(int id, float (&a)[NS]) {
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        // every lane takes part in the cross-lane reads (a read from a lane that skipped one is undefined)
+        const int oid = __shfl_down(id, off, kWave);
+        const bool same = (lane + off < kWave) && oid == id;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            const float v = __shfl_down(a[k], off, kWave);
+            a[k] = same ? a[k] + v : a[k];
+        }
+    }
+}
 
 __device__ __forceinline__ int wave_min_i(int v) {
 #pragma unroll
@@ -497,17 +559,50 @@ __device__ __forceinline__ int wave_max_i(int v) {
     return v;
 }
 
-__global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
+// Per hit: the 1-D echo quadratic along the grid scanline (see above) from the sample-0 position B and the step e;
+// returns A = d(J) and (Qc, 2 c1, c2, J).
+__device__ __forceinline__ void ur_echo_line(const float mu[3], const float S[6], const float B[3], const float e[3],
+                                             float A[3], float& Qc, float& c1x2, float& c2f, float& J) {
+    const double b0 = (double)B[0] - mu[0], b1 = (double)B[1] - mu[1], b2 = (double)B[2] - mu[2];
+    const double S00 = S[0], S01 = S[1], S02 = S[2], S11 = S[3], S12 = S[4], S22 = S[5];
+    const double e0 = e[0], e1 = e[1], e2 = e[2];
+    const double se0 = S00 * e0 + S01 * e1 + S02 * e2, se1 = S01 * e0 + S11 * e1 + S12 * e2,
+                 se2 = S02 * e0 + S12 * e1 + S22 * e2;
+    const double c2 = e0 * se0 + e1 * se1 + e2 * se2;
+    const double c1 = b0 * se0 + b1 * se1 + b2 * se2;
+    double j = c2 > 0.0 ? -c1 / c2 : 0.0;
+    j = fmin(fmax(j, -1.0e6), 1.0e6);
+    const double a0 = b0 + j * e0, a1 = b1 + j * e1, a2 = b2 + j * e2;
+    const double q = a0 * (S00 * a0 + S01 * a1 + S02 * a2) + a1 * (S01 * a0 + S11 * a1 + S12 * a2) +
+                     a2 * (S02 * a0 + S12 * a1 + S22 * a2);
+    A[0] = (float)a0; A[1] = (float)a1; A[2] = (float)a2;
+    Qc = (float)fmax(q, 0.0);
+    c1x2 = (float)(2.0 * (c1 + c2 * j));
+    c2f = (float)c2;
+    J = (float)j;
+}
+
+// Three waves per SIMD (at most 168 VGPRs): the echo-item phase would otherwise take the grid variant to 175 and two.
+template <bool kGrid>
+__global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void ur_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
     const float* __restrict__ ray_o, const float* __restrict__ ray_d, const float* __restrict__ ray_off, RayFlags F,
     const float4* __restrict__ aux, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
     float4* __restrict__ partial) {
-    __shared__ float4 s_p[kRayChunk];  // px, py, pz, tpx
+    // the general form needs the samples' positions; on the probe's grid only their depths are read
+    __shared__ float4 s_p[kGrid ? 1 : kRayChunk];   // px, py, pz, tpx
+    __shared__ float s_tp[kGrid ? kRayChunk : 1];   // tpx
     __shared__ float4 s_c[kRayChunk];  // cT, dNum, dS, E
     __shared__ double s_S[kRayChunk + 1];  // s_S[j] = sum of cT over samples j.. of the chunk
     __shared__ double s_wsum[kRayThreads / kWave];
+    auto tp_at = [&](int j) -> float {
+        if constexpr (kGrid) return s_tp[j]; else return s_p[j].w;
+    };
+    auto p_at = [&](int j) -> float4 {
+        if constexpr (kGrid) return make_float4(0.f, 0.f, 0.f, s_tp[j]); else return s_p[j];
+    };
     if (stats->overflow) return;
     const bool window_mode = F.Rw > 0.0f && !F.local_only;
     const int t = blockIdx.x;
@@ -520,6 +615,12 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
     float ro[3], rd[3];
     ray_origin(fr, x, ray_o, ray_d, ray_off, ro, rd);
     const int tid = threadIdx.x;
+    float B0[3] = {0.f, 0.f, 0.f}, estep[3] = {0.f, 0.f, 0.f};  // grid: sample 0 and the step between samples
+    if constexpr (kGrid) {
+        pixel_position(fr, nullptr, x, 0, B0[0], B0[1], B0[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) estep[k] = fr.delta_ax * fr.axis_ax[k];
+    }
 
     // Stage samples [c0, c0 + n) of the ray: positions and depths, the per-sample adjoint coefficients, and the
     // suffix sums of cT. Returns whether the sample depths are nondecreasing over the chunk.
@@ -544,7 +645,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                 dNum = dEraw / (a.z + 1e-6f);          // dL/dw = dNum (I - E) + dS (= dNum I + dDen + dS)
                 cT = gB * (gamma * E) * a.x;           // dL/dT * T
             }
-            s_p[j] = make_float4(p[0], p[1], p[2], tp);
+            if constexpr (kGrid) s_tp[j] = tp; else s_p[j] = make_float4(p[0], p[1], p[2], tp);
             s_c[j] = make_float4(cT, dNum, dS, E);
         }
         __syncthreads();
@@ -557,7 +658,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
             const int j = tid * kRayChunkPerThread + q;
             if (j < n) {
                 tot += (double)s_c[j].x;
-                if (j + 1 < n) mono = mono && (s_p[j].w <= s_p[j + 1].w);
+                if (j + 1 < n) mono = mono && (tp_at(j) <= tp_at(j + 1));
             }
             loc[q] = tot;
         }
@@ -595,7 +696,18 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
         float fin = 0.f, dtau = 0.f, dsh = 0.f;
         float u[3] = {0.f, 0.f, 0.f};
         float g00 = 0.f, g01 = 0.f, g02 = 0.f, g11 = 0.f, g12 = 0.f, g22 = 0.f;
+        float eQc = 0.f, ec1 = 0.f, ec2 = 0.f, eJ = 0.f;  // grid: the echo line (A is formed after the loop)
+        float M0 = 0.f, M1 = 0.f, M2 = 0.f;             // grid: sum(dQ tau^k)
+        if constexpr (kGrid) {
+            const float mu0[3] = {G.mu[0], G.mu[1], G.mu[2]};
+            const float S0[6] = {G.S00, G.S01, G.S02, G.S11, G.S12, G.S22};
+            float A0[3];
+            ur_echo_line(mu0, S0, B0, estep, A0, eQc, ec1, ec2, eJ);
+        }
 
+        // echo terms as items (see kEchoSeg): workgroup-uniform
+        const bool itemized = kGrid && one_chunk && window_mode && mono;
+        int jt0 = 0, jf0 = 0, je0 = 0, je1 = 0;  // this hit's open window and echo range (itemized)
         for (int c0 = 0; c0 < H; c0 += kRayChunk) {
             const int n = min(kRayChunk, H - c0);
             if (!one_chunk) mono = stage(c0, n);
@@ -604,13 +716,15 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
             int lo = n, hi = 0, jf = n;
             if (active) {
                 if (fast) {
-                    const int jt = first_true(0, n, [&](int m) { return G.tg < s_p[m].w; });
-                    const int jb = first_true(0, n, [&](int m) { return s_p[m].w >= G.b; });
+                    const int jt = first_true(0, n, [&](int m) { return G.tg < tp_at(m); });
+                    const int jb = first_true(0, n, [&](int m) { return tp_at(m) >= G.b; });
                     jf = max(jt, jb);
-                    const int je0 = first_true(0, n, [&](int m) { return (G.tg - s_p[m].w) <= G.rd; });
-                    const int je1 = first_true(je0, n, [&](int m) { return (G.tg - s_p[m].w) < -G.rd; });
-                    lo = min(jt, je0);
-                    hi = max(jf, je1);
+                    jt0 = jt;
+                    jf0 = jf;
+                    je0 = first_true(0, n, [&](int m) { return (G.tg - tp_at(m)) <= G.rd; });
+                    je1 = first_true(je0, n, [&](int m) { return (G.tg - tp_at(m)) < -G.rd; });
+                    lo = itemized ? n : min(jt, je0);  // itemized: both ranges run as segments below
+                    hi = itemized ? 0 : max(jf, je1);
                     const double S = s_S[jf];
                     if (S != 0.0 && G.f_final > 1e-6f) {
                         const float dTs = (float)(S / (double)G.f_final);
@@ -625,11 +739,11 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
             const int wlo = wave_min_i(lo), whi = wave_max_i(hi);
             // the next sample's position and coefficients are read before this sample's work, so the LDS
             // latency overlaps it
-            float4 pn = s_p[min(wlo, n - 1)], cn = s_c[min(wlo, n - 1)];
+            float4 pn = p_at(min(wlo, n - 1)), cn = s_c[min(wlo, n - 1)];
             for (int j = wlo; j < whi; ++j) {
                 const float4 pp = pn, cc = cn;
                 const int jn = (j + 1 < whi) ? j + 1 : j;
-                pn = s_p[jn];
+                pn = p_at(jn);
                 cn = s_c[jn];
                 if (j < lo || j >= hi) continue;
                 bool trans, echo;
@@ -684,25 +798,173 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                     }
                 }
                 if (echo) {
-                    float dx, dy, dz, v0, v1, v2;
-                    const float w = echo_weight(G, pp.x, pp.y, pp.z, dx, dy, dz, v0, v1, v2);
-                    dsh = __builtin_fmaf(cc.y, w, dsh);
-                    const float dw = __builtin_fmaf(cc.y, G.I - cc.w, cc.z);
-                    const float dQ = -0.5f * dw * w;
-                    u[0] = __builtin_fmaf(dQ, dx, u[0]);
-                    u[1] = __builtin_fmaf(dQ, dy, u[1]);
-                    u[2] = __builtin_fmaf(dQ, dz, u[2]);
-                    const float qx = dQ * dx, qy = dQ * dy;
-                    g00 = __builtin_fmaf(qx, dx, g00);
-                    g01 = __builtin_fmaf(qx, dy, g01);
-                    g02 = __builtin_fmaf(qx, dz, g02);
-                    g11 = __builtin_fmaf(qy, dy, g11);
-                    g12 = __builtin_fmaf(qy, dz, g12);
-                    g22 = __builtin_fmaf(dQ * dz, dz, g22);
+                    if constexpr (kGrid) {
+                        const float tq = (float)(c0 + j) - eJ;
+                        const float w = __expf(-0.5f * __builtin_fmaf(__builtin_fmaf(ec2, tq, ec1), tq, eQc));
+                        dsh = __builtin_fmaf(cc.y, w, dsh);
+                        const float dQ = __builtin_fmaf(cc.y, G.I - cc.w, cc.z) * w;  // -2 dL/dQ
+                        M0 += dQ;
+                        M1 = __builtin_fmaf(dQ, tq, M1);
+                        M2 = __builtin_fmaf(dQ * tq, tq, M2);
+                    } else {
+                        float dx, dy, dz, v0, v1, v2;
+                        const float w = echo_weight(G, pp.x, pp.y, pp.z, dx, dy, dz, v0, v1, v2);
+                        dsh = __builtin_fmaf(cc.y, w, dsh);
+                        const float dw = __builtin_fmaf(cc.y, G.I - cc.w, cc.z);
+                        const float dQ = -0.5f * dw * w;
+                        u[0] = __builtin_fmaf(dQ, dx, u[0]);
+                        u[1] = __builtin_fmaf(dQ, dy, u[1]);
+                        u[2] = __builtin_fmaf(dQ, dz, u[2]);
+                        const float qx = dQ * dx, qy = dQ * dy;
+                        g00 = __builtin_fmaf(qx, dx, g00);
+                        g01 = __builtin_fmaf(qx, dy, g01);
+                        g02 = __builtin_fmaf(qx, dz, g02);
+                        g11 = __builtin_fmaf(qy, dy, g11);
+                        g12 = __builtin_fmaf(qy, dz, g12);
+                        g22 = __builtin_fmaf(dQ * dz, dz, g22);
+                    }
+                }
+            }
+        }
+        if (itemized) {
+            const int lane = tid & (kWave - 1);
+            // open windows: per sample the GL3 transmission of the window [ts, l] and its VJP with the endpoints
+            // held fixed (backward.cu:57-72), as in the per-lane loop above; samples in [jt, jf) are in front of
+            // the hit and before its window end, so te = l
+            {
+                const int ns = (active && jf0 > jt0) ? (jf0 - jt0 + kWinSeg - 1) / kWinSeg : 0;
+                const int incl = wave_incl_scan(ns);
+                const int excl = incl - ns;
+                const int total = __shfl(incl, kWave - 1, kWave);
+                for (int base = 0; base < total; base += kWave) {
+                    const int k = base + lane;
+                    const int o = wave_seg_owner(incl, min(k, total - 1));
+                    const float og0 = __shfl(G.og[0], o, kWave), og1 = __shfl(G.og[1], o, kWave),
+                                og2 = __shfl(G.og[2], o, kWave);
+                    const float dg0 = __shfl(G.dg[0], o, kWave), dg1 = __shfl(G.dg[1], o, kWave),
+                                dg2 = __shfl(G.dg[2], o, kWave);
+                    const float ts = __shfl(G.ts, o, kWave), otau = __shfl(G.tau, o, kWave);
+                    const int oa = __shfl(jt0, o, kWave), ob = __shfl(jf0, o, kWave), oex = __shfl(excl, o, kWave);
+                    float acc7[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};  // dog, ddg, dtau
+                    if (k < total) {
+                        const int j0 = oa + (k - oex) * kWinSeg, j1 = min(j0 + kWinSeg, ob);
+                        for (int j = j0; j < j1; ++j) {
+                            const float l = tp_at(j), cT = s_c[j].x;
+                            if (!(l > 0.0f && l > ts)) continue;  // empty window: psi = 0, T_i = 1
+                            const float half = 0.5f * (l - ts), c = 0.5f * (l + ts);
+                            const float xi[3] = {-kXi, 0.0f, kXi};
+                            const float wq[3] = {kW0, kW1, kW0};
+                            float tq[3], x0[3], x1[3], x2[3], eq[3];
+                            float acc = 0.f;
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                tq[q] = __builtin_fmaf(half, xi[q], c);
+                                x0[q] = __builtin_fmaf(dg0, tq[q], og0);
+                                x1[q] = __builtin_fmaf(dg1, tq[q], og1);
+                                x2[q] = __builtin_fmaf(dg2, tq[q], og2);
+                                eq[q] = __expf(-0.5f * dot3(x0[q], x1[q], x2[q], x0[q], x1[q], x2[q]));
+                                acc = __builtin_fmaf(wq[q], eq[q], acc);
+                            }
+                            const float ep = __expf(-(half * acc));
+                            const float Ti = __builtin_fmaf(1.0f - otau, ep, otau);
+                            const float dTi = Ti > 1e-6f ? cT * __builtin_amdgcn_rcpf(Ti) : 0.f;
+                            acc7[6] = __builtin_fmaf(dTi, 1.0f - ep, acc7[6]);
+                            const float coeff = -dTi * (1.0f - otau) * ep * half;  // d psi * half
+#pragma unroll
+                            for (int q = 0; q < 3; ++q) {
+                                const float aq = -coeff * wq[q] * eq[q];
+                                acc7[0] = __builtin_fmaf(aq, x0[q], acc7[0]);
+                                acc7[1] = __builtin_fmaf(aq, x1[q], acc7[1]);
+                                acc7[2] = __builtin_fmaf(aq, x2[q], acc7[2]);
+                                const float at = aq * tq[q];
+                                acc7[3] = __builtin_fmaf(at, x0[q], acc7[3]);
+                                acc7[4] = __builtin_fmaf(at, x1[q], acc7[4]);
+                                acc7[5] = __builtin_fmaf(at, x2[q], acc7[5]);
+                            }
+                        }
+                    }
+                    wave_run_sums<7>(k < total ? o : -1 - lane, acc7);
+                    // this lane's hit: its segments in this pass start at lane max(excl, base) - base
+                    const bool mine = ns > 0 && excl < base + kWave && excl + ns > base;
+                    const int head = max(excl - base, 0) & (kWave - 1);
+                    float got[7];
+#pragma unroll
+                    for (int m = 0; m < 7; ++m) got[m] = __shfl(acc7[m], head, kWave);
+                    if (mine) {
+                        dog[0] += got[0]; dog[1] += got[1]; dog[2] += got[2];
+                        ddg[0] += got[3]; ddg[1] += got[4]; ddg[2] += got[5];
+                        dtau += got[6];
+                    }
+                }
+            }
+            // echo ranges: the one-dimensional echo quadratic along the grid scanline (ur_echo_line)
+            {
+                const int ns = (active && je1 > je0) ? (je1 - je0 + kEchoSeg - 1) / kEchoSeg : 0;
+                const int incl = wave_incl_scan(ns);
+                const int excl = incl - ns;
+                const int total = __shfl(incl, kWave - 1, kWave);
+                for (int base = 0; base < total; base += kWave) {
+                    const int k = base + lane;
+                    const int o = wave_seg_owner(incl, min(k, total - 1));
+                    const float qc = __shfl(eQc, o, kWave), q1 = __shfl(ec1, o, kWave), q2 = __shfl(ec2, o, kWave),
+                                qj = __shfl(eJ, o, kWave), qI = __shfl(G.I, o, kWave);
+                    const int oa = __shfl(je0, o, kWave), ob = __shfl(je1, o, kWave), oex = __shfl(excl, o, kWave);
+                    float acc4[4] = {0.f, 0.f, 0.f, 0.f};  // dsh, sum dQ, sum dQ tau, sum dQ tau^2
+                    if (k < total) {
+                        const int j0 = oa + (k - oex) * kEchoSeg, j1 = min(j0 + kEchoSeg, ob);
+                        for (int j = j0; j < j1; ++j) {
+                            const float4 cc = s_c[j];
+                            const float tq = (float)j - qj;
+                            const float w = __expf(-0.5f * __builtin_fmaf(__builtin_fmaf(q2, tq, q1), tq, qc));
+                            acc4[0] = __builtin_fmaf(cc.y, w, acc4[0]);
+                            const float dQ = __builtin_fmaf(cc.y, qI - cc.w, cc.z) * w;  // -2 dL/dQ
+                            acc4[1] += dQ;
+                            acc4[2] = __builtin_fmaf(dQ, tq, acc4[2]);
+                            acc4[3] = __builtin_fmaf(dQ * tq, tq, acc4[3]);
+                        }
+                    }
+                    wave_run_sums<4>(k < total ? o : -1 - lane, acc4);
+                    const bool mine = ns > 0 && excl < base + kWave && excl + ns > base;
+                    const int head = max(excl - base, 0) & (kWave - 1);
+                    float got[4];
+#pragma unroll
+                    for (int m = 0; m < 4; ++m) got[m] = __shfl(acc4[m], head, kWave);
+                    if (mine) {
+                        dsh += got[0];
+                        M0 += got[1];
+                        M1 += got[2];
+                        M2 += got[3];
+                    }
                 }
             }
         }
         if (!active) continue;
+        // the record's rotation, Sigma^-1 and mean, formed again here rather than kept in registers through the
+        // sample loop (the same operations as ray_gauss, so the same bits)
+        const int gi = (int)(uint32_t)key;
+        const float4 r0 = rec[4 * (int64_t)gi + 0];
+        const float4 r1 = rec[4 * (int64_t)gi + 1];
+        const float4 r2 = rec[4 * (int64_t)gi + 2];
+        const float s[3] = {r1.x, r1.y, r1.z};
+        const float mu[3] = {r0.x, r0.y, r0.z};
+        float Rm[9], Sm[6];
+        quat_rot(r2, Rm);
+        sigma_inv6(Rm, s, Sm);
+        if constexpr (kGrid) {
+            float eA[3], qc_, c1_, c2_, j_;
+            ur_echo_line(mu, Sm, B0, estep, eA, qc_, c1_, c2_, j_);
+            // sum(dQ d) and sum(dQ d d^T) with d = A + tau e and the factor -1/2 of dL/dQ
+            const float m0 = -0.5f * M0, m1 = -0.5f * M1, m2 = -0.5f * M2;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) u[k] = __builtin_fmaf(m1, estep[k], m0 * eA[k]);
+            auto gab = [&](int a, int b) {
+                return __builtin_fmaf(m2 * estep[a], estep[b],
+                                      __builtin_fmaf(m1, __builtin_fmaf(eA[a], estep[b], estep[a] * eA[b]),
+                                                     m0 * eA[a] * eA[b]));
+            };
+            g00 = gab(0, 0); g01 = gab(0, 1); g02 = gab(0, 2);
+            g11 = gab(1, 1); g12 = gab(1, 2); g22 = gab(2, 2);
+        }
         // samples past the window: one VJP with the summed d psi
         if (fin != 0.0f && G.b > G.ts && G.b > 0.0f) {
             const float ts = G.ts, te = G.b;
@@ -721,10 +983,7 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
                 ddg[0] += a * tq * x0; ddg[1] += a * tq * x1; ddg[2] += a * tq * x2;
             }
         }
-        const float4 r1 = rec[4 * (int64_t)(uint32_t)key + 1];
-        const float4 r2 = rec[4 * (int64_t)(uint32_t)key + 2];
-        const float s[3] = {r1.x, r1.y, r1.z};
-        const float diff[3] = {ro[0] - G.mu[0], ro[1] - G.mu[1], ro[2] - G.mu[2]};
+        const float diff[3] = {ro[0] - mu[0], ro[1] - mu[1], ro[2] - mu[2]};
         float dmu[3], ds[3], dR[9];
         float dol[3], ddl[3];
 #pragma unroll
@@ -735,25 +994,25 @@ __global__ __launch_bounds__(kRayThreads) void ur_backward_kernel(
         }
 #pragma unroll
         for (int r = 0; r < 3; ++r) {
-            dmu[r] = -(R[3 * r] * dol[0] + R[3 * r + 1] * dol[1] + R[3 * r + 2] * dol[2]);
+            dmu[r] = -(Rm[3 * r] * dol[0] + Rm[3 * r + 1] * dol[1] + Rm[3 * r + 2] * dol[2]);
 #pragma unroll
             for (int c = 0; c < 3; ++c) dR[3 * r + c] = dol[c] * diff[r] + ddl[c] * rd[r];
         }
         // echo: d mu = -2 Sigma^-1 u; Sigma^-1 = R D R^T with D = diag(1/s^2): dR += 2 Gm R D (Gm symmetric),
         // ds_k += (R^T Gm R)_kk * (-2 / s_k^3)
-        dmu[0] += -2.0f * (G.S00 * u[0] + G.S01 * u[1] + G.S02 * u[2]);
-        dmu[1] += -2.0f * (G.S01 * u[0] + G.S11 * u[1] + G.S12 * u[2]);
-        dmu[2] += -2.0f * (G.S02 * u[0] + G.S12 * u[1] + G.S22 * u[2]);
+        dmu[0] += -2.0f * (Sm[0] * u[0] + Sm[1] * u[1] + Sm[2] * u[2]);
+        dmu[1] += -2.0f * (Sm[1] * u[0] + Sm[3] * u[1] + Sm[4] * u[2]);
+        dmu[2] += -2.0f * (Sm[2] * u[0] + Sm[4] * u[1] + Sm[5] * u[2]);
         const float Gm[9] = {g00, g01, g02, g01, g11, g12, g02, g12, g22};
         const float D[3] = {1.0f / (s[0] * s[0]), 1.0f / (s[1] * s[1]), 1.0f / (s[2] * s[2])};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             float gr[3];  // Gm * column k of R
 #pragma unroll
-            for (int a = 0; a < 3; ++a) gr[a] = Gm[3 * a] * R[k] + Gm[3 * a + 1] * R[3 + k] + Gm[3 * a + 2] * R[6 + k];
+            for (int a = 0; a < 3; ++a) gr[a] = Gm[3 * a] * Rm[k] + Gm[3 * a + 1] * Rm[3 + k] + Gm[3 * a + 2] * Rm[6 + k];
 #pragma unroll
             for (int a = 0; a < 3; ++a) dR[3 * a + k] += 2.0f * gr[a] * D[k];
-            const float rgr = R[k] * gr[0] + R[3 + k] * gr[1] + R[6 + k] * gr[2];
+            const float rgr = Rm[k] * gr[0] + Rm[3 + k] * gr[1] + Rm[6 + k] * gr[2];
             ds[k] += rgr * (-2.0f / (s[k] * s[k] * s[k]));
         }
         const float4 dq = quat_vjp(r2, dR);
@@ -911,7 +1170,9 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
     hipStream_t stream = as_stream(stream_);
     {
         StageTimer timer_("ur_backward_kernel", stream);
-        hipLaunchKernelGGL(ur_backward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
+        // the grid form of the echo sums when the samples are the linear probe's grid (see ur_backward_kernel)
+        auto kernel = pixels ? ur_backward_kernel<false> : ur_backward_kernel<true>;
+        hipLaunchKernelGGL(kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
                            ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
                            ws_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
                            ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
