@@ -186,6 +186,64 @@ def cpu_baseline(n_rows: int, runs: int):
     }
 
 
+def parity_config2_frame(dev) -> dict:
+    """Checker run beside the CPU baseline: the middle frame of the benchmark sweep (200k Gaussians, 512x512 on the
+    frame's own grid, tile 16, the bench's image model) rendered and differentiated on the GPU through the C ABI,
+    against the oracle's FP32 (canonical order) and FP64 passes on the same inputs: relative L2 error of the image,
+    the alpha and every gradient group, for dL = <dI, image> + <dA, alpha> with seeded dI, dA."""
+    import numpy as np
+    import torch
+
+    from oracle import ug as oug
+    from us3d.batch import CapacityHint, SweepSpec
+    from us3d.scenes import config2, linear_pixels
+    from us3d.ultragauss import UltraGaussRenderConfig, render_frames
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64).reshape(-1)
+        b = np.asarray(b, np.float64).reshape(-1)
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    t0 = time.perf_counter()
+    scene, _target, frames = config2()
+    fr = frames[len(frames) // 2]
+    P = fr.height * fr.width
+    rng = np.random.default_rng(9)
+    dI = rng.normal(size=P).astype(np.float32) / P
+    dA = rng.normal(size=P).astype(np.float32) / P
+    cfg = UltraGaussRenderConfig(H=fr.height, W=fr.width, lateral_spacing_mm_per_pix=fr.delta_lat,
+                                 axial_spacing_mm_per_pix=fr.delta_ax, axial_near_mm=fr.near_mm,
+                                 lateral_col_offset=fr.col_offset, fixed_bg_alpha=1e-6)
+    ts = [t.to(dev).clone().requires_grad_(True) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)]
+    image, alpha, _depth = render_frames(*ts, None, [fr], cfg, sweeps=[SweepSpec(0, scene.means.shape[0])],
+                                         hint=CapacityHint())
+    ((image * torch.from_numpy(dI).to(dev)).sum() + (alpha * torch.from_numpy(dA).to(dev)).sum()).backward()
+    torch.cuda.synchronize()
+    sem = oug.Semantics()
+    geom = oug.FrameGeom.from_spec(fr)
+    pix = linear_pixels(fr).numpy()
+    m, t = scene.means.numpy(), scene.tril.numpy()
+    mr = oug.map_frame(m, t, geom, chi2=sem.chi2, beta=sem.beta, sort_by_depth=True, ts=16)
+    lists = oug.pixel_lists(m, t, pix, geom, mr, chi2=sem.chi2, beta=sem.beta, ts=16)
+    plane = oug.plane_form(m, t, geom)
+    args = (m, t, scene.opacity.numpy(), scene.intensity.numpy(), pix, geom, lists, sem)
+    img32, a32, _d, _s = oug.forward(*args, plane=plane)
+    img64, a64, _d, _s = oug.forward(*args, fp64=True, plane=plane)
+    refs = oug.backward_f64(*args, dI, dA, None, gate=True, plane=plane)
+    got_i, got_a = image.detach().cpu().numpy(), alpha.detach().cpu().numpy()
+    out = {"workload": f"frame {len(frames) // 2} of the benchmark sweep, {scene.means.shape[0]} Gaussians, "
+                       f"{fr.height}x{fr.width}, {int(mr.keys.size)} instances",
+           "image_vs_fp32": rel(got_i, img32), "image_vs_fp64": rel(got_i, img64),
+           "alpha_vs_fp32": rel(got_a, a32), "alpha_vs_fp64": rel(got_a, a64)}
+    for name, tt, r in zip(("means", "tril", "opacity", "intensity"), ts, refs):
+        out[f"d{name}_vs_fp64"] = rel(tt.grad.detach().cpu().numpy(), r)
+    out["bars"] = {"image_alpha": 1e-5, "gradients": 1e-4}
+    out["ok"] = bool(max(out["image_vs_fp32"], out["image_vs_fp64"], out["alpha_vs_fp32"], out["alpha_vs_fp64"]) < 1e-5
+                     and max(out[f"d{n}_vs_fp64"] for n in ("means", "tril", "opacity", "intensity")) < 1e-4)
+    out["wall_s"] = round(time.perf_counter() - t0, 1)
+    return out
+
+
 # ---------------------------------------------------------------- rocprof summaries and the figures read from them
 
 def _short(name: str) -> str:
@@ -958,6 +1016,11 @@ def main():
             log(f"nested workload {name} ...")
             line["workloads"][name] = nested(name, args)
     line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample_rows, args.cpu_runs)
+    if not args.no_cpu_baseline:  # the oracle leg: its checker figures beside its timing
+        import torch
+
+        log("parity of one config-2 frame against the oracle ...")
+        line["parity"] = parity_config2_frame(torch.device("cuda:0"))
     print(json.dumps(line), flush=True)
 
 

@@ -31,3 +31,10 @@ def pytest_terminal_summary(terminalreporter):
     tr.section("parity against the oracle: largest relative L2 error per output")
     for (case, output), (err, bar, n) in sorted(mod.RESULTS.items()):
         tr.write_line(f"{case:<52} {output:<26} {err:9.2e}  bar {bar:.0e}  ({n} checks)")
+    # one closing line: the worst error of each bar over every case, with its share of the bar
+    worst = {}
+    for (case, output), (err, bar, _n) in mod.RESULTS.items():
+        if bar not in worst or err / bar > worst[bar][0] / bar:
+            worst[bar] = (err, f"{case}: {output}")
+    tr.write_line("parity worst: " + "; ".join(f"{e:.2e} of bar {b:.0e} ({where})"
+                                                for b, (e, where) in sorted(worst.items())))
