@@ -445,12 +445,10 @@ def roofline(stages, launch_bytes, profile_key, workload_tag):
 
 
 def visible_pairs(trainer) -> int:
-    import torch
+    """Visible (frame, Gaussian) pairs of the trainer's last map: its us3d_stats.n_visible."""
+    from us3d.batch import read_stats
 
-    off = trainer.batch.layout_offsets(trainer.k_capacity)
-    lay = trainer.batch.layout
-    pc = trainer.workspace[off.pair_count: off.pair_count + 4 * lay.n_pairs].view(torch.int32)
-    return int((pc > 0).sum().item())
+    return int(read_stats(trainer.workspace).n_visible)
 
 
 # ---------------------------------------------------------------- workloads
@@ -535,8 +533,9 @@ def bench_slice(args):
         "ug_backward_kernel": 36 * K + 8 * P + 32 * g_vis,
         "ug_forward_kernel": 36 * K + 16 * P,
         "sort_small_segments_kernel": 24 * K,
-        "ug_emit_kernel": 4 * F * G + 28 * g_vis + 12 * K,
-        "ug_preprocess_kernel": 48 * G + 32 * F * G + 4 * K,
+        "emit_instances_kernel": 28 * g_vis + 12 * K,
+        "ug_cull_kernel": 48 * G + F * G // 8,
+        "ug_setup_kernel": 4 * g_vis + 48 * G + 80 * g_vis,
         "ug_reduce_kernel": 48 * K + 48 * G,
     }
     tag = f"{F}x{H}x{W}_g{G}"
@@ -735,9 +734,7 @@ def bench_fan(args):
         for c in tt.chunks:
             m = c[4].map(tt.means, tt.tril, tt.raw_o, tt.raw_c)
             K += int(m.stats.n_instances)
-            off = c[4].layout_offsets(m.k_capacity)
-            pc = m.workspace[off.pair_count: off.pair_count + 4 * c[4].layout.n_pairs].view(torch.int32)
-            g_vis += int((pc > 0).sum().item())
+            g_vis += int(m.stats.n_visible)
             del m
     del tt
     torch.cuda.empty_cache()

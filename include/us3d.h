@@ -14,10 +14,11 @@
  *     fixed static per-GPU array instead). Every entry point is stream-ordered on
  *     `stream` (a hipStream_t passed as void*; NULL = legacy default stream), performs no host
  *     synchronisation and issues no hipMemset, so a call sequence can be captured into a hipGraph.
- *   - Results that depend on data (instance counts, overflow) are written to the us3d_stats block at
- *     offset 0 of the workspace. If the instance count exceeds the plan's k_capacity, the kernels that
- *     would write past it return early, stats.overflow is set and the caller must retry with a larger
- *     capacity (the reference instead blocks on a device->host copy to learn the count).
+ *   - Results that depend on data (instance and visible-pair counts, overflow) are written to the us3d_stats
+ *     block at offset 0 of the workspace. If the instance count exceeds the plan's k_capacity, or the visible
+ *     (frame, Gaussian) pairs exceed its v_capacity, the kernels that would write past them return early,
+ *     stats.overflow is set and the caller must retry with larger capacities (the reference instead blocks on
+ *     a device->host copy to learn the count).
  *   - Return value: US3D_OK or an error code; us3d_last_error() describes the last failure on the
  *     calling thread.
  *   - Floating point: FP32 storage and arithmetic. Tile rectangles, keys, and the q <= chi2 test use
@@ -83,18 +84,23 @@ typedef struct us3d_sweep {
  * overflow_any, which only the caller clears (write 0 to it after allocating the workspace, e.g. with
  * us3d_fill_i32): a run of replayed steps can then be checked for any overflowed step with one read. */
 typedef struct us3d_stats {
-    int64_t n_instances;    /* K: (tile, Gaussian) or (ray, Gaussian) instances of the batch */
+    int64_t n_instances;    /* K: (tile, Gaussian) or (ray, Gaussian) instances of the batch (0 when the visible
+                             * pairs overflowed: the instances are then not counted) */
     int64_t max_segment;    /* longest per-tile / per-ray list */
-    int32_t overflow;       /* 1 if n_instances > k_capacity (outputs are then invalid) */
+    int32_t overflow;       /* 1 if n_visible > v_capacity or n_instances > k_capacity (outputs are then invalid) */
     int32_t n_sorted_big;   /* segments that took the multi-chunk merge path */
     int32_t overflow_any;   /* sticky: set to 1 by every map call that overflows, never cleared by the library */
     int32_t reserved_i32;
-    int64_t reserved[4];    /* mapper scratch (reserved[0]: tile total); zeroed by every map call */
+    int64_t n_visible;      /* visible (frame, Gaussian) pairs: those with at least one instance */
+    int64_t reserved[3];    /* mapper scratch (reserved[0]: tile total, reserved[1]: frame-major total); zeroed
+                             * by every map call */
 } us3d_stats;
 
 /* A batch: frames (possibly from several sweeps) that are mapped, rendered and differentiated together.
  * Host struct holding device pointers. n_tiles = sum of frames' tiles_x*tiles_y (UltraGauss) or of their
- * widths (UltraG-Ray); n_pairs = sum over sweeps of g_count*f_count. n_pairs and n_pixels must be < 2^31. */
+ * widths (UltraG-Ray); n_pairs = sum over sweeps of g_count*f_count. n_pairs and n_pixels must be < 2^31.
+ * Per-pair records are kept only for visible pairs (at least one instance), at most v_capacity of them; the
+ * workspace holds 0.625 bytes per (frame, Gaussian) pair besides (two bit masks and the scans of their words). */
 typedef struct us3d_plan {
     int32_t n_gauss;                /* rows of the concatenated Gaussian arrays */
     int32_t n_frames;
@@ -103,6 +109,7 @@ typedef struct us3d_plan {
     int64_t n_pairs;
     int64_t n_pixels;               /* sum of H*W */
     int64_t k_capacity;             /* capacity of instance arrays in the workspace (< 2^31) */
+    int64_t v_capacity;             /* capacity of the visible-pair arrays in the workspace (< 2^31) */
     const us3d_frame* frames;       /* [n_frames] */
     const int32_t* frame_tile_base; /* [n_frames+1] prefix of per-frame tile counts */
     const int32_t* frame_pixel_base;/* [n_frames+1] prefix of per-frame pixel counts */
@@ -133,26 +140,34 @@ typedef struct us3d_ug_settings {
                              * it only through the FP32 rounding of the per-Gaussian sums. */
 } us3d_ug_settings;
 
-/* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by both renderers.
- * Arrays: pair_count i32[n_pairs], pair_off i32[n_pairs+1], pair_rect i32x4[n_pairs] (tile rectangle
- * [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)) and pair_dk u32[n_pairs] (depth key) -- both
- * written only where pair_count > 0 --, tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor
- * i32[n_tiles] (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (emit index),
- * records (per Gaussian), partial (per instance), vis_list i32[n_pairs] (frame-major copy of the pair
- * index: per frame a g_count-sized region, entry = pair with a nonzero count, else -1), inst_mask u16[K]
- * (UltraGauss: the sub-block mask of each sorted instance, written by the forward, read by the backward;
- * empty for UltraG-Ray), pair_sect f32[n_pairs][12] (UltraGauss: per visible pair, the pair's part of the
- * sub-block test and the plane form of q on the frame's grid, written by the map, read by the forward and the
- * backward; empty for UltraG-Ray). */
+/* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by the renderers.
+ * (frame, Gaussian) pairs are numbered densely (us3d_sweep); visible pairs (at least one instance) get a compact
+ * index v, their rank in that order, so a Gaussian's visible pairs are consecutive in frame order.
+ * Arrays: pair_bits u64[ceil(n_pairs/64)] (bit p: pair p is visible), word_cnt i32[words] and word_off
+ * i32[words+1] (popcount of each word and its exclusive scan: v = word_off[p/64] + popcount of the bits below p),
+ * fm_bits u32[fm_words] (frame-major visibility: per frame ceil(g_count/32) words, bit gl for Gaussian
+ * g_begin+gl), fm_cnt / fm_off (their popcounts and scan: the slot of a visible pair in vis_list), vis_pair
+ * i32[v_cap] (dense pair index of v), pair_count i32[v_cap] (instances of v), pair_off i32[v_cap+1] (first emit
+ * index), pair_rect i32x4[v_cap] (tile rectangle [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)),
+ * pair_dk u32[v_cap] (depth key), vis_list i32[v_cap] (frame-major list of v: a frame's visible pairs are
+ * consecutive, in Gaussian order), tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor i32[n_tiles]
+ * (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (the instance's visible pair v;
+ * its emit index is pair_off[v] plus its row-major position in the pair's rectangle), records (per Gaussian),
+ * partial (per instance, at the emit index), inst_mask u16[K] (UltraGauss: the sub-block mask of each sorted
+ * instance, written by the forward, read by the backward; empty for UltraG-Ray), pair_sect (per visible pair:
+ * UltraGauss f32[12], its part of the sub-block test and the plane form of q on the frame's grid; GaussianPile
+ * f32[8], its focus footprint; written by the map, read by the forward and the backward). */
 typedef struct us3d_layout {
-    int64_t stats, pair_count, pair_off, pair_rect, pair_dk, tile_count, tile_off;
+    int64_t stats, pair_bits, word_cnt, word_off, fm_bits, fm_cnt, fm_off, vis_pair;
+    int64_t pair_count, pair_off, pair_rect, pair_dk, vis_list, tile_count, tile_off;
     int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
-    int64_t vis_list, inst_mask, pair_sect, total;
+    int64_t inst_mask, pair_sect, total;
 } us3d_layout;
 
-int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
+int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                                int64_t v_capacity);
 int us3d_ug_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
-                             us3d_layout* out);
+                             int64_t v_capacity, us3d_layout* out);
 
 /* Mapper: replaces MapperForwardCUDA (ops/ultragauss_renderer/src/mapper/wrapper_mapper.cu:35-206,
  * declared at ops/ultragauss_renderer/include/mapper/wrapper_mapper.cuh:4-22). Packs the Gaussians,
@@ -200,8 +215,9 @@ int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_u
 /* No-allocation render. No counterpart in the reference: for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). Runs us3d_ug_map then
  * us3d_ug_forward on the same arguments, with the workspace taken from a static per-GPU array of
- * US3D_STATIC_SCRATCH_BYTES instead of the plan's: plan->workspace, workspace_bytes and k_capacity are ignored (may
- * be 0), the instance capacity is the largest that fits. A batch whose workspace cannot fit is rejected with
+ * US3D_STATIC_SCRATCH_BYTES instead of the plan's: plan->workspace, workspace_bytes, k_capacity and v_capacity
+ * are ignored (may be 0); the instance capacity is the largest that fits with a visible-pair capacity of
+ * min(n_pairs, instance capacity), which cannot overflow before the instances do. A batch whose workspace cannot fit is rejected with
  * US3D_ERR_CAPACITY before any device work. overflow_dev (device int, may be NULL) receives 1 if the instances
  * exceeded that capacity (the outputs are then invalid), else 0. Allocates nothing (no hipMalloc/hipFree); the first
  * call on a device looks the array up with hipGetSymbolAddress. Each GPU's array keeps the last call's contents, so
@@ -227,9 +243,10 @@ typedef struct us3d_ur_settings {
     int32_t reserved1;
 } us3d_ur_settings;
 
-int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity);
+int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity,
+                                int64_t v_capacity);
 int us3d_ur_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity,
-                             us3d_layout* out);
+                             int64_t v_capacity, us3d_layout* out);
 
 /* Ray mapper: replaces MapperForwardCUDA of ops/ultragray_renderer (src/mapper/wrapper_mapper.cu:7-128,
  * core.cu:3-129). means [G,3], scales [G,3], quats [G,4] (w,x,y,z), attenuations [G] (raw logits),
@@ -282,9 +299,10 @@ typedef struct us3d_gp_settings {
 } us3d_gp_settings;
 
 /* Workspace for a batch (same plan structure as UltraGauss: tiles of tile_size px per frame). */
-int64_t us3d_gp_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity);
+int64_t us3d_gp_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                                int64_t v_capacity);
 int us3d_gp_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
-                             us3d_layout* out);
+                             int64_t v_capacity, us3d_layout* out);
 
 /* Mapper: replaces the tile mapper of ProjectedSliceForwardCUDA (src/shader/wrapper_shader.cu:107-129 ->
  * OursTileGaussMapperCUDA, src/mapper/core_tile.cu:25-379) for shader weight US_FOCUS. means/scales [G,3],

@@ -116,7 +116,7 @@ def test_config4_captured_step_equals_eager_and_sampled_frames_match_the_oracle(
             # the first step's images render the initial parameters; its packed records hold the activated
             # opacity and intensity the kernels used
             image0 = eager.image.detach().cpu()
-            off = eager.batch.layout_offsets(eager.k_capacity)
+            off = eager.batch.layout_offsets(eager.k_capacity, eager.v_capacity)
             rec = eager.workspace[off.records: off.records + 48 * G].view(torch.float32).view(G, 12).cpu()
             o_pk, c_pk = rec[:, 3].numpy(), rec[:, 4].numpy()
         states.append([t.cpu() for t in eager.state()])

@@ -77,6 +77,7 @@ def test_c_abi_rejects_a_zero_weight_min_before_any_device_work():
     L = _lib.lib()
     p = _lib.Plan()
     p.n_gauss, p.n_frames, p.n_sweeps, p.n_tiles, p.n_pairs, p.n_pixels, p.k_capacity = 4, 1, 1, 1, 4, 256, 64
+    p.v_capacity = 4
     dummy = ctypes.create_string_buffer(64)
     for f in ("frames", "frame_tile_base", "frame_pixel_base", "sweeps", "workspace"):
         setattr(p, f, ctypes.addressof(dummy))

@@ -60,12 +60,17 @@ def test_struct_layouts_match_header(tmp_path):
 def test_workspace_layout_is_aligned_and_monotone():
     lib = _lib.lib()
     out = _lib.Layout()
-    assert lib.us3d_ug_workspace_layout(1000, 3000, 64, 12345, out) == 0
+    assert lib.us3d_ug_workspace_layout(1000, 3000, 64, 12345, 700, out) == 0
     offs = [getattr(out, n) for n, _ in _lib.Layout._fields_]
     assert all(o % 256 == 0 for o in offs)
     assert offs == sorted(offs)
-    assert out.total == lib.us3d_ug_workspace_bytes(1000, 3000, 64, 12345)
-    assert lib.us3d_ur_workspace_bytes(1000, 3000, 64, 12345) > out.total  # 64 B records / partials
+    assert out.total == lib.us3d_ug_workspace_bytes(1000, 3000, 64, 12345, 700)
+    assert lib.us3d_ur_workspace_bytes(1000, 3000, 64, 12345, 700) > out.total  # 64 B records / partials
+    # per-pair state is two bits per (frame, Gaussian) pair and their word scans; records are per visible pair
+    assert out.vis_pair - out.pair_bits < 0.65 * 3000 + 12 * 64 + 6 * 256  # 0.625 B per pair, 12 B per tile
+    assert out.tile_count - out.vis_pair >= 700 * 36
+    big = lib.us3d_ug_workspace_bytes(1000, 3_000_000, 64, 12345, 700) - out.total
+    assert big < 0.65 * (3_000_000 - 3000)
 
 
 def _frame(H=64, W=48, sweep=0):

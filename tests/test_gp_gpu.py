@@ -260,7 +260,7 @@ def test_gp_trainer_gradients_match_torch_autograd_through_the_activations(dev):
     tr.step()
     torch.cuda.synchronize()
     G = raw[0].shape[0]
-    off = tr.batch.layout_offsets(tr.k_capacity)
+    off = tr.batch.layout_offsets(tr.k_capacity, tr.v_capacity)
     rec = tr.workspace[off.records: off.records + 48 * G].view(torch.float32).view(G, 12).clone()
     o_pk, s_pk = rec[:, 3], rec[:, 4:7]
     s_t = torch.exp(raw[1]).clamp_min(1e-5)

@@ -93,10 +93,12 @@ def test_capacity_overflow_is_skipped_and_retried(dev):
     frames, gt, cfg, init = _setup(dev)
     ref = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=True)
     t = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=True)
-    t._allocate(64)  # far below the instance count
-    t.step()  # overflows, is skipped by Adam, re-captured with a larger capacity, repeated
+    t._allocate((64, 16))  # far below the instance and visible-pair counts
+    # the visible pairs overflow first (no instances are counted then), then the instances: each time Adam skips the
+    # step, the capacity grows, the graph is re-captured and the step repeated
+    t.step()
     ref.step()
-    assert t.k_capacity > 64
+    assert t.k_capacity > 64 and t.v_capacity > 16
     for x, y in zip(t.state(), ref.state()):
         assert torch.equal(x, y)
     assert int(t.step_count.item()) == 1
@@ -160,10 +162,10 @@ def test_chunked_step_equals_the_whole_batch_step(dev):
     # an overflow in a chunk: Adam skips the step, the capacity grows, the step is repeated
     small = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=True, chunk_frames=2)
     ref = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=True, chunk_frames=2)
-    small._allocate(64)
+    small._allocate((64, 16))
     small.step()
     ref.step()
-    assert small.k_capacity > 64 and int(small.step_count.item()) == 1
+    assert small.k_capacity > 64 and small.v_capacity > 16 and int(small.step_count.item()) == 1
     for x, y in zip(small.state(), ref.state()):
         assert torch.equal(x, y)
 

@@ -150,12 +150,14 @@ def test_grid_pixels_plane_form_parity(dev, ts):
         m = b.map(*[t.to(dev) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)])
         geom, pix, mr, lists = oracle_frame(scene, fr, sem, ts=ts)
         plane = oug.plane_form(scene.means.numpy(), scene.tril.numpy(), geom)
-        off = b.layout_offsets(m.k_capacity)
-        sect = m.workspace[off.pair_sect: off.pair_sect + 48 * G].view(torch.float32).view(G, 12).cpu().numpy()
+        off = b.layout_offsets(m.k_capacity, m.v_capacity)
         vis = mr.count > 0
-        assert vis.sum() > 100
+        assert vis.sum() > 100 and int(m.stats.n_visible) == int(vis.sum())
+        # one frame: the visible pairs are its Gaussians with a tile, in index order
+        V = int(vis.sum())
+        sect = m.workspace[off.pair_sect: off.pair_sect + 48 * V].view(torch.float32).view(V, 12).cpu().numpy()
         got_plane = np.stack([sect[:, 8], sect[:, 9], sect[:, 10], sect[:, 4], sect[:, 5], sect[:, 6]], 1)
-        np.testing.assert_array_equal(got_plane[vis].view(np.uint32), plane[vis].view(np.uint32))
+        np.testing.assert_array_equal(got_plane.view(np.uint32), plane[vis].view(np.uint32))
         args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom,
                 lists, sem)
         img32, a32, d32, _ = oug.forward(*args, plane=plane)
@@ -478,10 +480,12 @@ def test_fan_frames_match_brute_force_oracle(dev, semantics):
 
 @pytest.mark.parametrize("n_frames", [1, 20, 260])
 def test_emit_split_modes_bitexact(dev, n_frames):
-    """The emit splits a batch by its frame count: parts of the visible list with global cursors below 16 frames,
-    bands of tile rows with LDS cursors below 256, one workgroup per frame beyond. In every mode the sorted keys
-    and tile ranges equal the oracle's for sampled frames, and every instance's emit index is a distinct slot of
-    its pair's range whose row-major position in the pair's tile rectangle is exactly the tile holding it."""
+    """The emit splits a batch by its frame count: parts of the frame's visible-pair list with global cursors below
+    16 frames, bands of tile rows with LDS cursors below 256, one workgroup per frame beyond. In every mode the sorted
+    keys and tile ranges equal the oracle's for sampled frames. The visible pairs are exactly the pairs with a tile,
+    ranked Gaussian-major (vis_pair increasing; counts, offsets and rectangles per rank), the frame-major list holds
+    each frame's ranks in Gaussian order, and the emit indices formed from an instance's rank and tile are a
+    permutation of [0, K) whose row-major position in the pair's tile rectangle is exactly the tile holding it."""
     scene = make_scene(2000, 64.0, seed=13)
     frames = straight_sweep(n_frames, 128, 48, 0.3, 64.0, 40.0, 5.0, seed=14)
     cfg = cfg_for(frames[0])
@@ -489,33 +493,54 @@ def test_emit_split_modes_bitexact(dev, n_frames):
     b = out["batch"]
     m = b.map(*[t.to(dev) for t in (scene.means, scene.tril, scene.opacity, scene.intensity)])
     views = b.sorted_view(m)
+    counts = {}
     for f in sorted({0, n_frames // 2, n_frames - 1}):
         _g, _p, mr, _l = oracle_frame(scene, frames[f], oug.Semantics())
         np.testing.assert_array_equal(views[f]["keys"].cpu().numpy().astype(np.uint64), mr.keys)
         np.testing.assert_array_equal(views[f]["values"].cpu().numpy(), mr.vals)
         np.testing.assert_array_equal(views[f]["tile_start"].cpu().numpy(), mr.tile_start)
         np.testing.assert_array_equal(views[f]["tile_end"].cpu().numpy(), mr.tile_end)
+        counts[f] = mr.count
     lay = b.layout
-    off = b.layout_offsets(m.k_capacity)
+    off = b.layout_offsets(m.k_capacity, m.v_capacity)
     ws = m.workspace.cpu()
-    K = int(ws[:8].view(torch.int64)[0])
+    K = int(m.stats.n_instances)
+    V = int(m.stats.n_visible)
+    i32 = lambda o, n: ws[o: o + 4 * n].view(torch.int32).numpy().astype(np.int64)  # noqa: E731
     key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64).numpy()
-    val = ws[off.seg_val: off.seg_val + 4 * K].view(torch.int32).numpy().astype(np.int64)
-    tile_off = ws[off.tile_off: off.tile_off + 4 * (lay.n_tiles + 1)].view(torch.int32).numpy().astype(np.int64)
-    pair_off = ws[off.pair_off: off.pair_off + 4 * (lay.n_pairs + 1)].view(torch.int32).numpy().astype(np.int64)
-    rect = ws[off.pair_rect: off.pair_rect + 16 * lay.n_pairs].view(torch.int32).numpy().reshape(-1, 4)
-    assert K > 300 * min(n_frames, 3) and np.array_equal(np.sort(val), np.arange(K))
+    val = i32(off.seg_val, K)
+    tile_off = i32(off.tile_off, lay.n_tiles + 1)
+    vis_pair = i32(off.vis_pair, V)
+    pair_count = i32(off.pair_count, V)
+    pair_off = i32(off.pair_off, V + 1)
+    rect = ws[off.pair_rect: off.pair_rect + 16 * V].view(torch.int32).numpy().reshape(-1, 4).astype(np.int64)
+    vis_list = i32(off.vis_list, V)
+    assert K > 300 * min(n_frames, 3) and 0 < V <= K
+    # ranks: dense pairs with a tile, Gaussian-major; the sampled frames' visible Gaussians are the oracle's
+    assert np.all(np.diff(vis_pair) > 0) and vis_pair[-1] < lay.n_pairs
+    vg, vf = vis_pair // n_frames, vis_pair % n_frames  # one sweep: pair = g * n_frames + frame
+    for f, cnt in counts.items():
+        np.testing.assert_array_equal(np.sort(vg[vf == f]), np.flatnonzero(cnt > 0))
+    np.testing.assert_array_equal(pair_count, (rect[:, 2] - rect[:, 0]) * (rect[:, 3] - rect[:, 1]))
+    assert np.all(pair_count > 0) and pair_off[0] == 0
+    np.testing.assert_array_equal(np.diff(pair_off), pair_count)
+    assert pair_off[V] == K
+    # the frame-major list: frame by frame, each frame's ranks in Gaussian order
+    order = np.lexsort((vg, vf))
+    np.testing.assert_array_equal(vis_list, order)
+    # instances: the key's Gaussian is the rank's; emit indices from (rank, tile) are a permutation of [0, K)
     tile = np.repeat(np.arange(lay.n_tiles), np.diff(tile_off))  # global tile of each instance
     frame = np.searchsorted(np.asarray(lay.frame_tile_base), tile, side="right") - 1
     g = key & 0xFFFFFFFF
-    pair = g * n_frames + frame  # one sweep: pairs are numbered Gaussian-major
-    j = val - pair_off[pair]
-    r = rect[pair]
-    w = r[:, 2] - r[:, 0]
-    assert np.all((j >= 0) & (j < w * (r[:, 3] - r[:, 1])))
+    np.testing.assert_array_equal(vg[val], g)
+    np.testing.assert_array_equal(vf[val], frame)
     tiles_x = np.asarray(lay.tiles_x)[frame]
-    local = (r[:, 1] + j // w) * tiles_x + r[:, 0] + j % w
-    np.testing.assert_array_equal(local, tile - np.asarray(lay.frame_tile_base)[frame])
+    lt = tile - np.asarray(lay.frame_tile_base)[frame]
+    tx, ty = lt % tiles_x, lt // tiles_x
+    r = rect[val]
+    assert np.all((tx >= r[:, 0]) & (tx < r[:, 2]) & (ty >= r[:, 1]) & (ty < r[:, 3]))
+    e = pair_off[val] + (ty - r[:, 1]) * (r[:, 2] - r[:, 0]) + (tx - r[:, 0])
+    np.testing.assert_array_equal(np.sort(e), np.arange(K))
 
 
 def test_fan_grid_in_kernel_matches_explicit_positions(dev):

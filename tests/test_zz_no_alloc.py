@@ -28,8 +28,8 @@ def _render_both(dev, scene, frames, cfg, explicit):
     n = b.layout.n_pixels
     outs = [torch.full((n,), -7.0, device=dev) for _ in range(3)] + [torch.full((n, 4), -7.0, device=dev)]
     overflow = torch.full((1,), -1, dtype=torch.int32, device=dev)
-    plan = b.tables.plan(torch.empty(1, dtype=torch.uint8, device=dev), 1)
-    plan.workspace, plan.workspace_bytes, plan.k_capacity = None, 0, 0  # ignored by the static variant
+    plan = b.tables.plan(torch.empty(1, dtype=torch.uint8, device=dev), 1, 1)
+    plan.workspace, plan.workspace_bytes, plan.k_capacity, plan.v_capacity = None, 0, 0, 0  # ignored by the static variant
     L = _lib.lib()
     _lib.check(L.us3d_ug_render_static(plan, *[_lib.ptr(t) for t in p], _lib.ptr(pix), b.settings,
                                        *[_lib.ptr(t) for t in outs], _lib.ptr(overflow), _lib.stream_ptr(dev)),

@@ -179,9 +179,11 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
                   const int32_t* seg_off, int n_segments, us3d_stats* stats, int64_t k_capacity,
                   hipStream_t stream);
 
-// Workspace layout shared by both renderers (record and partial sizes differ).
-us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
-                      int64_t partial_bytes, int64_t mask_bytes, int64_t sect_bytes);
+// Workspace layout shared by the renderers (record, partial, mask and per-pair record sizes differ).
+us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap,
+                      int64_t record_bytes, int64_t partial_bytes, int64_t mask_bytes, int64_t sect_bytes);
+int64_t pair_words(int64_t n_pairs);
+int64_t fm_words(int64_t n_pairs, int64_t n_tiles);
 
 template <typename T>
 inline T* ws_at(const us3d_plan* plan, int64_t off) {
@@ -195,15 +197,57 @@ inline unsigned blocks_for(int64_t n, int threads) {
 int check_plan(const us3d_plan* plan, int64_t ws_needed);
 
 
-// Before the preprocess: clear the per-tile instance counters the preprocess increments and mark every
-// frame-major visible-pair entry empty (-1).
-int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
-
-// After the preprocess has written per-pair counts (and, for pairs with a nonzero count, tile rectangles,
-// depth keys and their entry in the frame-major visible-pair list) and counted every tile's instances: scan pairs (instance count into stats, overflow flag), scan
-// the tile counts into offsets, emit instances into tile buckets, sort every bucket by (depth key, Gaussian).
-// Pairs carry tile rectangles in frame tile coordinates.
+// The map of every renderer (mapper.hip): begin_pairs resets the stats block, the tile counters and the frame-major
+// mask; the renderer's cull pass marks the visible pairs (mark_pair); rank_visible_pairs ranks them, checks
+// v_capacity and lists them (vis_pair, vis_list); the renderer's setup pass writes each visible pair's count, tile
+// rectangle, depth key and record and counts its tiles (add_rect_corners); build_instances scans the counts
+// (instance total into stats, overflow flag), scans the tile counts into offsets, emits the instances into tile
+// buckets (seg_val = the instance's visible pair) and sorts every bucket by (depth key, Gaussian).
+int begin_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+int rank_visible_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
 int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+
+// First frame-major word of frame f of sweep s: each sweep before s takes f_count * ceil(g_count / 32) words.
+__device__ __forceinline__ int64_t fm_word_base(const us3d_sweep* __restrict__ sweeps, int s, int f) {
+    int64_t base = 0;
+    for (int i = 0; i < s; ++i) base += (int64_t)sweeps[i].f_count * ((sweeps[i].g_count + 31) >> 5);
+    return base + (int64_t)(f - sweeps[s].f_begin) * ((sweeps[s].g_count + 31) >> 5);
+}
+
+// Compact index of dense pair p: the number of visible pairs before it (p may equal n_pairs: the total).
+__device__ __forceinline__ int pair_rank(const uint64_t* __restrict__ bits, const int32_t* __restrict__ word_off,
+                                         int64_t p) {
+    const int64_t w = p >> 6;
+    const int b = (int)(p & 63);
+    int r = word_off[w];
+    if (b) r += __popcll(bits[w] & ((1ull << b) - 1ull));
+    return r;
+}
+
+// Records the visibility of dense pair `pair` in a cull pass. Every thread of the launch calls it once, in converged
+// control flow, with visible = false past n_pairs: a wavefront holds 64 consecutive pairs starting at a multiple of
+// 64, so its ballot is their word. A visible pair also sets its frame-major bit (frame f of sweep s, Gaussian gl).
+__device__ __forceinline__ void mark_pair(bool visible, int64_t pair, int64_t n_pairs, const us3d_sweep* sweeps, int s,
+                                          int f, int gl, uint64_t* __restrict__ pair_bits,
+                                          int32_t* __restrict__ word_cnt, uint32_t* __restrict__ fm_bits) {
+    const uint64_t bal = __ballot(visible);
+    if ((threadIdx.x & (kWave - 1)) == 0 && pair < n_pairs) {
+        pair_bits[pair >> 6] = bal;
+        word_cnt[pair >> 6] = (int32_t)__popcll(bal);
+    }
+    if (visible) {
+        const int64_t fm = fm_word_base(sweeps, s, f) * 32 + gl;
+        atomicOr(&fm_bits[fm >> 5], 1u << (fm & 31));
+    }
+}
+
+// Emit index of the instance of visible pair v in tile (tx, ty) of its frame: the pair's first emit index plus the
+// tile's row-major position in the pair's rectangle (scanline ty for a ray pair, whose rectangle is one column wide).
+__device__ __forceinline__ int64_t emit_index(const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+                                              int v, int tx, int ty) {
+    const int4 r = pair_rect[v];
+    return (int64_t)pair_off[v] + (int64_t)(ty - r.y) * (r.z - r.x) + (tx - r.x);
+}
 
 // Wave-cooperative walk over the tile instances of the pairs held by the 64 lanes of a wavefront. Lane l holds
 // a pair with `count` instances covering `rect` = [x0,x1)x[y0,y1) of a frame whose tiles start at tile_base,

@@ -685,7 +685,7 @@ int us3d_stage_timing_read(char* buf, int64_t bytes) {
     return US3D_OK;
 }
 
-int us3d_version(void) { return 1; }
+int us3d_version(void) { return 2; }
 
 int64_t us3d_scan_tmp_bytes(int64_t n) { return us3d::scan_tmp_bytes(n); }
 

@@ -30,8 +30,13 @@
 
 namespace us3d {
 
-static us3d_layout gp_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 32, 0, 32);
+static us3d_layout gp_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 32, 0, 32);
+}
+
+static us3d_layout gp_layout(const us3d_plan* plan) {
+    return plan ? gp_layout(plan->n_gauss, plan->n_pairs, plan->n_tiles, plan->k_capacity, plan->v_capacity)
+                : us3d_layout{};
 }
 
 template <typename T>
@@ -208,50 +213,83 @@ __device__ __forceinline__ void gp_footprint(float4 r0, float4 r1, float4 r2, co
 }
 
 // ---------------------------------------------------------------- mapper
-// Thread per (frame, Gaussian) pair: footprint, tile rectangle (core_tile.cu:138-163), pair record.
-__global__ __launch_bounds__(256) void gp_preprocess_kernel(
-    int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
-    const us3d_sweep* __restrict__ sweeps, int n_sweeps, us3d_gp_settings st, int32_t* __restrict__ pair_count,
-    int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk, float4* __restrict__ pair_foot,
-    int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
+// Tile rectangle of one (frame, Gaussian) pair from its footprint (core_tile.cu:138-163); returns the instance
+// count, 0 when the footprint is invalid, its peak weight is below weight_min or its box misses the frame. The cull
+// and the setup pass call it on the same inputs, so they agree on every pair.
+__device__ __forceinline__ int gp_pair_tiles(const GPFoot& F, const us3d_frame& fr, const us3d_gp_settings& st,
+                                             int4& rect) {
+    if (!(F.ok && F.b >= st.weight_min)) return 0;
+    const float u = F.m0 / fr.delta_lat + fr.col_offset;
+    const float v = (F.m1 - fr.near_mm) / fr.delta_ax;
+    const float ru = fmaxf(1.0f, F.r_lat / fmaxf(1e-8f, fr.delta_lat));
+    const float rv = fmaxf(1.0f, F.r_ax / fmaxf(1e-8f, fr.delta_ax));
+    const float tsf = (float)st.tile_size;
+    const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
+    const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
+    if (!(isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1))) return 0;
+    const int x0 = max(0, gp_clamp_to_int(fx0)), y0 = max(0, gp_clamp_to_int(fy0));
+    const int x1 = min(fr.tiles_x, gp_clamp_to_int(fx1)), y1 = min(fr.tiles_y, gp_clamp_to_int(fy1));
+    if (!(x0 < x1 && y0 < y1)) return 0;
+    rect = make_int4(x0, y0, x1, y1);
+    return (x1 - x0) * (y1 - y0);
+}
+
+// Cull pass: thread per dense (frame, Gaussian) pair; marks the pairs with at least one tile.
+__global__ __launch_bounds__(256) void gp_cull_kernel(int64_t n_pairs, const float4* __restrict__ rec,
+                                                      const us3d_frame* __restrict__ frames,
+                                                      const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                                                      us3d_gp_settings st, uint64_t* __restrict__ pair_bits,
+                                                      int32_t* __restrict__ word_cnt, uint32_t* __restrict__ fm_bits) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
+    int count = 0, s = 0, f = 0, gl = 0;
+    if (pair < n_pairs) {
+        s = sweep_of_pair(sweeps, n_sweeps, pair);
+        const us3d_sweep sw = sweeps[s];
+        const int local = (int)(pair - sw.pair_base);
+        gl = local / sw.f_count;
+        const int g = sw.g_begin + gl;
+        f = sw.f_begin + (local - gl * sw.f_count);
+        const us3d_frame& fr = frames[f];
+        GPFoot F;
+        gp_footprint<false>(rec[3 * (int64_t)g + 0], rec[3 * (int64_t)g + 1], rec[3 * (int64_t)g + 2], fr,
+                            st.k_sigma, st.sigma_ele, st.det_eps, F, nullptr, st.weight_min);
+        int4 rect;
+        count = gp_pair_tiles(F, fr, st, rect);
+    }
+    mark_pair(count > 0, pair, n_pairs, sweeps, s, f, gl, pair_bits, word_cnt, fm_bits);
+}
+
+// Setup pass: thread per visible pair v < v_capacity (entries past the visible count, or every entry when the
+// visible pairs overflowed, get count 0): count, tile rectangle, the footprint record, tiles counted by corners.
+__global__ __launch_bounds__(256) void gp_setup_kernel(
+    int64_t v_cap, const int32_t* __restrict__ vis_pair, const float4* __restrict__ rec,
+    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, int n_sweeps, us3d_gp_settings st,
+    const us3d_stats* __restrict__ stats, int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect,
+    uint32_t* __restrict__ pair_dk, float4* __restrict__ pair_foot, int32_t* __restrict__ tile_count) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= v_cap) return;
+    if (stats->overflow || v >= stats->n_visible) {
+        pair_count[v] = 0;
+        return;
+    }
+    const int64_t pair = vis_pair[v];
     const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
     const int local = (int)(pair - sw.pair_base);
     const int gl = local / sw.f_count;
     const int g = sw.g_begin + gl;
-    const int f = sw.f_begin + (local - gl * sw.f_count);
-    const us3d_frame& fr = frames[f];
+    const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
     GPFoot F;
     gp_footprint<false>(rec[3 * (int64_t)g + 0], rec[3 * (int64_t)g + 1], rec[3 * (int64_t)g + 2], fr, st.k_sigma,
                         st.sigma_ele, st.det_eps, F, nullptr, st.weight_min);
-    int count = 0;
     int4 rect = make_int4(0, 0, 0, 0);
-    if (F.ok && F.b >= st.weight_min) {
-        const float u = F.m0 / fr.delta_lat + fr.col_offset;
-        const float v = (F.m1 - fr.near_mm) / fr.delta_ax;
-        const float ru = fmaxf(1.0f, F.r_lat / fmaxf(1e-8f, fr.delta_lat));
-        const float rv = fmaxf(1.0f, F.r_ax / fmaxf(1e-8f, fr.delta_ax));
-        const float tsf = (float)st.tile_size;
-        const float fx0 = floorf((u - ru) / tsf), fy0 = floorf((v - rv) / tsf);
-        const float fx1 = ceilf((u + ru) / tsf), fy1 = ceilf((v + rv) / tsf);
-        if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
-            const int x0 = max(0, gp_clamp_to_int(fx0)), y0 = max(0, gp_clamp_to_int(fy0));
-            const int x1 = min(fr.tiles_x, gp_clamp_to_int(fx1)), y1 = min(fr.tiles_y, gp_clamp_to_int(fy1));
-            if (x0 < x1 && y0 < y1) {
-                count = (x1 - x0) * (y1 - y0);
-                rect = make_int4(x0, y0, x1, y1);
-            }
-        }
-    }
-    pair_count[pair] = count;
-    if (count > 0) {
-        pair_rect[pair] = rect;
-        pair_dk[pair] = 0u;
-        pair_foot[2 * pair + 0] = make_float4(F.m0, F.m1, F.i00 * kGpExp2Scale, F.i01 * kGpExp2Scale);
-        pair_foot[2 * pair + 1] = make_float4(F.i11 * kGpExp2Scale, F.r_lat, F.r_ax, F.b);
+    const int count = gp_pair_tiles(F, fr, st, rect);
+    pair_count[v] = count;
+    if (count > 0) {  // always, for a listed pair
+        pair_rect[v] = rect;
+        pair_dk[v] = 0u;
+        pair_foot[2 * v + 0] = make_float4(F.m0, F.m1, F.i00 * kGpExp2Scale, F.i01 * kGpExp2Scale);
+        pair_foot[2 * v + 1] = make_float4(F.i11 * kGpExp2Scale, F.r_lat, F.r_ax, F.b);
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
-        vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
     }
 }
 
@@ -269,7 +307,7 @@ struct GpFwd {
 template <int TS>
 __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
-    const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min,
     const us3d_stats* __restrict__ stats, float* __restrict__ image) {
     using S = GpFwd<TS>;
@@ -292,15 +330,13 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
         xa[r] = (float)(iy0 + r * S::kRowStep) * fr.delta_ax + fr.near_mm;
         acc[r] = 0.f;
     }
-    const us3d_sweep& sw = sweeps[fr.sweep];
-    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     for (int base = s0; base < s1; base += NT) {
         const int n = min(NT, s1 - base);
         __syncthreads();
         if (tid < n) {
             const int g = (int)(uint32_t)seg_key[base + tid];
-            const int64_t p = pair_g0 + (int64_t)g * sw.f_count;
+            const int64_t p = seg_val[base + tid];  // the instance's visible pair
             s_f0[tid] = pair_foot[2 * p + 0];
             s_f1[tid] = pair_foot[2 * p + 1];
             s_I[tid] = rec[3 * (int64_t)g + 1].w;
@@ -341,10 +377,10 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
 template <int TS>
 __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
-    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
-    const us3d_sweep* __restrict__ sweeps, const int32_t* __restrict__ frame_tile_base, int n_frames,
-    float weight_min, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
-    float4* __restrict__ partial) {
+    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off,
+    const int4* __restrict__ pair_rect, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min, const float* __restrict__ d_image,
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial) {
     constexpr int NT = TS * TS;
     constexpr int HALF = TS / 2;
     static_assert(TS % 4 == 0, "rows of the staged gradient are read as float4");
@@ -367,16 +403,14 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         if (tid < TS) s_xl[tid] = ((float)(cx0 + tid) - fr.col_offset) * fr.delta_lat;
     }
     __syncthreads();
-    const us3d_sweep& sw = sweeps[fr.sweep];
-    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const int n_items = 2 * (s1 - s0);
     // NT is even, so both items of an instance run in the same iteration, on lanes tid and tid ^ 1
     for (int it = tid; it < n_items; it += NT) {
         const int i = s0 + (it >> 1);
         const int half = it & 1;
         const int g = (int)(uint32_t)seg_key[i];
-        const int64_t p = pair_g0 + (int64_t)g * sw.f_count;
-        const float4 a = pair_foot[2 * p + 0], c = pair_foot[2 * p + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
+        const int pv = seg_val[i];  // the instance's visible pair
+        const float4 a = pair_foot[2 * (int64_t)pv + 0], c = pair_foot[2 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
         // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
         // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
         // da is constant along a row, so per row only sum t, sum t dl and sum t dl^2 are accumulated per pixel;
@@ -432,7 +466,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         if (half == 0) {
             const float I = rec[3 * (int64_t)g + 1].w;
             const float mI = -2.f * kGpLn2 * I;
-            const int64_t ei = seg_val[i];
+            const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
             partial[2 * ei + 0] = make_float4(I * sT / c.w, sT, mI * sP, mI * sQ);
             partial[2 * ei + 1] = make_float4(-0.5f * I * sA, -I * sB, -0.5f * I * sC, 0.f);
         }
@@ -590,12 +624,15 @@ __device__ __forceinline__ float gp_sanitize_grad(float g) {
     return fminf(fmaxf(g, -kGpGradClamp), kGpGradClamp);
 }
 
-// One wavefront per Gaussian: its lanes take its pairs (frames); each sums the pair's instance partials in
-// emit order, applies the footprint's vector-Jacobian product and sanitises the pair's gradients; a fixed
-// butterfly sums the lanes.
+// One wavefront per Gaussian: its lanes take its visible pairs, v in [rank(p0), rank(p0 + f_count)) in frame order,
+// 64 at a time; each sums the pair's instance partials in emit order, applies the footprint's vector-Jacobian product
+// and sanitises the pair's gradients; a fixed butterfly sums the lanes.
 __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
                                                         int n_sweeps, const us3d_frame* __restrict__ frames,
-                                                        us3d_gp_settings st, const int32_t* __restrict__ pair_count,
+                                                        us3d_gp_settings st, const uint64_t* __restrict__ pair_bits,
+                                                        const int32_t* __restrict__ word_off,
+                                                        const int32_t* __restrict__ vis_pair,
+                                                        const int32_t* __restrict__ pair_count,
                                                         const int32_t* __restrict__ pair_off,
                                                         const float4* __restrict__ partial,
                                                         const float4* __restrict__ rec,
@@ -608,17 +645,15 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
     if (g >= n_gauss || stats->overflow) return;
     const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
+    const int v0 = pair_rank(pair_bits, word_off, p0), v1 = pair_rank(pair_bits, word_off, p0 + sw.f_count);
     const float4 r0 = rec[3 * (int64_t)g + 0], r1 = rec[3 * (int64_t)g + 1], r2 = rec[3 * (int64_t)g + 2];
     float acc[11];
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = 0.f;
     float accI = 0.f;
-    // One visible pair per lane: lane j of a full queue holds the j-th visible frame, so the footprint VJP
-    // runs once per 64 visible pairs instead of once per 64 frames (a Gaussian is seen by a few frames).
-    auto process = [&](int fl) {
-        const int64_t p = p0 + fl;
-        const int cnt = pair_count[p];
-        const int e0 = pair_off[p];
+    for (int v = v0 + lane; v < v1; v += kWave) {
+        const int cnt = pair_count[v];
+        const int e0 = pair_off[v];
         float gb = 0.f, gI = 0.f, gm0 = 0.f, gm1 = 0.f, g00 = 0.f, g01 = 0.f, g11 = 0.f;
 #pragma unroll 4
         for (int e = e0; e < e0 + cnt; ++e) {
@@ -627,7 +662,7 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
             g00 += b.x; g01 += b.y; g11 += b.z;
         }
         accI += gp_sanitize_grad(gI);
-        const us3d_frame& fr = frames[sw.f_begin + fl];
+        const us3d_frame& fr = frames[sw.f_begin + (int)(vis_pair[v] - p0)];
         GPFoot F;
         GPInter I;
         gp_footprint<true>(r0, r1, r2, fr, st.k_sigma, st.sigma_ele, st.det_eps, F, &I);
@@ -637,31 +672,7 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
         gp_footprint_vjp(I, fr, gb, gm0, gm1, g00, g01, g11, pacc);
 #pragma unroll
         for (int k = 0; k < 11; ++k) acc[k] += gp_sanitize_grad(pacc[k]);
-    };
-    const uint64_t below = (1ull << lane) - 1ull;
-    int q = -1;  // queued visible frame of this lane (lanes [0, nq))
-    int nq = 0;  // wave-uniform
-    for (int base = 0; base < sw.f_count; base += kWave) {
-        const int fl = base + lane;
-        const bool vis = fl < sw.f_count && pair_count[p0 + fl] > 0;
-        const uint64_t m = __ballot(vis);
-        const int nv = __popcll(m);
-        if (nv == 0) continue;
-        const int r = __popcll(m & below);
-        // a permutation of the lanes: visible ones to queue slots nq + r (mod 64), in frame order
-        const int dst = (nq + (vis ? r : nv + (lane - r))) & (kWave - 1);
-        const int got = __builtin_amdgcn_ds_permute(dst << 2, fl);
-        const int lo = nq + nv;  // queue length after appending; entries past 64 wrap into slots [0, lo - 64)
-        if (lane >= nq && lane < lo) q = got;
-        const int q2 = (lane < lo - kWave) ? got : -1;
-        nq = lo;
-        if (nq >= kWave) {
-            process(q);
-            q = q2;
-            nq -= kWave;
-        }
     }
-    if (lane < nq) process(q);
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
     accI = wave_sum(accI);
@@ -707,31 +718,29 @@ using namespace us3d;
 
 extern "C" {
 
-int64_t us3d_gp_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity) {
-    return gp_layout(n_gauss, n_pairs, n_tiles, k_capacity).total;
+int64_t us3d_gp_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                                int64_t v_capacity) {
+    return gp_layout(n_gauss, n_pairs, n_tiles, k_capacity, v_capacity).total;
 }
 
 int us3d_gp_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
-                             us3d_layout* out) {
+                             int64_t v_capacity, us3d_layout* out) {
     if (!out) return fail(US3D_ERR_ARGUMENT, "layout output is NULL");
-    *out = gp_layout(n_gauss, n_pairs, n_tiles, k_capacity);
+    *out = gp_layout(n_gauss, n_pairs, n_tiles, k_capacity, v_capacity);
     return US3D_OK;
 }
 
 int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, const float* quats,
                 const float* opacity, const float* intensity, const us3d_gp_settings* settings, void* stream_) {
-    const us3d_layout L = gp_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = gp_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = gp_check_settings(settings))) return rc;
     if (plan->n_gauss > 0 && (!means || !scales || !quats || !opacity))
         return fail(US3D_ERR_ARGUMENT, "Gaussian parameter pointer is NULL");
     hipStream_t stream = as_stream(stream_);
-    us3d_stats* stats = gp_at<us3d_stats>(plan, L.stats);
     float4* rec = gp_at<float4>(plan, L.records);
-    hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
-    if ((rc = begin_instances(plan, L, stream))) return rc;
+    if ((rc = begin_pairs(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(gp_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
                            means, scales, quats, opacity, intensity, settings->raw_activations, rec);
@@ -740,21 +749,29 @@ int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, 
     const int64_t np = plan->n_pairs;
     if (np > 0) {
         {
-            StageTimer timer_("gp_preprocess_kernel", stream);
-            hipLaunchKernelGGL(gp_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
-                               plan->frames, plan->sweeps, plan->n_sweeps, *settings,
+            StageTimer timer_("gp_cull_kernel", stream);
+            hipLaunchKernelGGL(gp_cull_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec, plan->frames,
+                               plan->sweeps, plan->n_sweeps, *settings, gp_at<uint64_t>(plan, L.pair_bits),
+                               gp_at<int32_t>(plan, L.word_cnt), gp_at<uint32_t>(plan, L.fm_bits));
+        }
+        if ((rc = check_launch("gp_cull_kernel"))) return rc;
+        if ((rc = rank_visible_pairs(plan, L, stream))) return rc;
+        {
+            StageTimer timer_("gp_setup_kernel", stream);
+            hipLaunchKernelGGL(gp_setup_kernel, dim3(blocks_for(plan->v_capacity, 256)), dim3(256), 0, stream,
+                               plan->v_capacity, gp_at<int32_t>(plan, L.vis_pair), rec, plan->frames, plan->sweeps,
+                               plan->n_sweeps, *settings, gp_at<us3d_stats>(plan, L.stats),
                                gp_at<int32_t>(plan, L.pair_count), gp_at<int4>(plan, L.pair_rect),
                                gp_at<uint32_t>(plan, L.pair_dk), gp_at<float4>(plan, L.pair_sect),
-                               gp_at<int32_t>(plan, L.tile_count), gp_at<int32_t>(plan, L.vis_list));
+                               gp_at<int32_t>(plan, L.tile_count));
         }
-        if ((rc = check_launch("gp_preprocess_kernel"))) return rc;
+        if ((rc = check_launch("gp_setup_kernel"))) return rc;
     }
     return build_instances(plan, L, stream);
 }
 
 int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, void* stream_) {
-    const us3d_layout L = gp_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = gp_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = gp_check_settings(settings))) return rc;
@@ -764,8 +781,9 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
 #define GP_FWD(TS)                                                                                                    \
     hipLaunchKernelGGL((gp_forward_kernel<TS>), dim3(plan->n_tiles), dim3(GpFwd<TS>::kThreads), 0, stream,            \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
-                       gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key), plan->frames,             \
-                       plan->sweeps, plan->frame_tile_base, plan->n_frames, settings->weight_min,                    \
+                       gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
+                       gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames,         \
+                       settings->weight_min,                                                                          \
                        gp_at<us3d_stats>(plan, L.stats), image)
     {
         StageTimer timer_("gp_forward_kernel", stream);
@@ -782,8 +800,7 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
 int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
                      float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
                      void* stream_) {
-    const us3d_layout L = gp_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = gp_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = gp_check_settings(settings))) return rc;
@@ -795,8 +812,9 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
     hipLaunchKernelGGL((gp_backward_kernel<TS>), dim3(plan->n_tiles), dim3(TS * TS), 0, stream,                       \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
                        gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
-                       gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->sweeps, plan->frame_tile_base,           \
-                       plan->n_frames, settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),             \
+                       gp_at<int32_t>(plan, L.seg_val), gp_at<int32_t>(plan, L.pair_off),                            \
+                       gp_at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames,          \
+                       settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),                               \
                        gp_at<float4>(plan, L.partial))
         {
             StageTimer timer_("gp_backward_kernel", stream);
@@ -814,7 +832,9 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
             StageTimer timer_("gp_reduce_kernel", stream);
             hipLaunchKernelGGL(gp_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream,
                                plan->n_gauss, plan->sweeps, plan->n_sweeps, plan->frames, *settings,
-                               gp_at<int32_t>(plan, L.pair_count), gp_at<int32_t>(plan, L.pair_off),
+                               gp_at<uint64_t>(plan, L.pair_bits), gp_at<int32_t>(plan, L.word_off),
+                               gp_at<int32_t>(plan, L.vis_pair), gp_at<int32_t>(plan, L.pair_count),
+                               gp_at<int32_t>(plan, L.pair_off),
                                gp_at<float4>(plan, L.partial), gp_at<float4>(plan, L.records),
                                gp_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_opacity, d_intensity);
         }

@@ -4,10 +4,11 @@
 // Data layout in HBM (one batch = frames of one or more sweeps):
 //   records   f32x12 per Gaussian: (mu.xyz, opacity | intensity, l11, l12, l13 | l22, l23, l33, r_max),
 //             l_ii = raw_ii^2 + beta, packed once per call so every gather is three aligned 16 B loads.
-//   pairs     (frame, Gaussian), numbered Gaussian-major per sweep: count, tile rect, depth key.
+//   pairs     (frame, Gaussian), numbered Gaussian-major per sweep; the visible ones (a few percent) ranked
+//             into a compact index v with count, tile rect, depth key and section record (mapper.hip).
 //   instances (tile, Gaussian) in emit order e (pair order, then tile rows, then columns) -- the
 //             reference's emit order for one frame -- bucketed by global tile, each bucket sorted by
-//             the 64-bit key (depth key << 32 | Gaussian), value = e.
+//             the 64-bit key (depth key << 32 | Gaussian), value = v (e follows from v and the tile).
 //   partial   f32x12 per instance, written at e by the adjoint, so each Gaussian's partials across all
 //             frames are contiguous and reduce in a fixed order.
 #include <algorithm>
@@ -19,39 +20,13 @@ namespace us3d {
 
 // ---------------------------------------------------------------- workspace layout
 
-us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t record_bytes,
-                      int64_t partial_bytes, int64_t mask_bytes, int64_t sect_bytes) {
-    us3d_layout L{};
-    int64_t off = 0;
-    auto take = [&](int64_t bytes) {
-        const int64_t o = off;
-        off += align_up(std::max<int64_t>(bytes, 0), 256);
-        return o;
-    };
-    L.stats = take(sizeof(us3d_stats));
-    L.pair_count = take(n_pairs * 4);
-    L.pair_off = take((n_pairs + 1) * 4);
-    L.pair_rect = take(n_pairs * 16);
-    L.pair_dk = take(n_pairs * 4);
-    L.tile_count = take(n_tiles * 4);
-    L.tile_off = take((n_tiles + 1) * 4);
-    L.tile_cursor = take(n_tiles * 4);
-    L.seg_key = take(k_cap * 8);
-    L.seg_val = take(k_cap * 4);
-    L.seg_key_tmp = take(k_cap * 8);
-    L.seg_val_tmp = take(k_cap * 4);
-    L.records = take(n_gauss * record_bytes);
-    L.partial = take(k_cap * partial_bytes);
-    L.scan_tmp = take(std::max(scan_tmp_bytes(n_pairs + 1), scan_tmp_bytes(n_tiles + 1)));
-    L.vis_list = take(n_pairs * 4);
-    L.inst_mask = take(k_cap * mask_bytes);
-    L.pair_sect = take(n_pairs * sect_bytes);
-    L.total = off;
-    return L;
+static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 48, 2, 48);
 }
 
-static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 48, 48, 2, 48);
+static us3d_layout ug_layout(const us3d_plan* plan) {
+    return plan ? ug_layout(plan->n_gauss, plan->n_pairs, plan->n_tiles, plan->k_capacity, plan->v_capacity)
+                : us3d_layout{};
 }
 
 // ---------------------------------------------------------------- mapper kernels
@@ -232,193 +207,114 @@ __device__ __forceinline__ void pair_section(float l11, float l12, float l13, fl
     s0 = make_float4((float)(-w0), (float)(-w1), (float)kk, (float)(M01 / M11));
 }
 
-__global__ __launch_bounds__(256) void ug_preprocess_kernel(
-    int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
-    const us3d_sweep* __restrict__ sweeps, int n_sweeps, float sqrt_chi2, int ts, int sort_by_depth,
-    int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
-    float4* __restrict__ pair_sect, int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
-    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
-    int count = 0;
-    int4 rect = make_int4(0, 0, 0, 0);
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
-    const int gl = local / sw.f_count;
-    const int g = sw.g_begin + gl;
-    const int f = sw.f_begin + (local - gl * sw.f_count);
-    const us3d_frame& fr = frames[f];
-    const float4 r0 = rec[3 * (int64_t)g + 0];
-    const float4 r1 = rec[3 * (int64_t)g + 1];
-    const float4 r2 = rec[3 * (int64_t)g + 2];
+// Tiles of one (frame, Gaussian) pair (core.cu:9-116 restated): a pair farther than r_max from the plane is culled
+// before its radii are computed (most pairs of a freehand sweep); otherwise the chi2 marginal radii, the elevation
+// cull, the tile rectangle (linear grid, or the polar extent of a fan frame) and the depth key. Returns the instance
+// count (0: culled); the cull and the setup pass call it on the same inputs, so they agree on every pair.
+struct UGPairTiles {
+    int4 rect;
+    uint32_t dk;
+    float lat, ax, r_lat, r_ax;
+};
+
+__device__ __forceinline__ int ug_pair_tiles(const float4& r0, const float4& r1, const float4& r2, const us3d_frame& fr,
+                                             float sqrt_chi2, int ts, int sort_by_depth, UGPairTiles& out) {
     const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
     const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
     const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
     const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
     const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-    uint32_t dk = 0;
-    const bool far = fabsf(ele) > r2.w;  // beyond every marginal radius (see ug_pack_kernel)
-    if (!far) {
-        const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
-        const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-        const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
-        if (fabsf(ele) <= r_ele) {
-            const float tsf = (float)ts;
-            float fx0, fy0, fx1, fy1, v;
-            if (fr.geometry == 1) {
-                fan_tile_bounds(fr, lat, ax, r_lat, r_ax, tsf, fx0, fy0, fx1, fy1, v);
-            } else {
-                const float u = lat / fr.delta_lat + fr.col_offset;
-                v = (ax - fr.near_mm) / fr.delta_ax;
-                const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
-                const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
-                fx0 = floorf((u - ru) / tsf);
-                fy0 = floorf((v - rv) / tsf);
-                fx1 = ceilf((u + ru) / tsf);
-                fy1 = ceilf((v + rv) / tsf);
-            }
-            if (isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1)) {
-                const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
-                const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
-                if (x0 < x1 && y0 < y1) {
-                    count = (x1 - x0) * (y1 - y0);
-                    rect = make_int4(x0, y0, x1, y1);
-                    dk = sort_by_depth ? depth_key(v) : 0u;
-                    float4 s0, s1, s2;
-                    pair_section(l11, l12, l13, l22, l23, l33, r0.x, r0.y, r0.z, lat, ax, r_lat, r_ax, fr, s0, s1,
-                                 s2);
-                    pair_sect[3 * pair + 0] = s0;
-                    pair_sect[3 * pair + 1] = s1;
-                    pair_sect[3 * pair + 2] = s2;
-                }
-            }
-        }
+    if (fabsf(ele) > r2.w) return 0;  // beyond every marginal radius (see ug_pack_kernel); NaN takes the exact path
+    const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
+    const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
+    const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+    if (!(fabsf(ele) <= r_ele)) return 0;
+    const float tsf = (float)ts;
+    float fx0, fy0, fx1, fy1, v;
+    if (fr.geometry == 1) {
+        fan_tile_bounds(fr, lat, ax, r_lat, r_ax, tsf, fx0, fy0, fx1, fy1, v);
+    } else {
+        const float u = lat / fr.delta_lat + fr.col_offset;
+        v = (ax - fr.near_mm) / fr.delta_ax;
+        const float ru = fmaxf(1.0f, r_lat / fr.delta_lat);
+        const float rv = fmaxf(1.0f, r_ax / fr.delta_ax);
+        fx0 = floorf((u - ru) / tsf);
+        fy0 = floorf((v - rv) / tsf);
+        fx1 = ceilf((u + ru) / tsf);
+        fy1 = ceilf((v + rv) / tsf);
     }
-    pair_count[pair] = count;
-    if (count > 0) {  // rect and key are read only for pairs with instances
-        pair_rect[pair] = rect;
-        pair_dk[pair] = dk;
-        add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
-        // frame-major copy of the visible pairs: entry gl of the frame's g_count-sized region
-        vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
+    if (!(isfinite(fx0) && isfinite(fy0) && isfinite(fx1) && isfinite(fy1))) return 0;
+    const int x0 = max(0, clamp_to_int(fx0)), y0 = max(0, clamp_to_int(fy0));
+    const int x1 = min(fr.tiles_x, clamp_to_int(fx1)), y1 = min(fr.tiles_y, clamp_to_int(fy1));
+    if (!(x0 < x1 && y0 < y1)) return 0;
+    out.rect = make_int4(x0, y0, x1, y1);
+    out.dk = sort_by_depth ? depth_key(v) : 0u;
+    out.lat = lat;
+    out.ax = ax;
+    out.r_lat = r_lat;
+    out.r_ax = r_ax;
+    return (x1 - x0) * (y1 - y0);
+}
+
+// Cull pass: thread per dense (frame, Gaussian) pair, numbered Gaussian-major inside a sweep; marks the visible ones.
+__global__ __launch_bounds__(256) void ug_cull_kernel(int64_t n_pairs, const float4* __restrict__ rec,
+                                                      const us3d_frame* __restrict__ frames,
+                                                      const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                                                      float sqrt_chi2, int ts, uint64_t* __restrict__ pair_bits,
+                                                      int32_t* __restrict__ word_cnt, uint32_t* __restrict__ fm_bits) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int count = 0, s = 0, f = 0, gl = 0;
+    if (pair < n_pairs) {
+        s = sweep_of_pair(sweeps, n_sweeps, pair);
+        const us3d_sweep sw = sweeps[s];
+        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+        gl = local / sw.f_count;
+        const int g = sw.g_begin + gl;
+        f = sw.f_begin + (local - gl * sw.f_count);
+        UGPairTiles pt;
+        count = ug_pair_tiles(rec[3 * (int64_t)g + 0], rec[3 * (int64_t)g + 1], rec[3 * (int64_t)g + 2], frames[f],
+                              sqrt_chi2, ts, 0, pt);
     }
+    mark_pair(count > 0, pair, n_pairs, sweeps, s, f, gl, pair_bits, word_cnt, fm_bits);
 }
 
-__global__ void init_stats_kernel(us3d_stats* stats) {
-    stats->n_instances = 0;
-    stats->max_segment = 0;
-    stats->overflow = 0;
-    stats->n_sorted_big = 0;
-    stats->reserved_i32 = 0;
-    for (int i = 0; i < 4; ++i) stats->reserved[i] = 0;  // overflow_any is the caller's: left as it is
-}
-
-__global__ void capacity_kernel(us3d_stats* stats, int64_t k_cap) {
-    const int over = stats->n_instances > k_cap ? 1 : 0;
-    stats->overflow = over;
-    if (over) stats->overflow_any = 1;
-}
-
-// off[n] = total (or 0 when the batch overflowed), closing an (n+1)-entry offset array.
-__global__ void close_offsets_kernel(int32_t* off, int64_t n, const int64_t* total, const int32_t* overflow) {
-    off[n] = (overflow && *overflow) ? 0 : (int32_t)*total;
-}
-
-// Write (depth key << 32 | Gaussian, emit index) of every instance into its tile's bucket. Workgroup w serves
-// frame w / (bands * parts), band b and part p of it (w % (bands * parts) = b * parts + p): the tile rows
-// [b * rows, (b + 1) * rows), rows = ceil(tiles_y / bands), and the chunks p, p + parts, ... of the frame's
-// region of vis_list (one entry per Gaussian of its sweep, -1 where the pair is culled). bands and parts exceed
-// 1 only when the batch has few frames, so the launch still fills the chip: bands split the frame's buckets
-// (every band reads the whole region), parts split the region (their buckets are shared). A workgroup reads
-// kEmitPerThread entries per thread at a time, gathers the visible pairs whose rectangle meets its rows into
-// an LDS queue (ballot compaction) and expands them, clipped to its rows, kEmitThreads at a time with the
-// wave-cooperative instance walk; the emit index stays that of the full rectangle. With one part the band's
-// bucket cursors live in LDS, seeded with the bucket offsets, so a slot costs one LDS atomic instead of a
-// returning atomic in L2; with several parts, or more than kEmitLdsTiles tiles in the band, slots come from
-// the global cursors. Slot order inside a bucket is arbitrary; the bucket sort by the unique keys makes the
-// final order exact.
-constexpr int kEmitThreads = 1024;
-constexpr int kEmitPerThread = 8;
-constexpr int kEmitChunk = kEmitThreads * kEmitPerThread;
-constexpr int kEmitQueue = kEmitChunk + kEmitThreads;
-constexpr int kEmitLdsTiles = 8192;
-constexpr int kEmitTargetGroups = 256;
-
-__global__ __launch_bounds__(kEmitThreads) void ug_emit_kernel(
-    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
-    const int32_t* __restrict__ vis_list, const int32_t* __restrict__ pair_count,
-    const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
-    const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor, uint64_t* __restrict__ seg_key,
-    int32_t* __restrict__ seg_val, int bands, int parts, const us3d_stats* stats) {
-    __shared__ int32_t s_cur[kEmitLdsTiles];  // next free slot of each bucket of the band
-    __shared__ int32_t s_q[kEmitQueue];       // visible pairs not yet expanded
-    __shared__ int s_n;
-    if (stats->overflow) return;
-    const int f = blockIdx.x / (bands * parts), band = (blockIdx.x % (bands * parts)) / parts;
-    const int part = blockIdx.x % parts;
-    const us3d_frame& fr = frames[f];
-    const int rows = (fr.tiles_y + bands - 1) / bands;
-    const int y_lo = band * rows, y_hi = min(fr.tiles_y, y_lo + rows);
-    if (y_lo >= y_hi) return;  // workgroup-uniform: a band past the frame's last tile row
-    const us3d_sweep sw = sweeps[fr.sweep];
-    const int32_t* list = vis_list + sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count;
-    const int band_base = fr.tile_base + y_lo * fr.tiles_x;  // first tile of the band
-    const int n_tiles = (y_hi - y_lo) * fr.tiles_x;
-    const bool local = parts == 1 && n_tiles <= kEmitLdsTiles;  // workgroup-uniform
-    const int tid = threadIdx.x, lane = tid & (kWave - 1);
-    if (local)
-        for (int i = tid; i < n_tiles; i += kEmitThreads) s_cur[i] = tile_off[band_base + i];
-    if (tid == 0) s_n = 0;
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int c0 = part * kEmitChunk; c0 < sw.g_count; c0 += parts * kEmitChunk) {
-        int pr[kEmitPerThread];
-#pragma unroll
-        for (int k = 0; k < kEmitPerThread; ++k) {
-            const int idx = c0 + k * kEmitThreads + tid;
-            pr[k] = idx < sw.g_count ? list[idx] : -1;
-            if (bands > 1 && pr[k] >= 0) {  // keep only pairs whose rectangle meets the band's rows
-                const int4 r = pair_rect[pr[k]];
-                if (r.w <= y_lo || r.y >= y_hi) pr[k] = -1;
-            }
-        }
-#pragma unroll
-        for (int k = 0; k < kEmitPerThread; ++k) {
-            const uint64_t bal = __ballot(pr[k] >= 0);
-            int wbase = 0;
-            if (lane == 0 && bal) wbase = atomicAdd(&s_n, (int)__popcll(bal));
-            wbase = __shfl(wbase, 0, kWave);
-            if (pr[k] >= 0) s_q[wbase + (int)__popcll(bal & below)] = pr[k];
-        }
-        __syncthreads();
-        int n = s_n;  // workgroup-uniform
-        const bool last = c0 + parts * kEmitChunk >= sw.g_count;
-        while (n >= kEmitThreads || (last && n > 0)) {
-            const int take = min(n, kEmitThreads);
-            n -= take;  // expand the queue's last `take` entries, so the rest stays in place
-            const int pair = tid < take ? s_q[n + tid] : -1;
-            int count = 0;
-            int4 rect = make_int4(0, 0, 0, 0);
-            uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, first emit index
-            if (pair >= 0) {
-                const int4 full = pair_rect[pair];
-                rect = make_int4(full.x, max(full.y, y_lo), full.z, min(full.w, y_hi));  // clipped to the band
-                count = (rect.z - rect.x) * (rect.w - rect.y);
-                pay[0] = (uint32_t)(sw.g_begin + (int)(pair - sw.pair_base) / sw.f_count);
-                pay[1] = pair_dk[pair];
-                // instance (x, y) is number (y - y0) * width + (x - x0) of the full rectangle
-                pay[2] = (uint32_t)(pair_off[pair] + (rect.y - full.y) * (full.z - full.x));
-            }
-            for_each_wave_instance<3>(count, rect, fr.tile_base, fr.tiles_x, pay,
-                                      [&](const uint32_t* o, int jj, int tile) {
-                                          const int slot = local ? atomicAdd(&s_cur[tile - band_base], 1)
-                                                                 : tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
-                                          seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
-                                          seg_val[slot] = (int32_t)o[2] + jj;
-                                      });
-        }
-        __syncthreads();  // every read of the queue is done
-        if (tid == 0) s_n = n;
-        __syncthreads();
+// Setup pass: thread per visible pair v < v_capacity (entries past the visible count, or every entry when the visible
+// pairs overflowed, get count 0): count, tile rectangle, depth key, the pair's part of the sub-block test and the
+// plane form of q (pair_section), and its tiles counted by rectangle corners.
+__global__ __launch_bounds__(256) void ug_setup_kernel(
+    int64_t v_cap, const int32_t* __restrict__ vis_pair, const float4* __restrict__ rec,
+    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, int n_sweeps, float sqrt_chi2,
+    int ts, int sort_by_depth, const us3d_stats* __restrict__ stats, int32_t* __restrict__ pair_count,
+    int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk, float4* __restrict__ pair_sect,
+    int32_t* __restrict__ tile_count) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= v_cap) return;
+    if (stats->overflow || v >= stats->n_visible) {
+        pair_count[v] = 0;
+        return;
+    }
+    const int64_t pair = vis_pair[v];
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int local = (int)(pair - sw.pair_base);
+    const int gl = local / sw.f_count;
+    const int g = sw.g_begin + gl;
+    const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
+    const float4 r0 = rec[3 * (int64_t)g + 0];
+    const float4 r1 = rec[3 * (int64_t)g + 1];
+    const float4 r2 = rec[3 * (int64_t)g + 2];
+    UGPairTiles pt;
+    const int count = ug_pair_tiles(r0, r1, r2, fr, sqrt_chi2, ts, sort_by_depth, pt);
+    pair_count[v] = count;
+    if (count > 0) {  // always, for a listed pair: the cull made the same decision
+        pair_rect[v] = pt.rect;
+        pair_dk[v] = pt.dk;
+        float4 s0, s1, s2;
+        pair_section(r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r0.x, r0.y, r0.z, pt.lat, pt.ax, pt.r_lat, pt.r_ax, fr, s0,
+                     s1, s2);
+        pair_sect[3 * v + 0] = s0;
+        pair_sect[3 * v + 1] = s1;
+        pair_sect[3 * v + 2] = s2;
+        add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, pt.rect);
     }
 }
 
@@ -580,8 +476,9 @@ struct FwdLds {
 // tile's list is staged kBatch Gaussians at a time.
 template <int TS, bool kDepth, bool kPlaneBody, int kBatch>
 __device__ __forceinline__ void ug_forward_tile(
-    FwdLds<TS, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key, const us3d_frame& fr,
-    const us3d_sweep& sw, int f, int tx, int ty, int s0, int s1, const float* __restrict__ pixels,
+    FwdLds<TS, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
+    const int32_t* __restrict__ seg_val, const us3d_frame& fr, int tx, int ty, int s0, int s1,
+    const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, float* __restrict__ out_image, float* __restrict__ out_alpha,
     float* __restrict__ out_depth, float4* __restrict__ out_aux, uint16_t* __restrict__ inst_mask) {
@@ -591,8 +488,6 @@ __device__ __forceinline__ void ug_forward_tile(
     constexpr int NC = kPlaneBody ? 2 : 3;  // per-pixel coordinates held: (lat, ax) or the world position
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
-    // pair index of Gaussian g in this frame: pair_g0 + g * f_count
-    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
     const float chi2w = sqrt_chi2 * sqrt_chi2 * 1.002f;
 
     float pc[NC][PPT];
@@ -636,13 +531,13 @@ __device__ __forceinline__ void ug_forward_tile(
             const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
-            const float4 ps0 = pair_sect[3 * pair + 0], ps1 = pair_sect[3 * pair + 1];
+            const int64_t pv = seg_val[base + j];  // the instance's visible pair
+            const float4 ps0 = pair_sect[3 * pv + 0], ps1 = pair_sect[3 * pv + 1];
             const uint32_t m = sub_mask<S::kSub>(L.sb, ps0, ps1, lat_g, ax_g, e, chi2w);
             L.mask[j] = m;
             inst_mask[base + j] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             if constexpr (kPlaneBody) {
-                const float4 ps2 = pair_sect[3 * pair + 2];
+                const float4 ps2 = pair_sect[3 * pv + 2];
                 L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
                 L.r1[j] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
                 L.r2[j] = make_float4(e, 0.f, 0.f, 0.f);
@@ -721,7 +616,7 @@ template <int TS, bool kDepth, bool kPlane, int kBatch>
 __global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
 void ug_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
+    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
@@ -735,13 +630,12 @@ void ug_forward_kernel(
     const int lt = t - fr.tile_base;
     const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    const us3d_sweep& sw = sweeps[fr.sweep];
     if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+        ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect, chi2,
                                           sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
                                           out_alpha, out_depth, out_aux, inst_mask);
     else
-        ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect, chi2,
+        ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect, chi2,
                                            sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
                                            out_alpha, out_depth, out_aux, inst_mask);
 }
@@ -860,6 +754,7 @@ struct BwdLds {
     float4 c[NSUB * kSubStride];  // general form: v, dX, dD; plane form: dX, dD
     float4 g0[kBatch], g1[kBatch], g2[kBatch];                     // the batch's Gaussian records
     float4 g3[kPlane ? kBatch : 1], g4[kPlane ? kBatch : 1];       // plane form: (cx, cy, K, o), (p, qq, s, c)
+    int32_t ei[kBatch];                                            // emit index of each instance of the batch
     uint16_t item[MAXI];
     int wtot[TileShape<TS>::kWaves];
     GradSums pend[MAXC];   // head part of a Gaussian that continues into the next chunk
@@ -872,8 +767,9 @@ struct BwdLds {
 template <int TS, bool kFull, bool kPlaneBody, bool kPlane, int kBatch>
 __device__ __forceinline__ void ug_backward_tile(
     BwdLds<TS, kPlane, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame& fr, const us3d_sweep& sw, int f, int tx, int ty, int s0,
-    int s1, const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
+    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+    const us3d_frame& fr, int tx, int ty, int s0, int s1, const float* __restrict__ pixels,
+    const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
     const float* __restrict__ d_alpha, const float* __restrict__ d_depth, float4* __restrict__ partial,
     const uint16_t* __restrict__ inst_mask) {
@@ -890,7 +786,6 @@ __device__ __forceinline__ void ug_backward_tile(
     static_assert(FwdShape<TS>::kSub == NSUB, "the forward's instance masks number the same sub-blocks");
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
-    const int64_t pair_g0 = sw.pair_base - (int64_t)sw.g_begin * sw.f_count + (f - sw.f_begin);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -943,18 +838,19 @@ __device__ __forceinline__ void ug_backward_tile(
                 const float4 q0 = rec[3 * (int64_t)g + 0];
                 const float4 q1 = rec[3 * (int64_t)g + 1];
                 const float4 q2 = rec[3 * (int64_t)g + 2];
+                const int pv = seg_val[base + j];  // the instance's visible pair
+                const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
                 m[i] = inst_mask[base + j];  // the sub-block mask the forward left for this instance
                 L.g0[j] = q0;
                 L.g1[j] = q1;
                 L.g2[j] = q2;
+                L.ei[j] = (int32_t)ei;
                 if constexpr (kPlaneBody) {
-                    const int64_t pair = pair_g0 + (int64_t)g * sw.f_count;
-                    const float4 ps1 = pair_sect[3 * pair + 1], ps2 = pair_sect[3 * pair + 2];
+                    const float4 ps1 = pair_sect[3 * (int64_t)pv + 1], ps2 = pair_sect[3 * (int64_t)pv + 2];
                     L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
                     L.g4[j] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
                 }
                 if (m[i] == 0) {  // touches no pixel of the tile: its partial is zero
-                    const int64_t ei = seg_val[base + j];
                     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
                     partial[3 * ei + 0] = z4;
                     partial[3 * ei + 1] = z4;
@@ -1152,7 +1048,7 @@ __device__ __forceinline__ void ug_backward_tile(
                 } else {
                     if constexpr (kPlaneBody)
                         a = plane_sums_to_grad(a, L.g0[slot], L.g1[slot], L.g2[slot], fr, TS, tx, ty);
-                    write_partial(partial, seg_val[base + slot], a, L.g1[slot], L.g2[slot], fr);
+                    write_partial(partial, L.ei[slot], a, L.g1[slot], L.g2[slot], fr);
                 }
             }
         }
@@ -1166,7 +1062,7 @@ __device__ __forceinline__ void ug_backward_tile(
 #pragma unroll
             for (int k = 0; k < NS; ++k) a.v[k] += b.v[k];
             if constexpr (kPlaneBody) a = plane_sums_to_grad(a, L.g0[slot], L.g1[slot], L.g2[slot], fr, TS, tx, ty);
-            write_partial(partial, seg_val[base + slot], a, L.g1[slot], L.g2[slot], fr);
+            write_partial(partial, L.ei[slot], a, L.g1[slot], L.g2[slot], fr);
         }
     }
 }
@@ -1177,9 +1073,10 @@ __device__ __forceinline__ void ug_backward_tile(
 template <int TS, bool kFull, bool kPlane, int kBatch>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps,
-    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
-    const float4* __restrict__ pair_sect, float chi2, float alpha_min, float bg_alpha, const float4* __restrict__ aux,
+    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
+    float bg_alpha, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
     __shared__ BwdLds<TS, kPlane, kBatch> lds;
@@ -1191,13 +1088,12 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
     const int s0 = tile_off[t], s1 = tile_off[t + 1];
     if (s0 == s1) return;
-    const us3d_sweep& sw = sweeps[fr.sweep];
     if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+        ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx, ty, s0, s1, pixels, pair_sect,
                                           chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
                                           inst_mask);
     else
-        ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, sw, f, tx, ty, s0, s1, pixels, pair_sect,
+        ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx, ty, s0, s1, pixels, pair_sect,
                                            chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
                                            inst_mask);
 }
@@ -1206,7 +1102,9 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
 // all frames of its sweep), then a fixed butterfly reduction -- bitwise reproducible. Applies the raw
 // parameter chain d raw_ii = dl_ii * 2 raw_ii (backward.cu:124-129).
 __global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
-                                                        int n_sweeps, const int32_t* __restrict__ pair_off,
+                                                        int n_sweeps, const uint64_t* __restrict__ pair_bits,
+                                                        const int32_t* __restrict__ word_off,
+                                                        const int32_t* __restrict__ pair_off,
                                                         const float4* __restrict__ partial,
                                                         const float* __restrict__ tril,
                                                         const float4* __restrict__ rec, int logits,
@@ -1219,7 +1117,9 @@ __global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_
     if (g >= n_gauss || stats->overflow) return;
     const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
-    const int e0 = pair_off[p0], e1 = pair_off[p0 + sw.f_count];
+    // the Gaussian's visible pairs are v in [rank(p0), rank(p0 + f_count)); their instances are consecutive
+    const int e0 = pair_off[pair_rank(pair_bits, word_off, p0)];
+    const int e1 = pair_off[pair_rank(pair_bits, word_off, p0 + sw.f_count)];
     float acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0.f;
@@ -1296,21 +1196,6 @@ __global__ void ug_pixel_lists_kernel(int64_t n_pixels, const us3d_frame* __rest
     if (!pass) counts[p] = c;
 }
 
-int check_plan(const us3d_plan* plan, int64_t ws_needed) {
-    if (!plan) return fail(US3D_ERR_ARGUMENT, "plan is NULL");
-    if (plan->n_gauss < 0 || plan->n_frames <= 0 || plan->n_sweeps <= 0 || plan->n_tiles <= 0 || plan->n_pairs < 0)
-        return fail(US3D_ERR_ARGUMENT, "plan sizes out of range");
-    if (plan->n_pairs >= (int64_t(1) << 31) || plan->n_pixels >= (int64_t(1) << 31))
-        return fail(US3D_ERR_ARGUMENT, "n_pairs and n_pixels must be < 2^31 (split the batch)");
-    if (plan->k_capacity < 1 || plan->k_capacity >= (int64_t(1) << 31))
-        return fail(US3D_ERR_ARGUMENT, "k_capacity must be in [1, 2^31)");
-    if (!plan->workspace || plan->workspace_bytes < ws_needed)
-        return fail(US3D_ERR_ARGUMENT, "workspace missing or smaller than us3d_ug_workspace_bytes()");
-    if (!plan->frames || !plan->frame_tile_base || !plan->sweeps)
-        return fail(US3D_ERR_ARGUMENT, "plan tables are NULL");
-    return US3D_OK;
-}
-
 static int check_settings(const us3d_ug_settings* s) {
     if (!s) return fail(US3D_ERR_ARGUMENT, "settings is NULL");
     if (!(s->tile_size == 8 || s->tile_size == 16 || s->tile_size == 32))
@@ -1329,67 +1214,6 @@ static int lds_batch(int chosen, int threads) { return chosen ? chosen : threads
 template <typename T>
 static T* at(const us3d_plan* plan, int64_t off) {
     return ws_at<T>(plan, off);
-}
-
-int begin_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream,
-                       at<int32_t>(plan, L.tile_count), (int64_t)plan->n_tiles, 0);
-    if (plan->n_pairs > 0)
-        hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
-                           at<int32_t>(plan, L.vis_list), plan->n_pairs, -1);
-    return check_launch("fill_i32_kernel");
-}
-
-int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
-    int rc;
-    us3d_stats* stats = at<us3d_stats>(plan, L.stats);
-    int32_t* pair_count = at<int32_t>(plan, L.pair_count);
-    int32_t* pair_off = at<int32_t>(plan, L.pair_off);
-    int4* pair_rect = at<int4>(plan, L.pair_rect);
-    uint32_t* pair_dk = at<uint32_t>(plan, L.pair_dk);
-    int32_t* tile_count = at<int32_t>(plan, L.tile_count);
-    int32_t* tile_off = at<int32_t>(plan, L.tile_off);
-    const int64_t np = plan->n_pairs;
-    if (np > 0) {
-        hipLaunchKernelGGL(prefix_tile_counts_kernel, dim3(plan->n_frames), dim3(256), 0, stream, plan->frames,
-                           tile_count);
-        if ((rc = check_launch("prefix_tile_counts_kernel"))) return rc;
-        if ((rc = scan_i32(pair_count, pair_off, np, &stats->n_instances, at<void>(plan, L.scan_tmp), stream))) return rc;
-    } else {
-        hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, stream, pair_off, (int64_t)1, 0);
-    }
-    hipLaunchKernelGGL(capacity_kernel, dim3(1), dim3(1), 0, stream, stats, plan->k_capacity);
-    if ((rc = check_launch("capacity_kernel"))) return rc;
-    // Close pair_off with the total (offsets have n_pairs+1 entries).
-    hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, pair_off, np, &stats->n_instances,
-                       (const int32_t*)nullptr);
-    int64_t* tile_total = &stats->reserved[0];
-    if ((rc = scan_i32(tile_count, tile_off, plan->n_tiles, tile_total, at<void>(plan, L.scan_tmp), stream))) return rc;
-    hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, tile_off, (int64_t)plan->n_tiles,
-                       tile_total, (const int32_t*)&stats->overflow);
-    int32_t* tile_cursor = at<int32_t>(plan, L.tile_cursor);
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(plan->n_tiles, 256)), dim3(256), 0, stream, tile_cursor,
-                       (int64_t)plan->n_tiles, 0);
-    if ((rc = check_launch("fill_i32_kernel"))) return rc;
-    if (np > 0) {
-        {
-            StageTimer timer_("ug_emit_kernel", stream);
-            // enough workgroups for the chip when the batch has few frames: bands (LDS cursors) down to 16
-            // frames, below that parts of the visible list (global cursors), since every band rereads the list
-            const int split = std::min(32, (kEmitTargetGroups + plan->n_frames - 1) / plan->n_frames);
-            const int bands = plan->n_frames >= 16 ? std::max(1, split) : 1;
-            const int parts = plan->n_frames >= 16 ? 1 : std::max(1, split);
-            hipLaunchKernelGGL(ug_emit_kernel, dim3((unsigned)(plan->n_frames * bands * parts)), dim3(kEmitThreads), 0,
-                               stream, plan->frames,
-                               plan->sweeps, at<int32_t>(plan, L.vis_list), pair_count, pair_off, pair_rect, pair_dk,
-                               tile_off, tile_cursor,
-                               at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), bands, parts,
-                               (const us3d_stats*)stats);
-        }
-        if ((rc = check_launch("ug_emit_kernel"))) return rc;
-    }
-    return sort_segments(at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<uint64_t>(plan, L.seg_key_tmp),
-                         at<int32_t>(plan, L.seg_val_tmp), tile_off, plan->n_tiles, stats, plan->k_capacity, stream);
 }
 
 // ---------------------------------------------------------------- static scratch (no-allocation render)
@@ -1420,13 +1244,18 @@ static int static_scratch_ptr(void** out) {
     return US3D_OK;
 }
 
+// Visible-pair capacity of the static render for instance capacity k: every visible pair has an instance, so
+// min(n_pairs, k) overflows only when the instances do.
+static int64_t static_v_capacity(int64_t n_pairs, int64_t k) { return std::max<int64_t>(1, std::min(n_pairs, k)); }
+
 // Largest instance capacity whose workspace fits `bytes` (0 if none does); the layout grows with k.
 static int64_t static_k_capacity(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t bytes) {
-    if (ug_layout(n_gauss, n_pairs, n_tiles, 1).total > bytes) return 0;
+    auto total = [&](int64_t k) { return ug_layout(n_gauss, n_pairs, n_tiles, k, static_v_capacity(n_pairs, k)).total; };
+    if (total(1) > bytes) return 0;
     int64_t lo = 1, hi = (int64_t(1) << 31) - 1;
     while (lo < hi) {
         const int64_t mid = lo + (hi - lo + 1) / 2;
-        if (ug_layout(n_gauss, n_pairs, n_tiles, mid).total <= bytes) lo = mid; else hi = mid - 1;
+        if (total(mid) <= bytes) lo = mid; else hi = mid - 1;
     }
     return lo;
 }
@@ -1439,21 +1268,21 @@ using namespace us3d;
 
 extern "C" {
 
-int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity) {
-    return ug_layout(n_gauss, n_pairs, n_tiles, k_capacity).total;
+int64_t us3d_ug_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
+                                int64_t v_capacity) {
+    return ug_layout(n_gauss, n_pairs, n_tiles, k_capacity, v_capacity).total;
 }
 
 int us3d_ug_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_capacity,
-                             us3d_layout* out) {
+                             int64_t v_capacity, us3d_layout* out) {
     if (!out) return fail(US3D_ERR_ARGUMENT, "out is NULL");
-    *out = ug_layout(n_gauss, n_pairs, n_tiles, k_capacity);
+    *out = ug_layout(n_gauss, n_pairs, n_tiles, k_capacity, v_capacity);
     return US3D_OK;
 }
 
 int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, const float* opacity,
                 const float* intensity, const us3d_ug_settings* settings, void* stream_) {
-    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                       plan ? plan->k_capacity : 0);
+    const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     rc = check_settings(settings);
@@ -1461,15 +1290,10 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     if (plan->n_gauss > 0 && (!means || !tril || !opacity || !intensity))
         return fail(US3D_ERR_ARGUMENT, "Gaussian parameter pointer is NULL");
     hipStream_t stream = as_stream(stream_);
-    us3d_stats* stats = at<us3d_stats>(plan, L.stats);
     float4* rec = at<float4>(plan, L.records);
-    int32_t* pair_count = at<int32_t>(plan, L.pair_count);
-    int4* pair_rect = at<int4>(plan, L.pair_rect);
-    uint32_t* pair_dk = at<uint32_t>(plan, L.pair_dk);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
 
-    hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
-    if ((rc = begin_instances(plan, L, stream))) return rc;
+    if ((rc = begin_pairs(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ug_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
                            means, tril, opacity, intensity, settings->diag_beta, sqrt_chi2,
@@ -1479,21 +1303,31 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     const int64_t np = plan->n_pairs;
     if (np > 0) {
         {
-            StageTimer timer_("ug_preprocess_kernel", stream);
-            hipLaunchKernelGGL(ug_preprocess_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec,
-                               plan->frames, plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
-                               settings->sort_by_depth, pair_count, pair_rect, pair_dk, at<float4>(plan, L.pair_sect),
-                               at<int32_t>(plan, L.tile_count), at<int32_t>(plan, L.vis_list));
+            StageTimer timer_("ug_cull_kernel", stream);
+            hipLaunchKernelGGL(ug_cull_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, rec, plan->frames,
+                               plan->sweeps, plan->n_sweeps, sqrt_chi2, settings->tile_size,
+                               at<uint64_t>(plan, L.pair_bits), at<int32_t>(plan, L.word_cnt),
+                               at<uint32_t>(plan, L.fm_bits));
         }
-        if ((rc = check_launch("ug_preprocess_kernel"))) return rc;
+        if ((rc = check_launch("ug_cull_kernel"))) return rc;
+        if ((rc = rank_visible_pairs(plan, L, stream))) return rc;
+        {
+            StageTimer timer_("ug_setup_kernel", stream);
+            hipLaunchKernelGGL(ug_setup_kernel, dim3(blocks_for(plan->v_capacity, 256)), dim3(256), 0, stream,
+                               plan->v_capacity, at<int32_t>(plan, L.vis_pair), rec, plan->frames, plan->sweeps,
+                               plan->n_sweeps, sqrt_chi2, settings->tile_size, settings->sort_by_depth,
+                               at<us3d_stats>(plan, L.stats), at<int32_t>(plan, L.pair_count), at<int4>(plan, L.pair_rect),
+                               at<uint32_t>(plan, L.pair_dk), at<float4>(plan, L.pair_sect),
+                               at<int32_t>(plan, L.tile_count));
+        }
+        if ((rc = check_launch("ug_setup_kernel"))) return rc;
     }
     return build_instances(plan, L, stream);
 }
 
 int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
                     float* alpha, float* depth, float* aux, void* stream_) {
-    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                       plan ? plan->k_capacity : 0);
+    const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_settings(settings))) return rc;
@@ -1504,7 +1338,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE, B>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads),   \
                        0, stream,                                                                                     \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
-                       plan->frames, plan->sweeps, plan->frame_tile_base, plan->n_frames, pixels,                     \
+                       at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
@@ -1538,8 +1372,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
 int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tril, const us3d_ug_settings* settings,
                      const float* aux, const float* d_image, const float* d_alpha, const float* d_depth,
                      float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream_) {
-    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                       plan ? plan->k_capacity : 0);
+    const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_settings(settings))) return rc;
@@ -1550,8 +1383,9 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
 #define UG_BWD_B(TS, FULL, PLANE, B)                                                                                  \
     hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE, B>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads),  \
                        0, stream, at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off),                         \
-                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), plan->frames, plan->sweeps,       \
-                       plan->frame_tile_base, plan->n_frames, pixels, at<float4>(plan, L.pair_sect),                  \
+                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<int32_t>(plan, L.pair_off),    \
+                       at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,      \
+                       at<float4>(plan, L.pair_sect),                                                                 \
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
                        reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask))
@@ -1584,7 +1418,8 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         {
             StageTimer timer_("ug_reduce_kernel", stream);
             hipLaunchKernelGGL(ug_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
-                               plan->sweeps, plan->n_sweeps, at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
+                               plan->sweeps, plan->n_sweeps, at<uint64_t>(plan, L.pair_bits),
+                               at<int32_t>(plan, L.word_off), at<int32_t>(plan, L.pair_off), at<float4>(plan, L.partial),
                                tril, at<float4>(plan, L.records), settings->raw_activations ? 1 : 0,
                                at<us3d_stats>(plan, L.stats), d_means, d_tril, d_opacity, d_intensity);
         }
@@ -1595,8 +1430,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
 
 int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, int pass,
                         int32_t* counts, const int32_t* offsets, int32_t* indices, void* stream_) {
-    const us3d_layout L = ug_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                       plan ? plan->k_capacity : 0);
+    const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_settings(settings))) return rc;
@@ -1632,10 +1466,11 @@ int us3d_ug_render_static(const us3d_plan* plan_in, const float* means, const fl
     plan.workspace = ws;
     plan.workspace_bytes = US3D_STATIC_SCRATCH_BYTES;
     plan.k_capacity = k_cap;
+    plan.v_capacity = static_v_capacity(plan.n_pairs, k_cap);
     if ((rc = us3d_ug_map(&plan, means, tril, opacity, intensity, settings, stream_))) return rc;
     if ((rc = us3d_ug_forward(&plan, pixels, settings, image, alpha, depth, aux, stream_))) return rc;
     if (overflow_dev) {
-        const us3d_layout L = ug_layout(plan.n_gauss, plan.n_pairs, plan.n_tiles, plan.k_capacity);
+        const us3d_layout L = ug_layout(&plan);
         hipLaunchKernelGGL(copy_overflow_kernel, dim3(1), dim3(1), 0, as_stream(stream_), at<us3d_stats>(&plan, L.stats),
                            overflow_dev);
         if ((rc = check_launch("copy_overflow_kernel"))) return rc;

@@ -20,8 +20,13 @@ constexpr float kXi = 0.7745966692f;
 constexpr float kW0 = 0.5555555556f;
 constexpr float kW1 = 0.8888888889f;
 
-us3d_layout ur_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, 64, 64, 0, 0);
+us3d_layout ur_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 64, 64, 0, 0);
+}
+
+us3d_layout ur_layout(const us3d_plan* plan) {
+    return plan ? ur_layout(plan->n_gauss, plan->n_pairs, plan->n_tiles, plan->k_capacity, plan->v_capacity)
+                : us3d_layout{};
 }
 
 // One record per Gaussian: (mu, raw attenuation | scales, m | quaternion | SH). With `log_scales` the scales
@@ -57,52 +62,84 @@ __global__ void ur_pack_kernel(int n, const float* __restrict__ means, const flo
 // kernel_project_and_cull, ops/ultragray_renderer/src/mapper/core.cu:3-68: per (frame, Gaussian) the inclusive
 // scanline range [floor(c - r/ps), ceil(c + r/ps)] clipped to [0, W-1] with r = k * max(s); culled when
 // z + r < near or |elev| > r + margin (and, defined here, when the projection is not finite). Depth key =
-// (uint32)(max(0, z) * 1000) (core.cu:86), saturating.
-__global__ __launch_bounds__(256) void ur_preprocess_kernel(
-    int64_t n_pairs, const float4* __restrict__ rec, const us3d_frame* __restrict__ frames,
-    const us3d_sweep* __restrict__ sweeps, int n_sweeps, float k_sigma, float elev_margin,
-    int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect, uint32_t* __restrict__ pair_dk,
-    int32_t* __restrict__ tile_count, int32_t* __restrict__ vis_list) {
-    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (pair >= n_pairs) return;
-    int count = 0;
-    int4 rect = make_int4(0, 0, 0, 0);
-    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
-    const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
-    const int gl = local / sw.f_count;
-    const int g = sw.g_begin + gl;
-    const int f = sw.f_begin + (local - gl * sw.f_count);
-    const us3d_frame& fr = frames[f];
-    const float4 r0 = rec[4 * (int64_t)g + 0];
-    const float4 r1 = rec[4 * (int64_t)g + 1];
+// (uint32)(max(0, z) * 1000) (core.cu:86), saturating. Returns the ray count (0: culled) with the range as the
+// rectangle (0, lo, 1, hi + 1); the cull and the setup pass call it on the same inputs.
+__device__ __forceinline__ int ur_pair_rays(const float4& r0, const float4& r1, const us3d_frame& fr, float k_sigma,
+                                            float elev_margin, int4& rect, uint32_t& dk) {
     const float dx = r0.x - fr.origin[0], dy = r0.y - fr.origin[1], dz = r0.z - fr.origin[2];
     const float x = dot3(dx, dy, dz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
     const float y = dot3(dx, dy, dz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
     const float z = dot3(dx, dy, dz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
     const float ms = fmaxf(r1.x, fmaxf(r1.y, r1.z));
     const float r = ms * k_sigma;
-    uint32_t dk = 0;
-    if (z + r >= fr.near_mm && fabsf(y) <= r + elev_margin) {
-        const float c = x / fr.delta_lat + fr.col_offset;
-        const float rp = r / fr.delta_lat;
-        const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
-        if (isfinite(f0) && isfinite(f1)) {
-            const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
-            const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
-            if (lo <= hi) {
-                count = hi - lo + 1;
-                rect = make_int4(0, lo, 1, hi + 1);
-                const float q = fmaxf(0.0f, z) * 1000.0f;
-                dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
-            }
-        }
+    if (!(z + r >= fr.near_mm && fabsf(y) <= r + elev_margin)) return 0;
+    const float c = x / fr.delta_lat + fr.col_offset;
+    const float rp = r / fr.delta_lat;
+    const float f0 = floorf(c - rp), f1 = ceilf(c + rp);
+    if (!(isfinite(f0) && isfinite(f1))) return 0;
+    const int lo = max(0, (int)fminf(fmaxf(f0, -1073741824.0f), 1073741824.0f));
+    const int hi = min(fr.width - 1, (int)fminf(fmaxf(f1, -1073741824.0f), 1073741824.0f));
+    if (!(lo <= hi)) return 0;
+    rect = make_int4(0, lo, 1, hi + 1);
+    const float q = fmaxf(0.0f, z) * 1000.0f;
+    dk = q >= 4294967296.0f ? 0xffffffffu : (uint32_t)q;
+    return hi - lo + 1;
+}
+
+// Cull pass: thread per dense (frame, Gaussian) pair; marks the pairs that reach at least one scanline.
+__global__ __launch_bounds__(256) void ur_cull_kernel(int64_t n_pairs, const float4* __restrict__ rec,
+                                                      const us3d_frame* __restrict__ frames,
+                                                      const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                                                      float k_sigma, float elev_margin, uint64_t* __restrict__ pair_bits,
+                                                      int32_t* __restrict__ word_cnt, uint32_t* __restrict__ fm_bits) {
+    const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int count = 0, s = 0, f = 0, gl = 0;
+    if (pair < n_pairs) {
+        s = sweep_of_pair(sweeps, n_sweeps, pair);
+        const us3d_sweep sw = sweeps[s];
+        const int local = (int)(pair - sw.pair_base);  // pair indices are < 2^31 (check_plan)
+        gl = local / sw.f_count;
+        const int g = sw.g_begin + gl;
+        f = sw.f_begin + (local - gl * sw.f_count);
+        int4 rect;
+        uint32_t dk;
+        count = ur_pair_rays(rec[4 * (int64_t)g + 0], rec[4 * (int64_t)g + 1], frames[f], k_sigma, elev_margin, rect,
+                             dk);
     }
-    pair_count[pair] = count;
-    if (count > 0) {  // rect and key are read only for pairs with instances
-        pair_rect[pair] = rect;
-        pair_dk[pair] = dk;
+    mark_pair(count > 0, pair, n_pairs, sweeps, s, f, gl, pair_bits, word_cnt, fm_bits);
+}
+
+// Setup pass: thread per visible pair v < v_capacity (entries past the visible count, or every entry when the
+// visible pairs overflowed, get count 0): ray count, ray range, depth key; rays counted by range ends.
+__global__ __launch_bounds__(256) void ur_setup_kernel(int64_t v_cap, const int32_t* __restrict__ vis_pair,
+                                                       const float4* __restrict__ rec,
+                                                       const us3d_frame* __restrict__ frames,
+                                                       const us3d_sweep* __restrict__ sweeps, int n_sweeps,
+                                                       float k_sigma, float elev_margin,
+                                                       const us3d_stats* __restrict__ stats,
+                                                       int32_t* __restrict__ pair_count, int4* __restrict__ pair_rect,
+                                                       uint32_t* __restrict__ pair_dk,
+                                                       int32_t* __restrict__ tile_count) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= v_cap) return;
+    if (stats->overflow || v >= stats->n_visible) {
+        pair_count[v] = 0;
+        return;
+    }
+    const int64_t pair = vis_pair[v];
+    const us3d_sweep sw = sweeps[sweep_of_pair(sweeps, n_sweeps, pair)];
+    const int local = (int)(pair - sw.pair_base);
+    const int gl = local / sw.f_count;
+    const int g = sw.g_begin + gl;
+    const us3d_frame& fr = frames[sw.f_begin + (local - gl * sw.f_count)];
+    int4 rect = make_int4(0, 0, 0, 0);
+    uint32_t dk = 0;
+    const int count = ur_pair_rays(rec[4 * (int64_t)g + 0], rec[4 * (int64_t)g + 1], fr, k_sigma, elev_margin, rect, dk);
+    pair_count[v] = count;
+    if (count > 0) {  // always, for a listed pair
+        pair_rect[v] = rect;
+        pair_dk[v] = dk;
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
-        vis_list[sw.pair_base + (int64_t)(f - sw.f_begin) * sw.g_count + gl] = (int32_t)pair;
     }
 }
 
@@ -586,9 +623,10 @@ __device__ __forceinline__ void ur_echo_line(const float mu[3], const float S[6]
 template <bool kGrid>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void ur_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
-    const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
-    const float* __restrict__ ray_o, const float* __restrict__ ray_d, const float* __restrict__ ray_off, RayFlags F,
+    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const float* __restrict__ pixels, const float* __restrict__ ray_o, const float* __restrict__ ray_d,
+    const float* __restrict__ ray_off, RayFlags F,
     const float4* __restrict__ aux, const float* __restrict__ d_image, const us3d_stats* __restrict__ stats,
     float4* __restrict__ partial) {
     // the general form needs the samples' positions; on the probe's grid only their depths are read
@@ -1017,7 +1055,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 
         }
         const float4 dq = quat_vjp(r2, dR);
         const float dt = dtau * G.tau * (1.0f - G.tau);
-        const int64_t e = seg_val[i];
+        const int64_t e = emit_index(pair_off, pair_rect, seg_val[i], 0, x);  // scanline x of the pair's range
         partial[4 * e + 0] = make_float4(dmu[0], dmu[1], dmu[2], ds[0]);
         partial[4 * e + 1] = make_float4(ds[1], ds[2], dt, 0.f);
         partial[4 * e + 2] = dq;
@@ -1026,7 +1064,9 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 
 }
 
 __global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
-                                                        int n_sweeps, const int32_t* __restrict__ pair_off,
+                                                        int n_sweeps, const uint64_t* __restrict__ pair_bits,
+                                                        const int32_t* __restrict__ word_off,
+                                                        const int32_t* __restrict__ pair_off,
                                                         const float4* __restrict__ partial,
                                                         const float4* __restrict__ rec, int log_scales,
                                                         const us3d_stats* __restrict__ stats, float* __restrict__ d_means,
@@ -1037,7 +1077,9 @@ __global__ __launch_bounds__(256) void ur_reduce_kernel(int n_gauss, const us3d_
     if (g >= n_gauss || stats->overflow) return;
     const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
-    const int e0 = pair_off[p0], e1 = pair_off[p0 + sw.f_count];
+    // the Gaussian's visible pairs are v in [rank(p0), rank(p0 + f_count)); their instances are consecutive
+    const int e0 = pair_off[pair_rank(pair_bits, word_off, p0)];
+    const int e1 = pair_off[pair_rank(pair_bits, word_off, p0 + sw.f_count)];
     float acc[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) acc[k] = 0.f;
@@ -1094,30 +1136,29 @@ using namespace us3d;
 
 extern "C" {
 
-int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity) {
-    return ur_layout(n_gauss, n_pairs, n_rays, k_capacity).total;
+int64_t us3d_ur_workspace_bytes(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity,
+                                int64_t v_capacity) {
+    return ur_layout(n_gauss, n_pairs, n_rays, k_capacity, v_capacity).total;
 }
 
-int us3d_ur_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity, us3d_layout* out) {
+int us3d_ur_workspace_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_rays, int64_t k_capacity,
+                             int64_t v_capacity, us3d_layout* out) {
     if (!out) return fail(US3D_ERR_ARGUMENT, "out is NULL");
-    *out = ur_layout(n_gauss, n_pairs, n_rays, k_capacity);
+    *out = ur_layout(n_gauss, n_pairs, n_rays, k_capacity, v_capacity);
     return US3D_OK;
 }
 
 int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, const float* quats,
                 const float* attenuations, const float* sh, const us3d_ur_settings* settings, void* stream_) {
-    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = ur_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
     if (plan->n_gauss > 0 && (!means || !scales || !quats || !attenuations || !sh))
         return fail(US3D_ERR_ARGUMENT, "Gaussian parameter pointer is NULL");
     hipStream_t stream = as_stream(stream_);
-    us3d_stats* stats = ws_at<us3d_stats>(plan, L.stats);
     float4* rec = ws_at<float4>(plan, L.records);
-    hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, stats);
-    if ((rc = begin_instances(plan, L, stream))) return rc;
+    if ((rc = begin_pairs(plan, L, stream))) return rc;
     if (plan->n_gauss > 0) {
         hipLaunchKernelGGL(ur_pack_kernel, dim3(blocks_for(plan->n_gauss, 256)), dim3(256), 0, stream, plan->n_gauss,
                            means, scales, quats, attenuations, sh, settings->scales_are_log ? 1 : 0, rec);
@@ -1125,14 +1166,24 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
     }
     if (plan->n_pairs > 0) {
         {
-            StageTimer timer_("ur_preprocess_kernel", stream);
-            hipLaunchKernelGGL(ur_preprocess_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
+            StageTimer timer_("ur_cull_kernel", stream);
+            hipLaunchKernelGGL(ur_cull_kernel, dim3(blocks_for(plan->n_pairs, 256)), dim3(256), 0, stream,
                                plan->n_pairs, rec, plan->frames, plan->sweeps, plan->n_sweeps, settings->k_sigma,
-                               settings->elev_cutoff_margin_mm, ws_at<int32_t>(plan, L.pair_count),
-                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk),
-                               ws_at<int32_t>(plan, L.tile_count), ws_at<int32_t>(plan, L.vis_list));
+                               settings->elev_cutoff_margin_mm, ws_at<uint64_t>(plan, L.pair_bits),
+                               ws_at<int32_t>(plan, L.word_cnt), ws_at<uint32_t>(plan, L.fm_bits));
         }
-        if ((rc = check_launch("ur_preprocess_kernel"))) return rc;
+        if ((rc = check_launch("ur_cull_kernel"))) return rc;
+        if ((rc = rank_visible_pairs(plan, L, stream))) return rc;
+        {
+            StageTimer timer_("ur_setup_kernel", stream);
+            hipLaunchKernelGGL(ur_setup_kernel, dim3(blocks_for(plan->v_capacity, 256)), dim3(256), 0, stream,
+                               plan->v_capacity, ws_at<int32_t>(plan, L.vis_pair), rec, plan->frames, plan->sweeps,
+                               plan->n_sweeps, settings->k_sigma, settings->elev_cutoff_margin_mm,
+                               ws_at<us3d_stats>(plan, L.stats), ws_at<int32_t>(plan, L.pair_count),
+                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk),
+                               ws_at<int32_t>(plan, L.tile_count));
+        }
+        if ((rc = check_launch("ur_setup_kernel"))) return rc;
     }
     return build_instances(plan, L, stream);
 }
@@ -1140,8 +1191,7 @@ int us3d_ur_map(const us3d_plan* plan, const float* means, const float* scales, 
 int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray_origins, const float* ray_dirs,
                     const float* ray_offsets, const us3d_ur_settings* settings, float* image, float* aux,
                     void* stream_) {
-    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = ur_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
@@ -1160,8 +1210,7 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
                      const float* ray_offsets, const us3d_ur_settings* settings, const float* aux, const float* d_image,
                      float* d_means, float* d_scales, float* d_quats, float* d_attenuations, float* d_sh,
                      void* stream_) {
-    const us3d_layout L = ur_layout(plan ? plan->n_gauss : 0, plan ? plan->n_pairs : 0, plan ? plan->n_tiles : 0,
-                                    plan ? plan->k_capacity : 0);
+    const us3d_layout L = ur_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_ur_settings(settings))) return rc;
@@ -1174,7 +1223,8 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
         auto kernel = pixels ? ur_backward_kernel<false> : ur_backward_kernel<true>;
         hipLaunchKernelGGL(kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
                            ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
-                           ws_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
+                           ws_at<int32_t>(plan, L.seg_val), ws_at<int32_t>(plan, L.pair_off),
+                           ws_at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
                            ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
                            d_image, ws_at<us3d_stats>(plan, L.stats), ws_at<float4>(plan, L.partial));
     }
@@ -1183,7 +1233,9 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
         {
             StageTimer timer_("ur_reduce_kernel", stream);
             hipLaunchKernelGGL(ur_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream, plan->n_gauss,
-                               plan->sweeps, plan->n_sweeps, ws_at<int32_t>(plan, L.pair_off), ws_at<float4>(plan, L.partial),
+                               plan->sweeps, plan->n_sweeps, ws_at<uint64_t>(plan, L.pair_bits),
+                               ws_at<int32_t>(plan, L.word_off), ws_at<int32_t>(plan, L.pair_off),
+                               ws_at<float4>(plan, L.partial),
                                ws_at<float4>(plan, L.records), settings->scales_are_log ? 1 : 0,
                                ws_at<us3d_stats>(plan, L.stats), d_means, d_scales, d_quats, d_attenuations, d_sh);
         }

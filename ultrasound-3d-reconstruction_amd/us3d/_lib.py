@@ -61,7 +61,8 @@ class Stats(ctypes.Structure):
         ("n_sorted_big", ctypes.c_int32),
         ("overflow_any", ctypes.c_int32),
         ("reserved_i32", ctypes.c_int32),
-        ("reserved", ctypes.c_int64 * 4),
+        ("n_visible", ctypes.c_int64),
+        ("reserved", ctypes.c_int64 * 3),
     ]
 
 
@@ -74,6 +75,7 @@ class Plan(ctypes.Structure):
         ("n_pairs", ctypes.c_int64),
         ("n_pixels", ctypes.c_int64),
         ("k_capacity", ctypes.c_int64),
+        ("v_capacity", ctypes.c_int64),
         ("frames", ctypes.c_void_p),
         ("frame_tile_base", ctypes.c_void_p),
         ("frame_pixel_base", ctypes.c_void_p),
@@ -125,8 +127,10 @@ class GPSettings(ctypes.Structure):
 
 
 _LAYOUT_FIELDS = [
-    "stats", "pair_count", "pair_off", "pair_rect", "pair_dk", "tile_count", "tile_off", "tile_cursor",
-    "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp", "vis_list", "inst_mask", "pair_sect", "total",
+    "stats", "pair_bits", "word_cnt", "word_off", "fm_bits", "fm_cnt", "fm_off", "vis_pair",
+    "pair_count", "pair_off", "pair_rect", "pair_dk", "vis_list", "tile_count", "tile_off",
+    "tile_cursor", "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp",
+    "inst_mask", "pair_sect", "total",
 ]
 
 
@@ -142,8 +146,8 @@ _F = ctypes.c_float
 _PP = ctypes.POINTER(Plan)
 
 SIGNATURES = {
-    "us3d_ug_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64]),
-    "us3d_ug_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
+    "us3d_ug_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64, _I64]),
+    "us3d_ug_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
     "us3d_ug_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, ctypes.POINTER(UGSettings), _VP]),
     "us3d_ug_forward": (ctypes.c_int, [_PP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP]),
     "us3d_ug_backward": (
@@ -155,16 +159,16 @@ SIGNATURES = {
         ctypes.c_int,
         [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _VP],
     ),
-    "us3d_ur_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64]),
-    "us3d_ur_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
+    "us3d_ur_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64, _I64]),
+    "us3d_ur_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
     "us3d_ur_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(URSettings), _VP]),
     "us3d_ur_forward": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, ctypes.POINTER(URSettings), _VP, _VP, _VP]),
     "us3d_ur_backward": (
         ctypes.c_int,
         [_PP, _VP, _VP, _VP, _VP, ctypes.POINTER(URSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
-    "us3d_gp_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64]),
-    "us3d_gp_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
+    "us3d_gp_workspace_bytes": (_I64, [_I64, _I64, _I64, _I64, _I64]),
+    "us3d_gp_workspace_layout": (ctypes.c_int, [_I64, _I64, _I64, _I64, _I64, ctypes.POINTER(Layout)]),
     "us3d_gp_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(GPSettings), _VP]),
     "us3d_gp_forward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP]),
     "us3d_gp_backward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP]),

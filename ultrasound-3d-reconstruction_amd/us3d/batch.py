@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -178,7 +178,7 @@ class DeviceTables:
             self.sweeps = torch.from_numpy(lay.sweep_rows.copy()).to(device)
         self.lay = lay
 
-    def plan(self, workspace: torch.Tensor, k_capacity: int) -> _lib.Plan:
+    def plan(self, workspace: torch.Tensor, k_capacity: int, v_capacity: int) -> _lib.Plan:
         lay = self.lay
         p = _lib.Plan()
         p.n_gauss = lay.n_gauss
@@ -188,6 +188,7 @@ class DeviceTables:
         p.n_pairs = lay.n_pairs
         p.n_pixels = lay.n_pixels
         p.k_capacity = int(k_capacity)
+        p.v_capacity = int(v_capacity)
         p.frames = _lib.ptr(self.frames)
         p.frame_tile_base = _lib.ptr(self.tile_base)
         p.frame_pixel_base = _lib.ptr(self.pixel_base)
@@ -211,20 +212,41 @@ def read_stats(workspace: torch.Tensor) -> _lib.Stats:
     return _lib.Stats.from_buffer_copy(raw)
 
 
+I32_LIMIT = (1 << 31) - 1
+
+
 @dataclass
 class CapacityHint:
-    """Remembers the last instance count so the next workspace rarely overflows."""
+    """Remembers the last instance and visible-pair counts so the next workspace rarely overflows."""
 
     k_last: int = 0
+    v_last: int = 0
     growth: float = 1.25
     floor: int = 1 << 16
 
     def capacity(self, n_pairs: int) -> int:
         guess = max(self.floor, int(self.k_last * self.growth) + 1024, min(4 * n_pairs, 1 << 22))
-        return min(guess, (1 << 31) - 1)
+        return min(guess, I32_LIMIT)
 
-    def update(self, k: int) -> None:
+    def v_capacity(self, n_pairs: int) -> int:
+        """Visible (frame, Gaussian) pairs: at most n_pairs."""
+        guess = max(1 << 12, int(self.v_last * self.growth) + 1024, min(n_pairs, 1 << 18))
+        return max(1, min(guess, n_pairs, I32_LIMIT))
+
+    def update(self, k: int, v: int = 0) -> None:
         self.k_last = max(self.k_last, int(k))
+        self.v_last = max(self.v_last, int(v))
+
+    def regrow(self, k_cap: int, v_cap: int, st: _lib.Stats, n_pairs: int) -> Tuple[int, int]:
+        """Capacities for the retry of a map that overflowed: the visible pairs and the instances it counted, with
+        headroom (a map whose visible pairs overflowed counts no instances, so K may need a second retry)."""
+        if st.n_visible > v_cap:
+            v_cap = max(1, min(int(st.n_visible * self.growth) + 1024, n_pairs, I32_LIMIT))
+        if st.n_instances > k_cap:
+            if st.n_instances >= I32_LIMIT:
+                raise RuntimeError("instance count exceeds 2^31; split the batch")
+            k_cap = min(int(st.n_instances * self.growth) + 1024, I32_LIMIT)
+        return k_cap, v_cap
 
 
 @dataclass
@@ -234,8 +256,9 @@ class Mapped:
     tables: DeviceTables
     workspace: torch.Tensor
     k_capacity: int
+    v_capacity: int
     stats: Optional[_lib.Stats] = None
     extra: dict = field(default_factory=dict)
 
     def plan(self) -> _lib.Plan:
-        return self.tables.plan(self.workspace, self.k_capacity)
+        return self.tables.plan(self.workspace, self.k_capacity, self.v_capacity)

@@ -14,12 +14,12 @@ parameters. The overflow check synchronises; `check_every` sets how often it run
 from __future__ import annotations
 
 import ctypes
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import torch
 
 from . import _lib
-from .batch import CapacityHint, FrameSpec, SweepSpec, new_workspace, read_stats
+from .batch import I32_LIMIT, CapacityHint, FrameSpec, SweepSpec, new_workspace, read_stats
 from .ultragauss import UGBatch, UltraGaussRenderConfig
 
 
@@ -103,23 +103,26 @@ class SliceSplatTrainer:
         self.graph = None
         self._allocate(self._sizing_capacity())
 
-    def _sizing_capacity(self) -> int:
-        # size the instance arrays from eager mappings of the current parameters (largest chunk), releasing each
-        # mapping's workspace before the next
-        k = 0
+    def _sizing_capacity(self) -> Tuple[int, int]:
+        # size the instance and visible-pair arrays from eager mappings of the current parameters (largest chunk),
+        # releasing each mapping's workspace before the next
+        k = v = 0
         with torch.no_grad():
             for c in self.chunks:
-                k = max(k, int(c[4].map(self.means, self.tril, self.raw_o, self.raw_c).stats.n_instances))
-        return int(k * self.headroom) + 4096
+                st = c[4].map(self.means, self.tril, self.raw_o, self.raw_c).stats
+                k, v = max(k, int(st.n_instances)), max(v, int(st.n_visible))
+        return int(k * self.headroom) + 4096, int(v * self.headroom) + 1024
 
     # ------------------------------------------------------------------ plumbing
-    def _allocate(self, k_cap: int) -> None:
-        self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+    def _allocate(self, caps: Tuple[int, int]) -> None:
+        self.k_capacity = min(int(caps[0]), I32_LIMIT)
+        self.v_capacity = max(1, min(int(caps[1]), max(c[4].layout.n_pairs for c in self.chunks), I32_LIMIT))
         self.graph = None
         self.workspace = None  # the old workspace goes before the new one is allocated
-        self.workspace = new_workspace(max(c[4].workspace_bytes(self.k_capacity) for c in self.chunks), self.dev)
-        self.plans = [c[4].tables.plan(self.workspace, self.k_capacity) for c in self.chunks]
-        off = self.batch.layout_offsets(self.k_capacity)
+        self.workspace = new_workspace(max(c[4].workspace_bytes(self.k_capacity, self.v_capacity)
+                                           for c in self.chunks), self.dev)
+        self.plans = [c[4].tables.plan(self.workspace, self.k_capacity, self.v_capacity) for c in self.chunks]
+        off = self.batch.layout_offsets(self.k_capacity, self.v_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
         self.overflow_view = self.workspace[off.stats + _lib.Stats.overflow.offset:
                                             off.stats + _lib.Stats.overflow.offset + 4].view(torch.int32)
@@ -194,7 +197,9 @@ class SliceSplatTrainer:
                 st = read_stats(self.workspace)
                 if not st.overflow:
                     return
-                self._allocate(int(st.n_instances * self.headroom) + 4096)
+                k = int(st.n_instances * self.headroom) + 4096 if st.n_instances > self.k_capacity else self.k_capacity
+                v = int(st.n_visible * self.headroom) + 1024 if st.n_visible > self.v_capacity else self.v_capacity
+                self._allocate((k, v))
             else:
                 if not int(self.skip.item()):
                     return
@@ -256,16 +261,17 @@ class GaussianPileTrainer:
         self.n_steps = 0
         self.graph = None
         with torch.no_grad():
-            k = int(self.batch.map(self.means, self.raw_s, self.raw_q, self.raw_o).stats.n_instances)
-        self._allocate(int(k * headroom) + 4096)
+            st = self.batch.map(self.means, self.raw_s, self.raw_q, self.raw_o).stats
+        self._allocate((int(st.n_instances * headroom) + 4096, int(st.n_visible * headroom) + 1024))
 
-    def _allocate(self, k_cap: int) -> None:
-        self.k_capacity = min(int(k_cap), (1 << 31) - 1)
+    def _allocate(self, caps: Tuple[int, int]) -> None:
+        self.k_capacity = min(int(caps[0]), I32_LIMIT)
+        self.v_capacity = max(1, min(int(caps[1]), self.batch.layout.n_pairs, I32_LIMIT))
         self.graph = None
         self.workspace = None  # the old workspace goes before the new one is allocated
-        self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity), self.dev)
-        self.plan = self.batch.tables.plan(self.workspace, self.k_capacity)
-        off = self.batch.layout_offsets(self.k_capacity)
+        self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity, self.v_capacity), self.dev)
+        self.plan = self.batch.tables.plan(self.workspace, self.k_capacity, self.v_capacity)
+        off = self.batch.layout_offsets(self.k_capacity, self.v_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
         self.graph = None
 

@@ -116,23 +116,26 @@ class UGBatch:
         self.tables = DeviceTables(frames, self.layout, device)
         self.hint = hint or CapacityHint()
 
-    def workspace_bytes(self, k_cap: int) -> int:
+    def workspace_bytes(self, k_cap: int, v_cap: int) -> int:
         lay = self.layout
-        return int(_lib.lib().us3d_ug_workspace_bytes(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap))
+        return int(_lib.lib().us3d_ug_workspace_bytes(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap, v_cap))
 
-    def layout_offsets(self, k_cap: int) -> _lib.Layout:
+    def layout_offsets(self, k_cap: int, v_cap: int) -> _lib.Layout:
         lay = self.layout
         out = _lib.Layout()
-        _lib.check(_lib.lib().us3d_ug_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap, out),
+        _lib.check(_lib.lib().us3d_ug_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap, v_cap, out),
                    "us3d_ug_workspace_layout")
         return out
 
-    def map(self, means, tril, opacity, intensity, k_capacity: Optional[int] = None, check: bool = True) -> Mapped:
+    def map(self, means, tril, opacity, intensity, k_capacity: Optional[int] = None, v_capacity: Optional[int] = None,
+            check: bool = True) -> Mapped:
         L = _lib.lib()
-        k_cap = int(k_capacity or self.hint.capacity(self.layout.n_pairs))
+        n_pairs = self.layout.n_pairs
+        k_cap = int(k_capacity or self.hint.capacity(n_pairs))
+        v_cap = int(v_capacity or self.hint.v_capacity(n_pairs))
         while True:
-            ws = new_workspace(self.workspace_bytes(k_cap), self.device)
-            m = Mapped(self.tables, ws, k_cap)
+            ws = new_workspace(self.workspace_bytes(k_cap, v_cap), self.device)
+            m = Mapped(self.tables, ws, k_cap, v_cap)
             plan = m.plan()
             _lib.check(L.us3d_ug_map(plan, _lib.ptr(means), _lib.ptr(tril), _lib.ptr(opacity), _lib.ptr(intensity),
                                      self.settings, _lib.stream_ptr(self.device)), "us3d_ug_map")
@@ -140,12 +143,10 @@ class UGBatch:
                 return m
             st = read_stats(ws)
             m.stats = st
-            self.hint.update(st.n_instances)
+            self.hint.update(st.n_instances, st.n_visible)
             if not st.overflow:
                 return m
-            k_cap = min(int(st.n_instances * self.hint.growth) + 1024, (1 << 31) - 1)
-            if st.n_instances >= (1 << 31) - 1:
-                raise RuntimeError("instance count exceeds 2^31; split the batch")
+            k_cap, v_cap = self.hint.regrow(k_cap, v_cap, st, n_pairs)
 
     def forward(self, m: Mapped, pixels: torch.Tensor):
         n = self.layout.n_pixels
@@ -194,7 +195,7 @@ class UGBatch:
 
     def sorted_view(self, m: Mapped) -> Dict[str, torch.Tensor]:
         """Reference-format mapper outputs per frame: keys (tile<<32 | depth key), values, tile ranges."""
-        off = self.layout_offsets(m.k_capacity)
+        off = self.layout_offsets(m.k_capacity, m.v_capacity)
         ws = m.workspace
         K = int(read_stats(ws).n_instances)
         lay = self.layout

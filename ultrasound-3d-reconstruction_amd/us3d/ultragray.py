@@ -138,26 +138,26 @@ class URBatch:
         self.tables = DeviceTables(frames, self.layout, device)
         self.hint = hint or CapacityHint()
 
-    def workspace_bytes(self, k_cap: int) -> int:
+    def workspace_bytes(self, k_cap: int, v_cap: int) -> int:
         lay = self.layout
-        return int(_lib.lib().us3d_ur_workspace_bytes(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap))
+        return int(_lib.lib().us3d_ur_workspace_bytes(lay.n_gauss, lay.n_pairs, lay.n_tiles, k_cap, v_cap))
 
     def map(self, means, scales, quats, atten, sh) -> Mapped:
         L = _lib.lib()
-        k_cap = self.hint.capacity(self.layout.n_pairs)
+        n_pairs = self.layout.n_pairs
+        k_cap = self.hint.capacity(n_pairs)
+        v_cap = self.hint.v_capacity(n_pairs)
         while True:
-            ws = new_workspace(self.workspace_bytes(k_cap), self.device)
-            m = Mapped(self.tables, ws, k_cap)
+            ws = new_workspace(self.workspace_bytes(k_cap, v_cap), self.device)
+            m = Mapped(self.tables, ws, k_cap, v_cap)
             _lib.check(L.us3d_ur_map(m.plan(), _lib.ptr(means), _lib.ptr(scales), _lib.ptr(quats), _lib.ptr(atten),
                                      _lib.ptr(sh), self.settings, _lib.stream_ptr(self.device)), "us3d_ur_map")
             st = read_stats(ws)
             m.stats = st
-            self.hint.update(st.n_instances)
+            self.hint.update(st.n_instances, st.n_visible)
             if not st.overflow:
                 return m
-            if st.n_instances >= (1 << 31) - 1:
-                raise RuntimeError("instance count exceeds 2^31; split the batch")
-            k_cap = min(int(st.n_instances * self.hint.growth) + 1024, (1 << 31) - 1)
+            k_cap, v_cap = self.hint.regrow(k_cap, v_cap, st, n_pairs)
 
     def forward(self, m: Mapped, pixels, ray_o, ray_d, ray_off):
         n = self.layout.n_pixels
@@ -181,7 +181,8 @@ class URBatch:
         """Reference-format mapper outputs per frame: ray counts/offsets, sorted Gaussian indices and depths."""
         off = _lib.Layout()
         lay = self.layout
-        _lib.check(_lib.lib().us3d_ur_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, m.k_capacity, off),
+        _lib.check(_lib.lib().us3d_ur_workspace_layout(lay.n_gauss, lay.n_pairs, lay.n_tiles, m.k_capacity,
+                                                       m.v_capacity, off),
                    "us3d_ur_workspace_layout")
         ws = m.workspace
         K = int(read_stats(ws).n_instances)
