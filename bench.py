@@ -45,6 +45,9 @@ def parse(argv=None):
     ap.add_argument("--frames", type=int, default=256)
     ap.add_argument("--gaussians", type=int, default=200_000)
     ap.add_argument("--tile-size", type=int, default=16)
+    ap.add_argument("--work-queue", type=int, choices=(0, 1), default=1,
+                    help="slice: 1 runs the forward and the adjoint as persistent workgroups over per-XCD (frame, tile) "
+                         "queues, 0 as one workgroup per tile (both are timed in the stage-timing leg either way)")
     ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -467,7 +470,7 @@ def bench_slice(args):
     scene, target, frames = config2(n_frames=F, n_gauss=G)
     H, W = frames[0].height, frames[0].width
     cfg = UltraGaussRenderConfig(H=H, W=W, lateral_spacing_mm_per_pix=0.2, axial_spacing_mm_per_pix=0.2,
-                                 axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=ts)
+                                 axial_near_mm=0.0, fixed_bg_alpha=1e-6, tile_size=ts, work_queue=bool(args.work_queue))
     hint = CapacityHint()
     with torch.no_grad():
         tgt = [t.to(dev) for t in (target.means, target.tril, target.opacity, target.intensity)]
@@ -510,6 +513,24 @@ def bench_slice(args):
     P = F * H * W
     del tt
     torch.cuda.empty_cache()
+    # the splat kernels in the other launch mode, same step
+    from dataclasses import replace as _replace
+
+    to = SliceSplatTrainer(*init, frames, gt, _replace(cfg, work_queue=not cfg.work_queue), use_graph=False,
+                           check_every=10 ** 9)
+    to.step()
+
+    def run_other():
+        for _ in range(args.timing_steps):
+            to.step()
+
+    stages_other = stage_timing(run_other)
+    del to
+    torch.cuda.empty_cache()
+    modes = {}
+    for wq, st_ in ((cfg.work_queue, stages), (not cfg.work_queue, stages_other)):
+        modes["work_queue" if wq else "one_group_per_tile"] = {
+            k: round(st_[k][1] / max(st_[k][0], 1), 3) for k in ("ug_forward_kernel", "ug_backward_kernel") if k in st_}
 
     fps_p = None
     if not args.no_plain:
@@ -550,11 +571,13 @@ def bench_slice(args):
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"slice splat fwd+bwd+Adam, {G} Gaussians in a 120 mm cube, {F} freehand frames "
                                f"{H}x{W} at 0.2 mm, L1+0.2(1-SSIM), batched {'eager' if args.eager else 'hipGraph'} "
-                               f"step, tile {ts}",
-                   "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis},
+                               f"step, tile {ts}, "
+                               f"{'per-XCD work queues' if cfg.work_queue else 'one workgroup per tile'}",
+                   "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis,
+                   "work_queue": bool(cfg.work_queue)},
         "plain_frames_per_s": fps_p, "batched_over_plain": (fps_b / fps_p) if fps_p else None,
         "iters_per_s": args.steps / dt_b, "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes),
-        "roofline": roof,
+        "roofline": roof, "splat_ms_per_launch_by_launch_mode": modes,
     }
 
 

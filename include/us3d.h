@@ -138,12 +138,17 @@ typedef struct us3d_ug_settings {
                              * splat reads the Gaussians one by one); the outputs do not depend on it. */
     int32_t bwd_lds_batch;  /* the same for the adjoint (0: 64/256/256 at tiles 8/16/32); the gradients depend on
                              * it only through the FP32 rounding of the per-Gaussian sums. */
+    int32_t work_queue;     /* 0: the forward and the adjoint launch one workgroup per (frame, tile) in an XCD-aware
+                             * order; 1: persistent workgroups (as many as the device holds) take the tiles from
+                             * per-XCD queues in the same order. No counterpart in the reference (one launch per
+                             * frame); the outputs do not depend on it. */
 } us3d_ug_settings;
 
 /* Byte offsets of the arrays inside a workspace (for inspection/debug outputs). Shared by the renderers.
  * (frame, Gaussian) pairs are numbered densely (us3d_sweep); visible pairs (at least one instance) get a compact
  * index v, their rank in that order, so a Gaussian's visible pairs are consecutive in frame order.
- * Arrays: pair_bits u64[ceil(n_pairs/64)] (bit p: pair p is visible), word_cnt i32[words] and word_off
+ * Arrays: queue (the heads of the per-XCD tile queues of the work_queue launches), pair_bits u64[ceil(n_pairs/64)]
+ * (bit p: pair p is visible), word_cnt i32[words] and word_off
  * i32[words+1] (popcount of each word and its exclusive scan: v = word_off[p/64] + popcount of the bits below p),
  * fm_bits u32[fm_words] (frame-major visibility: per frame ceil(g_count/32) words, bit gl for Gaussian
  * g_begin+gl), fm_cnt / fm_off (their popcounts and scan: the slot of a visible pair in vis_list), vis_pair
@@ -158,7 +163,7 @@ typedef struct us3d_ug_settings {
  * UltraGauss f32[12], its part of the sub-block test and the plane form of q on the frame's grid; GaussianPile
  * f32[8], its focus footprint; written by the map, read by the forward and the backward). */
 typedef struct us3d_layout {
-    int64_t stats, pair_bits, word_cnt, word_off, fm_bits, fm_cnt, fm_off, vis_pair;
+    int64_t stats, queue, pair_bits, word_cnt, word_off, fm_bits, fm_cnt, fm_off, vis_pair;
     int64_t pair_count, pair_off, pair_rect, pair_dk, vis_list, tile_count, tile_off;
     int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
     int64_t inst_mask, pair_sect, total;

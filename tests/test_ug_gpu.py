@@ -217,6 +217,47 @@ def test_lds_batch_sizes_give_the_same_image_and_gradients(dev, ts):
         assert rel_l2(g, r) < GRAD_TOL, f"ts {ts} {name}: rel L2 {rel_l2(g, r):.3e}"
 
 
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_work_queue_launch_equals_one_workgroup_per_tile(dev, ts):
+    """work_queue: persistent workgroups (as many as the device holds) take the (frame, tile) items from per-XCD
+    queues instead of one workgroup per tile. 8 frames of 512x512 give 2k-33k tiles, several per resident workgroup,
+    some empty. Every tile is rendered once by some workgroup, so the image, alpha, depth and every gradient equal
+    the one-workgroup-per-tile launch bit for bit, on the frames' grids and from explicit pixels; a captured training
+    step with the queues replays equal to the eager step without them."""
+    from dataclasses import replace
+
+    from us3d.train_step import SliceSplatTrainer
+
+    scene, frames = small_case(n=30000, n_frames=8, H=512, W=512, spacing=0.1)
+    rng = np.random.default_rng(21)
+    P = sum(f.height * f.width for f in frames)
+    grads = (rng.normal(size=P).astype(np.float32), rng.normal(size=P).astype(np.float32) * 0.1, None)
+    cfg = cfg_for(frames[0], tile_size=ts)
+    for pixels in ("procedural", "explicit"):
+        ref = gpu_render(scene, frames, replace(cfg, work_queue=False), dev, pixels=pixels, grads=grads)
+        out = gpu_render(scene, frames, replace(cfg, work_queue=True), dev, pixels=pixels, grads=grads)
+        assert float(ref["image"].detach().abs().max()) > 0
+        for key in ("image", "alpha", "depth"):
+            assert torch.equal(out[key], ref[key]), f"{pixels}: {key} differs"
+        for name, a, b in zip(("means", "tril", "opacity", "intensity"), out["grads"], ref["grads"]):
+            assert np.array_equal(a, b), f"{pixels}: d{name} differs"
+    if ts != 16:
+        return
+    with torch.no_grad():
+        gt = gpu_render(make_scene(30000, 64.0, seed=5), frames, cfg, dev, pixels="procedural")["image"].detach()
+    logit = lambda p: torch.log(p / (1 - p))  # noqa: E731
+    init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
+    a = SliceSplatTrainer(*init, frames, gt, replace(cfg, work_queue=True), use_graph=True)
+    b = SliceSplatTrainer(*init, frames, gt, replace(cfg, work_queue=False), use_graph=False)
+    for _ in range(3):
+        a.step()
+        b.step()
+        torch.cuda.synchronize()
+        for x, y in zip(a.state(), b.state()):
+            assert torch.equal(x, y)
+        assert torch.equal(a.losses(), b.losses())
+
+
 def test_bitwise_deterministic(dev):
     scene, frames = small_case(n_frames=3)
     cfg = cfg_for(frames[0])

@@ -108,6 +108,43 @@ __device__ __forceinline__ int remap_xcd(int b, int n) {
     return (q / kXcdRun) * kSuper + r * kXcdRun + (q % kXcdRun);
 }
 
+// Persistent (frame, tile) work queues. Instead of one workgroup per tile, a persistent launch keeps as many
+// workgroups resident as the chip holds, and they take tiles from eight queues, one per XCD: workgroup b serves
+// queue b % 8 (the XCD the round-robin dispatch places it on) and takes that XCD's tiles in the remap_xcd order, so
+// a queue walks runs of neighbouring tiles whose Gaussians are in the XCD's L2, and a workgroup whose tiles cost
+// more simply takes fewer of them. A workgroup whose queue is exhausted moves on to the following queues; it exits
+// when all eight are. The queue heads (int32 counters kQueueStride apart) are cleared by a fill kernel before each
+// launch, and every tile is handed out exactly once (one atomicAdd per take), so the outputs equal the one-workgroup-
+// per-tile launch's bit for bit.
+constexpr int kQueueStride = 64;
+
+// Tile of item i of XCD queue r over n tiles (the tiles remap_xcd gives workgroups b = r mod 8), -1 past its end.
+__device__ __forceinline__ int queue_tile(int r, int i, int n) {
+    constexpr int kSuper = kXcds * kXcdRun;
+    const int nsb = n / kSuper;
+    if (i < nsb * kXcdRun) return (i / kXcdRun) * kSuper + r * kXcdRun + (i % kXcdRun);
+    const int64_t t = (int64_t)nsb * kSuper + r + (int64_t)kXcds * (i - nsb * kXcdRun);
+    return t < n ? (int)t : -1;
+}
+
+// Next tile of a persistent workgroup (every thread calls it; -1: all queues exhausted). The first barrier also
+// separates the previous tile's use of LDS from the next tile's. q: the queue thread 0 draws from (starts at
+// blockIdx.x % kXcds; only thread 0's copy is used).
+__device__ __forceinline__ int next_queue_tile(int32_t* __restrict__ heads, int& q, int n, int* s_slot) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int t = -1;
+        for (int k = 0; k < kXcds; ++k) {
+            t = queue_tile(q, atomicAdd(&heads[q * kQueueStride], 1), n);
+            if (t >= 0) break;
+            q = (q + 1) % kXcds;
+        }
+        *s_slot = t;
+    }
+    __syncthreads();
+    return *s_slot;
+}
+
 // Binary search: largest i in [0, n) with base[i] <= v (base ascending, base[0] <= v).
 __device__ __forceinline__ int upper_index(const int32_t* __restrict__ base, int n, int v) {
     int lo = 0, hi = n - 1;

@@ -43,6 +43,7 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     };
     const int64_t nw = pair_words(n_pairs), nf = fm_words(n_pairs, n_tiles);
     L.stats = take(sizeof(us3d_stats));
+    L.queue = take(kXcds * kQueueStride * 4);
     L.pair_bits = take(nw * 8);
     L.word_cnt = take(nw * 4);
     L.word_off = take((nw + 1) * 4);

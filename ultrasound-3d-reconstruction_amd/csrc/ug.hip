@@ -12,7 +12,9 @@
 //   partial   f32x12 per instance, written at e by the adjoint, so each Gaussian's partials across all
 //             frames are contiguous and reduce in a fixed order.
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <utility>
 
 #include "common.h"
 
@@ -221,10 +223,12 @@ __device__ __forceinline__ int ug_pair_tiles(const float4& r0, const float4& r1,
                                              float sqrt_chi2, int ts, int sort_by_depth, UGPairTiles& out) {
     const float l11 = r1.y, l12 = r1.z, l13 = r1.w, l22 = r2.x, l23 = r2.y, l33 = r2.z;
     const float vx = r0.x - fr.origin[0], vy = r0.y - fr.origin[1], vz = r0.z - fr.origin[2];
-    const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
     const float ele = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
+    // beyond every marginal radius (see ug_pack_kernel); NaN takes the exact path. Most pairs leave here, before the
+    // frame's other axes are read.
+    if (fabsf(ele) > r2.w) return 0;
+    const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
     const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-    if (fabsf(ele) > r2.w) return 0;  // beyond every marginal radius (see ug_pack_kernel); NaN takes the exact path
     const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
     const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
     const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
@@ -621,23 +625,35 @@ void ug_forward_kernel(
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
     float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
-    uint16_t* __restrict__ inst_mask) {
+    uint16_t* __restrict__ inst_mask, int n_tiles, int32_t* __restrict__ queue_heads) {
     __shared__ FwdLds<TS, kBatch> lds;
+    __shared__ int s_tile;
     if (stats->overflow) return;
-    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
-    const int f = frame_of_tile(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect, chi2,
-                                          sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
-                                          out_alpha, out_depth, out_aux, inst_mask);
-    else
-        ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect, chi2,
-                                           sqrt_chi2, alpha_min, bg_alpha, bg_intensity, alpha_excludes_bg, out_image,
-                                           out_alpha, out_depth, out_aux, inst_mask);
+    auto item = [&](int t) {
+        const int f = frame_of_tile(frame_tile_base, n_frames, t);
+        const us3d_frame& fr = frames[f];
+        const int lt = t - fr.tile_base;
+        const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+        const int s0 = tile_off[t], s1 = tile_off[t + 1];
+        if (kPlane && fr.geometry == 0)  // workgroup-uniform
+            ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect,
+                                                      chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
+                                                      alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
+                                                      inst_mask);
+        else
+            ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels,
+                                                       pair_sect, chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
+                                                       alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
+                                                       inst_mask);
+    };
+    if (!queue_heads) {  // one workgroup per tile
+        item(remap_xcd(blockIdx.x, (int)gridDim.x));
+        return;
+    }
+    int q = blockIdx.x % kXcds;  // persistent workgroups over the per-XCD tile queues
+    for (int t = next_queue_tile(queue_heads, q, n_tiles, &s_tile); t >= 0;
+         t = next_queue_tile(queue_heads, q, n_tiles, &s_tile))
+        item(t);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -1078,24 +1094,35 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
-    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask) {
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask,
+    int n_tiles, int32_t* __restrict__ queue_heads) {
     __shared__ BwdLds<TS, kPlane, kBatch> lds;
+    __shared__ int s_tile;
     if (stats->overflow) return;
-    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
-    const int f = frame_of_tile(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    if (s0 == s1) return;
-    if (kPlane && fr.geometry == 0)  // workgroup-uniform
-        ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx, ty, s0, s1, pixels, pair_sect,
-                                          chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
-                                          inst_mask);
-    else
-        ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx, ty, s0, s1, pixels, pair_sect,
-                                           chi2, alpha_min, bg_alpha, aux, d_image, d_alpha, d_depth, partial,
-                                           inst_mask);
+    auto item = [&](int t) {
+        const int f = frame_of_tile(frame_tile_base, n_frames, t);
+        const us3d_frame& fr = frames[f];
+        const int lt = t - fr.tile_base;
+        const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+        const int s0 = tile_off[t], s1 = tile_off[t + 1];
+        if (s0 == s1) return;  // workgroup-uniform
+        if (kPlane && fr.geometry == 0)  // workgroup-uniform
+            ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx,
+                                                                ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
+                                                                aux, d_image, d_alpha, d_depth, partial, inst_mask);
+        else
+            ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx,
+                                                               ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
+                                                               aux, d_image, d_alpha, d_depth, partial, inst_mask);
+    };
+    if (!queue_heads) {  // one workgroup per tile
+        item(remap_xcd(blockIdx.x, (int)gridDim.x));
+        return;
+    }
+    int q = blockIdx.x % kXcds;  // persistent workgroups over the per-XCD tile queues
+    for (int t = next_queue_tile(queue_heads, q, n_tiles, &s_tile); t >= 0;
+         t = next_queue_tile(queue_heads, q, n_tiles, &s_tile))
+        item(t);
 }
 
 // One wavefront per Gaussian: lanes stride over the Gaussian's instances (contiguous in emit order across
@@ -1205,7 +1232,35 @@ static int check_settings(const us3d_ug_settings* s) {
     for (const int b : {s->fwd_lds_batch, s->bwd_lds_batch})
         if (!(b == 0 || b == 64 || b == 128 || b == 256))
             return fail(US3D_ERR_ARGUMENT, "fwd_lds_batch and bwd_lds_batch must be 0, 64, 128 or 256");
+    if (!(s->work_queue == 0 || s->work_queue == 1)) return fail(US3D_ERR_ARGUMENT, "work_queue must be 0 or 1");
     return US3D_OK;
+}
+
+// Workgroups of a splat launch: one per tile, or, with the per-XCD work queues, as many as the device holds at once
+// (occupancy of the kernel times the CU count, cached per kernel and device; never more than the tiles). With the
+// queues their heads are cleared first, by a kernel, so a captured graph replays correctly.
+static int tile_grid(const void* kernel, int threads, const us3d_plan* plan, int32_t* heads, hipStream_t stream) {
+    if (!heads) return plan->n_tiles;
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int groups = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({kernel, dev});
+        if (it != cache.end()) {
+            groups = it->second;
+        } else {
+            int per_cu = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) per_cu = 1;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 1;
+            groups = std::max(1, per_cu) * std::max(1, cus);
+            cache[{kernel, dev}] = groups;
+        }
+    }
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, stream, heads, (int64_t)kXcds * kQueueStride, 0);
+    return std::min(groups, plan->n_tiles);
 }
 
 // Gaussians a splat kernel stages in LDS per batch: the settings' choice, or the kernel's thread count.
@@ -1334,15 +1389,18 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     if (!image || !alpha || !aux) return fail(US3D_ERR_ARGUMENT, "forward: NULL array");
     hipStream_t stream = as_stream(stream_);
     const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
+    int32_t* heads = settings->work_queue ? at<int32_t>(plan, L.queue) : nullptr;
 #define UG_FWD_B(TS, DEPTH, PLANE, B)                                                                                 \
-    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE, B>), dim3(plan->n_tiles), dim3(FwdShape<TS>::kThreads),   \
-                       0, stream,                                                                                     \
+    hipLaunchKernelGGL((ug_forward_kernel<TS, DEPTH, PLANE, B>),                                                      \
+                       dim3(tile_grid((const void*)ug_forward_kernel<TS, DEPTH, PLANE, B>, FwdShape<TS>::kThreads,    \
+                                      plan, heads, stream)),                                                          \
+                       dim3(FwdShape<TS>::kThreads), 0, stream,                                                       \
                        at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
                        at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
-                       reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask))
+                       reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask), plan->n_tiles, heads)
 #define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
     do {                                                                                                              \
         switch (lds_batch(settings->fwd_lds_batch, FwdShape<TS>::kThreads)) {                                                        \
@@ -1379,16 +1437,20 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
     if (!tril || !aux || !d_image || !d_means || !d_tril || !d_opacity || !d_intensity)
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
-    const float sqrt_chi2 = sqrtf(settings->chi2_threshold);
+    int32_t* heads = settings->work_queue ? at<int32_t>(plan, L.queue) : nullptr;
 #define UG_BWD_B(TS, FULL, PLANE, B)                                                                                  \
-    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE, B>), dim3(plan->n_tiles), dim3(TileShape<TS>::kThreads),  \
-                       0, stream, at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off),                         \
+    hipLaunchKernelGGL((ug_backward_kernel<TS, FULL, PLANE, B>),                                                      \
+                       dim3(tile_grid((const void*)ug_backward_kernel<TS, FULL, PLANE, B>, TileShape<TS>::kThreads,   \
+                                      plan, heads, stream)),                                                          \
+                       dim3(TileShape<TS>::kThreads), 0, stream, at<float4>(plan, L.records),                         \
+                       at<int32_t>(plan, L.tile_off),                                                                 \
                        at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<int32_t>(plan, L.pair_off),    \
                        at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,      \
                        at<float4>(plan, L.pair_sect),                                                                 \
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
                        reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
-                       at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask))
+                       at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask),   \
+                       plan->n_tiles, heads)
 #define UG_BWD_(TS, FULL, PLANE)                                                                                      \
     do {                                                                                                              \
         switch (lds_batch(settings->bwd_lds_batch, TileShape<TS>::kThreads)) {                                                       \

@@ -98,6 +98,7 @@ class UGSettings(ctypes.Structure):
         ("raw_activations", ctypes.c_int32),
         ("fwd_lds_batch", ctypes.c_int32),
         ("bwd_lds_batch", ctypes.c_int32),
+        ("work_queue", ctypes.c_int32),
     ]
 
 
@@ -127,7 +128,7 @@ class GPSettings(ctypes.Structure):
 
 
 _LAYOUT_FIELDS = [
-    "stats", "pair_bits", "word_cnt", "word_off", "fm_bits", "fm_cnt", "fm_off", "vis_pair",
+    "stats", "queue", "pair_bits", "word_cnt", "word_off", "fm_bits", "fm_cnt", "fm_off", "vis_pair",
     "pair_count", "pair_off", "pair_rect", "pair_dk", "vis_list", "tile_count", "tile_off",
     "tile_cursor", "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp",
     "inst_mask", "pair_sect", "total",

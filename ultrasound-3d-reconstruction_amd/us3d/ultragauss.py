@@ -38,10 +38,12 @@ class SliceGeometry:
 
 @dataclass
 class UltraGaussRenderConfig:
-    """ops/ultragauss_renderer/ultragauss_renderer/types.py:14-40 (+ tile_size, the LDS batch sizes and semantics).
-    fwd_lds_batch / bwd_lds_batch: Gaussians the forward splat / the adjoint stage in LDS per batch, 64, 128 or
-    256, or 0 for the kernel's thread count; the image does not depend on them and the gradients only through
-    FP32 rounding (the autotuner picks them with the tile size)."""
+    """ops/ultragauss_renderer/ultragauss_renderer/types.py:14-40 (+ tile_size, the LDS batch sizes, the work queue
+    and semantics). fwd_lds_batch / bwd_lds_batch: Gaussians the forward splat / the adjoint stage in LDS per batch,
+    64, 128 or 256, or 0 for the kernel's thread count; the image does not depend on them and the gradients only
+    through FP32 rounding (the autotuner picks them with the tile size). work_queue: the forward and the adjoint run
+    as persistent workgroups taking (frame, tile) items from per-XCD queues instead of one workgroup per tile (same
+    outputs)."""
 
     H: int
     W: int
@@ -60,6 +62,7 @@ class UltraGaussRenderConfig:
     semantics: str = "cuda"
     fwd_lds_batch: int = 0
     bwd_lds_batch: int = 0
+    work_queue: bool = True
 
     def __post_init__(self) -> None:
         if self.H <= 0 or self.W <= 0:
@@ -98,6 +101,7 @@ class UltraGaussRenderConfig:
         s.tile_size = int(self.tile_size)
         s.fwd_lds_batch = int(self.fwd_lds_batch)
         s.bwd_lds_batch = int(self.bwd_lds_batch)
+        s.work_queue = 1 if self.work_queue else 0
         return s
 
 
