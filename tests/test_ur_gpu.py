@@ -238,10 +238,11 @@ def test_ray_sweep_table_that_leaves_gaussians_out(dev):
 
 def test_ray_linear_probe_geometry_in_kernel_equals_explicit_arrays(dev):
     """pixels / ray_origins / ray_dirs = NULL: the kernels evaluate the linear probe's samples and rays from the
-    frame table with the FP32 operations of the pipeline's grid (pipeline_god_file.py:292-333). Image and aux equal
-    those computed from the explicit arrays of that grid bit for bit (two frames, nonzero elevational offsets).
-    The adjoint on the grid forms the echo sums from the one-dimensional quadratic along the scanline, so its
-    gradients equal the explicit arrays' to FP32 rounding (1e-5) and the FP64 oracle's within 1e-4."""
+    frame table with the FP32 operations of the pipeline's grid (pipeline_god_file.py:292-333), two frames, nonzero
+    elevational offsets. On the grid the forward evaluates the window integral and the echo as one-dimensional
+    quadratics along the scanline and the adjoint forms the echo sums from the same quadratic, so image, aux and
+    gradients equal the explicit arrays' to FP32 rounding (image and aux 1e-6, gradients 1e-5), and the grid's image
+    meets the oracle's FP32 and FP64 passes within 1e-5 and its gradients the FP64 oracle's within 1e-4."""
     import torch
 
     from us3d.batch import CapacityHint, SweepSpec
@@ -265,12 +266,14 @@ def test_ray_linear_probe_geometry_in_kernel_equals_explicit_arrays(dev):
     img_e, aux_e = batch.forward(m, pix, ro, rd, off)
     img_p, aux_p = batch.forward(m, None, None, None, off)
     assert float(img_e.abs().max()) > 1e-3
-    assert torch.equal(img_e, img_p)
-    assert torch.equal(aux_e, aux_p)
+    case = "UltraG-Ray on the probe's grid (echo line form)"
+    assert note(case, "image vs explicit arrays", rel_l2(img_p.cpu().numpy(), img_e.cpu().numpy()), 1e-6) < 1e-6
+    for c, name in enumerate(("T", "E", "den")):
+        a, b = aux_p[:, c].cpu().numpy(), aux_e[:, c].cpu().numpy()
+        assert note(case, f"aux {name} vs explicit arrays", rel_l2(a, b), 1e-6) < 1e-6, name
     dB = torch.from_numpy((np.random.default_rng(3).normal(size=img_e.numel()) * 1e-2).astype(np.float32)).to(dev)
     g_e = batch.backward(m, pix, ro, rd, off, aux_e, dB)
     g_p = batch.backward(m, None, None, None, off, aux_p, dB)
-    case = "UltraG-Ray on the probe's grid (echo line form)"
     for name, a, b in zip(("means", "scales", "quats", "atten", "sh"), g_e, g_p):
         assert float(a.abs().max()) > 0
         assert note(case, f"d{name} vs explicit arrays", rel_l2(b.cpu().numpy(), a.cpu().numpy()), 1e-5) < 1e-5, name
@@ -282,6 +285,11 @@ def test_ray_linear_probe_geometry_in_kernel_equals_explicit_arrays(dev):
     for fr, sp in zip(frames, specs):
         n = sp.width * sp.height
         geo, st, params_np, rm = _oracle(scene, fr, cfg, off_np[r0:r0 + sp.width])
+        img32, _aux = our.forward(params_np, geo, rm, st)
+        img64 = our.forward(params_np, geo, rm, st, fp64=True)
+        got = img_p[p0:p0 + n].cpu().numpy()
+        assert note(case, "image vs FP32", rel_l2(got, img32), IMG_TOL) < IMG_TOL
+        assert note(case, "image vs FP64", rel_l2(got, img64), IMG_TOL) < IMG_TOL
         r = our.backward_f64(params_np, geo, rm, st, dB[p0:p0 + n].cpu().numpy())
         refs = r if refs is None else [x + y for x, y in zip(refs, r)]
         p0 += n

@@ -247,7 +247,8 @@ struct RayFlags {
 };
 
 // Which terms of pixel (depth tpx) this Gaussian enters (forward.cu:76-96).
-__device__ __forceinline__ void ray_terms(const RayGauss& G, float tpx, const RayFlags& F, bool& trans, bool& echo) {
+template <typename Hit>
+__device__ __forceinline__ void ray_terms(const Hit& G, float tpx, const RayFlags& F, bool& trans, bool& echo) {
     bool inside = true;
     if (F.local_only) inside = (G.dkf >= tpx - F.local_max) && !(G.dkf > tpx + F.local_max);
     trans = inside && (G.tg < tpx);
@@ -256,7 +257,8 @@ __device__ __forceinline__ void ray_terms(const RayGauss& G, float tpx, const Ra
 }
 
 // GL3 window of pixel depth l: returns false when psi = 0 (empty window or l <= 0).
-__device__ __forceinline__ bool ray_window(const RayGauss& G, float l, float Rw, float& ts, float& te) {
+template <typename Hit>
+__device__ __forceinline__ bool ray_window(const Hit& G, float l, float Rw, float& ts, float& te) {
     if (!(l > 0.0f)) return false;
     ts = Rw > 0.0f ? G.ts : 0.0f;
     te = Rw > 0.0f ? fminf(l, G.b) : l;
@@ -318,6 +320,73 @@ __device__ __forceinline__ RGPack pack_rg(const RayGauss& G) {
     p.e = make_float4(G.S00, G.S01, G.S02, G.S11);
     p.f = make_float4(G.S12, G.S22, G.dkf, G.e_final);
     return p;
+}
+
+// Per hit: the 1-D echo quadratic along a grid scanline, d^T Sigma^-1 d = Qc + 2 c1 tau + c2 tau^2 with d(y) = P(y)
+// - mu, P(y) = B + y e (B the sample-0 position, e the step), tau = y - J about the closest approach J, set up in
+// FP64; returns A = d(J) and (Qc, 2 c1, c2, J).
+__device__ __forceinline__ void ur_echo_line(const float mu[3], const float S[6], const float B[3], const float e[3],
+                                             float A[3], float& Qc, float& c1x2, float& c2f, float& J) {
+    const double b0 = (double)B[0] - mu[0], b1 = (double)B[1] - mu[1], b2 = (double)B[2] - mu[2];
+    const double S00 = S[0], S01 = S[1], S02 = S[2], S11 = S[3], S12 = S[4], S22 = S[5];
+    const double e0 = e[0], e1 = e[1], e2 = e[2];
+    const double se0 = S00 * e0 + S01 * e1 + S02 * e2, se1 = S01 * e0 + S11 * e1 + S12 * e2,
+                 se2 = S02 * e0 + S12 * e1 + S22 * e2;
+    const double c2 = e0 * se0 + e1 * se1 + e2 * se2;
+    const double c1 = b0 * se0 + b1 * se1 + b2 * se2;
+    double j = c2 > 0.0 ? -c1 / c2 : 0.0;
+    j = fmin(fmax(j, -1.0e6), 1.0e6);
+    const double a0 = b0 + j * e0, a1 = b1 + j * e1, a2 = b2 + j * e2;
+    const double q = a0 * (S00 * a0 + S01 * a1 + S02 * a2) + a1 * (S01 * a0 + S11 * a1 + S12 * a2) +
+                     a2 * (S02 * a0 + S12 * a1 + S22 * a2);
+    A[0] = (float)a0; A[1] = (float)a1; A[2] = (float)a2;
+    Qc = (float)fmax(q, 0.0);
+    c1x2 = (float)(2.0 * (c1 + c2 * j));
+    c2f = (float)c2;
+    J = (float)j;
+}
+
+// What the forward needs of one hit when the samples are the linear probe's grid (no explicit sample arrays): along
+// the ray both exponents are one-dimensional quadratics. The GL3 integrand |og + dg t|^2 = a (t - tc)^2 + qmin about
+// the closest approach tc (a = |dg|^2, qmin the whitened distance there), and the echo form
+// d^T Sigma^-1 d = Qc + 2 c1 tau + c2 tau^2 in tau = y - J over the sample index y (ur_echo_line, the same form the
+// adjoint differentiates). Each is a couple of multiply-adds per sample where the general form needs 3-vectors.
+struct RayLine {
+    float tg, rd, tau, f_final;
+    float ts, b, tc, I;
+    float a, qmin, dkf, unused;
+    float Qc, c1x2, c2, J;
+};
+
+struct RLPack {
+    float4 p, q, r, s;
+};
+
+__device__ __forceinline__ RLPack pack_line(const RayLine& L) {
+    return RLPack{make_float4(L.tg, L.rd, L.tau, L.f_final), make_float4(L.ts, L.b, L.tc, L.I),
+                  make_float4(L.a, L.qmin, L.dkf, 0.f), make_float4(L.Qc, L.c1x2, L.c2, L.J)};
+}
+
+__device__ __forceinline__ void unpack_line(const RLPack& k, RayLine& L) {
+    L.tg = k.p.x; L.rd = k.p.y; L.tau = k.p.z; L.f_final = k.p.w;
+    L.ts = k.q.x; L.b = k.q.y; L.tc = k.q.z; L.I = k.q.w;
+    L.a = k.r.x; L.qmin = k.r.y; L.dkf = k.r.z; L.unused = 0.f;
+    L.Qc = k.s.x; L.c1x2 = k.s.y; L.c2 = k.s.z; L.J = k.s.w;
+}
+
+// GL3 of the window [ts, te] in the line form: the same nodes and weights as gl3.
+__device__ __forceinline__ float gl3_line(const RayLine& L, float ts, float te) {
+    const float half = 0.5f * (te - ts);
+    const float u0 = 0.5f * (te + ts) - L.tc;
+    const float xi[3] = {-kXi, 0.0f, kXi};
+    const float wq[3] = {kW0, kW1, kW0};
+    float acc = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const float u = __builtin_fmaf(half, xi[i], u0);
+        acc = __builtin_fmaf(wq[i], __expf(-0.5f * __builtin_fmaf(L.a * u, u, L.qmin)), acc);
+    }
+    return half * acc;
 }
 
 __device__ __forceinline__ void unpack_rg(const RGPack& p, RayGauss& G) {
@@ -393,15 +462,40 @@ __device__ __forceinline__ void ray_accumulate(const RayGauss& G, const RayFlags
     }
 }
 
+// ray_accumulate on the probe's grid: sample index yf, the line forms of the window integral and the echo.
+__device__ __forceinline__ void ray_accumulate_line(const RayLine& G, const RayFlags& F, float yf, float tpx,
+                                                    double& T, double& num, double& den) {
+    bool trans, echo;
+    ray_terms(G, tpx, F, trans, echo);
+    if (trans) {
+        if (F.Rw > 0.0f && tpx >= G.b) {
+            T *= (double)G.f_final;
+        } else {
+            float ts, te, psi = 0.f;
+            if (ray_window(G, tpx, F.Rw, ts, te)) psi = gl3_line(G, ts, te);
+            T *= (double)__builtin_fmaf(1.0f - G.tau, __expf(-psi), G.tau);
+        }
+    }
+    if (echo) {
+        const float tq = yf - G.J;
+        const float w = __expf(-0.5f * __builtin_fmaf(__builtin_fmaf(G.c2, tq, G.c1x2), tq, G.Qc));
+        num = __builtin_fma((double)G.I, (double)w, num);
+        den += (double)w;
+    }
+}
+
 // Four waves per SIMD (at most 128 VGPRs) measured faster than the compiler's three, despite a small spill:
 // the kernel's per-hit LDS reads and branches need the extra waves to hide their latency.
+// kGrid (pixels == NULL): the samples are the linear probe's grid and the hits are staged as line forms (RayLine).
+template <bool kGrid>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void ur_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, const float* __restrict__ ray_o, const float* __restrict__ ray_d,
     const float* __restrict__ ray_off, RayFlags F, const us3d_stats* __restrict__ stats, float* __restrict__ image,
     float4* __restrict__ aux) {
-    __shared__ RGPack s_g[kRayThreads];
+    __shared__ RGPack s_g[kGrid ? 1 : kRayThreads];
+    __shared__ RLPack s_l[kGrid ? kRayThreads : 1];
     __shared__ float s_tp[kRaySamples];  // sample depths of the chunk
     __shared__ int2 s_rng[kRayThreads];  // per staged hit: [jlo, jhi) in chunk samples
     if (stats->overflow) return;
@@ -415,6 +509,12 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
     ray_origin(fr, x, ray_o, ray_d, ray_off, ro, rd);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const bool window_mode = F.Rw > 0.0f && !F.local_only;
+    float B0[3] = {0.f, 0.f, 0.f}, estep[3] = {0.f, 0.f, 0.f};  // grid: sample 0 and the step between samples
+    if constexpr (kGrid) {
+        pixel_position(fr, nullptr, x, 0, B0[0], B0[1], B0[2]);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) estep[k] = fr.delta_ax * fr.axis_ax[k];
+    }
 
     for (int c0 = 0; c0 < H; c0 += kRaySamples) {
         const int ns = min(kRaySamples, H - c0);
@@ -447,7 +547,22 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
                 RayGauss G;
                 float R[9];
                 ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
-                s_g[tid] = pack_rg(G);
+                if constexpr (kGrid) {
+                    RayLine Lh;
+                    Lh.tg = G.tg; Lh.rd = G.rd; Lh.tau = G.tau; Lh.f_final = G.f_final;
+                    Lh.ts = G.ts; Lh.b = G.b; Lh.tc = G.tc; Lh.I = G.I; Lh.dkf = G.dkf; Lh.unused = 0.f;
+                    Lh.a = dot3(G.dg[0], G.dg[1], G.dg[2], G.dg[0], G.dg[1], G.dg[2]);
+                    const float m0 = __builtin_fmaf(G.dg[0], G.tc, G.og[0]);
+                    const float m1 = __builtin_fmaf(G.dg[1], G.tc, G.og[1]);
+                    const float m2 = __builtin_fmaf(G.dg[2], G.tc, G.og[2]);
+                    Lh.qmin = dot3(m0, m1, m2, m0, m1, m2);
+                    const float S6[6] = {G.S00, G.S01, G.S02, G.S11, G.S12, G.S22};
+                    float A[3];
+                    ur_echo_line(G.mu, S6, B0, estep, A, Lh.Qc, Lh.c1x2, Lh.c2, Lh.J);
+                    s_l[tid] = pack_line(Lh);
+                } else {
+                    s_g[tid] = pack_rg(G);
+                }
                 if (fast) {
                     const int jt = first_true(0, ns, [&](int m) { return G.tg < s_tp[m]; });
                     const int jb = first_true(0, ns, [&](int m) { return s_tp[m] >= G.b; });
@@ -462,7 +577,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
                     const int h = jb0 + lane;
                     const bool hv = h < n;
                     const int2 rg = hv ? s_rng[h] : make_int2(ns, ns);
-                    const float ff = hv ? s_g[h].a.w : 1.0f;  // f_final
+                    const float ff = hv ? (kGrid ? s_l[h].p.w : s_g[h].a.w) : 1.0f;  // f_final
                     uint64_t bk[kPPT];
                     uint64_t any = 0;
 #pragma unroll
@@ -477,21 +592,42 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
                     while (any) {
                         const int bit = (int)__builtin_ctzll(any);
                         any &= any - 1;
-                        RayGauss G;
-                        unpack_rg(s_g[jb0 + bit], G);
+                        if constexpr (kGrid) {
+                            RayLine G;
+                            unpack_line(s_l[jb0 + bit], G);
 #pragma unroll
-                        for (int k = 0; k < kPPT; ++k) {
-                            if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
-                            ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
+                            for (int k = 0; k < kPPT; ++k) {
+                                if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
+                                ray_accumulate_line(G, F, (float)(c0 + tid + k * kRayThreads), tpx[k], T[k], num[k],
+                                                    den[k]);
+                            }
+                        } else {
+                            RayGauss G;
+                            unpack_rg(s_g[jb0 + bit], G);
+#pragma unroll
+                            for (int k = 0; k < kPPT; ++k) {
+                                if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
+                                ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
+                            }
                         }
                     }
                 }
             } else {
                 for (int j = 0; j < n; ++j) {
-                    RayGauss G;
-                    unpack_rg(s_g[j], G);
+                    if constexpr (kGrid) {
+                        RayLine G;
+                        unpack_line(s_l[j], G);
 #pragma unroll
-                    for (int k = 0; k < kPPT; ++k) ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
+                        for (int k = 0; k < kPPT; ++k)
+                            ray_accumulate_line(G, F, (float)(c0 + tid + k * kRayThreads), tpx[k], T[k], num[k],
+                                                den[k]);
+                    } else {
+                        RayGauss G;
+                        unpack_rg(s_g[j], G);
+#pragma unroll
+                        for (int k = 0; k < kPPT; ++k)
+                            ray_accumulate(G, F, px[k], py[k], pz[k], tpx[k], T[k], num[k], den[k]);
+                    }
                 }
             }
         }
@@ -594,29 +730,6 @@ __device__ __forceinline__ int wave_max_i(int v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = max(v, __shfl_xor(v, off, kWave));
     return v;
-}
-
-// Per hit: the 1-D echo quadratic along the grid scanline (see above) from the sample-0 position B and the step e;
-// returns A = d(J) and (Qc, 2 c1, c2, J).
-__device__ __forceinline__ void ur_echo_line(const float mu[3], const float S[6], const float B[3], const float e[3],
-                                             float A[3], float& Qc, float& c1x2, float& c2f, float& J) {
-    const double b0 = (double)B[0] - mu[0], b1 = (double)B[1] - mu[1], b2 = (double)B[2] - mu[2];
-    const double S00 = S[0], S01 = S[1], S02 = S[2], S11 = S[3], S12 = S[4], S22 = S[5];
-    const double e0 = e[0], e1 = e[1], e2 = e[2];
-    const double se0 = S00 * e0 + S01 * e1 + S02 * e2, se1 = S01 * e0 + S11 * e1 + S12 * e2,
-                 se2 = S02 * e0 + S12 * e1 + S22 * e2;
-    const double c2 = e0 * se0 + e1 * se1 + e2 * se2;
-    const double c1 = b0 * se0 + b1 * se1 + b2 * se2;
-    double j = c2 > 0.0 ? -c1 / c2 : 0.0;
-    j = fmin(fmax(j, -1.0e6), 1.0e6);
-    const double a0 = b0 + j * e0, a1 = b1 + j * e1, a2 = b2 + j * e2;
-    const double q = a0 * (S00 * a0 + S01 * a1 + S02 * a2) + a1 * (S01 * a0 + S11 * a1 + S12 * a2) +
-                     a2 * (S02 * a0 + S12 * a1 + S22 * a2);
-    A[0] = (float)a0; A[1] = (float)a1; A[2] = (float)a2;
-    Qc = (float)fmax(q, 0.0);
-    c1x2 = (float)(2.0 * (c1 + c2 * j));
-    c2f = (float)c2;
-    J = (float)j;
 }
 
 // Three waves per SIMD (at most 168 VGPRs): the echo-item phase would otherwise take the grid variant to 175 and two.
@@ -1198,7 +1311,9 @@ int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray
     if (!image || !aux) return fail(US3D_ERR_ARGUMENT, "ur_forward: NULL array");
     {
         StageTimer timer_("ur_forward_kernel", as_stream(stream_));
-        hipLaunchKernelGGL(ur_forward_kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
+        // the line forms when the samples are the linear probe's grid (see ur_forward_kernel)
+        auto kernel = pixels ? ur_forward_kernel<false> : ur_forward_kernel<true>;
+        hipLaunchKernelGGL(kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
                            ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
                            plan->frames, plan->frame_tile_base, plan->n_frames, pixels, ray_origins, ray_dirs, ray_offsets,
                            flags_of(settings), ws_at<us3d_stats>(plan, L.stats), image, reinterpret_cast<float4*>(aux));
