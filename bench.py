@@ -533,6 +533,8 @@ def bench_slice(args):
             k: round(st_[k][1] / max(st_[k][0], 1), 3) for k in ("ug_forward_kernel", "ug_backward_kernel") if k in st_}
 
     fps_p = None
+    plain_kernel_ms = None
+    batched_kernel_ms = sum(v[1] for v in stages.values()) / args.timing_steps
     if not args.no_plain:
         opt = torch.optim.Adam([means, tril, raw_o, raw_c], lr=1e-3)
 
@@ -547,6 +549,10 @@ def bench_slice(args):
         dt_p = timed(step_plain, args.plain_steps, 1)
         fps_p = F * args.plain_steps / dt_p
         log(f"slice plain: {dt_p / args.plain_steps * 1e3:.1f} ms/step, {fps_p:.1f} frames/s")
+        # the library's kernel time inside one plain step (HIP events around its launches), so the plain/batched
+        # ratio can be split into kernel time and the per-frame host work (tables, workspace, autograd, torch Adam)
+        st_p = stage_timing(step_plain)
+        plain_kernel_ms = sum(v[1] for v in st_p.values())
 
     launch_bytes = {
         # algorithmic HBM bytes per launch over the batch: the per-frame terms 96 G + 80 G_vis + 96 K + 24 P of
@@ -576,6 +582,9 @@ def bench_slice(args):
                    "frames": F, "gaussians": G, "tile_size": ts, "instances": K, "visible_pairs": g_vis,
                    "work_queue": bool(cfg.work_queue)},
         "plain_frames_per_s": fps_p, "batched_over_plain": (fps_b / fps_p) if fps_p else None,
+        "kernel_ms_per_step": {"batched": batched_kernel_ms, "plain": plain_kernel_ms,
+                               "plain_host_ms": (dt_p / args.plain_steps * 1e3 - plain_kernel_ms)
+                               if plain_kernel_ms is not None else None},
         "iters_per_s": args.steps / dt_b, "peak_hbm_bytes": int(peak), "workspace_bytes": int(ws_bytes),
         "roofline": roof, "splat_ms_per_launch_by_launch_mode": modes,
     }

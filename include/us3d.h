@@ -300,7 +300,9 @@ typedef struct us3d_gp_settings {
                                 (pipeline_god_file.py:360-370); the backward then returns the gradients w.r.t.
                                 those raw values. 0: both are used as given. Quaternions are always normalised
                                 by the kernels. Pass the same value to the map, forward and backward. */
-    int32_t reserved[2];
+    int32_t work_queue; /* 0: one workgroup per (frame, tile); 1: persistent workgroups over per-XCD tile queues (as
+                           us3d_ug_settings.work_queue; the outputs do not depend on it) */
+    int32_t reserved;
 } us3d_gp_settings;
 
 /* Workspace for a batch (same plan structure as UltraGauss: tiles of tile_size px per frame). */

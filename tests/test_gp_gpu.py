@@ -160,6 +160,30 @@ def test_gp_batch_equals_per_frame_and_is_deterministic(dev):
     assert float(batched.abs().max()) > 0.05
 
 
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_gp_work_queue_equals_one_workgroup_per_tile(dev, ts):
+    """work_queue: persistent workgroups take the (frame, tile) items from per-XCD queues. 8 frames of 512x512 give
+    2k-33k tiles, several per resident workgroup, many empty; the image and every gradient equal the
+    one-workgroup-per-tile launch's bit for bit."""
+    from dataclasses import replace
+
+    frames = _frames(8, H=512, W=512, seed=11)
+    params = _scene(frames[0], G=3000, seed=12)
+    cfg = _cfg(frames[0], ts=ts)
+    dI = torch.from_numpy(np.random.default_rng(13).normal(size=8 * 512 * 512).astype(np.float32)).to(dev)
+    outs = []
+    for wq in (False, True):
+        tens = [p.to(dev).clone().requires_grad_(True) for p in params]
+        img = render_frames(tens[0], tens[1], tens[2], tens[3], frames, replace(cfg, work_queue=wq),
+                            intensity=tens[4])
+        (img * dI).sum().backward()
+        outs.append((img.detach(), [t.grad.detach() for t in tens]))
+    assert float(outs[0][0].abs().max()) > 0.05
+    assert torch.equal(outs[0][0], outs[1][0])
+    for name, a, b in zip(("means", "scales", "quats", "opacity", "intensity"), outs[0][1], outs[1][1]):
+        assert torch.equal(a, b), f"d{name} differs"
+
+
 def test_gp_facade_backends(dev):
     fr = _frames(1, seed=7)[0]
     params = _scene(fr, G=300, seed=8)

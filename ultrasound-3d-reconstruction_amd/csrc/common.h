@@ -241,6 +241,10 @@ int check_plan(const us3d_plan* plan, int64_t ws_needed);
 // (instance total into stats, overflow flag), scans the tile counts into offsets, emits the instances into tile
 // buckets (seg_val = the instance's visible pair) and sorts every bucket by (depth key, Gaussian).
 int begin_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+
+// Grid of a splat launch over the plan's tiles: n_tiles workgroups, or with per-XCD work queues (heads != NULL) the
+// workgroups the device holds at once, after a fill kernel has cleared the queue heads on `stream`.
+int tile_grid(const void* kernel, int threads, const us3d_plan* plan, int32_t* heads, hipStream_t stream);
 int rank_visible_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
 int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
 

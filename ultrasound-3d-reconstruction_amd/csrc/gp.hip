@@ -309,62 +309,72 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
     const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min,
-    const us3d_stats* __restrict__ stats, float* __restrict__ image) {
+    const us3d_stats* __restrict__ stats, float* __restrict__ image, int n_tiles, int32_t* __restrict__ queue_heads) {
     using S = GpFwd<TS>;
     constexpr int NT = S::kThreads;
     constexpr int NP = S::kPix;
     __shared__ float4 s_f0[NT], s_f1[NT];
     __shared__ float s_I[NT];
+    __shared__ int s_tile;
     if (stats->overflow) return;
-    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
-    const int f = upper_index(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int tid = threadIdx.x;
-    const int ix = tx * TS + tid % TS, iy0 = ty * TS + tid / TS;
-    const float xl = ((float)ix - fr.col_offset) * fr.delta_lat;
-    float xa[NP], acc[NP];
+    auto item = [&](int t) {
+        const int f = upper_index(frame_tile_base, n_frames, t);
+        const us3d_frame& fr = frames[f];
+        const int lt = t - fr.tile_base;
+        const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+        const int tid = threadIdx.x;
+        const int ix = tx * TS + tid % TS, iy0 = ty * TS + tid / TS;
+        const float xl = ((float)ix - fr.col_offset) * fr.delta_lat;
+        float xa[NP], acc[NP];
 #pragma unroll
-    for (int r = 0; r < NP; ++r) {
-        xa[r] = (float)(iy0 + r * S::kRowStep) * fr.delta_ax + fr.near_mm;
-        acc[r] = 0.f;
-    }
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    for (int base = s0; base < s1; base += NT) {
-        const int n = min(NT, s1 - base);
-        __syncthreads();
-        if (tid < n) {
-            const int g = (int)(uint32_t)seg_key[base + tid];
-            const int64_t p = seg_val[base + tid];  // the instance's visible pair
-            s_f0[tid] = pair_foot[2 * p + 0];
-            s_f1[tid] = pair_foot[2 * p + 1];
-            s_I[tid] = rec[3 * (int64_t)g + 1].w;
+        for (int r = 0; r < NP; ++r) {
+            xa[r] = (float)(iy0 + r * S::kRowStep) * fr.delta_ax + fr.near_mm;
+            acc[r] = 0.f;
         }
-        __syncthreads();
-        for (int k = 0; k < n; ++k) {
-            const float4 a = s_f0[k], c = s_f1[k];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
-            const float I = s_I[k];
-            const float dl = xl - a.x;
-            const bool lat_in = fabsf(dl) <= c.y;
+        const int s0 = tile_off[t], s1 = tile_off[t + 1];
+        for (int base = s0; base < s1; base += NT) {
+            const int n = min(NT, s1 - base);
+            __syncthreads();
+            if (tid < n) {
+                const int g = (int)(uint32_t)seg_key[base + tid];
+                const int64_t p = seg_val[base + tid];  // the instance's visible pair
+                s_f0[tid] = pair_foot[2 * p + 0];
+                s_f1[tid] = pair_foot[2 * p + 1];
+                s_I[tid] = rec[3 * (int64_t)g + 1].w;
+            }
+            __syncthreads();
+            for (int k = 0; k < n; ++k) {
+                const float4 a = s_f0[k], c = s_f1[k];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
+                const float I = s_I[k];
+                const float dl = xl - a.x;
+                const bool lat_in = fabsf(dl) <= c.y;
 #pragma unroll
-            for (int r = 0; r < NP; ++r) {
-                const float da = xa[r] - a.y;
-                const float l = __builtin_fmaf(a.z, dl, a.w * da);
-                const float q = __builtin_fmaf(a.w, dl, c.x * da);
-                const float e = __builtin_fmaf(dl, l, da * q);
-                const float wgt = c.w * __builtin_amdgcn_exp2f(e);
-                // box, quad >= 0 (a NaN fails it), weight >= weight_min (an infinite quad gives weight 0)
-                const bool in = lat_in && fabsf(da) <= c.z && e <= 0.f && wgt >= weight_min;
-                acc[r] += in ? I * wgt : 0.f;
+                for (int r = 0; r < NP; ++r) {
+                    const float da = xa[r] - a.y;
+                    const float l = __builtin_fmaf(a.z, dl, a.w * da);
+                    const float q = __builtin_fmaf(a.w, dl, c.x * da);
+                    const float e = __builtin_fmaf(dl, l, da * q);
+                    const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+                    // box, quad >= 0 (a NaN fails it), weight >= weight_min (an infinite quad gives weight 0)
+                    const bool in = lat_in && fabsf(da) <= c.z && e <= 0.f && wgt >= weight_min;
+                    acc[r] += in ? I * wgt : 0.f;
+                }
             }
         }
-    }
 #pragma unroll
-    for (int r = 0; r < NP; ++r) {
-        const int iy = iy0 + r * S::kRowStep;
-        if (ix < fr.width && iy < fr.height) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc[r];
+        for (int r = 0; r < NP; ++r) {
+            const int iy = iy0 + r * S::kRowStep;
+            if (ix < fr.width && iy < fr.height) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc[r];
+        }
+    };
+    if (!queue_heads) {  // one workgroup per tile
+        item(remap_xcd(blockIdx.x, (int)gridDim.x));
+        return;
     }
+    int q = blockIdx.x % kXcds;  // persistent workgroups over the per-XCD tile queues
+    for (int t = next_queue_tile(queue_heads, q, n_tiles, &s_tile); t >= 0;
+         t = next_queue_tile(queue_heads, q, n_tiles, &s_tile))
+        item(t);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -380,97 +390,108 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off,
     const int4* __restrict__ pair_rect, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min, const float* __restrict__ d_image,
-    const us3d_stats* __restrict__ stats, float4* __restrict__ partial) {
+    const us3d_stats* __restrict__ stats, float4* __restrict__ partial, int n_tiles,
+    int32_t* __restrict__ queue_heads) {
     constexpr int NT = TS * TS;
     constexpr int HALF = TS / 2;
     static_assert(TS % 4 == 0, "rows of the staged gradient are read as float4");
     __shared__ __align__(16) float s_g[NT];  // dL/dimage of the tile's pixels
     __shared__ __align__(16) float s_xl[TS];  // lateral position of each column
+    __shared__ int s_tile;
     if (stats->overflow) return;
-    const int t = remap_xcd(blockIdx.x, (int)gridDim.x);
-    const int s0 = tile_off[t], s1 = tile_off[t + 1];
-    if (s0 == s1) return;
-    const int f = upper_index(frame_tile_base, n_frames, t);
-    const us3d_frame& fr = frames[f];
-    const int lt = t - fr.tile_base;
-    const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
-    const int tid = threadIdx.x;
-    const int cx0 = tx * TS, cy0 = ty * TS;
-    {
-        const int ix = cx0 + tid % TS, iy = cy0 + tid / TS;
-        s_g[tid] = (ix < fr.width && iy < fr.height) ? d_image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix]
-                                                     : 0.f;
-        if (tid < TS) s_xl[tid] = ((float)(cx0 + tid) - fr.col_offset) * fr.delta_lat;
-    }
-    __syncthreads();
-    const int n_items = 2 * (s1 - s0);
-    // NT is even, so both items of an instance run in the same iteration, on lanes tid and tid ^ 1
-    for (int it = tid; it < n_items; it += NT) {
-        const int i = s0 + (it >> 1);
-        const int half = it & 1;
-        const int g = (int)(uint32_t)seg_key[i];
-        const int pv = seg_val[i];  // the instance's visible pair
-        const float4 a = pair_foot[2 * (int64_t)pv + 0], c = pair_foot[2 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
-        // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
-        // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
-        // da is constant along a row, so per row only sum t, sum t dl and sum t dl^2 are accumulated per pixel;
-        // the sums over da (and the pre-scaled half-gradients l, q, which are linear in dl and da) follow
-        // from them once per row
-        float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sD = 0.f, sE = 0.f;  // sD = sum t dl, sE = sum t da
+    auto item = [&](int t) {
+        const int s0 = tile_off[t], s1 = tile_off[t + 1];
+        if (s0 == s1) return;
+        const int f = upper_index(frame_tile_base, n_frames, t);
+        const us3d_frame& fr = frames[f];
+        const int lt = t - fr.tile_base;
+        const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
+        const int tid = threadIdx.x;
+        const int cx0 = tx * TS, cy0 = ty * TS;
+        {
+            const int ix = cx0 + tid % TS, iy = cy0 + tid / TS;
+            s_g[tid] = (ix < fr.width && iy < fr.height) ? d_image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix]
+                                                         : 0.f;
+            if (tid < TS) s_xl[tid] = ((float)(cx0 + tid) - fr.col_offset) * fr.delta_lat;
+        }
+        __syncthreads();
+        const int n_items = 2 * (s1 - s0);
+        // NT is even, so both items of an instance run in the same iteration, on lanes tid and tid ^ 1
+        for (int it = tid; it < n_items; it += NT) {
+            const int i = s0 + (it >> 1);
+            const int half = it & 1;
+            const int g = (int)(uint32_t)seg_key[i];
+            const int pv = seg_val[i];  // the instance's visible pair
+            const float4 a = pair_foot[2 * (int64_t)pv + 0], c = pair_foot[2 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
+            // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
+            // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
+            // da is constant along a row, so per row only sum t, sum t dl and sum t dl^2 are accumulated per pixel;
+            // the sums over da (and the pre-scaled half-gradients l, q, which are linear in dl and da) follow
+            // from them once per row
+            float sT = 0.f, sA = 0.f, sB = 0.f, sC = 0.f, sD = 0.f, sE = 0.f;  // sD = sum t dl, sE = sum t da
 #pragma unroll 1
-        for (int r = 0; r < HALF; ++r) {
-            const int yy = half * HALF + r;
-            const float xa_ = (float)(cy0 + yy) * fr.delta_ax + fr.near_mm;
-            const float da = xa_ - a.y;
-            const bool row_in = fabsf(da) <= c.z;
-            const float c01da = a.w * da, c11da = c.x * da;
-            const float4* grow = reinterpret_cast<const float4*>(s_g + yy * TS);
-            float r0 = 0.f, r1 = 0.f, r2 = 0.f;
+            for (int r = 0; r < HALF; ++r) {
+                const int yy = half * HALF + r;
+                const float xa_ = (float)(cy0 + yy) * fr.delta_ax + fr.near_mm;
+                const float da = xa_ - a.y;
+                const bool row_in = fabsf(da) <= c.z;
+                const float c01da = a.w * da, c11da = c.x * da;
+                const float4* grow = reinterpret_cast<const float4*>(s_g + yy * TS);
+                float r0 = 0.f, r1 = 0.f, r2 = 0.f;
 #pragma unroll 2
-            for (int x4 = 0; x4 < TS / 4; ++x4) {
-                const float4 go4 = grow[x4];
-                const float4 xl4 = reinterpret_cast<const float4*>(s_xl)[x4];
-                const float gov[4] = {go4.x, go4.y, go4.z, go4.w};
-                const float xlv[4] = {xl4.x, xl4.y, xl4.z, xl4.w};
+                for (int x4 = 0; x4 < TS / 4; ++x4) {
+                    const float4 go4 = grow[x4];
+                    const float4 xl4 = reinterpret_cast<const float4*>(s_xl)[x4];
+                    const float gov[4] = {go4.x, go4.y, go4.z, go4.w};
+                    const float xlv[4] = {xl4.x, xl4.y, xl4.z, xl4.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const float dl = xlv[j] - a.x;
-                    const float l = __builtin_fmaf(a.z, dl, c01da);
-                    const float q = __builtin_fmaf(a.w, dl, c11da);
-                    const float e = __builtin_fmaf(dl, l, da * q);
-                    const float wgt = c.w * __builtin_amdgcn_exp2f(e);
-                    const bool in = row_in && fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
-                    const float tw = in ? gov[j] * wgt : 0.f;
-                    r0 += tw;
-                    const float td = tw * dl;
-                    r1 += td;
-                    r2 = __builtin_fmaf(td, dl, r2);
+                    for (int j = 0; j < 4; ++j) {
+                        const float dl = xlv[j] - a.x;
+                        const float l = __builtin_fmaf(a.z, dl, c01da);
+                        const float q = __builtin_fmaf(a.w, dl, c11da);
+                        const float e = __builtin_fmaf(dl, l, da * q);
+                        const float wgt = c.w * __builtin_amdgcn_exp2f(e);
+                        const bool in = row_in && fabsf(dl) <= c.y && e <= 0.f && wgt >= weight_min;
+                        const float tw = in ? gov[j] * wgt : 0.f;
+                        r0 += tw;
+                        const float td = tw * dl;
+                        r1 += td;
+                        r2 = __builtin_fmaf(td, dl, r2);
+                    }
                 }
+                sT += r0;
+                sA += r2;
+                sD += r1;
+                sE = __builtin_fmaf(da, r0, sE);
+                sB = __builtin_fmaf(da, r1, sB);
+                sC = __builtin_fmaf(da * da, r0, sC);
             }
-            sT += r0;
-            sA += r2;
-            sD += r1;
-            sE = __builtin_fmaf(da, r0, sE);
-            sB = __builtin_fmaf(da, r1, sB);
-            sC = __builtin_fmaf(da * da, r0, sC);
+            sT += __shfl_xor(sT, 1, kWave);
+            sA += __shfl_xor(sA, 1, kWave);
+            sB += __shfl_xor(sB, 1, kWave);
+            sC += __shfl_xor(sC, 1, kWave);
+            sD += __shfl_xor(sD, 1, kWave);
+            sE += __shfl_xor(sE, 1, kWave);
+            // sum t l = C i00 sum t dl + C i01 sum t da, sum t q = C i01 sum t dl + C i11 sum t da
+            const float sP = __builtin_fmaf(a.z, sD, a.w * sE);
+            const float sQ = __builtin_fmaf(a.w, sD, c.x * sE);
+            if (half == 0) {
+                const float I = rec[3 * (int64_t)g + 1].w;
+                const float mI = -2.f * kGpLn2 * I;
+                const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
+                partial[2 * ei + 0] = make_float4(I * sT / c.w, sT, mI * sP, mI * sQ);
+                partial[2 * ei + 1] = make_float4(-0.5f * I * sA, -I * sB, -0.5f * I * sC, 0.f);
+            }
         }
-        sT += __shfl_xor(sT, 1, kWave);
-        sA += __shfl_xor(sA, 1, kWave);
-        sB += __shfl_xor(sB, 1, kWave);
-        sC += __shfl_xor(sC, 1, kWave);
-        sD += __shfl_xor(sD, 1, kWave);
-        sE += __shfl_xor(sE, 1, kWave);
-        // sum t l = C i00 sum t dl + C i01 sum t da, sum t q = C i01 sum t dl + C i11 sum t da
-        const float sP = __builtin_fmaf(a.z, sD, a.w * sE);
-        const float sQ = __builtin_fmaf(a.w, sD, c.x * sE);
-        if (half == 0) {
-            const float I = rec[3 * (int64_t)g + 1].w;
-            const float mI = -2.f * kGpLn2 * I;
-            const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
-            partial[2 * ei + 0] = make_float4(I * sT / c.w, sT, mI * sP, mI * sQ);
-            partial[2 * ei + 1] = make_float4(-0.5f * I * sA, -I * sB, -0.5f * I * sC, 0.f);
-        }
+    };
+    if (!queue_heads) {  // one workgroup per tile
+        item(remap_xcd(blockIdx.x, (int)gridDim.x));
+        return;
     }
+    int q = blockIdx.x % kXcds;  // persistent workgroups over the per-XCD tile queues
+    for (int t = next_queue_tile(queue_heads, q, n_tiles, &s_tile); t >= 0;
+         t = next_queue_tile(queue_heads, q, n_tiles, &s_tile))
+        item(t);
 }
 
 // Vector-Jacobian product of the footprint: record-space gradients of one pair -> parameter gradients.
@@ -709,6 +730,7 @@ static int gp_check_settings(const us3d_gp_settings* s) {
     // the reference floors it at 1e-12 (max(threshold, 1e-12)); at 0 a listed pair whose base weight underflowed
     // to 0 would make the adjoint divide 0 by 0
     if (!(s->weight_min > 0.0f)) return fail(US3D_ERR_ARGUMENT, "weight_min must be > 0");
+    if (!(s->work_queue == 0 || s->work_queue == 1)) return fail(US3D_ERR_ARGUMENT, "work_queue must be 0 or 1");
     return US3D_OK;
 }
 
@@ -778,13 +800,16 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
     if (!image) return fail(US3D_ERR_ARGUMENT, "forward: image is NULL");
     if (plan->n_tiles <= 0) return US3D_OK;
     hipStream_t stream = as_stream(stream_);
+    int32_t* heads = settings->work_queue ? gp_at<int32_t>(plan, L.queue) : nullptr;
 #define GP_FWD(TS)                                                                                                    \
-    hipLaunchKernelGGL((gp_forward_kernel<TS>), dim3(plan->n_tiles), dim3(GpFwd<TS>::kThreads), 0, stream,            \
+    hipLaunchKernelGGL((gp_forward_kernel<TS>),                                                                       \
+                       dim3(tile_grid((const void*)gp_forward_kernel<TS>, GpFwd<TS>::kThreads, plan, heads, stream)), \
+                       dim3(GpFwd<TS>::kThreads), 0, stream,                                                          \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
                        gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
                        gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames,         \
                        settings->weight_min,                                                                          \
-                       gp_at<us3d_stats>(plan, L.stats), image)
+                       gp_at<us3d_stats>(plan, L.stats), image, plan->n_tiles, heads)
     {
         StageTimer timer_("gp_forward_kernel", stream);
         switch (settings->tile_size) {
@@ -808,14 +833,17 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     if (plan->n_tiles > 0) {
+        int32_t* heads = settings->work_queue ? gp_at<int32_t>(plan, L.queue) : nullptr;
 #define GP_BWD(TS)                                                                                                    \
-    hipLaunchKernelGGL((gp_backward_kernel<TS>), dim3(plan->n_tiles), dim3(TS * TS), 0, stream,                       \
+    hipLaunchKernelGGL((gp_backward_kernel<TS>),                                                                      \
+                       dim3(tile_grid((const void*)gp_backward_kernel<TS>, TS * TS, plan, heads, stream)),            \
+                       dim3(TS * TS), 0, stream,                                                                      \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
                        gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
                        gp_at<int32_t>(plan, L.seg_val), gp_at<int32_t>(plan, L.pair_off),                            \
                        gp_at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames,          \
                        settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),                               \
-                       gp_at<float4>(plan, L.partial))
+                       gp_at<float4>(plan, L.partial), plan->n_tiles, heads)
         {
             StageTimer timer_("gp_backward_kernel", stream);
             switch (settings->tile_size) {

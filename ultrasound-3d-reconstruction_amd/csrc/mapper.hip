@@ -20,6 +20,9 @@
 // Every rank is a scan over fixed positions, so the lists are deterministic; the workspace holds 0.625 B per pair for
 // the masks and their scans, and the records of at most v_capacity visible pairs.
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "common.h"
 
@@ -253,6 +256,34 @@ __global__ __launch_bounds__(kEmitThreads) void emit_instances_kernel(
 }
 
 // ---------------------------------------------------------------- host orchestration
+
+// Workgroups of a splat launch: one per tile, or, with the per-XCD work queues, as many as the device holds at once
+// (occupancy of the kernel times the CU count, cached per kernel and device; never more than the tiles). With the
+// queues their heads are cleared first, by a kernel, so a captured graph replays correctly.
+int tile_grid(const void* kernel, int threads, const us3d_plan* plan, int32_t* heads, hipStream_t stream) {
+    if (!heads) return plan->n_tiles;
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    int groups = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find({kernel, dev});
+        if (it != cache.end()) {
+            groups = it->second;
+        } else {
+            int per_cu = 0, cus = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) per_cu = 1;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 1;
+            groups = std::max(1, per_cu) * std::max(1, cus);
+            cache[{kernel, dev}] = groups;
+        }
+    }
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, stream, heads, (int64_t)kXcds * kQueueStride, 0);
+    return std::min(groups, plan->n_tiles);
+}
+
 
 int begin_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
     hipLaunchKernelGGL(init_stats_kernel, dim3(1), dim3(1), 0, stream, ws_at<us3d_stats>(plan, L.stats));

@@ -12,9 +12,7 @@
 //   partial   f32x12 per instance, written at e by the adjoint, so each Gaussian's partials across all
 //             frames are contiguous and reduce in a fixed order.
 #include <algorithm>
-#include <map>
 #include <mutex>
-#include <utility>
 
 #include "common.h"
 
@@ -1234,33 +1232,6 @@ static int check_settings(const us3d_ug_settings* s) {
             return fail(US3D_ERR_ARGUMENT, "fwd_lds_batch and bwd_lds_batch must be 0, 64, 128 or 256");
     if (!(s->work_queue == 0 || s->work_queue == 1)) return fail(US3D_ERR_ARGUMENT, "work_queue must be 0 or 1");
     return US3D_OK;
-}
-
-// Workgroups of a splat launch: one per tile, or, with the per-XCD work queues, as many as the device holds at once
-// (occupancy of the kernel times the CU count, cached per kernel and device; never more than the tiles). With the
-// queues their heads are cleared first, by a kernel, so a captured graph replays correctly.
-static int tile_grid(const void* kernel, int threads, const us3d_plan* plan, int32_t* heads, hipStream_t stream) {
-    if (!heads) return plan->n_tiles;
-    static std::mutex mu;
-    static std::map<std::pair<const void*, int>, int> cache;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    int groups = 0;
-    {
-        std::lock_guard<std::mutex> lock(mu);
-        auto it = cache.find({kernel, dev});
-        if (it != cache.end()) {
-            groups = it->second;
-        } else {
-            int per_cu = 0, cus = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess) per_cu = 1;
-            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 1;
-            groups = std::max(1, per_cu) * std::max(1, cus);
-            cache[{kernel, dev}] = groups;
-        }
-    }
-    hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(256), 0, stream, heads, (int64_t)kXcds * kQueueStride, 0);
-    return std::min(groups, plan->n_tiles);
 }
 
 // Gaussians a splat kernel stages in LDS per batch: the settings' choice, or the kernel's thread count.

@@ -123,7 +123,8 @@ class GPSettings(ctypes.Structure):
         ("weight_min", ctypes.c_float),
         ("tile_size", ctypes.c_int32),
         ("raw_activations", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("work_queue", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

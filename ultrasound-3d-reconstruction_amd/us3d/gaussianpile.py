@@ -28,7 +28,8 @@ from .batch import CapacityHint, DeviceTables, FrameSpec, Mapped, SweepSpec, new
 
 @dataclass
 class GaussianPlieRenderConfig:
-    """ops/gaussianplie_renderer/gaussianplie_renderer/types.py:5-48."""
+    """ops/gaussianplie_renderer/gaussianplie_renderer/types.py:5-48 (+ work_queue: the forward and the adjoint run as
+    persistent workgroups over per-XCD (frame, tile) queues instead of one workgroup per tile; same outputs)."""
 
     H: int
     W: int
@@ -46,6 +47,7 @@ class GaussianPlieRenderConfig:
     render_chunk_size: int = 256
     enable_input_checks: bool = False
     lateral_col_offset: Optional[float] = None
+    work_queue: bool = True
 
     def __post_init__(self) -> None:
         if self.H <= 0 or self.W <= 0:
@@ -83,6 +85,7 @@ class GaussianPlieRenderConfig:
         s.det_eps = float(self.det_eps)
         s.weight_min = max(float(self.opacity_cull_threshold), 1.0e-12)
         s.tile_size = int(self.tile_size_px)
+        s.work_queue = 1 if self.work_queue else 0
         return s
 
 
