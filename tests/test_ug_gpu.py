@@ -155,7 +155,7 @@ def test_grid_pixels_plane_form_parity(dev, ts):
         assert vis.sum() > 100 and int(m.stats.n_visible) == int(vis.sum())
         # one frame: the visible pairs are its Gaussians with a tile, in index order
         V = int(vis.sum())
-        sect = m.workspace[off.pair_sect: off.pair_sect + 48 * V].view(torch.float32).view(V, 12).cpu().numpy()
+        sect = m.workspace[off.pair_sect: off.pair_sect + 64 * V].view(torch.float32).view(V, 16).cpu().numpy()
         got_plane = np.stack([sect[:, 8], sect[:, 9], sect[:, 10], sect[:, 4], sect[:, 5], sect[:, 6]], 1)
         np.testing.assert_array_equal(got_plane.view(np.uint32), plane[vis].view(np.uint32))
         args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom,
@@ -582,6 +582,9 @@ def test_emit_split_modes_bitexact(dev, n_frames):
     assert np.all((tx >= r[:, 0]) & (tx < r[:, 2]) & (ty >= r[:, 1]) & (ty < r[:, 3]))
     e = pair_off[val] + (ty - r[:, 1]) * (r[:, 2] - r[:, 0]) + (tx - r[:, 0])
     np.testing.assert_array_equal(np.sort(e), np.arange(K))
+    # each visible pair's record carries its emit base (first emit index, x0, y0, width) for the adjoint
+    base = ws[off.pair_sect: off.pair_sect + 64 * V].view(torch.int32).numpy().reshape(V, 16)[:, 12:16]
+    np.testing.assert_array_equal(base, np.stack([pair_off[:V], rect[:, 0], rect[:, 1], rect[:, 2] - rect[:, 0]], 1))
 
 
 def test_fan_grid_in_kernel_matches_explicit_positions(dev):

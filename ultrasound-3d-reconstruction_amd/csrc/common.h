@@ -246,7 +246,11 @@ int begin_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream)
 // workgroups the device holds at once, after a fill kernel has cleared the queue heads on `stream`.
 int tile_grid(const void* kernel, int threads, const us3d_plan* plan, int32_t* heads, hipStream_t stream);
 int rank_visible_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
-int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
+// sect_f4 / emit_slot: when sect_f4 > 0, each visible pair's record in pair_sect (sect_f4 float4s) gets its emit base
+// (first emit index, rectangle x0, y0 and width, as int bits) in float4 number emit_slot once the counts are scanned,
+// so an adjoint reads it from the line it reads the record from (emit_index_rec).
+int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream, int sect_f4 = 0,
+                    int emit_slot = 0);
 
 // First frame-major word of frame f of sweep s: each sweep before s takes f_count * ceil(g_count / 32) words.
 __device__ __forceinline__ int64_t fm_word_base(const us3d_sweep* __restrict__ sweeps, int s, int f) {
@@ -280,6 +284,12 @@ __device__ __forceinline__ void mark_pair(bool visible, int64_t pair, int64_t n_
         const int64_t fm = fm_word_base(sweeps, s, f) * 32 + gl;
         atomicOr(&fm_bits[fm >> 5], 1u << (fm & 31));
     }
+}
+
+// Emit index of an instance in tile (tx, ty) from its pair's emit base (first emit index, x0, y0, width as int bits).
+__device__ __forceinline__ int64_t emit_index_rec(float4 base, int tx, int ty) {
+    return (int64_t)__float_as_int(base.x) + (int64_t)(ty - __float_as_int(base.z)) * __float_as_int(base.w) +
+           (tx - __float_as_int(base.y));
 }
 
 // Emit index of the instance of visible pair v in tile (tx, ty) of its frame: the pair's first emit index plus the

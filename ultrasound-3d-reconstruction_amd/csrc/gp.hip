@@ -30,8 +30,12 @@
 
 namespace us3d {
 
+// Per visible pair, kFootF4 float4s in pair_sect: its footprint (two float4) and its emit base (first emit index,
+// rectangle x0, y0, width; written once the counts are scanned), read by the adjoint from the same record.
+constexpr int kFootF4 = 3;
+
 static us3d_layout gp_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 32, 0, 32);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 32, 0, 16 * kFootF4);
 }
 
 static us3d_layout gp_layout(const us3d_plan* plan) {
@@ -287,8 +291,8 @@ __global__ __launch_bounds__(256) void gp_setup_kernel(
     if (count > 0) {  // always, for a listed pair
         pair_rect[v] = rect;
         pair_dk[v] = 0u;
-        pair_foot[2 * v + 0] = make_float4(F.m0, F.m1, F.i00 * kGpExp2Scale, F.i01 * kGpExp2Scale);
-        pair_foot[2 * v + 1] = make_float4(F.i11 * kGpExp2Scale, F.r_lat, F.r_ax, F.b);
+        pair_foot[kFootF4 * v + 0] = make_float4(F.m0, F.m1, F.i00 * kGpExp2Scale, F.i01 * kGpExp2Scale);
+        pair_foot[kFootF4 * v + 1] = make_float4(F.i11 * kGpExp2Scale, F.r_lat, F.r_ax, F.b);
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, rect);
     }
 }
@@ -338,8 +342,8 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
             if (tid < n) {
                 const int g = (int)(uint32_t)seg_key[base + tid];
                 const int64_t p = seg_val[base + tid];  // the instance's visible pair
-                s_f0[tid] = pair_foot[2 * p + 0];
-                s_f1[tid] = pair_foot[2 * p + 1];
+                s_f0[tid] = pair_foot[kFootF4 * p + 0];
+                s_f1[tid] = pair_foot[kFootF4 * p + 1];
                 s_I[tid] = rec[3 * (int64_t)g + 1].w;
             }
             __syncthreads();
@@ -387,8 +391,7 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
 template <int TS>
 __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
-    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off,
-    const int4* __restrict__ pair_rect, const us3d_frame* __restrict__ frames,
+    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min, const float* __restrict__ d_image,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, int n_tiles,
     int32_t* __restrict__ queue_heads) {
@@ -422,7 +425,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
             const int half = it & 1;
             const int g = (int)(uint32_t)seg_key[i];
             const int pv = seg_val[i];  // the instance's visible pair
-            const float4 a = pair_foot[2 * (int64_t)pv + 0], c = pair_foot[2 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
+            const float4 a = pair_foot[kFootF4 * (int64_t)pv + 0], c = pair_foot[kFootF4 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
             // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
             // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
             // da is constant along a row, so per row only sum t, sum t dl and sum t dl^2 are accumulated per pixel;
@@ -478,7 +481,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
             if (half == 0) {
                 const float I = rec[3 * (int64_t)g + 1].w;
                 const float mI = -2.f * kGpLn2 * I;
-                const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
+                const int64_t ei = emit_index_rec(pair_foot[kFootF4 * (int64_t)pv + 2], tx, ty);
                 partial[2 * ei + 0] = make_float4(I * sT / c.w, sT, mI * sP, mI * sQ);
                 partial[2 * ei + 1] = make_float4(-0.5f * I * sA, -I * sB, -0.5f * I * sC, 0.f);
             }
@@ -789,7 +792,7 @@ int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, 
         }
         if ((rc = check_launch("gp_setup_kernel"))) return rc;
     }
-    return build_instances(plan, L, stream);
+    return build_instances(plan, L, stream, kFootF4, 2);
 }
 
 int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, void* stream_) {
@@ -840,8 +843,7 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
                        dim3(TS * TS), 0, stream,                                                                      \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
                        gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
-                       gp_at<int32_t>(plan, L.seg_val), gp_at<int32_t>(plan, L.pair_off),                            \
-                       gp_at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames,          \
+                       gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames,         \
                        settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),                               \
                        gp_at<float4>(plan, L.partial), plan->n_tiles, heads)
         {

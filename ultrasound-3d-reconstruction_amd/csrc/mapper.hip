@@ -130,6 +130,17 @@ __global__ void popcount_words_kernel(const uint32_t* __restrict__ bits, int64_t
         cnt[i] = __popc(bits[i]);
 }
 
+// Emit base of every visible pair into its record (see build_instances): first emit index, rectangle x0, y0, width.
+__global__ void emit_base_kernel(int64_t v_cap, const us3d_stats* __restrict__ stats, const int32_t* __restrict__ pair_off,
+                                 const int4* __restrict__ pair_rect, float4* __restrict__ sect, int sect_f4,
+                                 int emit_slot) {
+    const int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (v >= v_cap || stats->overflow || v >= stats->n_visible) return;
+    const int4 r = pair_rect[v];
+    sect[v * sect_f4 + emit_slot] = make_float4(__int_as_float(pair_off[v]), __int_as_float(r.x), __int_as_float(r.y),
+                                                __int_as_float(r.z - r.x));
+}
+
 // Thread per dense pair: a visible pair writes its dense index at its compact index v and v at its frame-major slot.
 __global__ __launch_bounds__(256) void list_visible_kernel(
     int64_t n_pairs, const us3d_sweep* __restrict__ sweeps, int n_sweeps, const uint64_t* __restrict__ pair_bits,
@@ -325,7 +336,7 @@ int rank_visible_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t 
     return check_launch("list_visible_kernel");
 }
 
-int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream) {
+int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream, int sect_f4, int emit_slot) {
     int rc;
     us3d_stats* stats = ws_at<us3d_stats>(plan, L.stats);
     int32_t* pair_off = ws_at<int32_t>(plan, L.pair_off);
@@ -345,6 +356,12 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
     if (np > 0) {
         hipLaunchKernelGGL(close_offsets_kernel, dim3(1), dim3(1), 0, stream, pair_off, vc, &stats->n_instances,
                            (const int32_t*)nullptr);
+        if (sect_f4 > 0) {
+            hipLaunchKernelGGL(emit_base_kernel, dim3(blocks_for(vc, 256)), dim3(256), 0, stream, vc,
+                               (const us3d_stats*)stats, (const int32_t*)pair_off, ws_at<int4>(plan, L.pair_rect),
+                               ws_at<float4>(plan, L.pair_sect), sect_f4, emit_slot);
+            if ((rc = check_launch("emit_base_kernel"))) return rc;
+        }
     } else {
         hipLaunchKernelGGL(fill_i32_kernel, dim3(blocks_for(vc + 1, 256)), dim3(256), 0, stream, pair_off, vc + 1, 0);
     }

@@ -19,9 +19,13 @@
 namespace us3d {
 
 // ---------------------------------------------------------------- workspace layout
+// Per visible pair, kSectF4 float4s in pair_sect: the pair's part of the sub-block test (s0), the plane form of q
+// (s1, s2; pair_section) and its emit base (s3: first emit index, rectangle x0, y0 and width, written once the counts
+// are scanned), so the forward (s0-s2) and the adjoint (s1-s3) each read one 64 B record.
+constexpr int kSectF4 = 4;
 
 static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
-    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 48, 2, 48);
+    return ws_layout(n_gauss, n_pairs, n_tiles, k_cap, v_cap, 48, 48, 2, 16 * kSectF4);
 }
 
 static us3d_layout ug_layout(const us3d_plan* plan) {
@@ -313,9 +317,9 @@ __global__ __launch_bounds__(256) void ug_setup_kernel(
         float4 s0, s1, s2;
         pair_section(r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r0.x, r0.y, r0.z, pt.lat, pt.ax, pt.r_lat, pt.r_ax, fr, s0,
                      s1, s2);
-        pair_sect[3 * v + 0] = s0;
-        pair_sect[3 * v + 1] = s1;
-        pair_sect[3 * v + 2] = s2;
+        pair_sect[kSectF4 * v + 0] = s0;
+        pair_sect[kSectF4 * v + 1] = s1;
+        pair_sect[kSectF4 * v + 2] = s2;
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, pt.rect);
     }
 }
@@ -534,12 +538,12 @@ __device__ __forceinline__ void ug_forward_tile(
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
             const int64_t pv = seg_val[base + j];  // the instance's visible pair
-            const float4 ps0 = pair_sect[3 * pv + 0], ps1 = pair_sect[3 * pv + 1];
+            const float4 ps0 = pair_sect[kSectF4 * pv + 0], ps1 = pair_sect[kSectF4 * pv + 1];
             const uint32_t m = sub_mask<S::kSub>(L.sb, ps0, ps1, lat_g, ax_g, e, chi2w);
             L.mask[j] = m;
             inst_mask[base + j] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             if constexpr (kPlaneBody) {
-                const float4 ps2 = pair_sect[3 * pv + 2];
+                const float4 ps2 = pair_sect[kSectF4 * pv + 2];
                 L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
                 L.r1[j] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
                 L.r2[j] = make_float4(e, 0.f, 0.f, 0.f);
@@ -781,8 +785,8 @@ struct BwdLds {
 template <int TS, bool kFull, bool kPlaneBody, bool kPlane, int kBatch>
 __device__ __forceinline__ void ug_backward_tile(
     BwdLds<TS, kPlane, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
-    const us3d_frame& fr, int tx, int ty, int s0, int s1, const float* __restrict__ pixels,
+    const int32_t* __restrict__ seg_val, const us3d_frame& fr, int tx, int ty, int s0, int s1,
+    const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
     const float* __restrict__ d_alpha, const float* __restrict__ d_depth, float4* __restrict__ partial,
@@ -811,7 +815,14 @@ __device__ __forceinline__ void ug_backward_tile(
         if (valid) {
             const int64_t p = (int64_t)fr.pixel_base + iy * fr.width + ix;
             if constexpr (!kPlaneBody) pixel_position(fr, pixels, ix, iy, x, y, z);
-            const float4 sm = aux[p];  // sum alpha_hat, unclamped image, sum alpha_hat * elev
+            // sum alpha_hat, unclamped image, sum alpha_hat * elev (the third only with alpha or depth gradients)
+            float4 sm;
+            if constexpr (kFull) {
+                sm = aux[p];
+            } else {
+                const float2 h = reinterpret_cast<const float2*>(aux)[2 * p];
+                sm = make_float4(h.x, h.y, 0.f, 0.f);
+            }
             const float A = sm.x + bg_alpha;
             const float inv = 1.0f / fmaxf(A, 1e-8f);
             const float gi = d_image[p];
@@ -853,14 +864,14 @@ __device__ __forceinline__ void ug_backward_tile(
                 const float4 q1 = rec[3 * (int64_t)g + 1];
                 const float4 q2 = rec[3 * (int64_t)g + 2];
                 const int pv = seg_val[base + j];  // the instance's visible pair
-                const int64_t ei = emit_index(pair_off, pair_rect, pv, tx, ty);
+                const int64_t ei = emit_index_rec(pair_sect[kSectF4 * (int64_t)pv + 3], tx, ty);
                 m[i] = inst_mask[base + j];  // the sub-block mask the forward left for this instance
                 L.g0[j] = q0;
                 L.g1[j] = q1;
                 L.g2[j] = q2;
                 L.ei[j] = (int32_t)ei;
                 if constexpr (kPlaneBody) {
-                    const float4 ps1 = pair_sect[3 * (int64_t)pv + 1], ps2 = pair_sect[3 * (int64_t)pv + 2];
+                    const float4 ps1 = pair_sect[kSectF4 * (int64_t)pv + 1], ps2 = pair_sect[kSectF4 * (int64_t)pv + 2];
                     L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
                     L.g4[j] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
                 }
@@ -1087,8 +1098,8 @@ __device__ __forceinline__ void ug_backward_tile(
 template <int TS, bool kFull, bool kPlane, int kBatch>
 __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
-    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
@@ -1105,11 +1116,11 @@ __global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
         const int s0 = tile_off[t], s1 = tile_off[t + 1];
         if (s0 == s1) return;  // workgroup-uniform
         if (kPlane && fr.geometry == 0)  // workgroup-uniform
-            ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx,
+            ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, tx,
                                                                 ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
                                                                 aux, d_image, d_alpha, d_depth, partial, inst_mask);
         else
-            ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, pair_off, pair_rect, fr, tx,
+            ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, tx,
                                                                ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
                                                                aux, d_image, d_alpha, d_depth, partial, inst_mask);
     };
@@ -1348,7 +1359,7 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
         }
         if ((rc = check_launch("ug_setup_kernel"))) return rc;
     }
-    return build_instances(plan, L, stream);
+    return build_instances(plan, L, stream, kSectF4, 3);
 }
 
 int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
@@ -1415,8 +1426,8 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                                       plan, heads, stream)),                                                          \
                        dim3(TileShape<TS>::kThreads), 0, stream, at<float4>(plan, L.records),                         \
                        at<int32_t>(plan, L.tile_off),                                                                 \
-                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), at<int32_t>(plan, L.pair_off),    \
-                       at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,      \
+                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), plan->frames,                     \
+                       plan->frame_tile_base, plan->n_frames, pixels,                                                 \
                        at<float4>(plan, L.pair_sect),                                                                 \
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
                        reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
