@@ -180,7 +180,7 @@ constexpr int kEmitPerThread = 8;
 constexpr int kEmitChunk = kEmitThreads * kEmitPerThread;
 constexpr int kEmitQueue = kEmitChunk + kEmitThreads;
 constexpr int kEmitLdsTiles = 8192;
-constexpr int kEmitTargetGroups = 256;
+constexpr int kEmitTargetGroups = 1024;
 
 __global__ __launch_bounds__(kEmitThreads) void emit_instances_kernel(
     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, const int32_t* __restrict__ fm_off,
