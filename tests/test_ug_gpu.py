@@ -742,3 +742,37 @@ def test_config5_fan_gradients_vs_fallback_on_64_frames(dev):
     for name, r in zip(errs, refs):
         assert np.abs(r).max() > 0, name
         assert errs[name] < GRAD_TOL, f"config 5 {name}: rel L2 {errs[name]:.3e}"
+
+
+def test_fallback_semantics_vs_reference_fallback_golden(dev):
+    """The reference's own fallback renderer output (tests/golden/fallback_ref_24x32_g384.npz, written by
+    _render_ultragauss_torch_fallback, pipeline_god_file.py:556-603, over 384 anisotropic Gaussians): the HIP
+    render with fallback semantics, from the explicit pixel array and from the frame's own grid, and the
+    gradients w.r.t. the raw parameters through the same sigmoids (pipeline_god_file.py:369-373)."""
+    from tests.test_cpu_fallback_golden import load_fallback_golden
+    from us3d.batch import FrameSpec
+    from us3d.ultragauss import render_frames
+
+    z, H, W = load_fallback_golden()
+    _h, _w, dlat, dax, near, col = (float(x) for x in z["frame"])
+    ax = torch.from_numpy(z["axes"])
+    fr = FrameSpec(origin=torch.from_numpy(z["origin"]), axis_lat=ax[0], axis_ax=ax[1], axis_ele=ax[2], height=H,
+                   width=W, delta_lat=dlat, delta_ax=dax, near_mm=near, col_offset=col)
+    cfg = cfg_for(fr, semantics="fallback", diag_beta=float(z["beta"]))
+    dI = torch.from_numpy(z["dI"]).to(dev)
+    dA = torch.from_numpy(z["dA"]).to(dev)
+    case = "UltraGauss fallback semantics vs reference fallback output"
+    for pixels in ("explicit", "grid"):
+        raw = [torch.from_numpy(z[k]).to(dev).requires_grad_(True)
+               for k in ("means", "raw_tril", "raw_opacity", "raw_intensity")]
+        pix = torch.from_numpy(z["pixels"]).to(dev) if pixels == "explicit" else None
+        image, alpha, _depth = render_frames(raw[0], raw[1], torch.sigmoid(raw[2]), torch.sigmoid(raw[3]), pix,
+                                             [fr], cfg, sweeps=[SweepSpec(0, raw[0].shape[0])], hint=CapacityHint())
+        ((image * dI).sum() + (alpha * dA).sum()).backward()
+        assert note(case, f"image ({pixels} pixels)", rel_l2(image.detach().cpu().numpy(), z["image"]),
+                    IMG_TOL) < IMG_TOL
+        assert note(case, f"alpha ({pixels} pixels)", rel_l2(alpha.detach().cpu().numpy(), z["alpha"]),
+                    IMG_TOL) < IMG_TOL
+        for r, k in zip(raw, ("d_means", "d_raw_tril", "d_raw_opacity", "d_raw_intensity")):
+            err = note(case, f"{k} ({pixels} pixels)", rel_l2(r.grad.cpu().numpy(), z[k]), GRAD_TOL)
+            assert err < GRAD_TOL, f"{k}: rel L2 {err:.3e}"
