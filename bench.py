@@ -76,6 +76,9 @@ def parse(argv=None):
     ap.add_argument("--rocprof-summary", nargs=2, metavar=("MODE", "DIR"),
                     help="print a markdown summary of a rocprofv3 output directory: MODE 'trace' (per-kernel "
                          "durations of --kernel-trace) or 'pmc' (per-dispatch means of --pmc counters), then exit")
+    ap.add_argument("--rocprof-report", nargs="+", metavar="ARG",
+                    help="KERNEL TRACE_DIR PMC_DIR [PMC_DIR ...]: print the body of the kernel's profile summary (its "
+                         "average duration in the trace, traffic, VALU, LDS figures and counter sums), then exit")
     ap.add_argument("--rocprof-derive", nargs="+", metavar="ARG",
                     help="KERNEL DIR [DIR ...]: print the derived figures of one kernel (traffic, LDS bytes, VALU "
                          "issue and active lanes) from rocprofv3 --pmc directories of the same command, then exit")
@@ -294,7 +297,7 @@ def rocprof_summary(mode: str, directory: str) -> None:
             print(f"| {n} | {c} | {len(v)} | {sum(v) / len(v):.1f} |")
 
 
-def rocprof_derive(kernel: str, dirs) -> None:
+def rocprof_derive(kernel: str, dirs, quiet: bool = False):
     """Per-dispatch figures of one kernel from --pmc runs of the same command. SQ counters are summed over their
     instances per dispatch (rows / dispatches instances each); dispatches = the rows of a GRBM_GUI_ACTIVE or
     FETCH_SIZE counter divided by its instances. Prints `key: value` lines that bench.py reads back:
@@ -341,8 +344,63 @@ def rocprof_derive(kernel: str, dirs) -> None:
         out["lds_bytes"] = 64 * (total("SQ_INSTS_LDS_LOAD_BANDWIDTH") + total("SQ_INSTS_LDS_STORE_BANDWIDTH"))
     if "SQ_LDS_BANK_CONFLICT" in per and "SQ_LDS_IDX_ACTIVE" in per:
         out["lds_bank_conflict_frac"] = total("SQ_LDS_BANK_CONFLICT") / max(total("SQ_LDS_IDX_ACTIVE"), 1.0)
-    for k, v in out.items():
-        print(f"{k}: {v:.6g}" if isinstance(v, float) else f"{k}: {v}")
+    if not quiet:
+        for k, v in out.items():
+            print(f"{k}: {v:.6g}" if isinstance(v, float) else f"{k}: {v}")
+    return out, {c: (len(v), sum(v) / dispatches) for c, v in per.items()}
+
+
+def rocprof_report(kernel: str, trace_dir: str, pmc_dirs) -> None:
+    """The body of a kernel's profile summary (profiles/<kernel>_<workload>_pmc.md): its average duration in the
+    kernel trace, the figures bench.py reads back (HBM traffic scaled by the 1 GiB copy calibration, VALU issue and
+    lanes, LDS bytes and bank conflicts) and the per-dispatch counter sums."""
+    import glob
+    import sqlite3
+
+    durs = []
+    for db in glob.glob(f"{trace_dir}/**/*.db", recursive=True):
+        con = sqlite3.connect(db)
+        durs += [t1 - t0 for n, t0, t1 in con.execute("select name, start, end from kernels") if _short(n) == kernel]
+    if not durs:
+        raise SystemExit(f"{kernel} is not in the kernel trace")
+    avg_ms = sum(durs) / len(durs) / 1e6
+    cal = (ROOT / "profiles" / "copy_1GiB_calibration_pmc.md").read_text()
+    m = re.search(r"FETCH_SIZE scale = ([0-9.]+), WRITE_SIZE scale = ([0-9.]+)", cal)
+    sf, sw = float(m.group(1)), float(m.group(2))
+    out, rows = rocprof_derive(kernel, pmc_dirs, quiet=True)
+    print(f"Per dispatch ({out['dispatches']} dispatches; kernel trace: {len(durs)} launches, average {avg_ms:.4f} ms):")
+    print()
+    fig = {}
+    if "fetch_kib" in out and "write_kib" in out:
+        fig["fetch_bytes_per_launch"] = out["fetch_kib"] * 1024 * sf
+        fig["write_bytes_per_launch"] = out["write_kib"] * 1024 * sw
+        fig["traffic_bytes_per_launch"] = fig["fetch_bytes_per_launch"] + fig["write_bytes_per_launch"]
+    fig["avg_launch_ms"] = avg_ms
+    for k in ("valu_issue_frac", "valu_active_threads", "lds_bytes", "lds_bank_conflict_frac"):
+        if k in out:
+            fig[k] = out[k]
+    if "lds_bytes" in out:
+        fig["lds_GBs"] = out["lds_bytes"] / (avg_ms * 1e-3) / 1e9
+    for k in ("traffic_bytes_per_launch", "fetch_bytes_per_launch", "write_bytes_per_launch", "avg_launch_ms",
+              "valu_issue_frac", "valu_active_threads", "lds_bytes", "lds_GBs", "lds_bank_conflict_frac"):
+        if k in fig:
+            print(f"{k}: {fig[k]:.6g}")
+    print()
+    print(f"- traffic = FETCH_SIZE x 1024 x {sf} + WRITE_SIZE x 1024 x {sw} (KiB per dispatch, scaled by the copy "
+          "calibration, copy_1GiB_calibration_pmc.md).")
+    print("- valu_issue_frac = SQ_INSTS_VALU x 4 cycles / (1024 SIMDs x GRBM_GUI_ACTIVE cycles): the nominal issue load "
+          "at one wave64 instruction per 4 cycles per SIMD; above 1 some instructions retire faster than that "
+          "(transcendentals and moves co-issue), so it reads as 'VALU-issue bound', not as an exact utilisation.")
+    print("- valu_active_threads = SQ_THREAD_CYCLES_VALU / SQ_ACTIVE_INST_VALU: lanes busy per VALU instruction (of 64).")
+    print("- lds_bytes = (SQ_INSTS_LDS_LOAD_BANDWIDTH + SQ_INSTS_LDS_STORE_BANDWIDTH) x 64 B; lds_GBs over the trace's "
+          "average launch; LDS peak 128 B/clk/CU x 256 CUs x 2.4 GHz = 78643 GB/s.")
+    print("- lds_bank_conflict_frac = SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE.")
+    print()
+    print("| counter | rows (dispatches x instances) | sum per dispatch |")
+    print("|---|---|---|")
+    for c, (n, v) in sorted(rows.items()):
+        unit = " (KiB)" if c in ("FETCH_SIZE", "WRITE_SIZE") else ""
+        print(f"| {c}{unit} | {n} | {v:.6g} |")
 
 
 def profile_figures(kernel: str, workload_tag: str):
@@ -567,8 +625,11 @@ def bench_slice(args):
     }
     tag = f"{F}x{H}x{W}_g{G}"
     full = (F, G, H, W, ts) == (256, 200_000, 512, 512, 16)
-    key = (lambda k: {"ug_backward_kernel": f"ug_backward_kernel<{ts}, false, true>",
-                      "ug_forward_kernel": f"ug_forward_kernel<{ts}, false, true>"}.get(k, k)) if full else None
+    # the template instances the step launches: (tile, depth/alpha terms, frames' grids, LDS batch)
+    fb = cfg.fwd_lds_batch or {8: 64, 16: 128, 32: 256}[ts]
+    bb = cfg.bwd_lds_batch or {8: 64, 16: 256, 32: 256}[ts]
+    key = (lambda k: {"ug_backward_kernel": f"ug_backward_kernel<{ts}, false, true, {bb}>",
+                      "ug_forward_kernel": f"ug_forward_kernel<{ts}, false, true, {fb}>"}.get(k, k)) if full else None
     roof = roofline(stages, launch_bytes, key, tag)
     roof["measured_copy_GBs"] = measure_copy_peak()
     return {
@@ -694,7 +755,8 @@ def bench_ray(args):
 
     launch_bytes = {"ur_forward_kernel": 64 * K + 8 * P, "ur_backward_kernel": 64 * K + 8 * P}
     full = (F, G, W, H) == (128, 200_000, 128, 1024)
-    roof = roofline(stages, launch_bytes, (lambda k: k) if full else None, f"ray_{F}x{W}x{H}_g{G}")
+    # the bench renders on the probe's grid: the <true> instances of the ray kernels
+    roof = roofline(stages, launch_bytes, (lambda k: f"{k}<true>") if full else None, f"ray_{F}x{W}x{H}_g{G}")
     cpu = None if args.no_cpu_baseline else ray_cpu_baseline(scene, frames[F // 2], cfg, args.cpu_sample_rays)
     return {
         "metric": "UltraG-Ray fwd+bwd frames/s per sweep, batched vs plain on 1 GPU; % of HBM roofline",
@@ -1027,6 +1089,9 @@ def main():
         return
     if args.rocprof_derive:
         rocprof_derive(args.rocprof_derive[0], args.rocprof_derive[1:])
+        return
+    if args.rocprof_report:
+        rocprof_report(args.rocprof_report[0], args.rocprof_report[1], args.rocprof_report[2:])
         return
     if args.copy_calibration:
         copy_calibration()
