@@ -154,11 +154,18 @@ inline void grid_lat_ax(const Grid& gr, int64_t p, float& lat, float& ax) {
     ax = gr.near_mm + (float)iy * gr.dax;
 }
 
+// The kernels evaluate the plane form scaled by log2(e)/2 (ug.hip, the splat kernels' staging and plane_q_x):
+// p, qq, s times kQScaleSqrt and K times kQScale in FP32, q' = (p' x + qq' y)^2 + (s' y)^2 + K', the test q' <=
+// chi2 * kQScale and the weight exp2(-q').
+constexpr float kQScale = 0.721347510814666748046875f;      // log2(e)/2 rounded to FP32
+constexpr float kQScaleSqrt = 0.849321782588958740234375f;  // the FP32 square root of kQScale
+
 inline float plane_q(const Plane& P, float lat, float ax) {
     const float x = lat - P.cx, y = ax - P.cy;
-    const float u = std::fmaf(P.p, x, P.qq * y);
-    const float v = P.s * y;
-    return std::fmaf(u, u, std::fmaf(v, v, P.K));
+    const float ps = P.p * kQScaleSqrt, qqs = P.qq * kQScaleSqrt, ss = P.s * kQScaleSqrt, Ks = P.K * kQScale;
+    const float u = std::fmaf(ps, x, qqs * y);
+    const float v = ss * y;
+    return std::fmaf(u, u, std::fmaf(v, v, Ks));
 }
 
 }  // namespace
@@ -321,8 +328,9 @@ int ug_oracle_pixel_lists(int G, const float* means, const float* tril, const fl
 namespace {
 
 // Forward (forward.cu:17-76) for one frame over per-pixel candidate lists.
-// With `plane` (one Plane per Gaussian) the frame is rendered on its own grid `grid`: the q <= chi2 decisions are
-// the plane form's, and so is the FP32 value of q; the FP64 value is |L^T (p - mu)|^2 from the given pixels.
+// With `plane` (one Plane per Gaussian) the frame is rendered on its own grid `grid`: the decisions are the scaled
+// plane form's (q' <= chi2 kQScale), and so is the FP32 weight exp2(-q'); the FP64 weight is exp(-q/2) with q =
+// |L^T (p - mu)|^2 from the given pixels.
 template <typename T>
 void forward_impl(int G, const T* means, const T* tril, const T* opac, const T* inten,
                   const T* pixels, int64_t P, const float* origin, const float* a_ele, const int32_t* counts,
@@ -341,14 +349,17 @@ void forward_impl(int G, const T* means, const T* tril, const T* opac, const T* 
             const T mx = means[3 * g], my = means[3 * g + 1], mz = means[3 * g + 2];
             T y[3];
             T q = quad_form<T>(l, px - mx, py - my, pz - mz, y);
+            T w = 0;
             if (plane) {
-                const float q32 = plane_q(plane[g], glat, gax);
-                if (q32 > chi2) continue;
-                if (sizeof(T) == sizeof(float)) q = T(q32);
+                const float q32 = plane_q(plane[g], glat, gax);  // q' = q log2(e)/2
+                if (q32 > chi2 * kQScale) continue;
+                w = (sizeof(T) == sizeof(float)) ? T(std::exp2(-q32)) : exp_(T(-0.5) * q);
             } else if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, mx, my, mz, chi2)) {
                 continue;
+            } else {
+                w = exp_(T(-0.5) * q);
             }
-            const T ah = T(opac[g]) * exp_(T(-0.5) * q);
+            const T ah = T(opac[g]) * w;
             if (!(ah > T(alpha_min))) continue;
             const T e = dot3<T>(mx - T(origin[0]), my - T(origin[1]), mz - T(origin[2]), T(a_ele[0]), T(a_ele[1]), T(a_ele[2]));
             S += ah;
@@ -406,11 +417,13 @@ void backward_impl(int G, const T* means, const T* tril, const T* opac, const T*
             T y[3];
             T q = quad_form<T>(l, dx, dy, dz, y);
             if (plane) {
-                const float q32 = plane_q(plane[g], glat, gax);
-                if (q32 > chi2) return false;
-                if (sizeof(T) == sizeof(float)) q = T(q32);
-            } else if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, T(means[3 * g]), T(means[3 * g + 1]),
-                                     T(means[3 * g + 2]), chi2)) {
+                const float q32 = plane_q(plane[g], glat, gax);  // q' = q log2(e)/2
+                if (q32 > chi2 * kQScale) return false;
+                w = (sizeof(T) == sizeof(float)) ? T(std::exp2(-q32)) : exp_(T(-0.5) * q);
+                return true;
+            }
+            if (!keep_pair<T>(tril + 6 * (int64_t)g, beta, px, py, pz, T(means[3 * g]), T(means[3 * g + 1]),
+                              T(means[3 * g + 2]), chi2)) {
                 return false;
             }
             w = exp_(T(-0.5) * q);

@@ -189,6 +189,23 @@ __device__ __forceinline__ float exp_neg_half(float q) {
     return __builtin_amdgcn_exp2f(q * -0.721347510814666748046875f);
 }
 
+// The plane form of q (frames rendered on their own grids) is evaluated scaled by log2(e)/2, q' = q log2(e)/2, so
+// that the weight exp(-q/2) is one exp2 of -q': the splat kernels multiply p, qq, s by kQScaleSqrt and K by kQScale
+// in FP32 when they stage a pair, and compare q' with chi2 * kQScale.
+constexpr float kQScale = 0.721347510814666748046875f;      // log2(e)/2 rounded to FP32
+constexpr float kQScaleSqrt = 0.849321782588958740234375f;  // the FP32 square root of kQScale
+__device__ __forceinline__ float exp2_neg(float qs) { return __builtin_amdgcn_exp2f(-qs); }
+
+// Opacities o for which the alpha_hat > alpha_min test cannot fail at a pair that passes the q test (scaled plane
+// form: q' <= chi2' = chi2 * kQScale): o * wmin_half > alpha_min, wmin_half = exp2_neg(chi2') / 2. v_exp_f32 is
+// within 1 ulp, so such a pair has alpha_hat = o * exp2_neg(q') >= o * exp2_neg(chi2') (1 - 2^-21) > alpha_min
+// (and with alpha_min < 0 the test never fails). The slice adjoint leaves the test out for these Gaussians; no
+// decision changes.
+__device__ __forceinline__ float alpha_free_wmin_half(float chi2s) { return 0.5f * exp2_neg(chi2s); }
+__device__ __forceinline__ bool alpha_free(float o, float wmin_half, float alpha_min) {
+    return o * wmin_half > alpha_min;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, kWave);

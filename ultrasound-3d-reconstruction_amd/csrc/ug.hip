@@ -13,6 +13,7 @@
 //             frames are contiguous and reduce in a fixed order.
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "common.h"
 
@@ -345,6 +346,10 @@ struct TileShape {
     static constexpr int kThreads = (TS >= 16) ? 256 : 64;
     static constexpr int kWaves = kThreads / kWave;
     static constexpr int kSubPerWave = kSub / kWaves;  // pixels per thread
+    // Minimum waves per SIMD asked of the compiler for the adjoint on 16-pixel tiles (at most 128 VGPRs): its LDS
+    // allows four work-groups of 256 threads per CU at the default batch, and left to itself the compiler takes
+    // more registers than that occupancy leaves (144 for the plane form); 1 leaves the choice to the compiler.
+    static constexpr int kBwdMinWavesPerSimd = (TS == 16) ? 4 : 1;
 };
 
 // Launch shape of the forward splat: a wavefront owns kSubPerWave sub-blocks (wave w: w, w + kWaves, ...),
@@ -449,15 +454,18 @@ __device__ __forceinline__ int frame_of_tile(const int32_t* __restrict__ frame_t
 // choose 64, 128 or 256, and the sums do not depend on it). Follows kernel_pixel_ultragauss_forward,
 // ops/ultragauss_renderer/src/shader/forward.cu:17-76: alpha_hat = o * exp(-q/2) for q <= chi2 and
 // alpha_hat > alpha_min; A = sum alpha_hat + bg; image = clamp((sum alpha_hat c + bg I_bg) / max(A,1e-8));
-// depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8). The per-pixel sums are accumulated in FP64: the
-// adjoint's coefficient dL/dalpha_hat = dN (c - image) depends on the image to far better than FP32-sum
-// accuracy wherever c is close to it.
+// depth = sum alpha_hat elev / max(sum alpha_hat, 1e-8). The per-pixel sums are FP32 in list order, as in the
+// reference kernel (its accumulators are floats): per evaluated pair one add and one multiply-add with no
+// conversions. Against an FP64 sum that is an error of a few 1e-7 relative in the image, and the adjoint's
+// coefficient dL/dalpha_hat = dN (c - image) inherits it only where c is close to the image, where the term is
+// small.
 
-// Plane form of q (pair_plane_form) at a pixel with probe coordinates (lat, ax): two subtractions and five
-// multiply-adds in one fixed order, shared by the forward and the adjoint so that their q <= chi2 decisions are
-// the same bits. pa = (cx, cy, K, .), pb = (p, qq, s, .).
-__device__ __forceinline__ float plane_q(float lat, float ax, float cx, float cy, float K, float p, float qq, float s) {
-    const float x = lat - cx, y = ax - cy;
+// Plane form of q (pair_plane_form) at a pixel with probe coordinates (lat, ax), given x = lat - cx: with that
+// subtraction, two subtractions and five multiply-adds in one fixed order, the same in the forward and the adjoint
+// so that their decisions are the same bits. The splat kernels pass the coefficients scaled (kQScaleSqrt,
+// kQScale), which gives q' = q log2(e)/2.
+__device__ __forceinline__ float plane_q_x(float x, float ax, float cy, float K, float p, float qq, float s) {
+    const float y = ax - cy;
     const float u = __builtin_fmaf(p, x, qq * y);
     const float v = s * y;
     return __builtin_fmaf(u, u, __builtin_fmaf(v, v, K));
@@ -495,11 +503,16 @@ __device__ __forceinline__ void ug_forward_tile(
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
     const float chi2w = sqrt_chi2 * sqrt_chi2 * 1.002f;
+    const float thr = kPlaneBody ? chi2 * kQScale : chi2;  // the q test (scaled plane form: q')
 
+    // A lane's pixels share their column: sub-block sb = wave + k kWaves sits in column sb % kSubPerRow, and
+    // kWaves is a multiple of kSubPerRow whenever a lane holds more than one pixel; so with the plane form the
+    // lateral offset lat - cx is formed once per Gaussian for all of them.
+    static_assert(PPT == 1 || S::kWaves % S::kSubPerRow == 0, "a lane's pixels share their column");
     float pc[NC][PPT];
     bool valid[PPT];
     int pidx[PPT];
-    double S_[PPT], N_[PPT], D_[PPT];
+    float S_[PPT], N_[PPT], D_[PPT];
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         const int sb = wave + k * S::kWaves;
@@ -522,7 +535,7 @@ __device__ __forceinline__ void ug_forward_tile(
             pc[1][k] = y;
             pc[NC - 1][k] = z;
         }
-        S_[k] = N_[k] = D_[k] = 0.0;
+        S_[k] = N_[k] = D_[k] = 0.f;
     }
 
     for (int base = s0; base < s1; base += kBatch) {
@@ -544,8 +557,9 @@ __device__ __forceinline__ void ug_forward_tile(
             inst_mask[base + j] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             if constexpr (kPlaneBody) {
                 const float4 ps2 = pair_sect[kSectF4 * pv + 2];
-                L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z, r0.w);  // cx, cy, K, opacity
-                L.r1[j] = make_float4(ps1.x, ps1.y, ps1.z, r1.x);  // p, qq, s, intensity
+                L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z * kQScale, r0.w);  // cx, cy, K', opacity
+                L.r1[j] = make_float4(ps1.x * kQScaleSqrt, ps1.y * kQScaleSqrt, ps1.z * kQScaleSqrt,
+                                      r1.x);  // p', qq', s', intensity
                 L.r2[j] = make_float4(e, 0.f, 0.f, 0.f);
             } else {
                 L.r0[j] = r0;                                  // mu, opacity
@@ -567,6 +581,7 @@ __device__ __forceinline__ void ug_forward_tile(
                 bk[k] = __ballot((mm >> (wave + k * S::kWaves)) & 1u);
                 any |= bk[k];
             }
+
             while (any) {
                 const int bit = (int)__builtin_ctzll(any);
                 any &= any - 1;
@@ -574,25 +589,26 @@ __device__ __forceinline__ void ug_forward_tile(
                 const float4 a = L.r0[j];
                 const float4 b = L.r1[j];
                 const float4 c = L.r2[j];
-                const double cd = (double)(kPlaneBody ? b.w : b.x);
+                const float cf = kPlaneBody ? b.w : b.x;
+                const float ef = kPlaneBody ? c.x : b.y;
+                const float xg = kPlaneBody ? pc[0][0] - a.x : 0.f;  // lat - cx, the lane's column
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
                     float q;
                     if constexpr (kPlaneBody) {
-                        q = plane_q(pc[0][k], pc[1][k], a.x, a.y, a.z, b.x, b.y, b.z);
+                        q = plane_q_x(xg, pc[1][k], a.y, a.z, b.x, b.y, b.z);
                     } else {
                         float y0, y1, y2;
                         q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, pc[0][k] - a.x, pc[1][k] - a.y,
                                       pc[NC - 1][k] - a.z, y0, y1, y2);
                     }
-                    if (q <= chi2) {
-                        const float ah = a.w * exp_neg_half(q);
+                    if (q <= thr) {
+                        const float ah = a.w * (kPlaneBody ? exp2_neg(q) : exp_neg_half(q));
                         if (ah > alpha_min) {
-                            const double ahd = (double)ah;
-                            S_[k] += ahd;
-                            N_[k] = __builtin_fma(ahd, cd, N_[k]);
-                            if constexpr (kDepth) D_[k] = __builtin_fma(ahd, (double)(kPlaneBody ? c.x : b.y), D_[k]);
+                            S_[k] += ah;
+                            N_[k] = __builtin_fmaf(ah, cf, N_[k]);
+                            if constexpr (kDepth) D_[k] = __builtin_fmaf(ah, ef, D_[k]);
                         }
                     }
                 }
@@ -603,13 +619,13 @@ __device__ __forceinline__ void ug_forward_tile(
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
-        const double A = S_[k] + (double)bg_alpha;
-        const double num = __builtin_fma((double)bg_alpha, (double)bg_intensity, N_[k]);
-        const double img = num / fmax(A, 1e-8);
-        out_image[pidx[k]] = (float)fmax(0.0, fmin(1.0, img));
-        out_alpha[pidx[k]] = (float)(alpha_excludes_bg ? S_[k] : A);
-        if constexpr (kDepth) out_depth[pidx[k]] = (float)(D_[k] / fmax(S_[k], 1e-8));
-        out_aux[pidx[k]] = make_float4((float)S_[k], (float)img, kDepth ? (float)D_[k] : 0.f, 0.f);
+        const float A = S_[k] + bg_alpha;
+        const float num = __builtin_fmaf(bg_alpha, bg_intensity, N_[k]);
+        const float img = num / fmaxf(A, 1e-8f);
+        out_image[pidx[k]] = fmaxf(0.f, fminf(1.f, img));
+        out_alpha[pidx[k]] = alpha_excludes_bg ? S_[k] : A;
+        if constexpr (kDepth) out_depth[pidx[k]] = D_[k] / fmaxf(S_[k], 1e-8f);
+        out_aux[pidx[k]] = make_float4(S_[k], img, kDepth ? D_[k] : 0.f, 0.f);
     }
 }
 
@@ -700,7 +716,7 @@ __device__ __forceinline__ void write_partial(float4* __restrict__ partial, int6
 
 // Plane form (linear-probe tiles with the frames' own grids): an item's lane walks its sub-block row by row. Along
 // a row the axial offset is fixed, so q = (p x + qq y)^2 + (s y)^2 + K costs one subtraction and two multiply-adds
-// per pixel (the same operations as plane_q, so the decisions equal the forward's), and the gradient needs only
+// per pixel (the same operations as plane_q_x, so the decisions equal the forward's), and the gradient needs only
 // six moments of m = -dah ah over the pixel offsets (X, Y) from the tile centre: with d = p - mu and y = L^T d
 // affine in (X, Y), d = d_c + X U + Y V, sum(m y_k) and sum(m y_k d_i) follow from sum(m), sum(m X), sum(m Y),
 // sum(m X^2), sum(m X Y), sum(m Y^2) once per instance (plane_sums_to_grad). Per pixel the moments cost three
@@ -754,9 +770,9 @@ __device__ __forceinline__ GradSums plane_sums_to_grad(const GradSums& a, const 
     return o;
 }
 
-// Plane-form staging inside the adjoint's pixel arrays: per sub-block kPlanePixStride float2 of (dN, v) in p and
-// of (dX, dD) in c (66: 16-byte aligned rows, sub-blocks in different banks), then from float4 NSUB *
-// kPlaneCoordBase of c, per sub-block 8 column lateral and 8 row axial coordinates.
+// Plane-form staging inside the adjoint's pixel arrays: per sub-block kPlanePixStride float2 of (dN, v) (or
+// (dN, dN v)) in p and of (dX, dD) in c (66: 16-byte aligned rows, sub-blocks in different banks), then from
+// float4 NSUB * kPlaneCoordBase of c, per sub-block 8 column lateral and 8 row axial coordinates.
 constexpr int kPlanePixStride = 66;
 constexpr int kPlaneCoordBase = 33;
 static_assert(kPlanePixStride * 8 <= kSubStride * 16 && (kPlaneCoordBase + 4) <= kSubStride, "plane staging fits");
@@ -771,7 +787,8 @@ struct BwdLds {
     float4 p[NSUB * kSubStride];  // general form: px, py, pz, dN; plane form: lat, ax, dN, v
     float4 c[NSUB * kSubStride];  // general form: v, dX, dD; plane form: dX, dD
     float4 g0[kBatch], g1[kBatch], g2[kBatch];                     // the batch's Gaussian records
-    float4 g3[kPlane ? kBatch : 1], g4[kPlane ? kBatch : 1];       // plane form: (cx, cy, K, o), (p, qq, s, c)
+    // the scaled plane form: (cx, cy, K', o), (p', qq', s', c)
+    float4 g3[kPlane ? kBatch : 1], g4[kPlane ? kBatch : 1];
     int32_t ei[kBatch];                                            // emit index of each instance of the batch
     uint16_t item[MAXI];
     int wtot[TileShape<TS>::kWaves];
@@ -804,6 +821,8 @@ __device__ __forceinline__ void ug_backward_tile(
     static_assert(FwdShape<TS>::kSub == NSUB, "the forward's instance masks number the same sub-blocks");
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     const float ox = fr.origin[0], oy = fr.origin[1], oz = fr.origin[2];
+    const float chi2s = chi2 * kQScale;  // the plane form's q' test
+    const float wmin_half = alpha_free_wmin_half(chi2s);
 
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
@@ -835,10 +854,11 @@ __device__ __forceinline__ void ug_backward_tile(
             dX = ga - gd * sm.z * invS * invS;
         }
         if constexpr (kPlaneBody) {
-            // per pixel (dN, v) and (dX, dD); per sub-block the lateral coordinate of each column and the axial
-            // coordinate of each row (every pixel of the sub-block: dN = 0 outside the frame)
+            // per pixel (dN, v) and (dX, dD), or (dN, dN v) without alpha and depth gradients (then dL/dah =
+            // dN c - dN v is one multiply-add per pair); per sub-block the lateral coordinate of each column and the
+            // axial coordinate of each row (every pixel of the sub-block: dN = 0 outside the frame)
             grid_lat_ax(fr, ix, iy, x, y);
-            reinterpret_cast<float2*>(L.p)[sb * kPlanePixStride + lane] = make_float2(dN, vimg);
+            reinterpret_cast<float2*>(L.p)[sb * kPlanePixStride + lane] = make_float2(dN, kFull ? vimg : dN * vimg);
             reinterpret_cast<float2*>(L.c)[sb * kPlanePixStride + lane] = make_float2(dX, dD);
             float* la = reinterpret_cast<float*>(L.c + NSUB * kPlaneCoordBase) + sb * 16;
             if (lane < 8) la[lane] = x;
@@ -872,8 +892,9 @@ __device__ __forceinline__ void ug_backward_tile(
                 L.ei[j] = (int32_t)ei;
                 if constexpr (kPlaneBody) {
                     const float4 ps1 = pair_sect[kSectF4 * (int64_t)pv + 1], ps2 = pair_sect[kSectF4 * (int64_t)pv + 2];
-                    L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z, q0.w);  // cx, cy, K, opacity
-                    L.g4[j] = make_float4(ps1.x, ps1.y, ps1.z, q1.x);  // p, qq, s, intensity
+                    L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z * kQScale, q0.w);  // cx, cy, K', opacity
+                    L.g4[j] = make_float4(ps1.x * kQScaleSqrt, ps1.y * kQScaleSqrt, ps1.z * kQScaleSqrt,
+                                          q1.x);  // p', qq', s', intensity
                 }
                 if (m[i] == 0) {  // touches no pixel of the tile: its partial is zero
                     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -923,21 +944,24 @@ __device__ __forceinline__ void ug_backward_tile(
             GradSums a;
 #pragma unroll
             for (int k = 0; k < 12; ++k) a.v[k] = 0.f;
+            // wave-uniform: every item's opacity passes alpha_free, so no pixel can fail the alpha_hat test alone
+            const float o_item = L.g0[valid ? slot : 0].w;
+            const bool chunk_alpha_free = __ballot(valid && !alpha_free(o_item, wmin_half, alpha_min)) == 0;
             if (valid) {
                 const float4 r0 = L.g0[slot];
                 const float e = dot3(r0.x - ox, r0.y - oy, r0.z - oz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
                 const float4* __restrict__ sp = L.p + sb * kSubStride;
                 const float4* __restrict__ sc = L.c + sb * kSubStride;
                 if constexpr (kPlaneBody) {
-                    const float4 A = L.g3[slot];  // cx, cy, K, o
-                    const float4 B = L.g4[slot];  // p, qq, s, c
+                    const float4 A = L.g3[slot];  // cx, cy, K', o
+                    const float4 B = L.g4[slot];  // p', qq', s', c
                     const float* la = reinterpret_cast<const float*>(L.c + NSUB * kPlaneCoordBase) + sb * 16;
                     const float4 la0 = *reinterpret_cast<const float4*>(la), la1 = *reinterpret_cast<const float4*>(la + 4);
-                    // lateral offsets from the minimum, once per item (the subtraction of plane_q, same bits)
+                    // lateral offsets from the minimum, once per item (the subtraction before plane_q_x, same bits)
                     const float xs[8] = {la0.x - A.x, la0.y - A.x, la0.z - A.x, la0.w - A.x,
                                          la1.x - A.x, la1.y - A.x, la1.z - A.x, la1.w - A.x};
-                    // (dN, v) of the sub-block's pixels, two per float4, a row = four float4 read together at the
-                    // start of the row, so the LDS latency is paid once per row
+                    // (dN, v) or (dN, dN v) of the sub-block's pixels, two per float4, a row = four float4 read
+                    // together at the start of the row, so the LDS latency is paid once per row
                     const float4* __restrict__ dv = reinterpret_cast<const float4*>(reinterpret_cast<const float2*>(L.p) +
                                                                                     sb * kPlanePixStride);
                     const float2* __restrict__ dxd = reinterpret_cast<const float2*>(L.c) + sb * kPlanePixStride;
@@ -946,6 +970,9 @@ __device__ __forceinline__ void ug_backward_tile(
                     // -o t, so the moments of m are -o times those of t (applied once per item below).
                     float acw = 0.f, aedw = 0.f;
                     float T0 = 0.f, TX = 0.f, TY = 0.f, TXX = 0.f, TXY = 0.f, TYY = 0.f;
+                    // the sub-block's 64 pixels, with or without the alpha_hat test (the same sums either way)
+                    auto walk = [&](auto alpha_test) {
+                    constexpr bool kAlphaTest = decltype(alpha_test)::value;
 #pragma unroll
                     for (int r = 0; r < 8; ++r) {
                         float4 cur[4];
@@ -959,14 +986,14 @@ __device__ __forceinline__ void ug_backward_tile(
 #pragma unroll
                         for (int jx = 0; jx < 8; ++jx) {
                             const float dN = (jx & 1) ? cur[jx >> 1].z : cur[jx >> 1].x;
-                            const float vi = (jx & 1) ? cur[jx >> 1].w : cur[jx >> 1].y;
+                            const float vi = (jx & 1) ? cur[jx >> 1].w : cur[jx >> 1].y;  // v, or dN v (!kFull)
                             // branch-free: a pixel that fails the forward's tests (q <= chi2, ah > alpha_min)
                             // enters every sum through wp = 0; the coefficients of a listed pair and the pixel
                             // coefficients are finite, so the masked terms are exact zeros
                             const float u = __builtin_fmaf(B.x, xs[jx], qy);
                             const float q = __builtin_fmaf(u, u, vv);
-                            const float w = exp_neg_half(q);
-                            const bool pass = (q <= chi2) && (A.w * w > alpha_min);
+                            const float w = exp2_neg(q);
+                            const bool pass = (q <= chi2s) && (!kAlphaTest || A.w * w > alpha_min);
                             const float wp = pass ? w : 0.f;
                             float da;
                             if constexpr (kFull) {
@@ -974,7 +1001,7 @@ __device__ __forceinline__ void ug_backward_tile(
                                 da = __builtin_fmaf(dN, B.w - vi, __builtin_fmaf(cc.y, e, cc.x));
                                 aedw = __builtin_fmaf(cc.y, wp, aedw);
                             } else {
-                                da = dN * (B.w - vi);
+                                da = __builtin_fmaf(dN, B.w, -vi);
                             }
                             acw = __builtin_fmaf(dN, wp, acw);
                             const float t = da * wp;
@@ -991,6 +1018,11 @@ __device__ __forceinline__ void ug_backward_tile(
                         TYY = __builtin_fmaf(R0, Y * Y, TYY);
                         TXY = __builtin_fmaf(RX, Y, TXY);
                     }
+                    };
+                    if (chunk_alpha_free)
+                        walk(std::false_type{});
+                    else
+                        walk(std::true_type{});
                     const float mo = -A.w;
                     const float M0 = mo * T0, MX = mo * TX, MY = mo * TY, MXX = mo * TXX, MXY = mo * TXY,
                                 MYY = mo * TYY;
@@ -1096,7 +1128,8 @@ __device__ __forceinline__ void ug_backward_tile(
 // dN (c - v) alone and the depth term is not accumulated. kPlane: the pixels are the frames' own grids (pixels ==
 // NULL); linear-probe tiles then take the plane form of q (as in the forward) and fan tiles the general form.
 template <int TS, bool kFull, bool kPlane, int kBatch>
-__global__ __launch_bounds__(TileShape<TS>::kThreads) void ug_backward_kernel(
+__global__ __launch_bounds__(TileShape<TS>::kThreads)
+__attribute__((amdgpu_waves_per_eu(TileShape<TS>::kBwdMinWavesPerSimd, 8))) void ug_backward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
     const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames,
