@@ -352,20 +352,22 @@ struct TileShape {
     static constexpr int kBwdMinWavesPerSimd = (TS == 16) ? 4 : 1;
 };
 
-// Launch shape of the forward splat: a wavefront owns kSubPerWave sub-blocks (wave w: w, w + kWaves, ...),
-// two per wavefront for 16-pixel tiles, so every Gaussian record read from LDS serves two pixels per lane
-// whenever the Gaussian touches both of the wavefront's (vertically adjacent) sub-blocks.
+// Launch shape of the forward splat: a wavefront owns kSubPerWave sub-blocks (wave w: w, w + kWaves, ...). A
+// 16-pixel tile is one wavefront holding all four sub-blocks, so every Gaussian record read from LDS serves up to
+// four pixels per lane; a 32-pixel tile is four wavefronts of four sub-blocks each. kDefaultBatch: the Gaussians
+// staged per LDS batch unless the settings choose.
 template <int TS>
 struct FwdShape {
     static constexpr int kSubPerRow = TS / 8;
     static constexpr int kSub = kSubPerRow * kSubPerRow;
-    static constexpr int kThreads = (TS == 8) ? 64 : (TS == 16 ? 128 : 256);
+    static constexpr int kThreads = (TS == 32) ? 256 : 64;
     static constexpr int kWaves = kThreads / kWave;
     static constexpr int kSubPerWave = kSub / kWaves;
-    // Minimum waves per SIMD asked of the compiler: for 16-pixel tiles eight (at most 64 VGPRs) measured
-    // faster than its own choice of five, the few values it then spills being live only at the ends of the
-    // kernel; 1 leaves the choice to the compiler.
-    static constexpr int kMinWavesPerSimd = (TS == 16) ? 8 : 1;
+    static constexpr int kDefaultBatch = (TS == 8) ? 64 : (TS == 16 ? 128 : 256);
+    // Minimum waves per SIMD asked of the compiler: for 16-pixel tiles five (at most 96 VGPRs, no spill); config 2
+    // measured 2.14 ms against 2.26 at six (80 VGPRs, spilling) and 2.37 at four; 1 leaves the choice to the
+    // compiler.
+    static constexpr int kMinWavesPerSimd = (TS == 16) ? 5 : 1;
 };
 
 struct SubBounds {
@@ -508,7 +510,9 @@ __device__ __forceinline__ void ug_forward_tile(
     // A lane's pixels share their column: sub-block sb = wave + k kWaves sits in column sb % kSubPerRow, and
     // kWaves is a multiple of kSubPerRow whenever a lane holds more than one pixel; so with the plane form the
     // lateral offset lat - cx is formed once per Gaussian for all of them.
-    static_assert(PPT == 1 || S::kWaves % S::kSubPerRow == 0, "a lane's pixels share their column");
+    // With one wavefront per tile a lane's pixels fall in kSubPerRow columns, pixel k in column k % kSubPerRow.
+    constexpr int kCols = (PPT == 1 || S::kWaves % S::kSubPerRow == 0) ? 1 : S::kSubPerRow;
+    static_assert(kCols == 1 || S::kWaves == 1, "a lane's pixels share their column, or one wave holds the tile");
     float pc[NC][PPT];
     bool valid[PPT];
     int pidx[PPT];
@@ -591,13 +595,15 @@ __device__ __forceinline__ void ug_forward_tile(
                 const float4 c = L.r2[j];
                 const float cf = kPlaneBody ? b.w : b.x;
                 const float ef = kPlaneBody ? c.x : b.y;
-                const float xg = kPlaneBody ? pc[0][0] - a.x : 0.f;  // lat - cx, the lane's column
+                float xg[kCols];  // lat - cx of the lane's columns
+#pragma unroll
+                for (int cc = 0; cc < kCols; ++cc) xg[cc] = kPlaneBody ? pc[0][cc] - a.x : 0.f;
 #pragma unroll
                 for (int k = 0; k < PPT; ++k) {
                     if (!((bk[k] >> bit) & 1ull)) continue;  // scalar test
                     float q;
                     if constexpr (kPlaneBody) {
-                        q = plane_q_x(xg, pc[1][k], a.y, a.z, b.x, b.y, b.z);
+                        q = plane_q_x(xg[k % kCols], pc[1][k], a.y, a.z, b.x, b.y, b.z);
                     } else {
                         float y0, y1, y2;
                         q = quad_form(b.z, b.w, c.x, c.y, c.z, c.w, pc[0][k] - a.x, pc[1][k] - a.y,
@@ -1418,7 +1424,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                        reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask), plan->n_tiles, heads)
 #define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
     do {                                                                                                              \
-        switch (lds_batch(settings->fwd_lds_batch, FwdShape<TS>::kThreads)) {                                                        \
+        switch (lds_batch(settings->fwd_lds_batch, FwdShape<TS>::kDefaultBatch)) {                                    \
             case 64: UG_FWD_B(TS, DEPTH, PLANE, 64); break;                                                           \
             case 128: UG_FWD_B(TS, DEPTH, PLANE, 128); break;                                                         \
             default: UG_FWD_B(TS, DEPTH, PLANE, 256); break;                                                          \
