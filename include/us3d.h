@@ -154,18 +154,20 @@ typedef struct us3d_ug_settings {
  * g_begin+gl), fm_cnt / fm_off (their popcounts and scan: the slot of a visible pair in vis_list), vis_pair
  * i32[v_cap] (dense pair index of v), pair_count i32[v_cap] (instances of v), pair_off i32[v_cap+1] (first emit
  * index), pair_rect i32x4[v_cap] (tile rectangle [x0,x1)x[y0,y1); UltraG-Ray: (0, min_ray, 1, max_ray+1)),
- * pair_dk u32[v_cap] (depth key), vis_list i32[v_cap] (frame-major list of v: a frame's visible pairs are
- * consecutive, in Gaussian order), tile_count i32[n_tiles], tile_off i32[n_tiles+1], tile_cursor i32[n_tiles]
- * (emit cursors), seg_key u64[K] (depth key << 32 | Gaussian), seg_val i32[K] (the instance's visible pair v;
- * its emit index is pair_off[v] plus its row-major position in the pair's rectangle), records (per Gaussian),
- * partial (per instance, at the emit index), inst_mask u16[K] (UltraGauss: the sub-block mask of each sorted
- * instance, written by the forward, read by the backward; empty for UltraG-Ray), pair_sect (per visible pair:
- * UltraGauss f32[12], its part of the sub-block test and the plane form of q on the frame's grid; GaussianPile
- * f32[8], its focus footprint; written by the map, read by the forward and the backward). */
+ * pair_dk u32[v_cap] (depth key), pair_g i32[v_cap] (Gaussian of v), vis_list i32[v_cap] (frame-major list of
+ * v: a frame's visible pairs are consecutive, in Gaussian order), tile_count i32[n_tiles], tile_off
+ * i32[n_tiles+1], tile_cursor i32[n_tiles] (emit cursors), seg_key u64[K] (the instances bucket by bucket, each
+ * as the key depth key << 32 | v of its visible pair v, sorted: inside a tile v ascends with the Gaussian, so this
+ * is the reference's (depth key, Gaussian) order; an instance's emit index is pair_off[v] plus its row-major
+ * position in the pair's rectangle), seg_key_tmp u64[K] (sort scratch), records (per Gaussian), partial (per
+ * instance, at the emit index), inst_mask u16[K] (UltraGauss: the sub-block mask of each sorted instance, written
+ * by the forward, read by the backward; empty for UltraG-Ray), pair_sect (per visible pair: UltraGauss f32[16],
+ * its part of the sub-block test, the plane form of q on the frame's grid and its emit base; GaussianPile
+ * f32[12], its focus footprint and emit base; written by the map, read by the forward and the backward). */
 typedef struct us3d_layout {
     int64_t stats, queue, pair_bits, word_cnt, word_off, fm_bits, fm_cnt, fm_off, vis_pair;
-    int64_t pair_count, pair_off, pair_rect, pair_dk, vis_list, tile_count, tile_off;
-    int64_t tile_cursor, seg_key, seg_val, seg_key_tmp, seg_val_tmp, records, partial, scan_tmp;
+    int64_t pair_count, pair_off, pair_rect, pair_dk, pair_g, vis_list, tile_count, tile_off;
+    int64_t tile_cursor, seg_key, seg_key_tmp, records, partial, scan_tmp;
     int64_t inst_mask, pair_sect, total;
 } us3d_layout;
 

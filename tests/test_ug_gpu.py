@@ -549,13 +549,14 @@ def test_emit_split_modes_bitexact(dev, n_frames):
     V = int(m.stats.n_visible)
     i32 = lambda o, n: ws[o: o + 4 * n].view(torch.int32).numpy().astype(np.int64)  # noqa: E731
     key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64).numpy()
-    val = i32(off.seg_val, K)
+    val = key & 0xFFFFFFFF  # each instance's visible pair v (the key's low word)
     tile_off = i32(off.tile_off, lay.n_tiles + 1)
     vis_pair = i32(off.vis_pair, V)
     pair_count = i32(off.pair_count, V)
     pair_off = i32(off.pair_off, V + 1)
     rect = ws[off.pair_rect: off.pair_rect + 16 * V].view(torch.int32).numpy().reshape(-1, 4).astype(np.int64)
     vis_list = i32(off.vis_list, V)
+    pair_g = i32(off.pair_g, V)
     assert K > 300 * min(n_frames, 3) and 0 < V <= K
     # ranks: dense pairs with a tile, Gaussian-major; the sampled frames' visible Gaussians are the oracle's
     assert np.all(np.diff(vis_pair) > 0) and vis_pair[-1] < lay.n_pairs
@@ -569,11 +570,11 @@ def test_emit_split_modes_bitexact(dev, n_frames):
     # the frame-major list: frame by frame, each frame's ranks in Gaussian order
     order = np.lexsort((vg, vf))
     np.testing.assert_array_equal(vis_list, order)
-    # instances: the key's Gaussian is the rank's; emit indices from (rank, tile) are a permutation of [0, K)
+    # every rank's Gaussian is its dense pair's; an instance lies in its pair's frame and rectangle, and the emit
+    # indices from (rank, tile) are a permutation of [0, K)
+    np.testing.assert_array_equal(pair_g, vg)
     tile = np.repeat(np.arange(lay.n_tiles), np.diff(tile_off))  # global tile of each instance
     frame = np.searchsorted(np.asarray(lay.frame_tile_base), tile, side="right") - 1
-    g = key & 0xFFFFFFFF
-    np.testing.assert_array_equal(vg[val], g)
     np.testing.assert_array_equal(vf[val], frame)
     tiles_x = np.asarray(lay.tiles_x)[frame]
     lt = tile - np.asarray(lay.frame_tile_base)[frame]

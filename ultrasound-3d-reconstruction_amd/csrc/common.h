@@ -227,11 +227,10 @@ __device__ __forceinline__ float wave_max(float v) {
 int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* tmp, hipStream_t stream);
 int64_t scan_tmp_bytes(int64_t n);
 
-// Per-segment sort of 64-bit keys carrying a 32-bit value. Segment t is [seg_off[t], seg_off[t+1]).
-// Keys must be unique within a segment (they are: the low word is the Gaussian index).
-int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp,
-                  const int32_t* seg_off, int n_segments, us3d_stats* stats, int64_t k_capacity,
-                  hipStream_t stream);
+// Per-segment sort of unique 64-bit keys (segment t is [seg_off[t], seg_off[t+1])); keys_tmp is scratch of the
+// same size for segments longer than 4096.
+int sort_segments(uint64_t* keys, uint64_t* keys_tmp, const int32_t* seg_off, int n_segments, us3d_stats* stats,
+                  int64_t k_capacity, hipStream_t stream);
 
 // Workspace layout shared by the renderers (record, partial, mask and per-pair record sizes differ).
 us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap,

@@ -246,13 +246,17 @@ int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* t
 }
 
 // ---------------------------------------------------------------- per-segment sort
+// A segment holds instances as 64-bit keys depth_key << 32 | v (v: the instance's visible pair), unique since v is,
+// and sorted without a payload. Inside a tile every instance belongs to one frame, where v ascends with the
+// Gaussian index, so the order is that of (depth key, Gaussian).
 // Segments of up to kWaveSortCap keys are sorted by one wavefront each, four per workgroup: a bitonic network
 // over registers (up to 8 keys per lane) with cross-lane shuffles, no LDS. Segments of up to kWaveSortCapMid
 // keys get two wavefronts (two register-sorted halves, one LDS stage, a register merge). Longer segments get
-// one workgroup each (a workgroup examines kLongSortSegments consecutive segments and sorts the long ones): runs of kWaveSortCap keys are sorted by the wavefronts in registers the same way and joined by a
-// bitonic merge whose long-distance stages run in LDS and the rest in registers (kMergeCap keys); beyond
-// kMergeCap, chunks sorted that way are merged in global memory (ping-pong with the tmp arrays), where an
-// element's place is its index in its run plus its rank in the partner run, so a pass is fully parallel.
+// one workgroup each (a workgroup examines kLongSortSegments consecutive segments and sorts the long ones): runs
+// of kWaveSortCap keys are sorted by the wavefronts in registers the same way and joined by a bitonic merge whose
+// long-distance stages run in LDS and the rest in registers (kMergeCap keys); beyond kMergeCap, chunks sorted that
+// way are merged in global memory (ping-pong with the tmp array), where an element's place is its index in its run
+// plus its rank in the partner run, so a pass is fully parallel.
 
 constexpr int kSortThreads = 256;
 constexpr int kWaveSortCap = 512;      // segments sorted by one wavefront, <= 8 keys per lane
@@ -261,7 +265,8 @@ constexpr int kMergeCap = 4096;        // segments (and chunks) sorted in one wo
 constexpr int kWaveSortWaves = 4;
 constexpr int kLongSortSegments = 4;   // segments examined by one workgroup of the long-segment sort
 
-// Bitonic sort of 64*E (key, value) pairs held by one wavefront in registers: lane l holds elements
+
+// Bitonic sort of 64*E keys held by one wavefront in registers: lane l holds elements
 // l*E .. l*E+E-1. The network is unrolled completely (the partner distance and direction of every exchange
 // are compile-time constants, so shuffles and register indices are static). Stages whose partner index
 // differs only in the low log2(E) bits exchange inside a lane; the others exchange with lane ^ (j / E)
@@ -283,7 +288,7 @@ __device__ __forceinline__ int shfl_xor_pow2(int v, int m) {
 // kDpp: shuffles through shfl_xor_pow2 (only where the network unrolls completely, so every distance is a
 // constant); otherwise ds_bpermute.
 template <int E, bool kDpp>
-__device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], int t) {
+__device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int t) {
     constexpr int N = kWave * E;
 #pragma unroll
     for (int kk = 2; kk <= N; kk <<= 1) {
@@ -297,7 +302,6 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], 
                         const bool asc = (((t * E + e) & kk) == 0);
                         if ((k[e] > k[p]) == asc) {
                             const uint64_t tk = k[e]; k[e] = k[p]; k[p] = tk;
-                            const int32_t tv = v[e]; v[e] = v[p]; v[p] = tv;
                         }
                     }
                 }
@@ -309,16 +313,14 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], 
                     const int klo = (int)(uint32_t)k[e], khi = (int)(uint32_t)(k[e] >> 32);
                     const uint32_t lo = (uint32_t)(kDpp ? shfl_xor_pow2(klo, m) : __shfl_xor(klo, m, kWave));
                     const uint32_t hi = (uint32_t)(kDpp ? shfl_xor_pow2(khi, m) : __shfl_xor(khi, m, kWave));
-                    const int32_t ov = kDpp ? shfl_xor_pow2(v[e], m) : __shfl_xor(v[e], m, kWave);
                     const uint64_t ok = ((uint64_t)hi << 32) | lo;
                     const bool asc = (i & kk) == 0;
                     const bool lower = (i & j) == 0;
-                    // the lower element keeps the min when ascending, the max when descending
-                    const bool take = (lower == asc) ? (ok < k[e]) : (ok > k[e]);
-                    if (take) {
-                        k[e] = ok;
-                        v[e] = ov;
-                    }
+                    // the lower element keeps the min when ascending, the max when descending; keys are unique
+                    // (padding keys are equal, and taking an equal key changes nothing), so the max is taken
+                    // exactly when the partner is not smaller, and the choice is a mask, not a branch
+                    const bool take = (ok < k[e]) != (lower != asc);
+                    if (take) k[e] = ok;
                 }
             }
         }
@@ -326,29 +328,24 @@ __device__ __forceinline__ void bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E], 
 }
 
 template <int E>
-__device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E], int32_t (&v)[E]) {
-    bitonic_regs<E, (E <= 8)>(k, v, (int)(threadIdx.x & (kWave - 1)));
+__device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E]) {
+    bitonic_regs<E, (E <= 8)>(k, (int)(threadIdx.x & (kWave - 1)));
 }
 
 template <int E>
-__device__ __forceinline__ void wave_sort_segment(uint64_t* __restrict__ keys, int32_t* __restrict__ vals, int n) {
+__device__ __forceinline__ void wave_sort_segment(uint64_t* __restrict__ keys, int n) {
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t k[E];
-    int32_t v[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int i = lane * E + e;
         k[e] = (i < n) ? keys[i] : ~0ull;
-        v[e] = (i < n) ? vals[i] : -1;
     }
-    wave_bitonic_regs<E>(k, v);
+    wave_bitonic_regs<E>(k);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         const int i = lane * E + e;
-        if (i < n) {
-            keys[i] = k[e];
-            vals[i] = v[e];
-        }
+        if (i < n) keys[i] = k[e];
     }
 }
 
@@ -370,25 +367,26 @@ __global__ __launch_bounds__(256) void segment_max_kernel(const int32_t* __restr
 }
 
 __global__ __launch_bounds__(kWaveSortWaves * kWave) void sort_small_segments_kernel(
-    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
+    uint64_t* __restrict__ keys, const int32_t* __restrict__ seg_off, int n_segments,
     const us3d_stats* __restrict__ stats) {
     if (stats->overflow) return;
-    const int seg = blockIdx.x * kWaveSortWaves + threadIdx.x / kWave;
-    if (seg >= n_segments) return;  // wave-uniform
+    // wave-uniform, and stated so: the segment's size then selects the register network by a scalar branch
+    const int seg = blockIdx.x * kWaveSortWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    if (seg >= n_segments) return;
     const int s0 = seg_off[seg];
     const int n = seg_off[seg + 1] - s0;
     if (n <= 1 || n > kWaveSortCap) return;
-    if (n <= kWave) wave_sort_segment<1>(keys + s0, vals + s0, n);
-    else if (n <= 2 * kWave) wave_sort_segment<2>(keys + s0, vals + s0, n);
-    else if (n <= 4 * kWave) wave_sort_segment<4>(keys + s0, vals + s0, n);
-    else wave_sort_segment<8>(keys + s0, vals + s0, n);
+    if (n <= kWave) wave_sort_segment<1>(keys + s0, n);
+    else if (n <= 2 * kWave) wave_sort_segment<2>(keys + s0, n);
+    else if (n <= 4 * kWave) wave_sort_segment<4>(keys + s0, n);
+    else wave_sort_segment<8>(keys + s0, n);
 }
 
-// Ascending bitonic merge of a bitonic sequence of 64*E (key, value) pairs held by a wavefront (lane l holds
-// elements l*E .. l*E+E-1): the stages j = 32E .. 1 of the network's last level, in registers, exchanges across
-// lanes through the same shuffles as bitonic_regs.
+// Ascending bitonic merge of a bitonic sequence of 64*E keys held by a wavefront (lane l holds elements
+// l*E .. l*E+E-1): the stages j = 32E .. 1 of the network's last level, in registers, exchanges across lanes
+// through the same shuffles as bitonic_regs.
 template <int E>
-__device__ __forceinline__ void wave_bitonic_merge_regs(uint64_t (&k)[E], int32_t (&v)[E], int t) {
+__device__ __forceinline__ void wave_bitonic_merge_regs(uint64_t (&k)[E], int t) {
     constexpr int N = kWave * E;
 #pragma unroll
     for (int j = N >> 1; j > 0; j >>= 1) {
@@ -398,7 +396,6 @@ __device__ __forceinline__ void wave_bitonic_merge_regs(uint64_t (&k)[E], int32_
                 const int p = e ^ j;
                 if (p > e && k[e] > k[p]) {
                     const uint64_t tk = k[e]; k[e] = k[p]; k[p] = tk;
-                    const int32_t tv = v[e]; v[e] = v[p]; v[p] = tv;
                 }
             }
         } else {
@@ -409,27 +406,20 @@ __device__ __forceinline__ void wave_bitonic_merge_regs(uint64_t (&k)[E], int32_
                 const int klo = (int)(uint32_t)k[e], khi = (int)(uint32_t)(k[e] >> 32);
                 const uint32_t lo = (uint32_t)shfl_xor_pow2(klo, m);
                 const uint32_t hi = (uint32_t)shfl_xor_pow2(khi, m);
-                const int32_t ov = shfl_xor_pow2(v[e], m);
                 const uint64_t ok = ((uint64_t)hi << 32) | lo;
-                const bool take = ((i & j) == 0) ? (ok < k[e]) : (ok > k[e]);
-                if (take) {
-                    k[e] = ok;
-                    v[e] = ov;
-                }
+                const bool take = (ok < k[e]) != ((i & j) != 0);  // as in bitonic_regs
+                if (take) k[e] = ok;
             }
         }
     }
 }
 
 // Compare-exchange of LDS entries a < b, ascending.
-__device__ __forceinline__ void lds_cas(uint64_t* sk, int32_t* sv, int a, int b) {
+__device__ __forceinline__ void lds_cas(uint64_t* sk, int a, int b) {
     const uint64_t x = sk[a], y = sk[b];
     if (x > y) {
         sk[a] = y;
         sk[b] = x;
-        const int32_t t = sv[a];
-        sv[a] = sv[b];
-        sv[b] = t;
     }
 }
 
@@ -440,10 +430,9 @@ __device__ __forceinline__ void lds_cas(uint64_t* sk, int32_t* sv, int a, int b)
 constexpr int kMidThreads = 2 * kWave;
 
 __global__ __launch_bounds__(kMidThreads) void sort_mid_segments_kernel(
-    uint64_t* __restrict__ keys, int32_t* __restrict__ vals, const int32_t* __restrict__ seg_off, int n_segments,
+    uint64_t* __restrict__ keys, const int32_t* __restrict__ seg_off, int n_segments,
     const us3d_stats* __restrict__ stats) {
     __shared__ uint64_t sk[kWaveSortCapMid];
-    __shared__ int32_t sv[kWaveSortCapMid];
     if (stats->overflow) return;
     const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
     for (int it = 0; it < kLongSortSegments; ++it) {
@@ -451,50 +440,38 @@ __global__ __launch_bounds__(kMidThreads) void sort_mid_segments_kernel(
         if (seg >= n_segments) return;  // workgroup-uniform
         const int s0 = seg_off[seg], n = seg_off[seg + 1] - s0;
         if (n <= kWaveSortCap || n > kWaveSortCapMid) continue;
-        __syncthreads();  // a previous segment's reads of sk/sv are done
+        __syncthreads();  // a previous segment's reads of sk are done
         uint64_t k[8];
-        int32_t v[8];
         const int b = wave * kWaveSortCap + lane * 8;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int i = b + e;
             k[e] = (i < n) ? keys[s0 + i] : ~0ull;
-            v[e] = (i < n) ? vals[s0 + i] : -1;
         }
-        wave_bitonic_regs<8>(k, v);
+        wave_bitonic_regs<8>(k);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            sk[b + e] = k[e];
-            sv[b + e] = v[e];
-        }
+        for (int e = 0; e < 8; ++e) sk[b + e] = k[e];
         __syncthreads();
-        for (int q = threadIdx.x; q < kWaveSortCap; q += kMidThreads) lds_cas(sk, sv, q, kWaveSortCapMid - 1 - q);
+        for (int q = threadIdx.x; q < kWaveSortCap; q += kMidThreads) lds_cas(sk, q, kWaveSortCapMid - 1 - q);
         __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            k[e] = sk[b + e];
-            v[e] = sv[b + e];
-        }
-        wave_bitonic_merge_regs<8>(k, v, lane);
+        for (int e = 0; e < 8; ++e) k[e] = sk[b + e];
+        wave_bitonic_merge_regs<8>(k, lane);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int i = b + e;
-            if (i < n) {
-                keys[s0 + i] = k[e];
-                vals[s0 + i] = v[e];
-            }
+            if (i < n) keys[s0 + i] = k[e];
         }
     }
 }
 
-// Sorts n (1 <= n <= kMergeCap) keys with their values in LDS. The data is padded with ~0 keys to 1024, 2048 or
+// Sorts the keys of n (1 <= n <= kMergeCap) instances in LDS. The data is padded with ~0 keys to 1024, 2048 or
 // 4096 entries; the wavefronts sort runs of kWaveSortCap ascending in registers; then every level of the
 // bitonic merge doubles the sorted blocks: a first stage compares element i of a block's first half with
 // element kk-1-i (two ascending runs become two bitonic halves, the smaller elements in the first), then the
 // half-cleaners with partner distance >= kWaveSortCap run in LDS and the remaining ones in registers, each
-// wavefront merging whole 512-entry blocks. Ends with the sorted entries in sk/sv (the first n are the data).
-__device__ __forceinline__ void lds_merge_sort(const uint64_t* __restrict__ keys, const int32_t* __restrict__ vals,
-                                               int n, uint64_t* sk, int32_t* sv) {
+// wavefront merging whole 512-entry blocks. Ends with the sorted keys in sk (the first n are the data).
+__device__ __forceinline__ void lds_merge_sort(const uint64_t* __restrict__ keys, int n, uint64_t* sk) {
     constexpr int kWaves = kSortThreads / kWave;
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
     int padded = 2 * kWaveSortCap;
@@ -502,76 +479,57 @@ __device__ __forceinline__ void lds_merge_sort(const uint64_t* __restrict__ keys
     const int runs = padded / kWaveSortCap;
     for (int r = wave; r < runs; r += kWaves) {
         uint64_t k[8];
-        int32_t v[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int i = r * kWaveSortCap + lane * 8 + e;
             k[e] = (i < n) ? keys[i] : ~0ull;
-            v[e] = (i < n) ? vals[i] : -1;
         }
-        if (r * kWaveSortCap < n) wave_bitonic_regs<8>(k, v);  // wave-uniform
+        if (r * kWaveSortCap < n) wave_bitonic_regs<8>(k);  // wave-uniform
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            sk[r * kWaveSortCap + lane * 8 + e] = k[e];
-            sv[r * kWaveSortCap + lane * 8 + e] = v[e];
-        }
+        for (int e = 0; e < 8; ++e) sk[r * kWaveSortCap + lane * 8 + e] = k[e];
     }
     __syncthreads();
     const int half = padded >> 1;  // compare-exchanges per stage
     for (int kk = 2 * kWaveSortCap; kk <= padded; kk <<= 1) {
         for (int q = tid; q < half; q += kSortThreads) {  // first stage: i against kk-1-i within the block
             const int base = (q / (kk >> 1)) * kk, off = q % (kk >> 1);
-            lds_cas(sk, sv, base + off, base + kk - 1 - off);
+            lds_cas(sk, base + off, base + kk - 1 - off);
         }
         __syncthreads();
         for (int j = kk >> 2; j >= kWaveSortCap; j >>= 1) {
             for (int q = tid; q < half; q += kSortThreads) {
                 const int lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                lds_cas(sk, sv, lo, lo + j);
+                lds_cas(sk, lo, lo + j);
             }
             __syncthreads();
         }
         for (int r = wave; r < runs; r += kWaves) {  // distances < kWaveSortCap inside each 512-entry block
             uint64_t k[8];
-            int32_t v[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                k[e] = sk[r * kWaveSortCap + lane * 8 + e];
-                v[e] = sv[r * kWaveSortCap + lane * 8 + e];
-            }
-            wave_bitonic_merge_regs<8>(k, v, lane);
+            for (int e = 0; e < 8; ++e) k[e] = sk[r * kWaveSortCap + lane * 8 + e];
+            wave_bitonic_merge_regs<8>(k, lane);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                sk[r * kWaveSortCap + lane * 8 + e] = k[e];
-                sv[r * kWaveSortCap + lane * 8 + e] = v[e];
-            }
+            for (int e = 0; e < 8; ++e) sk[r * kWaveSortCap + lane * 8 + e] = k[e];
         }
         __syncthreads();
     }
 }
 
 // Sorts segment [s0, s0 + n) (n > kWaveSortCapMid) with the whole workgroup.
-__device__ __forceinline__ void sort_long_segment_body(uint64_t* __restrict__ keys, int32_t* __restrict__ vals,
-                                                       uint64_t* __restrict__ keys_tmp, int32_t* __restrict__ vals_tmp,
-                                                       int s0, int n, us3d_stats* stats, uint64_t* sk, int32_t* sv) {
+__device__ __forceinline__ void sort_long_segment_body(uint64_t* __restrict__ keys, uint64_t* __restrict__ keys_tmp,
+                                                       int s0, int n, us3d_stats* stats, uint64_t* sk) {
     uint64_t* k = keys + s0;
-    int32_t* v = vals + s0;
     if (n > kMergeCap && threadIdx.x == 0) atomicAdd(&stats->n_sorted_big, 1);
     // chunks of kMergeCap sorted in LDS (one chunk when n <= kMergeCap)
     for (int c = 0; c < n; c += kMergeCap) {
         const int m = min(kMergeCap, n - c);
-        lds_merge_sort(k + c, v + c, m, sk, sv);
-        for (int i = threadIdx.x; i < m; i += kSortThreads) {
-            k[c + i] = sk[i];
-            v[c + i] = sv[i];
-        }
+        lds_merge_sort(k + c, m, sk);
+        for (int i = threadIdx.x; i < m; i += kSortThreads) k[c + i] = sk[i];
         __syncthreads();
     }
     if (n <= kMergeCap) return;
     uint64_t* src_k = k;
-    int32_t* src_v = v;
     uint64_t* dst_k = keys_tmp + s0;
-    int32_t* dst_v = vals_tmp + s0;
     for (int width = kMergeCap; width < n; width <<= 1) {
         for (int i = threadIdx.x; i < n; i += kSortThreads) {
             const int run = i / width;
@@ -584,45 +542,35 @@ __device__ __forceinline__ void sort_long_segment_body(uint64_t* __restrict__ ke
                 const int mid = (lo + hi) >> 1;
                 if (src_k[partner_start + mid] < key) lo = mid + 1; else hi = mid;
             }
-            const int pos = pair_start + (i - run * width) + lo;
-            dst_k[pos] = key;
-            dst_v[pos] = src_v[i];
+            dst_k[pair_start + (i - run * width) + lo] = key;
         }
         __syncthreads();
         uint64_t* tk = src_k; src_k = dst_k; dst_k = tk;
-        int32_t* tv = src_v; src_v = dst_v; dst_v = tv;
     }
-    if (src_k != k) {
-        for (int i = threadIdx.x; i < n; i += kSortThreads) {
-            k[i] = src_k[i];
-            v[i] = src_v[i];
-        }
-    }
+    if (src_k != k)
+        for (int i = threadIdx.x; i < n; i += kSortThreads) k[i] = src_k[i];
 }
 
 // Segments kLongSortSegments * blockIdx.x + (0 .. kLongSortSegments-1): each one longer than kWaveSortCapMid
 // is sorted by the whole workgroup.
 __global__ __launch_bounds__(kSortThreads) void sort_segments_kernel(uint64_t* __restrict__ keys,
-                                                                     int32_t* __restrict__ vals,
                                                                      uint64_t* __restrict__ keys_tmp,
-                                                                     int32_t* __restrict__ vals_tmp,
                                                                      const int32_t* __restrict__ seg_off,
                                                                      int n_segments, us3d_stats* stats) {
     __shared__ uint64_t sk[kMergeCap];
-    __shared__ int32_t sv[kMergeCap];
     if (stats->overflow) return;
     for (int i = 0; i < kLongSortSegments; ++i) {
         const int seg = blockIdx.x * kLongSortSegments + i;
         if (seg >= n_segments) return;  // workgroup-uniform
         const int s0 = seg_off[seg], n = seg_off[seg + 1] - s0;
         if (n <= kWaveSortCapMid) continue;  // sorted by the wavefront kernels
-        __syncthreads();  // a previous segment's last reads of sk/sv are done
-        sort_long_segment_body(keys, vals, keys_tmp, vals_tmp, s0, n, stats, sk, sv);
+        __syncthreads();  // a previous segment's last reads of sk are done
+        sort_long_segment_body(keys, keys_tmp, s0, n, stats, sk);
     }
 }
 
-int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* vals_tmp, const int32_t* seg_off,
-                  int n_segments, us3d_stats* stats, int64_t k_capacity, hipStream_t stream) {
+int sort_segments(uint64_t* keys, uint64_t* keys_tmp, const int32_t* seg_off, int n_segments, us3d_stats* stats,
+                  int64_t k_capacity, hipStream_t stream) {
     (void)k_capacity;
     if (n_segments <= 0) return US3D_OK;
     hipLaunchKernelGGL(segment_max_kernel, dim3(std::min<unsigned>(blocks_for(n_segments, 256), 1024u)), dim3(256), 0,
@@ -631,19 +579,19 @@ int sort_segments(uint64_t* keys, int32_t* vals, uint64_t* keys_tmp, int32_t* va
     {
         StageTimer timer("sort_small_segments_kernel", stream);
         hipLaunchKernelGGL(sort_small_segments_kernel, dim3(blocks_for(n_segments, kWaveSortWaves)),
-                           dim3(kWaveSortWaves * kWave), 0, stream, keys, vals, seg_off, n_segments, stats);
+                           dim3(kWaveSortWaves * kWave), 0, stream, keys, seg_off, n_segments, stats);
     }
     int rc = check_launch("sort_small_segments_kernel");
     if (rc) return rc;
     {
         StageTimer timer("sort_mid_segments_kernel", stream);
         hipLaunchKernelGGL(sort_mid_segments_kernel, dim3(blocks_for(n_segments, kLongSortSegments)),
-                           dim3(kMidThreads), 0, stream, keys, vals, seg_off, n_segments, stats);
+                           dim3(kMidThreads), 0, stream, keys, seg_off, n_segments, stats);
     }
     if ((rc = check_launch("sort_mid_segments_kernel"))) return rc;
     StageTimer timer("sort_segments_kernel", stream);
     hipLaunchKernelGGL(sort_segments_kernel, dim3(blocks_for(n_segments, kLongSortSegments)), dim3(kSortThreads), 0,
-                       stream, keys, vals, keys_tmp, vals_tmp, seg_off, n_segments, stats);
+                       stream, keys, keys_tmp, seg_off, n_segments, stats);
     return check_launch("sort_segments_kernel");
 }
 
@@ -685,7 +633,7 @@ int us3d_stage_timing_read(char* buf, int64_t bytes) {
     return US3D_OK;
 }
 
-int us3d_version(void) { return 2; }
+int us3d_version(void) { return 3; }
 
 int64_t us3d_scan_tmp_bytes(int64_t n) { return us3d::scan_tmp_bytes(n); }
 

@@ -311,7 +311,7 @@ struct GpFwd {
 template <int TS>
 __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
-    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ pair_g, const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min,
     const us3d_stats* __restrict__ stats, float* __restrict__ image, int n_tiles, int32_t* __restrict__ queue_heads) {
     using S = GpFwd<TS>;
@@ -340,8 +340,8 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
             const int n = min(NT, s1 - base);
             __syncthreads();
             if (tid < n) {
-                const int g = (int)(uint32_t)seg_key[base + tid];
-                const int64_t p = seg_val[base + tid];  // the instance's visible pair
+                const int64_t p = (uint32_t)seg_key[base + tid];  // the instance's visible pair
+                const int g = pair_g[p];
                 s_f0[tid] = pair_foot[kFootF4 * p + 0];
                 s_f1[tid] = pair_foot[kFootF4 * p + 1];
                 s_I[tid] = rec[3 * (int64_t)g + 1].w;
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
 template <int TS>
 __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
-    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ pair_g, const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min, const float* __restrict__ d_image,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, int n_tiles,
     int32_t* __restrict__ queue_heads) {
@@ -423,8 +423,8 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         for (int it = tid; it < n_items; it += NT) {
             const int i = s0 + (it >> 1);
             const int half = it & 1;
-            const int g = (int)(uint32_t)seg_key[i];
-            const int pv = seg_val[i];  // the instance's visible pair
+            const int pv = (int)(uint32_t)seg_key[i];  // the instance's visible pair
+            const int g = pair_g[pv];
             const float4 a = pair_foot[kFootF4 * (int64_t)pv + 0], c = pair_foot[kFootF4 * (int64_t)pv + 1];  // (m0, m1, C i00, C i01), (C i11, r_lat, r_ax, b)
             // every lane walks all pixels of its half tile in the same order (uniform loops, columns unrolled); the
             // box test excludes the pixels outside the instance's box and pixels outside the frame have gradient 0
@@ -809,8 +809,8 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
                        dim3(tile_grid((const void*)gp_forward_kernel<TS>, GpFwd<TS>::kThreads, plan, heads, stream)), \
                        dim3(GpFwd<TS>::kThreads), 0, stream,                                                          \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
-                       gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
-                       gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames,         \
+                       gp_at<int32_t>(plan, L.tile_off), gp_at<int32_t>(plan, L.pair_g),                             \
+                       gp_at<uint64_t>(plan, L.seg_key), plan->frames, plan->frame_tile_base, plan->n_frames,        \
                        settings->weight_min,                                                                          \
                        gp_at<us3d_stats>(plan, L.stats), image, plan->n_tiles, heads)
     {
@@ -842,8 +842,8 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
                        dim3(tile_grid((const void*)gp_backward_kernel<TS>, TS * TS, plan, heads, stream)),            \
                        dim3(TS * TS), 0, stream,                                                                      \
                        gp_at<float4>(plan, L.records), gp_at<float4>(plan, L.pair_sect),                             \
-                       gp_at<int32_t>(plan, L.tile_off), gp_at<uint64_t>(plan, L.seg_key),                           \
-                       gp_at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames,         \
+                       gp_at<int32_t>(plan, L.tile_off), gp_at<int32_t>(plan, L.pair_g),                             \
+                       gp_at<uint64_t>(plan, L.seg_key), plan->frames, plan->frame_tile_base, plan->n_frames,        \
                        settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),                               \
                        gp_at<float4>(plan, L.partial), plan->n_tiles, heads)
         {

@@ -12,11 +12,11 @@
 //   2. exclusive scans of the words' popcounts give every visible pair its rank in both orders: v, the Gaussian-major
 //      rank (a Gaussian's visible pairs are consecutive, in frame order), and its slot in vis_list, the frame-major
 //      list of v (a frame's visible pairs are consecutive, in Gaussian order); list_visible_kernel writes vis_pair[v]
-//      (the dense index) and vis_list[slot] = v;
+//      (the dense index), pair_g[v] (its Gaussian) and vis_list[slot] = v;
 //   3. the renderer's setup pass (thread per v) writes the pair's instance count, tile rectangle, depth key and its
 //      own per-pair record, and counts the pair's tiles by rectangle corners;
 //   4. build_instances scans the counts (v order, so a Gaussian's instances are consecutive across frames), emits
-//      every instance into its tile's bucket and sorts each bucket.
+//      every instance into its tile's bucket as the key pair_dk[v] << 32 | v and sorts each bucket.
 // Every rank is a scan over fixed positions, so the lists are deterministic; the workspace holds 0.625 B per pair for
 // the masks and their scans, and the records of at most v_capacity visible pairs.
 #include <algorithm>
@@ -58,14 +58,13 @@ us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t
     L.pair_off = take((v_cap + 1) * 4);
     L.pair_rect = take(v_cap * 16);
     L.pair_dk = take(v_cap * 4);
+    L.pair_g = take(v_cap * 4);
     L.vis_list = take(v_cap * 4);
     L.tile_count = take(n_tiles * 4);
     L.tile_off = take((n_tiles + 1) * 4);
     L.tile_cursor = take(n_tiles * 4);
     L.seg_key = take(k_cap * 8);
-    L.seg_val = take(k_cap * 4);
     L.seg_key_tmp = take(k_cap * 8);
-    L.seg_val_tmp = take(k_cap * 4);
     L.records = take(n_gauss * record_bytes);
     L.partial = take(k_cap * partial_bytes);
     L.scan_tmp = take(std::max(std::max(scan_tmp_bytes(nw + 1), scan_tmp_bytes(nf + 1)),
@@ -141,11 +140,13 @@ __global__ void emit_base_kernel(int64_t v_cap, const us3d_stats* __restrict__ s
                                                 __int_as_float(r.z - r.x));
 }
 
-// Thread per dense pair: a visible pair writes its dense index at its compact index v and v at its frame-major slot.
+// Thread per dense pair: a visible pair writes its dense index and its Gaussian at its compact index v and v at its
+// frame-major slot.
 __global__ __launch_bounds__(256) void list_visible_kernel(
     int64_t n_pairs, const us3d_sweep* __restrict__ sweeps, int n_sweeps, const uint64_t* __restrict__ pair_bits,
     const int32_t* __restrict__ word_off, const uint32_t* __restrict__ fm_bits, const int32_t* __restrict__ fm_off,
-    const us3d_stats* __restrict__ stats, int32_t* __restrict__ vis_pair, int32_t* __restrict__ vis_list) {
+    const us3d_stats* __restrict__ stats, int32_t* __restrict__ vis_pair, int32_t* __restrict__ pair_g,
+    int32_t* __restrict__ vis_list) {
     const int64_t pair = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (pair >= n_pairs || stats->overflow) return;
     const uint64_t word = pair_bits[pair >> 6];
@@ -160,11 +161,12 @@ __global__ __launch_bounds__(256) void list_visible_kernel(
     const int64_t fm = fm_word_base(sweeps, s, f) * 32 + gl;
     const int slot = fm_off[fm >> 5] + __popc(fm_bits[fm >> 5] & ((1u << (fm & 31)) - 1u));
     vis_pair[v] = (int32_t)pair;
+    pair_g[v] = sw.g_begin + gl;
     vis_list[slot] = v;
 }
 
 // ---------------------------------------------------------------- emit
-// Write (depth key << 32 | Gaussian, visible pair v) of every instance into its tile's bucket. Workgroup w serves
+// Write the key (depth key << 32 | visible pair v) of every instance into its tile's bucket. Workgroup w serves
 // frame w / (bands * parts), band b and part p of it (w % (bands * parts) = b * parts + p): the tile rows
 // [b * rows, (b + 1) * rows), rows = ceil(tiles_y / bands), and the chunks p, p + parts, ... of the frame's run of
 // vis_list. bands and parts exceed 1 only when the batch has few frames, so the launch still fills the chip: bands
@@ -184,9 +186,9 @@ constexpr int kEmitTargetGroups = 1024;
 
 __global__ __launch_bounds__(kEmitThreads) void emit_instances_kernel(
     const us3d_frame* __restrict__ frames, const us3d_sweep* __restrict__ sweeps, const int32_t* __restrict__ fm_off,
-    const int32_t* __restrict__ vis_list, const int32_t* __restrict__ vis_pair, const int4* __restrict__ pair_rect,
-    const uint32_t* __restrict__ pair_dk, const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor,
-    uint64_t* __restrict__ seg_key, int32_t* __restrict__ seg_val, int bands, int parts, const us3d_stats* stats) {
+    const int32_t* __restrict__ vis_list, const int4* __restrict__ pair_rect, const uint32_t* __restrict__ pair_dk,
+    const int32_t* __restrict__ tile_off, int32_t* __restrict__ tile_cursor, uint64_t* __restrict__ seg_key, int bands,
+    int parts, const us3d_stats* stats) {
     __shared__ int32_t s_cur[kEmitLdsTiles];  // next free slot of each bucket of the band
     __shared__ int32_t s_q[kEmitQueue];       // visible pairs not yet expanded
     __shared__ int s_n;
@@ -243,21 +245,19 @@ __global__ __launch_bounds__(kEmitThreads) void emit_instances_kernel(
             const int v = tid < take ? s_q[n + tid] : -1;
             int count = 0;
             int4 rect = make_int4(0, 0, 0, 0);
-            uint32_t pay[3] = {0u, 0u, 0u};  // Gaussian, depth key, visible pair
+            uint32_t pay[2] = {0u, 0u};  // visible pair, depth key
             if (v >= 0) {
                 const int4 full = pair_rect[v];
                 rect = make_int4(full.x, max(full.y, y_lo), full.z, min(full.w, y_hi));  // clipped to the band
                 count = (rect.z - rect.x) * (rect.w - rect.y);
-                pay[0] = (uint32_t)(sw.g_begin + (int)(vis_pair[v] - sw.pair_base) / sw.f_count);
+                pay[0] = (uint32_t)v;
                 pay[1] = pair_dk[v];
-                pay[2] = (uint32_t)v;
             }
-            for_each_wave_instance<3>(count, rect, fr.tile_base, fr.tiles_x, pay,
+            for_each_wave_instance<2>(count, rect, fr.tile_base, fr.tiles_x, pay,
                                       [&](const uint32_t* o, int, int tile) {
                                           const int slot = local ? atomicAdd(&s_cur[tile - band_base], 1)
                                                                  : tile_off[tile] + atomicAdd(&tile_cursor[tile], 1);
                                           seg_key[slot] = ((uint64_t)o[1] << 32) | o[0];
-                                          seg_val[slot] = (int32_t)o[2];
                                       });
         }
         __syncthreads();  // every read of the queue is done
@@ -331,7 +331,8 @@ int rank_visible_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t 
         hipLaunchKernelGGL(list_visible_kernel, dim3(blocks_for(np, 256)), dim3(256), 0, stream, np, plan->sweeps,
                            plan->n_sweeps, ws_at<uint64_t>(plan, L.pair_bits), word_off,
                            ws_at<uint32_t>(plan, L.fm_bits), fm_off, (const us3d_stats*)stats,
-                           ws_at<int32_t>(plan, L.vis_pair), ws_at<int32_t>(plan, L.vis_list));
+                           ws_at<int32_t>(plan, L.vis_pair), ws_at<int32_t>(plan, L.pair_g),
+                           ws_at<int32_t>(plan, L.vis_list));
     }
     return check_launch("list_visible_kernel");
 }
@@ -384,16 +385,13 @@ int build_instances(const us3d_plan* plan, const us3d_layout& L, hipStream_t str
             hipLaunchKernelGGL(emit_instances_kernel, dim3((unsigned)(plan->n_frames * bands * parts)),
                                dim3(kEmitThreads), 0, stream, plan->frames, plan->sweeps,
                                ws_at<int32_t>(plan, L.fm_off), ws_at<int32_t>(plan, L.vis_list),
-                               ws_at<int32_t>(plan, L.vis_pair), ws_at<int4>(plan, L.pair_rect),
-                               ws_at<uint32_t>(plan, L.pair_dk), tile_off, tile_cursor,
-                               ws_at<uint64_t>(plan, L.seg_key), ws_at<int32_t>(plan, L.seg_val), bands, parts,
-                               (const us3d_stats*)stats);
+                               ws_at<int4>(plan, L.pair_rect), ws_at<uint32_t>(plan, L.pair_dk), tile_off, tile_cursor,
+                               ws_at<uint64_t>(plan, L.seg_key), bands, parts, (const us3d_stats*)stats);
         }
         if ((rc = check_launch("emit_instances_kernel"))) return rc;
     }
-    return sort_segments(ws_at<uint64_t>(plan, L.seg_key), ws_at<int32_t>(plan, L.seg_val),
-                         ws_at<uint64_t>(plan, L.seg_key_tmp), ws_at<int32_t>(plan, L.seg_val_tmp), tile_off,
-                         plan->n_tiles, stats, plan->k_capacity, stream);
+    return sort_segments(ws_at<uint64_t>(plan, L.seg_key), ws_at<uint64_t>(plan, L.seg_key_tmp), tile_off, plan->n_tiles,
+                         stats, plan->k_capacity, stream);
 }
 
 }  // namespace us3d

@@ -492,8 +492,8 @@ struct FwdLds {
 // tile's list is staged kBatch Gaussians at a time.
 template <int TS, bool kDepth, bool kPlaneBody, int kBatch>
 __device__ __forceinline__ void ug_forward_tile(
-    FwdLds<TS, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame& fr, int tx, int ty, int s0, int s1,
+    FwdLds<TS, kBatch>& L, const float4* __restrict__ rec, const int32_t* __restrict__ pair_g,
+    const uint64_t* __restrict__ seg_key, const us3d_frame& fr, int tx, int ty, int s0, int s1,
     const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, float* __restrict__ out_image, float* __restrict__ out_alpha,
@@ -546,7 +546,8 @@ __device__ __forceinline__ void ug_forward_tile(
         __syncthreads();
         const int n = min(kBatch, s1 - base);
         for (int j = tid; j < n; j += NT) {
-            const int g = (int)(uint32_t)seg_key[base + j];
+            const int64_t pv = (uint32_t)seg_key[base + j];  // the instance's visible pair
+            const int g = pair_g[pv];
             const float4 r0 = rec[3 * (int64_t)g + 0];
             const float4 r1 = rec[3 * (int64_t)g + 1];
             const float4 r2 = rec[3 * (int64_t)g + 2];
@@ -554,7 +555,6 @@ __device__ __forceinline__ void ug_forward_tile(
             const float e = dot3(vx, vy, vz, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2]);
             const float lat_g = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
             const float ax_g = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
-            const int64_t pv = seg_val[base + j];  // the instance's visible pair
             const float4 ps0 = pair_sect[kSectF4 * pv + 0], ps1 = pair_sect[kSectF4 * pv + 1];
             const uint32_t m = sub_mask<S::kSub>(L.sb, ps0, ps1, lat_g, ax_g, e, chi2w);
             L.mask[j] = m;
@@ -643,8 +643,8 @@ __device__ __forceinline__ void ug_forward_tile(
 template <int TS, bool kDepth, bool kPlane, int kBatch>
 __global__ __launch_bounds__(FwdShape<TS>::kThreads) __attribute__((amdgpu_waves_per_eu(FwdShape<TS>::kMinWavesPerSimd, 8)))
 void ug_forward_kernel(
-    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const int32_t* __restrict__ pair_g,
+    const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
@@ -660,12 +660,12 @@ void ug_forward_kernel(
         const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
         const int s0 = tile_off[t], s1 = tile_off[t + 1];
         if (kPlane && fr.geometry == 0)  // workgroup-uniform
-            ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels, pair_sect,
+            ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, pair_g, seg_key, fr, tx, ty, s0, s1, pixels, pair_sect,
                                                       chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
                                                       alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
                                                       inst_mask);
         else
-            ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, seg_key, seg_val, fr, tx, ty, s0, s1, pixels,
+            ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, pair_g, seg_key, fr, tx, ty, s0, s1, pixels,
                                                        pair_sect, chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
                                                        alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
                                                        inst_mask);
@@ -807,8 +807,8 @@ struct BwdLds {
 // is staged kBatch Gaussians at a time, slot j of a batch by thread j mod kThreads.
 template <int TS, bool kFull, bool kPlaneBody, bool kPlane, int kBatch>
 __device__ __forceinline__ void ug_backward_tile(
-    BwdLds<TS, kPlane, kBatch>& L, const float4* __restrict__ rec, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame& fr, int tx, int ty, int s0, int s1,
+    BwdLds<TS, kPlane, kBatch>& L, const float4* __restrict__ rec, const int32_t* __restrict__ pair_g,
+    const uint64_t* __restrict__ seg_key, const us3d_frame& fr, int tx, int ty, int s0, int s1,
     const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
@@ -885,11 +885,11 @@ __device__ __forceinline__ void ug_backward_tile(
             const int j = tid + i * NT;
             m[i] = 0;
             if (j < n) {
-                const int g = (int)(uint32_t)seg_key[base + j];
+                const int pv = (int)(uint32_t)seg_key[base + j];  // the instance's visible pair
+                const int g = pair_g[pv];
                 const float4 q0 = rec[3 * (int64_t)g + 0];
                 const float4 q1 = rec[3 * (int64_t)g + 1];
                 const float4 q2 = rec[3 * (int64_t)g + 2];
-                const int pv = seg_val[base + j];  // the instance's visible pair
                 const int64_t ei = emit_index_rec(pair_sect[kSectF4 * (int64_t)pv + 3], tx, ty);
                 m[i] = inst_mask[base + j];  // the sub-block mask the forward left for this instance
                 L.g0[j] = q0;
@@ -1136,8 +1136,8 @@ __device__ __forceinline__ void ug_backward_tile(
 template <int TS, bool kFull, bool kPlane, int kBatch>
 __global__ __launch_bounds__(TileShape<TS>::kThreads)
 __attribute__((amdgpu_waves_per_eu(TileShape<TS>::kBwdMinWavesPerSimd, 8))) void ug_backward_kernel(
-    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const us3d_frame* __restrict__ frames,
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const int32_t* __restrict__ pair_g,
+    const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux,
@@ -1155,11 +1155,11 @@ __attribute__((amdgpu_waves_per_eu(TileShape<TS>::kBwdMinWavesPerSimd, 8))) void
         const int s0 = tile_off[t], s1 = tile_off[t + 1];
         if (s0 == s1) return;  // workgroup-uniform
         if (kPlane && fr.geometry == 0)  // workgroup-uniform
-            ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, tx,
+            ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, pair_g, seg_key, fr, tx,
                                                                 ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
                                                                 aux, d_image, d_alpha, d_depth, partial, inst_mask);
         else
-            ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, seg_key, seg_val, fr, tx,
+            ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, pair_g, seg_key, fr, tx,
                                                                ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
                                                                aux, d_image, d_alpha, d_depth, partial, inst_mask);
     };
@@ -1236,7 +1236,8 @@ __global__ __launch_bounds__(256) void ug_reduce_kernel(int n_gauss, const us3d_
 __global__ void ug_pixel_lists_kernel(int64_t n_pixels, const us3d_frame* __restrict__ frames, int n_frames,
                                       const int32_t* __restrict__ frame_pixel_base, const us3d_sweep* __restrict__ sweeps,
                                       const float4* __restrict__ rec, const int32_t* __restrict__ tile_off,
-                                      const uint64_t* __restrict__ seg_key, const float* __restrict__ pixels,
+                                      const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ pair_g,
+                                      const float* __restrict__ pixels,
                                       float sqrt_chi2, int ts, int pass, int32_t* __restrict__ counts,
                                       const int32_t* __restrict__ offsets, int32_t* __restrict__ indices,
                                       const us3d_stats* __restrict__ stats) {
@@ -1254,7 +1255,7 @@ __global__ void ug_pixel_lists_kernel(int64_t n_pixels, const us3d_frame* __rest
     int c = 0;
     int w = pass ? offsets[p] : 0;
     for (int i = s0; i < s1; ++i) {
-        const int g = (int)(uint32_t)seg_key[i];
+        const int g = pair_g[(uint32_t)seg_key[i]];
         const float4 r0 = rec[3 * (int64_t)g + 0];
         const float4 r1 = rec[3 * (int64_t)g + 1];
         const float4 r2 = rec[3 * (int64_t)g + 2];
@@ -1416,8 +1417,8 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                        dim3(tile_grid((const void*)ug_forward_kernel<TS, DEPTH, PLANE, B>, FwdShape<TS>::kThreads,    \
                                       plan, heads, stream)),                                                          \
                        dim3(FwdShape<TS>::kThreads), 0, stream,                                                       \
-                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),     \
-                       at<int32_t>(plan, L.seg_val), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
+                       at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<int32_t>(plan, L.pair_g),       \
+                       at<uint64_t>(plan, L.seg_key), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,     \
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
@@ -1465,7 +1466,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                                       plan, heads, stream)),                                                          \
                        dim3(TileShape<TS>::kThreads), 0, stream, at<float4>(plan, L.records),                         \
                        at<int32_t>(plan, L.tile_off),                                                                 \
-                       at<uint64_t>(plan, L.seg_key), at<int32_t>(plan, L.seg_val), plan->frames,                     \
+                       at<int32_t>(plan, L.pair_g), at<uint64_t>(plan, L.seg_key), plan->frames,                      \
                        plan->frame_tile_base, plan->n_frames, pixels,                                                 \
                        at<float4>(plan, L.pair_sect),                                                                 \
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
@@ -1526,7 +1527,8 @@ int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_u
         StageTimer timer_("ug_pixel_lists_kernel", stream);
         hipLaunchKernelGGL(ug_pixel_lists_kernel, dim3(blocks_for(plan->n_pixels, 256)), dim3(256), 0, stream,
                            plan->n_pixels, plan->frames, plan->n_frames, frame_pixel_base, plan->sweeps,
-                           at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key), pixels,
+                           at<float4>(plan, L.records), at<int32_t>(plan, L.tile_off), at<uint64_t>(plan, L.seg_key),
+                           at<int32_t>(plan, L.pair_g), pixels,
                            sqrtf(settings->chi2_threshold), settings->tile_size, pass != 0, counts, offsets, indices,
                            at<us3d_stats>(plan, L.stats));
     }

@@ -490,7 +490,8 @@ __device__ __forceinline__ void ray_accumulate_line(const RayLine& G, const RayF
 template <bool kGrid>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void ur_forward_kernel(
     const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
+    const int32_t* __restrict__ pair_g, const us3d_frame* __restrict__ frames,
+    const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, const float* __restrict__ ray_o, const float* __restrict__ ray_d,
     const float* __restrict__ ray_off, RayFlags F, const us3d_stats* __restrict__ stats, float* __restrict__ image,
     float4* __restrict__ aux) {
@@ -543,10 +544,10 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(4, 
             __syncthreads();
             const int n = min(kRayThreads, s1 - base);
             if (tid < n) {
-                const uint64_t key = seg_key[base + tid];
+                const uint64_t key = seg_key[base + tid];  // depth key << 32 | the hit's visible pair
                 RayGauss G;
                 float R[9];
-                ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
+                ray_gauss(rec, pair_g[(uint32_t)key], (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
                 if constexpr (kGrid) {
                     RayLine Lh;
                     Lh.tg = G.tg; Lh.rd = G.rd; Lh.tau = G.tau; Lh.f_final = G.f_final;
@@ -735,8 +736,8 @@ __device__ __forceinline__ int wave_max_i(int v) {
 // Three waves per SIMD (at most 168 VGPRs): the echo-item phase would otherwise take the grid variant to 175 and two.
 template <bool kGrid>
 __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 8))) void ur_backward_kernel(
-    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const uint64_t* __restrict__ seg_key,
-    const int32_t* __restrict__ seg_val, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
+    const float4* __restrict__ rec, const int32_t* __restrict__ tile_off, const int32_t* __restrict__ pair_g,
+    const uint64_t* __restrict__ seg_key, const int32_t* __restrict__ pair_off, const int4* __restrict__ pair_rect,
     const us3d_frame* __restrict__ frames, const int32_t* __restrict__ frame_tile_base, int n_frames,
     const float* __restrict__ pixels, const float* __restrict__ ray_o, const float* __restrict__ ray_d,
     const float* __restrict__ ray_off, RayFlags F,
@@ -841,8 +842,9 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 
         const bool active = i < s1;
         RayGauss G;
         float R[9];
-        uint64_t key = active ? seg_key[i] : seg_key[s0];
-        ray_gauss(rec, (int)(uint32_t)key, (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
+        const uint64_t key = seg_key[active ? i : s0];  // depth key << 32 | the hit's visible pair
+        const int hv = (int)(uint32_t)key;
+        ray_gauss(rec, pair_g[hv], (uint32_t)(key >> 32), ro, rd, F.k_sigma, F.Rw, G, R);
         float dog[3] = {0.f, 0.f, 0.f}, ddg[3] = {0.f, 0.f, 0.f};
         float fin = 0.f, dtau = 0.f, dsh = 0.f;
         float u[3] = {0.f, 0.f, 0.f};
@@ -1092,7 +1094,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 
         if (!active) continue;
         // the record's rotation, Sigma^-1 and mean, formed again here rather than kept in registers through the
         // sample loop (the same operations as ray_gauss, so the same bits)
-        const int gi = (int)(uint32_t)key;
+        const int gi = pair_g[hv];
         const float4 r0 = rec[4 * (int64_t)gi + 0];
         const float4 r1 = rec[4 * (int64_t)gi + 1];
         const float4 r2 = rec[4 * (int64_t)gi + 2];
@@ -1168,7 +1170,7 @@ __global__ __launch_bounds__(kRayThreads) __attribute__((amdgpu_waves_per_eu(3, 
         }
         const float4 dq = quat_vjp(r2, dR);
         const float dt = dtau * G.tau * (1.0f - G.tau);
-        const int64_t e = emit_index(pair_off, pair_rect, seg_val[i], 0, x);  // scanline x of the pair's range
+        const int64_t e = emit_index(pair_off, pair_rect, hv, 0, x);  // scanline x of the pair's range
         partial[4 * e + 0] = make_float4(dmu[0], dmu[1], dmu[2], ds[0]);
         partial[4 * e + 1] = make_float4(ds[1], ds[2], dt, 0.f);
         partial[4 * e + 2] = dq;
@@ -1314,7 +1316,8 @@ int us3d_ur_forward(const us3d_plan* plan, const float* pixels, const float* ray
         // the line forms when the samples are the linear probe's grid (see ur_forward_kernel)
         auto kernel = pixels ? ur_forward_kernel<false> : ur_forward_kernel<true>;
         hipLaunchKernelGGL(kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, as_stream(stream_),
-                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
+                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off),
+                           ws_at<uint64_t>(plan, L.seg_key), ws_at<int32_t>(plan, L.pair_g),
                            plan->frames, plan->frame_tile_base, plan->n_frames, pixels, ray_origins, ray_dirs, ray_offsets,
                            flags_of(settings), ws_at<us3d_stats>(plan, L.stats), image, reinterpret_cast<float4*>(aux));
     }
@@ -1337,8 +1340,9 @@ int us3d_ur_backward(const us3d_plan* plan, const float* pixels, const float* ra
         // the grid form of the echo sums when the samples are the linear probe's grid (see ur_backward_kernel)
         auto kernel = pixels ? ur_backward_kernel<false> : ur_backward_kernel<true>;
         hipLaunchKernelGGL(kernel, dim3(plan->n_tiles), dim3(kRayThreads), 0, stream,
-                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off), ws_at<uint64_t>(plan, L.seg_key),
-                           ws_at<int32_t>(plan, L.seg_val), ws_at<int32_t>(plan, L.pair_off),
+                           ws_at<float4>(plan, L.records), ws_at<int32_t>(plan, L.tile_off),
+                           ws_at<int32_t>(plan, L.pair_g), ws_at<uint64_t>(plan, L.seg_key),
+                           ws_at<int32_t>(plan, L.pair_off),
                            ws_at<int4>(plan, L.pair_rect), plan->frames, plan->frame_tile_base, plan->n_frames, pixels,
                            ray_origins, ray_dirs, ray_offsets, flags_of(settings), reinterpret_cast<const float4*>(aux),
                            d_image, ws_at<us3d_stats>(plan, L.stats), ws_at<float4>(plan, L.partial));

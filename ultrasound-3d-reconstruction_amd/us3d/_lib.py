@@ -130,8 +130,8 @@ class GPSettings(ctypes.Structure):
 
 _LAYOUT_FIELDS = [
     "stats", "queue", "pair_bits", "word_cnt", "word_off", "fm_bits", "fm_cnt", "fm_off", "vis_pair",
-    "pair_count", "pair_off", "pair_rect", "pair_dk", "vis_list", "tile_count", "tile_off",
-    "tile_cursor", "seg_key", "seg_val", "seg_key_tmp", "seg_val_tmp", "records", "partial", "scan_tmp",
+    "pair_count", "pair_off", "pair_rect", "pair_dk", "pair_g", "vis_list", "tile_count", "tile_off",
+    "tile_cursor", "seg_key", "seg_key_tmp", "records", "partial", "scan_tmp",
     "inst_mask", "pair_sect", "total",
 ]
 

@@ -203,7 +203,9 @@ class UGBatch:
         ws = m.workspace
         K = int(read_stats(ws).n_instances)
         lay = self.layout
-        seg_key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64)
+        V = int(read_stats(ws).n_visible)
+        seg_key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64)  # depth key << 32 | visible pair v
+        pair_g = ws[off.pair_g: off.pair_g + 4 * V].view(torch.int32).to(torch.int64)
         tile_off = ws[off.tile_off: off.tile_off + 4 * (lay.n_tiles + 1)].view(torch.int32).to(torch.int64)
         out = {}
         sw_rows = lay.sweep_rows
@@ -214,7 +216,7 @@ class UGBatch:
             tiles = torch.repeat_interleave(torch.arange(t1 - t0, device=ws.device, dtype=torch.int64), counts)
             kk = seg_key[s0:s1]
             dk = (kk >> 32) & 0xFFFFFFFF
-            g = kk & 0xFFFFFFFF
+            g = pair_g[kk & 0xFFFFFFFF]
             sweep = int(self.tables.frames[f, 22].item())
             g_begin = int(sw_rows[sweep, 0])
             start = torch.where(counts > 0, tile_off[t0:t1] - s0, torch.full_like(counts, -1))

@@ -186,7 +186,9 @@ class URBatch:
                    "us3d_ur_workspace_layout")
         ws = m.workspace
         K = int(read_stats(ws).n_instances)
-        seg_key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64)
+        V = int(read_stats(ws).n_visible)
+        seg_key = ws[off.seg_key: off.seg_key + 8 * K].view(torch.int64)  # depth key << 32 | visible pair v
+        pair_g = ws[off.pair_g: off.pair_g + 4 * V].view(torch.int32).to(torch.int64)
         tile_off = ws[off.tile_off: off.tile_off + 4 * (lay.n_tiles + 1)].view(torch.int32).to(torch.int64)
         out = []
         for f in range(lay.n_frames):
@@ -197,7 +199,7 @@ class URBatch:
             kk = seg_key[s0:s1]
             g_begin = int(lay.sweep_rows[int(self.tables.frames[f, 22].item()), 0])
             out.append({"counts": counts.to(torch.int32), "offsets": offsets.to(torch.int32),
-                        "indices": ((kk & 0xFFFFFFFF) - g_begin).to(torch.int32),
+                        "indices": (pair_g[kk & 0xFFFFFFFF] - g_begin).to(torch.int32),
                         "sorted_depths": ((kk >> 32) & 0xFFFFFFFF).to(torch.float32) * 0.001})
         return out
 
