@@ -158,6 +158,7 @@ def test_grid_pixels_plane_form_parity(dev, ts):
         sect = m.workspace[off.pair_sect: off.pair_sect + 64 * V].view(torch.float32).view(V, 16).cpu().numpy()
         got_plane = np.stack([sect[:, 8], sect[:, 9], sect[:, 10], sect[:, 4], sect[:, 5], sect[:, 6]], 1)
         np.testing.assert_array_equal(got_plane.view(np.uint32), plane[vis].view(np.uint32))
+        np.testing.assert_array_equal(sect[:, 11].view(np.int32), np.flatnonzero(vis))  # each pair's Gaussian
         args = (scene.means.numpy(), scene.tril.numpy(), scene.opacity.numpy(), scene.intensity.numpy(), pix, geom,
                 lists, sem)
         img32, a32, d32, _ = oug.forward(*args, plane=plane)

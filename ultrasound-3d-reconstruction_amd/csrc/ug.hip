@@ -21,8 +21,9 @@ namespace us3d {
 
 // ---------------------------------------------------------------- workspace layout
 // Per visible pair, kSectF4 float4s in pair_sect: the pair's part of the sub-block test (s0), the plane form of q
-// (s1, s2; pair_section) and its emit base (s3: first emit index, rectangle x0, y0 and width, written once the counts
-// are scanned), so the forward (s0-s2) and the adjoint (s1-s3) each read one 64 B record.
+// (s1, s2; pair_section) with the pair's Gaussian in s2.w, and its emit base (s3: first emit index, rectangle x0, y0
+// and width, written once the counts are scanned), so the forward (s0-s2) and the adjoint (s1-s3) each read one
+// 64 B record, which also names the Gaussian record to gather.
 constexpr int kSectF4 = 4;
 
 static us3d_layout ug_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap) {
@@ -148,7 +149,7 @@ __device__ __forceinline__ float sub_pad(float lat_g, float ax_g) {
 // evaluation per pixel is two subtractions and five multiply-adds without cancellation. Set up in FP64 per
 // visible pair (the preprocess), evaluated in FP32 per (pixel, instance) by the splat kernels. With det M <= 0
 // (only for parallel axes or a singular factor) the minimum is not unique: x* = y* = 0, K = |c0|^2, s = 0.
-// Returns s1 = (p, qq, s, t_y = M01 / M00) and s2 = (cx, cy, K, 0).
+// Returns s1 = (p, qq, s, t_y = M01 / M00) and s2 = (cx, cy, K, 0); the setup pass stores the Gaussian in s2.w.
 __device__ __forceinline__ void pair_plane_form(double L11, double L12, double L13, double L22, double L23,
                                                 double L33, float mx, float my, float mz, float lat_g, float ax_g,
                                                 const us3d_frame& fr, double& M00, double& M01, double& M11,
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(256) void ug_setup_kernel(
                      s1, s2);
         pair_sect[kSectF4 * v + 0] = s0;
         pair_sect[kSectF4 * v + 1] = s1;
-        pair_sect[kSectF4 * v + 2] = s2;
+        pair_sect[kSectF4 * v + 2] = make_float4(s2.x, s2.y, s2.z, __int_as_float(g));  // .w: the pair's Gaussian
         add_rect_corners(tile_count, fr.tile_base, fr.tiles_x, fr.tiles_y, pt.rect);
     }
 }
@@ -547,7 +548,8 @@ __device__ __forceinline__ void ug_forward_tile(
         const int n = min(kBatch, s1 - base);
         for (int j = tid; j < n; j += NT) {
             const int64_t pv = (uint32_t)seg_key[base + j];  // the instance's visible pair
-            const int g = pair_g[pv];
+            const float4 ps2 = pair_sect[kSectF4 * pv + 2];   // plane form (cx, cy, K) and the pair's Gaussian
+            const int g = __float_as_int(ps2.w);
             const float4 r0 = rec[3 * (int64_t)g + 0];
             const float4 r1 = rec[3 * (int64_t)g + 1];
             const float4 r2 = rec[3 * (int64_t)g + 2];
@@ -560,7 +562,6 @@ __device__ __forceinline__ void ug_forward_tile(
             L.mask[j] = m;
             inst_mask[base + j] = (uint16_t)m;  // the adjoint reads it instead of computing it again
             if constexpr (kPlaneBody) {
-                const float4 ps2 = pair_sect[kSectF4 * pv + 2];
                 L.r0[j] = make_float4(ps2.x, ps2.y, ps2.z * kQScale, r0.w);  // cx, cy, K', opacity
                 L.r1[j] = make_float4(ps1.x * kQScaleSqrt, ps1.y * kQScaleSqrt, ps1.z * kQScaleSqrt,
                                       r1.x);  // p', qq', s', intensity
@@ -886,7 +887,8 @@ __device__ __forceinline__ void ug_backward_tile(
             m[i] = 0;
             if (j < n) {
                 const int pv = (int)(uint32_t)seg_key[base + j];  // the instance's visible pair
-                const int g = pair_g[pv];
+                const float4 ps2 = pair_sect[kSectF4 * (int64_t)pv + 2];  // plane form and the pair's Gaussian
+                const int g = __float_as_int(ps2.w);
                 const float4 q0 = rec[3 * (int64_t)g + 0];
                 const float4 q1 = rec[3 * (int64_t)g + 1];
                 const float4 q2 = rec[3 * (int64_t)g + 2];
@@ -897,7 +899,7 @@ __device__ __forceinline__ void ug_backward_tile(
                 L.g2[j] = q2;
                 L.ei[j] = (int32_t)ei;
                 if constexpr (kPlaneBody) {
-                    const float4 ps1 = pair_sect[kSectF4 * (int64_t)pv + 1], ps2 = pair_sect[kSectF4 * (int64_t)pv + 2];
+                    const float4 ps1 = pair_sect[kSectF4 * (int64_t)pv + 1];
                     L.g3[j] = make_float4(ps2.x, ps2.y, ps2.z * kQScale, q0.w);  // cx, cy, K', opacity
                     L.g4[j] = make_float4(ps1.x * kQScaleSqrt, ps1.y * kQScaleSqrt, ps1.z * kQScaleSqrt,
                                           q1.x);  // p', qq', s', intensity
