@@ -648,9 +648,13 @@ __device__ __forceinline__ float gp_sanitize_grad(float g) {
     return fminf(fmaxf(g, -kGpGradClamp), kGpGradClamp);
 }
 
-// One wavefront per Gaussian: its lanes take its visible pairs, v in [rank(p0), rank(p0 + f_count)) in frame order,
-// 64 at a time; each sums the pair's instance partials in emit order, applies the footprint's vector-Jacobian product
-// and sanitises the pair's gradients; a fixed butterfly sums the lanes.
+// kGpReduceLanes lanes per Gaussian (four Gaussians per wavefront: a Gaussian of a freehand sweep has about ten
+// visible pairs, so a wavefront per Gaussian left most lanes idle in the footprint's vector-Jacobian product). The
+// lanes take its visible pairs, v in [rank(p0), rank(p0 + f_count)) in frame order, kGpReduceLanes at a time; each
+// sums the pair's instance partials in emit order, applies the footprint's vector-Jacobian product and sanitises the
+// pair's gradients; a fixed butterfly over the Gaussian's lanes sums them.
+constexpr int kGpReduceLanes = 16;
+
 __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us3d_sweep* __restrict__ sweeps,
                                                         int n_sweeps, const us3d_frame* __restrict__ frames,
                                                         us3d_gp_settings st, const uint64_t* __restrict__ pair_bits,
@@ -664,9 +668,9 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
                                                         float* __restrict__ d_means, float* __restrict__ d_scales,
                                                         float* __restrict__ d_quats, float* __restrict__ d_opacity,
                                                         float* __restrict__ d_intensity) {
-    const int g = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (g >= n_gauss || stats->overflow) return;
+    const int g = (blockIdx.x * blockDim.x + threadIdx.x) / kGpReduceLanes;
+    const int lane = threadIdx.x & (kGpReduceLanes - 1);
+    if (g >= n_gauss || stats->overflow) return;  // uniform over a Gaussian's lanes
     const us3d_sweep sw = sweep_for_reduce(sweeps, n_sweeps, g);
     const int64_t p0 = sw.pair_base + (int64_t)(g - sw.g_begin) * sw.f_count;
     const int v0 = pair_rank(pair_bits, word_off, p0), v1 = pair_rank(pair_bits, word_off, p0 + sw.f_count);
@@ -675,7 +679,7 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
 #pragma unroll
     for (int k = 0; k < 11; ++k) acc[k] = 0.f;
     float accI = 0.f;
-    for (int v = v0 + lane; v < v1; v += kWave) {
+    for (int v = v0 + lane; v < v1; v += kGpReduceLanes) {
         const int cnt = pair_count[v];
         const int e0 = pair_off[v];
         float gb = 0.f, gI = 0.f, gm0 = 0.f, gm1 = 0.f, g00 = 0.f, g01 = 0.f, g11 = 0.f;
@@ -697,9 +701,11 @@ __global__ __launch_bounds__(256, 4) void gp_reduce_kernel(int n_gauss, const us
 #pragma unroll
         for (int k = 0; k < 11; ++k) acc[k] += gp_sanitize_grad(pacc[k]);
     }
+    for (int off = kGpReduceLanes / 2; off >= 1; off >>= 1) {  // inside the Gaussian's lanes
 #pragma unroll
-    for (int k = 0; k < 11; ++k) acc[k] = wave_sum(acc[k]);
-    accI = wave_sum(accI);
+        for (int k = 0; k < 11; ++k) acc[k] += __shfl_xor(acc[k], off, kWave);
+        accI += __shfl_xor(accI, off, kWave);
+    }
     if (lane == 0) {
         d_means[3 * (int64_t)g + 0] = acc[0];
         d_means[3 * (int64_t)g + 1] = acc[1];
@@ -860,8 +866,8 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
     if (plan->n_gauss > 0) {
         {
             StageTimer timer_("gp_reduce_kernel", stream);
-            hipLaunchKernelGGL(gp_reduce_kernel, dim3(blocks_for(plan->n_gauss, 4)), dim3(256), 0, stream,
-                               plan->n_gauss, plan->sweeps, plan->n_sweeps, plan->frames, *settings,
+            hipLaunchKernelGGL(gp_reduce_kernel, dim3(blocks_for(plan->n_gauss, 256 / kGpReduceLanes)), dim3(256), 0,
+                               stream, plan->n_gauss, plan->sweeps, plan->n_sweeps, plan->frames, *settings,
                                gp_at<uint64_t>(plan, L.pair_bits), gp_at<int32_t>(plan, L.word_off),
                                gp_at<int32_t>(plan, L.vis_pair), gp_at<int32_t>(plan, L.pair_count),
                                gp_at<int32_t>(plan, L.pair_off),
