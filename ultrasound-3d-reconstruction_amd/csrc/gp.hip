@@ -238,6 +238,23 @@ __device__ __forceinline__ int gp_pair_tiles(const GPFoot& F, const us3d_frame& 
     return (x1 - x0) * (y1 - y0);
 }
 
+// A pair gp_footprint's early cull rejects whatever the Gaussian's orientation: the footprint's elevational variance
+// a_ele^T Sigma a_ele is at most max(s)^2 (+ the 1e-6 it adds), so the base weight is at most
+// o exp(-ele^2 / (2 dmax)) 1.001 / sqrt(2 det_eps), dmax = max(s)^2 + 1e-6 + sigma_ele^2 + det_eps. The elevation is
+// formed as gp_footprint forms it; dmax takes 0.1 % more for the rounding of the projected variance and the bound
+// another 0.1 %, so a pair rejected here is rejected there. Costs the elevation and one exp, where the footprint
+// builds the rotation and the projected covariance.
+__device__ __forceinline__ bool gp_far_from_plane(const float4& r0, const float4& r1, const us3d_frame& fr,
+                                                  const us3d_gp_settings& st) {
+    const float* ae = fr.axis_ele;
+    const float d0 = r0.x - fr.origin[0], d1 = r0.y - fr.origin[1], d2 = r0.z - fr.origin[2];
+    const float ele = gp_dot3(d0, d1, d2, ae[0], ae[1], ae[2]);
+    const float smax = fmaxf(fmaxf(fmaxf(r1.x, r1.y), r1.z), 1e-6f);
+    const float dmax = smax * smax * 1.001f + 1e-6f + st.sigma_ele * st.sigma_ele + st.det_eps;
+    const float bound = r0.w * expf(-0.5f * ele * ele / dmax) * (1.001f / sqrtf(2.f * st.det_eps));
+    return bound * 1.001f < st.weight_min;  // NaN: false, the footprint decides
+}
+
 // Cull pass: thread per dense (frame, Gaussian) pair; marks the pairs with at least one tile.
 __global__ __launch_bounds__(256) void gp_cull_kernel(int64_t n_pairs, const float4* __restrict__ rec,
                                                       const us3d_frame* __restrict__ frames,
@@ -254,11 +271,14 @@ __global__ __launch_bounds__(256) void gp_cull_kernel(int64_t n_pairs, const flo
         const int g = sw.g_begin + gl;
         f = sw.f_begin + (local - gl * sw.f_count);
         const us3d_frame& fr = frames[f];
-        GPFoot F;
-        gp_footprint<false>(rec[3 * (int64_t)g + 0], rec[3 * (int64_t)g + 1], rec[3 * (int64_t)g + 2], fr,
-                            st.k_sigma, st.sigma_ele, st.det_eps, F, nullptr, st.weight_min);
-        int4 rect;
-        count = gp_pair_tiles(F, fr, st, rect);
+        const float4 r0 = rec[3 * (int64_t)g + 0], r1 = rec[3 * (int64_t)g + 1];
+        if (!gp_far_from_plane(r0, r1, fr, st)) {  // most pairs of a freehand sweep leave here
+            GPFoot F;
+            gp_footprint<false>(r0, r1, rec[3 * (int64_t)g + 2], fr, st.k_sigma, st.sigma_ele, st.det_eps, F, nullptr,
+                                st.weight_min);
+            int4 rect;
+            count = gp_pair_tiles(F, fr, st, rect);
+        }
     }
     mark_pair(count > 0, pair, n_pairs, sweeps, s, f, gl, pair_bits, word_cnt, fm_bits);
 }
