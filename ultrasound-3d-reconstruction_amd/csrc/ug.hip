@@ -990,7 +990,7 @@ __device__ __forceinline__ void ug_backward_tile(
                         const float qy = B.y * y;
                         const float v = B.z * y;
                         const float vv = __builtin_fmaf(v, v, A.z);
-                        float R0 = 0.f, RX = 0.f, RXX = 0.f;
+                        float tt[8];  // t of the row's pixels
 #pragma unroll
                         for (int jx = 0; jx < 8; ++jx) {
                             const float dN = (jx & 1) ? cur[jx >> 1].z : cur[jx >> 1].x;
@@ -1012,11 +1012,21 @@ __device__ __forceinline__ void ug_backward_tile(
                                 da = __builtin_fmaf(dN, B.w, -vi);
                             }
                             acw = __builtin_fmaf(dN, wp, acw);
-                            const float t = da * wp;
-                            const float X = (float)jx - 3.5f;  // constants once the loops are unrolled
-                            R0 += t;
-                            RX = __builtin_fmaf(t, X, RX);
-                            RXX = __builtin_fmaf(t, X * X, RXX);
+                            tt[jx] = da * wp;
+                        }
+                        // the row's moments in X = jx - 3.5 from the symmetric columns jx and 7 - jx: with
+                        // s = t_jx + t_(7-jx), d = t_(7-jx) - t_jx and Xh = 3.5 - jx, sum t = sum s,
+                        // sum t X = sum d Xh and sum t X^2 = sum s Xh^2 (five operations per two pixels)
+                        float R0 = tt[0] + tt[7];
+                        float RX = (tt[7] - tt[0]) * 3.5f;
+                        float RXX = R0 * 12.25f;
+#pragma unroll
+                        for (int h = 1; h < 4; ++h) {
+                            const float sp = tt[h] + tt[7 - h], dp = tt[7 - h] - tt[h];
+                            const float Xh = 3.5f - (float)h;  // constants once the loop is unrolled
+                            R0 += sp;
+                            RX = __builtin_fmaf(dp, Xh, RX);
+                            RXX = __builtin_fmaf(sp, Xh * Xh, RXX);
                         }
                         const float Y = (float)r - 3.5f;
                         T0 += R0;
