@@ -252,10 +252,11 @@ int check_plan(const us3d_plan* plan, int64_t ws_needed);
 
 // The map of every renderer (mapper.hip): begin_pairs resets the stats block, the tile counters and the frame-major
 // mask; the renderer's cull pass marks the visible pairs (mark_pair); rank_visible_pairs ranks them, checks
-// v_capacity and lists them (vis_pair, vis_list); the renderer's setup pass writes each visible pair's count, tile
-// rectangle, depth key and record and counts its tiles (add_rect_corners); build_instances scans the counts
+// v_capacity and lists them (vis_pair, pair_g, vis_list); the renderer's setup pass writes each visible pair's count,
+// tile rectangle, depth key and record and counts its tiles (add_rect_corners); build_instances scans the counts
 // (instance total into stats, overflow flag), scans the tile counts into offsets, emits the instances into tile
-// buckets (seg_val = the instance's visible pair) and sorts every bucket by (depth key, Gaussian).
+// buckets (seg_key = depth key << 32 | the instance's visible pair) and sorts every bucket, which orders it by
+// (depth key, Gaussian).
 int begin_pairs(const us3d_plan* plan, const us3d_layout& L, hipStream_t stream);
 
 // Grid of a splat launch over the plan's tiles: n_tiles workgroups, or with per-XCD work queues (heads != NULL) the
