@@ -57,6 +57,8 @@ def parse(argv=None):
     ap.add_argument("--cpu-sample-rows", type=int, default=2)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
+    ap.add_argument("--separate-loss", action="store_true",
+                    help="slice steps with the image loss as its own kernels instead of fused into the splats")
     ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan", "gaussianpile"), default="slice",
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
                          "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
@@ -545,7 +547,8 @@ def bench_slice(args):
     init = [p.detach() for p in (means, tril, raw_o, raw_c)]
 
     reset_peak(dev)
-    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9)
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9,
+                                fused_loss=not args.separate_loss)
     dt_b = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():  # sticky over every replay since the workspace was allocated
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
@@ -558,7 +561,8 @@ def bench_slice(args):
     torch.cuda.empty_cache()
 
     # per-kernel durations of the same launch sequence (eager, HIP event pair around every launch)
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9,
+                           fused_loss=not args.separate_loss)
     tt.step()
 
     def run():
@@ -575,7 +579,7 @@ def bench_slice(args):
     from dataclasses import replace as _replace
 
     to = SliceSplatTrainer(*init, frames, gt, _replace(cfg, work_queue=not cfg.work_queue), use_graph=False,
-                           check_every=10 ** 9)
+                           check_every=10 ** 9, fused_loss=not args.separate_loss)
     to.step()
 
     def run_other():
@@ -808,6 +812,7 @@ def bench_fan(args):
     init = [scene.means.to(dev), scene.tril.to(dev), logit(scene.opacity).to(dev), logit(scene.intensity).to(dev)]
 
     trainer = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=not args.eager, check_every=10 ** 9,
+                                fused_loss=not args.separate_loss,
                                 chunk_frames=chunk)
     dt = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():
@@ -820,7 +825,8 @@ def bench_fan(args):
         f"frames/s, peak HBM {peak / 2**30:.1f} GiB, workspace {ws_bytes / 2**30:.1f} GiB")
     del trainer
     torch.cuda.empty_cache()
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9, chunk_frames=chunk)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, check_every=10 ** 9,
+                           fused_loss=not args.separate_loss, chunk_frames=chunk)
     tt.step()
     stages = stage_timing(tt.step)
     K, g_vis = 0, 0  # instances and visible pairs summed over the chunks
@@ -1012,7 +1018,8 @@ def bench_multisweep(args):
     cfg = apply(cfg, tune)
     log(f"multisweep autotuner: {tune}")
     torch.cuda.empty_cache()
-    trainer = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=not args.eager, check_every=10 ** 9)
+    trainer = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=not args.eager, check_every=10 ** 9,
+                                fused_loss=not args.separate_loss)
     dt = timed(trainer.step, args.steps, args.warmup)
     if trainer.overflowed():
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
@@ -1023,7 +1030,8 @@ def bench_multisweep(args):
         f"{fps:.1f} frames/s, workspace {ws_bytes / 2**30:.1f} GiB")
     del trainer
     torch.cuda.empty_cache()
-    tt = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False, check_every=10 ** 9)
+    tt = SliceSplatTrainer(*init, frames, gt, cfg, sweeps=sweeps, use_graph=False, check_every=10 ** 9,
+                           fused_loss=not args.separate_loss)
     tt.step()
     stages = stage_timing(tt.step)
     K = int(read_stats(tt.workspace).n_instances)

@@ -211,6 +211,24 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                      const float* d_alpha, const float* d_depth, float* d_means, float* d_tril,
                      float* d_opacity, float* d_intensity, void* stream);
 
+/* Training step with the image loss fused into the splat: the pair us3d_ug_forward + us3d_image_loss +
+ * us3d_ug_backward(d_image = the loss gradient, d_alpha = d_depth = NULL) of the training loop
+ * (pipeline_god_file.py:739-764 around image_loss, :604-637), without the two per-pixel loss passes.
+ * us3d_ug_forward_loss renders as us3d_ug_forward and, in the same kernel, sums the frame statistics of the loss
+ * against gt (indexed like the image) per (tile, wavefront); a small kernel then writes loss[n_frames] and the
+ * frames' gradient coefficients into tmp (us3d_ug_loss_tmp_bytes(plan) bytes; scale, w_l1 and w_ssim as in
+ * us3d_image_loss). us3d_ug_backward_loss is us3d_ug_backward with d_image formed from the clamped image, gt and
+ * those coefficients while the adjoint stages its pixels; it needs the same plan, gt and tmp. The FP64 sums run in
+ * a different (fixed) order than us3d_image_loss's, so loss and gradients agree with the unfused sequence to
+ * rounding, and each is bitwise reproducible. No counterpart in the reference (its loss is torch ops). */
+int64_t us3d_ug_loss_tmp_bytes(const us3d_plan* plan);
+int us3d_ug_forward_loss(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
+                         float* image, float* alpha, float* depth, float* aux, const float* gt, float w_l1,
+                         float w_ssim, float scale, float* loss, double* tmp, void* stream);
+int us3d_ug_backward_loss(const us3d_plan* plan, const float* pixels, const float* tril,
+                          const us3d_ug_settings* settings, const float* aux, const float* gt, const double* tmp,
+                          float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream);
+
 /* Per-pixel candidate lists (the reference's pixel_gauss_counts/offsets/indices extras,
  * core.cu:166-287): candidates of each pixel in sorted tile order filtered by the lateral/axial window.
  * counts [n_pixels] is written by pass 0; offsets [n_pixels] must be the exclusive scan of counts

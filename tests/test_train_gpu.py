@@ -68,6 +68,30 @@ def test_fused_loss_and_gradient_match_torch(dev):
     assert rel_l2(d.cpu().numpy(), x.grad.numpy()) < 1e-5
 
 
+@pytest.mark.parametrize("tile", [8, 16, 32])
+@pytest.mark.parametrize("chunk", [None, 2])
+def test_loss_fused_into_the_splat_equals_the_separate_loss_kernels(dev, tile, chunk):
+    """us3d_ug_forward_loss / us3d_ug_backward_loss (loss sums in the forward's epilogue, the loss gradient formed in
+    the adjoint's pixel staging) against us3d_ug_forward + us3d_image_loss + us3d_ug_backward: the same losses and
+    gradients up to the FP64 summation order, at every tile size (one or four wavefronts per tile) and with the
+    frames split into chunks (target and loss offsets per chunk)."""
+    frames, gt, cfg, init = _setup(dev, n_frames=5)
+    cfg = replace(cfg, tile_size=tile)
+    a = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, lr=0.0, chunk_frames=chunk, fused_loss=True)
+    b = SliceSplatTrainer(*init, frames, gt, cfg, use_graph=False, lr=0.0, chunk_frames=chunk, fused_loss=False)
+    a.step()
+    b.step()
+    torch.cuda.synchronize()
+    la, lb = a.losses().double().cpu().numpy(), b.losses().double().cpu().numpy()
+    assert np.all(lb > 0)
+    assert rel_l2(la, lb) < 1e-6, (la, lb)
+    for name, ga, gb in zip(("means", "tril", "raw_opacity", "raw_intensity"), a.grads, b.grads):
+        ref = gb.detach().cpu().numpy()
+        assert np.abs(ref).max() > 0, name
+        err = rel_l2(ga.detach().cpu().numpy(), ref)
+        assert err < 1e-6, f"{name}: rel L2 {err:.3e}"
+
+
 def test_adam_kernel_matches_torch_adam(dev):
     g = torch.Generator(device="cpu").manual_seed(0)
     p0 = torch.randn(10_000, generator=g).to(dev)

@@ -222,6 +222,34 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+// v of lane ^ m (m a power of two < 64, a constant once the caller is unrolled): DPP quad_perm for m = 1, 2,
+// DPP row_ror:8 for m = 8 (rotation by 8 inside a 16-lane row is xor 8), ds_swizzle in bitmask mode for m = 4,
+// 16 (source lane ((lane & 31) ^ m) inside each 32-lane half), ds_bpermute for m = 32. Every lane must be active.
+__device__ __forceinline__ int shfl_xor_pow2(int v, int m) {
+    switch (m) {
+        case 1: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+        case 2: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+        case 4: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (4 << 10));
+        case 8: return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (16 << 10));
+        default: return __shfl_xor(v, m, kWave);
+    }
+}
+
+// Sum of v over the wavefront in a fixed order, the same value in every lane: xor butterflies over the
+// distances 1 .. 16 (both halves of the double through shfl_xor_pow2), then the sums of the two 32-lane halves
+// read from lanes 0 and 32 and added. Every lane must be active.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int m = 1; m < 32; m <<= 1)
+        v += __hiloint2double(shfl_xor_pow2(__double2hiint(v), m), shfl_xor_pow2(__double2loint(v), m));
+    const double a = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 0),
+                                      __builtin_amdgcn_readlane(__double2loint(v), 0));
+    const double b = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 32),
+                                      __builtin_amdgcn_readlane(__double2loint(v), 32));
+    return a + b;
+}
+
 // ---------------------------------------------------------------- shared pipeline pieces
 // Exclusive scan (int64 accumulation, int32 output) with the total stored to *total.
 int scan_i32(const int32_t* in, int32_t* out, int64_t n, int64_t* total, void* tmp, hipStream_t stream);
@@ -231,6 +259,23 @@ int64_t scan_tmp_bytes(int64_t n);
 // same size for segments longer than 4096.
 int sort_segments(uint64_t* keys, uint64_t* keys_tmp, const int32_t* seg_off, int n_segments, us3d_stats* stats,
                   int64_t k_capacity, hipStream_t stream);
+
+// Image loss of a training step fused into a slice splat (train.hip holds the frame statistics and their
+// gradient): the forward reads the targets gt (indexed like the image) and writes, per (tile, wavefront), the six
+// FP64 sums sum|x-y|, sum x, sum y, sum x^2, sum y^2, sum xy of its pixels (x the clamped image) into part; the
+// finish reduces them per frame into the loss and four gradient coefficients per frame (coeff), from which the
+// adjoint forms dL/dx = cs sign(x-y) + ca + cx x + cy y while it stages its pixels. NULL pointers: no loss.
+struct LossFuse {
+    const float* gt;
+    double* part;
+    const double* coeff;
+};
+constexpr int kLossSums = 6;
+// Per-frame loss and coefficients from the per-(tile, wavefront) sums of a fused forward: frame f owns the
+// entries [frame_tile_base[f] * waves_per_tile, frame_tile_base[f + 1] * waves_per_tile) of part.
+int image_loss_from_tiles(const double* part, const int32_t* frame_tile_base, const int32_t* frame_pixel_base,
+                          int n_frames, int waves_per_tile, float w_l1, float w_ssim, float scale, float* loss,
+                          double* coeff, hipStream_t stream);
 
 // Workspace layout shared by the renderers (record, partial, mask and per-pair record sizes differ).
 us3d_layout ws_layout(int64_t n_gauss, int64_t n_pairs, int64_t n_tiles, int64_t k_cap, int64_t v_cap,

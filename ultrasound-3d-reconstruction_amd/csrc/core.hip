@@ -271,20 +271,6 @@ constexpr int kLongSortSegments = 4;   // segments examined by one workgroup of 
 // are compile-time constants, so shuffles and register indices are static). Stages whose partner index
 // differs only in the low log2(E) bits exchange inside a lane; the others exchange with lane ^ (j / E)
 // through cross-lane shuffles. Keys must be unique.
-// v of lane ^ m (m a power of two < 64, a constant once the caller is unrolled): DPP quad_perm for m = 1, 2,
-// DPP row_ror:8 for m = 8 (rotation by 8 inside a 16-lane row is xor 8), ds_swizzle in bitmask mode for m = 4,
-// 16 (source lane ((lane & 31) ^ m) inside each 32-lane half), ds_bpermute for m = 32.
-__device__ __forceinline__ int shfl_xor_pow2(int v, int m) {
-    switch (m) {
-        case 1: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-        case 2: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-        case 4: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (4 << 10));
-        case 8: return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror:8
-        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x1F | (16 << 10));
-        default: return __shfl_xor(v, m, kWave);
-    }
-}
-
 // kDpp: shuffles through shfl_xor_pow2 (only where the network unrolls completely, so every distance is a
 // constant); otherwise ds_bpermute.
 template <int E, bool kDpp>
@@ -332,13 +318,16 @@ __device__ __forceinline__ void wave_bitonic_regs(uint64_t (&k)[E]) {
     bitonic_regs<E, (E <= 8)>(k, (int)(threadIdx.x & (kWave - 1)));
 }
 
+// The network sorts whatever it is given, so the keys are loaded lane-interleaved (element e*64 + lane: one
+// 512-byte row per load) and stored in the network's layout (lane l's elements l*E .. l*E+E-1). Every lane has
+// loaded all its keys before any store.
 template <int E>
 __device__ __forceinline__ void wave_sort_segment(uint64_t* __restrict__ keys, int n) {
     const int lane = threadIdx.x & (kWave - 1);
     uint64_t k[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-        const int i = lane * E + e;
+        const int i = e * kWave + lane;
         k[e] = (i < n) ? keys[i] : ~0ull;
     }
     wave_bitonic_regs<E>(k);
@@ -444,8 +433,8 @@ __global__ __launch_bounds__(kMidThreads) void sort_mid_segments_kernel(
         uint64_t k[8];
         const int b = wave * kWaveSortCap + lane * 8;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = b + e;
+        for (int e = 0; e < 8; ++e) {  // the wavefront's half, loaded lane-interleaved (any order sorts)
+            const int i = wave * kWaveSortCap + e * kWave + lane;
             k[e] = (i < n) ? keys[s0 + i] : ~0ull;
         }
         wave_bitonic_regs<8>(k);
@@ -480,8 +469,8 @@ __device__ __forceinline__ void lds_merge_sort(const uint64_t* __restrict__ keys
     for (int r = wave; r < runs; r += kWaves) {
         uint64_t k[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int i = r * kWaveSortCap + lane * 8 + e;
+        for (int e = 0; e < 8; ++e) {  // the run, loaded lane-interleaved (any order sorts)
+            const int i = r * kWaveSortCap + e * kWave + lane;
             k[e] = (i < n) ? keys[i] : ~0ull;
         }
         if (r * kWaveSortCap < n) wave_bitonic_regs<8>(k);  // wave-uniform

@@ -47,8 +47,25 @@ __global__ __launch_bounds__(kLossThreads) void loss_partial_kernel(const float*
     }
 }
 
-// Pass 2 (one thread per frame): frame statistics -> loss and the per-frame gradient coefficients
+// Frame statistics (the six sums over the frame's P pixels) -> loss and the frame's gradient coefficients
 // d/dx_i = cs * sign(x_i - y_i) + ca + cb * x_i + cc * y_i.
+__device__ void finish_frame(const double (&s)[6], double P, float w_l1, float w_ssim, float scale, float* loss,
+                             double* coeff) {
+    const double c1 = 0.01 * 0.01, c2 = 0.03 * 0.03;
+    const double mx = s[1] / P, my = s[2] / P;
+    const double vx = s[3] / P - mx * mx, vy = s[4] / P - my * my, vxy = s[5] / P - mx * my;
+    const double A = 2 * mx * my + c1, B = 2 * vxy + c2, C = mx * mx + my * my + c1, D = vx + vy + c2;
+    const double ssim = (A * B) / (C * D);
+    *loss = (float)(w_l1 * s[0] / P + w_ssim * (1.0 - ssim));
+    // d ssim / d x_i = ssim * [2 my/(A P) + 2 (y_i - my)/(B P) - 2 mx/(C P) - 2 (x_i - mx)/(D P)]
+    const double k = -w_ssim * ssim * scale / P;
+    coeff[0] = w_l1 * scale / P;
+    coeff[1] = k * (2 * my / A - 2 * my / B - 2 * mx / C + 2 * mx / D);
+    coeff[2] = k * (-2.0 / D);
+    coeff[3] = k * (2.0 / B);
+}
+
+// Pass 2 (one thread per frame): the frame's block partials summed in block order, then finish_frame.
 __global__ void loss_finish_kernel(const int32_t* __restrict__ frame_base, int n_frames, int n_blocks, float w_l1,
                                    float w_ssim, float scale, const double* __restrict__ partial,
                                    float* __restrict__ loss, double* __restrict__ coeff) {
@@ -57,19 +74,38 @@ __global__ void loss_finish_kernel(const int32_t* __restrict__ frame_base, int n
     double s[6] = {0, 0, 0, 0, 0, 0};
     for (int b = 0; b < n_blocks; ++b)
         for (int k = 0; k < 6; ++k) s[k] += partial[((int64_t)f * n_blocks + b) * 8 + k];
-    const double P = (double)(frame_base[f + 1] - frame_base[f]);
-    const double c1 = 0.01 * 0.01, c2 = 0.03 * 0.03;
-    const double mx = s[1] / P, my = s[2] / P;
-    const double vx = s[3] / P - mx * mx, vy = s[4] / P - my * my, vxy = s[5] / P - mx * my;
-    const double A = 2 * mx * my + c1, B = 2 * vxy + c2, C = mx * mx + my * my + c1, D = vx + vy + c2;
-    const double ssim = (A * B) / (C * D);
-    loss[f] = (float)(w_l1 * s[0] / P + w_ssim * (1.0 - ssim));
-    // d ssim / d x_i = ssim * [2 my/(A P) + 2 (y_i - my)/(B P) - 2 mx/(C P) - 2 (x_i - mx)/(D P)]
-    const double k = -w_ssim * ssim * scale / P;
-    coeff[4 * f + 0] = w_l1 * scale / P;
-    coeff[4 * f + 1] = k * (2 * my / A - 2 * my / B - 2 * mx / C + 2 * mx / D);
-    coeff[4 * f + 2] = k * (-2.0 / D);
-    coeff[4 * f + 3] = k * (2.0 / B);
+    finish_frame(s, (double)(frame_base[f + 1] - frame_base[f]), w_l1, w_ssim, scale, loss + f, coeff + 4 * f);
+}
+
+// The same from the per-(tile, wavefront) sums of a fused forward (kLossSums doubles each): one workgroup per
+// frame, thread i sums the entries i, i + 256, ... of the frame's run, then a fixed tree over the threads, so the
+// result is bitwise reproducible.
+__global__ __launch_bounds__(kLossThreads) void loss_finish_tiles_kernel(
+    const double* __restrict__ part, const int32_t* __restrict__ frame_tile_base,
+    const int32_t* __restrict__ frame_pixel_base, int waves_per_tile, float w_l1, float w_ssim, float scale,
+    float* __restrict__ loss, double* __restrict__ coeff) {
+    const int f = blockIdx.x;
+    const int64_t e0 = (int64_t)frame_tile_base[f] * waves_per_tile, e1 = (int64_t)frame_tile_base[f + 1] * waves_per_tile;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += kLossThreads)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) s[k] += part[e * kLossSums + k];
+    __shared__ double red[6][kLossThreads];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = s[k];
+    for (int h = kLossThreads / 2; h > 0; h >>= 1) {
+        __syncthreads();
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+    }
+    if (threadIdx.x == 0) {
+        double t[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) t[k] = red[k][0];
+        finish_frame(t, (double)(frame_pixel_base[f + 1] - frame_pixel_base[f]), w_l1, w_ssim, scale, loss + f,
+                     coeff + 4 * f);
+    }
 }
 
 // Pass 3: d/dx_i for every pixel; blockIdx.y is the frame, so each block reads its frame's four coefficients once.
@@ -115,6 +151,15 @@ __global__ void step_increment_kernel(int32_t* step, const int32_t* skip) {
 }
 
 }  // namespace
+
+int image_loss_from_tiles(const double* part, const int32_t* frame_tile_base, const int32_t* frame_pixel_base,
+                          int n_frames, int waves_per_tile, float w_l1, float w_ssim, float scale, float* loss,
+                          double* coeff, hipStream_t stream) {
+    if (n_frames <= 0) return US3D_OK;
+    hipLaunchKernelGGL(loss_finish_tiles_kernel, dim3(n_frames), dim3(kLossThreads), 0, stream, part,
+                       frame_tile_base, frame_pixel_base, waves_per_tile, w_l1, w_ssim, scale, loss, coeff);
+    return check_launch("loss_finish_tiles_kernel");
+}
 
 }  // namespace us3d
 

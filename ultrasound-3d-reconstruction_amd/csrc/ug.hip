@@ -498,7 +498,8 @@ __device__ __forceinline__ void ug_forward_tile(
     const float* __restrict__ pixels,
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, float* __restrict__ out_image, float* __restrict__ out_alpha,
-    float* __restrict__ out_depth, float4* __restrict__ out_aux, uint16_t* __restrict__ inst_mask) {
+    float* __restrict__ out_depth, float4* __restrict__ out_aux, uint16_t* __restrict__ inst_mask,
+    const LossFuse& loss) {
     using S = FwdShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
@@ -542,7 +543,6 @@ __device__ __forceinline__ void ug_forward_tile(
         }
         S_[k] = N_[k] = D_[k] = 0.f;
     }
-
     for (int base = s0; base < s1; base += kBatch) {
         __syncthreads();
         const int n = min(kBatch, s1 - base);
@@ -623,16 +623,38 @@ __device__ __forceinline__ void ug_forward_tile(
         }
     }
 
+    double ls[kLossSums] = {0, 0, 0, 0, 0, 0};  // the loss sums of the lane's pixels (fused training step)
+    float gtv[PPT];  // their targets, loaded before the image is formed so the loads overlap it
+#pragma unroll
+    for (int k = 0; k < PPT; ++k) gtv[k] = (loss.gt && valid[k]) ? loss.gt[pidx[k]] : 0.f;
 #pragma unroll
     for (int k = 0; k < PPT; ++k) {
         if (!valid[k]) continue;
         const float A = S_[k] + bg_alpha;
         const float num = __builtin_fmaf(bg_alpha, bg_intensity, N_[k]);
         const float img = num / fmaxf(A, 1e-8f);
-        out_image[pidx[k]] = fmaxf(0.f, fminf(1.f, img));
+        const float pred = fmaxf(0.f, fminf(1.f, img));
+        out_image[pidx[k]] = pred;
         out_alpha[pidx[k]] = alpha_excludes_bg ? S_[k] : A;
         if constexpr (kDepth) out_depth[pidx[k]] = D_[k] / fmaxf(S_[k], 1e-8f);
         out_aux[pidx[k]] = make_float4(S_[k], img, kDepth ? D_[k] : 0.f, 0.f);
+        if (loss.gt) {
+            const double x = pred, y = gtv[k];
+            ls[0] += fabs(x - y);
+            ls[1] += x;
+            ls[2] += y;
+            ls[3] += x * x;
+            ls[4] += y * y;
+            ls[5] += x * y;
+        }
+    }
+    if (loss.gt) {  // uniform: one entry per (tile, wavefront), the lanes' sums combined in a fixed order
+        const int64_t e = (int64_t)(fr.tile_base + ty * fr.tiles_x + tx) * S::kWaves + wave;
+#pragma unroll
+        for (int j = 0; j < kLossSums; ++j) {
+            const double v = wave_sum_f64(ls[j]);
+            if (lane == j) loss.part[e * kLossSums + j] = v;
+        }
     }
 }
 
@@ -650,7 +672,7 @@ void ug_forward_kernel(
     const float4* __restrict__ pair_sect, float chi2, float sqrt_chi2, float alpha_min, float bg_alpha,
     float bg_intensity, int alpha_excludes_bg, const us3d_stats* __restrict__ stats, float* __restrict__ out_image,
     float* __restrict__ out_alpha, float* __restrict__ out_depth, float4* __restrict__ out_aux,
-    uint16_t* __restrict__ inst_mask, int n_tiles, int32_t* __restrict__ queue_heads) {
+    uint16_t* __restrict__ inst_mask, int n_tiles, int32_t* __restrict__ queue_heads, LossFuse loss) {
     __shared__ FwdLds<TS, kBatch> lds;
     __shared__ int s_tile;
     if (stats->overflow) return;
@@ -664,12 +686,12 @@ void ug_forward_kernel(
             ug_forward_tile<TS, kDepth, true, kBatch>(lds, rec, pair_g, seg_key, fr, tx, ty, s0, s1, pixels, pair_sect,
                                                       chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
                                                       alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
-                                                      inst_mask);
+                                                      inst_mask, loss);
         else
             ug_forward_tile<TS, kDepth, false, kBatch>(lds, rec, pair_g, seg_key, fr, tx, ty, s0, s1, pixels,
                                                        pair_sect, chi2, sqrt_chi2, alpha_min, bg_alpha, bg_intensity,
                                                        alpha_excludes_bg, out_image, out_alpha, out_depth, out_aux,
-                                                       inst_mask);
+                                                       inst_mask, loss);
     };
     if (!queue_heads) {  // one workgroup per tile
         item(remap_xcd(blockIdx.x, (int)gridDim.x));
@@ -814,7 +836,7 @@ __device__ __forceinline__ void ug_backward_tile(
     const float4* __restrict__ pair_sect, float chi2, float alpha_min,
     float bg_alpha, const float4* __restrict__ aux, const float* __restrict__ d_image,
     const float* __restrict__ d_alpha, const float* __restrict__ d_depth, float4* __restrict__ partial,
-    const uint16_t* __restrict__ inst_mask) {
+    const uint16_t* __restrict__ inst_mask, const float* __restrict__ loss_gt, const double* __restrict__ loss_coeff) {
     using S = TileShape<TS>;
     constexpr int NT = S::kThreads;
     constexpr int PPT = S::kSubPerWave;
@@ -851,7 +873,14 @@ __device__ __forceinline__ void ug_backward_tile(
             }
             const float A = sm.x + bg_alpha;
             const float inv = 1.0f / fmaxf(A, 1e-8f);
-            const float gi = d_image[p];
+            float gi;
+            if (loss_coeff) {  // uniform: dL/dx of the fused image loss, x the clamped image the forward wrote
+                const double xi = fmaxf(0.f, fminf(1.f, sm.y)), yi = loss_gt[p];
+                const double sg = (xi > yi) ? 1.0 : ((xi < yi) ? -1.0 : 0.0);
+                gi = (float)(loss_coeff[0] * sg + loss_coeff[1] + loss_coeff[2] * xi + loss_coeff[3] * yi);
+            } else {
+                gi = d_image[p];
+            }
             const float ga = (kFull && d_alpha) ? d_alpha[p] : 0.f;
             const float gd = (kFull && d_depth) ? d_depth[p] : 0.f;
             const float invS = 1.0f / fmaxf(sm.x, 1e-8f);
@@ -1155,7 +1184,7 @@ __attribute__((amdgpu_waves_per_eu(TileShape<TS>::kBwdMinWavesPerSimd, 8))) void
     float bg_alpha, const float4* __restrict__ aux,
     const float* __restrict__ d_image, const float* __restrict__ d_alpha, const float* __restrict__ d_depth,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, const uint16_t* __restrict__ inst_mask,
-    int n_tiles, int32_t* __restrict__ queue_heads) {
+    int n_tiles, int32_t* __restrict__ queue_heads, LossFuse loss) {
     __shared__ BwdLds<TS, kPlane, kBatch> lds;
     __shared__ int s_tile;
     if (stats->overflow) return;
@@ -1166,14 +1195,17 @@ __attribute__((amdgpu_waves_per_eu(TileShape<TS>::kBwdMinWavesPerSimd, 8))) void
         const int tx = lt % fr.tiles_x, ty = lt / fr.tiles_x;
         const int s0 = tile_off[t], s1 = tile_off[t + 1];
         if (s0 == s1) return;  // workgroup-uniform
+        const double* coeff = loss.coeff ? loss.coeff + 4 * f : nullptr;
         if (kPlane && fr.geometry == 0)  // workgroup-uniform
             ug_backward_tile<TS, kFull, kPlane, kPlane, kBatch>(lds, rec, pair_g, seg_key, fr, tx,
                                                                 ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
-                                                                aux, d_image, d_alpha, d_depth, partial, inst_mask);
+                                                                aux, d_image, d_alpha, d_depth, partial, inst_mask,
+                                                                loss.gt, coeff);
         else
             ug_backward_tile<TS, kFull, false, kPlane, kBatch>(lds, rec, pair_g, seg_key, fr, tx,
                                                                ty, s0, s1, pixels, pair_sect, chi2, alpha_min, bg_alpha,
-                                                               aux, d_image, d_alpha, d_depth, partial, inst_mask);
+                                                               aux, d_image, d_alpha, d_depth, partial, inst_mask,
+                                                               loss.gt, coeff);
     };
     if (!queue_heads) {  // one workgroup per tile
         item(remap_xcd(blockIdx.x, (int)gridDim.x));
@@ -1414,8 +1446,12 @@ int us3d_ug_map(const us3d_plan* plan, const float* means, const float* tril, co
     return build_instances(plan, L, stream, kSectF4, 3);
 }
 
-int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
-                    float* alpha, float* depth, float* aux, void* stream_) {
+}  // extern "C"
+
+namespace {
+
+int ug_forward_impl(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
+                    float* alpha, float* depth, float* aux, const LossFuse& loss, void* stream_) {
     const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
@@ -1434,7 +1470,7 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
                        at<float4>(plan, L.pair_sect), settings->chi2_threshold, sqrt_chi2, settings->alpha_min,      \
                        settings->bg_alpha, settings->bg_intensity, settings->alpha_excludes_bg,                       \
                        at<us3d_stats>(plan, L.stats), image, alpha, depth,                                            \
-                       reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask), plan->n_tiles, heads)
+                       reinterpret_cast<float4*>(aux), at<uint16_t>(plan, L.inst_mask), plan->n_tiles, heads, loss)
 #define UG_FWD_(TS, DEPTH, PLANE)                                                                                     \
     do {                                                                                                              \
         switch (lds_batch(settings->fwd_lds_batch, FwdShape<TS>::kDefaultBatch)) {                                    \
@@ -1461,14 +1497,15 @@ int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_se
     return check_launch("ug_forward_kernel");
 }
 
-int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tril, const us3d_ug_settings* settings,
+int ug_backward_impl(const us3d_plan* plan, const float* pixels, const float* tril, const us3d_ug_settings* settings,
                      const float* aux, const float* d_image, const float* d_alpha, const float* d_depth,
-                     float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream_) {
+                     float* d_means, float* d_tril, float* d_opacity, float* d_intensity, const LossFuse& loss,
+                     void* stream_) {
     const us3d_layout L = ug_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = check_settings(settings))) return rc;
-    if (!tril || !aux || !d_image || !d_means || !d_tril || !d_opacity || !d_intensity)
+    if (!tril || !aux || !(d_image || loss.coeff) || !d_means || !d_tril || !d_opacity || !d_intensity)
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     int32_t* heads = settings->work_queue ? at<int32_t>(plan, L.queue) : nullptr;
@@ -1484,7 +1521,7 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
                        settings->chi2_threshold, settings->alpha_min, settings->bg_alpha,                             \
                        reinterpret_cast<const float4*>(aux), d_image, d_alpha, d_depth,                               \
                        at<us3d_stats>(plan, L.stats), at<float4>(plan, L.partial), at<uint16_t>(plan, L.inst_mask),   \
-                       plan->n_tiles, heads)
+                       plan->n_tiles, heads, loss)
 #define UG_BWD_(TS, FULL, PLANE)                                                                                      \
     do {                                                                                                              \
         switch (lds_batch(settings->bwd_lds_batch, TileShape<TS>::kThreads)) {                                                       \
@@ -1522,6 +1559,52 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
         if ((rc = check_launch("ug_reduce_kernel"))) return rc;
     }
     return US3D_OK;
+}
+
+// Per-(tile, wavefront) loss sums of a fused forward: one wavefront per tile below 32-pixel tiles, four at 32.
+int64_t loss_part_entries(const us3d_plan* plan) { return (int64_t)plan->n_tiles * FwdShape<32>::kWaves; }
+
+}  // namespace
+
+extern "C" {
+
+int us3d_ug_forward(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
+                    float* alpha, float* depth, float* aux, void* stream_) {
+    return ug_forward_impl(plan, pixels, settings, image, alpha, depth, aux, LossFuse{nullptr, nullptr, nullptr},
+                           stream_);
+}
+
+int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tril, const us3d_ug_settings* settings,
+                     const float* aux, const float* d_image, const float* d_alpha, const float* d_depth,
+                     float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream_) {
+    if (!d_image) return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
+    return ug_backward_impl(plan, pixels, tril, settings, aux, d_image, d_alpha, d_depth, d_means, d_tril, d_opacity,
+                            d_intensity, LossFuse{nullptr, nullptr, nullptr}, stream_);
+}
+
+int64_t us3d_ug_loss_tmp_bytes(const us3d_plan* plan) {
+    if (!plan) return 0;
+    return (loss_part_entries(plan) * kLossSums + 4 * (int64_t)plan->n_frames) * (int64_t)sizeof(double);
+}
+
+int us3d_ug_forward_loss(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, float* image,
+                         float* alpha, float* depth, float* aux, const float* gt, float w_l1, float w_ssim,
+                         float scale, float* loss, double* tmp, void* stream_) {
+    if (!plan || !settings || !gt || !loss || !tmp) return fail(US3D_ERR_ARGUMENT, "forward_loss: NULL argument");
+    int rc = ug_forward_impl(plan, pixels, settings, image, alpha, depth, aux, LossFuse{gt, tmp, nullptr}, stream_);
+    if (rc) return rc;
+    const int waves = settings->tile_size == 32 ? FwdShape<32>::kWaves
+                                                : (settings->tile_size == 16 ? FwdShape<16>::kWaves : FwdShape<8>::kWaves);
+    return image_loss_from_tiles(tmp, plan->frame_tile_base, plan->frame_pixel_base, plan->n_frames, waves, w_l1,
+                                 w_ssim, scale, loss, tmp + loss_part_entries(plan) * kLossSums, as_stream(stream_));
+}
+
+int us3d_ug_backward_loss(const us3d_plan* plan, const float* pixels, const float* tril,
+                          const us3d_ug_settings* settings, const float* aux, const float* gt, const double* tmp,
+                          float* d_means, float* d_tril, float* d_opacity, float* d_intensity, void* stream_) {
+    if (!plan || !gt || !tmp) return fail(US3D_ERR_ARGUMENT, "backward_loss: NULL argument");
+    return ug_backward_impl(plan, pixels, tril, settings, aux, nullptr, nullptr, nullptr, d_means, d_tril, d_opacity,
+                            d_intensity, LossFuse{gt, nullptr, tmp + loss_part_entries(plan) * kLossSums}, stream_);
 }
 
 int us3d_ug_pixel_lists(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings, int pass,

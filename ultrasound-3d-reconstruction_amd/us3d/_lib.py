@@ -156,6 +156,15 @@ SIGNATURES = {
         ctypes.c_int,
         [_PP, _VP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
     ),
+    "us3d_ug_loss_tmp_bytes": (_I64, [_PP]),
+    "us3d_ug_forward_loss": (
+        ctypes.c_int,
+        [_PP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _F, _F, _F, _VP, _VP, _VP],
+    ),
+    "us3d_ug_backward_loss": (
+        ctypes.c_int,
+        [_PP, _VP, _VP, ctypes.POINTER(UGSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    ),
     "us3d_ug_pixel_lists": (ctypes.c_int, [_PP, _VP, ctypes.POINTER(UGSettings), ctypes.c_int, _VP, _VP, _VP, _VP]),
     "us3d_ug_render_static": (
         ctypes.c_int,

@@ -47,15 +47,19 @@ class SliceSplatTrainer:
     """The captured training step. With chunk_frames, the frames are rendered chunk by chunk inside the same step
     (map -> forward -> loss -> adjoint per chunk of at most chunk_frames frames, the chunks' gradients summed, then
     one Adam update), so the workspace holds one chunk's instances and pairs: memory is bounded by the chunk, not
-    the sweep. The loss is the mean over all frames either way, so the update is the same."""
+    the sweep. The loss is the mean over all frames either way, so the update is the same.
+    fused_loss: the loss statistics are summed inside the forward splat and the loss gradient is formed inside the
+    adjoint (us3d_ug_forward_loss / us3d_ug_backward_loss); False runs the separate loss kernels
+    (us3d_image_loss) between them. Both give the same step up to FP64 summation order."""
 
     def __init__(self, means: torch.Tensor, tril: torch.Tensor, raw_opacity: torch.Tensor,
                  raw_intensity: torch.Tensor, frames: Sequence[FrameSpec], target: torch.Tensor,
                  cfg: UltraGaussRenderConfig, sweeps: Optional[Sequence[SweepSpec]] = None, lr: float = 1e-3,
                  betas=(0.9, 0.999), eps: float = 1e-8, w_l1: float = 1.0, w_ssim: float = 0.2,
                  headroom: float = 1.3, use_graph: bool = True, check_every: int = 1,
-                 chunk_frames: Optional[int] = None):
+                 chunk_frames: Optional[int] = None, fused_loss: bool = True):
         dev = means.device
+        self.fused_loss = bool(fused_loss)
         self.dev = dev
         G = means.shape[0]
         f32 = dict(dtype=torch.float32, device=dev)
@@ -122,6 +126,8 @@ class SliceSplatTrainer:
         self.workspace = new_workspace(max(c[4].workspace_bytes(self.k_capacity, self.v_capacity)
                                            for c in self.chunks), self.dev)
         self.plans = [c[4].tables.plan(self.workspace, self.k_capacity, self.v_capacity) for c in self.chunks]
+        fuse_bytes = max(int(_lib.lib().us3d_ug_loss_tmp_bytes(ctypes.byref(p))) for p in self.plans)
+        self.fuse_tmp = torch.empty(fuse_bytes // 8, dtype=torch.float64, device=self.dev)
         off = self.batch.layout_offsets(self.k_capacity, self.v_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
         self.overflow_view = self.workspace[off.stats + _lib.Stats.overflow.offset:
@@ -140,16 +146,25 @@ class SliceSplatTrainer:
                                      _lib.ptr(self.raw_c), st, s), "us3d_ug_map")
             if chunked:
                 torch.maximum(self.skip, self.overflow_view, out=self.skip)
-            _lib.check(L.us3d_ug_forward(plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
-                                         None, _lib.ptr(self.aux), s), "us3d_ug_forward")
-            _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target) + 4 * p0,
-                                         _lib.ptr(batch.tables.pixel_base), nf, self.w_l1, self.w_ssim, 1.0 / self.F,
-                                         _lib.ptr(self.loss) + 4 * a, _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp),
-                                         s), "us3d_image_loss")
             out = self.grads if ci == 0 else self.grads_chunk
-            _lib.check(L.us3d_ug_backward(plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
-                                          _lib.ptr(self.d_image), None, None, _lib.ptr(out[0]), _lib.ptr(out[1]),
-                                          _lib.ptr(out[2]), _lib.ptr(out[3]), s), "us3d_ug_backward")
+            gt = _lib.ptr(self.target) + 4 * p0
+            if self.fused_loss:
+                _lib.check(L.us3d_ug_forward_loss(plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha), None,
+                                                  _lib.ptr(self.aux), gt, self.w_l1, self.w_ssim, 1.0 / self.F,
+                                                  _lib.ptr(self.loss) + 4 * a, _lib.ptr(self.fuse_tmp), s),
+                           "us3d_ug_forward_loss")
+                _lib.check(L.us3d_ug_backward_loss(plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux), gt,
+                                                   _lib.ptr(self.fuse_tmp), _lib.ptr(out[0]), _lib.ptr(out[1]),
+                                                   _lib.ptr(out[2]), _lib.ptr(out[3]), s), "us3d_ug_backward_loss")
+            else:
+                _lib.check(L.us3d_ug_forward(plan, None, st, _lib.ptr(self.image), _lib.ptr(self.alpha),
+                                             None, _lib.ptr(self.aux), s), "us3d_ug_forward")
+                _lib.check(L.us3d_image_loss(_lib.ptr(self.image), gt, _lib.ptr(batch.tables.pixel_base), nf,
+                                             self.w_l1, self.w_ssim, 1.0 / self.F, _lib.ptr(self.loss) + 4 * a,
+                                             _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
+                _lib.check(L.us3d_ug_backward(plan, None, _lib.ptr(self.tril), st, _lib.ptr(self.aux),
+                                              _lib.ptr(self.d_image), None, None, _lib.ptr(out[0]), _lib.ptr(out[1]),
+                                              _lib.ptr(out[2]), _lib.ptr(out[3]), s), "us3d_ug_backward")
             if ci > 0:
                 for g, gc in zip(self.grads, self.grads_chunk):
                     g.add_(gc)
