@@ -231,12 +231,13 @@ __device__ __forceinline__ int ug_pair_tiles(const float4& r0, const float4& r1,
     // beyond every marginal radius (see ug_pack_kernel); NaN takes the exact path. Most pairs leave here, before the
     // frame's other axes are read.
     if (fabsf(ele) > r2.w) return 0;
+    // the elevation cull first: a pair it removes never computes its in-plane radii
+    const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
+    if (!(fabsf(ele) <= r_ele)) return 0;
     const float lat = dot3(vx, vy, vz, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2]);
     const float ax = dot3(vx, vy, vz, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2]);
     const float r_lat = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_lat[0], fr.axis_lat[1], fr.axis_lat[2], sqrt_chi2);
     const float r_ax = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ax[0], fr.axis_ax[1], fr.axis_ax[2], sqrt_chi2);
-    const float r_ele = marginal_radius(l11, l12, l13, l22, l23, l33, fr.axis_ele[0], fr.axis_ele[1], fr.axis_ele[2], sqrt_chi2);
-    if (!(fabsf(ele) <= r_ele)) return 0;
     const float tsf = (float)ts;
     float fx0, fy0, fx1, fy1, v;
     if (fr.geometry == 1) {
