@@ -218,9 +218,9 @@ int us3d_ug_backward(const us3d_plan* plan, const float* pixels, const float* tr
  * against gt (indexed like the image) per (tile, wavefront); a small kernel then writes loss[n_frames] and the
  * frames' gradient coefficients into tmp (us3d_ug_loss_tmp_bytes(plan) bytes; scale, w_l1 and w_ssim as in
  * us3d_image_loss). us3d_ug_backward_loss is us3d_ug_backward with d_image formed from the clamped image, gt and
- * those coefficients while the adjoint stages its pixels; it needs the same plan, gt and tmp. The FP64 sums run in
- * a different (fixed) order than us3d_image_loss's, so loss and gradients agree with the unfused sequence to
- * rounding, and each is bitwise reproducible. No counterpart in the reference (its loss is torch ops). */
+ * those coefficients while the adjoint stages its pixels; it needs the same plan, gt and tmp. The sums run in a
+ * different (fixed) order than us3d_image_loss's (a thread's at most four pixels in FP32, then FP64), so loss and
+ * gradients agree with the unfused sequence to rounding (1e-7 relative), and each is bitwise reproducible. No counterpart in the reference (its loss is torch ops). */
 int64_t us3d_ug_loss_tmp_bytes(const us3d_plan* plan);
 int us3d_ug_forward_loss(const us3d_plan* plan, const float* pixels, const us3d_ug_settings* settings,
                          float* image, float* alpha, float* depth, float* aux, const float* gt, float w_l1,

@@ -262,7 +262,8 @@ int sort_segments(uint64_t* keys, uint64_t* keys_tmp, const int32_t* seg_off, in
 
 // Image loss of a training step fused into a slice splat (train.hip holds the frame statistics and their
 // gradient): the forward reads the targets gt (indexed like the image) and writes, per (tile, wavefront), the six
-// FP64 sums sum|x-y|, sum x, sum y, sum x^2, sum y^2, sum xy of its pixels (x the clamped image) into part; the
+// sums sum|x-y|, sum x, sum y, sum x^2, sum y^2, sum xy of its pixels (x the clamped image; a lane's at most four
+// pixels summed in FP32, the lanes in FP64) into part; the
 // finish reduces them per frame into the loss and four gradient coefficients per frame (coeff), from which the
 // adjoint forms dL/dx = cs sign(x-y) + ca + cx x + cy y while it stages its pixels. NULL pointers: no loss.
 struct LossFuse {

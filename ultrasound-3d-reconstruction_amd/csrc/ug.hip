@@ -624,7 +624,9 @@ __device__ __forceinline__ void ug_forward_tile(
         }
     }
 
-    double ls[kLossSums] = {0, 0, 0, 0, 0, 0};  // the loss sums of the lane's pixels (fused training step)
+    // the loss sums of the lane's pixels (fused training step): at most four pixels in FP32, then FP64 across the
+    // wavefront and the frame
+    float ls[kLossSums] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     float gtv[PPT];  // their targets, loaded before the image is formed so the loads overlap it
 #pragma unroll
     for (int k = 0; k < PPT; ++k) gtv[k] = (loss.gt && valid[k]) ? loss.gt[pidx[k]] : 0.f;
@@ -640,20 +642,20 @@ __device__ __forceinline__ void ug_forward_tile(
         if constexpr (kDepth) out_depth[pidx[k]] = D_[k] / fmaxf(S_[k], 1e-8f);
         out_aux[pidx[k]] = make_float4(S_[k], img, kDepth ? D_[k] : 0.f, 0.f);
         if (loss.gt) {
-            const double x = pred, y = gtv[k];
-            ls[0] += fabs(x - y);
+            const float x = pred, y = gtv[k];
+            ls[0] += fabsf(x - y);
             ls[1] += x;
             ls[2] += y;
-            ls[3] += x * x;
-            ls[4] += y * y;
-            ls[5] += x * y;
+            ls[3] = __builtin_fmaf(x, x, ls[3]);
+            ls[4] = __builtin_fmaf(y, y, ls[4]);
+            ls[5] = __builtin_fmaf(x, y, ls[5]);
         }
     }
     if (loss.gt) {  // uniform: one entry per (tile, wavefront), the lanes' sums combined in a fixed order
         const int64_t e = (int64_t)(fr.tile_base + ty * fr.tiles_x + tx) * S::kWaves + wave;
 #pragma unroll
         for (int j = 0; j < kLossSums; ++j) {
-            const double v = wave_sum_f64(ls[j]);
+            const double v = wave_sum_f64((double)ls[j]);
             if (lane == j) loss.part[e * kLossSums + j] = v;
         }
     }
