@@ -58,7 +58,7 @@ def parse(argv=None):
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--eager", action="store_true", help="batched step without hipGraph capture")
     ap.add_argument("--separate-loss", action="store_true",
-                    help="slice steps with the image loss as its own kernels instead of fused into the splats")
+                    help="training steps with the image loss as its own kernels instead of fused into the splats")
     ap.add_argument("--workload", choices=("slice", "ray", "multisweep", "fan", "gaussianpile"), default="slice",
                     help="slice: the slice splat training step (the headline metric); ray: UltraG-Ray B-mode "
                          "forward+backward, 200k Gaussians, --ray-frames frames of 128 scanlines x 1024 samples; "
@@ -906,7 +906,8 @@ def bench_gaussianpile(args):
     op = scene.opacity.clamp(1e-6, 1 - 1e-6)
     raw = [t.to(dev).float().contiguous() for t in (scene.means, torch.log(scene.scales), scene.quats,
                                                      torch.log(op / (1 - op)))]
-    tr = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=not args.eager, check_every=10 ** 9)
+    tr = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=not args.eager, check_every=10 ** 9,
+                             fused_loss=not args.separate_loss)
     dt = timed(tr.step, args.steps, args.warmup)
     if tr.overflowed():  # sticky over every replay since the workspace was allocated
         raise RuntimeError("instance capacity overflowed inside the timed region; the measurement is invalid")
@@ -935,7 +936,8 @@ def bench_gaussianpile(args):
         dtp = timed(plain, 1, 1)
         fps_plain = n_plain / dtp
         log(f"gaussianpile plain (per frame): {fps_plain:.1f} frames/s")
-    tr_t = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=False, check_every=10 ** 9)
+    tr_t = GaussianPileTrainer(*raw, frames, gt, cfg, lr=1e-3, use_graph=False, check_every=10 ** 9,
+                               fused_loss=not args.separate_loss)
     tr_t.step()
     stages = stage_timing(tr_t.step)
     K = int(read_stats(tr_t.workspace).n_instances)

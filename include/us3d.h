@@ -358,6 +358,20 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
                      float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
                      void* stream);
 
+/* Training step with the image loss fused into the GaussianPile splat, as us3d_ug_forward_loss /
+ * us3d_ug_backward_loss do for the slice splat: us3d_gp_forward_loss renders as us3d_gp_forward and sums the
+ * frame statistics of the loss against gt in the same kernel, then writes loss[n_frames] and the frames' gradient
+ * coefficients into tmp (us3d_gp_loss_tmp_bytes(plan) bytes); us3d_gp_backward_loss is us3d_gp_backward with
+ * d_image formed from that image, gt and the coefficients while the adjoint stages its pixels (same plan, image,
+ * gt and tmp). Agrees with us3d_gp_forward + us3d_image_loss + us3d_gp_backward to rounding. No counterpart in
+ * the reference (its loss is torch ops). */
+int64_t us3d_gp_loss_tmp_bytes(const us3d_plan* plan);
+int us3d_gp_forward_loss(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, const float* gt,
+                         float w_l1, float w_ssim, float scale, float* loss, double* tmp, void* stream);
+int us3d_gp_backward_loss(const us3d_plan* plan, const us3d_gp_settings* settings, const float* image,
+                          const float* gt, const double* tmp, float* d_means, float* d_scales, float* d_quats,
+                          float* d_opacity, float* d_intensity, void* stream);
+
 /* ---------------------------------------------------------------- training-step helpers */
 
 /* Exclusive scan of n int32 values into out[0..n) and the int64 total into *total_dev (device).

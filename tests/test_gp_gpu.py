@@ -280,7 +280,8 @@ def test_gp_trainer_gradients_match_torch_autograd_through_the_activations(dev):
     from us3d.pipeline import simple_ssim
 
     Trainer, frames, cfg, gt, raw = _trainer_setup(dev)
-    tr = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=0.0)
+    # the separate loss kernels, whose dL/dimage is checked here; the fused path is checked against them below
+    tr = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=0.0, fused_loss=False)
     tr.step()
     torch.cuda.synchronize()
     G = raw[0].shape[0]
@@ -315,6 +316,30 @@ def test_gp_trainer_gradients_match_torch_autograd_through_the_activations(dev):
         assert float(ref.abs().max()) > 0, name
         err = float((g.detach().double() - ref).norm() / ref.norm())
         assert err < 1e-5, f"{name}: rel L2 {err:.3e}"
+
+
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_gp_loss_fused_into_the_splat_equals_the_separate_loss_kernels(dev, ts):
+    """us3d_gp_forward_loss / us3d_gp_backward_loss (loss sums in the forward's epilogue, dL/dimage formed while the
+    adjoint stages its pixels) against us3d_gp_forward + us3d_image_loss + us3d_gp_backward: the same losses and
+    gradients up to summation order, at every tile size (one or four wavefronts per tile)."""
+    from dataclasses import replace
+
+    Trainer, frames, cfg, gt, raw = _trainer_setup(dev)
+    cfg = replace(cfg, tile_size_px=ts)
+    a = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=0.0, fused_loss=True)
+    b = Trainer(*raw, frames, gt, cfg, use_graph=False, lr=0.0, fused_loss=False)
+    a.step()
+    b.step()
+    torch.cuda.synchronize()
+    la, lb = a.losses().double().cpu().numpy(), b.losses().double().cpu().numpy()
+    assert np.all(lb > 0)
+    assert float(np.linalg.norm(la - lb) / np.linalg.norm(lb)) < 1e-6, (la, lb)
+    for name, ga, gb in zip(("means", "raw_scales", "raw_quats", "raw_opacity"), a.grads, b.grads):
+        ref = gb.detach().double()
+        assert float(ref.abs().max()) > 0, name
+        err = float((ga.detach().double() - ref).norm() / ref.norm())
+        assert err < 1e-6, f"{name}: rel L2 {err:.3e}"
 
 
 def test_gp_trainer_graph_replays_equal_eager_steps(dev):

@@ -333,10 +333,12 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
     const float4* __restrict__ rec, const float4* __restrict__ pair_foot, const int32_t* __restrict__ tile_off,
     const int32_t* __restrict__ pair_g, const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min,
-    const us3d_stats* __restrict__ stats, float* __restrict__ image, int n_tiles, int32_t* __restrict__ queue_heads) {
+    const us3d_stats* __restrict__ stats, float* __restrict__ image, int n_tiles, int32_t* __restrict__ queue_heads,
+    LossFuse loss) {
     using S = GpFwd<TS>;
     constexpr int NT = S::kThreads;
     constexpr int NP = S::kPix;
+    static_assert(NT % kWave == 0, "the fused loss sums whole wavefronts");
     __shared__ float4 s_f0[NT], s_f1[NT];
     __shared__ float s_I[NT];
     __shared__ int s_tile;
@@ -385,10 +387,37 @@ __global__ __launch_bounds__(GpFwd<TS>::kThreads) void gp_forward_kernel(
                 }
             }
         }
+        // the loss sums of the thread's pixels (fused training step): at most four pixels in FP32, then FP64
+        // across the wavefront and the frame; the targets are loaded before the image stores
+        float gtv[NP];
 #pragma unroll
         for (int r = 0; r < NP; ++r) {
             const int iy = iy0 + r * S::kRowStep;
-            if (ix < fr.width && iy < fr.height) image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc[r];
+            gtv[r] = (loss.gt && ix < fr.width && iy < fr.height)
+                         ? loss.gt[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] : 0.f;
+        }
+        float ls[kLossSums] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int r = 0; r < NP; ++r) {
+            const int iy = iy0 + r * S::kRowStep;
+            if (ix < fr.width && iy < fr.height) {
+                image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix] = acc[r];
+                const float x = acc[r], y = gtv[r];
+                ls[0] += fabsf(x - y);
+                ls[1] += x;
+                ls[2] += y;
+                ls[3] = __builtin_fmaf(x, x, ls[3]);
+                ls[4] = __builtin_fmaf(y, y, ls[4]);
+                ls[5] = __builtin_fmaf(x, y, ls[5]);
+            }
+        }
+        if (loss.gt) {  // uniform: one entry per (tile, wavefront)
+            const int64_t e = (int64_t)t * (NT / kWave) + tid / kWave;
+#pragma unroll
+            for (int j = 0; j < kLossSums; ++j) {
+                const double v = wave_sum_f64((double)ls[j]);
+                if ((tid & (kWave - 1)) == j) loss.part[e * kLossSums + j] = v;
+            }
         }
     };
     if (!queue_heads) {  // one workgroup per tile
@@ -414,7 +443,7 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
     const int32_t* __restrict__ pair_g, const uint64_t* __restrict__ seg_key, const us3d_frame* __restrict__ frames,
     const int32_t* __restrict__ frame_tile_base, int n_frames, float weight_min, const float* __restrict__ d_image,
     const us3d_stats* __restrict__ stats, float4* __restrict__ partial, int n_tiles,
-    int32_t* __restrict__ queue_heads) {
+    int32_t* __restrict__ queue_heads, const float* __restrict__ image, LossFuse loss) {
     constexpr int NT = TS * TS;
     constexpr int HALF = TS / 2;
     static_assert(TS % 4 == 0, "rows of the staged gradient are read as float4");
@@ -433,8 +462,19 @@ __global__ __launch_bounds__(TS * TS) void gp_backward_kernel(
         const int cx0 = tx * TS, cy0 = ty * TS;
         {
             const int ix = cx0 + tid % TS, iy = cy0 + tid / TS;
-            s_g[tid] = (ix < fr.width && iy < fr.height) ? d_image[(int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix]
-                                                         : 0.f;
+            float gi = 0.f;
+            if (ix < fr.width && iy < fr.height) {
+                const int64_t p = (int64_t)fr.pixel_base + (int64_t)iy * fr.width + ix;
+                if (loss.coeff) {  // uniform: dL/dx of the fused image loss, x the image the forward wrote
+                    const double* cf = loss.coeff + 4 * f;
+                    const double xi = image[p], yi = loss.gt[p];
+                    const double sg = (xi > yi) ? 1.0 : ((xi < yi) ? -1.0 : 0.0);
+                    gi = (float)(cf[0] * sg + cf[1] + cf[2] * xi + cf[3] * yi);
+                } else {
+                    gi = d_image[p];
+                }
+            }
+            s_g[tid] = gi;
             if (tid < TS) s_xl[tid] = ((float)(cx0 + tid) - fr.col_offset) * fr.delta_lat;
         }
         __syncthreads();
@@ -821,7 +861,12 @@ int us3d_gp_map(const us3d_plan* plan, const float* means, const float* scales, 
     return build_instances(plan, L, stream, kFootF4, 2);
 }
 
-int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, void* stream_) {
+}  // extern "C"
+
+namespace {
+
+int gp_forward_impl(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, const LossFuse& loss,
+                    void* stream_) {
     const us3d_layout L = gp_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
@@ -838,7 +883,7 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
                        gp_at<int32_t>(plan, L.tile_off), gp_at<int32_t>(plan, L.pair_g),                             \
                        gp_at<uint64_t>(plan, L.seg_key), plan->frames, plan->frame_tile_base, plan->n_frames,        \
                        settings->weight_min,                                                                          \
-                       gp_at<us3d_stats>(plan, L.stats), image, plan->n_tiles, heads)
+                       gp_at<us3d_stats>(plan, L.stats), image, plan->n_tiles, heads, loss)
     {
         StageTimer timer_("gp_forward_kernel", stream);
         switch (settings->tile_size) {
@@ -851,14 +896,14 @@ int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, flo
     return check_launch("gp_forward_kernel");
 }
 
-int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
+int gp_backward_impl(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
                      float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
-                     void* stream_) {
+                     const float* image, const LossFuse& loss, void* stream_) {
     const us3d_layout L = gp_layout(plan);
     int rc = check_plan(plan, L.total);
     if (rc) return rc;
     if ((rc = gp_check_settings(settings))) return rc;
-    if (!d_image || !d_means || !d_scales || !d_quats || !d_opacity)
+    if (!(d_image || (loss.coeff && image && loss.gt)) || !d_means || !d_scales || !d_quats || !d_opacity)
         return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
     hipStream_t stream = as_stream(stream_);
     if (plan->n_tiles > 0) {
@@ -871,7 +916,7 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
                        gp_at<int32_t>(plan, L.tile_off), gp_at<int32_t>(plan, L.pair_g),                             \
                        gp_at<uint64_t>(plan, L.seg_key), plan->frames, plan->frame_tile_base, plan->n_frames,        \
                        settings->weight_min, d_image, gp_at<us3d_stats>(plan, L.stats),                               \
-                       gp_at<float4>(plan, L.partial), plan->n_tiles, heads)
+                       gp_at<float4>(plan, L.partial), plan->n_tiles, heads, image, loss)
         {
             StageTimer timer_("gp_backward_kernel", stream);
             switch (settings->tile_size) {
@@ -897,6 +942,51 @@ int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, co
         if ((rc = check_launch("gp_reduce_kernel"))) return rc;
     }
     return US3D_OK;
+}
+
+// Per-(tile, wavefront) loss sums of a fused forward: up to four wavefronts per tile (32-pixel tiles).
+int64_t gp_loss_part_entries(const us3d_plan* plan) { return (int64_t)plan->n_tiles * (GpFwd<32>::kThreads / kWave); }
+
+}  // namespace
+
+extern "C" {
+
+int us3d_gp_forward(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, void* stream_) {
+    return gp_forward_impl(plan, settings, image, LossFuse{nullptr, nullptr, nullptr}, stream_);
+}
+
+int us3d_gp_backward(const us3d_plan* plan, const us3d_gp_settings* settings, const float* d_image,
+                     float* d_means, float* d_scales, float* d_quats, float* d_opacity, float* d_intensity,
+                     void* stream_) {
+    if (!d_image) return fail(US3D_ERR_ARGUMENT, "backward: NULL array");
+    return gp_backward_impl(plan, settings, d_image, d_means, d_scales, d_quats, d_opacity, d_intensity, nullptr,
+                            LossFuse{nullptr, nullptr, nullptr}, stream_);
+}
+
+int64_t us3d_gp_loss_tmp_bytes(const us3d_plan* plan) {
+    if (!plan) return 0;
+    return (gp_loss_part_entries(plan) * kLossSums + 4 * (int64_t)plan->n_frames) * (int64_t)sizeof(double);
+}
+
+int us3d_gp_forward_loss(const us3d_plan* plan, const us3d_gp_settings* settings, float* image, const float* gt,
+                         float w_l1, float w_ssim, float scale, float* loss, double* tmp, void* stream_) {
+    if (!plan || !settings || !gt || !loss || !tmp) return fail(US3D_ERR_ARGUMENT, "forward_loss: NULL argument");
+    int rc = gp_forward_impl(plan, settings, image, LossFuse{gt, tmp, nullptr}, stream_);
+    if (rc) return rc;
+    const int waves = settings->tile_size == 32 ? GpFwd<32>::kThreads / kWave
+                                                : (settings->tile_size == 16 ? GpFwd<16>::kThreads / kWave
+                                                                             : GpFwd<8>::kThreads / kWave);
+    return image_loss_from_tiles(tmp, plan->frame_tile_base, plan->frame_pixel_base, plan->n_frames, waves, w_l1,
+                                 w_ssim, scale, loss, tmp + gp_loss_part_entries(plan) * kLossSums,
+                                 as_stream(stream_));
+}
+
+int us3d_gp_backward_loss(const us3d_plan* plan, const us3d_gp_settings* settings, const float* image,
+                          const float* gt, const double* tmp, float* d_means, float* d_scales, float* d_quats,
+                          float* d_opacity, float* d_intensity, void* stream_) {
+    if (!plan || !image || !gt || !tmp) return fail(US3D_ERR_ARGUMENT, "backward_loss: NULL argument");
+    return gp_backward_impl(plan, settings, nullptr, d_means, d_scales, d_quats, d_opacity, d_intensity, image,
+                            LossFuse{gt, nullptr, tmp + gp_loss_part_entries(plan) * kLossSums}, stream_);
 }
 
 }  // extern "C"

@@ -183,6 +183,12 @@ SIGNATURES = {
     "us3d_gp_map": (ctypes.c_int, [_PP, _VP, _VP, _VP, _VP, _VP, ctypes.POINTER(GPSettings), _VP]),
     "us3d_gp_forward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP]),
     "us3d_gp_backward": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP]),
+    "us3d_gp_loss_tmp_bytes": (_I64, [_PP]),
+    "us3d_gp_forward_loss": (ctypes.c_int, [_PP, ctypes.POINTER(GPSettings), _VP, _VP, _F, _F, _F, _VP, _VP, _VP]),
+    "us3d_gp_backward_loss": (
+        ctypes.c_int,
+        [_PP, ctypes.POINTER(GPSettings), _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP, _VP],
+    ),
     "us3d_scan_tmp_bytes": (_I64, [_I64]),
     "us3d_scan_i32": (ctypes.c_int, [_VP, _VP, _I64, _VP, _VP, _VP]),
     "us3d_image_loss_tmp_bytes": (_I64, [_I32]),

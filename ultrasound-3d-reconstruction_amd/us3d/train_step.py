@@ -232,16 +232,18 @@ class GaussianPileTrainer:
     pipeline_god_file.py:435-452): us3d_gp_map with raw activations (its pack applies the opacity sigmoid and
     the scales' max(exp(x), 1e-5), :360-370; the kernels normalise the quaternions as :367 does) ->
     us3d_gp_forward -> us3d_image_loss -> us3d_gp_backward (gradients w.r.t. the raw values) -> us3d_adam_step
-    per group -> us3d_adam_advance. Intensities are one, as the facade passes them. Overflow handling and
-    `check_every` as in SliceSplatTrainer."""
+    per group -> us3d_adam_advance. Intensities are one, as the facade passes them. Overflow handling,
+    `check_every` and `fused_loss` (us3d_gp_forward_loss / us3d_gp_backward_loss in place of the three calls) as
+    in SliceSplatTrainer."""
 
     def __init__(self, means: torch.Tensor, raw_scales: torch.Tensor, raw_quats: torch.Tensor,
                  raw_opacity: torch.Tensor, frames: Sequence[FrameSpec], target: torch.Tensor, cfg,
                  sweeps: Optional[Sequence[SweepSpec]] = None, lr: float = 1e-3, betas=(0.9, 0.999),
                  eps: float = 1e-8, w_l1: float = 1.0, w_ssim: float = 0.2, headroom: float = 1.3,
-                 use_graph: bool = True, check_every: int = 1):
+                 use_graph: bool = True, check_every: int = 1, fused_loss: bool = True):
         from .gaussianpile import GPBatch
 
+        self.fused_loss = bool(fused_loss)
         dev = means.device
         self.dev = dev
         G = means.shape[0]
@@ -286,6 +288,8 @@ class GaussianPileTrainer:
         self.workspace = None  # the old workspace goes before the new one is allocated
         self.workspace = new_workspace(self.batch.workspace_bytes(self.k_capacity, self.v_capacity), self.dev)
         self.plan = self.batch.tables.plan(self.workspace, self.k_capacity, self.v_capacity)
+        self.fuse_tmp = torch.empty(int(_lib.lib().us3d_gp_loss_tmp_bytes(ctypes.byref(self.plan))) // 8,
+                                    dtype=torch.float64, device=self.dev)
         off = self.batch.layout_offsets(self.k_capacity, self.v_capacity)
         self.overflow_ptr = self.workspace.data_ptr() + off.stats + _lib.Stats.overflow.offset
         self.graph = None
@@ -297,13 +301,22 @@ class GaussianPileTrainer:
         lay = self.batch.layout
         _lib.check(L.us3d_gp_map(self.plan, _lib.ptr(self.means), _lib.ptr(self.raw_s), _lib.ptr(self.raw_q),
                                  _lib.ptr(self.raw_o), None, st, s), "us3d_gp_map")
-        _lib.check(L.us3d_gp_forward(self.plan, st, _lib.ptr(self.image), s), "us3d_gp_forward")
-        _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target), _lib.ptr(self.batch.tables.pixel_base),
-                                     lay.n_frames, self.w_l1, self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
-                                     _lib.ptr(self.d_image), _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
-        _lib.check(L.us3d_gp_backward(self.plan, st, _lib.ptr(self.d_image), _lib.ptr(self.grads[0]),
-                                      _lib.ptr(self.grads[1]), _lib.ptr(self.grads[2]), _lib.ptr(self.grads[3]), None,
-                                      s), "us3d_gp_backward")
+        g = [_lib.ptr(x) for x in self.grads]
+        if self.fused_loss:
+            _lib.check(L.us3d_gp_forward_loss(self.plan, st, _lib.ptr(self.image), _lib.ptr(self.target), self.w_l1,
+                                              self.w_ssim, 1.0 / lay.n_frames, _lib.ptr(self.loss),
+                                              _lib.ptr(self.fuse_tmp), s), "us3d_gp_forward_loss")
+            _lib.check(L.us3d_gp_backward_loss(self.plan, st, _lib.ptr(self.image), _lib.ptr(self.target),
+                                               _lib.ptr(self.fuse_tmp), g[0], g[1], g[2], g[3], None, s),
+                       "us3d_gp_backward_loss")
+        else:
+            _lib.check(L.us3d_gp_forward(self.plan, st, _lib.ptr(self.image), s), "us3d_gp_forward")
+            _lib.check(L.us3d_image_loss(_lib.ptr(self.image), _lib.ptr(self.target),
+                                         _lib.ptr(self.batch.tables.pixel_base), lay.n_frames, self.w_l1, self.w_ssim,
+                                         1.0 / lay.n_frames, _lib.ptr(self.loss), _lib.ptr(self.d_image),
+                                         _lib.ptr(self.loss_tmp), s), "us3d_image_loss")
+            _lib.check(L.us3d_gp_backward(self.plan, st, _lib.ptr(self.d_image), g[0], g[1], g[2], g[3], None, s),
+                       "us3d_gp_backward")
         b1, b2 = self.betas
         for p, g, m, v in zip(self.params, self.grads, self.exp_avg, self.exp_avg_sq):
             _lib.check(L.us3d_adam_step(_lib.ptr(p), _lib.ptr(g), _lib.ptr(m), _lib.ptr(v), p.numel(),
